@@ -1,0 +1,171 @@
+"""Benchmark: DOF-matvec/s of the matrix-free hexa8 PCG solve on one MI355X.
+
+Workload: 96 x 48 x 48 hexa8 cantilever (221,184 elements, 691,488 free DOFs),
+E0 = 1, Emin = 1e-9, nu = 0.3, p = 3, densities i.i.d. U[0.01, 1] (seed 20261203),
+x = 0 face clamped, unit load on the edge {x = L, z = 0}; one solve to relative
+residual 1e-8.  A step is one such solve in the mixed-precision form (FP32 operator,
+FP64 residual replacement); the FP64 plain form is timed beside it.
+
+value = free DOFs x (operator applications of the step) / step wall time.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "icarat-contact_amd"))
+
+HBM_PEAK = 8.0e12
+SEED = 20261203
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, nargs=3, default=[96, 48, 48])
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(nx, ny, nz, rho, seconds):
+    """The oracle's CSR Jacobi-PCG (Eigen loop) on the host cores, bounded sample."""
+    import ctypes as C
+
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    pr = O.cantilever_3d(nx, ny, nz)
+    t0 = time.perf_counter()
+    b = pr.assemble(rho)
+    t_asm = time.perf_counter() - t0
+    # time PCG passes with a growing iteration cap until the sample reaches the budget
+    its, dt = 0, 0.0
+    cap = 20
+    while True:
+        x = np.zeros(pr.numeq)
+        err = C.c_double()
+        t0 = time.perf_counter()
+        it = O.lib().icxo_sys_cg(pr.system(), b, x, 1e-8, cap, C.byref(err))
+        dt = time.perf_counter() - t0
+        its = it + 1 if err.value < 1e-8 else it
+        if dt > seconds / 3 or err.value < 1e-8:
+            break
+        cap = int(cap * max(2.0, (seconds / 2) / max(dt, 1e-3)))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    return {
+        "value": pr.numeq * its / dt, "unit": "DOF-matvec/s", "cores": threads,
+        "kind": "port",
+        "sample": "oracle CSR Jacobi-PCG (Eigen ConjugateGradient loop), %d iterations on the "
+                  "same %dx%dx%d system (assembly %.2f s not included)" % (its, nx, ny, nz, t_asm),
+        "assembly_s": t_asm, "pcg_s": dt, "iterations": its,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the single-solve path would shard by x-slabs with one halo exchange per "
+              "iteration; not implemented)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import icx
+    nx, ny, nz = args.size
+    ctx = icx.Context(0)
+    job = icx.cantilever_3d(ctx, nx, ny, nz)
+    rho = icx.random_density(job.nelx, SEED)
+    job.set_design(rho)
+    N, Ne = job.numeq, job.nelx
+
+    def sync():
+        ctx.synchronize()
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.synchronize()
+        except Exception:
+            pass
+
+    # warmup (mixed)
+    for _ in range(args.warmup):
+        job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=False)
+    sync()
+    ctx.profile_begin(icx.PROF_FUSED_A)
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, info = job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=False)
+        infos.append(info)
+    sync()
+    wall = time.perf_counter() - t0
+    kern_s, launches = ctx.profile_end()
+    ms_step = wall / args.steps * 1e3
+    inner = infos[-1]["iterations"]
+    mv = infos[-1]["matvecs"]
+    value = N * sum(i["matvecs"] for i in infos) / wall
+    # algorithmic bytes of the fused FP32 operator kernel per launch: reads r, p_old,
+    # invdiag, x_lo (4 words/DOF) and E (1 word/element); writes p, Ap, x_lo (3 words/DOF)
+    bytes_A = (7 * N + Ne) * 4
+    avg_A = kern_s / max(launches, 1)
+    achieved = bytes_A / avg_A if avg_A > 0 else 0.0
+    # whole-iteration algorithmic bytes (9 N w + N_e w_E per inner iteration,
+    # 6 N 8 + N_e 8 per FP64 replacement)
+    refin = infos[-1]["refinements"]
+    bytes_solve = inner * (9 * N * 4 + Ne * 4) + refin * (6 * N * 8 + Ne * 8)
+    solve_frac = bytes_solve / (ms_step * 1e-3) / HBM_PEAK
+
+    plain = None
+    if not args.no_plain:
+        sync()
+        t1 = time.perf_counter()
+        _, pinfo = job.solve(icx.SOLVE_PLAIN, tol=args.tol, maxit=50000, want_u=False)
+        sync()
+        pt = time.perf_counter() - t1
+        plain = {"ms": pt * 1e3, "iterations": pinfo["iterations"], "relres": pinfo["relres"],
+                 "dof_matvec_per_s": N * pinfo["matvecs"] / pt,
+                 "speedup_mixed_vs_plain": pt * 1e3 / ms_step}
+        t1 = time.perf_counter()
+        _, finfo = job.solve(icx.SOLVE_FUSED, tol=args.tol, maxit=50000, want_u=False)
+        sync()
+        ft = time.perf_counter() - t1
+        plain["fused_fp64_ms"] = ft * 1e3
+        plain["fused_fp64_iterations"] = finfo["iterations"]
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(nx, ny, nz, rho, args.cpu_seconds)
+
+    line = {
+        "metric": "DOF-matvec/s + % HBM peak, FP32-refined vs FP64 at res 1e-8; p50/p99 ms",
+        "value": value, "unit": "DOF-matvec/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 operator / f64 residual",
+        "data": "synthetic (seeded densities)",
+        "config": {"workload": "hexa8 cantilever %dx%dx%d, single solve to rel-res %g, mixed "
+                               "precision" % (nx, ny, nz, args.tol),
+                   "elements": Ne, "free_dofs": N, "inner_iterations": inner,
+                   "fp64_replacements": refin, "matvecs": mv,
+                   "true_relres": infos[-1]["relres"],
+                   "step_ms_p50": float(np.percentile([i["seconds"] * 1e3 for i in infos], 50)),
+                   "step_ms_p99": float(np.percentile([i["seconds"] * 1e3 for i in infos], 99)),
+                   "solve_hbm_frac": solve_frac, "fp64_plain": plain},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_hex8<float,15,15,1> (fused direction update + operator)",
+                     "avg_launch_us": avg_A * 1e6, "launches": launches,
+                     "bytes_per_launch": bytes_A},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

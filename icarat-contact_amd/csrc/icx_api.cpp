@@ -1,0 +1,1590 @@
+// Host side of the libicx C ABI (include/icx.h).
+//
+// Problem set-up follows the reference's sequence (MeshIcarat box mesh,
+// BCIcarat Dirichlet ranges, countDOF, Neumann tractions into fext_org); the
+// element operators are built once per job in FP64 and applied matrix-free on
+// the device; the PCG loops drive the kernels of icx_kernels.hip and poll a
+// small device-side state block every few iterations.
+#include "../../include/icx.h"
+#include "../../include/icx_profile.h"
+#include "icx_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace icx;
+
+struct icx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  SolverState *h_state = nullptr;  // pinned mirror for polling
+  double *h_scalar = nullptr;      // pinned scalars
+  // profile window (icx_profile.h)
+  int prof_kernel = 0;
+  std::vector<hipEvent_t> ev_pool;
+  int ev_used = 0;
+  double prof_ms = 0.0;
+  long long prof_launches = 0;
+};
+
+namespace {
+
+// -------------------------------------------------------------------------------------
+// errors
+// -------------------------------------------------------------------------------------
+icx_status fail(icx_ctx *ctx, icx_status code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail((ctx), ICX_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                        \
+  } while (0)
+
+// device buffer with RAII
+template <typename T> struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count == n && p) return hipSuccess;
+    release();
+    n = count;
+    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// events around launches of the profiled kernel
+struct ProfScope {
+  icx_ctx *c;
+  bool on;
+  ProfScope(icx_ctx *ctx, int kernel) : c(ctx), on(ctx->prof_kernel == kernel) {
+    if (!on) return;
+    if ((int)c->ev_pool.size() < c->ev_used + 2) {
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { on = false; return; }
+      c->ev_pool.push_back(a);
+      c->ev_pool.push_back(b);
+    }
+    (void)hipEventRecord(c->ev_pool[c->ev_used], c->stream);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(c->ev_pool[c->ev_used + 1], c->stream);
+    c->ev_used += 2;
+    c->prof_launches += 1;
+  }
+};
+
+// accumulate recorded event pairs (the stream must be synchronised)
+void prof_collect(icx_ctx *c) {
+  for (int i = 0; i + 1 < c->ev_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]) == hipSuccess) c->prof_ms += ms;
+  }
+  c->ev_used = 0;
+}
+
+// -------------------------------------------------------------------------------------
+// element matrices of an axis-aligned box element (FP64, 2-point Gauss rules as in
+// src/basis/gauss_integration.cpp:72-88, 268-289; B and De as in
+// src/basis/bmatrix.hpp:28-162 and src/problem/elasticity.cpp:34-98).  Local node
+// q = a + 2b + 4c is the corner (a, b, c) of the box.
+// -------------------------------------------------------------------------------------
+void material_matrix(int state, double E, double nu, double *D, int &voigt) {
+  if (state == ICX_SOLID3D) {
+    voigt = 6;
+    std::fill(D, D + 36, 0.0);
+    const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double mu = E / (2 * (1.0 + nu));
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) D[i * 6 + j] = (i == j) ? lam + 2 * mu : lam;
+    D[3 * 6 + 3] = D[4 * 6 + 4] = D[5 * 6 + 5] = mu;
+    return;
+  }
+  voigt = 3;
+  std::fill(D, D + 9, 0.0);
+  double f;
+  if (state == ICX_PLANE_STRESS) {
+    D[0] = D[4] = 1.0; D[1] = D[3] = nu; D[8] = 0.5 * (1.0 - nu);
+    f = E / (1.0 - nu * nu);
+  } else {
+    D[0] = D[4] = 1.0; D[1] = D[3] = nu / (1.0 - nu);
+    D[8] = (1.0 - 2.0 * nu) / (2.0 * (1.0 - nu));
+    f = E * ((1.0 - nu) / ((1.0 - 2.0 * nu) * (1.0 + nu)));
+  }
+  for (int i = 0; i < 9; ++i) D[i] *= f;
+}
+
+// shape-function derivatives w.r.t. physical coordinates at Gauss point gp (box element)
+struct BoxElement {
+  int ndim, nq;
+  double h[3];
+  // N[q] and dN/dX[d][q] at Gauss point ip, jac = det J * w
+  void eval(int ip, double *N, double (*dNdX)[8], double &jac) const {
+    const double g = 1.0 / std::sqrt(3.0);
+    double r[3];
+    for (int d = 0; d < ndim; ++d) r[d] = ((ip >> d) & 1) ? g : -g;
+    jac = 1.0;
+    for (int d = 0; d < ndim; ++d) jac *= 0.5 * h[d];
+    const double sc = ndim == 3 ? 0.125 : 0.25;
+    for (int q = 0; q < nq; ++q) {
+      double f[3], df[3];
+      for (int d = 0; d < ndim; ++d) {
+        const double s = ((q >> d) & 1) ? 1.0 : -1.0;
+        f[d] = 1.0 + s * r[d];
+        df[d] = s;
+      }
+      double n = sc;
+      for (int d = 0; d < ndim; ++d) n *= f[d];
+      N[q] = n;
+      for (int d = 0; d < ndim; ++d) {
+        double v = sc * df[d];
+        for (int e = 0; e < ndim; ++e)
+          if (e != d) v *= f[e];
+        dNdX[d][q] = v * 2.0 / h[d];  // dr/dX = 2/h on a box
+      }
+    }
+  }
+};
+
+// K0 (unit modulus), ndof = nq*ndim, node-major (q*ndim + c); returns the volume
+double elastic_K0(const BoxElement &el, int state, double nu, std::vector<double> &K) {
+  const int nd = el.nq * el.ndim;
+  K.assign(nd * nd, 0.0);
+  double D[36];
+  int voigt;
+  material_matrix(state, 1.0, nu, D, voigt);
+  double vol = 0.0;
+  for (int ip = 0; ip < el.nq; ++ip) {
+    double N[8], dN[3][8], jac;
+    el.eval(ip, N, dN, jac);
+    std::vector<double> B(voigt * nd, 0.0);
+    for (int q = 0; q < el.nq; ++q) {
+      if (el.ndim == 2) {
+        B[0 * nd + 2 * q] = dN[0][q];
+        B[1 * nd + 2 * q + 1] = dN[1][q];
+        B[2 * nd + 2 * q] = dN[1][q];
+        B[2 * nd + 2 * q + 1] = dN[0][q];
+      } else {
+        B[0 * nd + 3 * q] = dN[0][q];
+        B[1 * nd + 3 * q + 1] = dN[1][q];
+        B[2 * nd + 3 * q + 2] = dN[2][q];
+        B[3 * nd + 3 * q] = dN[1][q]; B[3 * nd + 3 * q + 1] = dN[0][q];
+        B[4 * nd + 3 * q + 1] = dN[2][q]; B[4 * nd + 3 * q + 2] = dN[1][q];
+        B[5 * nd + 3 * q] = dN[2][q]; B[5 * nd + 3 * q + 2] = dN[0][q];
+      }
+    }
+    for (int i = 0; i < nd; ++i)
+      for (int j = 0; j < nd; ++j) {
+        double s = 0.0;
+        for (int v = 0; v < voigt; ++v)
+          for (int w = 0; w < voigt; ++w) s += B[v * nd + i] * D[v * voigt + w] * B[w * nd + j];
+        K[i * nd + j] += s * jac;
+      }
+    vol += jac;
+  }
+  return vol;
+}
+
+// Helmholtz element matrix  sum (N N^T + r_h^2 dN^T dN) jac, r_h = R/(2 sqrt 3)
+// (src/optimization/filter_helm.hpp:178-222)
+void helm_K0(const BoxElement &el, double R, std::vector<double> &K) {
+  const int nq = el.nq;
+  K.assign(nq * nq, 0.0);
+  const double rh2 = std::pow(R / (2 * std::sqrt(3.0)), 2);
+  for (int ip = 0; ip < nq; ++ip) {
+    double N[8], dN[3][8], jac;
+    el.eval(ip, N, dN, jac);
+    for (int a = 0; a < nq; ++a)
+      for (int b = 0; b < nq; ++b) {
+        double g = 0.0;
+        for (int d = 0; d < el.ndim; ++d) g += dN[d][a] * rh2 * dN[d][b];
+        K[a * nq + b] += (N[a] * N[b] + g) * jac;
+      }
+  }
+}
+
+inline double haar_sign(int sigma, int q, int ndim) {
+  double s = 1.0;
+  for (int d = 0; d < ndim; ++d)
+    if ((sigma >> d) & 1) s *= ((q >> d) & 1) ? 1.0 : -1.0;
+  return s;
+}
+
+constexpr int popc(int v) { return (v & 1) + ((v >> 1) & 1) + ((v >> 2) & 1); }
+bool host_nz3(int chi, int c, int cc) {
+  const int pc = popc(chi);
+  if (pc == 0 || pc == 3) return true;
+  if (pc == 1) {
+    const int d = (chi & 1) ? 0 : ((chi & 2) ? 1 : 2);
+    return c != d && cc != d;
+  }
+  const int d3 = !(chi & 1) ? 0 : (!(chi & 2) ? 1 : 2);
+  return (c == d3) == (cc == d3);
+}
+bool host_nz2(int chi, int c, int cc) {
+  if (chi == 0 || chi == 3) return true;
+  const int d = (chi & 1) ? 0 : 1;
+  return c != d && cc != d;
+}
+
+// Haar-basis blocks of K0; fails if the element matrix is not block diagonal
+bool build_elastic_op(const BoxElement &el, const std::vector<double> &K, ElasticOp<double> &op,
+                      std::string &why) {
+  const int nq = el.nq, nc = el.ndim, nd = nq * nc;
+  std::vector<double> Kh(nd * nd, 0.0);  // index (c*nq + sigma)
+  for (int c = 0; c < nc; ++c)
+    for (int s = 0; s < nq; ++s)
+      for (int c2 = 0; c2 < nc; ++c2)
+        for (int s2 = 0; s2 < nq; ++s2) {
+          double v = 0.0;
+          for (int q = 0; q < nq; ++q)
+            for (int q2 = 0; q2 < nq; ++q2)
+              v += haar_sign(s, q, nc) * K[(q * nc + c) * nd + q2 * nc + c2] * haar_sign(s2, q2, nc);
+          Kh[(c * nq + s) * nd + c2 * nq + s2] = v;
+        }
+  double mx = 0.0;
+  for (double v : Kh) mx = std::max(mx, std::fabs(v));
+  const double tol = 1e-12 * mx;
+  std::memset(&op, 0, sizeof op);
+  const double scale = 1.0 / double(nq * nq);
+  for (int c = 0; c < nc; ++c)
+    for (int s = 0; s < nq; ++s)
+      for (int c2 = 0; c2 < nc; ++c2)
+        for (int s2 = 0; s2 < nq; ++s2) {
+          const int chi = s ^ (1 << c), chi2 = s2 ^ (1 << c2);
+          const double v = Kh[(c * nq + s) * nd + c2 * nq + s2];
+          const bool nz = chi == chi2 && (nc == 3 ? host_nz3(chi, c, c2) : host_nz2(chi, c, c2));
+          if (!nz) {
+            if (std::fabs(v) > tol) {
+              why = "element stiffness is not block diagonal in the Haar basis";
+              return false;
+            }
+            continue;
+          }
+          op.blk[chi][c][c2] = v * scale;
+        }
+  for (int c = 0; c < nc; ++c) {
+    op.kd[c] = K[(0 * nc + c) * nd + 0 * nc + c];
+    for (int q = 1; q < nq; ++q)
+      if (std::fabs(K[(q * nc + c) * nd + q * nc + c] - op.kd[c]) > 1e-12 * mx) {
+        why = "element diagonal differs between nodes";
+        return false;
+      }
+  }
+  return true;
+}
+
+bool build_helm_op(const BoxElement &el, const std::vector<double> &K, HelmOp<double> &op,
+                   std::string &why) {
+  const int nq = el.nq;
+  double mx = 0.0;
+  for (double v : K) mx = std::max(mx, std::fabs(v));
+  std::memset(&op, 0, sizeof op);
+  for (int s = 0; s < nq; ++s)
+    for (int s2 = 0; s2 < nq; ++s2) {
+      double v = 0.0;
+      for (int q = 0; q < nq; ++q)
+        for (int q2 = 0; q2 < nq; ++q2)
+          v += haar_sign(s, q, el.ndim) * K[q * nq + q2] * haar_sign(s2, q2, el.ndim);
+      if (s == s2) op.kf[s] = v / double(nq * nq);
+      else if (std::fabs(v) > 1e-12 * mx * nq) {
+        why = "Helmholtz element matrix is not diagonal in the Haar basis";
+        return false;
+      }
+    }
+  op.kdiag = K[0];
+  return true;
+}
+
+template <typename T> ElasticOp<T> cast_op(const ElasticOp<double> &a) {
+  ElasticOp<T> b;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) b.blk[i][j][k] = (T)a.blk[i][j][k];
+  for (int j = 0; j < 3; ++j) b.kd[j] = (T)a.kd[j];
+  return b;
+}
+
+}  // namespace
+
+// =====================================================================================
+// job
+// =====================================================================================
+struct PcgWork64 {
+  DBuf<double> r, p, t, z, invd;
+};
+struct MixedWork {
+  DBuf<float> r, pa, pb, Ap, xlo, invd, E;
+  DBuf<double> Kx;
+};
+struct FusedWork {
+  DBuf<double> r, pa, pb, Ap, invd;
+};
+struct HelmWork {
+  bool ready = false;
+  double radius = 0;
+  HelmOp<double> op;
+  DBuf<double> invd, b, x, r, p, t, z;
+};
+struct ConeWork {
+  bool ready = false;
+  double radius = 0;
+  int n = 0;
+  DBuf<int> off;
+  DBuf<double> w, aa;
+};
+struct MMAWork {
+  bool ready = false;
+  int iter = 0;
+  DBuf<double> low, upp, alpha, beta, p0, q0, p1, q1, xold1, xold2;
+};
+struct OptWork {
+  bool ready = false;
+  icx_opt_params par;
+  int step = 0;
+  double V0 = 0.0;
+  DBuf<double> x, fil, thr, dthr, dfds, dgds, tmp, tmp2;
+};
+
+struct icx_job {
+  icx_ctx *ctx = nullptr;
+  icx_problem prob{};
+  Grid g{};
+  int nc = 0;          // dofnp
+  int numeq = -1;
+  BoxElement el{};
+  double volume = 0.0; // element volume
+  ElasticOp<double> op64{};
+  ElasticOp<float> op32{};
+  // host-side BC state (reference Node::dirich)
+  std::vector<int> has, node_flag;
+  std::vector<double> node_val;
+  std::vector<int> dof;
+  std::vector<double> fext;
+  bool any_dirich_value = false;
+  // device state
+  DBuf<int> d_dof;
+  DBuf<uint8_t> d_mask;
+  DBuf<double> d_fext, d_dirich, d_b, d_u, d_design, d_E, d_vec, d_vec2;
+  DBuf<double> d_partial;
+  DBuf<unsigned> d_counter;
+  DBuf<SolverState> d_state;
+  DBuf<double> d_scalars;
+  bool design_set = false;
+  bool u_valid = false;
+  PcgWork64 pw;
+  MixedWork mw;
+  FusedWork fw;
+  HelmWork hw;
+  ConeWork cw;
+  MMAWork mma;
+  OptWork opt;
+
+  Reduce red() const { return Reduce{d_partial.p, d_counter.p}; }
+  long long nvec() const { return (long long)nc * g.nn; }
+};
+
+namespace {
+
+hipStream_t S(icx_job *j) { return j->ctx->stream; }
+
+icx_status read_state(icx_job *j, SolverState &st) {
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, hipMemcpyAsync(c->h_state, j->d_state.p, sizeof(SolverState), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  st = *c->h_state;
+  if (c->ev_used) prof_collect(c);
+  return ICX_OK;
+}
+
+icx_status read_scalars(icx_job *j, int n, double *out) {
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, hipMemcpyAsync(c->h_scalar, j->d_scalars.p, n * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  for (int i = 0; i < n; ++i) out[i] = c->h_scalar[i];
+  return ICX_OK;
+}
+
+icx_status upload_fext(icx_job *j) {
+  icx_ctx *c = j->ctx;
+  DBuf<double> tmp;
+  HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
+  HIPCHK(c, hipMemcpyAsync(tmp.p, j->fext.data(), j->numeq * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  launch_dof_to_grid(j->g, j->nc, j->d_dof.p, j->numeq, tmp.p, nullptr, j->d_fext.p, S(j));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+// rhs b = fext - K u_D on the free DOFs (src/analysis/linear.hpp:62-80)
+icx_status build_rhs(icx_job *j) {
+  const long long N = j->nvec();
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, hipMemcpyAsync(j->d_b.p, j->d_fext.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  if (j->any_dirich_value) {
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr,
+                                  Reduce{nullptr, nullptr}, S(j));
+    launch_axpby(N, -1.0, j->d_vec.p, 1.0, j->d_b.p, S(j));
+    launch_mask_zero(j->g, j->nc, j->d_mask.p, j->d_b.p, S(j));
+  }
+  return ICX_OK;
+}
+
+// ---- FP64 plain PCG, Eigen ConjugateGradient loop --------------------------------
+template <class MatvecFn>
+icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const double *invd,
+                     PcgWork64 &w, double tol, int maxit, MatvecFn matvec, icx_solve_info &info) {
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, hipMemsetAsync(x, 0, N * sizeof(double), S(j)));
+  launch_pcg_init<double>(N, j->d_state.p, b, w.r.p, w.p.p, invd, j->red(), tol, maxit, 1, S(j));
+  SolverState st;
+  int polled = 0;
+  const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
+  for (int it = 0; it < maxit + 1; ++it) {
+    matvec(w.p.p, w.t.p);  // t = A p, pAp, alpha
+    launch_pcg_update<double>(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, j->red(), S(j));
+    launch_pcg_pupdate<double>(N, j->d_state.p, w.p.p, w.z.p, S(j));
+    if (++polled == every || it == maxit) {
+      polled = 0;
+      icx_status rc = read_state(j, st);
+      if (rc) return rc;
+      if (st.done) break;
+    }
+  }
+  icx_status rc = read_state(j, st);
+  if (rc) return rc;
+  info.iterations = st.iter;
+  info.matvecs = st.iter + (st.done == 1 && st.rhsNorm2 > 0 && st.iter < maxit ? 1 : 0);
+  info.fp64_matvecs = info.matvecs;
+  info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.rr / st.rhsNorm2) : 0.0;
+  return st.done == 1 ? ICX_OK : ICX_ERR_NOCONV;
+}
+
+}  // namespace
+
+// =====================================================================================
+// C ABI
+// =====================================================================================
+extern "C" {
+
+icx_status icx_ctx_create(int device, icx_ctx **out) {
+  if (!out) return ICX_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return ICX_ERR_HIP;
+  if (device < 0 || device >= ndev) return ICX_ERR_ARG;
+  auto *c = new icx_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_scalar, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return ICX_ERR_HIP;
+  }
+  *out = c;
+  return ICX_OK;
+}
+
+icx_status icx_profile_begin(icx_ctx *ctx, int kernel) {
+  if (!ctx) return ICX_ERR_ARG;
+  if (kernel < ICX_PROF_MATVEC64 || kernel > ICX_PROF_FUSED_B) return fail(ctx, ICX_ERR_ARG, "unknown kernel id");
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  ctx->prof_kernel = kernel;
+  ctx->prof_ms = 0.0;
+  ctx->prof_launches = 0;
+  return ICX_OK;
+}
+
+icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches) {
+  if (!ctx) return ICX_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  prof_collect(ctx);
+  if (seconds) *seconds = ctx->prof_ms * 1e-3;
+  if (launches) *launches = ctx->prof_launches;
+  ctx->prof_kernel = 0;
+  return ICX_OK;
+}
+
+icx_status icx_ctx_destroy(icx_ctx *ctx) {
+  if (!ctx) return ICX_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->h_state) (void)hipHostFree(ctx->h_state);
+  if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
+  delete ctx;
+  return ICX_OK;
+}
+
+const char *icx_last_error(const icx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+icx_status icx_ctx_synchronize(icx_ctx *ctx) {
+  if (!ctx) return ICX_ERR_ARG;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ICX_OK;
+}
+
+icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
+  if (!ctx || !p || !out) return ICX_ERR_ARG;
+  *out = nullptr;
+  if (p->eltype != ICX_QUAD4 && p->eltype != ICX_HEXA8)
+    return fail(ctx, ICX_ERR_ARG, "eltype must be ICX_QUAD4 or ICX_HEXA8");
+  const int ndim = p->eltype == ICX_QUAD4 ? 2 : 3;
+  for (int d = 0; d < ndim; ++d)
+    if (p->div[d] < 1 || !(p->length[d] > 0.0))
+      return fail(ctx, ICX_ERR_ARG, "division and length must be positive");
+  if ((ndim == 3) != (p->state == ICX_SOLID3D))
+    return fail(ctx, ICX_ERR_ARG, "state must be ICX_SOLID3D exactly for hexa8");
+  if (!(p->poisson > -1.0 && p->poisson < 0.5)) return fail(ctx, ICX_ERR_ARG, "poisson out of range");
+  auto *j = new icx_job();
+  j->ctx = ctx;
+  j->prob = *p;
+  Grid &g = j->g;
+  g.ndim = ndim;
+  g.nx = p->div[0]; g.ny = p->div[1]; g.nz = ndim == 3 ? p->div[2] : 1;
+  g.sx = g.nx + 1; g.sy = g.ny + 1; g.sz = ndim == 3 ? g.nz + 1 : 1;
+  g.nn = (long long)g.sx * g.sy * g.sz;
+  g.ne = (long long)g.nx * g.ny * g.nz;
+  if (g.nn > (1LL << 31) / 4) { delete j; return fail(ctx, ICX_ERR_ARG, "mesh too large"); }
+  j->nc = ndim;
+  j->el.ndim = ndim;
+  j->el.nq = ndim == 3 ? 8 : 4;
+  for (int d = 0; d < 3; ++d) j->el.h[d] = d < ndim ? p->length[d] / p->div[d] : 1.0;
+  std::vector<double> K;
+  j->volume = elastic_K0(j->el, p->state, p->poisson, K);
+  std::string why;
+  if (!build_elastic_op(j->el, K, j->op64, why)) { delete j; return fail(ctx, ICX_ERR_NUMERIC, "%s", why.c_str()); }
+  j->op32 = cast_op<float>(j->op64);
+  j->has.assign(g.nn, 0);
+  j->node_flag.assign(g.nn * 6, 0);
+  j->node_val.assign(g.nn * 6, 0.0);
+  (void)hipSetDevice(ctx->device);
+  const long long N = j->nvec();
+  if (j->d_partial.alloc(2 * RED_MAXBLOCKS) || j->d_counter.alloc(8) || j->d_state.alloc(1) ||
+      j->d_scalars.alloc(64) || j->d_fext.alloc(N) || j->d_b.alloc(N) || j->d_u.alloc(N) ||
+      j->d_design.alloc(g.ne) || j->d_E.alloc(g.ne) || j->d_vec.alloc(N) || j->d_vec2.alloc(N) ||
+      j->d_mask.alloc(g.nn)) {
+    delete j;
+    return fail(ctx, ICX_ERR_NOMEM, "device allocation failed");
+  }
+  (void)hipMemsetAsync(j->d_counter.p, 0, 8 * sizeof(unsigned), ctx->stream);
+  (void)hipMemsetAsync(j->d_state.p, 0, sizeof(SolverState), ctx->stream);
+  (void)hipMemsetAsync(j->d_fext.p, 0, N * sizeof(double), ctx->stream);
+  (void)hipMemsetAsync(j->d_mask.p, 0, g.nn, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) { delete j; return fail(ctx, ICX_ERR_HIP, "stream sync failed"); }
+  *out = j;
+  return ICX_OK;
+}
+
+icx_status icx_job_destroy(icx_job *j) {
+  if (!j) return ICX_ERR_ARG;
+  (void)hipSetDevice(j->ctx->device);
+  (void)hipStreamSynchronize(j->ctx->stream);
+  delete j;
+  return ICX_OK;
+}
+
+icx_status icx_job_sizes(const icx_job *j, int *numnp, int *nelx, int *neq) {
+  if (!j) return ICX_ERR_ARG;
+  if (numnp) *numnp = (int)j->g.nn;
+  if (nelx) *nelx = (int)j->g.ne;
+  if (neq) *neq = (int)(j->g.nn * j->nc);
+  return ICX_OK;
+}
+
+// node coordinates x = (L/div) * index (src/io/mesh_icarat.hpp:325-335, 440-455)
+static void node_coord(const icx_job *j, long long n, double *x) {
+  const Grid &g = j->g;
+  const long long i = n % g.sx, jj = (n / g.sx) % g.sy, k = n / ((long long)g.sx * g.sy);
+  x[0] = (j->prob.length[0] / j->prob.div[0]) * (double)i;
+  x[1] = (j->prob.length[1] / j->prob.div[1]) * (double)jj;
+  x[2] = g.ndim == 3 ? (j->prob.length[2] / j->prob.div[2]) * (double)k : 0.0;
+}
+
+// local corner (a,b,c) of the reference's element node order
+static const int kRefCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                     {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+static long long elem_node(const icx_job *j, long long e, int k) {
+  const Grid &g = j->g;
+  const long long ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / ((long long)g.nx * g.ny);
+  const int *cc = kRefCorner[k];
+  return (ex + cc[0]) + (long long)g.sx * ((ey + cc[1]) + (long long)g.sy * (ez + cc[2]));
+}
+
+icx_status icx_get_mesh(const icx_job *j, double *x, int *conn) {
+  if (!j) return ICX_ERR_ARG;
+  if (x)
+    for (long long n = 0; n < j->g.nn; ++n) node_coord(j, n, x + 3 * n);
+  if (conn) {
+    const int ne = j->el.nq;
+    for (long long e = 0; e < j->g.ne; ++e)
+      for (int k = 0; k < ne; ++k) conn[e * ne + k] = (int)elem_node(j, e, k);
+  }
+  return ICX_OK;
+}
+
+icx_status icx_add_dirichlet(icx_job *j, const int *flag6, const double *val6, const double *range6) {
+  if (!j || !flag6 || !val6 || !range6) return ICX_ERR_ARG;
+  if (j->numeq >= 0) return fail(j->ctx, ICX_ERR_STATE, "Dirichlet ranges must precede icx_make_dof");
+  const double tol = 1.0e-7;
+  const double lo[3] = {range6[0] - tol, range6[2] - tol, range6[4] - tol};
+  const double hi[3] = {range6[1] + tol, range6[3] + tol, range6[5] + tol};
+  for (long long n = 0; n < j->g.nn; ++n) {
+    double x[3];
+    node_coord(j, n, x);
+    bool in = true;
+    for (int d = 0; d < j->g.ndim; ++d) in = in && x[d] >= lo[d] && x[d] <= hi[d];
+    if (!in) continue;
+    if (!j->has[n]) {
+      j->has[n] = 1;
+      for (int k = 0; k < 6; ++k) { j->node_flag[6 * n + k] = 0; j->node_val[6 * n + k] = 0.0; }
+    }
+    for (int k = 0; k < j->nc; ++k) {
+      if (j->node_flag[6 * n + k] != 1) j->node_flag[6 * n + k] = flag6[k];
+      j->node_val[6 * n + k] = val6[k];
+    }
+  }
+  return ICX_OK;
+}
+
+icx_status icx_make_dof(icx_job *j, int *numeq) {
+  if (!j) return ICX_ERR_ARG;
+  const int nc = j->nc;
+  const long long nn = j->g.nn;
+  j->dof.assign(nn * nc, 0);
+  int counter = 0;
+  for (long long n = 0; n < nn; ++n)
+    for (int c = 0; c < nc; ++c)
+      if (!j->has[n] || j->node_flag[6 * n + c] == 0) j->dof[n * nc + c] = counter++;
+  const int neq_free = counter;
+  for (long long n = 0; n < nn; ++n)
+    if (j->has[n])
+      for (int c = 0; c < nc; ++c)
+        if (j->node_flag[6 * n + c] != 0) j->dof[n * nc + c] = counter++;
+  j->numeq = neq_free;
+  j->fext.assign(neq_free, 0.0);
+  std::vector<uint8_t> mask(nn, 0);
+  std::vector<double> dir(nn * 6, 0.0);
+  j->any_dirich_value = false;
+  for (long long n = 0; n < nn; ++n)
+    for (int c = 0; c < nc; ++c)
+      if (j->dof[n * nc + c] >= neq_free) {
+        mask[n] |= (uint8_t)(1u << c);
+        dir[6 * n + c] = j->node_val[6 * n + c];
+        if (dir[6 * n + c] != 0.0) j->any_dirich_value = true;
+      }
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, j->d_dof.alloc(nn * nc));
+  HIPCHK(c, hipMemcpyAsync(j->d_dof.p, j->dof.data(), nn * nc * sizeof(int), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(j->d_mask.p, mask.data(), nn, hipMemcpyHostToDevice, S(j)));
+  if (j->any_dirich_value) {
+    // Dirichlet values as a grid vector (zero on free DOFs)
+    std::vector<double> grid(nn * nc, 0.0);
+    for (long long n = 0; n < nn; ++n)
+      for (int cc = 0; cc < nc; ++cc)
+        if (mask[n] >> cc & 1) grid[cc * nn + n] = dir[6 * n + cc];
+    HIPCHK(c, j->d_dirich.alloc(nn * 6));
+    HIPCHK(c, hipMemcpyAsync(j->d_dirich.p, grid.data(), nn * nc * sizeof(double), hipMemcpyHostToDevice, S(j)));
+    // keep the per-node layout too for node-value output
+    j->node_val.resize(nn * 6);
+  }
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  icx_status rc = upload_fext(j);
+  if (rc) return rc;
+  if (numeq) *numeq = neq_free;
+  return ICX_OK;
+}
+
+icx_status icx_get_dof_map(const icx_job *j, int *dof) {
+  if (!j || !dof) return ICX_ERR_ARG;
+  if (j->numeq < 0) return ICX_ERR_STATE;
+  std::memcpy(dof, j->dof.data(), j->dof.size() * sizeof(int));
+  return ICX_OK;
+}
+
+// Neumann tractions: generateLoadMain + SurfaceLoad::generate
+// (src/io/boundary_condition_icarat.hpp:300-396, src/io/boundary_condition_base.cpp:16-180).
+// For each element, at the numnode-th of its nodes (local order) inside the box, the
+// side is chosen by judgeLine and the side's consistent load sum N^T t |J| w is added.
+icx_status icx_add_neumann(icx_job *j, const double *val3, const double *range6) {
+  if (!j || !val3 || !range6) return ICX_ERR_ARG;
+  if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_add_neumann needs icx_make_dof first");
+  const double tol = 1.0e-7;
+  const double lo[3] = {range6[0] - tol, range6[2] - tol, range6[4] - tol};
+  const double hi[3] = {range6[1] + tol, range6[3] + tol, range6[5] + tol};
+  const int ne = j->el.nq, nd = j->g.ndim, nc = j->nc;
+  const int numnode = ne == 4 ? 2 : 4;
+  static const int side4[5][2] = {{0, 0}, {0, 1}, {1, 2}, {2, 3}, {3, 0}};
+  static const int side8[7][4] = {{0, 0, 0, 0}, {0, 1, 2, 3}, {3, 0, 4, 7}, {0, 1, 5, 4},
+                                  {1, 2, 6, 5}, {2, 3, 7, 6}, {4, 5, 6, 7}};
+  const double g = 1.0 / std::sqrt(3.0);
+  for (long long e = 0; e < j->g.ne; ++e) {
+    int hits[8], cnt = 0;
+    for (int k = 0; k < ne; ++k) {
+      double x[3];
+      node_coord(j, elem_node(j, e, k), x);
+      bool in = true;
+      for (int d = 0; d < nd; ++d) in = in && lo[d] <= x[d] && x[d] <= hi[d];
+      if (!in) continue;
+      hits[cnt++] = k;
+      if (cnt != numnode) continue;
+      int side;
+      if (ne == 4) side = (k == 1) ? 1 : (k == 2 ? 2 : (hits[1] - hits[0] == 1 ? 3 : 4));
+      else side = (k == 3) ? 1 : (k == 5 ? 3 : (k == 6 ? 4 : (hits[2] - hits[1] == 1 ? 2 : (hits[1] - hits[0] == 1 ? 5 : 6))));
+      const int *sn = ne == 4 ? side4[side] : side8[side];
+      const int ns = ne == 4 ? 2 : 4;
+      double X[4][3];
+      for (int a = 0; a < ns; ++a) node_coord(j, elem_node(j, e, sn[a]), X[a]);
+      double fe[4][3] = {};
+      if (ne == 4) {
+        for (int ip = 0; ip < 2; ++ip) {
+          const double r = ip == 0 ? -g : g;
+          const double N[2] = {0.5 * (1.0 - r), 0.5 * (1.0 + r)};
+          const double d0 = 0.5 * (X[1][0] - X[0][0]), d1 = 0.5 * (X[1][1] - X[0][1]);
+          const double jac = std::sqrt(d0 * d0 + d1 * d1);
+          for (int a = 0; a < 2; ++a)
+            for (int cc = 0; cc < nc; ++cc) fe[a][cc] += N[a] * val3[cc] * jac;
+        }
+      } else {
+        static const int sg[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+        static const int gp[4][2] = {{-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+        for (int ip = 0; ip < 4; ++ip) {
+          const double r0 = gp[ip][0] * g, r1 = gp[ip][1] * g;
+          double N[4], t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
+          for (int a = 0; a < 4; ++a) {
+            N[a] = 0.25 * (1.0 + sg[a][0] * r0) * (1.0 + sg[a][1] * r1);
+            const double dr = 0.25 * sg[a][0] * (1.0 + sg[a][1] * r1);
+            const double ds = 0.25 * sg[a][1] * (1.0 + sg[a][0] * r0);
+            for (int d = 0; d < 3; ++d) { t1[d] += dr * X[a][d]; t2[d] += ds * X[a][d]; }
+          }
+          const double cx = t1[1] * t2[2] - t1[2] * t2[1], cy = t1[2] * t2[0] - t1[0] * t2[2],
+                       cz = t1[0] * t2[1] - t1[1] * t2[0];
+          const double jac = std::sqrt(cx * cx + cy * cy + cz * cz);
+          for (int a = 0; a < 4; ++a)
+            for (int cc = 0; cc < 3; ++cc) fe[a][cc] += N[a] * val3[cc] * jac;
+        }
+      }
+      for (int a = 0; a < ns; ++a) {
+        const long long n = elem_node(j, e, sn[a]);
+        for (int cc = 0; cc < nc; ++cc) {
+          const int id = j->dof[n * nc + cc];
+          if (id < j->numeq) j->fext[id] += fe[a][cc];
+        }
+      }
+    }
+  }
+  return upload_fext(j);
+}
+
+icx_status icx_set_fext(icx_job *j, const double *fext) {
+  if (!j || !fext) return ICX_ERR_ARG;
+  if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_set_fext needs icx_make_dof first");
+  std::memcpy(j->fext.data(), fext, j->numeq * sizeof(double));
+  return upload_fext(j);
+}
+
+icx_status icx_get_fext(const icx_job *j, double *fext) {
+  if (!j || !fext) return ICX_ERR_ARG;
+  if (j->numeq < 0) return ICX_ERR_STATE;
+  std::memcpy(fext, j->fext.data(), j->numeq * sizeof(double));
+  return ICX_OK;
+}
+
+icx_status icx_add_nodal_load(icx_job *j, int node, int comp, double value) {
+  if (!j) return ICX_ERR_ARG;
+  if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_add_nodal_load needs icx_make_dof first");
+  if (node < 0 || node >= j->g.nn || comp < 0 || comp >= j->nc) return fail(j->ctx, ICX_ERR_ARG, "node/comp out of range");
+  const int id = j->dof[(long long)node * j->nc + comp];
+  if (id < j->numeq) j->fext[id] += value;
+  return upload_fext(j);
+}
+
+icx_status icx_set_design(icx_job *j, const double *design) {
+  if (!j || !design) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, hipMemcpyAsync(j->d_design.p, design, j->g.ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  launch_simp(j->g.ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  j->design_set = true;
+  j->u_valid = false;
+  return ICX_OK;
+}
+
+icx_status icx_matvec(icx_job *j, int prec, const double *x, double *y) {
+  if (!j || !x || !y) return ICX_ERR_ARG;
+  if (j->numeq < 0 || !j->design_set) return fail(j->ctx, ICX_ERR_STATE, "icx_matvec needs icx_make_dof and icx_set_design");
+  icx_ctx *c = j->ctx;
+  const long long N = j->nvec();
+  DBuf<double> tmp;
+  HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
+  HIPCHK(c, hipMemcpyAsync(tmp.p, x, j->numeq * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  launch_dof_to_grid(j->g, j->nc, j->d_dof.p, j->numeq, tmp.p, nullptr, j->d_vec.p, S(j));
+  if (prec == 0) {
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr,
+                                  Reduce{nullptr, nullptr}, S(j));
+  } else {
+    DBuf<float> xf, yf, Ef;
+    HIPCHK(c, xf.alloc(N)); HIPCHK(c, yf.alloc(N)); HIPCHK(c, Ef.alloc(j->g.ne));
+    launch_convert_f64_f32(N, j->d_vec.p, xf.p, S(j));
+    launch_convert_f64_f32(j->g.ne, j->d_E.p, Ef.p, S(j));
+    launch_elastic_matvec<float>(j->g, j->op32, Ef.p, j->d_mask.p, xf.p, yf.p, nullptr,
+                                 Reduce{nullptr, nullptr}, S(j));
+    // widen
+    std::vector<float> hy(N);
+    HIPCHK(c, hipMemcpyAsync(hy.data(), yf.p, N * sizeof(float), hipMemcpyDeviceToHost, S(j)));
+    HIPCHK(c, hipStreamSynchronize(S(j)));
+    std::vector<double> hd(hy.begin(), hy.end());
+    HIPCHK(c, hipMemcpyAsync(j->d_vec2.p, hd.data(), N * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  }
+  launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_vec2.p, tmp.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(y, tmp.p, j->numeq * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_diagonal(icx_job *j, double *diag) {
+  if (!j || !diag) return ICX_ERR_ARG;
+  if (j->numeq < 0 || !j->design_set) return ICX_ERR_STATE;
+  icx_ctx *c = j->ctx;
+  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, j->d_vec.p, nullptr, S(j));
+  DBuf<double> tmp;
+  HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
+  launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_vec.p, tmp.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(diag, tmp.p, j->numeq * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  for (int i = 0; i < j->numeq; ++i) diag[i] = 1.0 / diag[i];
+  return ICX_OK;
+}
+
+// ---- structural solves ------------------------------------------------------------------
+static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info &info) {
+  icx_ctx *c = j->ctx;
+  const long long N = j->nvec();
+  PcgWork64 &w = j->pw;
+  HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.p.alloc(N)); HIPCHK(c, w.t.alloc(N));
+  HIPCHK(c, w.z.alloc(N)); HIPCHK(c, w.invd.alloc(N));
+  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
+  auto mv = [&](const double *p, double *t) {
+    ProfScope ps(c, ICX_PROF_MATVEC64);
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, j->red(), S(j));
+  };
+  return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info);
+}
+
+static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info &info) {
+  icx_ctx *c = j->ctx;
+  const long long N = j->nvec();
+  FusedWork &w = j->fw;
+  HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N));
+  HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N));
+  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
+  HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
+  HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(double), S(j)));
+  launch_pcg_init<double>(N, j->d_state.p, j->d_b.p, w.r.p, nullptr, w.invd.p, j->red(), tol, maxit, 0, S(j));
+  // no replacement triggers in the FP64 fused form
+  SolverState st;
+  double *pa = w.pa.p, *pb = w.pb.p;
+  const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
+  int k = 0;
+  for (;;) {
+    {
+      ProfScope ps(c, ICX_PROF_FUSED_A);
+      launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, pa, pb, w.Ap.p,
+                                    j->d_u.p, j->d_state.p, j->red(), S(j));
+    }
+    {
+      ProfScope ps(c, ICX_PROF_FUSED_B);
+      launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, j->red(), S(j));
+    }
+    std::swap(pa, pb);
+    if (++k % every == 0) {
+      icx_status rc = read_state(j, st);
+      if (rc) return rc;
+      if (st.done) break;
+    }
+  }
+  // x += alpha p for the last step (pa is the latest direction)
+  launch_flush_x<double>(N, j->d_state.p, j->d_u.p, pa, S(j));
+  icx_status rc = read_state(j, st);
+  if (rc) return rc;
+  info.matvecs = st.iter;
+  info.fp64_matvecs = st.iter;
+  info.iterations = st.done == 3 ? st.iter - 1 : st.iter;
+  info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.rr / st.rhsNorm2) : 0.0;
+  return (st.done == 3 || st.done == 1) ? ICX_OK : ICX_ERR_NOCONV;
+}
+
+// FP32 inner PCG on the FP32 operator, FP64 x accumulation and FP64 residual
+// replacement (reliable updates): whenever the FP32 recursive residual has dropped
+// by a factor delta since the last replacement (or meets the tolerance), x_hi += x_lo
+// and r = b - K x_hi is recomputed with the FP64 operator; the search direction is
+// kept.  Convergence is declared on the FP64 residual only.
+static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info &info) {
+  icx_ctx *c = j->ctx;
+  const long long N = j->nvec();
+  MixedWork &w = j->mw;
+  HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N)); HIPCHK(c, w.Ap.alloc(N));
+  HIPCHK(c, w.xlo.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.E.alloc(j->g.ne)); HIPCHK(c, w.Kx.alloc(N));
+  launch_convert_f64_f32(j->g.ne, j->d_E.p, w.E.p, S(j));
+  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, nullptr, w.invd.p, S(j));
+  HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
+  HIPCHK(c, hipMemsetAsync(w.xlo.p, 0, N * sizeof(float), S(j)));
+  HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(float), S(j)));
+  // r32 = b (rounded); norms from the FP64 b
+  launch_convert_f64_f32(N, j->d_b.p, w.r.p, S(j));
+  launch_pcg_init<float>(N, j->d_state.p, w.r.p, w.r.p, nullptr, w.invd.p, j->red(), tol, maxit, 0, S(j));
+  launch_dot(N, j->d_b.p, j->d_b.p, j->d_scalars.p, j->red(), S(j));
+  double bb;
+  icx_status rc = read_scalars(j, 1, &bb);
+  if (rc) return rc;
+  SolverState st;
+  rc = read_state(j, st);
+  if (rc) return rc;
+  st.rhsNorm2 = bb;
+  st.threshold = std::max(tol * tol * bb, 2.2250738585072014e-308);
+  st.done = (bb == 0.0 || bb < st.threshold) ? 1 : 0;
+  st.delta2 = 0.01;        // replace when |r| fell by 10x
+  st.replace_every = 0;
+  st.rr_ref = st.rr;
+  HIPCHK(c, hipMemcpy(j->d_state.p, &st, sizeof(SolverState), hipMemcpyHostToDevice));
+  float *pa = w.pa.p, *pb = w.pb.p;
+  const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
+  int refinements = 0, fp64mv = 0;
+  bool converged = st.done == 1;
+  int k = 0;
+  while (!converged) {
+    {
+      ProfScope ps(c, ICX_PROF_FUSED_A);
+      launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, pa, pb, w.Ap.p, w.xlo.p,
+                                   j->d_state.p, j->red(), S(j));
+    }
+    {
+      ProfScope ps(c, ICX_PROF_FUSED_B);
+      launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, j->red(), S(j));
+    }
+    std::swap(pa, pb);
+    if (++k % every) continue;
+    rc = read_state(j, st);
+    if (rc) return rc;
+    if (st.done == 0) continue;
+    // replacement: x_hi += x_lo + alpha p; r = b - K x_hi (FP64)
+    launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, pa, S(j));
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr,
+                                  Reduce{nullptr, nullptr}, S(j));
+    launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, j->red(), S(j));
+    ++refinements; ++fp64mv;
+    const int was = st.done;
+    rc = read_state(j, st);
+    if (rc) return rc;
+    if (st.done == 1) { converged = true; break; }
+    if (was == 2 || st.iter >= maxit) break;
+  }
+  rc = read_state(j, st);
+  if (rc) return rc;
+  info.iterations = st.iter;
+  info.matvecs = st.iter + fp64mv;
+  info.fp64_matvecs = fp64mv;
+  info.refinements = refinements;
+  info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.true_rr / st.rhsNorm2) : 0.0;
+  return converged ? ICX_OK : ICX_ERR_NOCONV;
+}
+
+icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx_solve_info *info) {
+  if (!j) return ICX_ERR_ARG;
+  if (j->numeq < 0 || !j->design_set)
+    return fail(j->ctx, ICX_ERR_STATE, "icx_solve needs icx_make_dof and icx_set_design");
+  if (!(tol > 0.0) || maxit < 1) return fail(j->ctx, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
+  icx_ctx *c = j->ctx;
+  (void)hipSetDevice(c->device);
+  icx_solve_info inf{};
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventRecord(e0, S(j)));
+  icx_status rc = build_rhs(j);
+  if (!rc) {
+    if (form == ICX_SOLVE_PLAIN) rc = solve_plain(j, tol, maxit, inf);
+    else if (form == ICX_SOLVE_FUSED) rc = solve_fused(j, tol, maxit, inf);
+    else if (form == ICX_SOLVE_MIXED) rc = solve_mixed(j, tol, maxit, inf);
+    else rc = fail(c, ICX_ERR_ARG, "unknown solve form %d", form);
+  }
+  (void)hipEventRecord(e1, S(j));
+  (void)hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  inf.seconds = ms * 1e-3;
+  if (info) *info = inf;
+  if (rc && rc != ICX_ERR_NOCONV) return rc;
+  j->u_valid = true;
+  if (u) {
+    icx_status r2 = icx_get_displacement(j, u);
+    if (r2) return r2;
+  }
+  if (rc == ICX_ERR_NOCONV) fail(c, rc, "PCG reached maxit without meeting the tolerance");
+  return rc;
+}
+
+icx_status icx_get_displacement(const icx_job *jc, double *u) {
+  icx_job *j = const_cast<icx_job *>(jc);
+  if (!j || !u) return ICX_ERR_ARG;
+  if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
+  icx_ctx *c = j->ctx;
+  DBuf<double> tmp;
+  HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
+  launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_u.p, tmp.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(u, tmp.p, j->numeq * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_get_node_values(const icx_job *jc, double *nodeval) {
+  icx_job *j = const_cast<icx_job *>(jc);
+  if (!j || !nodeval) return ICX_ERR_ARG;
+  if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
+  icx_ctx *c = j->ctx;
+  DBuf<double> tmp, dir;
+  HIPCHK(c, tmp.alloc(j->g.nn * 3));
+  const double *dptr = nullptr;
+  if (j->any_dirich_value) {
+    HIPCHK(c, dir.alloc(j->g.nn * 6));
+    HIPCHK(c, hipMemcpyAsync(dir.p, j->node_val.data(), j->g.nn * 6 * sizeof(double), hipMemcpyHostToDevice, S(j)));
+    dptr = dir.p;
+  }
+  launch_node_values(j->g, j->nc, j->d_dof.p, j->numeq, j->d_u.p, dptr, tmp.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(nodeval, tmp.p, j->g.nn * 3 * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_compliance(icx_job *j, double *cval) {
+  if (!j || !cval) return ICX_ERR_ARG;
+  if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
+  launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
+  return read_scalars(j, 1, cval);
+}
+
+icx_status icx_sensitivity(icx_job *j, double vscale, double *dfds, double *dgds) {
+  if (!j) return ICX_ERR_ARG;
+  if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
+  icx_ctx *c = j->ctx;
+  DBuf<double> a, b;
+  HIPCHK(c, a.alloc(j->g.ne));
+  HIPCHK(c, b.alloc(j->g.ne));
+  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_u.p,
+                     j->volume, vscale, a.p, b.p, S(j));
+  if (dfds) HIPCHK(c, hipMemcpyAsync(dfds, a.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  if (dgds) HIPCHK(c, hipMemcpyAsync(dgds, b.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_element_volume(const icx_job *j, double *vol) {
+  if (!j || !vol) return ICX_ERR_ARG;
+  for (long long e = 0; e < j->g.ne; ++e) vol[e] = j->volume;
+  return ICX_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// filters, projection, updates, design loop
+// =====================================================================================
+namespace {
+
+icx_status cone_setup(icx_job *j, double radius) {
+  icx_ctx *c = j->ctx;
+  ConeWork &w = j->cw;
+  if (w.ready && w.radius == radius) return ICX_OK;
+  const Grid &g = j->g;
+  const double *h = j->el.h;
+  const int R[3] = {(int)std::ceil(radius / h[0]) + 1, (int)std::ceil(radius / h[1]) + 1,
+                    g.ndim == 3 ? (int)std::ceil(radius / h[2]) + 1 : 0};
+  std::vector<int> off;
+  std::vector<double> wt;
+  // increasing neighbour id = lexicographic (dz, dy, dx)
+  for (int dz = -R[2]; dz <= R[2]; ++dz)
+    for (int dy = -R[1]; dy <= R[1]; ++dy)
+      for (int dx = -R[0]; dx <= R[0]; ++dx) {
+        const double a = dx * h[0], b = dy * h[1], cc = dz * h[2];
+        double s = a * a + b * b;
+        if (g.ndim == 3) s += cc * cc;
+        const double d = std::sqrt(s);
+        if (d <= radius) {
+          off.push_back(dx); off.push_back(dy); off.push_back(dz);
+          wt.push_back(radius - d);
+        }
+      }
+  w.n = (int)wt.size();
+  HIPCHK(c, w.off.alloc(off.size()));
+  HIPCHK(c, w.w.alloc(wt.size()));
+  HIPCHK(c, w.aa.alloc(g.ne));
+  HIPCHK(c, hipMemcpyAsync(w.off.p, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(w.w.p, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  launch_cone_aa(g, Stencil{w.n, w.off.p, w.w.p}, w.aa.p, S(j));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  w.ready = true;
+  w.radius = radius;
+  return ICX_OK;
+}
+
+Stencil stencil(icx_job *j) { return Stencil{j->cw.n, j->cw.off.p, j->cw.w.p}; }
+
+icx_status helm_setup(icx_job *j, double radius) {
+  icx_ctx *c = j->ctx;
+  HelmWork &w = j->hw;
+  if (w.ready && w.radius == radius) return ICX_OK;
+  std::vector<double> K;
+  helm_K0(j->el, radius, K);
+  std::string why;
+  if (!build_helm_op(j->el, K, w.op, why)) return fail(c, ICX_ERR_NUMERIC, "%s", why.c_str());
+  const long long nn = j->g.nn;
+  HIPCHK(c, w.invd.alloc(nn)); HIPCHK(c, w.b.alloc(nn)); HIPCHK(c, w.x.alloc(nn));
+  HIPCHK(c, w.r.alloc(nn)); HIPCHK(c, w.p.alloc(nn)); HIPCHK(c, w.t.alloc(nn)); HIPCHK(c, w.z.alloc(nn));
+  launch_helm_invdiag(j->g, w.op.kdiag, w.invd.p, S(j));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  w.ready = true;
+  w.radius = radius;
+  return ICX_OK;
+}
+
+// device-to-device Helmholtz filter: in/out are element vectors on the device
+icx_status helm_apply_dev(icx_job *j, int sens, double tol, const double *din, double *dout, int *iters) {
+  HelmWork &w = j->hw;
+  const double nq = j->el.nq;
+  const double rhs_scale = sens ? j->volume / (nq * nq) : j->volume / nq;
+  launch_helm_rhs(j->g, rhs_scale, din, w.b.p, S(j));
+  PcgWork64 pw;  // borrow the Helmholtz buffers
+  pw.r.p = w.r.p; pw.p.p = w.p.p; pw.t.p = w.t.p; pw.z.p = w.z.p;
+  auto mv = [&](const double *p, double *t) { launch_helm_matvec(j->g, w.op, p, t, j->d_state.p, j->red(), S(j)); };
+  icx_solve_info info{};
+  icx_status rc = pcg_plain(j, j->g.nn, w.b.p, w.x.p, w.invd.p, pw, tol, 10000, mv, info);
+  pw.r.p = pw.p.p = pw.t.p = pw.z.p = nullptr;  // not owned
+  if (rc && rc != ICX_ERR_NOCONV) return rc;
+  launch_helm_to_element(j->g, sens ? 1.0 : 1.0 / nq, w.x.p, dout, S(j));
+  if (iters) *iters = info.iterations;
+  return ICX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+icx_status icx_cone_filter_setup(icx_job *j, double radius) {
+  if (!j || !(radius > 0.0)) return ICX_ERR_ARG;
+  return cone_setup(j, radius);
+}
+
+icx_status icx_cone_filter(icx_job *j, int adjoint, const double *in, double *out) {
+  if (!j || !in || !out) return ICX_ERR_ARG;
+  if (!j->cw.ready) return fail(j->ctx, ICX_ERR_STATE, "icx_cone_filter_setup first");
+  icx_ctx *c = j->ctx;
+  const long long ne = j->g.ne;
+  DBuf<double> a, b;
+  HIPCHK(c, a.alloc(ne)); HIPCHK(c, b.alloc(ne));
+  HIPCHK(c, hipMemcpyAsync(a.p, in, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  if (adjoint) launch_cone_adjoint(j->g, stencil(j), j->cw.aa.p, a.p, b.p, S(j));
+  else launch_cone_forward(j->g, stencil(j), j->cw.aa.p, a.p, b.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(out, b.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_helmholtz_setup(icx_job *j, double radius) {
+  if (!j || !(radius > 0.0)) return ICX_ERR_ARG;
+  return helm_setup(j, radius);
+}
+
+icx_status icx_helmholtz_filter(icx_job *j, int sens, double tol, const double *in, double *out, int *iters) {
+  if (!j || !in || !out) return ICX_ERR_ARG;
+  if (!j->hw.ready) return fail(j->ctx, ICX_ERR_STATE, "icx_helmholtz_setup first");
+  icx_ctx *c = j->ctx;
+  const long long ne = j->g.ne;
+  DBuf<double> a, b;
+  HIPCHK(c, a.alloc(ne)); HIPCHK(c, b.alloc(ne));
+  HIPCHK(c, hipMemcpyAsync(a.p, in, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  icx_status rc = helm_apply_dev(j, sens, tol, a.p, b.p, iters);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, b.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_project(icx_job *j, double beta, double T, const double *in, double *out, double *dout) {
+  if (!j || !in) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  const long long ne = j->g.ne;
+  DBuf<double> a, b, d;
+  HIPCHK(c, a.alloc(ne)); HIPCHK(c, b.alloc(ne)); HIPCHK(c, d.alloc(ne));
+  HIPCHK(c, hipMemcpyAsync(a.p, in, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  launch_project(ne, beta, T, a.p, out ? b.p : nullptr, dout ? d.p : nullptr, S(j));
+  if (out) HIPCHK(c, hipMemcpyAsync(out, b.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  if (dout) HIPCHK(c, hipMemcpyAsync(dout, d.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// OC bisection with the volume evaluated on the device (OCSolver.hpp:20-68)
+icx_status oc_update_dev(icx_job *j, double *dx, const double *dfdx, double g0, const double *dgdx,
+                         double xmin, double xmax, double move, int filter, double beta, double T,
+                         double *dxnew, double *dtmp, int *bisections) {
+  const long long ne = j->g.ne;
+  double l0 = 0.0, l1 = 1.0e4, lambda = 0.0;
+  int counter = 0;
+  const int project = beta > 0.0;
+  while ((l1 - l0) / (l1 + l0) > 1.0e-5 && 100 > counter) {
+    lambda = 0.5 * (l1 + l0);
+    if (filter == ICX_FILTER_CONE) {
+      launch_oc_volume_cone(j->g, stencil(j), j->cw.aa.p, lambda, move, xmin, xmax, dx, dfdx, dgdx, project,
+                            beta, T, j->volume, j->d_scalars.p, j->red(), S(j));
+    } else {
+      launch_oc_candidate(ne, lambda, move, xmin, xmax, dx, dfdx, dgdx, dxnew, S(j));
+      const double *src = dxnew;
+      if (filter == ICX_FILTER_HELMHOLTZ) {
+        icx_status rc = helm_apply_dev(j, 0, j->opt.ready ? j->opt.par.helm_tol : 1e-10, dxnew, dtmp, nullptr);
+        if (rc) return rc;
+        src = dtmp;
+      }
+      launch_volume(ne, project, beta, T, src, nullptr, j->volume, j->d_scalars.p, j->red(), S(j));
+    }
+    double gk;
+    icx_status rc = read_scalars(j, 1, &gk);
+    if (rc) return rc;
+    if (gk - g0 > 0.0) l0 = lambda;
+    else l1 = lambda;
+    counter++;
+  }
+  launch_oc_candidate(ne, lambda, move, xmin, xmax, dx, dfdx, dgdx, dxnew, S(j));
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, hipMemcpyAsync(dx, dxnew, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  if (bisections) *bisections = counter;
+  return ICX_OK;
+}
+
+MMAVecs mma_vecs(icx_job *j) {
+  MMAWork &m = j->mma;
+  return MMAVecs{m.low.p, m.upp.p, m.alpha.p, m.beta.p, m.p0.p, m.q0.p, m.p1.p, m.q1.p, m.xold1.p, m.xold2.p};
+}
+
+icx_status mma_alloc(icx_job *j) {
+  MMAWork &m = j->mma;
+  if (m.ready) return ICX_OK;
+  icx_ctx *c = j->ctx;
+  const long long n = j->g.ne;
+  for (DBuf<double> *b : {&m.low, &m.upp, &m.alpha, &m.beta, &m.p0, &m.q0, &m.p1, &m.q1, &m.xold1, &m.xold2}) {
+    HIPCHK(c, b->alloc(n));
+    HIPCHK(c, hipMemsetAsync(b->p, 0, n * sizeof(double), S(j)));
+  }
+  m.iter = 0;
+  m.ready = true;
+  return ICX_OK;
+}
+
+// MMASolver::Update with m = 1 (MMASolver.cpp:66-154, 184-398): per-element work on the
+// device, the scalar dual Newton iteration on the host.
+icx_status mma_update_dev(icx_job *j, double *dx, const double *dfdx, double gx, const double *dgdx,
+                          double xmin, double xmax, int *steps) {
+  icx_status rc = mma_alloc(j);
+  if (rc) return rc;
+  MMAWork &m = j->mma;
+  const long long n = j->g.ne;
+  const double move = 0.15, albefa = 0.2, asyminit = 0.2, asymdec = 0.7, asyminc = 1.2, raa0 = 0.00001,
+               xmamieps = 1.0e-5;
+  const double a = 0.0, cc = 100000.0;
+  const double epsimin = std::sqrt((double)(n + 1)) * 1e-9;
+  m.iter++;
+  MMAVecs v = mma_vecs(j);
+  launch_mma_gensub(n, m.iter, v, dx, dfdx, dgdx, xmin, xmax, move, albefa, asyminit, asymdec, asyminc, raa0,
+                    xmamieps, j->d_scalars.p, j->red(), S(j));
+  double bsum;
+  rc = read_scalars(j, 1, &bsum);
+  if (rc) return rc;
+  const double b = -gx + bsum;
+  double lam = cc / 2.0, mu = 1.0;
+  double epsi = 1.0, err = 1.0;
+  int outloop = 0, total = 0;
+  double S1 = 0.0, S2 = 0.0, y = 0.0, z = 0.0;
+  auto xyz = [&](void) -> icx_status {
+    if (lam < 0.0) lam = 0;
+    y = std::max(0.0, lam - cc);
+    z = std::max(0.0, 10.0 * (lam * a - 1.0));
+    launch_mma_dual(n, lam, v, dx, 1, j->d_scalars.p, j->red(), S(j));
+    double o[2];
+    icx_status r = read_scalars(j, 2, o);
+    S1 = o[0];
+    S2 = o[1];
+    return r;
+  };
+  rc = xyz();
+  if (rc) return rc;
+  while (epsi > epsimin && outloop < 100) {
+    outloop++;
+    int loop = 0;
+    while (err > 0.9 * epsi && loop < 100) {
+      loop++;
+      total++;
+      // DualGrad / DualHess at the current lambda (x(lambda), S1, S2 are current)
+      double grad = -b - a * z - y + S1;
+      grad = -1.0 * grad - epsi / lam;
+      if (lam < 0.0) lam = 0.0;
+      double hess = S2;
+      if (lam > cc) hess += -1.0;
+      hess += -mu / lam;
+      double corr = 1e-4 * hess / 1;
+      if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+      hess += corr;
+      const double s0 = grad / hess;
+      const double s1 = -mu + epsi / lam - s0 * mu / lam;
+      double theta = 1.005;
+      if (theta < -1.01 * s0 / lam) theta = -1.01 * s0 / lam;
+      if (theta < -1.01 * s1 / mu) theta = -1.01 * s1 / mu;
+      theta = 1.0 / theta;
+      lam = lam + theta * s0;
+      mu = mu + theta * s1;
+      rc = xyz();
+      if (rc) return rc;
+      const double r0 = -b - a * z - y + mu + S1;
+      const double r1 = mu * lam - epsi;
+      err = std::max(std::fabs(r0), std::fabs(r1));
+    }
+    epsi = epsi * 0.1;
+  }
+  if (steps) *steps = total;
+  return ICX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+icx_status icx_oc_update(icx_job *j, double *x, const double *dfdx, double g, const double *dgdx, double xmin,
+                         double xmax, double move, int filter, double beta, double T, int *bisections) {
+  if (!j || !x || !dfdx || !dgdx) return ICX_ERR_ARG;
+  if (filter == ICX_FILTER_CONE && !j->cw.ready) return fail(j->ctx, ICX_ERR_STATE, "cone filter not set up");
+  if (filter == ICX_FILTER_HELMHOLTZ && !j->hw.ready) return fail(j->ctx, ICX_ERR_STATE, "Helmholtz filter not set up");
+  icx_ctx *c = j->ctx;
+  const long long ne = j->g.ne;
+  DBuf<double> dx, df, dg, xn, tmp;
+  HIPCHK(c, dx.alloc(ne)); HIPCHK(c, df.alloc(ne)); HIPCHK(c, dg.alloc(ne)); HIPCHK(c, xn.alloc(ne));
+  HIPCHK(c, tmp.alloc(ne));
+  HIPCHK(c, hipMemcpyAsync(dx.p, x, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(df.p, dfdx, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(dg.p, dgdx, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  icx_status rc = oc_update_dev(j, dx.p, df.p, g, dg.p, xmin, xmax, move, filter, beta, T, xn.p, tmp.p, bisections);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(x, dx.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_mma_reset(icx_job *j) {
+  if (!j) return ICX_ERR_ARG;
+  j->mma.iter = 0;
+  return ICX_OK;
+}
+
+icx_status icx_mma_update(icx_job *j, double *x, const double *dfdx, double gx, const double *dgdx, double xmin,
+                          double xmax, int *newton_steps) {
+  if (!j || !x || !dfdx || !dgdx) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  const long long ne = j->g.ne;
+  DBuf<double> dx, df, dg;
+  HIPCHK(c, dx.alloc(ne)); HIPCHK(c, df.alloc(ne)); HIPCHK(c, dg.alloc(ne));
+  HIPCHK(c, hipMemcpyAsync(dx.p, x, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(df.p, dfdx, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(dg.p, dgdx, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  icx_status rc = mma_update_dev(j, dx.p, df.p, gx, dg.p, xmin, xmax, newton_steps);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(x, dx.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+// ---- device-resident design loop ------------------------------------------------------------
+icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0) {
+  if (!j || !par || !x0) return ICX_ERR_ARG;
+  if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
+  icx_ctx *c = j->ctx;
+  OptWork &o = j->opt;
+  o.par = *par;
+  o.step = 0;
+  o.V0 = 0.0;
+  const long long ne = j->g.ne;
+  for (DBuf<double> *b : {&o.x, &o.fil, &o.thr, &o.dthr, &o.dfds, &o.dgds, &o.tmp, &o.tmp2}) HIPCHK(c, b->alloc(ne));
+  HIPCHK(c, hipMemcpyAsync(o.x.p, x0, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  icx_status rc = ICX_OK;
+  if (par->filter == ICX_FILTER_CONE) rc = cone_setup(j, par->filter_radius);
+  else if (par->filter == ICX_FILTER_HELMHOLTZ) rc = helm_setup(j, par->filter_radius);
+  if (rc) return rc;
+  if (par->driver == 1) {
+    rc = mma_alloc(j);
+    if (rc) return rc;
+    j->mma.iter = 0;
+  }
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  o.ready = true;
+  return ICX_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+icx_status icx_opt_step(icx_job *j, icx_iter_record *rec) {
+  if (!j) return ICX_ERR_ARG;
+  OptWork &o = j->opt;
+  if (!o.ready) return fail(j->ctx, ICX_ERR_STATE, "icx_opt_init first");
+  icx_ctx *c = j->ctx;
+  const auto t0 = std::chrono::steady_clock::now();
+  const icx_opt_params &P = o.par;
+  const long long ne = j->g.ne;
+  const int k = ++o.step;
+  icx_iter_record r{};
+  r.step = k;
+  icx_status rc = ICX_OK;
+  // filter
+  if (P.filter == ICX_FILTER_CONE) launch_cone_forward(j->g, stencil(j), j->cw.aa.p, o.x.p, o.fil.p, S(j));
+  else if (P.filter == ICX_FILTER_HELMHOLTZ) {
+    rc = helm_apply_dev(j, 0, P.helm_tol, o.x.p, o.fil.p, nullptr);
+    if (rc) return rc;
+  } else HIPCHK(c, hipMemcpyAsync(o.fil.p, o.x.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  // projection, beta = min(2^(k/40), 16)
+  double beta = std::min(std::pow(2, (int)(k / 40)), 16.0);
+  if (P.projection) launch_project(ne, beta, 0.5, o.fil.p, o.thr.p, o.dthr.p, S(j));
+  else HIPCHK(c, hipMemcpyAsync(o.thr.p, o.fil.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  // design -> E
+  HIPCHK(c, hipMemcpyAsync(j->d_design.p, o.thr.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  launch_simp(ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
+  j->design_set = true;
+  icx_solve_info info{};
+  rc = build_rhs(j);
+  if (rc) return rc;
+  if (P.form == ICX_SOLVE_PLAIN) rc = solve_plain(j, P.tol, P.maxit, info);
+  else if (P.form == ICX_SOLVE_FUSED) rc = solve_fused(j, P.tol, P.maxit, info);
+  else rc = solve_mixed(j, P.tol, P.maxit, info);
+  if (rc && rc != ICX_ERR_NOCONV) return rc;
+  j->u_valid = true;
+  r.cg_iterations = info.iterations;
+  // objective and volume
+  launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
+  launch_volume(ne, 0, 0.0, 0.5, o.thr.p, nullptr, j->volume, j->d_scalars.p + 1, j->red(), S(j));
+  double sc[2];
+  rc = read_scalars(j, 2, sc);
+  if (rc) return rc;
+  r.objective = sc[0];
+  r.volume = sc[1];
+  if (k == 1) {
+    o.V0 = sc[1];
+    r.constraint = P.driver == 0 ? o.V0 : 0.0;
+  } else {
+    r.constraint = P.driver == 0 ? sc[1] : sc[1] / o.V0 - 1.0;
+  }
+  // sensitivity and chain rule
+  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_u.p,
+                     j->volume, P.driver == 0 ? 1.0 : o.V0, o.dfds.p, o.dgds.p, S(j));
+  // dfds *= dthr, dgds *= dthr (projection derivative; 1 without projection)
+  if (P.projection) {
+    launch_mul(ne, o.dthr.p, o.dfds.p, S(j));
+    launch_mul(ne, o.dthr.p, o.dgds.p, S(j));
+  }
+  // filter adjoint
+  if (P.filter == ICX_FILTER_CONE) {
+    launch_cone_adjoint(j->g, stencil(j), j->cw.aa.p, o.dfds.p, o.tmp.p, S(j));
+    launch_cone_adjoint(j->g, stencil(j), j->cw.aa.p, o.dgds.p, o.tmp2.p, S(j));
+  } else if (P.filter == ICX_FILTER_HELMHOLTZ) {
+    rc = helm_apply_dev(j, 1, P.helm_tol, o.dfds.p, o.tmp.p, nullptr);
+    if (rc) return rc;
+    rc = helm_apply_dev(j, 1, P.helm_tol, o.dgds.p, o.tmp2.p, nullptr);
+    if (rc) return rc;
+  } else {
+    HIPCHK(c, hipMemcpyAsync(o.tmp.p, o.dfds.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+    HIPCHK(c, hipMemcpyAsync(o.tmp2.p, o.dgds.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  }
+  // update
+  int steps = 0;
+  if (P.driver == 0) {
+    rc = oc_update_dev(j, o.x.p, o.tmp.p, o.V0, o.tmp2.p, P.xmin, P.xmax, P.move, P.filter,
+                       P.projection ? beta : 0.0, 0.5, o.fil.p, o.dfds.p, &steps);
+  } else {
+    rc = mma_update_dev(j, o.x.p, o.tmp.p, r.constraint, o.tmp2.p, P.xmin, P.xmax, &steps);
+  }
+  if (rc) return rc;
+  r.update_steps = steps;
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (rec) *rec = r;
+  return ICX_OK;
+}
+
+icx_status icx_opt_get_design(const icx_job *jc, double *x) {
+  icx_job *j = const_cast<icx_job *>(jc);
+  if (!j || !x) return ICX_ERR_ARG;
+  if (!j->opt.ready) return ICX_ERR_STATE;
+  icx_ctx *c = j->ctx;
+  HIPCHK(c, hipMemcpyAsync(x, j->opt.x.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, double tol, int maxit,
+                           icx_solve_info *infos) {
+  if (!ctx || (!jobs && njobs)) return ICX_ERR_ARG;
+  icx_status worst = ICX_OK;
+  for (int i = 0; i < njobs; ++i) {
+    icx_status rc = icx_solve(jobs[i], form, tol, maxit, nullptr, infos ? &infos[i] : nullptr);
+    if (rc && rc != ICX_ERR_NOCONV) return rc;
+    if (rc) worst = rc;
+  }
+  return worst;
+}
+
+icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
+  if (!j || !bytes) return ICX_ERR_ARG;
+  size_t b = 0;
+  auto add = [&](size_t v) { b += v; };
+  add(j->d_fext.bytes()); add(j->d_b.bytes()); add(j->d_u.bytes()); add(j->d_design.bytes()); add(j->d_E.bytes());
+  add(j->d_vec.bytes()); add(j->d_vec2.bytes()); add(j->d_mask.bytes()); add(j->d_dof.bytes()); add(j->d_partial.bytes());
+  for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(d->bytes());
+  for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd}) add(d->bytes());
+  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.xlo, &j->mw.invd, &j->mw.E}) add(d->bytes());
+  add(j->mw.Kx.bytes());
+  *bytes = b;
+  return ICX_OK;
+}
+
+}  // extern "C"

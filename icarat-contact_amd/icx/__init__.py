@@ -1,0 +1,372 @@
+"""Python front end of libicx (ctypes over the C ABI in include/icx.h).
+
+The product path: every call goes to the HIP library built for gfx950.  If the
+library is missing, importing this package still works but `lib()` raises --
+there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "_build", "libicx.so"))
+
+QUAD4, HEXA8 = 4, 8
+PLANE_STRESS, PLANE_STRAIN, SOLID3D = 0, 1, 2
+SOLVE_PLAIN, SOLVE_FUSED, SOLVE_MIXED = 0, 1, 2
+FILTER_NONE, FILTER_CONE, FILTER_HELMHOLTZ = 0, 1, 2
+ERR_NOCONV = 5
+
+EXPORTS = [
+    "icx_ctx_create", "icx_ctx_destroy", "icx_last_error", "icx_ctx_synchronize",
+    "icx_job_create", "icx_job_destroy", "icx_job_sizes", "icx_get_mesh",
+    "icx_add_dirichlet", "icx_make_dof", "icx_get_dof_map", "icx_add_neumann",
+    "icx_set_fext", "icx_get_fext", "icx_add_nodal_load",
+    "icx_set_design", "icx_matvec", "icx_diagonal", "icx_solve", "icx_get_displacement",
+    "icx_get_node_values", "icx_compliance", "icx_sensitivity", "icx_element_volume",
+    "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
+    "icx_project", "icx_oc_update", "icx_mma_reset", "icx_mma_update",
+    "icx_opt_init", "icx_opt_step", "icx_opt_get_design",
+    "icx_solve_batch", "icx_job_footprint",
+    "icx_profile_begin", "icx_profile_end",
+]
+PROF_MATVEC64, PROF_FUSED_A, PROF_FUSED_B = 1, 2, 3
+
+
+class Problem(C.Structure):
+    _fields_ = [("eltype", C.c_int), ("div", C.c_int * 3), ("length", C.c_double * 3),
+                ("state", C.c_int), ("young1", C.c_double), ("young2", C.c_double),
+                ("poisson", C.c_double), ("penalty", C.c_double)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("matvecs", C.c_int), ("fp64_matvecs", C.c_int),
+                ("refinements", C.c_int), ("relres", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class OptParams(C.Structure):
+    _fields_ = [("driver", C.c_int), ("filter", C.c_int), ("filter_radius", C.c_double),
+                ("projection", C.c_int), ("form", C.c_int), ("tol", C.c_double),
+                ("maxit", C.c_int), ("helm_tol", C.c_double), ("xmin", C.c_double),
+                ("xmax", C.c_double), ("move", C.c_double)]
+
+
+class IterRecord(C.Structure):
+    _fields_ = [("step", C.c_int), ("objective", C.c_double), ("constraint", C.c_double),
+                ("volume", C.c_double), ("cg_iterations", C.c_int), ("update_steps", C.c_int),
+                ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libicx.so (raises if it has not been built: no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libicx.so not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.icx_last_error.restype = C.c_char_p
+        L.icx_last_error.argtypes = [C.c_void_p]
+        _lib = L
+    return _lib
+
+
+class IcxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("icx error %d: %s" % (code, msg))
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().icx_ctx_create(device, C.byref(self.h))
+        if rc != 0:
+            raise IcxError(rc, "icx_ctx_create failed (no GPU?)")
+
+    def check(self, rc, allow=()):
+        if rc != 0 and rc not in allow:
+            raise IcxError(rc, lib().icx_last_error(self.h).decode())
+        return rc
+
+    def synchronize(self):
+        self.check(lib().icx_ctx_synchronize(self.h))
+
+    def profile_begin(self, kernel):
+        self.check(lib().icx_profile_begin(self.h, int(kernel)))
+
+    def profile_end(self):
+        s, n = C.c_double(), C.c_longlong()
+        self.check(lib().icx_profile_end(self.h, C.byref(s), C.byref(n)))
+        return s.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().icx_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Job:
+    """One structured box problem resident on the device."""
+
+    def __init__(self, ctx, eltype, div, length, state, young=(1e-9, 1.0), nu=0.3, penalty=3.0):
+        self.ctx = ctx
+        p = Problem()
+        p.eltype = eltype
+        d = list(div) + [1] * (3 - len(div))
+        L = list(length) + [1.0] * (3 - len(length))
+        for i in range(3):
+            p.div[i] = int(d[i])
+            p.length[i] = float(L[i])
+        p.state, p.young1, p.young2 = state, float(young[0]), float(young[1])
+        p.poisson, p.penalty = float(nu), float(penalty)
+        self.h = C.c_void_p()
+        ctx.check(lib().icx_job_create(ctx.h, C.byref(p), C.byref(self.h)))
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        lib().icx_job_sizes(self.h, C.byref(a), C.byref(b), C.byref(c))
+        self.numnp, self.nelx, self.neq = a.value, b.value, c.value
+        self.ndim = 2 if eltype == QUAD4 else 3
+        self.dofnp = self.ndim
+        self.div = d
+        self.numeq = None
+
+    def close(self):
+        if self.h:
+            lib().icx_job_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- set-up --------------------------------------------------------------
+    def mesh(self):
+        x = np.zeros(self.numnp * 3)
+        conn = np.zeros(self.nelx * (4 if self.ndim == 2 else 8), dtype=np.int32)
+        self.ctx.check(lib().icx_get_mesh(self.h, _dp(x), _ip(conn)))
+        return x, conn
+
+    def add_dirichlet(self, flag6, val6, range6):
+        f = np.asarray(flag6, dtype=np.int32)
+        v, r = _f64(val6), _f64(range6)
+        self.ctx.check(lib().icx_add_dirichlet(self.h, _ip(f), _dp(v), _dp(r)))
+
+    def make_dof(self):
+        n = C.c_int()
+        self.ctx.check(lib().icx_make_dof(self.h, C.byref(n)))
+        self.numeq = n.value
+        return self.numeq
+
+    def dof_map(self):
+        d = np.zeros(self.numnp * self.dofnp, dtype=np.int32)
+        self.ctx.check(lib().icx_get_dof_map(self.h, _ip(d)))
+        return d
+
+    def add_neumann(self, val3, range6):
+        v, r = _f64(val3), _f64(range6)
+        self.ctx.check(lib().icx_add_neumann(self.h, _dp(v), _dp(r)))
+
+    def set_fext(self, f):
+        f = _f64(f)
+        self.ctx.check(lib().icx_set_fext(self.h, _dp(f)))
+
+    def fext(self):
+        f = np.zeros(self.numeq)
+        self.ctx.check(lib().icx_get_fext(self.h, _dp(f)))
+        return f
+
+    def add_nodal_load(self, node, comp, value):
+        self.ctx.check(lib().icx_add_nodal_load(self.h, int(node), int(comp), C.c_double(value)))
+
+    def node_index(self, i, j, k=0):
+        d0, d1 = self.div[0], self.div[1]
+        return i + (d0 + 1) * (j + (d1 + 1) * k)
+
+    # -- analysis ---------------------------------------------------------------
+    def set_design(self, x):
+        x = _f64(x)
+        self.ctx.check(lib().icx_set_design(self.h, _dp(x)))
+
+    def matvec(self, x, prec=0):
+        x = _f64(x)
+        y = np.zeros(self.numeq)
+        self.ctx.check(lib().icx_matvec(self.h, int(prec), _dp(x), _dp(y)))
+        return y
+
+    def diagonal(self):
+        d = np.zeros(self.numeq)
+        self.ctx.check(lib().icx_diagonal(self.h, _dp(d)))
+        return d
+
+    def solve(self, form=SOLVE_PLAIN, tol=1e-10, maxit=10000, want_u=True):
+        u = np.zeros(self.numeq) if want_u else None
+        info = SolveInfo()
+        rc = lib().icx_solve(self.h, int(form), C.c_double(tol), int(maxit),
+                             _dp(u) if want_u else None, C.byref(info))
+        self.ctx.check(rc, allow=(ERR_NOCONV,))
+        d = info.as_dict()
+        d["converged"] = rc == 0
+        return u, d
+
+    def displacement(self):
+        u = np.zeros(self.numeq)
+        self.ctx.check(lib().icx_get_displacement(self.h, _dp(u)))
+        return u
+
+    def node_values(self):
+        v = np.zeros(self.numnp * 3)
+        self.ctx.check(lib().icx_get_node_values(self.h, _dp(v)))
+        return v
+
+    def compliance(self):
+        c = C.c_double()
+        self.ctx.check(lib().icx_compliance(self.h, C.byref(c)))
+        return c.value
+
+    def sensitivity(self, vscale=1.0):
+        a, b = np.zeros(self.nelx), np.zeros(self.nelx)
+        self.ctx.check(lib().icx_sensitivity(self.h, C.c_double(vscale), _dp(a), _dp(b)))
+        return a, b
+
+    def volumes(self):
+        v = np.zeros(self.nelx)
+        self.ctx.check(lib().icx_element_volume(self.h, _dp(v)))
+        return v
+
+    # -- filters -----------------------------------------------------------------
+    def cone_setup(self, radius):
+        self.ctx.check(lib().icx_cone_filter_setup(self.h, C.c_double(radius)))
+
+    def cone(self, v, adjoint=False):
+        v = _f64(v)
+        out = np.zeros(self.nelx)
+        self.ctx.check(lib().icx_cone_filter(self.h, int(adjoint), _dp(v), _dp(out)))
+        return out
+
+    def helmholtz_setup(self, radius):
+        self.ctx.check(lib().icx_helmholtz_setup(self.h, C.c_double(radius)))
+
+    def helmholtz(self, v, sens=False, tol=1e-10):
+        v = _f64(v)
+        out = np.zeros(self.nelx)
+        it = C.c_int()
+        self.ctx.check(lib().icx_helmholtz_filter(self.h, int(sens), C.c_double(tol), _dp(v),
+                                                  _dp(out), C.byref(it)))
+        return out, it.value
+
+    def project(self, v, beta, T=0.5):
+        v = _f64(v)
+        out, dout = np.zeros(self.nelx), np.zeros(self.nelx)
+        self.ctx.check(lib().icx_project(self.h, C.c_double(beta), C.c_double(T), _dp(v),
+                                         _dp(out), _dp(dout)))
+        return out, dout
+
+    def oc_update(self, x, dfdx, g, dgdx, xmin=1e-4, xmax=1.0, move=0.15,
+                  filter=FILTER_CONE, beta=1.0, T=0.5):
+        x = _f64(x).copy()
+        df, dg = _f64(dfdx), _f64(dgdx)
+        n = C.c_int()
+        self.ctx.check(lib().icx_oc_update(self.h, _dp(x), _dp(df), C.c_double(g), _dp(dg),
+                                           C.c_double(xmin), C.c_double(xmax), C.c_double(move),
+                                           int(filter), C.c_double(beta), C.c_double(T),
+                                           C.byref(n)))
+        return x, n.value
+
+    def mma_reset(self):
+        self.ctx.check(lib().icx_mma_reset(self.h))
+
+    def mma_update(self, x, dfdx, gx, dgdx, xmin=1e-4, xmax=1.0):
+        x = _f64(x).copy()
+        df, dg = _f64(dfdx), _f64(dgdx)
+        n = C.c_int()
+        self.ctx.check(lib().icx_mma_update(self.h, _dp(x), _dp(df), C.c_double(gx), _dp(dg),
+                                            C.c_double(xmin), C.c_double(xmax), C.byref(n)))
+        return x, n.value
+
+    # -- device-resident design loop ------------------------------------------------
+    def opt_init(self, x0, driver=0, filter=FILTER_CONE, radius=1.5, projection=True,
+                 form=SOLVE_PLAIN, tol=1e-10, maxit=10000, helm_tol=1e-10, xmin=1e-4,
+                 xmax=1.0, move=0.15):
+        p = OptParams(driver, filter, radius, int(projection), form, tol, maxit, helm_tol,
+                      xmin, xmax, move)
+        x0 = _f64(x0)
+        self.ctx.check(lib().icx_opt_init(self.h, C.byref(p), _dp(x0)))
+
+    def opt_step(self):
+        r = IterRecord()
+        self.ctx.check(lib().icx_opt_step(self.h, C.byref(r)))
+        return r.as_dict()
+
+    def opt_design(self):
+        x = np.zeros(self.nelx)
+        self.ctx.check(lib().icx_opt_get_design(self.h, _dp(x)))
+        return x
+
+    def footprint(self):
+        b = C.c_size_t()
+        self.ctx.check(lib().icx_job_footprint(self.h, C.byref(b)))
+        return b.value
+
+
+# -- the benchmark problems (same conventions as the oracle's) -------------------------
+
+def random_density(n, seed, lo=0.01, hi=1.0):
+    """Element densities i.i.d. U[lo, hi) in element-id order (numpy PCG64 stream)."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    return lo + (hi - lo) * rng.random(n)
+
+
+def cantilever_2d(ctx, nx=120, ny=40, state=PLANE_STRESS, young=(1e-9, 1.0), nu=0.3, p=3.0):
+    """Q4 nx x ny, h = 1, left edge clamped, unit downward nodal load at the right
+    mid-edge node."""
+    j = Job(ctx, QUAD4, (nx, ny, 1), (float(nx), float(ny), 1.0), state, young, nu, p)
+    j.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0.0, 0.0, 0.0, float(ny), 0.0, 0.0])
+    j.make_dof()
+    j.add_nodal_load(j.node_index(nx, ny // 2), 1, -1.0)
+    return j
+
+
+def cantilever_3d(ctx, nx, ny, nz, young=(1e-9, 1.0), nu=0.3, p=3.0):
+    """hex8 nx x ny x nz, h = 1, x = 0 face clamped, total F_z = -1 on the edge
+    {x = L, z = 0}, end nodes carrying half weight."""
+    j = Job(ctx, HEXA8, (nx, ny, nz), (float(nx), float(ny), float(nz)), SOLID3D, young, nu, p)
+    j.add_dirichlet([1, 1, 1, 0, 0, 0], [0] * 6, [0.0, 0.0, 0.0, float(ny), 0.0, float(nz)])
+    j.make_dof()
+    f = np.zeros(j.numeq)
+    dof = j.dof_map()
+    for jj in range(ny + 1):
+        w = 0.5 if jj in (0, ny) else 1.0
+        d = dof[3 * j.node_index(nx, jj, 0) + 2]
+        if d < j.numeq:
+            f[d] += -w / ny
+    j.set_fext(f)
+    return j

@@ -1,0 +1,214 @@
+/*
+ * icx.h -- C ABI of the MI355X (gfx950) implementation of ICARAT's
+ * SIMP linear-elastic topology-optimisation hot path.
+ *
+ * Conventions
+ *  - Every entry point returns an icx_status (0 = ok).  Nothing calls exit().
+ *    icx_last_error() returns a message for the last failure of a context.
+ *  - Host buffers are caller-owned and copied; device state is owned by the
+ *    job.  Vectors on the DOF axis use the reference's equation numbering
+ *    (free DOFs 0..numeq-1 in node order, src/io/boundary_condition_base.hpp:35-88);
+ *    element vectors use the reference's element ids (x fastest,
+ *    src/io/mesh_icarat.hpp:325-355, 440-494).
+ *  - One context = one GPU + one HIP stream; use a context from one host thread
+ *    at a time.  Jobs of the same context run in submission order.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef ICX_H
+#define ICX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int icx_status;
+enum {
+  ICX_OK = 0,
+  ICX_ERR_ARG = 1,      /* invalid argument */
+  ICX_ERR_STATE = 2,    /* call out of order (e.g. solve before icx_make_dof) */
+  ICX_ERR_HIP = 3,      /* HIP runtime error */
+  ICX_ERR_NOMEM = 4,    /* device allocation failed / over the memory budget */
+  ICX_ERR_NOCONV = 5,   /* iterative solve hit maxit (results are still written) */
+  ICX_ERR_NUMERIC = 6   /* non-finite value / zero pivot */
+};
+
+/* element types (src/io/mesh_base.hpp:50-115) */
+enum { ICX_QUAD4 = 4, ICX_HEXA8 = 8 };
+/* material state (src/problem/elasticity.cpp:19-32) */
+enum { ICX_PLANE_STRESS = 0, ICX_PLANE_STRAIN = 1, ICX_SOLID3D = 2 };
+/* solver forms */
+enum {
+  ICX_SOLVE_PLAIN = 0, /* FP64 Jacobi-PCG, Eigen ConjugateGradient loop (parity form) */
+  ICX_SOLVE_FUSED = 1, /* FP64 Jacobi-PCG, fused kernels (same iterates up to rounding) */
+  ICX_SOLVE_MIXED = 2  /* FP32 inner PCG + FP64 residual replacement (reliable updates) */
+};
+/* filter kinds */
+enum { ICX_FILTER_NONE = 0, ICX_FILTER_CONE = 1, ICX_FILTER_HELMHOLTZ = 2 };
+
+typedef struct icx_ctx icx_ctx;
+typedef struct icx_job icx_job;
+
+/* Structured box problem (src/io/mesh_icarat.hpp:69-128 [mesh] keys and
+ * src/problem/elasticity.cpp:19-32 [material] keys). */
+typedef struct {
+  int eltype;        /* ICX_QUAD4 or ICX_HEXA8 */
+  int div[3];        /* [mesh] division (div[2] ignored for quad4) */
+  double length[3];  /* [mesh] length */
+  int state;         /* ICX_PLANE_STRESS / ICX_PLANE_STRAIN / ICX_SOLID3D */
+  double young1;     /* [material] young[0] (E1; Emin when young2 > young1) */
+  double young2;     /* [material] young[1] (E2; E0) */
+  double poisson;    /* [material] poisson */
+  double penalty;    /* [material] penalty (SIMP p) */
+} icx_problem;
+
+typedef struct {
+  int iterations;        /* Eigen semantics: passes that did not converge */
+  int matvecs;           /* operator applications (inner + FP64 residual) */
+  int fp64_matvecs;      /* of which FP64 */
+  int refinements;       /* FP64 residual replacements (mixed form) */
+  double relres;         /* |b - K u| / |b| (recursive for plain, true for mixed) */
+  double seconds;        /* device time of the solve (HIP events) */
+} icx_solve_info;
+
+/* ---- context ---------------------------------------------------------------- */
+icx_status icx_ctx_create(int device, icx_ctx **out);
+icx_status icx_ctx_destroy(icx_ctx *ctx);
+const char *icx_last_error(const icx_ctx *ctx);
+icx_status icx_ctx_synchronize(icx_ctx *ctx);
+
+/* ---- job set-up --------------------------------------------------------------
+ * Replaces MeshIcarat::setParameter/generate (src/io/mesh_icarat.hpp:69-136),
+ * LinearElastic::setParameter (src/problem/elasticity.cpp:19-32). */
+icx_status icx_job_create(icx_ctx *ctx, const icx_problem *prob, icx_job **out);
+icx_status icx_job_destroy(icx_job *job);
+/* numnp, nelx, neq (= numnp * dofnp) */
+icx_status icx_job_sizes(const icx_job *job, int *numnp, int *nelx, int *neq);
+/* node coordinates (numnp x 3) and connectivity (nelx x ne), as MeshIcarat builds them */
+icx_status icx_get_mesh(const icx_job *job, double *x, int *conn);
+
+/* BCIcarat::input_dirich for one [boundary_condition].dirichlet entry
+ * (src/io/boundary_condition_icarat.hpp:68-121): flag[6], val[6],
+ * range[6] = {xmin,xmax,ymin,ymax,zmin,zmax}; tolerance 1e-7. */
+icx_status icx_add_dirichlet(icx_job *job, const int *flag6, const double *val6,
+                             const double *range6);
+/* BCIcarat::makeDOF -> countDOF (src/io/boundary_condition_icarat.hpp:191-200). */
+icx_status icx_make_dof(icx_job *job, int *numeq);
+/* node.dof[] as numnp x dofnp (free 0..numeq-1, constrained numeq..neq-1) */
+icx_status icx_get_dof_map(const icx_job *job, int *dof);
+/* BCIcarat::input_neumann for one [boundary_condition].neumann entry
+ * (src/io/boundary_condition_icarat.hpp:253-396), accumulated into fext_org. */
+icx_status icx_add_neumann(icx_job *job, const double *val3, const double *range6);
+/* Force::fext_org (src/problem/base.hpp:108-123) read/write, numeq doubles. */
+icx_status icx_set_fext(icx_job *job, const double *fext);
+icx_status icx_get_fext(const icx_job *job, double *fext);
+/* adds `value` to fext_org of (node, comp) if that DOF is free */
+icx_status icx_add_nodal_load(icx_job *job, int node, int comp, double value);
+
+/* ---- analysis ------------------------------------------------------------------
+ * LinearElastic::setDesignS for every element (src/problem/elasticity.hpp:60):
+ * the SIMP modulus of element e is computed from design[e] on the device. */
+icx_status icx_set_design(icx_job *job, const double *design);
+/* y = K x on the free DOFs (matrix-free; x, y numeq doubles).
+ * prec: 0 = FP64, 1 = FP32 operator (inputs rounded to FP32). */
+icx_status icx_matvec(icx_job *job, int prec, const double *x, double *y);
+/* Jacobi diagonal of K (numeq doubles) */
+icx_status icx_diagonal(icx_job *job, double *diag);
+/* LinearAnalysis::solve (src/analysis/linear.hpp:54-104) with the solver
+ * facade Solvers::solve "Eigen_CG" (src/analysis/solver/solvers.cpp:64-92):
+ * rhs = fext_org - K u_dirichlet; x0 = 0; stop on |r| < tol |b|.
+ * u (numeq doubles) may be NULL: the solution stays on the device. */
+icx_status icx_solve(icx_job *job, int form, double tol, int maxit, double *u,
+                     icx_solve_info *info);
+/* Value::val of the last solve (numeq doubles) */
+icx_status icx_get_displacement(const icx_job *job, double *u);
+/* node.val after divideValIntoNode (numnp x 3, |v| < 1e-15 flushed,
+ * src/analysis/analysis_utility.hpp:8-31) */
+icx_status icx_get_node_values(const icx_job *job, double *nodeval);
+/* compliance u . fext (opt-macro-oc/opt_macro_oc.cpp:110) */
+icx_status icx_compliance(icx_job *job, double *c);
+/* the drivers' sensitivity() (opt-macro-oc/opt_macro_oc.cpp:182-212,
+ * opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:173-205):
+ * dfds = -(dE/E) u_e^T Ke u_e, dgds = v_e / vscale (vscale = 1 or V0).
+ * Either output may be NULL. */
+icx_status icx_sensitivity(icx_job *job, double vscale, double *dfds, double *dgds);
+/* element volumes (Element::volume, src/problem/elasticity.cpp:249) */
+icx_status icx_element_volume(const icx_job *job, double *vol);
+
+/* ---- filters and projection -------------------------------------------------
+ * Filter(fem, element, node, r) + densityFilter(v, isSens)
+ * (src/optimization/filter.hpp:17-151, 191-234); adjoint = isSens. */
+icx_status icx_cone_filter_setup(icx_job *job, double radius);
+icx_status icx_cone_filter(icx_job *job, int adjoint, const double *in, double *out);
+/* FilterHelmholtz(fem, element, node, R) + densityFilter(v, "Eigen_CG", log, sensFlag)
+ * (src/optimization/filter_helm.hpp:23-137, 178-256); solves to tol (1e-10 in
+ * the reference).  iters may be NULL. */
+icx_status icx_helmholtz_setup(icx_job *job, double radius);
+icx_status icx_helmholtz_filter(icx_job *job, int sens, double tol, const double *in,
+                                double *out, int *iters);
+/* threshold / thresholdDerivative (src/optimization/filter.hpp:153-185);
+ * out or dout may be NULL. */
+icx_status icx_project(icx_job *job, double beta, double T, const double *in, double *out,
+                       double *dout);
+
+/* ---- design updates -----------------------------------------------------------
+ * OC::UpdateVariables (src/optimization/opt-solver/OCSolver.hpp:20-68) with the
+ * volume lambda of opt-macro-oc/opt_macro_oc.cpp:155-163 evaluated on the
+ * device: g(x) = sum_e threshold(filter(x))_e v_e (filter = the job's
+ * filter of kind `filter`, projection skipped when beta <= 0).
+ * x is updated in place (nelx doubles); xmin/xmax scalars. */
+icx_status icx_oc_update(icx_job *job, double *x, const double *dfdx, double g,
+                         const double *dgdx, double xmin, double xmax, double move,
+                         int filter, double beta, double T, int *bisections);
+/* MMASolver(n, m=1) + Update (src/optimization/opt-solver/MMASolver.hpp:62-116,
+ * MMASolver.cpp:41-553); the optimiser history lives on the device. */
+icx_status icx_mma_reset(icx_job *job);
+icx_status icx_mma_update(icx_job *job, double *x, const double *dfdx, double gx,
+                          const double *dgdx, double xmin, double xmax, int *newton_steps);
+
+/* ---- device-resident design loops (drivers A and B) ---------------------------
+ * One design iteration of opt-macro-oc/opt_macro_oc.cpp:84-172 (driver 0: cone
+ * filter + OC) or opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163
+ * (driver 1: Helmholtz + MMA), with every vector kept on the device. */
+typedef struct {
+  int driver;          /* 0 = OC driver (A), 1 = MMA + Helmholtz driver (B) */
+  int filter;          /* ICX_FILTER_CONE / ICX_FILTER_HELMHOLTZ */
+  double filter_radius;
+  int projection;      /* 1: beta = min(2^(k/40), 16) with tanh projection */
+  int form;            /* ICX_SOLVE_* for the structural solve */
+  double tol;          /* structural solve tolerance */
+  int maxit;
+  double helm_tol;     /* Helmholtz solve tolerance */
+  double xmin, xmax, move;
+} icx_opt_params;
+
+typedef struct {
+  int step;
+  double objective;    /* compliance */
+  double constraint;   /* const_h[0] as the driver stores it */
+  double volume;       /* sum rho_hat v */
+  int cg_iterations;
+  int update_steps;    /* OC bisections or MMA Newton steps */
+  double seconds;      /* wall time of the design iteration */
+} icx_iter_record;
+
+icx_status icx_opt_init(icx_job *job, const icx_opt_params *par, const double *x0);
+icx_status icx_opt_step(icx_job *job, icx_iter_record *rec);
+icx_status icx_opt_get_design(const icx_job *job, double *x);
+
+/* ---- batched / serving -------------------------------------------------------------
+ * No reference counterpart: the reference runs one process per job.  A batch
+ * runs the PCG iterations of many resident jobs together (one launch per
+ * kernel for all jobs) until each job meets its tolerance. */
+icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, double tol,
+                           int maxit, icx_solve_info *infos);
+/* device bytes a job holds (admission control) */
+icx_status icx_job_footprint(const icx_job *job, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,29 @@
+/*
+ * icx_profile.h -- measurement hooks of libicx (no reference counterpart).
+ *
+ * While a profile window is open, every launch of the selected kernel on the
+ * context's stream is bracketed by a pair of HIP events; icx_profile_end returns
+ * the summed kernel time and the launch count.  Used by bench.py to measure the
+ * dominant kernel's average launch duration inside the timed region.
+ */
+#ifndef ICX_PROFILE_H
+#define ICX_PROFILE_H
+#include "icx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ICX_PROF_MATVEC64 = 1,  /* FP64 operator kernel of the plain form */
+  ICX_PROF_FUSED_A = 2,   /* fused direction-update + operator kernel (fused/mixed forms) */
+  ICX_PROF_FUSED_B = 3    /* fused residual-update kernel (fused/mixed forms) */
+};
+
+icx_status icx_profile_begin(icx_ctx *ctx, int kernel);
+icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

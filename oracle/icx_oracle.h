@@ -1,0 +1,127 @@
+/*
+ * icx_oracle.h -- CPU restatement of ICARAT's SIMP linear-elastic topology
+ * optimisation hot path (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * path and the timed CPU baseline.  Only tests/, the smoke check and the CPU
+ * baseline leg of bench.py may load it.  The product library (libicx) never
+ * links or calls it.
+ *
+ * Every function names the reference file:line it restates.  Paths are
+ * relative to the reference repository root.
+ */
+#ifndef ICX_ORACLE_H
+#define ICX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* material state, elasticity.cpp:19-32 ("plane_stress", "plane_strain", "3D") */
+enum { ICXO_PLANE_STRESS = 0, ICXO_PLANE_STRAIN = 1, ICXO_SOLID3D = 2 };
+
+/* ---- mesh / boundary conditions (integer, bit-exact) ---------------------- */
+int icxo_mesh_box(int ne, const int *div, const double *len, double *x, int *conn);
+void icxo_dirichlet(int ndim, int dofnp, int numnp, const double *x,
+                    const int *flag6, const double *val6, const double *range6,
+                    int *has, int *node_flag, double *node_val);
+int icxo_count_dof(int numnp, int dofnp, const int *has, const int *node_flag, int *dof);
+void icxo_neumann(int ndim, int dofnp, int ne, int nelx, const double *x,
+                  const int *conn, const int *dof, int numeq,
+                  const double *val3, const double *range6, double *fext);
+
+/* ---- element level --------------------------------------------------------- */
+double icxo_simp(double young1, double young2, double design, double p);
+void icxo_element_stiffness(int ne, int ndim, int state, double E, double nu,
+                            const double *X, double *Ke, double *volume);
+void icxo_helm_element(int ne, int ndim, double radius, const double *X,
+                       double *Ke, double *Nint);
+
+/* ---- global structural system -------------------------------------------- */
+typedef struct icxo_sys icxo_sys;
+icxo_sys *icxo_sys_create(int ne, int ndim, int dofnp, int state, int numnp, int nelx,
+                          const double *x, const int *conn, const int *dof, int numeq,
+                          double young1, double young2, double nu, double p);
+void icxo_sys_destroy(icxo_sys *s);
+long long icxo_sys_nnz(const icxo_sys *s);
+void icxo_sys_assemble(icxo_sys *s, const double *design, const int *has,
+                       const int *node_flag, const double *node_val, double *fint);
+void icxo_sys_csr(const icxo_sys *s, int *rowptr, int *col, double *val);
+void icxo_sys_spmv(const icxo_sys *s, const double *x, double *y);
+int icxo_sys_cg(const icxo_sys *s, const double *b, double *x, double tol, int maxit,
+                double *err);
+int icxo_sys_direct(const icxo_sys *s, const double *b, double *x);
+void icxo_sys_element_energy(const icxo_sys *s, const double *nodeval, double *ue_K_ue);
+
+/* ---- Eigen CG restatement on a CSR matrix -------------------------------- */
+int icxo_csr_cg(int n, const int *rowptr, const int *col, const double *val,
+                const double *b, double *x, double tol, int maxit, double *err);
+
+/* ---- solve-level helpers ---------------------------------------------------- */
+void icxo_divide_val_into_node(int numnp, int dofnp, int numeq, const int *dof,
+                               const double *u, const double *node_val_dirich,
+                               double *nodeval);
+void icxo_sensitivity(int nelx, int driver_b, double young1, double young2, double p,
+                      const double *design, const double *ue_K0_ue_over_E,
+                      const double *volume, double V0, double *dfds, double *dgds);
+
+/* ---- cone filter, projection ------------------------------------------------ */
+typedef struct icxo_filter icxo_filter;
+icxo_filter *icxo_filter_create(int ne, int ndim, int nelx, const double *x,
+                                const int *conn, double radius, int brute_force);
+void icxo_filter_destroy(icxo_filter *f);
+long long icxo_filter_nnz(const icxo_filter *f);
+void icxo_filter_neighbours(const icxo_filter *f, int *ptr, int *id, double *dist);
+void icxo_filter_apply(icxo_filter *f, const double *in, double *out, int is_sens);
+void icxo_threshold(int n, const double *in, double beta, double T, double *out);
+void icxo_threshold_derivative(int n, const double *in, double beta, double T, double *out);
+
+/* ---- Helmholtz PDE filter --------------------------------------------------- */
+typedef struct icxo_helm icxo_helm;
+icxo_helm *icxo_helm_create(int ne, int ndim, int numnp, int nelx, const double *x,
+                            const int *conn, double radius);
+void icxo_helm_destroy(icxo_helm *h);
+int icxo_helm_apply(icxo_helm *h, const double *in, double *out, int sens,
+                    double tol, int maxit, int direct);
+
+/* ---- design updaters ------------------------------------------------------- */
+typedef double (*icxo_volume_fn)(const double *x, void *user);
+int icxo_oc_update(int n, double *x, const double *dfdx, double g, const double *dgdx,
+                   const double *xmin, const double *xmax, double movelimit,
+                   icxo_volume_fn gk, void *user);
+
+typedef struct icxo_mma icxo_mma;
+icxo_mma *icxo_mma_create(int n, int m, double a, double c, double d);
+void icxo_mma_destroy(icxo_mma *m);
+int icxo_mma_update(icxo_mma *m, double *x, const double *dfdx, const double *gx,
+                    const double *dgdx, const double *xmin, const double *xmax);
+
+/* ---- drivers ----------------------------------------------------------------
+ * A: opt-macro-oc/opt_macro_oc.cpp:84-172  (cone filter, projection, CG, OC)
+ * B: opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163 (Helmholtz, MMA) */
+typedef struct {
+  int driver;          /* 0 = A (OC + cone), 1 = B (MMA + Helmholtz) */
+  int ne, ndim, dofnp, state;
+  int numnp, nelx, numeq;
+  const double *x; const int *conn; const int *dof;
+  const int *has; const int *node_flag; const double *node_val;
+  const double *fext;
+  double young1, young2, nu, p;
+  double filter_radius;
+  double xmin, xmax, move;
+  int iterations;
+  int projection;      /* 1: beta schedule + tanh projection (reference), 0: off */
+  int direct;          /* 1: direct solve for the structure, 0: Eigen CG restatement */
+  double cg_tol; int cg_maxit;
+  double helm_tol;     /* Helmholtz CG tolerance (reference 1e-10) */
+  int helm_direct;
+} icxo_driver_cfg;
+
+int icxo_run_driver(const icxo_driver_cfg *cfg, double *design /* in: x0, out: final */,
+                    double *obj_hist, double *vol_hist, double *last_dfds,
+                    double *last_u /* numeq */, int *cg_iters_hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

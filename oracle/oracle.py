@@ -1,0 +1,396 @@
+"""ctypes front end of the CPU oracle (oracle/icx_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU-baseline
+leg of bench.py as the parity checker.  The product library never imports it.
+
+`Problem` mirrors the reference's problem set-up sequence
+(opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:27-65): MeshIcarat box mesh,
+BCIcarat Dirichlet ranges, countDOF numbering, Neumann tractions into
+Force::fext_org, plus direct nodal loads written into fext_org.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libicx_oracle.so")
+
+PLANE_STRESS, PLANE_STRAIN, SOLID3D = 0, 1, 2
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        sig = {
+            "icxo_mesh_box": (i, [i, _ip, _dp, _dp, _ip]),
+            "icxo_dirichlet": (None, [i, i, i, _dp, _ip, _dp, _dp, _ip, _ip, _dp]),
+            "icxo_count_dof": (i, [i, i, _ip, _ip, _ip]),
+            "icxo_neumann": (None, [i, i, i, i, _dp, _ip, _ip, i, _dp, _dp, _dp]),
+            "icxo_simp": (d, [d, d, d, d]),
+            "icxo_element_stiffness": (None, [i, i, i, d, d, _dp, _dp, _dp]),
+            "icxo_helm_element": (None, [i, i, d, _dp, _dp, _dp]),
+            "icxo_sys_create": (vp, [i, i, i, i, i, i, _dp, _ip, _ip, i, d, d, d, d]),
+            "icxo_sys_destroy": (None, [vp]),
+            "icxo_sys_nnz": (C.c_longlong, [vp]),
+            "icxo_sys_assemble": (None, [vp, _dp, vp, vp, vp, vp]),
+            "icxo_sys_csr": (None, [vp, _ip, _ip, _dp]),
+            "icxo_sys_spmv": (None, [vp, _dp, _dp]),
+            "icxo_sys_cg": (i, [vp, _dp, _dp, d, i, C.POINTER(d)]),
+            "icxo_sys_direct": (i, [vp, _dp, _dp]),
+            "icxo_sys_element_energy": (None, [vp, _dp, _dp]),
+            "icxo_csr_cg": (i, [i, _ip, _ip, _dp, _dp, _dp, d, i, C.POINTER(d)]),
+            "icxo_divide_val_into_node": (None, [i, i, i, _ip, _dp, vp, _dp]),
+            "icxo_sensitivity": (None, [i, i, d, d, d, _dp, _dp, _dp, d, _dp, _dp]),
+            "icxo_filter_create": (vp, [i, i, i, _dp, _ip, d, i]),
+            "icxo_filter_destroy": (None, [vp]),
+            "icxo_filter_nnz": (C.c_longlong, [vp]),
+            "icxo_filter_neighbours": (None, [vp, _ip, _ip, _dp]),
+            "icxo_filter_apply": (None, [vp, _dp, _dp, i]),
+            "icxo_threshold": (None, [i, _dp, d, d, _dp]),
+            "icxo_threshold_derivative": (None, [i, _dp, d, d, _dp]),
+            "icxo_helm_create": (vp, [i, i, i, i, _dp, _ip, d]),
+            "icxo_helm_destroy": (None, [vp]),
+            "icxo_helm_apply": (i, [vp, _dp, _dp, i, d, i, i]),
+            "icxo_mma_create": (vp, [i, i, d, d, d]),
+            "icxo_mma_destroy": (None, [vp]),
+            "icxo_mma_update": (i, [vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L.icxo_oc_update.restype = i
+        L.icxo_run_driver.restype = i
+        _lib = L
+    return _lib
+
+
+VOLFN = C.CFUNCTYPE(C.c_double, C.POINTER(C.c_double), C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Problem:
+    """Structured box problem following the reference's set-up order."""
+
+    def __init__(self, ne, div, length, state, young=(1e-9, 1.0), nu=0.3, penalty=3.0):
+        self.ne = int(ne)
+        self.ndim = 2 if ne == 4 else 3
+        self.dofnp = self.ndim
+        self.div = np.array(list(div) + [1] * (3 - len(div)), dtype=np.int32)[:3]
+        if self.ndim == 2:
+            self.div[2] = 1
+        self.length = _f64(list(length) + [1.0] * (3 - len(length)))[:3].copy()
+        self.state = state
+        self.young1, self.young2 = float(young[0]), float(young[1])
+        self.nu, self.p = float(nu), float(penalty)
+        d0, d1, d2 = (int(v) for v in self.div)
+        if self.ndim == 2:
+            self.numnp, self.nelx = (d0 + 1) * (d1 + 1), d0 * d1
+        else:
+            self.numnp, self.nelx = (d0 + 1) * (d1 + 1) * (d2 + 1), d0 * d1 * d2
+        self.x = np.zeros(self.numnp * 3)
+        self.conn = np.zeros(self.nelx * self.ne, dtype=np.int32)
+        lib().icxo_mesh_box(self.ne, self.div, self.length, self.x, self.conn)
+        self.has = np.zeros(self.numnp, dtype=np.int32)
+        self.node_flag = np.zeros(self.numnp * 6, dtype=np.int32)
+        self.node_val = np.zeros(self.numnp * 6)
+        self.dof = None
+        self.numeq = None
+        self.fext = None
+        self._sys = None
+
+    # -- boundary conditions ------------------------------------------------
+    def add_dirichlet(self, flag6, val6, range6):
+        assert self.dof is None, "Dirichlet ranges must precede make_dof"
+        lib().icxo_dirichlet(self.ndim, self.dofnp, self.numnp, self.x,
+                             np.asarray(flag6, dtype=np.int32), _f64(val6), _f64(range6),
+                             self.has, self.node_flag, self.node_val)
+
+    def make_dof(self):
+        self.dof = np.zeros(self.numnp * self.dofnp, dtype=np.int32)
+        self.numeq = lib().icxo_count_dof(self.numnp, self.dofnp, self.has, self.node_flag,
+                                          self.dof)
+        self.fext = np.zeros(self.numeq)
+        return self.numeq
+
+    def add_neumann(self, val3, range6):
+        lib().icxo_neumann(self.ndim, self.dofnp, self.ne, self.nelx, self.x, self.conn,
+                           self.dof, self.numeq, _f64(val3), _f64(range6), self.fext)
+
+    def add_nodal_load(self, node, comp, value):
+        d = self.dof[self.dofnp * node + comp]
+        if d < self.numeq:
+            self.fext[d] += value
+
+    def node_index(self, i, j, k=0):
+        d0, d1 = int(self.div[0]), int(self.div[1])
+        return i + (d0 + 1) * (j + (d1 + 1) * k)
+
+    # -- structural system ---------------------------------------------------
+    def system(self):
+        if self._sys is None:
+            self._sys = lib().icxo_sys_create(self.ne, self.ndim, self.dofnp, self.state,
+                                              self.numnp, self.nelx, self.x, self.conn,
+                                              self.dof, self.numeq, self.young1, self.young2,
+                                              self.nu, self.p)
+        return self._sys
+
+    def __del__(self):
+        if getattr(self, "_sys", None) and _lib is not None:
+            _lib.icxo_sys_destroy(self._sys)
+            self._sys = None
+
+    def _bc_ptrs(self):
+        if self.has.any():
+            return (self.has.ctypes.data, self.node_flag.ctypes.data, self.node_val.ctypes.data)
+        return (None, None, None)
+
+    def assemble(self, design):
+        """Assemble K(design) on the free DOFs; returns rhs = fext - fint."""
+        s = self.system()
+        fint = np.zeros(self.numeq)
+        h, f, v = self._bc_ptrs()
+        lib().icxo_sys_assemble(s, _f64(design), h, f, v, fint.ctypes.data)
+        return self.fext - fint
+
+    def csr(self):
+        import scipy.sparse as sp
+        s = self.system()
+        nnz = lib().icxo_sys_nnz(s)
+        rp = np.zeros(self.numeq + 1, dtype=np.int32)
+        col = np.zeros(nnz, dtype=np.int32)
+        val = np.zeros(nnz)
+        lib().icxo_sys_csr(s, rp, col, val)
+        return sp.csr_matrix((val, col, rp), shape=(self.numeq, self.numeq))
+
+    def spmv(self, x):
+        y = np.zeros(self.numeq)
+        lib().icxo_sys_spmv(self.system(), _f64(x), y)
+        return y
+
+    def cg(self, b, tol=1e-10, maxit=10000):
+        x = np.zeros(self.numeq)
+        err = C.c_double()
+        it = lib().icxo_sys_cg(self.system(), _f64(b), x, tol, maxit, C.byref(err))
+        return x, it, err.value
+
+    def direct(self, b):
+        x = np.zeros(self.numeq)
+        rc = lib().icxo_sys_direct(self.system(), _f64(b), x)
+        if rc != 0:
+            raise RuntimeError("banded Cholesky failed (%d)" % rc)
+        return x
+
+    def node_values(self, u):
+        out = np.zeros(self.numnp * 3)
+        lib().icxo_divide_val_into_node(self.numnp, self.dofnp, self.numeq, self.dof, _f64(u),
+                                        self.node_val.ctypes.data, out)
+        return out
+
+    def element_energy(self, nodeval):
+        out = np.zeros(self.nelx)
+        lib().icxo_sys_element_energy(self.system(), _f64(nodeval), out)
+        return out
+
+    def volumes(self):
+        X = self.x.reshape(-1, 3)[:, :self.ndim]
+        vol = np.zeros(self.nelx)
+        Ke = np.zeros((self.ne * self.dofnp) ** 2)
+        v = np.zeros(1)
+        # all elements of a box mesh share one shape; volumes via quadrature per element
+        for e in range(self.nelx):
+            Xe = _f64(X[self.conn[e * self.ne:(e + 1) * self.ne]])
+            lib().icxo_element_stiffness(self.ne, self.ndim, self.state, 1.0, self.nu, Xe, Ke, v)
+            vol[e] = v[0]
+        return vol
+
+    def sensitivity(self, design, energy, volume, V0=1.0, driver_b=False):
+        dfds = np.zeros(self.nelx)
+        dgds = np.zeros(self.nelx)
+        lib().icxo_sensitivity(self.nelx, int(driver_b), self.young1, self.young2, self.p,
+                               _f64(design), _f64(energy), _f64(volume), V0, dfds, dgds)
+        return dfds, dgds
+
+    def element_stiffness(self, E=1.0, e=0):
+        X = _f64(self.x.reshape(-1, 3)[self.conn[e * self.ne:(e + 1) * self.ne], :self.ndim])
+        n = self.ne * self.dofnp
+        Ke = np.zeros(n * n)
+        v = np.zeros(1)
+        lib().icxo_element_stiffness(self.ne, self.ndim, self.state, E, self.nu, X, Ke, v)
+        return Ke.reshape(n, n), v[0]
+
+    # -- filters ---------------------------------------------------------------
+    def cone_filter(self, radius, brute_force=False):
+        return ConeFilter(self, radius, brute_force)
+
+    def helmholtz(self, radius):
+        return Helmholtz(self, radius)
+
+    # -- drivers ---------------------------------------------------------------
+    def run_driver(self, driver, x0, iterations, radius, projection=True, direct=False,
+                   cg_tol=1e-10, cg_maxit=10000, helm_tol=1e-10, helm_direct=False,
+                   xmin=1e-4, xmax=1.0, move=0.15):
+        class Cfg(C.Structure):
+            _fields_ = [("driver", C.c_int), ("ne", C.c_int), ("ndim", C.c_int),
+                        ("dofnp", C.c_int), ("state", C.c_int), ("numnp", C.c_int),
+                        ("nelx", C.c_int), ("numeq", C.c_int),
+                        ("x", C.c_void_p), ("conn", C.c_void_p), ("dof", C.c_void_p),
+                        ("has", C.c_void_p), ("node_flag", C.c_void_p), ("node_val", C.c_void_p),
+                        ("fext", C.c_void_p),
+                        ("young1", C.c_double), ("young2", C.c_double), ("nu", C.c_double),
+                        ("p", C.c_double), ("filter_radius", C.c_double),
+                        ("xmin", C.c_double), ("xmax", C.c_double), ("move", C.c_double),
+                        ("iterations", C.c_int), ("projection", C.c_int), ("direct", C.c_int),
+                        ("cg_tol", C.c_double), ("cg_maxit", C.c_int),
+                        ("helm_tol", C.c_double), ("helm_direct", C.c_int)]
+        h, f, v = self._bc_ptrs()
+        cfg = Cfg(driver, self.ne, self.ndim, self.dofnp, self.state, self.numnp, self.nelx,
+                  self.numeq, self.x.ctypes.data, self.conn.ctypes.data, self.dof.ctypes.data,
+                  h, f, v, self.fext.ctypes.data, self.young1, self.young2, self.nu, self.p,
+                  radius, xmin, xmax, move, iterations, int(projection), int(direct), cg_tol,
+                  cg_maxit, helm_tol, int(helm_direct))
+        x = _f64(x0).copy()
+        obj = np.zeros(iterations)
+        vol = np.zeros(iterations)
+        dfds = np.zeros(self.nelx)
+        u = np.zeros(self.numeq)
+        its = np.zeros(iterations, dtype=np.int32)
+        L = lib()
+        L.icxo_run_driver.argtypes = [C.POINTER(Cfg), _dp, _dp, _dp, _dp, _dp, _ip]
+        rc = L.icxo_run_driver(C.byref(cfg), x, obj, vol, dfds, u, its)
+        if rc != 0:
+            raise RuntimeError("driver failed %d" % rc)
+        return dict(design=x, obj=obj, vol=vol, dfds=dfds, u=u, cg_iters=its)
+
+
+class ConeFilter:
+    def __init__(self, prob, radius, brute_force=False):
+        self.n = prob.nelx
+        self.h = lib().icxo_filter_create(prob.ne, prob.ndim, prob.nelx, prob.x, prob.conn,
+                                          float(radius), int(brute_force))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.icxo_filter_destroy(self.h)
+
+    def neighbours(self):
+        nnz = lib().icxo_filter_nnz(self.h)
+        ptr = np.zeros(self.n + 1, dtype=np.int32)
+        ids = np.zeros(nnz, dtype=np.int32)
+        dist = np.zeros(nnz)
+        lib().icxo_filter_neighbours(self.h, ptr, ids, dist)
+        return ptr, ids, dist
+
+    def apply(self, v, sens=False):
+        out = np.zeros(self.n)
+        lib().icxo_filter_apply(self.h, _f64(v), out, int(sens))
+        return out
+
+
+class Helmholtz:
+    def __init__(self, prob, radius):
+        self.n = prob.nelx
+        self.h = lib().icxo_helm_create(prob.ne, prob.ndim, prob.numnp, prob.nelx, prob.x,
+                                        prob.conn, float(radius))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.icxo_helm_destroy(self.h)
+
+    def apply(self, v, sens=False, tol=1e-10, maxit=10000, direct=False):
+        out = np.zeros(self.n)
+        it = lib().icxo_helm_apply(self.h, _f64(v), out, int(sens), tol, maxit, int(direct))
+        return out, it
+
+
+def threshold(v, beta, T=0.5):
+    out = np.zeros(len(v))
+    lib().icxo_threshold(len(v), _f64(v), beta, T, out)
+    return out
+
+
+def threshold_derivative(v, beta, T=0.5):
+    out = np.zeros(len(v))
+    lib().icxo_threshold_derivative(len(v), _f64(v), beta, T, out)
+    return out
+
+
+def oc_update(x, dfdx, g, dgdx, xmin, xmax, volume_fn, move=0.15):
+    x = _f64(x).copy()
+    n = len(x)
+
+    def cb(ptr, _user):
+        arr = np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+        return float(volume_fn(arr))
+
+    fn = VOLFN(cb)
+    L = lib()
+    L.icxo_oc_update.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, _dp, C.c_double,
+                                 VOLFN, C.c_void_p]
+    it = L.icxo_oc_update(n, x, _f64(dfdx), float(g), _f64(dgdx), _f64(xmin), _f64(xmax),
+                          move, fn, None)
+    return x, it
+
+
+class MMA:
+    def __init__(self, n, m=1, a=0.0, c=1e5, d=0.0):
+        self.n, self.m = n, m
+        self.h = lib().icxo_mma_create(n, m, a, c, d)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.icxo_mma_destroy(self.h)
+
+    def update(self, x, dfdx, gx, dgdx, xmin, xmax):
+        x = _f64(x).copy()
+        it = lib().icxo_mma_update(self.h, x, _f64(dfdx), _f64(np.atleast_1d(gx)), _f64(dgdx),
+                                   _f64(xmin), _f64(xmax))
+        return x, it
+
+
+# -- the benchmark problems ----------------------------------------------------
+
+def cantilever_2d(nx=120, ny=40, state=PLANE_STRESS, young=(1e-9, 1.0), nu=0.3, p=3.0):
+    """Q4 nx x ny, h = 1, left edge clamped, unit downward nodal load at the right
+    mid-edge node (written straight into fext_org)."""
+    pr = Problem(4, (nx, ny, 1), (float(nx), float(ny), 1.0), state, young, nu, p)
+    pr.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0.0, 0.0, 0.0, float(ny), 0.0, 0.0])
+    pr.make_dof()
+    pr.add_nodal_load(pr.node_index(nx, ny // 2), 1, -1.0)
+    return pr
+
+
+def cantilever_3d(nx, ny, nz, young=(1e-9, 1.0), nu=0.3, p=3.0):
+    """hex8 nx x ny x nz, h = 1, x = 0 face clamped, total F_z = -1 on the edge
+    {x = L, z = 0} with consistent line-load weights (end nodes 1/2)."""
+    pr = Problem(8, (nx, ny, nz), (float(nx), float(ny), float(nz)), SOLID3D, young, nu, p)
+    pr.add_dirichlet([1, 1, 1, 0, 0, 0], [0] * 6, [0.0, 0.0, 0.0, float(ny), 0.0, float(nz)])
+    pr.make_dof()
+    for j in range(ny + 1):
+        w = 0.5 if j in (0, ny) else 1.0
+        pr.add_nodal_load(pr.node_index(nx, j, 0), 2, -w / ny)
+    return pr
+
+
+def random_density(n, seed, lo=0.01, hi=1.0):
+    """Element densities i.i.d. U[lo, hi) in element-id order from numpy's PCG64
+    stream with the given seed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return lo + (hi - lo) * rng.random(n)

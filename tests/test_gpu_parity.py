@@ -1,0 +1,230 @@
+"""GPU parity of libicx against the CPU oracle on the same seeded inputs.
+
+Bars: integer set-up bit-exact; FP64 operator and vector kernels to rounding
+(1e-13 relative); solutions to 1e-10 relative against the oracle's direct solve;
+sensitivities to 1e-8 relative (scaled by max |dc|)."""
+import numpy as np
+import pytest
+
+import icx
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(ctx, ne, div, length, state, bcs, young=(1e-9, 1.0)):
+    job = icx.Job(ctx, ne, div, length, state, young)
+    pr = O.Problem(ne, div, length, state, young)
+    for f, v, r in bcs:
+        job.add_dirichlet(f, v, r)
+        pr.add_dirichlet(f, v, r)
+    assert job.make_dof() == pr.make_dof()
+    return job, pr
+
+
+BC2 = [([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, 5, 0, 0]),
+       ([0, 1, 0, 0, 0, 0], [0] * 6, [7, 7, 0, 2, 0, 0])]
+BC3 = [([1, 1, 1, 0, 0, 0], [0] * 6, [0, 0, 0, 3, 0, 4]),
+       ([0, 0, 1, 0, 0, 0], [0] * 6, [0, 5, 0, 3, 0, 0])]
+
+
+def test_mesh_dof_map_bit_exact(ctx):
+    for ne, div, L, st, bcs in [(4, (7, 5, 1), (7.0, 5.0, 1.0), O.PLANE_STRAIN, BC2),
+                                (8, (5, 3, 4), (5.0, 3.0, 4.0), O.SOLID3D, BC3)]:
+        job, pr = _pair(ctx, ne, div, L, st, bcs)
+        x, conn = job.mesh()
+        assert np.array_equal(x, pr.x) and np.array_equal(conn, pr.conn)
+        assert np.array_equal(job.dof_map(), pr.dof)
+
+
+def test_neumann_tractions(ctx):
+    job, pr = _pair(ctx, 8, (6, 3, 4), (6.0, 3.0, 4.0), O.SOLID3D, BC3)
+    for val, rng in [([0.0, 0.0, -1.0], [6, 6, 0, 3, 0, 4]), ([0.3, 0.1, 0.0], [4, 6, 0, 3, 0, 1]),
+                     ([0.0, -2.0, 0.0], [0, 6, 3, 3, 2, 4])]:
+        job.add_neumann(val, rng)
+        pr.add_neumann(val, rng)
+    assert np.abs(job.fext() - pr.fext).max() <= 1e-15
+    j2, p2 = _pair(ctx, 4, (7, 5, 1), (7.0, 5.0, 1.0), O.PLANE_STRESS, BC2)
+    j2.add_neumann([0.0, -1.0, 0.0], [7, 7, 0, 5, 0, 0])
+    p2.add_neumann([0.0, -1.0, 0.0], [7, 7, 0, 5, 0, 0])
+    j2.add_neumann([0.5, 0.0, 0.0], [3, 7, 0, 1, 0, 0])
+    p2.add_neumann([0.5, 0.0, 0.0], [3, 7, 0, 1, 0, 0])
+    assert np.abs(j2.fext() - p2.fext).max() <= 1e-15
+
+
+CASES = [
+    (4, (9, 5, 1), (9.0, 5.0, 1.0), O.PLANE_STRESS, BC2),
+    (4, (7, 5, 1), (10.5, 3.0, 1.0), O.PLANE_STRAIN, BC2),
+    (8, (5, 3, 4), (5.0, 3.0, 4.0), O.SOLID3D, BC3),
+    (8, (17, 19, 9), (17.0, 38.0, 4.5), O.SOLID3D, BC3),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_matvec_fp64_and_fp32(ctx, case):
+    ne, div, L, st, bcs = CASES[case]
+    job, pr = _pair(ctx, ne, div, L, st, bcs)
+    rho = O.random_density(pr.nelx, 100 + case)
+    job.set_design(rho)
+    pr.assemble(rho)
+    xv = np.random.default_rng(case).standard_normal(pr.numeq)
+    y_ref = pr.spmv(xv)
+    y64 = job.matvec(xv, prec=0)
+    scale = np.abs(y_ref).max()
+    assert np.abs(y64 - y_ref).max() <= 1e-13 * scale
+    y32 = job.matvec(xv, prec=1)
+    assert np.abs(y32 - y_ref).max() <= 2e-5 * scale
+    d = job.diagonal()
+    K = pr.csr()
+    assert np.abs(d - K.diagonal()).max() <= 1e-14 * np.abs(K.diagonal()).max()
+
+
+def _solve_case(ctx, nx=12, ny=6, nz=6, seed=5):
+    job = icx.cantilever_3d(ctx, nx, ny, nz)
+    pr = O.cantilever_3d(nx, ny, nz)
+    rho = O.random_density(pr.nelx, seed)
+    job.set_design(rho)
+    ud = pr.direct(pr.assemble(rho))
+    return job, pr, rho, ud
+
+
+def test_plain_form_follows_eigen_loop(ctx):
+    job, pr, rho, ud = _solve_case(ctx)
+    u, info = job.solve(icx.SOLVE_PLAIN, tol=1e-10)
+    uo, ito, erro = pr.cg(pr.assemble(rho), tol=1e-10)
+    assert info["converged"]
+    assert abs(info["iterations"] - ito) <= max(3, int(0.01 * ito))
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+
+
+@pytest.mark.parametrize("form", [icx.SOLVE_PLAIN, icx.SOLVE_FUSED])
+def test_fp64_forms_match_direct_solution(ctx, form):
+    job, pr, rho, ud = _solve_case(ctx)
+    u, info = job.solve(form, tol=1e-13, maxit=20000)
+    assert info["converged"]
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-10
+    c_ref = ud @ pr.fext
+    assert abs(job.compliance() - c_ref) / c_ref < 1e-10
+
+
+def test_mixed_form_meets_true_residual(ctx):
+    job, pr, rho, ud = _solve_case(ctx)
+    u, info = job.solve(icx.SOLVE_MIXED, tol=1e-10, maxit=20000)
+    assert info["converged"] and info["refinements"] >= 1
+    K = pr.csr()
+    b = pr.assemble(rho)
+    assert np.linalg.norm(b - K @ u) / np.linalg.norm(b) < 1e-10
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-7
+
+
+def test_2d_solve_and_node_values(ctx):
+    job = icx.cantilever_2d(ctx, 40, 14)
+    pr = O.cantilever_2d(40, 14)
+    rho = O.random_density(pr.nelx, 9)
+    job.set_design(rho)
+    ud = pr.direct(pr.assemble(rho))
+    u, info = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=20000)
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-10
+    nv = job.node_values()
+    assert np.abs(nv - pr.node_values(u)).max() == 0.0
+
+
+def test_sensitivity(ctx):
+    for nx, ny, nz in [(12, 6, 6), (7, 3, 5)]:
+        job, pr, rho, ud = _solve_case(ctx, nx, ny, nz, seed=nx)
+        u, _ = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=20000)
+        dc, dv = job.sensitivity()
+        dco, dvo = pr.sensitivity(rho, pr.element_energy(pr.node_values(ud)), pr.volumes())
+        assert np.abs(dc - dco).max() <= 1e-8 * np.abs(dco).max()
+        assert np.abs(dv - dvo).max() <= 1e-15
+        dc2, dv2 = job.sensitivity(vscale=7.5)
+        assert np.array_equal(dc2, dc) and np.abs(dv2 - dvo / 7.5).max() < 1e-16
+    j2 = icx.cantilever_2d(ctx, 30, 10)
+    p2 = O.cantilever_2d(30, 10)
+    rho = O.random_density(p2.nelx, 2)
+    j2.set_design(rho)
+    ud = p2.direct(p2.assemble(rho))
+    j2.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=20000)
+    dc, _ = j2.sensitivity()
+    dco, _ = p2.sensitivity(rho, p2.element_energy(p2.node_values(ud)), p2.volumes())
+    assert np.abs(dc - dco).max() <= 1e-8 * np.abs(dco).max()
+
+
+@pytest.mark.parametrize("ne,div,r", [(4, (30, 11, 1), 1.5), (4, (20, 9, 1), 3.0),
+                                      (8, (9, 7, 6), 1.5), (8, (8, 6, 7), 2.3)])
+def test_cone_filter_forward_adjoint(ctx, ne, div, r):
+    st = O.PLANE_STRESS if ne == 4 else O.SOLID3D
+    L = tuple(float(d) for d in div)
+    job = icx.Job(ctx, ne, div, L, st)
+    pr = O.Problem(ne, div, L, st)
+    job.cone_setup(r)
+    f = pr.cone_filter(r)
+    x = O.random_density(pr.nelx, 4)
+    s = np.random.default_rng(2).standard_normal(pr.nelx)
+    assert np.abs(job.cone(x) - f.apply(x)).max() <= 1e-15
+    assert np.abs(job.cone(s, adjoint=True) - f.apply(s, sens=True)).max() <= 1e-14
+
+
+def test_projection(ctx):
+    job = icx.Job(ctx, 8, (5, 4, 3), (5.0, 4.0, 3.0), O.SOLID3D)
+    x = np.linspace(0, 1, job.nelx)
+    for beta in (1.0, 4.0, 16.0):
+        h, dh = job.project(x, beta)
+        assert np.abs(h - O.threshold(x, beta)).max() <= 2e-16
+        assert np.abs(dh - O.threshold_derivative(x, beta)).max() <= 1e-14 * beta
+
+
+@pytest.mark.parametrize("ne,div,R", [(4, (24, 10, 1), 2.0), (8, (9, 6, 5), 2.0), (8, (10, 7, 6), 3.0)])
+def test_helmholtz_filter(ctx, ne, div, R):
+    st = O.PLANE_STRESS if ne == 4 else O.SOLID3D
+    L = tuple(float(d) for d in div)
+    job = icx.Job(ctx, ne, div, L, st)
+    pr = O.Problem(ne, div, L, st)
+    job.helmholtz_setup(R)
+    H = pr.helmholtz(R)
+    x = O.random_density(pr.nelx, 8)
+    for sens in (False, True):
+        g, it = job.helmholtz(x, sens=sens, tol=1e-13)
+        o, _ = H.apply(x, sens=sens, tol=1e-13)
+        assert np.abs(g - o).max() <= 1e-10 * np.abs(o).max()
+    # reference tolerance 1e-10: same iteration count as the oracle's Eigen loop (+-2)
+    g, it = job.helmholtz(x, tol=1e-10)
+    o, ito = H.apply(x, tol=1e-10)
+    assert abs(it - ito) <= 2
+
+
+def test_oc_update_matches(ctx):
+    job = icx.Job(ctx, 8, (12, 6, 6), (12.0, 6.0, 6.0), O.SOLID3D)
+    pr = O.Problem(8, (12, 6, 6), (12.0, 6.0, 6.0), O.SOLID3D)
+    job.cone_setup(1.5)
+    f = pr.cone_filter(1.5)
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0.2, 0.8, pr.nelx)
+    dc = -rng.uniform(0.01, 1.0, pr.nelx)
+    dv = np.ones(pr.nelx)
+    beta = 2.0
+    V0 = O.threshold(f.apply(np.full(pr.nelx, 0.5)), beta).sum()
+    xg, nb = job.oc_update(x, dc, V0, dv, filter=icx.FILTER_CONE, beta=beta)
+    xo, no = O.oc_update(x, dc, V0, dv, np.full(pr.nelx, 1e-4), np.ones(pr.nelx),
+                         lambda z: O.threshold(f.apply(z), beta).sum())
+    assert nb == no
+    assert np.abs(xg - xo).max() <= 1e-13
+
+
+def test_mma_updates_match(ctx):
+    n = 300
+    job = icx.Job(ctx, 8, (10, 6, 5), (10.0, 6.0, 5.0), O.SOLID3D)
+    assert job.nelx == n
+    mma = O.MMA(n, 1)
+    rng = np.random.default_rng(7)
+    xg = rng.uniform(0.2, 0.6, n)
+    xo = xg.copy()
+    c = rng.uniform(0.5, 2.0, n)
+    for it in range(4):
+        df = -c / xo ** 2
+        g = xo.sum() / (0.4 * n) - 1.0
+        dg = np.full(n, 1.0 / (0.4 * n))
+        xg, sg = job.mma_update(xg, df, g, dg, 1e-4, 1.0)
+        xo, so = mma.update(xo, df, g, dg, np.full(n, 1e-4), np.ones(n))
+        assert np.abs(xg - xo).max() <= 1e-9, it
+        xg = xo.copy()   # teacher forcing: continue both from the oracle's iterate
