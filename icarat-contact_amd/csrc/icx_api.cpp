@@ -29,6 +29,7 @@ struct icx_ctx {
   SolverState *h_state = nullptr;  // pinned mirror for polling
   double *h_scalar = nullptr;      // pinned scalars
   // profile window (icx_profile.h)
+  std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
   int prof_kernel = 0;
   std::vector<hipEvent_t> ev_pool;
   int ev_used = 0;
@@ -459,9 +460,13 @@ icx_status build_rhs(icx_job *j) {
 }
 
 // ---- FP64 plain PCG, Eigen ConjugateGradient loop --------------------------------
+// literal = true: one kernel per Eigen vector expression (the parity form of the
+// structural solve); false: the same loop with the updates fused into two kernels
+// (used for the Helmholtz filter solves).
 template <class MatvecFn>
 icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const double *invd,
-                     PcgWork64 &w, double tol, int maxit, MatvecFn matvec, icx_solve_info &info) {
+                     PcgWork64 &w, double tol, int maxit, MatvecFn matvec, icx_solve_info &info,
+                     bool literal = false) {
   icx_ctx *c = j->ctx;
   HIPCHK(c, hipMemsetAsync(x, 0, N * sizeof(double), S(j)));
   launch_pcg_init<double>(N, j->d_state.p, b, w.r.p, w.p.p, invd, j->red(), tol, maxit, 1, S(j));
@@ -469,9 +474,13 @@ icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const 
   int polled = 0;
   const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
   for (int it = 0; it < maxit + 1; ++it) {
-    matvec(w.p.p, w.t.p);  // t = A p, pAp, alpha
-    launch_pcg_update<double>(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, j->red(), S(j));
-    launch_pcg_pupdate<double>(N, j->d_state.p, w.p.p, w.z.p, S(j));
+    matvec(w.p.p, w.t.p);  // t = A p (fused variant also reduces pAp, alpha)
+    if (literal) {
+      launch_eigen_iteration_tail(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, j->red(), S(j));
+    } else {
+      launch_pcg_update<double>(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, j->red(), S(j));
+      launch_pcg_pupdate<double>(N, j->d_state.p, w.p.p, w.z.p, S(j));
+    }
     if (++polled == every || it == maxit) {
       polled = 0;
       icx_status rc = read_state(j, st);
@@ -537,6 +546,8 @@ icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches) {
 icx_status icx_ctx_destroy(icx_ctx *ctx) {
   if (!ctx) return ICX_ERR_ARG;
   (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  while (!ctx->jobs.empty()) icx_job_destroy(ctx->jobs.back());
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->h_state) (void)hipHostFree(ctx->h_state);
@@ -601,14 +612,19 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   (void)hipMemsetAsync(j->d_fext.p, 0, N * sizeof(double), ctx->stream);
   (void)hipMemsetAsync(j->d_mask.p, 0, g.nn, ctx->stream);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) { delete j; return fail(ctx, ICX_ERR_HIP, "stream sync failed"); }
+  ctx->jobs.push_back(j);
   *out = j;
   return ICX_OK;
 }
 
 icx_status icx_job_destroy(icx_job *j) {
   if (!j) return ICX_ERR_ARG;
-  (void)hipSetDevice(j->ctx->device);
-  (void)hipStreamSynchronize(j->ctx->stream);
+  icx_ctx *c = j->ctx;
+  auto it = std::find(c->jobs.begin(), c->jobs.end(), j);
+  if (it == c->jobs.end()) return ICX_ERR_ARG;  // not a live job of its context
+  c->jobs.erase(it);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   delete j;
   return ICX_OK;
 }
@@ -896,9 +912,10 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
   launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
   auto mv = [&](const double *p, double *t) {
     ProfScope ps(c, ICX_PROF_MATVEC64);
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, j->red(), S(j));
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p,
+                                  Reduce{nullptr, nullptr}, S(j));
   };
-  return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info);
+  return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info, true);
 }
 
 static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info &info) {
@@ -914,6 +931,14 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
   // no replacement triggers in the FP64 fused form
   SolverState st;
   double *pa = w.pa.p, *pb = w.pb.p;
+  // The A kernel of inner iteration i (1-based) writes the new direction into pb
+  // and the host swaps; kernels that early-exit after the device set `done` still
+  // get swapped here, so the live buffers are re-derived from the device's count
+  // of executed iterations (each executed A is followed by an executed B).
+  auto resync = [&](int iters) {
+    pa = (iters & 1) ? w.pb.p : w.pa.p;
+    pb = (iters & 1) ? w.pa.p : w.pb.p;
+  };
   const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
   int k = 0;
   for (;;) {
@@ -930,6 +955,7 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
     if (++k % every == 0) {
       icx_status rc = read_state(j, st);
       if (rc) return rc;
+      resync(st.iter);
       if (st.done) break;
     }
   }
@@ -978,6 +1004,12 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
   st.rr_ref = st.rr;
   HIPCHK(c, hipMemcpy(j->d_state.p, &st, sizeof(SolverState), hipMemcpyHostToDevice));
   float *pa = w.pa.p, *pb = w.pb.p;
+  // live direction buffers from the device's count of executed iterations
+  // (see solve_fused)
+  auto resync = [&](int iters) {
+    pa = (iters & 1) ? w.pb.p : w.pa.p;
+    pb = (iters & 1) ? w.pa.p : w.pb.p;
+  };
   const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
   int refinements = 0, fp64mv = 0;
   bool converged = st.done == 1;
@@ -996,6 +1028,7 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
     if (++k % every) continue;
     rc = read_state(j, st);
     if (rc) return rc;
+    resync(st.iter);
     if (st.done == 0) continue;
     // replacement: x_hi += x_lo + alpha p; r = b - K x_hi (FP64)
     launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, pa, S(j));

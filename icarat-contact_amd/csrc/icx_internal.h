@@ -86,6 +86,11 @@ void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, co
 void launch_helm_matvec(const Grid &g, const HelmOp<double> &op, const double *x, double *y,
                         SolverState *st, Reduce red, hipStream_t s);
 
+// the Eigen loop after tmp = A p: p.tmp, x += a p, r -= a tmp, |r|^2, z = invd r,
+// r.z, p = z + beta p -- one kernel each, in Eigen's order
+void launch_eigen_iteration_tail(long long n, SolverState *st, double *x, double *r, double *p,
+                                 const double *t, double *z, const double *invd, Reduce red,
+                                 hipStream_t s);
 // plain (Eigen-structured) vector kernels
 template <typename T>
 void launch_pcg_update(long long n, SolverState *st, T *x, T *r, const T *p, const T *t, T *z,

@@ -266,7 +266,7 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   __shared__ T s_in[2][3][LN];
   __shared__ T s_out[24][NT];
   const Grid &g = a.g;
-  if (MODE == 1 && a.st->done) return;
+  if (a.st && a.st->done) return;  // solve already finished (uniform per launch)
   const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
   const int t = xcd_tile(blockIdx.x, nblocks);
   const int ti = t % a.tiles_x, tj = (t / a.tiles_x) % a.tiles_y, tk = t / (a.tiles_x * a.tiles_y);
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
   __shared__ T s_in[2][2][LW];
   __shared__ T s_out[8][NT];
   const Grid &g = a.g;
-  if (MODE == 1 && a.st->done) return;
+  if (a.st && a.st->done) return;  // solve already finished (uniform per launch)
   const int nblocks = a.tiles_x * a.tiles_y;
   const int t = xcd_tile(blockIdx.x, nblocks);
   const int ti = t % a.tiles_x, tj = t / a.tiles_x;
@@ -689,6 +689,83 @@ __global__ void k_pcg_update(long long n, SolverState *st, T *x, T *r, const T *
       if (st->iter >= st->maxit) st->done = 2;
     }
   }
+}
+
+template <typename T>
+__global__ void k_pcg_pupdate(long long n, SolverState *st, T *p, const T *z);
+
+// ---- the Eigen ConjugateGradient loop, one kernel per vector expression ----------
+// (Eigen 3.4 ConjugateGradient.h as called by src/analysis/solver/solvers.cpp:64-92)
+// alpha = absNew / p.dot(tmp)
+__global__ void k_eig_dot_pt(long long n, SolverState *st, const double *p, const double *t, Reduce red) {
+  if (st->done) return;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s += p[i] * t[i];
+  double v[1] = {s}, tot[1];
+  if (grid_reduce<1>(red, v, tot) && threadIdx.x == 0) {
+    st->pAp = tot[0];
+    st->alpha = st->absNew / tot[0];
+  }
+}
+// y += sign * alpha * v   (x += alpha p; residual -= alpha tmp)
+__global__ void k_eig_axpy(long long n, SolverState *st, double sign, double *y, const double *v) {
+  if (st->done) return;
+  const double a = sign * st->alpha;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] += a * v[i];
+}
+// residualNorm2 = residual.squaredNorm(); break if below threshold
+__global__ void k_eig_norm(long long n, SolverState *st, const double *r, Reduce red) {
+  if (st->done) return;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s += r[i] * r[i];
+  double v[1] = {s}, tot[1];
+  if (grid_reduce<1>(red, v, tot) && threadIdx.x == 0) {
+    st->rr = tot[0];
+    if (tot[0] < st->threshold) st->done = 1;
+  }
+}
+// z = precond.solve(residual)
+__global__ void k_eig_precond(long long n, SolverState *st, double *z, const double *invd, const double *r) {
+  if (st->done) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    z[i] = invd[i] * r[i];
+}
+// absOld = absNew; absNew = residual.dot(z); beta = absNew / absOld; i++
+__global__ void k_eig_dot_rz(long long n, SolverState *st, const double *r, const double *z, Reduce red) {
+  if (st->done) return;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s += r[i] * z[i];
+  double v[1] = {s}, tot[1];
+  if (grid_reduce<1>(red, v, tot) && threadIdx.x == 0) {
+    st->absOld = st->absNew;
+    st->absNew = tot[0];
+    st->beta = tot[0] / st->absOld;
+    st->iter += 1;
+    if (st->iter >= st->maxit) st->done = 2;
+  }
+}
+
+void launch_eigen_iteration_tail(long long n, SolverState *st, double *x, double *r, double *p,
+                                 const double *t, double *z, const double *invd, Reduce red,
+                                 hipStream_t s) {
+  const int nb = ew_blocks(n, 4);
+  hipLaunchKernelGGL(k_eig_dot_pt, dim3(nb), dim3(256), 0, s, n, st, p, t, red);
+  hipLaunchKernelGGL(k_eig_axpy, dim3(nb), dim3(256), 0, s, n, st, 1.0, x, p);
+  hipLaunchKernelGGL(k_eig_axpy, dim3(nb), dim3(256), 0, s, n, st, -1.0, r, t);
+  hipLaunchKernelGGL(k_eig_norm, dim3(nb), dim3(256), 0, s, n, st, r, red);
+  hipLaunchKernelGGL(k_eig_precond, dim3(nb), dim3(256), 0, s, n, st, z, invd, r);
+  hipLaunchKernelGGL(k_eig_dot_rz, dim3(nb), dim3(256), 0, s, n, st, r, z, red);
+  hipLaunchKernelGGL(k_pcg_pupdate<double>, dim3(nb), dim3(256), 0, s, n, st, p, z);
+  ICX_CHECK_LAUNCH();
 }
 
 template <typename T>

@@ -1,0 +1,369 @@
+// C++ drop-in layer over libicx: the reference's analysis / optimisation call
+// surface for structured quad4 / hexa8 meshes, with every hot loop on the GPU.
+//
+// Class and method names, argument meaning and the order of calls follow the
+// reference (paths relative to the reference repository root):
+//   FEM, Force, Value, Optimize ........ src/problem/base.hpp:16-129, src/optimization/opt_base.hpp:14-76
+//   StructuredModel .................... MeshIcarat (src/io/mesh_icarat.hpp:69-136) + BCIcarat
+//                                        (src/io/boundary_condition_icarat.hpp:68-396) + LinearElastic
+//                                        ([material], src/problem/elasticity.cpp:19-32)
+//   LinearAnalysis::solve(name, log) ... src/analysis/linear.hpp:54-104 + Solvers::solve
+//                                        (src/analysis/solver/solvers.cpp:28-62)
+//   Filter ............................. src/optimization/filter.hpp:17-185
+//   FilterHelmholtz .................... src/optimization/filter_helm.hpp:23-173
+//   OC ................................. src/optimization/opt-solver/OCSolver.hpp:13-80
+//   MMASolver .......................... src/optimization/opt-solver/MMASolver.hpp:62-116
+// Differences: vectors are std::vector<double> (Eigen is not required); the model
+// object replaces the reference's vector<LinearElastic>/vector<Node> pair; solver
+// names gain "HIP_PCG", "HIP_PCG_FUSED" and "HIP_PCG_MIXED" ("Eigen_CG" runs the
+// FP64 plain form at the reference's tolerance, "Eigen_LDLT" the plain form at 1e-14).
+// As in the reference, an unknown solver name or a degenerate filter prints to
+// std::cerr and exits with status 1.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/icx.h"
+#include "toml_lite.hpp"
+
+namespace icarat {
+
+// ---------------------------------------------------------------------------------
+inline void check(icx_ctx *ctx, icx_status rc, const char *what) {
+  if (rc != ICX_OK && rc != ICX_ERR_NOCONV)
+    throw std::runtime_error(std::string(what) + ": " + icx_last_error(ctx));
+}
+
+// one context (device 0, one stream) per process
+inline icx_ctx *default_context() {
+  static icx_ctx *ctx = [] {
+    icx_ctx *c = nullptr;
+    if (icx_ctx_create(0, &c) != ICX_OK) throw std::runtime_error("no HIP device available");
+    return c;
+  }();
+  return ctx;
+}
+
+struct FEM {
+  int neq = 0, numeq = 0, ndim = 0, nelx = 0, voigt = 0, dofnp = 0, numnp = 0;
+};
+
+struct Force {
+  explicit Force(int numeq) : fext_org(numeq, 0.0), fext(numeq, 0.0), fint(numeq, 0.0), fres(numeq, 0.0) {}
+  std::vector<double> fext_org, fext, fint, fres;
+};
+
+struct Value {
+  explicit Value(int numeq) : val(numeq, 0.0) {}
+  std::vector<double> val;
+};
+
+class Optimize {
+ public:
+  Optimize(int design, int constraint, double xmin_, double xmax_)
+      : design_s(design), const_h(constraint), dfds(design), dgds(constraint * design),
+        xmin(design, xmin_), xmax(design, xmax_) {}
+  int isConvergence(double epsilon, double minstep) {
+    if (std::abs(object_old - object_f) / object_f < epsilon && optstep > minstep) {
+      std::cout << "\n--------------------Optimized--------------------" << std::endl;
+      return 0;
+    }
+    object_old = object_f;
+    return 1;
+  }
+  std::vector<double> design_s, const_h, dfds, dgds, xmin, xmax;
+  int optstep = 0;
+  double object_f = 0.0;
+
+ private:
+  double object_old = 0.0;
+};
+
+// ---------------------------------------------------------------------------------
+// structured box model on the device
+class StructuredModel {
+ public:
+  StructuredModel(const toml_lite::Table &config, FEM &fem) : fem_(fem) {
+    const std::string type = config.at("mesh", "type").as_string();
+    if (type != "quad4" && type != "hexa8")
+      fatal("element type " + type + " is not supported by the GPU path (quad4, hexa8)");
+    icx_problem p{};
+    p.eltype = type == "quad4" ? ICX_QUAD4 : ICX_HEXA8;
+    const auto &div = config.at("mesh", "division").as_array();
+    const auto &len = config.at("mesh", "length").as_array();
+    for (int d = 0; d < 3; ++d) {
+      p.div[d] = div[d].as_int();
+      p.length[d] = len[d].as_double();
+    }
+    const int ndim = config.at("mesh", "dimension").as_int();
+    std::string state = config.at("material", "state").as_string();
+    if (ndim == 3) state = "3D";
+    p.state = state == "plane_stress" ? ICX_PLANE_STRESS : (state == "plane_strain" ? ICX_PLANE_STRAIN : ICX_SOLID3D);
+    const auto &young = config.at("material", "young").as_array();
+    p.young1 = young[0].as_double();
+    p.young2 = young[1].as_double();
+    p.poisson = config.at("material", "poisson").as_double();
+    p.penalty = config.at("material", "penalty").as_double();
+    design_s0_ = config.at("material", "designs_0").as_double();
+    prob_ = p;
+    check(ctx(), icx_job_create(ctx(), &p, &job_), "icx_job_create");
+    int numnp, nelx, neq;
+    icx_job_sizes(job_, &numnp, &nelx, &neq);
+    fem_.ndim = ndim;
+    fem_.dofnp = ndim;
+    fem_.voigt = ndim == 2 ? 3 : 6;
+    fem_.numnp = numnp;
+    fem_.nelx = nelx;
+    fem_.neq = neq;
+    volume_.assign(nelx, 0.0);
+    icx_element_volume(job_, volume_.data());
+  }
+  ~StructuredModel() {
+    if (job_) icx_job_destroy(job_);
+  }
+  StructuredModel(const StructuredModel &) = delete;
+  StructuredModel &operator=(const StructuredModel &) = delete;
+
+  // BCIcarat::input_dirich
+  void input_dirich(const toml_lite::Table &config) {
+    for (const auto &entry : config.at("boundary_condition", "dirichlet").as_array()) {
+      const auto &e = entry.as_array();
+      int flag[6];
+      double val[6], range[6];
+      for (int k = 0; k < 6; ++k) {
+        flag[k] = e[0].as_array()[k].as_int();
+        val[k] = e[1].as_array()[k].as_double();
+      }
+      for (int d = 0; d < 3; ++d)
+        for (int k = 0; k < 2; ++k) range[2 * d + k] = e[2].as_array()[d].as_array()[k].as_double();
+      check(ctx(), icx_add_dirichlet(job_, flag, val, range), "icx_add_dirichlet");
+    }
+  }
+  // BCIcarat::makeDOF
+  int makeDOF() {
+    int numeq = 0;
+    check(ctx(), icx_make_dof(job_, &numeq), "icx_make_dof");
+    return numeq;
+  }
+  // BCIcarat::input_neumann (accumulates into force.fext_org)
+  void input_neumann(const toml_lite::Table &config, Force &force) {
+    if (config.has("boundary_condition", "neumann"))
+      for (const auto &entry : config.at("boundary_condition", "neumann").as_array()) {
+        const auto &e = entry.as_array();
+        double val[3], range[6];
+        for (int k = 0; k < 3; ++k) val[k] = e[0].as_array()[k].as_double();
+        for (int d = 0; d < 3; ++d)
+          for (int k = 0; k < 2; ++k) range[2 * d + k] = e[1].as_array()[d].as_array()[k].as_double();
+        check(ctx(), icx_add_neumann(job_, val, range), "icx_add_neumann");
+      }
+    force.fext_org.assign(fem_.numeq, 0.0);
+    check(ctx(), icx_get_fext(job_, force.fext_org.data()), "icx_get_fext");
+  }
+  // element[i].setDesignS(d[i]) for all elements
+  void setDesignS(const std::vector<double> &d) {
+    check(ctx(), icx_set_design(job_, d.data()), "icx_set_design");
+    design_ = d;
+  }
+  const std::vector<double> &design_s() const { return design_; }
+  double design_s0() const { return design_s0_; }
+  const std::vector<double> &volume() const { return volume_; }
+  const icx_problem &problem() const { return prob_; }
+  icx_job *job() const { return job_; }
+  icx_ctx *ctx() const { return default_context(); }
+  static void fatal(const std::string &msg) {
+    std::cerr << msg << std::endl;
+    std::exit(1);
+  }
+
+ private:
+  FEM &fem_;
+  icx_job *job_ = nullptr;
+  icx_problem prob_{};
+  double design_s0_ = 0.0;
+  std::vector<double> volume_, design_;
+};
+
+// ---------------------------------------------------------------------------------
+class LinearAnalysis {
+ public:
+  LinearAnalysis(FEM &fem, StructuredModel &model, Force &force, Value &value)
+      : fem_(fem), model_(model), force_(force), value_(value) {}
+
+  // LinearAnalysis::solve with the Solvers facade dispatch on the name
+  void solve(const std::string &solvName, bool log) {
+    int form;
+    double tol = tol_;
+    if (solvName == "HIP_PCG" || solvName == "Eigen_CG") form = ICX_SOLVE_PLAIN;
+    else if (solvName == "Eigen_LDLT") { form = ICX_SOLVE_PLAIN; tol = 1e-14; }
+    else if (solvName == "HIP_PCG_FUSED") form = ICX_SOLVE_FUSED;
+    else if (solvName == "HIP_PCG_MIXED") form = ICX_SOLVE_MIXED;
+    else {
+      std::cerr << "Error ! Solver type name is wrong." << std::endl;
+      std::exit(1);
+    }
+    if (!force_.fext_org.empty()) check(model_.ctx(), icx_set_fext(model_.job(), force_.fext_org.data()), "icx_set_fext");
+    force_.fext = force_.fext_org;
+    value_.val.assign(fem_.numeq, 0.0);
+    icx_solve_info info{};
+    icx_status rc = icx_solve(model_.job(), form, tol, maxit_, value_.val.data(), &info);
+    check(model_.ctx(), rc, "icx_solve");
+    info_ = info;
+    if (log) {
+      std::cout << "#iterations:     " << info.iterations << std::endl;
+      std::cout << "estimated error: " << info.relres << std::endl;
+      if (rc == ICX_ERR_NOCONV) std::cerr << "solving failed" << std::endl;
+    }
+  }
+  void setTolerance(double tol) { tol_ = tol; }
+  void setMaxIterations(int it) { maxit_ = it; }
+  const icx_solve_info &info() const { return info_; }
+
+ private:
+  FEM &fem_;
+  StructuredModel &model_;
+  Force &force_;
+  Value &value_;
+  double tol_ = 1.0e-10;   // src/analysis/solver/solvers.hpp:90
+  int maxit_ = 10000;      // src/analysis/solver/solvers.hpp:91
+  icx_solve_info info_{};
+};
+
+// the drivers' sensitivity(): dfds = -(dE/E) u^T Ke u, dgds = v / vscale
+inline void sensitivity(StructuredModel &model, Optimize &optim, double vscale = 1.0) {
+  check(model.ctx(), icx_sensitivity(model.job(), vscale, optim.dfds.data(), optim.dgds.data()), "icx_sensitivity");
+}
+
+// ---------------------------------------------------------------------------------
+inline std::vector<double> threshold_host(StructuredModel &m, const std::vector<double> &v, double beta, double T,
+                                          bool derivative) {
+  std::vector<double> out(v.size());
+  check(m.ctx(), icx_project(m.job(), beta, T, v.data(), derivative ? nullptr : out.data(),
+                             derivative ? out.data() : nullptr), "icx_project");
+  return out;
+}
+
+class Filter {
+ public:
+  Filter(FEM &, StructuredModel &model, double radius) : model_(model) {
+    check(model.ctx(), icx_cone_filter_setup(model.job(), radius), "icx_cone_filter_setup");
+  }
+  std::vector<double> densityFilter(const std::vector<double> &design_s, bool isSens) {
+    std::vector<double> out(design_s.size());
+    check(model_.ctx(), icx_cone_filter(model_.job(), isSens ? 1 : 0, design_s.data(), out.data()), "icx_cone_filter");
+    return out;
+  }
+  std::vector<double> threshold(const std::vector<double> &v, double beta, double T) {
+    return threshold_host(model_, v, beta, T, false);
+  }
+  std::vector<double> thresholdDerivative(const std::vector<double> &v, double beta, double T) {
+    return threshold_host(model_, v, beta, T, true);
+  }
+
+ private:
+  StructuredModel &model_;
+};
+
+class FilterHelmholtz {
+ public:
+  FilterHelmholtz(FEM &, StructuredModel &model, double radius) : model_(model) {
+    check(model.ctx(), icx_helmholtz_setup(model.job(), radius), "icx_helmholtz_setup");
+  }
+  std::vector<double> densityFilter(const std::vector<double> &design_s, const std::string &solvName, int log,
+                                    bool sensFlag) {
+    if (solvName != "Eigen_CG" && solvName != "HIP_PCG") {
+      std::cerr << "Error ! Solver type name is wrong." << std::endl;
+      std::exit(1);
+    }
+    std::vector<double> out(design_s.size());
+    int it = 0;
+    check(model_.ctx(), icx_helmholtz_filter(model_.job(), sensFlag ? 1 : 0, 1.0e-10, design_s.data(), out.data(), &it),
+          "icx_helmholtz_filter");
+    if (log) std::cout << "#iterations:     " << it << std::endl;
+    return out;
+  }
+  std::vector<double> threshold(const std::vector<double> &v, double beta, double T) {
+    return threshold_host(model_, v, beta, T, false);
+  }
+  std::vector<double> thresholdDerivative(const std::vector<double> &v, double beta, double T) {
+    return threshold_host(model_, v, beta, T, true);
+  }
+
+ private:
+  StructuredModel &model_;
+};
+
+// ---------------------------------------------------------------------------------
+class OC {
+ public:
+  explicit OC(int n_) : n(n_) {}
+  // the reference's host bisection with a caller-supplied volume function
+  template <class F>
+  void UpdateVariables(std::vector<double> &xk, std::vector<double> &dfdx, double g, std::vector<double> &dgdx,
+                       std::vector<double> &xmin, std::vector<double> &xmax, F gkp1) {
+    double l0 = lambdamin, l1 = lambdamax, lambda = 0.0;
+    std::vector<double> xkp1(n);
+    int counter = 0;
+    while ((l1 - l0) / (l1 + l0) > lambdaeps && maxiter > counter) {
+      lambda = 0.5 * (l1 + l0);
+      for (int i = 0; i < n; i++) {
+        xkp1[i] = std::pow(std::max(0.0, -dfdx[i] / (dgdx[i] * lambda)), iota) * xk[i];
+        if (xkp1[i] < std::max(xmin[i], (1.0 - movelimit) * xk[i])) xkp1[i] = std::max(xmin[i], (1.0 - movelimit) * xk[i]);
+        else if (xkp1[i] > std::min(xmax[i], (1.0 + movelimit) * xk[i])) xkp1[i] = std::min(xmax[i], (1.0 + movelimit) * xk[i]);
+      }
+      if (gkp1(xkp1) - g > 0.0) l0 = lambda;
+      else l1 = lambda;
+      counter++;
+    }
+    if (counter >= maxiter) std::cerr << "not converge in OCSolver.hpp" << std::endl;
+    xk = xkp1;
+  }
+  // the same bisection with the volume (filter + projection) evaluated on the device
+  int UpdateVariablesDevice(StructuredModel &model, std::vector<double> &xk, std::vector<double> &dfdx, double g,
+                            std::vector<double> &dgdx, double xmin, double xmax, int filter, double beta,
+                            double T) {
+    int bis = 0;
+    check(model.ctx(), icx_oc_update(model.job(), xk.data(), dfdx.data(), g, dgdx.data(), xmin, xmax, movelimit,
+                                     filter, beta, T, &bis), "icx_oc_update");
+    return bis;
+  }
+  void setMoveLimit(double m) { movelimit = m; }
+
+ private:
+  int n;
+  double iota = 0.5, lambdamin = 0.0, lambdamax = 1.0e4, lambdaeps = 1.0e-5, movelimit = 0.15;
+  int maxiter = 100;
+};
+
+class MMASolver {
+ public:
+  // m = 1, a = 0, c = 1e5 (the drivers' use, MMASolver.hpp:67); the optimiser history
+  // lives with the model's job on the device
+  MMASolver(StructuredModel &model, int n, int m = 1, double a = 0.0, double c = 100000.0, double d = 0.0)
+      : model_(model), n_(n) {
+    if (m != 1 || a != 0.0 || c != 100000.0 || d != 0.0)
+      StructuredModel::fatal("MMASolver: the GPU path implements m = 1, a = 0, c = 1e5");
+    icx_mma_reset(model.job());
+  }
+  void Update(double *xval, const double *dfdx, const double *gx, const double *dgdx, const double *xmin,
+              const double *xmax) {
+    for (int i = 1; i < n_; ++i)
+      if (xmin[i] != xmin[0] || xmax[i] != xmax[0]) StructuredModel::fatal("MMASolver: xmin/xmax must be uniform");
+    int steps = 0;
+    check(model_.ctx(), icx_mma_update(model_.job(), xval, dfdx, gx[0], dgdx, xmin[0], xmax[0], &steps),
+          "icx_mma_update");
+    std::cout << "MMA loop: " << steps << std::endl;
+  }
+  void Reset() { icx_mma_reset(model_.job()); }
+
+ private:
+  StructuredModel &model_;
+  int n_;
+};
+
+}  // namespace icarat

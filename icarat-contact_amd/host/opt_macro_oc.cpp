@@ -1,0 +1,83 @@
+// Driver A on the GPU path: cone density filter + projection + linear solve +
+// sensitivity + OC, step for step as opt-macro-oc/opt_macro_oc.cpp:13-180, on a
+// structured [mesh] box (the reference reads an unstructured .mdpa mesh instead).
+//
+// usage: opt_macro_oc <job.toml> [solver=Eigen_CG] [max_steps=1000] [radius=3.0]
+// The OC volume function (filter + threshold of the candidate) runs on the device.
+#include <chrono>
+#include <fstream>
+
+#include "icarat_hip.hpp"
+
+using namespace icarat;
+
+int main(int argc, char *argv[]) {
+  if (argc < 2) {
+    std::cout << "usage: " << argv[0] << " <.toml file> [solver] [max_steps] [radius]\n";
+    return 1;
+  }
+  const std::string solver = argc > 2 ? argv[2] : "Eigen_CG";
+  const int max_steps = argc > 3 ? std::atoi(argv[3]) : 1000;
+  const double filter_radious = argc > 4 ? std::atof(argv[4]) : 3.0;  // opt_macro_oc.cpp:74
+  auto t_start = std::chrono::system_clock::now();
+  int convergence = 1;
+  double V0 = 0.0, T = 0.5;
+  const auto config = toml_lite::parse(argv[1]);
+
+  FEM fem;
+  StructuredModel model(config, fem);
+  model.input_dirich(config);
+  fem.numeq = model.makeDOF();
+  Force force(fem.numeq);
+  Value dis(fem.numeq);
+  model.input_neumann(config, force);
+
+  Optimize optim(fem.nelx, 1, 1.0e-4, 1.0);
+  OC oc(fem.nelx);
+  Filter filter(fem, model, filter_radious);
+  optim.optstep = 0;
+  for (int i = 0; i < fem.nelx; i++) optim.design_s[i] = model.design_s0();
+  LinearAnalysis compt(fem, model, force, dis);
+  std::ofstream csv("function.csv");
+
+  while (convergence == 1) {
+    optim.optstep++;
+    std::cout << "step no. " << optim.optstep << std::endl;
+    auto design_filtered = filter.densityFilter(optim.design_s, false);
+    double beta = std::min(std::pow(2, (int)(optim.optstep / 40)), 16.0);
+    auto design_threshold = filter.threshold(design_filtered, beta, T);
+    auto threDeriv = filter.thresholdDerivative(design_filtered, beta, T);
+    model.setDesignS(design_threshold);
+    compt.solve(solver, 1);
+    double obj = 0.0;
+    for (int i = 0; i < fem.numeq; i++) obj += dis.val[i] * force.fext[i];
+    optim.object_f = obj;
+    std::cout << "function value = " << optim.object_f << std::endl;
+    if (optim.optstep == 1) {
+      V0 = 0.0;
+      for (int i = 0; i < fem.nelx; i++) V0 += design_threshold[i] * model.volume()[i];
+      optim.const_h[0] = V0;
+    } else {
+      optim.const_h[0] = 0.0;
+      for (int i = 0; i < fem.nelx; i++) optim.const_h[0] += design_threshold[i] * model.volume()[i];
+    }
+    csv << optim.optstep << "," << optim.object_f << "," << optim.const_h[0] << "\n";
+    convergence = optim.isConvergence(1.0e-7, 50);
+    sensitivity(model, optim);
+    for (int i = 0; i < fem.nelx; i++) {
+      optim.dfds[i] *= threDeriv[i];
+      optim.dgds[i] *= threDeriv[i];
+    }
+    optim.dfds = filter.densityFilter(optim.dfds, true);
+    optim.dgds = filter.densityFilter(optim.dgds, true);
+    oc.UpdateVariablesDevice(model, optim.design_s, optim.dfds, V0, optim.dgds, optim.xmin[0], optim.xmax[0],
+                             ICX_FILTER_CONE, beta, T);
+    if (optim.optstep == max_steps) {
+      std::cerr << "not convergence in this optimization" << std::endl;
+      break;
+    }
+  }
+  auto t_sec = std::chrono::duration_cast<std::chrono::seconds>(std::chrono::system_clock::now() - t_start).count();
+  std::cout << "Computation time: " << t_sec << " sec \n";
+  return 0;
+}

@@ -6,6 +6,7 @@ there is no CPU fallback.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -101,6 +102,7 @@ def _f64(a):
 class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self.jobs = weakref.WeakSet()
         rc = lib().icx_ctx_create(device, C.byref(self.h))
         if rc != 0:
             raise IcxError(rc, "icx_ctx_create failed (no GPU?)")
@@ -122,7 +124,10 @@ class Context:
         return s.value, n.value
 
     def close(self):
+        """Destroys the context and, with it, every job still open on it."""
         if self.h:
+            for j in list(self.jobs):
+                j.h = C.c_void_p()
             lib().icx_ctx_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -149,6 +154,7 @@ class Job:
         p.poisson, p.penalty = float(nu), float(penalty)
         self.h = C.c_void_p()
         ctx.check(lib().icx_job_create(ctx.h, C.byref(p), C.byref(self.h)))
+        ctx.jobs.add(self)
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         lib().icx_job_sizes(self.h, C.byref(a), C.byref(b), C.byref(c))
         self.numnp, self.nelx, self.neq = a.value, b.value, c.value
@@ -158,9 +164,9 @@ class Job:
         self.numeq = None
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().icx_job_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:

@@ -1,0 +1,66 @@
+"""The design loops on the GPU against the oracle's driver loops
+(opt-macro-oc/opt_macro_oc.cpp:84-172 and opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163).
+Both sides start from the same design and run the same number of design steps; the
+trajectories are compared step by step."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import icx
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_driver_a_oc_cone(ctx):
+    nx, ny, steps = 60, 20, 15
+    job = icx.cantilever_2d(ctx, nx, ny)
+    pr = O.cantilever_2d(nx, ny)
+    x0 = np.full(pr.nelx, 0.5)
+    ref = pr.run_driver(0, x0, steps, 1.5, projection=True, direct=True)
+    job.opt_init(x0, driver=0, filter=icx.FILTER_CONE, radius=1.5, form=icx.SOLVE_PLAIN,
+                 tol=1e-13, maxit=50000)
+    obj = np.array([job.opt_step()["objective"] for _ in range(steps)])
+    rel = np.abs(obj - ref["obj"]) / ref["obj"]
+    assert rel.max() < 1e-8, rel
+    assert np.abs(job.opt_design() - ref["design"]).max() < 1e-7
+
+
+def test_driver_b_mma_helmholtz(ctx):
+    nx, ny, steps = 40, 14, 10
+    job = icx.cantilever_2d(ctx, nx, ny)
+    pr = O.cantilever_2d(nx, ny)
+    x0 = np.full(pr.nelx, 0.4)
+    ref = pr.run_driver(1, x0, steps, 2.0, projection=True, direct=True, helm_direct=True)
+    job.opt_init(x0, driver=1, filter=icx.FILTER_HELMHOLTZ, radius=2.0, form=icx.SOLVE_PLAIN,
+                 tol=1e-13, maxit=50000, helm_tol=1e-13)
+    recs = [job.opt_step() for _ in range(steps)]
+    obj = np.array([r["objective"] for r in recs])
+    rel = np.abs(obj - ref["obj"]) / ref["obj"]
+    assert rel.max() < 1e-7, rel
+    con = np.array([r["constraint"] for r in recs])
+    assert np.abs(con - ref["vol"]).max() < 1e-8
+    assert np.abs(job.opt_design() - ref["design"]).max() < 1e-6
+
+
+def test_cxx_driver_b_executable_matches_oracle():
+    exe = os.path.join(ROOT, "icarat-contact_amd", "_build", "opt_mma_helmholtz")
+    toml = os.path.join(ROOT, "icarat-contact_amd", "host", "examples", "mbb_fdm_q4.toml")
+    steps = 6
+    with tempfile.TemporaryDirectory() as d:
+        out = subprocess.run([exe, toml, "Eigen_LDLT", str(steps)], cwd=d, capture_output=True,
+                             text=True, timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
+        rows = np.loadtxt(os.path.join(d, "function.csv"), delimiter=",", ndmin=2)
+    pr = O.Problem(4, (20, 10, 1), (20.0, 10.0, 1.0), O.PLANE_STRAIN, young=(0.001, 200.0))
+    pr.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, 10, 0, 10])
+    pr.make_dof()
+    pr.add_neumann([0.0, -0.01, 0.0], [20, 20, 0, 10, 0, 0])
+    ref = pr.run_driver(1, np.full(pr.nelx, 0.4), steps, 2.0, projection=True, direct=True,
+                        helm_direct=True)
+    assert rows.shape[0] == steps
+    assert np.abs(rows[:, 1] - ref["obj"]).max() <= 1e-7 * np.abs(ref["obj"]).max()

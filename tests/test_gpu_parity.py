@@ -170,7 +170,7 @@ def test_projection(ctx):
     x = np.linspace(0, 1, job.nelx)
     for beta in (1.0, 4.0, 16.0):
         h, dh = job.project(x, beta)
-        assert np.abs(h - O.threshold(x, beta)).max() <= 2e-16
+        assert np.abs(h - O.threshold(x, beta)).max() <= 1e-15
         assert np.abs(dh - O.threshold_derivative(x, beta)).max() <= 1e-14 * beta
 
 
@@ -228,3 +228,22 @@ def test_mma_updates_match(ctx):
         xo, so = mma.update(xo, df, g, dg, np.full(n, 1e-4), np.ones(n))
         assert np.abs(xg - xo).max() <= 1e-9, it
         xg = xo.copy()   # teacher forcing: continue both from the oracle's iterate
+
+
+@pytest.mark.parametrize("name", ["q4_cantilever_24x8.npz", "hex8_cantilever_10x4x4.npz"])
+def test_gpu_matches_golden(ctx, name):
+    import os
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", name), allow_pickle=False))
+    div = [int(v) for v in g["div"]]
+    job = icx.cantilever_2d(ctx, *div) if len(div) == 2 else icx.cantilever_3d(ctx, *div)
+    job.set_design(g["rho"])
+    u, info = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=20000)
+    assert np.linalg.norm(u - g["u"]) / np.linalg.norm(g["u"]) < 1e-10
+    assert abs(job.compliance() - g["c"]) / g["c"] < 1e-10
+    dc, _ = job.sensitivity()
+    assert np.abs(dc - g["dc"]).max() <= 1e-8 * np.abs(g["dc"]).max()
+    job.cone_setup(1.5)
+    assert np.abs(job.cone(g["rho"]) - g["cone"]).max() < 1e-14
+    job.helmholtz_setup(2.0)
+    h, _ = job.helmholtz(g["rho"], tol=1e-13)
+    assert np.abs(h - g["helm"]).max() < 1e-10
