@@ -28,6 +28,7 @@ struct icx_ctx {
   std::string err;
   SolverState *h_state = nullptr;  // pinned mirror for polling
   double *h_scalar = nullptr;      // pinned scalars
+  int *h_int = nullptr;            // pinned ints (iteration base)
   // profile window (icx_profile.h)
   std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
   int prof_kernel = 0;
@@ -344,7 +345,7 @@ struct PcgWork64 {
   DBuf<double> r, p, t, z, invd;
 };
 struct MixedWork {
-  DBuf<float> r, pa, pb, Ap, xlo, invd, E;
+  DBuf<float> r, pa, pb, Ap, xlo, invd, E, b32;
   DBuf<double> Kx;
 };
 struct FusedWork {
@@ -396,8 +397,8 @@ struct icx_job {
   DBuf<int> d_dof;
   DBuf<uint8_t> d_mask;
   DBuf<double> d_fext, d_dirich, d_b, d_u, d_design, d_E, d_vec, d_vec2;
-  DBuf<double> d_partial;
-  DBuf<unsigned> d_counter;
+  DBuf<double> d_partial;   // generic reductions [2][RED_MAXBLOCKS]
+  DBuf<double> d_pcgpart;   // PCG partials: PA, PB (x2), PN, PZ
   DBuf<SolverState> d_state;
   DBuf<double> d_scalars;
   bool design_set = false;
@@ -410,7 +411,18 @@ struct icx_job {
   MMAWork mma;
   OptWork opt;
 
-  Reduce red() const { return Reduce{d_partial.p, d_counter.p}; }
+  Reduce red() const { return Reduce{d_partial.p}; }
+  PcgPartials parts(int nA, int nB) const {
+    PcgPartials P;
+    double *b = d_pcgpart.p;
+    P.PA = b;
+    P.PB = b + 2 * RED_MAXBLOCKS;
+    P.PN = b + 6 * RED_MAXBLOCKS;
+    P.PZ = b + 8 * RED_MAXBLOCKS;
+    P.nA = nA;
+    P.nB = nB;
+    return P;
+  }
   long long nvec() const { return (long long)nc * g.nn; }
 };
 
@@ -451,50 +463,55 @@ icx_status build_rhs(icx_job *j) {
   icx_ctx *c = j->ctx;
   HIPCHK(c, hipMemcpyAsync(j->d_b.p, j->d_fext.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
   if (j->any_dirich_value) {
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr,
-                                  Reduce{nullptr, nullptr}, S(j));
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr, S(j));
     launch_axpby(N, -1.0, j->d_vec.p, 1.0, j->d_b.p, S(j));
     launch_mask_zero(j->g, j->nc, j->d_mask.p, j->d_b.p, S(j));
   }
   return ICX_OK;
 }
 
-// ---- FP64 plain PCG, Eigen ConjugateGradient loop --------------------------------
-// literal = true: one kernel per Eigen vector expression (the parity form of the
-// structural solve); false: the same loop with the updates fused into two kernels
-// (used for the Helmholtz filter solves).
+// sets the index of the first iteration of the next launch chunk
+icx_status set_iter_base(icx_job *j, int base) {
+  icx_ctx *c = j->ctx;
+  c->h_int[0] = base;
+  HIPCHK(c, hipMemcpyAsync(&j->d_state.p->iter_base, c->h_int, sizeof(int), hipMemcpyHostToDevice, S(j)));
+  return ICX_OK;
+}
+
+inline int poll_every(long long N) { return N > 2000000 ? 16 : (N > 100000 ? 12 : 8); }
+
+// ---- FP64 plain PCG: the Eigen ConjugateGradient loop ------------------------------
+// One kernel per Eigen vector expression (launch_eigen_iteration_tail) after the
+// operator; used for the structural parity form and for the Helmholtz filter solves.
 template <class MatvecFn>
 icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const double *invd,
-                     PcgWork64 &w, double tol, int maxit, MatvecFn matvec, icx_solve_info &info,
-                     bool literal = false) {
+                     PcgWork64 &w, double tol, int maxit, MatvecFn matvec, icx_solve_info &info) {
   icx_ctx *c = j->ctx;
+  const PcgPartials P = j->parts(0, pcg_stream_blocks(N));
   HIPCHK(c, hipMemsetAsync(x, 0, N * sizeof(double), S(j)));
-  launch_pcg_init<double>(N, j->d_state.p, b, w.r.p, w.p.p, invd, j->red(), tol, maxit, 1, S(j));
+  launch_eigen_setup(N, j->d_state.p, b, w.r.p, w.p.p, invd, P, tol, maxit, S(j));
   SolverState st;
-  int polled = 0;
-  const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
-  for (int it = 0; it < maxit + 1; ++it) {
-    matvec(w.p.p, w.t.p);  // t = A p (fused variant also reduces pAp, alpha)
-    if (literal) {
-      launch_eigen_iteration_tail(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, j->red(), S(j));
-    } else {
-      launch_pcg_update<double>(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, j->red(), S(j));
-      launch_pcg_pupdate<double>(N, j->d_state.p, w.p.p, w.z.p, S(j));
-    }
-    if (++polled == every || it == maxit) {
-      polled = 0;
-      icx_status rc = read_state(j, st);
-      if (rc) return rc;
-      if (st.done) break;
-    }
-  }
   icx_status rc = read_state(j, st);
   if (rc) return rc;
-  info.iterations = st.iter;
-  info.matvecs = st.iter + (st.done == 1 && st.rhsNorm2 > 0 && st.iter < maxit ? 1 : 0);
+  const int every = poll_every(N);
+  int base = 0;
+  while (!st.done) {
+    for (int k = 0; k < every; ++k) {
+      matvec(w.p.p, w.t.p);  // tmp = A p
+      launch_eigen_iteration_tail(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, P, k, S(j));
+    }
+    rc = read_state(j, st);
+    if (rc) return rc;
+    base += every;
+    if (!st.done && (rc = set_iter_base(j, base))) return rc;
+  }
+  const bool conv = st.done == 1;
+  info.iterations = conv ? st.stop_iter : st.iter;
+  info.matvecs = (conv && st.rhsNorm2 > 0 && st.rr < st.rhsNorm2) ? st.stop_iter + 1 : st.iter;
+  if (st.rhsNorm2 == 0 || (conv && st.stop_iter == 0 && st.rr == st.rhsNorm2)) info.matvecs = 0;
   info.fp64_matvecs = info.matvecs;
   info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.rr / st.rhsNorm2) : 0.0;
-  return st.done == 1 ? ICX_OK : ICX_ERR_NOCONV;
+  return conv ? ICX_OK : ICX_ERR_NOCONV;
 }
 
 }  // namespace
@@ -514,7 +531,8 @@ icx_status icx_ctx_create(int device, icx_ctx **out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void **)&c->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_scalar, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void **)&c->h_scalar, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_int, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     delete c;
     return ICX_ERR_HIP;
   }
@@ -552,6 +570,7 @@ icx_status icx_ctx_destroy(icx_ctx *ctx) {
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->h_state) (void)hipHostFree(ctx->h_state);
   if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
+  if (ctx->h_int) (void)hipHostFree(ctx->h_int);
   delete ctx;
   return ICX_OK;
 }
@@ -600,14 +619,14 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   j->node_val.assign(g.nn * 6, 0.0);
   (void)hipSetDevice(ctx->device);
   const long long N = j->nvec();
-  if (j->d_partial.alloc(2 * RED_MAXBLOCKS) || j->d_counter.alloc(8) || j->d_state.alloc(1) ||
+  if (j->d_partial.alloc(2 * RED_MAXBLOCKS) || j->d_pcgpart.alloc(10 * RED_MAXBLOCKS) || j->d_state.alloc(1) ||
       j->d_scalars.alloc(64) || j->d_fext.alloc(N) || j->d_b.alloc(N) || j->d_u.alloc(N) ||
       j->d_design.alloc(g.ne) || j->d_E.alloc(g.ne) || j->d_vec.alloc(N) || j->d_vec2.alloc(N) ||
       j->d_mask.alloc(g.nn)) {
     delete j;
     return fail(ctx, ICX_ERR_NOMEM, "device allocation failed");
   }
-  (void)hipMemsetAsync(j->d_counter.p, 0, 8 * sizeof(unsigned), ctx->stream);
+  (void)hipMemsetAsync(j->d_pcgpart.p, 0, 10 * RED_MAXBLOCKS * sizeof(double), ctx->stream);
   (void)hipMemsetAsync(j->d_state.p, 0, sizeof(SolverState), ctx->stream);
   (void)hipMemsetAsync(j->d_fext.p, 0, N * sizeof(double), ctx->stream);
   (void)hipMemsetAsync(j->d_mask.p, 0, g.nn, ctx->stream);
@@ -866,21 +885,14 @@ icx_status icx_matvec(icx_job *j, int prec, const double *x, double *y) {
   HIPCHK(c, hipMemcpyAsync(tmp.p, x, j->numeq * sizeof(double), hipMemcpyHostToDevice, S(j)));
   launch_dof_to_grid(j->g, j->nc, j->d_dof.p, j->numeq, tmp.p, nullptr, j->d_vec.p, S(j));
   if (prec == 0) {
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr,
-                                  Reduce{nullptr, nullptr}, S(j));
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
   } else {
     DBuf<float> xf, yf, Ef;
     HIPCHK(c, xf.alloc(N)); HIPCHK(c, yf.alloc(N)); HIPCHK(c, Ef.alloc(j->g.ne));
     launch_convert_f64_f32(N, j->d_vec.p, xf.p, S(j));
     launch_convert_f64_f32(j->g.ne, j->d_E.p, Ef.p, S(j));
-    launch_elastic_matvec<float>(j->g, j->op32, Ef.p, j->d_mask.p, xf.p, yf.p, nullptr,
-                                 Reduce{nullptr, nullptr}, S(j));
-    // widen
-    std::vector<float> hy(N);
-    HIPCHK(c, hipMemcpyAsync(hy.data(), yf.p, N * sizeof(float), hipMemcpyDeviceToHost, S(j)));
-    HIPCHK(c, hipStreamSynchronize(S(j)));
-    std::vector<double> hd(hy.begin(), hy.end());
-    HIPCHK(c, hipMemcpyAsync(j->d_vec2.p, hd.data(), N * sizeof(double), hipMemcpyHostToDevice, S(j)));
+    launch_elastic_matvec<float>(j->g, j->op32, Ef.p, j->d_mask.p, xf.p, yf.p, nullptr, S(j));
+    launch_convert_f32_f64(N, yf.p, j->d_vec2.p, S(j));
   }
   launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_vec2.p, tmp.p, S(j));
   HIPCHK(c, hipMemcpyAsync(y, tmp.p, j->numeq * sizeof(double), hipMemcpyDeviceToHost, S(j)));
@@ -912,67 +924,63 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
   launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
   auto mv = [&](const double *p, double *t) {
     ProfScope ps(c, ICX_PROF_MATVEC64);
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p,
-                                  Reduce{nullptr, nullptr}, S(j));
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j));
   };
-  return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info, true);
+  return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info);
 }
 
+// Fused forms: kernel A (direction, operator, deferred x update, p.Ap) and kernel B
+// (residual update, |r|^2, r.z) per iteration; the iteration index lives on the device
+// (iter_base + position in the chunk), so the host only launches chunks and polls.
+// FP64 fused form:
 static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info &info) {
   icx_ctx *c = j->ctx;
   const long long N = j->nvec();
   FusedWork &w = j->fw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N));
   HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N));
+  const PcgPartials P = j->parts(elastic_fusedA_blocks<double>(j->g), pcg_stream_blocks(N));
   launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(double), S(j)));
-  launch_pcg_init<double>(N, j->d_state.p, j->d_b.p, w.r.p, nullptr, w.invd.p, j->red(), tol, maxit, 0, S(j));
-  // no replacement triggers in the FP64 fused form
+  HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(double), S(j)));
+  launch_pcg_setup<double>(N, j->d_state.p, j->d_b.p, w.r.p, w.invd.p, P, tol, maxit, 0.0, S(j));
   SolverState st;
-  double *pa = w.pa.p, *pb = w.pb.p;
-  // The A kernel of inner iteration i (1-based) writes the new direction into pb
-  // and the host swaps; kernels that early-exit after the device set `done` still
-  // get swapped here, so the live buffers are re-derived from the device's count
-  // of executed iterations (each executed A is followed by an executed B).
-  auto resync = [&](int iters) {
-    pa = (iters & 1) ? w.pb.p : w.pa.p;
-    pb = (iters & 1) ? w.pa.p : w.pb.p;
-  };
-  const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
-  int k = 0;
-  for (;;) {
-    {
-      ProfScope ps(c, ICX_PROF_FUSED_A);
-      launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, pa, pb, w.Ap.p,
-                                    j->d_u.p, j->d_state.p, j->red(), S(j));
-    }
-    {
-      ProfScope ps(c, ICX_PROF_FUSED_B);
-      launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, j->red(), S(j));
-    }
-    std::swap(pa, pb);
-    if (++k % every == 0) {
-      icx_status rc = read_state(j, st);
-      if (rc) return rc;
-      resync(st.iter);
-      if (st.done) break;
-    }
-  }
-  // x += alpha p for the last step (pa is the latest direction)
-  launch_flush_x<double>(N, j->d_state.p, j->d_u.p, pa, S(j));
   icx_status rc = read_state(j, st);
   if (rc) return rc;
-  info.matvecs = st.iter;
-  info.fp64_matvecs = st.iter;
-  info.iterations = st.done == 3 ? st.iter - 1 : st.iter;
+  const int every = poll_every(N);
+  int base = 0;
+  while (!st.done) {
+    for (int k = 0; k < every; ++k) {
+      {
+        ProfScope ps(c, ICX_PROF_FUSED_A);
+        launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p,
+                                      w.Ap.p, j->d_u.p, j->d_state.p, P, k, S(j));
+      }
+      {
+        ProfScope ps(c, ICX_PROF_FUSED_B);
+        launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j));
+      }
+    }
+    rc = read_state(j, st);
+    if (rc) return rc;
+    base += every;
+    if (!st.done && (rc = set_iter_base(j, base))) return rc;
+  }
+  if (st.done != 1) launch_pcg_flush(N, j->d_state.p, j->d_u.p, w.pa.p, w.pb.p, S(j));
+  rc = read_state(j, st);
+  if (rc) return rc;
+  const bool conv = st.done == 1 || st.done == 3;
+  info.matvecs = st.done == 1 ? 0 : st.stop_iter;
+  info.fp64_matvecs = info.matvecs;
+  info.iterations = st.done == 3 ? st.stop_iter - 1 : (st.done == 1 ? 0 : st.stop_iter);
   info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.rr / st.rhsNorm2) : 0.0;
-  return (st.done == 3 || st.done == 1) ? ICX_OK : ICX_ERR_NOCONV;
+  return conv ? ICX_OK : ICX_ERR_NOCONV;
 }
 
 // FP32 inner PCG on the FP32 operator, FP64 x accumulation and FP64 residual
 // replacement (reliable updates): whenever the FP32 recursive residual has dropped
-// by a factor delta since the last replacement (or meets the tolerance), x_hi += x_lo
+// by a factor 10 since the last replacement (or meets the tolerance), x_hi += x_lo
 // and r = b - K x_hi is recomputed with the FP64 operator; the search direction is
 // kept.  Convergence is declared on the FP64 residual only.
 static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info &info) {
@@ -981,14 +989,17 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
   MixedWork &w = j->mw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N)); HIPCHK(c, w.Ap.alloc(N));
   HIPCHK(c, w.xlo.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.E.alloc(j->g.ne)); HIPCHK(c, w.Kx.alloc(N));
+  HIPCHK(c, w.b32.alloc(N));
+  const PcgPartials P = j->parts(elastic_fusedA_blocks<float>(j->g), pcg_stream_blocks(N));
   launch_convert_f64_f32(j->g.ne, j->d_E.p, w.E.p, S(j));
   launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, nullptr, w.invd.p, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.xlo.p, 0, N * sizeof(float), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(float), S(j)));
-  // r32 = b (rounded); norms from the FP64 b
-  launch_convert_f64_f32(N, j->d_b.p, w.r.p, S(j));
-  launch_pcg_init<float>(N, j->d_state.p, w.r.p, w.r.p, nullptr, w.invd.p, j->red(), tol, maxit, 0, S(j));
+  HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(float), S(j)));
+  // r32 = b; the tolerance is tested on the FP64 |b| (the replacement residual is FP64)
+  launch_convert_f64_f32(N, j->d_b.p, w.b32.p, S(j));
+  launch_pcg_setup<float>(N, j->d_state.p, w.b32.p, w.r.p, w.invd.p, P, tol, maxit, 0.01, S(j));
   launch_dot(N, j->d_b.p, j->d_b.p, j->d_scalars.p, j->red(), S(j));
   double bb;
   icx_status rc = read_scalars(j, 1, &bb);
@@ -998,58 +1009,49 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
   if (rc) return rc;
   st.rhsNorm2 = bb;
   st.threshold = std::max(tol * tol * bb, 2.2250738585072014e-308);
-  st.done = (bb == 0.0 || bb < st.threshold) ? 1 : 0;
-  st.delta2 = 0.01;        // replace when |r| fell by 10x
-  st.replace_every = 0;
-  st.rr_ref = st.rr;
-  HIPCHK(c, hipMemcpy(j->d_state.p, &st, sizeof(SolverState), hipMemcpyHostToDevice));
-  float *pa = w.pa.p, *pb = w.pb.p;
-  // live direction buffers from the device's count of executed iterations
-  // (see solve_fused)
-  auto resync = [&](int iters) {
-    pa = (iters & 1) ? w.pb.p : w.pa.p;
-    pb = (iters & 1) ? w.pa.p : w.pb.p;
-  };
-  const int every = N > 2000000 ? 16 : (N > 100000 ? 12 : 8);
-  int refinements = 0, fp64mv = 0;
-  bool converged = st.done == 1;
-  int k = 0;
-  while (!converged) {
-    {
-      ProfScope ps(c, ICX_PROF_FUSED_A);
-      launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, pa, pb, w.Ap.p, w.xlo.p,
-                                   j->d_state.p, j->red(), S(j));
+  if (bb == 0.0) st.done = 1;
+  HIPCHK(c, hipMemcpyAsync(j->d_state.p, &st, sizeof(SolverState), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  const int every = poll_every(N);
+  int refinements = 0;
+  while (!st.done || st.done >= 2) {
+    if (!st.done) {
+      for (int k = 0; k < every; ++k) {
+        {
+          ProfScope ps(c, ICX_PROF_FUSED_A);
+          launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
+                                       w.xlo.p, j->d_state.p, P, k, S(j));
+        }
+        {
+          ProfScope ps(c, ICX_PROF_FUSED_B);
+          launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j));
+        }
+      }
+      rc = read_state(j, st);
+      if (rc) return rc;
+      if (!st.done) {
+        if ((rc = set_iter_base(j, st.iter_base + every))) return rc;
+        st.iter_base += every;
+        continue;
+      }
     }
-    {
-      ProfScope ps(c, ICX_PROF_FUSED_B);
-      launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, j->red(), S(j));
-    }
-    std::swap(pa, pb);
-    if (++k % every) continue;
-    rc = read_state(j, st);
-    if (rc) return rc;
-    resync(st.iter);
-    if (st.done == 0) continue;
     // replacement: x_hi += x_lo + alpha p; r = b - K x_hi (FP64)
-    launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, pa, S(j));
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr,
-                                  Reduce{nullptr, nullptr}, S(j));
-    launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, j->red(), S(j));
-    ++refinements; ++fp64mv;
     const int was = st.done;
+    launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pb.p, S(j));
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
+    launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, S(j));
+    ++refinements;
     rc = read_state(j, st);
     if (rc) return rc;
-    if (st.done == 1) { converged = true; break; }
-    if (was == 2 || st.iter >= maxit) break;
+    if (st.done == 1) break;
+    if (was == 2 || st.done == 2) { st.done = 2; break; }
   }
-  rc = read_state(j, st);
-  if (rc) return rc;
-  info.iterations = st.iter;
-  info.matvecs = st.iter + fp64mv;
-  info.fp64_matvecs = fp64mv;
+  info.iterations = st.stop_iter;
+  info.matvecs = st.stop_iter + refinements;
+  info.fp64_matvecs = refinements;
   info.refinements = refinements;
   info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.true_rr / st.rhsNorm2) : 0.0;
-  return converged ? ICX_OK : ICX_ERR_NOCONV;
+  return st.done == 1 ? ICX_OK : ICX_ERR_NOCONV;
 }
 
 icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx_solve_info *info) {
@@ -1220,7 +1222,7 @@ icx_status helm_apply_dev(icx_job *j, int sens, double tol, const double *din, d
   launch_helm_rhs(j->g, rhs_scale, din, w.b.p, S(j));
   PcgWork64 pw;  // borrow the Helmholtz buffers
   pw.r.p = w.r.p; pw.p.p = w.p.p; pw.t.p = w.t.p; pw.z.p = w.z.p;
-  auto mv = [&](const double *p, double *t) { launch_helm_matvec(j->g, w.op, p, t, j->d_state.p, j->red(), S(j)); };
+  auto mv = [&](const double *p, double *t) { launch_helm_matvec(j->g, w.op, p, t, j->d_state.p, S(j)); };
   icx_solve_info info{};
   icx_status rc = pcg_plain(j, j->g.nn, w.b.p, w.x.p, w.invd.p, pw, tol, 10000, mv, info);
   pw.r.p = pw.p.p = pw.t.p = pw.z.p = nullptr;  // not owned
@@ -1313,7 +1315,7 @@ icx_status oc_update_dev(icx_job *j, double *dx, const double *dfdx, double g0, 
         if (rc) return rc;
         src = dtmp;
       }
-      launch_volume(ne, project, beta, T, src, nullptr, j->volume, j->d_scalars.p, j->red(), S(j));
+      launch_volume(ne, project, beta, T, src, j->volume, j->d_scalars.p, j->red(), S(j));
     }
     double gk;
     icx_status rc = read_scalars(j, 1, &gk);
@@ -1376,7 +1378,7 @@ icx_status mma_update_dev(icx_job *j, double *dx, const double *dfdx, double gx,
     if (lam < 0.0) lam = 0;
     y = std::max(0.0, lam - cc);
     z = std::max(0.0, 10.0 * (lam * a - 1.0));
-    launch_mma_dual(n, lam, v, dx, 1, j->d_scalars.p, j->red(), S(j));
+    launch_mma_dual(n, lam, v, dx, j->d_scalars.p, j->red(), S(j));
     double o[2];
     icx_status r = read_scalars(j, 2, o);
     S1 = o[0];
@@ -1535,7 +1537,7 @@ icx_status icx_opt_step(icx_job *j, icx_iter_record *rec) {
   r.cg_iterations = info.iterations;
   // objective and volume
   launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
-  launch_volume(ne, 0, 0.0, 0.5, o.thr.p, nullptr, j->volume, j->d_scalars.p + 1, j->red(), S(j));
+  launch_volume(ne, 0, 0.0, 0.5, o.thr.p, j->volume, j->d_scalars.p + 1, j->red(), S(j));
   double sc[2];
   rc = read_scalars(j, 2, sc);
   if (rc) return rc;
@@ -1611,10 +1613,11 @@ icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
   size_t b = 0;
   auto add = [&](size_t v) { b += v; };
   add(j->d_fext.bytes()); add(j->d_b.bytes()); add(j->d_u.bytes()); add(j->d_design.bytes()); add(j->d_E.bytes());
-  add(j->d_vec.bytes()); add(j->d_vec2.bytes()); add(j->d_mask.bytes()); add(j->d_dof.bytes()); add(j->d_partial.bytes());
+  add(j->d_vec.bytes()); add(j->d_vec2.bytes()); add(j->d_mask.bytes()); add(j->d_dof.bytes()); add(j->d_partial.bytes()); add(j->d_pcgpart.bytes());
   for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(d->bytes());
   for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd}) add(d->bytes());
-  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.xlo, &j->mw.invd, &j->mw.E}) add(d->bytes());
+  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.xlo, &j->mw.invd, &j->mw.E, &j->mw.b32})
+    add(d->bytes());
   add(j->mw.Kx.bytes());
   *bytes = b;
   return ICX_OK;

@@ -1,0 +1,243 @@
+// Device helpers shared by the libicx kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "icx_internal.h"
+
+namespace icx {
+
+#define ICX_CHECK_LAUNCH() (void)hipGetLastError()
+
+// ---------------------------------------------------------------------------------
+// reductions
+//
+// Kernels never finish a reduction themselves: every block writes one partial per
+// reduced quantity (plain store) and the kernel that needs the total -- the next
+// one on the stream -- sums the partials in index order.  Kernel boundaries make
+// the partials visible; no device-scope atomics or fences are involved (on an
+// 8-XCD part a per-block fence + atomic ticket costs ~20 ns per block, serialised),
+// and every block of the consumer computes the same total bit for bit.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum of one value per thread over the block; the total is returned to every
+// thread.  scratch: LDS of >= BS/64 doubles.  Contains two barriers.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double *scratch) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) t += scratch[w];
+  __syncthreads();
+  return t;
+}
+
+// Total of p[0], p[s], ..., p[(n-1)s] in a fixed order, returned to every thread.
+template <int BS>
+__device__ __forceinline__ double block_total(const double *p, int n, int s, double *scratch) {
+  double a = 0.0;
+  for (int q = threadIdx.x; q < n; q += BS) a += p[(long long)q * s];
+  return block_sum<BS>(a, scratch);
+}
+
+// block-count policy of the streaming kernels (each partial array holds <= RED_MAXBLOCKS)
+static inline int ew_blocks(long long n, int per_thread = 1, int cap = 2048) {
+  long long b = (n + 256LL * per_thread - 1) / (256LL * per_thread);
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+constexpr int RED_BLOCKS = 512;   // grid of the reducing streaming kernels
+
+// ---------------------------------------------------------------------------------
+// element operators in the per-element Haar basis
+// ---------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void haar_fwd8(T (&v)[8]) {
+#pragma unroll
+  for (int q = 0; q < 8; q += 2) { T s = v[q] + v[q + 1], d = v[q + 1] - v[q]; v[q] = s; v[q + 1] = d; }
+#pragma unroll
+  for (int q : {0, 1, 4, 5}) { T s = v[q] + v[q + 2], d = v[q + 2] - v[q]; v[q] = s; v[q + 2] = d; }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { T s = v[q] + v[q + 4], d = v[q + 4] - v[q]; v[q] = s; v[q + 4] = d; }
+}
+template <typename T>
+__device__ __forceinline__ void haar_inv8(T (&v)[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) { T a = v[q], b = v[q + 4]; v[q] = a - b; v[q + 4] = a + b; }
+#pragma unroll
+  for (int q : {0, 1, 4, 5}) { T a = v[q], b = v[q + 2]; v[q] = a - b; v[q + 2] = a + b; }
+#pragma unroll
+  for (int q = 0; q < 8; q += 2) { T a = v[q], b = v[q + 1]; v[q] = a - b; v[q + 1] = a + b; }
+}
+template <typename T>
+__device__ __forceinline__ void haar_fwd4(T (&v)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; q += 2) { T s = v[q] + v[q + 1], d = v[q + 1] - v[q]; v[q] = s; v[q + 1] = d; }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) { T s = v[q] + v[q + 2], d = v[q + 2] - v[q]; v[q] = s; v[q + 2] = d; }
+}
+template <typename T>
+__device__ __forceinline__ void haar_inv4(T (&v)[4]) {
+#pragma unroll
+  for (int q = 0; q < 2; ++q) { T a = v[q], b = v[q + 2]; v[q] = a - b; v[q + 2] = a + b; }
+#pragma unroll
+  for (int q = 0; q < 4; q += 2) { T a = v[q], b = v[q + 1]; v[q] = a - b; v[q + 1] = a + b; }
+}
+
+// Structural non-zeros of the character blocks.  Character chi has bit d set
+// when the mode is odd under the mirror of axis d.  Member c of block chi is the
+// Haar coefficient of displacement component c with difference pattern
+// chi ^ (1 << c).  Blocks with one odd axis d hold the rigid translation along
+// d (row/column c = d vanishes); blocks with two odd axes hold a rigid rotation
+// coupling those two components, and the remaining component is uncoupled.
+__host__ __device__ constexpr int popc3(int v) { return (v & 1) + ((v >> 1) & 1) + ((v >> 2) & 1); }
+__host__ __device__ constexpr int first_odd(int chi) { return (chi & 1) ? 0 : ((chi & 2) ? 1 : 2); }
+__host__ __device__ constexpr int first_even(int chi) { return !(chi & 1) ? 0 : (!(chi & 2) ? 1 : 2); }
+__host__ __device__ constexpr bool blk_nz3(int chi, int c, int cc) {
+  return (popc3(chi) == 0 || popc3(chi) == 3)
+             ? true
+             : (popc3(chi) == 1 ? (c != first_odd(chi) && cc != first_odd(chi))
+                                : ((c == first_even(chi)) == (cc == first_even(chi))));
+}
+__host__ __device__ constexpr bool blk_nz2(int chi, int c, int cc) {
+  return (chi == 0 || chi == 3) ? true : (c != ((chi & 1) ? 0 : 1) && cc != ((chi & 1) ? 0 : 1));
+}
+
+// v <- E * K0 v for one hexa8 element (v[c][q], corner q = a + 2b + 4c)
+template <typename T>
+__device__ __forceinline__ void elastic_hex8(T (&v)[3][8], const ElasticOp<T> &op, T E) {
+  haar_fwd8(v[0]); haar_fwd8(v[1]); haar_fwd8(v[2]);
+  T o[3][8];
+#pragma unroll
+  for (int chi = 0; chi < 8; ++chi) {
+    const T m[3] = {v[0][chi ^ 1], v[1][chi ^ 2], v[2][chi ^ 4]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T acc = T(0);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+        if (blk_nz3(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+      o[c][chi ^ (1 << c)] = E * acc;
+    }
+  }
+  haar_inv8(o[0]); haar_inv8(o[1]); haar_inv8(o[2]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[c][q] = o[c][q];
+}
+
+template <typename T>
+__device__ __forceinline__ void elastic_quad4(T (&v)[2][4], const ElasticOp<T> &op, T E) {
+  haar_fwd4(v[0]); haar_fwd4(v[1]);
+  T o[2][4];
+#pragma unroll
+  for (int chi = 0; chi < 4; ++chi) {
+    const T m[2] = {v[0][chi ^ 1], v[1][chi ^ 2]};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      T acc = T(0);
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc)
+        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+      o[c][chi ^ (1 << c)] = E * acc;
+    }
+  }
+  haar_inv4(o[0]); haar_inv4(o[1]);
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[c][q] = o[c][q];
+}
+
+// u^T K0 u in the Haar basis: sum over characters of m^T B m
+template <typename T>
+__device__ __forceinline__ T energy_hex8(T (&v)[3][8], const ElasticOp<T> &op) {
+  haar_fwd8(v[0]); haar_fwd8(v[1]); haar_fwd8(v[2]);
+  T e = T(0);
+#pragma unroll
+  for (int chi = 0; chi < 8; ++chi) {
+    const T m[3] = {v[0][chi ^ 1], v[1][chi ^ 2], v[2][chi ^ 4]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      T acc = T(0);
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc)
+        if (blk_nz3(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+      e += m[c] * acc;
+    }
+  }
+  return e;
+}
+template <typename T>
+__device__ __forceinline__ T energy_quad4(T (&v)[2][4], const ElasticOp<T> &op) {
+  haar_fwd4(v[0]); haar_fwd4(v[1]);
+  T e = T(0);
+#pragma unroll
+  for (int chi = 0; chi < 4; ++chi) {
+    const T m[2] = {v[0][chi ^ 1], v[1][chi ^ 2]};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      T acc = T(0);
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc)
+        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+      e += m[c] * acc;
+    }
+  }
+  return e;
+}
+
+// ---------------------------------------------------------------------------------
+// XCD-aware tile order: consecutive hardware block ids go round-robin over the 8
+// XCDs; remap so each XCD works on a contiguous run of tiles (shared halo lines
+// stay in that XCD's L2).
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+  const int per = nblocks >> 3, rem = nblocks & 7;
+  const int xcd = b & 7, local = b >> 3;
+  return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+// ---------------------------------------------------------------------------------
+// scalars of the fused / mixed PCG iteration i, computed identically by every block
+// from the partials of the previous kernels (see PcgPartials in icx_internal.h)
+// ---------------------------------------------------------------------------------
+struct IterScalars {
+  double rr, rz, rz_prev, pAp_prev, beta, alpha_prev;
+  int stop;  // 0 continue, 2 maxit, 3 tolerance met by the recursive residual, 4 replace
+};
+
+template <int BS>
+__device__ __forceinline__ IterScalars iter_scalars(const SolverState *st, const PcgPartials &P, int i,
+                                                    double *scratch) {
+  IterScalars s;
+  const double *pb_prev = P.PB + (size_t)((i - 1) & 1) * 2 * RED_MAXBLOCKS;
+  const double *pb_prev2 = P.PB + (size_t)(i & 1) * 2 * RED_MAXBLOCKS;
+  s.rr = block_total<BS>(pb_prev, P.nB, 2, scratch);
+  s.rz = block_total<BS>(pb_prev + 1, P.nB, 2, scratch);
+  if (i > 0) {
+    s.rz_prev = block_total<BS>(pb_prev2 + 1, P.nB, 2, scratch);
+    s.pAp_prev = block_total<BS>(P.PA + (size_t)((i - 1) & 1) * RED_MAXBLOCKS, P.nA, 1, scratch);
+    s.beta = s.rz / s.rz_prev;
+    s.alpha_prev = s.rz_prev / s.pAp_prev;
+  } else {
+    s.rz_prev = s.pAp_prev = 1.0;
+    s.beta = 0.0;
+    s.alpha_prev = 0.0;
+  }
+  s.stop = 0;
+  if (s.rr < st->threshold) s.stop = 3;
+  else if (st->delta2 > 0.0 && s.rr < st->delta2 * st->rr_ref) s.stop = 4;
+  else if (i >= st->maxit) s.stop = 2;
+  return s;
+}
+
+}  // namespace icx

@@ -32,99 +32,107 @@ template <typename T> struct HelmOp {
   T kdiag;             // diagonal entry of the element matrix
 };
 
-// Scalars of one PCG solve; written only by the last block of a kernel.
+enum { RED_MAXBLOCKS = 4096 };
+
+// State of one PCG solve.  Written by the host between launches, or by block 0 of a
+// kernel for fields no other block of that kernel reads.
 struct SolverState {
   double rhsNorm2;
   double threshold;    // max(tol^2 |b|^2, DBL_MIN)
-  double absNew;       // r.z
-  double absOld;
-  double pAp;
-  double alpha;        // step of the last matvec kernel
-  double alpha_x;      // step still owed to x (mixed form, deferred update)
-  double beta;
-  double rr;           // |r|^2, recursive
-  double rz_new;
-  double true_rr;      // |b - K x|^2 in FP64 (mixed form)
-  double rr_ref;       // residual norm^2 at the last replacement
-  double delta2;       // replacement trigger (rr < delta2 * rr_ref)
-  int done;            // 0 running, 1 converged, 2 maxit, 3 recursive converged, 4 replace
-  int iter;            // Eigen iteration counter
+  double rr_ref;       // |r|^2 at the last residual replacement (mixed form)
+  double delta2;       // replacement trigger: rr < delta2 * rr_ref (0 = off)
+  double rr;           // |r|^2 at the stop
+  double true_rr;      // |b - K x|^2 in FP64 at the last replacement
+  double alpha_owed;   // step still owed to x at the stop (fused/mixed forms)
+  int done;            // 0 running, 1 converged, 2 maxit, 3 recursive residual met, 4 replace
   int maxit;
-  int inner_since;     // inner iterations since the last replacement
-  int replace_every;   // forced replacement period (0 = off)
-  int pad;
-  unsigned counters[8];
+  int iter_base;       // index of the first iteration of the next launch chunk
+  int iter;            // progress: iterations completed
+  int stop_iter;       // iteration index at which `done` was set
+  int flush_iter;      // iteration whose owed x update was already applied
 };
 
-enum { RED_MAXBLOCKS = 4096 };
+// Per-block partial sums of a PCG solve, double-buffered by iteration parity.
+struct PcgPartials {
+  double *PA;          // [2][RED_MAXBLOCKS]     p.Ap of kernel A / p.tmp of the plain loop
+  double *PB;          // [2][RED_MAXBLOCKS][2]  (|r|^2, r.z) of kernel B / set-up / replacement
+  double *PN;          // [2][RED_MAXBLOCKS]     plain loop |r|^2
+  double *PZ;          // [2][RED_MAXBLOCKS]     plain loop r.z
+  int nA, nB;          // partial counts written by the A-type and B-type kernels
+};
 
+// generic (non-iteration) reductions: partials + one-block finalisation
 struct Reduce {
-  double *partial;     // [2 * RED_MAXBLOCKS]
-  unsigned *counter;
+  double *partial;     // [NV][RED_MAXBLOCKS]
 };
 
-}  // namespace icx
-
-// ---- kernel launchers (icx_kernels.hip) ------------------------------------------
-namespace icx {
-
-// y = K(E) x on the node grid (SoA, nc components).  mask: per-node bit c set =
-// constrained (output forced to zero); may be null.  If red != null the dot
-// x.y is reduced and, in the last block, st->pAp and st->alpha = absNew/pAp are
-// written.
+// ---- structured operators (icx_operator.hip) -------------------------------------
+// y = K(E) x on the node grid (SoA).  mask: per-node bit c = constrained (output
+// forced to zero); may be null.  st may be null; otherwise the kernel exits at once
+// when st->done is set.
 template <typename T>
 void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *x, T *y, SolverState *st, Reduce red, hipStream_t s);
+                           const T *x, T *y, const SolverState *st, hipStream_t s);
 
-// Mixed/fused inner kernel A: p = invd*r + beta*p_old (beta from st), y = K p,
-// x += alpha_x * p_old, dot p.y reduced into st->pAp / st->alpha.
+// Fused kernel A of iteration i = st->iter_base + k: scalars from the partials,
+// p_i = invd r + beta_i p_{i-1} (P[i&1] <- ..., P[(i-1)&1] read), Ap_i = K p_i,
+// x += alpha_{i-1} p_{i-1}, partial p_i.Ap_i -> PA[i&1].
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *r, const T *invd, const T *pold, T *pnew, T *y, T *xacc,
-                           SolverState *st, Reduce red, hipStream_t s);
+                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
+                           PcgPartials P, int k, hipStream_t s);
+// number of blocks of kernel A for this grid (its partial count)
+template <typename T>
+int elastic_fusedA_blocks(const Grid &g);
 
 void launch_helm_matvec(const Grid &g, const HelmOp<double> &op, const double *x, double *y,
-                        SolverState *st, Reduce red, hipStream_t s);
+                        const SolverState *st, hipStream_t s);
 
-// the Eigen loop after tmp = A p: p.tmp, x += a p, r -= a tmp, |r|^2, z = invd r,
-// r.z, p = z + beta p -- one kernel each, in Eigen's order
-void launch_eigen_iteration_tail(long long n, SolverState *st, double *x, double *r, double *p,
-                                 const double *t, double *z, const double *invd, Reduce red,
-                                 hipStream_t s);
-// plain (Eigen-structured) vector kernels
+// ---- PCG vector kernels (icx_pcg.hip) -----------------------------------------------
+// number of blocks of the B-type streaming kernels for n entries (their partial count)
+int pcg_stream_blocks(long long n);
+// set-up: r = b; partials (b.b, r.z) -> PB[1] (the slot of "iteration -1"); then a
+// one-block kernel fills rhsNorm2, threshold, done
 template <typename T>
-void launch_pcg_update(long long n, SolverState *st, T *x, T *r, const T *p, const T *t, T *z,
-                       const T *invd, Reduce red, hipStream_t s);
+void launch_pcg_setup(long long n, SolverState *st, const T *b, T *r, const T *invd, PcgPartials P,
+                      double tol, int maxit, double delta2, hipStream_t s);
+// fused kernel B of iteration i: alpha_i = rz_i / pAp_i; r -= alpha_i Ap; (|r|^2, r.z) -> PB[i&1]
 template <typename T>
-void launch_pcg_pupdate(long long n, SolverState *st, T *p, const T *z, hipStream_t s);
-// fused/mixed kernel B: r -= alpha*Ap; rr, rz reduced; beta, absNew, iteration control.
-template <typename T>
-void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, Reduce red,
-                       hipStream_t s);
-// mixed replacement: xhi += xlo + alpha_x*p; xlo = 0
-void launch_refine_x(long long n, SolverState *st, double *xhi, float *xlo, const float *p,
-                     hipStream_t s);
-// mixed replacement: rhi = b - Kxhi (given), r32 = rhi, true rr and rz reduced
+void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
+                       int k, hipStream_t s);
+// fused FP64 form at the stop: x += alpha_owed p_{stop-1}
+void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0, const double *P1,
+                      hipStream_t s);
+// mixed form replacement: xhi += xlo + alpha_owed p_{stop-1}; xlo = 0
+void launch_refine_x(long long n, SolverState *st, double *xhi, float *xlo, const float *P0,
+                     const float *P1, hipStream_t s);
+// mixed form replacement: r32 = b - Kx (FP64), (|r|^2, r.z) -> PB[(stop-1)&1], then a
+// one-block kernel decides converged / continue and resets the recurrence
 void launch_refine_r(long long n, SolverState *st, const double *b, const double *Kx, float *r32,
-                     const float *invd32, Reduce red, hipStream_t s);
-// fused FP64 replacement-free variant keeps x in FP64: final flush x += alpha_x p
-template <typename T>
-void launch_flush_x(long long n, SolverState *st, T *x, const T *p, hipStream_t s);
+                     const float *invd32, PcgPartials P, hipStream_t s);
 
-// PCG set-up: r = b (copy), rr = b.b, threshold; z = invd*r; p = z; absNew = r.z
-template <typename T>
-void launch_pcg_init(long long n, SolverState *st, const T *b, T *r, T *p, const T *invd,
-                     Reduce red, double tol, int maxit, int set_p, hipStream_t s);
+// the Eigen ConjugateGradient loop after tmp = A p, one kernel per vector expression
+// (p.tmp, x += a p, r -= a tmp, |r|^2, z = M^-1 r, r.z, p = z + beta p)
+void launch_eigen_iteration_tail(long long n, SolverState *st, double *x, double *r, double *p,
+                                 const double *t, double *z, const double *invd, PcgPartials P,
+                                 int k, hipStream_t s);
+// set-up of the Eigen loop: r = b, p = z = invd r; partials (b.b) -> PN[1], (r.z) -> PZ[1]
+void launch_eigen_setup(long long n, SolverState *st, const double *b, double *r, double *p,
+                        const double *invd, PcgPartials P, double tol, int maxit, hipStream_t s);
 
-// element modulus E = SIMP(design) in FP64 and FP32
+// ---- generic reductions ---------------------------------------------------------------
+// out[k] = sum of partial[k][0..nblocks) (one block, fixed order)
+void launch_finalize(const Reduce &red, int nblocks, int nv, double *out, hipStream_t s);
+// dot product a.b -> out[0]
+void launch_dot(long long n, const double *a, const double *b, double *out, Reduce red, hipStream_t s);
+
+// ---- element moduli, diagonals, layout (icx_misc.hip) -------------------------------
 void launch_simp(long long ne, double y1, double y2, double p, const double *design, double *E64,
                  float *E32, hipStream_t s);
 // Jacobi inverse diagonal from E: invd[c][n] = 1/(kd_c * sum_{e ni n} E_e) (0 at constrained)
 void launch_elastic_invdiag(const Grid &g, const double *kd, const double *E, const uint8_t *mask,
                             double *invd64, float *invd32, hipStream_t s);
 void launch_helm_invdiag(const Grid &g, double kdiag, double *invd, hipStream_t s);
-
-// layout conversions
 void launch_dof_to_grid(const Grid &g, int nc, const int *dof, int numeq, const double *v,
                         const double *dirich /* nullable: per node*6 */, double *grid,
                         hipStream_t s);
@@ -133,19 +141,16 @@ void launch_grid_to_dof(const Grid &g, int nc, const int *dof, int numeq, const 
 void launch_node_values(const Grid &g, int nc, const int *dof, int numeq, const double *grid,
                         const double *dirich, double *nodeval, hipStream_t s);
 void launch_convert_f64_f32(long long n, const double *a, float *b, hipStream_t s);
+void launch_convert_f32_f64(long long n, const float *a, double *b, hipStream_t s);
 void launch_axpby(long long n, double a, const double *x, double b, double *y, hipStream_t s);
 void launch_mul(long long n, const double *a, double *b, hipStream_t s);  // b *= a
 void launch_mask_zero(const Grid &g, int nc, const uint8_t *mask, double *v, hipStream_t s);
-
-// reductions producing a single double in out[0] (deterministic)
-void launch_dot(long long n, const double *a, const double *b, double *out, Reduce red,
-                hipStream_t s);
 
 // sensitivity: dfds = -(dE/E) u_e^T K(E) u_e = -dE * u_e^T K0 u_e; dgds = vol/vscale
 void launch_sensitivity(const Grid &g, const ElasticOp<double> &op, double y1, double y2, double p,
                         const double *design, const double *u, double vol, double vscale,
                         double *dfds, double *dgds, hipStream_t s);
-// element energy sum_e E_e u_e^T K0 u_e (for the compliance identity)
+// element energy E_e u_e^T K0 u_e (for the compliance identity)
 void launch_energy(const Grid &g, const ElasticOp<double> &op, const double *E, const double *u,
                    double *energy, hipStream_t s);
 
@@ -172,8 +177,9 @@ void launch_helm_to_element(const Grid &g, double scale, const double *phi, doub
 void launch_oc_candidate(long long n, double lambda, double move, double xmin, double xmax,
                          const double *x, const double *dfdx, const double *dgdx, double *xnew,
                          hipStream_t s);
+// sum_i proj(rho_i) * vol -> out[0]
 void launch_volume(long long n, int project, double beta, double T, const double *rho,
-                   const double *vol_or_null, double vol, double *out, Reduce red, hipStream_t s);
+                   double vol, double *out, Reduce red, hipStream_t s);
 void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double lambda, double move,
                            double xmin, double xmax, const double *x, const double *dfdx,
                            const double *dgdx, int project, double beta, double T, double vol,
@@ -188,7 +194,7 @@ void launch_mma_gensub(long long n, int iter, const MMAVecs &m, const double *xv
                        double move, double albefa, double asyminit, double asymdec, double asyminc,
                        double raa0, double xmamieps, double *bsum, Reduce red, hipStream_t s);
 // x(lambda) and sums: out[0] = sum p1/(upp-x) + q1/(x-low), out[1] = sum PQ^2 df2
-void launch_mma_dual(long long n, double lam, const MMAVecs &m, double *x, int want_hess,
-                     double *out, Reduce red, hipStream_t s);
+void launch_mma_dual(long long n, double lam, const MMAVecs &m, double *x, double *out, Reduce red,
+                     hipStream_t s);
 
 }  // namespace icx

@@ -1,0 +1,547 @@
+// Element-level and streaming kernels of libicx (gfx950): SIMP moduli, Jacobi
+// diagonals, layout conversions, sensitivity, cone filter, projection, Helmholtz
+// right-hand side, OC and MMA pieces.
+#include "icx_common.cuh"
+
+#include <math.h>
+
+namespace icx {
+
+// ---------------------------------------------------------------------------------
+// element moduli, diagonals, layout conversions
+// ---------------------------------------------------------------------------------
+// SIMP, src/problem/elasticity.hpp:15-22
+__device__ __forceinline__ double simp(double y1, double y2, double s, double p) {
+  if (y2 - y1 >= 0.0) return (1.0 - pow(s, p)) * y1 + pow(s, p) * y2;
+  return pow(1.0 - s, p) * y1 + (1.0 - pow(1.0 - s, p)) * y2;
+}
+
+__global__ void k_simp(long long ne, double y1, double y2, double p, const double *design, double *E64,
+                       float *E32) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const double E = simp(y1, y2, design[e], p);
+    E64[e] = E;
+    if (E32) E32[e] = (float)E;
+  }
+}
+void launch_simp(long long ne, double y1, double y2, double p, const double *design, double *E64,
+                 float *E32, hipStream_t s) {
+  hipLaunchKernelGGL(k_simp, dim3(ew_blocks(ne)), dim3(256), 0, s, ne, y1, y2, p, design, E64, E32);
+  ICX_CHECK_LAUNCH();
+}
+
+struct KD { double v[3]; };
+__global__ void k_elastic_invdiag(Grid g, KD kd, const double *E, const uint8_t *mask, double *invd64,
+                                  float *invd32) {
+  const int nc = g.ndim;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)((n / g.sx) % g.sy), k = (int)(n / ((long long)g.sx * g.sy));
+    double S = 0.0;
+    for (int dz = (g.ndim == 3 ? -1 : 0); dz <= 0; ++dz)
+      for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+          const int ex = i + dx, ey = j + dy, ez = k + dz;
+          if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+          S += E[ex + (long long)g.nx * (ey + (long long)g.ny * ez)];
+        }
+    const uint8_t m = mask ? mask[n] : 0;
+    for (int c = 0; c < nc; ++c) {
+      const double d = kd.v[c] * S;
+      double inv = (d != 0.0) ? 1.0 / d : 1.0;
+      if ((m >> c) & 1) inv = 0.0;
+      if (invd64) invd64[c * g.nn + n] = inv;
+      if (invd32) invd32[c * g.nn + n] = (float)inv;
+    }
+  }
+}
+void launch_elastic_invdiag(const Grid &g, const double *kd, const double *E, const uint8_t *mask,
+                            double *invd64, float *invd32, hipStream_t s) {
+  KD k{{kd[0], kd[1], kd[2]}};
+  hipLaunchKernelGGL(k_elastic_invdiag, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, k, E, mask, invd64,
+                     invd32);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_helm_invdiag(Grid g, double kdiag, double *invd) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)((n / g.sx) % g.sy), k = (int)(n / ((long long)g.sx * g.sy));
+    int cnt = 0;
+    for (int dz = (g.ndim == 3 ? -1 : 0); dz <= 0; ++dz)
+      for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+          const int ex = i + dx, ey = j + dy, ez = k + dz;
+          cnt += !(ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz);
+        }
+    const double d = kdiag * cnt;
+    invd[n] = d != 0.0 ? 1.0 / d : 1.0;
+  }
+}
+void launch_helm_invdiag(const Grid &g, double kdiag, double *invd, hipStream_t s) {
+  hipLaunchKernelGGL(k_helm_invdiag, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, kdiag, invd);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_dof_to_grid(Grid g, int nc, const int *dof, int numeq, const double *v,
+                              const double *dirich, double *grid) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x)
+    for (int c = 0; c < nc; ++c) {
+      const int d = dof[n * nc + c];
+      grid[c * g.nn + n] = d < numeq ? v[d] : (dirich ? dirich[6 * n + c] : 0.0);
+    }
+}
+void launch_dof_to_grid(const Grid &g, int nc, const int *dof, int numeq, const double *v,
+                        const double *dirich, double *grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_dof_to_grid, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, nc, dof, numeq, v, dirich,
+                     grid);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_grid_to_dof(Grid g, int nc, const int *dof, int numeq, const double *grid, double *v) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x)
+    for (int c = 0; c < nc; ++c) {
+      const int d = dof[n * nc + c];
+      if (d < numeq) v[d] = grid[c * g.nn + n];
+    }
+}
+void launch_grid_to_dof(const Grid &g, int nc, const int *dof, int numeq, const double *grid,
+                        double *v, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_to_dof, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, nc, dof, numeq, grid, v);
+  ICX_CHECK_LAUNCH();
+}
+
+// divideValIntoNode, src/analysis/analysis_utility.hpp:8-31 (|v| < 1e-15 -> 0)
+__global__ void k_node_values(Grid g, int nc, const int *dof, int numeq, const double *grid,
+                              const double *dirich, double *nodeval) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x)
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      if (c < nc) {
+        const int d = dof[n * nc + c];
+        v = d < numeq ? grid[c * g.nn + n] : (dirich ? dirich[6 * n + c] : 0.0);
+        if (fabs(v) < 1.0e-15) v = 0.0;
+      }
+      nodeval[3 * n + c] = v;
+    }
+}
+void launch_node_values(const Grid &g, int nc, const int *dof, int numeq, const double *grid,
+                        const double *dirich, double *nodeval, hipStream_t s) {
+  hipLaunchKernelGGL(k_node_values, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, nc, dof, numeq, grid,
+                     dirich, nodeval);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_f64_f32(long long n, const double *a, float *b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+void launch_convert_f64_f32(long long n, const double *a, float *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_f64_f32, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, a, b);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_f32_f64(long long n, const float *a, double *b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+void launch_convert_f32_f64(long long n, const float *a, double *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_f32_f64, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, a, b);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_axpby(long long n, double a, const double *x, double b, double *y) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+void launch_axpby(long long n, double a, const double *x, double b, double *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, a, x, b, y);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_mul(long long n, const double *a, double *b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] *= a[i];
+}
+void launch_mul(long long n, const double *a, double *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_mul, dim3(ew_blocks(n)), dim3(256), 0, s, n, a, b);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_mask_zero(Grid g, int nc, const uint8_t *mask, double *v) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const uint8_t m = mask[n];
+    for (int c = 0; c < nc; ++c)
+      if ((m >> c) & 1) v[c * g.nn + n] = 0.0;
+  }
+}
+void launch_mask_zero(const Grid &g, int nc, const uint8_t *mask, double *v, hipStream_t s) {
+  hipLaunchKernelGGL(k_mask_zero, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, nc, mask, v);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------------
+// sensitivity (opt_macro_oc.cpp:182-212, opt_mma_Helmholtz.cpp:173-205)
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double flush15(double v) { return fabs(v) < 1.0e-15 ? 0.0 : v; }
+
+template <bool SENS>
+__global__ void k_element_energy(Grid g, ElasticOp<double> op, double y1, double y2, double p,
+                                 const double *design, const double *E, const double *u, double vol,
+                                 double vscale, double *dfds, double *dgds) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int ex = (int)(e % g.nx), ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
+    double en;
+    if (g.ndim == 3) {
+      double v[3][8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const long long m = (ex + (q & 1)) + (long long)g.sx * ((ey + ((q >> 1) & 1)) + (long long)g.sy * (ez + (q >> 2)));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c][q] = flush15(u[c * g.nn + m]);
+      }
+      en = energy_hex8<double>(v, op);
+    } else {
+      double v[2][4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long long m = (ex + (q & 1)) + (long long)g.sx * (ey + (q >> 1));
+#pragma unroll
+        for (int c = 0; c < 2; ++c) v[c][q] = flush15(u[c * g.nn + m]);
+      }
+      en = energy_quad4<double>(v, op);
+    }
+    if (SENS) {
+      const double dens = design[e];
+      double dyoung;
+      if (y2 >= y1) dyoung = p * pow(dens, p - 1.0) * (y2 - y1);
+      else dyoung = (y2 - y1) * p * pow(1.0 - dens, p - 1.0);
+      if (dfds) dfds[e] = -dyoung * en;
+      if (dgds) dgds[e] = vol / vscale;
+    } else {
+      dfds[e] = E[e] * en;
+    }
+  }
+}
+void launch_sensitivity(const Grid &g, const ElasticOp<double> &op, double y1, double y2, double p,
+                        const double *design, const double *u, double vol, double vscale,
+                        double *dfds, double *dgds, hipStream_t s) {
+  hipLaunchKernelGGL(k_element_energy<true>, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, op, y1, y2, p,
+                     design, nullptr, u, vol, vscale, dfds, dgds);
+  ICX_CHECK_LAUNCH();
+}
+void launch_energy(const Grid &g, const ElasticOp<double> &op, const double *E, const double *u,
+                   double *energy, hipStream_t s) {
+  hipLaunchKernelGGL(k_element_energy<false>, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, op, 0.0, 0.0,
+                     0.0, nullptr, E, u, 0.0, 1.0, energy, nullptr);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------------
+// cone filter (src/optimization/filter.hpp:102-151); stencil offsets are sorted in
+// increasing neighbour id, the order of the reference's neighbour lists.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ bool nb_elem(const Grid &g, long long e, const int *off, long long &j) {
+  const int ex = (int)(e % g.nx), ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
+  const int X = ex + off[0], Y = ey + off[1], Z = ez + off[2];
+  if (X < 0 || Y < 0 || Z < 0 || X >= g.nx || Y >= g.ny || Z >= g.nz) return false;
+  j = X + (long long)g.nx * (Y + (long long)g.ny * Z);
+  return true;
+}
+
+__global__ void k_cone_aa(Grid g, Stencil st, double *aa) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < st.n; ++q) {
+      long long j;
+      if (nb_elem(g, e, st.off + 3 * q, j)) s += st.w[q];
+    }
+    aa[e] = s;
+  }
+}
+__global__ void k_cone_fwd(Grid g, Stencil st, const double *aa, const double *in, double *out) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    double bb = 0.0;
+    for (int q = 0; q < st.n; ++q) {
+      long long j;
+      if (nb_elem(g, e, st.off + 3 * q, j)) bb += st.w[q] * in[j];
+    }
+    out[e] = bb / aa[e];
+  }
+}
+__global__ void k_cone_adj(Grid g, Stencil st, const double *aa, const double *in, double *out) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < st.n; ++q) {
+      long long j;
+      if (nb_elem(g, e, st.off + 3 * q, j)) s += st.w[q] * in[j] / aa[j];
+    }
+    out[e] = s;
+  }
+}
+void launch_cone_aa(const Grid &g, Stencil st, double *aa, hipStream_t s) {
+  hipLaunchKernelGGL(k_cone_aa, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, st, aa);
+  ICX_CHECK_LAUNCH();
+}
+void launch_cone_forward(const Grid &g, Stencil st, const double *aa, const double *in, double *out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_cone_fwd, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, st, aa, in, out);
+  ICX_CHECK_LAUNCH();
+}
+void launch_cone_adjoint(const Grid &g, Stencil st, const double *aa, const double *in, double *out,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_cone_adj, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, st, aa, in, out);
+  ICX_CHECK_LAUNCH();
+}
+
+// threshold / thresholdDerivative, src/optimization/filter.hpp:153-185
+__device__ __forceinline__ double proj(double x, double beta, double T) {
+  return 0.5 * (tanh(T * beta) + tanh(beta * (x - T))) / tanh(T * beta);
+}
+__device__ __forceinline__ double dproj(double x, double beta, double T) {
+  const double th = tanh(beta * (x - T));
+  return (0.5 * (beta * (1.0 - th * th))) / tanh(T * beta);
+}
+__global__ void k_project(long long n, double beta, double T, const double *in, double *out, double *dout) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double x = in[i];
+    if (out) out[i] = proj(x, beta, T);
+    if (dout) dout[i] = dproj(x, beta, T);
+  }
+}
+void launch_project(long long n, double beta, double T, const double *in, double *out, double *dout,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_project, dim3(ew_blocks(n)), dim3(256), 0, s, n, beta, T, in, out, dout);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------------
+// Helmholtz rhs f_n = scale * sum_{e ni n} rho_e, element value scale * sum_{n in e} phi_n
+// ---------------------------------------------------------------------------------
+__global__ void k_helm_rhs(Grid g, double scale, const double *rho, double *f) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)((n / g.sx) % g.sy), k = (int)(n / ((long long)g.sx * g.sy));
+    double s = 0.0;
+    for (int dz = (g.ndim == 3 ? -1 : 0); dz <= 0; ++dz)
+      for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+          const int ex = i + dx, ey = j + dy, ez = k + dz;
+          if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+          s += rho[ex + (long long)g.nx * (ey + (long long)g.ny * ez)] * scale;
+        }
+    f[n] = s;
+  }
+}
+void launch_helm_rhs(const Grid &g, double scale, const double *rho, double *f, hipStream_t s) {
+  hipLaunchKernelGGL(k_helm_rhs, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, scale, rho, f);
+  ICX_CHECK_LAUNCH();
+}
+__global__ void k_helm_to_element(Grid g, double scale, const double *phi, double *out) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int ex = (int)(e % g.nx), ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
+    const int nq = g.ndim == 3 ? 8 : 4;
+    double s = 0.0;
+    for (int q = 0; q < nq; ++q)
+      s += phi[(ex + (q & 1)) + (long long)g.sx * ((ey + ((q >> 1) & 1)) + (long long)g.sy * (ez + (q >> 2)))];
+    out[e] = s * scale;
+  }
+}
+void launch_helm_to_element(const Grid &g, double scale, const double *phi, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_helm_to_element, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, scale, phi, out);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------------
+// OC, src/optimization/opt-solver/OCSolver.hpp:33-48
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ double oc_x(double lambda, double move, double xmin, double xmax, double x,
+                                       double dfdx, double dgdx) {
+  const double t = -dfdx / (dgdx * lambda);
+  double v = pow(t > 0.0 ? t : 0.0, 0.5) * x;
+  const double lo = fmax(xmin, (1.0 - move) * x);
+  const double hi = fmin(xmax, (1.0 + move) * x);
+  if (v < lo) v = lo;
+  else if (v > hi) v = hi;
+  return v;
+}
+__global__ void k_oc_candidate(long long n, double lambda, double move, double xmin, double xmax,
+                               const double *x, const double *dfdx, const double *dgdx, double *xnew) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    xnew[i] = oc_x(lambda, move, xmin, xmax, x[i], dfdx[i], dgdx[i]);
+}
+void launch_oc_candidate(long long n, double lambda, double move, double xmin, double xmax,
+                         const double *x, const double *dfdx, const double *dgdx, double *xnew,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_oc_candidate, dim3(ew_blocks(n)), dim3(256), 0, s, n, lambda, move, xmin, xmax, x,
+                     dfdx, dgdx, xnew);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void __launch_bounds__(256) k_volume(long long n, int project, double beta, double T,
+                                                const double *rho, double vol, Reduce red) {
+  __shared__ double scr[4];
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s += (project ? proj(rho[i], beta, T) : rho[i]) * vol;
+  s = block_sum<256>(s, scr);
+  if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
+}
+void launch_volume(long long n, int project, double beta, double T, const double *rho, double vol,
+                   double *out, Reduce red, hipStream_t s) {
+  const int nb = ew_blocks(n, 4, RED_BLOCKS);
+  hipLaunchKernelGGL(k_volume, dim3(nb), dim3(256), 0, s, n, project, beta, T, rho, vol, red);
+  launch_finalize(red, nb, 1, out, s);
+}
+
+// one bisection step of the OC volume lambda with the cone filter, fused:
+// sum_i threshold( sum_j w_ij x_new_j(lambda) / aa_i ) v
+__global__ void __launch_bounds__(256) k_oc_volume_cone(Grid g, Stencil st, const double *aa, double lambda,
+                                                        double move, double xmin, double xmax, const double *x,
+                                                        const double *dfdx, const double *dgdx, int project,
+                                                        double beta, double T, double vol, Reduce red) {
+  __shared__ double scr[4];
+  double s = 0.0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    double bb = 0.0;
+    for (int q = 0; q < st.n; ++q) {
+      long long j;
+      if (nb_elem(g, e, st.off + 3 * q, j))
+        bb += st.w[q] * oc_x(lambda, move, xmin, xmax, x[j], dfdx[j], dgdx[j]);
+    }
+    const double f = bb / aa[e];
+    s += (project ? proj(f, beta, T) : f) * vol;
+  }
+  s = block_sum<256>(s, scr);
+  if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
+}
+void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double lambda, double move,
+                           double xmin, double xmax, const double *x, const double *dfdx,
+                           const double *dgdx, int project, double beta, double T, double vol,
+                           double *out, Reduce red, hipStream_t s) {
+  const int nb = ew_blocks(g.ne, 1, RED_BLOCKS);
+  hipLaunchKernelGGL(k_oc_volume_cone, dim3(nb), dim3(256), 0, s, g, st, aa, lambda, move, xmin, xmax, x,
+                     dfdx, dgdx, project, beta, T, vol, red);
+  launch_finalize(red, nb, 1, out, s);
+}
+
+// ---------------------------------------------------------------------------------
+// MMA (m = 1), src/optimization/opt-solver/MMASolver.cpp:400-512 (GenSub) and
+// 184-398 (XYZofLAMBDA, DualGrad, DualHess, DualResidual sums)
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mma_gensub(long long n, int iter, MMAVecs m, const double *xval,
+                                                    const double *dfdx, const double *dgdx, double xmin,
+                                                    double xmax, double move, double albefa, double asyminit,
+                                                    double asymdec, double asyminc, double raa0,
+                                                    double xmamieps, Reduce red) {
+  __shared__ double scr[4];
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double xv = xval[i];
+    double low, upp;
+    const double xo1 = m.xold1[i], xo2 = m.xold2[i];
+    if (iter < 3) {
+      low = xv - asyminit * (xmax - xmin);
+      upp = xv + asyminit * (xmax - xmin);
+    } else {
+      low = m.low[i]; upp = m.upp[i];
+      const double zzz = (xv - xo1) * (xo1 - xo2);
+      const double gamma = zzz < 0.0 ? asymdec : (zzz > 0.0 ? asyminc : 1.0);
+      low = xv - gamma * (xo1 - low);
+      upp = xv + gamma * (upp - xo1);
+      const double xmami = fmax(xmamieps, xmax - xmin);
+      low = fmax(low, xv - 100.0 * xmami);
+      low = fmin(low, xv - 1.0e-5 * xmami);
+      upp = fmax(upp, xv + 1.0e-5 * xmami);
+      upp = fmin(upp, xv + 100.0 * xmami);
+      const double xmi = xmin - 1.0e-6, xma = xmax + 1.0e-6;
+      if (xv < xmi) { low = xv - (xma - xv) / 0.9; upp = xv + (xma - xv) / 0.9; }
+      if (xv > xma) { low = xv - (xv - xmi) / 0.9; upp = xv + (xv - xmi) / 0.9; }
+    }
+    m.low[i] = low; m.upp[i] = upp;
+    double al = fmax(xmin, low + albefa * (xv - low));
+    al = fmax(al, xv - move * (xmax - xmin));
+    m.alpha[i] = fmin(al, xmax);
+    double be = fmin(xmax, upp - albefa * (upp - xv));
+    be = fmin(be, xv + move * (xmax - xmin));
+    m.beta[i] = fmax(be, xmin);
+    const double xmamiinv = 1.0 / fmax(xmamieps, xmax - xmin);
+    const double df = dfdx[i];
+    const double pq = 0.001 * fabs(df) + raa0 * xmamiinv;
+    const double u2 = pow(upp - xv, 2.0), l2 = pow(xv - low, 2.0);
+    m.p0[i] = u2 * (fmax(0.0, df) + pq);
+    m.q0[i] = l2 * (fmax(0.0, -1.0 * df) + pq);
+    const double dg = dgdx[i];
+    const double pqj = 0.001 * fabs(dg) + raa0 * xmamiinv;
+    const double p1 = u2 * (fmax(0.0, dg) + pqj), q1 = l2 * (fmax(0.0, -1.0 * dg) + pqj);
+    m.p1[i] = p1; m.q1[i] = q1;
+    s += p1 / (upp - xv) + q1 / (xv - low);
+    m.xold2[i] = xo1;
+    m.xold1[i] = xv;
+  }
+  s = block_sum<256>(s, scr);
+  if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
+}
+void launch_mma_gensub(long long n, int iter, const MMAVecs &m, const double *xval,
+                       const double *dfdx, const double *dgdx, double xmin, double xmax,
+                       double move, double albefa, double asyminit, double asymdec, double asyminc,
+                       double raa0, double xmamieps, double *bsum, Reduce red, hipStream_t s) {
+  const int nb = ew_blocks(n, 4, RED_BLOCKS);
+  hipLaunchKernelGGL(k_mma_gensub, dim3(nb), dim3(256), 0, s, n, iter, m, xval, dfdx, dgdx, xmin, xmax, move,
+                     albefa, asyminit, asymdec, asyminc, raa0, xmamieps, red);
+  launch_finalize(red, nb, 1, bsum, s);
+}
+
+__global__ void __launch_bounds__(256) k_mma_dual(long long n, double lam, MMAVecs m, double *x, Reduce red) {
+  __shared__ double scr[4];
+  double s1 = 0.0, s2 = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double low = m.low[i], upp = m.upp[i];
+    const double pj = m.p0[i] + m.p1[i] * lam, qj = m.q0[i] + m.q1[i] * lam;
+    const double xp = (sqrt(pj) * low + sqrt(qj) * upp) / (sqrt(pj) + sqrt(qj));
+    double xi = xp;
+    if (xi < m.alpha[i]) xi = m.alpha[i];
+    if (xi > m.beta[i]) xi = m.beta[i];
+    x[i] = xi;
+    s1 += m.p1[i] / (upp - xi) + m.q1[i] / (xi - low);
+    const double PQ = m.p1[i] / pow(upp - xi, 2.0) - m.q1[i] / pow(xi - low, 2.0);
+    double df2 = -1.0 / (2.0 * pj / pow(upp - xi, 3.0) + 2.0 * qj / pow(xi - low, 3.0));
+    if (xp < m.alpha[i]) df2 = 0.0;
+    if (xp > m.beta[i]) df2 = 0.0;
+    s2 += (0.0 + PQ * df2) * PQ;
+  }
+  s1 = block_sum<256>(s1, scr);
+  s2 = block_sum<256>(s2, scr);
+  if (threadIdx.x == 0) {
+    red.partial[blockIdx.x] = s1;
+    red.partial[RED_MAXBLOCKS + blockIdx.x] = s2;
+  }
+}
+void launch_mma_dual(long long n, double lam, const MMAVecs &m, double *x, double *out, Reduce red,
+                     hipStream_t s) {
+  const int nb = ew_blocks(n, 4, RED_BLOCKS);
+  hipLaunchKernelGGL(k_mma_dual, dim3(nb), dim3(256), 0, s, n, lam, m, x, red);
+  launch_finalize(red, nb, 2, out, s);
+}
+
+}  // namespace icx

