@@ -39,6 +39,7 @@ int main(int argc, char *argv[]) {
   for (int i = 0; i < fem.nelx; i++) optim.design_s[i] = model.design_s0();
   LinearAnalysis compt(fem, model, force, dis);
   std::ofstream csv("function.csv");
+  csv.precision(17);
 
   while (convergence == 1) {
     optim.optstep++;

@@ -67,4 +67,4 @@ def test_cxx_driver_b_executable_matches_oracle():
                         helm_tol=1e-10)
     assert rows.shape[0] == steps
     rel = np.abs(rows[:, 1] - ref["obj"]) / np.abs(ref["obj"])
-    assert rel.max() <= 1e-6, rel
+    assert rel.max() <= 1e-7, rel
