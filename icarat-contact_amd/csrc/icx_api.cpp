@@ -29,6 +29,7 @@ struct icx_ctx {
   SolverState *h_state = nullptr;  // pinned mirror for polling
   double *h_scalar = nullptr;      // pinned scalars
   int *h_int = nullptr;            // pinned ints (iteration base)
+  bool use_graphs = true;          // replay captured iteration chunks (icx_set_option)
   // profile window (icx_profile.h)
   std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
   int prof_kernel = 0;
@@ -341,6 +342,21 @@ template <typename T> ElasticOp<T> cast_op(const ElasticOp<double> &a) {
 // =====================================================================================
 // job
 // =====================================================================================
+// One captured chunk of PCG iterations.  The kernels take the iteration index from
+// the device state, so one graph serves every chunk of a solve (and every later solve
+// of the same job); buffers and operator constants of a job never move.
+struct ChunkGraph {
+  hipGraphExec_t exec = nullptr;
+  ChunkGraph() = default;
+  ChunkGraph(const ChunkGraph &) = delete;
+  ChunkGraph &operator=(const ChunkGraph &) = delete;
+  ~ChunkGraph() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+};
+
 struct PcgWork64 {
   DBuf<double> r, p, t, z, invd;
 };
@@ -404,6 +420,7 @@ struct icx_job {
   bool design_set = false;
   bool u_valid = false;
   PcgWork64 pw;
+  ChunkGraph gr_plain, gr_fused, gr_mixed, gr_helm;
   MixedWork mw;
   FusedWork fw;
   HelmWork hw;
@@ -480,12 +497,41 @@ icx_status set_iter_base(icx_job *j, int base) {
 
 inline int poll_every(long long N) { return N > 2000000 ? 16 : (N > 100000 ? 12 : 8); }
 
+// Launches one chunk of iterations: replays the job's captured graph, capturing it on
+// first use.  Eager launches while a profile window is open (events time single kernels)
+// or when graphs are switched off.
+template <class F>
+icx_status run_chunk(icx_job *j, ChunkGraph &g, F launch_chunk) {
+  icx_ctx *c = j->ctx;
+  if (!c->use_graphs || c->prof_kernel) {
+    launch_chunk();
+    return ICX_OK;
+  }
+  if (!g.exec) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(S(j), hipStreamCaptureModeThreadLocal));
+    launch_chunk();
+    HIPCHK(c, hipStreamEndCapture(S(j), &graph));
+    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g.exec = nullptr;
+      c->use_graphs = false;
+      launch_chunk();
+      return ICX_OK;
+    }
+  }
+  HIPCHK(c, hipGraphLaunch(g.exec, S(j)));
+  return ICX_OK;
+}
+
 // ---- FP64 plain PCG: the Eigen ConjugateGradient loop ------------------------------
 // One kernel per Eigen vector expression (launch_eigen_iteration_tail) after the
 // operator; used for the structural parity form and for the Helmholtz filter solves.
 template <class MatvecFn>
 icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const double *invd,
-                     PcgWork64 &w, double tol, int maxit, MatvecFn matvec, icx_solve_info &info) {
+                     PcgWork64 &w, double tol, int maxit, MatvecFn matvec, icx_solve_info &info,
+                     ChunkGraph &graph) {
   icx_ctx *c = j->ctx;
   const PcgPartials P = j->parts(0, pcg_stream_blocks(N));
   HIPCHK(c, hipMemsetAsync(x, 0, N * sizeof(double), S(j)));
@@ -495,11 +541,14 @@ icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const 
   if (rc) return rc;
   const int every = poll_every(N);
   int base = 0;
-  while (!st.done) {
+  auto chunk = [&]() {
     for (int k = 0; k < every; ++k) {
       matvec(w.p.p, w.t.p);  // tmp = A p
       launch_eigen_iteration_tail(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, P, k, S(j));
     }
+  };
+  while (!st.done) {
+    if ((rc = run_chunk(j, graph, chunk))) return rc;
     rc = read_state(j, st);
     if (rc) return rc;
     base += every;
@@ -549,6 +598,15 @@ icx_status icx_profile_begin(icx_ctx *ctx, int kernel) {
   ctx->prof_ms = 0.0;
   ctx->prof_launches = 0;
   return ICX_OK;
+}
+
+icx_status icx_set_option(icx_ctx *ctx, int option, long long value) {
+  if (!ctx) return ICX_ERR_ARG;
+  if (option == ICX_OPT_GRAPHS) {
+    ctx->use_graphs = value != 0;
+    return ICX_OK;
+  }
+  return fail(ctx, ICX_ERR_ARG, "unknown option %d", option);
 }
 
 icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches) {
@@ -926,7 +984,7 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
     ProfScope ps(c, ICX_PROF_MATVEC64);
     launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j));
   };
-  return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info);
+  return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info, j->gr_plain);
 }
 
 // Fused forms: kernel A (direction, operator, deferred x update, p.Ap) and kernel B
@@ -950,7 +1008,7 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
   if (rc) return rc;
   const int every = poll_every(N);
   int base = 0;
-  while (!st.done) {
+  auto chunk = [&]() {
     for (int k = 0; k < every; ++k) {
       {
         ProfScope ps(c, ICX_PROF_FUSED_A);
@@ -962,6 +1020,9 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
         launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j));
       }
     }
+  };
+  while (!st.done) {
+    if ((rc = run_chunk(j, j->gr_fused, chunk))) return rc;
     rc = read_state(j, st);
     if (rc) return rc;
     base += every;
@@ -1014,19 +1075,22 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
   HIPCHK(c, hipStreamSynchronize(S(j)));
   const int every = poll_every(N);
   int refinements = 0;
+  auto chunk = [&]() {
+    for (int k = 0; k < every; ++k) {
+      {
+        ProfScope ps(c, ICX_PROF_FUSED_A);
+        launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
+                                     w.xlo.p, j->d_state.p, P, k, S(j));
+      }
+      {
+        ProfScope ps(c, ICX_PROF_FUSED_B);
+        launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j));
+      }
+    }
+  };
   while (!st.done || st.done >= 2) {
     if (!st.done) {
-      for (int k = 0; k < every; ++k) {
-        {
-          ProfScope ps(c, ICX_PROF_FUSED_A);
-          launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
-                                       w.xlo.p, j->d_state.p, P, k, S(j));
-        }
-        {
-          ProfScope ps(c, ICX_PROF_FUSED_B);
-          launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j));
-        }
-      }
+      if ((rc = run_chunk(j, j->gr_mixed, chunk))) return rc;
       rc = read_state(j, st);
       if (rc) return rc;
       if (!st.done) {
@@ -1224,7 +1288,7 @@ icx_status helm_apply_dev(icx_job *j, int sens, double tol, const double *din, d
   pw.r.p = w.r.p; pw.p.p = w.p.p; pw.t.p = w.t.p; pw.z.p = w.z.p;
   auto mv = [&](const double *p, double *t) { launch_helm_matvec(j->g, w.op, p, t, j->d_state.p, S(j)); };
   icx_solve_info info{};
-  icx_status rc = pcg_plain(j, j->g.nn, w.b.p, w.x.p, w.invd.p, pw, tol, 10000, mv, info);
+  icx_status rc = pcg_plain(j, j->g.nn, w.b.p, w.x.p, w.invd.p, pw, tol, 10000, mv, info, j->gr_helm);
   pw.r.p = pw.p.p = pw.t.p = pw.z.p = nullptr;  // not owned
   if (rc && rc != ICX_ERR_NOCONV) return rc;
   launch_helm_to_element(j->g, sens ? 1.0 : 1.0 / nq, w.x.p, dout, S(j));

@@ -30,8 +30,9 @@ EXPORTS = [
     "icx_project", "icx_oc_update", "icx_mma_reset", "icx_mma_update",
     "icx_opt_init", "icx_opt_step", "icx_opt_get_design",
     "icx_solve_batch", "icx_job_footprint",
-    "icx_profile_begin", "icx_profile_end",
+    "icx_profile_begin", "icx_profile_end", "icx_set_option",
 ]
+OPT_GRAPHS = 1
 PROF_MATVEC64, PROF_FUSED_A, PROF_FUSED_B = 1, 2, 3
 
 
@@ -114,6 +115,9 @@ class Context:
 
     def synchronize(self):
         self.check(lib().icx_ctx_synchronize(self.h))
+
+    def set_option(self, option, value):
+        self.check(lib().icx_set_option(self.h, int(option), C.c_longlong(int(value))))
 
     def profile_begin(self, kernel):
         self.check(lib().icx_profile_begin(self.h, int(kernel)))

@@ -21,6 +21,10 @@ enum {
 };
 
 icx_status icx_profile_begin(icx_ctx *ctx, int kernel);
+/* ICX_OPT_GRAPHS (default 1): replay each job's PCG iteration chunk from a captured
+ * HIP graph instead of launching its kernels one by one; results are identical. */
+enum { ICX_OPT_GRAPHS = 1 };
+icx_status icx_set_option(icx_ctx *ctx, int option, long long value);
 icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
 
 #ifdef __cplusplus
