@@ -30,6 +30,7 @@ struct icx_ctx {
   double *h_scalar = nullptr;      // pinned scalars
   int *h_int = nullptr;            // pinned ints (iteration base)
   bool use_graphs = true;          // replay captured iteration chunks (icx_set_option)
+  int tile_depth = 0;              // hexa8 operator z depth per block, 0 = automatic
   // profile window (icx_profile.h)
   std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
   int prof_kernel = 0;
@@ -421,6 +422,12 @@ struct icx_job {
   bool u_valid = false;
   PcgWork64 pw;
   ChunkGraph gr_plain, gr_fused, gr_mixed, gr_helm;
+  void reset_graphs() {
+    gr_plain.reset();
+    gr_fused.reset();
+    gr_mixed.reset();
+    gr_helm.reset();
+  }
   MixedWork mw;
   FusedWork fw;
   HelmWork hw;
@@ -606,6 +613,16 @@ icx_status icx_set_option(icx_ctx *ctx, int option, long long value) {
     ctx->use_graphs = value != 0;
     return ICX_OK;
   }
+  if (option == ICX_OPT_TILE_DEPTH) {
+    if (value < 0 || value > 64) return fail(ctx, ICX_ERR_ARG, "tile depth %lld outside 0..64", value);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (icx_job *j : ctx->jobs) {
+      j->g.tile_depth = (int)value;
+      j->reset_graphs();  // captured launches carry the old grid
+    }
+    ctx->tile_depth = (int)value;
+    return ICX_OK;
+  }
   return fail(ctx, ICX_ERR_ARG, "unknown option %d", option);
 }
 
@@ -662,6 +679,7 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   g.sx = g.nx + 1; g.sy = g.ny + 1; g.sz = ndim == 3 ? g.nz + 1 : 1;
   g.nn = (long long)g.sx * g.sy * g.sz;
   g.ne = (long long)g.nx * g.ny * g.nz;
+  g.tile_depth = ctx->tile_depth;
   if (g.nn > (1LL << 31) / 4) { delete j; return fail(ctx, ICX_ERR_ARG, "mesh too large"); }
   j->nc = ndim;
   j->el.ndim = ndim;

@@ -8,6 +8,12 @@ namespace icx {
 
 #define ICX_CHECK_LAUNCH() (void)hipGetLastError()
 
+// s_waitcnt vmcnt(0): every vector-memory load and store of this wave has completed
+// (gfx9-family encoding; the expcnt and lgkmcnt fields are at their maxima, i.e. not
+// waited on).  Placed where a pipelined kernel's previous-step traffic is due anyway,
+// it also tells the compiler's wait insertion that nothing older is pending.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---------------------------------------------------------------------------------
 // reductions
 //
@@ -206,30 +212,55 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
   return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
+// Totals of the pairs (p[2q], p[2q+1]), q < n, in a fixed order, returned to every
+// thread.  One round of 16-byte loads and a single barrier; scratch: 2 BS/64 doubles
+// used by this call only.
+template <int BS>
+__device__ __forceinline__ void block_total2(const double *p, int n, double *scratch, double &s0, double &s1) {
+  double a = 0.0, b = 0.0;
+#pragma unroll 4
+  for (int q = threadIdx.x; q < n; q += BS) {
+    const double2 v = reinterpret_cast<const double2 *>(p)[q];
+    a += v.x;
+    b += v.y;
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if ((threadIdx.x & 63) == 0) {
+    scratch[threadIdx.x >> 6] = a;
+    scratch[BS / 64 + (threadIdx.x >> 6)] = b;
+  }
+  __syncthreads();
+  s0 = s1 = 0.0;
+#pragma unroll
+  for (int w = 0; w < BS / 64; ++w) {
+    s0 += scratch[w];
+    s1 += scratch[BS / 64 + w];
+  }
+}
+
 // ---------------------------------------------------------------------------------
-// scalars of the fused / mixed PCG iteration i, computed identically by every block
-// from the partials of the previous kernels (see PcgPartials in icx_internal.h)
+// scalars of the fused / mixed PCG iteration i.  Kernel A of iteration i sums the
+// (|r_i|^2, r_i.z_i) partials of kernel B of iteration i-1 and stores r_i.z_i in
+// SolverState::rz_hist[i & 1]; kernel B of iteration i stores alpha_i in
+// alpha_hist[i & 1].  Each total is formed once, so beta_i and the deferred step
+// alpha_{i-1} are bit-identical wherever they are used.
 // ---------------------------------------------------------------------------------
 struct IterScalars {
-  double rr, rz, rz_prev, pAp_prev, beta, alpha_prev;
+  double rr, rz, beta, alpha_prev;
   int stop;  // 0 continue, 2 maxit, 3 tolerance met by the recursive residual, 4 replace
 };
 
+// scratch: 2 BS/64 doubles reserved for this call
 template <int BS>
 __device__ __forceinline__ IterScalars iter_scalars(const SolverState *st, const PcgPartials &P, int i,
                                                     double *scratch) {
   IterScalars s;
-  const double *pb_prev = P.PB + (size_t)((i - 1) & 1) * 2 * RED_MAXBLOCKS;
-  const double *pb_prev2 = P.PB + (size_t)(i & 1) * 2 * RED_MAXBLOCKS;
-  s.rr = block_total<BS>(pb_prev, P.nB, 2, scratch);
-  s.rz = block_total<BS>(pb_prev + 1, P.nB, 2, scratch);
+  block_total2<BS>(P.PB + (size_t)((i - 1) & 1) * 2 * RED_MAXBLOCKS, P.nB, scratch, s.rr, s.rz);
   if (i > 0) {
-    s.rz_prev = block_total<BS>(pb_prev2 + 1, P.nB, 2, scratch);
-    s.pAp_prev = block_total<BS>(P.PA + (size_t)((i - 1) & 1) * RED_MAXBLOCKS, P.nA, 1, scratch);
-    s.beta = s.rz / s.rz_prev;
-    s.alpha_prev = s.rz_prev / s.pAp_prev;
+    s.beta = s.rz / st->rz_hist[(i - 1) & 1];
+    s.alpha_prev = st->alpha_hist[(i - 1) & 1];
   } else {
-    s.rz_prev = s.pAp_prev = 1.0;
     s.beta = 0.0;
     s.alpha_prev = 0.0;
   }

@@ -13,6 +13,7 @@ struct Grid {
   int sx, sy, sz;      // node counts per axis (sz = 1 for 2D)
   long long nn;        // nodes
   long long ne;        // elements
+  int tile_depth;      // hexa8 operator: node layers per block along z (0 = automatic)
 };
 
 // Element operator in the per-element Haar basis.  For an axis-aligned box
@@ -50,6 +51,8 @@ struct SolverState {
   int iter;            // progress: iterations completed
   int stop_iter;       // iteration index at which `done` was set
   int flush_iter;      // iteration whose owed x update was already applied
+  double rz_hist[2];   // fused/mixed forms: r_i.z_i, stored by kernel A of iteration i in slot i & 1
+  double alpha_hist[2];// fused/mixed forms: alpha_i, stored by kernel B of iteration i in slot i & 1
 };
 
 // Per-block partial sums of a PCG solve, double-buffered by iteration parity.

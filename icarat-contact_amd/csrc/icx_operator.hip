@@ -20,44 +20,51 @@ template <typename T> struct OpArgs {
   int k;              // MODE 1: position of the launch in its chunk
 };
 
-// Iteration prologue of kernel A (MODE 1): returns false if the solve stops here.
+// Iteration scalars of kernel A (MODE 1) for iteration i, formed by every block.
+// Block 0 publishes r_i.z_i (and, at a stop, the stop record).  Returns false if the
+// solve stops at this iteration.  scratch: 2 NT/64 doubles for this call only.
 template <typename T, int NT>
-__device__ __forceinline__ bool fused_prologue(const OpArgs<T> &a, double *scratch, int &i, T &beta,
-                                               T &alpha_x, const T *&pold, T *&pnew) {
+__device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, double *scratch, int i, T &beta, T &alpha_x) {
   SolverState *st = a.st;
-  i = st->iter_base + a.k;
   const IterScalars sc = iter_scalars<NT>(st, a.P, i, scratch);
   const bool owed = i > 0 && st->flush_iter != i;
-  if (sc.stop) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rz_hist[i & 1] = sc.rz;
+    if (sc.stop) {
       st->done = sc.stop;
       st->stop_iter = i;
       st->rr = sc.rr;
       st->alpha_owed = owed ? sc.alpha_prev : 0.0;
     }
-    return false;
   }
+  if (sc.stop) return false;
   beta = (T)sc.beta;
   alpha_x = owed ? (T)sc.alpha_prev : T(0);
-  pold = (i & 1) ? a.P0 : a.P1;
-  pnew = (i & 1) ? a.P1 : a.P0;
   return true;
 }
 
 // ---------------------------------------------------------------------------------
 // hexa8: a block of (TI+1)(TJ+1) threads owns a TI x TJ x TK brick of nodes and
-// marches along z.  Node layers rotate through three LDS buffers; the global loads of
-// layer ez+2, the moduli of element layer ez+1 and the owned nodes' x / p_old of layer
-// ez+1 are issued before the element phase of layer ez, so their latency overlaps
-// one layer of work.  Per layer: every thread applies one element (the elements
-// touching the owned nodes), writes its 24 outputs to LDS, and every owned node sums
-// its four in-layer contributions; contributions to the layer above are carried in
-// registers.  Two barriers per layer; summation order fixed (deterministic).
+// marches along z.  Node layers rotate through three LDS buffers.  Every global load
+// is issued one layer step before its data is used -- the nodes of layer ez+2, the
+// moduli of element layer ez+1 and the owned nodes' x / p_old / constraint mask of
+// layer ez+1 go out before the element phase of layer ez -- and the loop is unrolled
+// by two with alternating register sets, so no wait on a load in flight is ever
+// forced before its consumer.  Per layer: every thread applies one element (the
+// elements touching the owned nodes), writes its 24 outputs to LDS, and every owned
+// node sums its four in-layer contributions; contributions to the layer above are
+// carried in registers.  Two barriers per layer; summation order fixed.
 //
 // MODE 0: y = K x.   MODE 1 (fused kernel A of PCG iteration i): p_i = invd r +
 // beta_i p_{i-1} formed in the LDS loader, y = K p_i, x += alpha_{i-1} p_{i-1} on the
-// owned nodes, partial p_i . y.
+// owned nodes, partial p_i . y.  The loads of the first node layer are issued before
+// the iteration scalars are reduced.
 // ---------------------------------------------------------------------------------
+template <typename T> struct OwnedNode {
+  T x[3], po[3];
+  uint32_t m;
+};
+
 template <typename T, int TI, int TJ, int MODE>
 __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   constexpr int NT = (TI + 1) * (TJ + 1);
@@ -66,13 +73,17 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   __shared__ T s_in[3][3][LN];
   __shared__ T s_out[24][NT];
   __shared__ double s_red[NT / 64];
+  __shared__ double s_sc[2 * (NT / 64)];
   const Grid &g = a.g;
   if (a.st && a.st->done) return;  // solve already finished (uniform per launch)
   int it = 0;
-  T beta = T(0), alpha_x = T(0);
   const T *pold = nullptr;
   T *pnew = nullptr;
-  if (MODE == 1 && !fused_prologue<T, NT>(a, s_red, it, beta, alpha_x, pold, pnew)) return;
+  if (MODE == 1) {
+    it = a.st->iter_base + a.k;
+    pold = (it & 1) ? a.P0 : a.P1;
+    pnew = (it & 1) ? a.P1 : a.P0;
+  }
 
   const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
   const int t = xcd_tile(blockIdx.x, nblocks);
@@ -121,6 +132,7 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
       }
     }
   };
+  T beta = T(0), alpha_x = T(0);
   auto commit = [&](int buf) {
 #pragma unroll
     for (int p = 0; p < NPOS; ++p) {
@@ -132,38 +144,59 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
       }
     }
   };
-  T E_next = T(0);
-  auto prefetch_E = [&](int ez) {
-    E_next = (ex_ok && ez >= 0 && ez < g.nz) ? a.E[e_xy + e_layer * ez] : T(0);
+  auto load_E = [&](int ez) -> T {
+    return (ex_ok && ez >= 0 && ez < g.nz) ? a.E[e_xy + e_layer * ez] : T(0);
   };
-  T nx_x[3] = {T(0), T(0), T(0)}, nx_po[3] = {T(0), T(0), T(0)};
-  auto prefetch_node = [&](int k) {
-    if (MODE == 1 && node_ok && k >= k0 && k < k1) {
+  auto load_node = [&](int k, OwnedNode<T> &o) {
+    if (node_ok && k >= k0 && k < k1) {
       const long long n = n_xy + n_layer * k;
+      o.m = a.mask ? a.mask[n] : 0u;
+      if (MODE == 1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        nx_x[c] = a.x[c * nn + n];
-        nx_po[c] = pold[c * nn + n];
+        for (int c = 0; c < 3; ++c) {
+          o.x[c] = a.x[c * nn + n];
+          o.po[c] = pold[c * nn + n];
+        }
       }
     }
   };
 
   double dot = 0.0;
   T acc_top[3] = {T(0), T(0), T(0)};
-  prefetch(k0 - 1);
-  commit(0);
-  prefetch(k0);
-  prefetch_E(k0 - 1);
-  T E_cur = E_next;
-  for (int ez = k0 - 1, tt = 0; ez < k1; ++ez, ++tt) {
-    const int b0 = tt % 3, b1 = (tt + 1) % 3;
-    commit(b1);                          // node layer ez+1
-    if (ez + 2 <= k1) prefetch(ez + 2);  // needed at the next layer step
-    prefetch_E(ez + 1);
-    T cur_x[3], cur_po[3];
+  // Outputs of an owned node are stored one step late, next to that step's loads:
+  // loads and stores share one counter, and a store still in flight when a load is
+  // waited for makes the wait total, so stores issued at the end of a step would put
+  // their full latency on the next step's critical path.
+  T yv[3] = {T(0), T(0), T(0)};
+  bool have_out = false;
+  auto store_out = [&](int ez, int buf, const OwnedNode<T> &o) {
+    if (have_out) {
+      const long long n = n_xy + n_layer * ez;
+      const int ctr = (lx + 1) + LW * (ly + 1);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) { cur_x[c] = nx_x[c]; cur_po[c] = nx_po[c]; }
-    prefetch_node(ez + 1);
+      for (int c = 0; c < 3; ++c) {
+        a.y[c * nn + n] = yv[c];
+        if (MODE == 1) {
+          const T pv_ = s_in[buf][c][ctr];
+          pnew[c * nn + n] = pv_;
+          a.x[c * nn + n] = o.x[c] + alpha_x * o.po[c];
+          dot += (double)pv_ * (double)yv[c];
+        }
+      }
+    }
+  };
+  // one layer step: element layer ez between node layers ez and ez+1 (LDS buffers
+  // tt % 3 and (tt+1) % 3); Ec / oc are this step's operands, En / on receive the
+  // next step's (on still holds the previous step's operands until its outputs are
+  // stored)
+  auto layer = [&](int ez, int tt, const T &Ec, T &En, const OwnedNode<T> &oc, OwnedNode<T> &on) {
+    const int b0 = tt % 3, b1 = (tt + 1) % 3;
+    wait_vmem();                         // the previous step's loads and stores
+    commit(b1);                          // node layer ez+1
+    store_out(ez - 1, (tt + 2) % 3, on);  // node layer ez-1
+    if (ez + 2 <= k1) prefetch(ez + 2);  // committed at the next step
+    En = load_E(ez + 1);
+    load_node(ez + 1, on);
     __syncthreads();
     {
       T v[3][8];
@@ -175,13 +208,14 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
           const int qa = q & 1, qb = (q >> 1) & 1, qc = q >> 2;
           v[c][q] = s_in[qc ? b1 : b0][c][(tx + qa) + LW * (ty + qb)];
         }
-      elastic_hex8<T>(v, a.op, ok ? E_cur : T(0));
+      elastic_hex8<T>(v, a.op, ok ? Ec : T(0));
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int q = 0; q < 8; ++q) s_out[c * 8 + q][tid] = v[c][q];
     }
     __syncthreads();
+    have_out = false;
     if (node_thread) {
       T yb[3];
 #pragma unroll
@@ -199,23 +233,29 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
           }
         }
       if (ez >= k0 && node_ok) {
-        const long long n = n_xy + n_layer * ez;
-        const int ctr = (lx + 1) + LW * (ly + 1);
-        const uint8_t m = a.mask ? a.mask[n] : 0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const T yv = ((m >> c) & 1) ? T(0) : yb[c];
-          a.y[c * nn + n] = yv;
-          if (MODE == 1) {
-            const T pv_ = s_in[b0][c][ctr];
-            pnew[c * nn + n] = pv_;
-            a.x[c * nn + n] = cur_x[c] + alpha_x * cur_po[c];
-            dot += (double)pv_ * (double)yv;
-          }
-        }
+        for (int c = 0; c < 3; ++c) yv[c] = ((oc.m >> c) & 1) ? T(0) : yb[c];
+        have_out = true;
       }
     }
-    E_cur = E_next;
+  };
+
+  prefetch(k0 - 1);
+  T Ea = load_E(k0 - 1), Eb = T(0);
+  OwnedNode<T> na{}, nb{};
+  if (MODE == 1 && !fused_scalars<T, NT>(a, s_sc, it, beta, alpha_x)) return;
+  commit(0);
+  prefetch(k0);
+  int ez = k0 - 1, tt = 0;
+  for (; ez + 1 < k1; ez += 2, tt += 2) {
+    layer(ez, tt, Ea, Eb, na, nb);
+    layer(ez + 1, tt + 1, Eb, Ea, nb, na);
+  }
+  if (ez < k1) {
+    layer(ez, tt, Ea, Eb, na, nb);
+    store_out(ez, tt % 3, na);
+  } else {
+    store_out(ez - 1, (tt + 2) % 3, nb);
   }
   if (MODE == 1) {
     const double tot = block_sum<NT>(dot, s_red);
@@ -232,13 +272,19 @@ __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
   __shared__ T s_in[3][2][LW];
   __shared__ T s_out[8][NT];
   __shared__ double s_red[(NT + 63) / 64];
+  __shared__ double s_sc[2 * ((NT + 63) / 64)];
   const Grid &g = a.g;
   if (a.st && a.st->done) return;
   int it = 0;
   T beta = T(0), alpha_x = T(0);
   const T *pold = nullptr;
   T *pnew = nullptr;
-  if (MODE == 1 && !fused_prologue<T, NT>(a, s_red, it, beta, alpha_x, pold, pnew)) return;
+  if (MODE == 1) {
+    it = a.st->iter_base + a.k;
+    pold = (it & 1) ? a.P0 : a.P1;
+    pnew = (it & 1) ? a.P1 : a.P0;
+    if (!fused_scalars<T, ((NT + 63) / 64) * 64>(a, s_sc, it, beta, alpha_x)) return;
+  }
   const int nblocks = a.tiles_x * a.tiles_y;
   const int t = xcd_tile(blockIdx.x, nblocks);
   const int ti = t % a.tiles_x, tj = t / a.tiles_x;
@@ -340,7 +386,9 @@ static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads
     // z depth: enough blocks to keep 256 CUs busy, at most one partial per block
     const long long cols = (long long)a.tiles_x * a.tiles_y;
     int tk = 16;
-    while (tk > 4 && cols * ((g.sz + tk - 1) / tk) < 2048) tk >>= 1;
+    if (g.tile_depth > 0) tk = g.tile_depth;
+    else
+      while (tk > 4 && cols * ((g.sz + tk - 1) / tk) < 2048) tk >>= 1;
     while (cols * ((g.sz + tk - 1) / tk) > RED_MAXBLOCKS) tk <<= 1;
     a.TK = tk;
     a.tiles_z = (g.sz + tk - 1) / tk;

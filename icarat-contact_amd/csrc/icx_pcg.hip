@@ -65,29 +65,101 @@ void launch_pcg_setup(long long n, SolverState *st, const T *b, T *r, const T *i
   ICX_CHECK_LAUNCH();
 }
 
-template <typename T>
+// 16-byte vector of T
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; static constexpr int W = 4; };
+template <> struct Vec16<double> { using type = double2; static constexpr int W = 2; };
+
+// element w of a vector (w is a compile-time constant after unrolling)
+__device__ __forceinline__ float lane(const float4 &v, int w) { return w == 0 ? v.x : w == 1 ? v.y : w == 2 ? v.z : v.w; }
+__device__ __forceinline__ double lane(const double2 &v, int w) { return w == 0 ? v.x : v.y; }
+__device__ __forceinline__ void set_lane(float4 &v, int w, float x) {
+  if (w == 0) v.x = x; else if (w == 1) v.y = x; else if (w == 2) v.z = x; else v.w = x;
+}
+__device__ __forceinline__ void set_lane(double2 &v, int w, double x) {
+  if (w == 0) v.x = x; else v.y = x;
+}
+
+// Kernel B of PCG iteration i: alpha_i = r_i.z_i / p_i.Ap_i, r -= alpha_i Ap,
+// partials (|r|^2, r.invd r).  Each thread moves U 16-byte vectors of r, Ap and invd
+// per round; the first round's loads are issued before alpha_i is reduced.  Block 0
+// publishes alpha_i.  The vectors are 16-byte aligned (device allocations).
+template <typename T, int U>
 __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd,
                                                    PcgPartials P, int k) {
+  using V = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::W;
   __shared__ double scr[BS / 64];
+  __shared__ double scr2[2 * (BS / 64)];
   if (st->done) return;
   const int it = st->iter_base + k;
-  const double rz = block_total<BS>(pb_slot(P, it - 1) + 1, P.nB, 2, scr);
+  const long long nv = n / W;
+  const long long stride = (long long)gridDim.x * BS;
+  const long long v0 = blockIdx.x * (long long)BS + threadIdx.x;
+  V *rv = reinterpret_cast<V *>(r);
+  const V *av = reinterpret_cast<const V *>(Ap);
+  const V *dv = reinterpret_cast<const V *>(invd);
+  V R[U], A[U], D[U];
+  auto load = [&](long long base) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long v = base + u * stride;
+      if (v < nv) {
+        R[u] = rv[v];
+        A[u] = av[v];
+        D[u] = dv[v];
+      }
+    }
+  };
+  load(v0);
   const double pAp = block_total<BS>(P.PA + (size_t)(it & 1) * RED_MAXBLOCKS, P.nA, 1, scr);
-  const T alpha = (T)(rz / pAp);
+  const double alpha_d = st->rz_hist[it & 1] / pAp;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha_hist[it & 1] = alpha_d;
+  const T alpha = (T)alpha_d;
   double rr = 0.0, rzn = 0.0;
-  for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
-    T ri = r[i];
-    ri -= alpha * Ap[i];
-    r[i] = ri;
-    rr += (double)ri * (double)ri;
-    rzn += (double)ri * (double)(invd[i] * ri);
+  for (long long base = v0; base < nv; base += U * stride) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long v = base + u * stride;
+      if (v < nv) {
+        V out;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const T ri = lane(R[u], w) - alpha * lane(A[u], w);
+          set_lane(out, w, ri);
+          rr += (double)ri * (double)ri;
+          rzn += (double)ri * (double)(lane(D[u], w) * ri);
+        }
+        rv[v] = out;
+      }
+    }
+    if (base + U * stride < nv) load(base + U * stride);
   }
-  rr = block_sum<BS>(rr, scr);
-  rzn = block_sum<BS>(rzn, scr);
+  // scalar tail (n not a multiple of W)
+  const long long tail = nv * W + v0;
+  if (tail < n) {
+    const T ri = r[tail] - alpha * Ap[tail];
+    r[tail] = ri;
+    rr += (double)ri * (double)ri;
+    rzn += (double)ri * (double)(invd[tail] * ri);
+  }
+  rr = wave_sum(rr);
+  rzn = wave_sum(rzn);
+  if ((threadIdx.x & 63) == 0) {
+    scr2[threadIdx.x >> 6] = rr;
+    scr2[BS / 64 + (threadIdx.x >> 6)] = rzn;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) {
+      s0 += scr2[w];
+      s1 += scr2[BS / 64 + w];
+    }
     double *slot = pb_slot(P, it);
-    slot[2 * blockIdx.x] = rr;
-    slot[2 * blockIdx.x + 1] = rzn;
+    slot[2 * blockIdx.x] = s0;
+    slot[2 * blockIdx.x + 1] = s1;
     if (blockIdx.x == 0) st->iter = it + 1;
   }
 }
@@ -95,7 +167,7 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P, int k,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_fusedB<T>, dim3(P.nB), dim3(BS), 0, s, n, st, r, Ap, invd, P, k);
+  hipLaunchKernelGGL((k_pcg_fusedB<T, 4>), dim3(P.nB), dim3(BS), 0, s, n, st, r, Ap, invd, P, k);
   ICX_CHECK_LAUNCH();
 }
 

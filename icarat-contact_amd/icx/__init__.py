@@ -32,7 +32,7 @@ EXPORTS = [
     "icx_solve_batch", "icx_job_footprint",
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
 ]
-OPT_GRAPHS = 1
+OPT_GRAPHS, OPT_TILE_DEPTH = 1, 2
 PROF_MATVEC64, PROF_FUSED_A, PROF_FUSED_B = 1, 2, 3
 
 

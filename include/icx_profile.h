@@ -21,11 +21,17 @@ enum {
 };
 
 icx_status icx_profile_begin(icx_ctx *ctx, int kernel);
-/* ICX_OPT_GRAPHS (default 1): replay each job's PCG iteration chunk from a captured
- * HIP graph instead of launching its kernels one by one; results are identical. */
-enum { ICX_OPT_GRAPHS = 1 };
-icx_status icx_set_option(icx_ctx *ctx, int option, long long value);
 icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
+
+/* Tuning options of a context (no counterpart in the reference).
+ * ICX_OPT_GRAPHS (default 1): replay each job's PCG iteration chunk from a captured
+ *   HIP graph instead of launching its kernels one by one; results are identical.
+ * ICX_OPT_TILE_DEPTH (default 0 = automatic): node layers along z per block of the
+ *   hexa8 operator kernels, 1..64 (raised as needed to keep the block count within
+ *   the partial-sum buffers).  Applies to every job of the context; results of the
+ *   operator are identical, PCG dot products change only in rounding. */
+enum { ICX_OPT_GRAPHS = 1, ICX_OPT_TILE_DEPTH = 2 };
+icx_status icx_set_option(icx_ctx *ctx, int option, long long value);
 
 #ifdef __cplusplus
 }
