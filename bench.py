@@ -1,16 +1,23 @@
 """Benchmark: DOF-matvec/s of the matrix-free hexa8 PCG solve on one MI355X.
 
-Workload: 96 x 48 x 48 hexa8 cantilever (221,184 elements, 691,488 free DOFs),
-E0 = 1, Emin = 1e-9, nu = 0.3, p = 3, densities i.i.d. U[0.01, 1] (seed 20261203),
-x = 0 face clamped, unit load on the edge {x = L, z = 0}; one solve to relative
-residual 1e-8.  A step is one such solve in the mixed-precision form (FP32 operator,
-FP64 residual replacement); the FP64 plain form is timed beside it.
+Workload (the headline line): 96 x 48 x 48 hexa8 cantilever (221,184 elements,
+691,488 free DOFs), E0 = 1, Emin = 1e-9, nu = 0.3, p = 3, densities i.i.d.
+U[0.01, 1] (seed 20261203), x = 0 face clamped, unit load on the edge
+{x = L, z = 0}; one solve to relative residual 1e-8.  A step is one such solve in
+the mixed-precision form (FP32 operator, FP64 residual replacement), replayed from
+captured HIP graphs as a caller would run it; the FP64 plain form (the Eigen loop)
+and the fused FP64 form are timed beside it, and the 192 x 96 x 96 cantilever
+(1,769,472 elements) is measured the same way under "config.large".
 
 value = free DOFs x (operator applications of the step) / step wall time.
+
+--pmc-workload runs a bounded counter workload instead (a 256 MiB device copy of
+known size, then eager mixed-form iterations), for rocprofv3 --pmc passes.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -19,6 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "icarat-contact_amd"))
 
 HBM_PEAK = 8.0e12
 SEED = 20261203
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_hex8_fusedA.md")
 
 
 def parse():
@@ -27,14 +35,19 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, nargs=3, default=[96, 48, 48])
+    ap.add_argument("--large", type=int, nargs=3, default=[192, 96, 96])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly")
+    ap.add_argument("--pmc-workload", action="store_true")
+    ap.add_argument("--pmc-iters", type=int, default=200)
     return ap.parse_args()
 
 
-def cpu_baseline(nx, ny, nz, rho, seconds):
+def cpu_baseline(nx, ny, nz, seconds):
     """The oracle's CSR Jacobi-PCG (Eigen loop) on the host cores, bounded sample."""
     import ctypes as C
 
@@ -42,6 +55,7 @@ def cpu_baseline(nx, ny, nz, rho, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     pr = O.cantilever_3d(nx, ny, nz)
+    rho = O.random_density(nx * ny * nz, SEED)
     t0 = time.perf_counter()
     b = pr.assemble(rho)
     t_asm = time.perf_counter() - t0
@@ -68,6 +82,121 @@ def cpu_baseline(nx, ny, nz, rho, seconds):
     }
 
 
+def measured_traffic(nx, ny, nz):
+    """HBM bytes per launch of the fused operator kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size)."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic\[%dx%dx%d\]\s*=\s*([0-9.eE+]+)" % (nx, ny, nz), text)
+    return float(m.group(1)) if m else None
+
+
+def pmc_workload(args):
+    import torch
+
+    import icx
+    a = torch.ones(64 * 1024 * 1024, device="cuda")  # 256 MiB
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy: 268435456 bytes read and written per launch", flush=True)
+    nx, ny, nz = args.size
+    ctx = icx.Context(0)
+    ctx.set_option(icx.OPT_GRAPHS, 0)
+    job = icx.cantilever_3d(ctx, nx, ny, nz)
+    job.set_design(icx.random_density(job.nelx, SEED))
+    _, info = job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=args.pmc_iters, want_u=False)
+    ctx.synchronize()
+    print("mixed form: %d iterations of %dx%dx%d (N = %d, Ne = %d)" % (
+        info["iterations"], nx, ny, nz, job.numeq, job.nelx), flush=True)
+
+
+def measure(icx, ctx, size, tol, steps, warmup, plain_too):
+    """Mixed-form solves of one cantilever: graph-replayed steps, then one solve with
+    HIP events around every launch of the fused kernels A and B."""
+    import numpy as np
+    nx, ny, nz = size
+    job = icx.cantilever_3d(ctx, nx, ny, nz)
+    job.set_design(icx.random_density(job.nelx, SEED))
+    N, Ne = job.numeq, job.nelx
+    for _ in range(warmup):
+        job.solve(icx.SOLVE_MIXED, tol=tol, maxit=50000, want_u=False)
+    ctx.synchronize()
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, info = job.solve(icx.SOLVE_MIXED, tol=tol, maxit=50000, want_u=False)
+        infos.append(info)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    # Dominant kernel: HIP events around every launch of the fused FP32 operator
+    # kernel A (the library's stream; launched one by one while the window is open).
+    # Algorithmic bytes per launch: reads r, p_old, invdiag, x_lo (4 words per DOF)
+    # and E (1 word per element); writes p, Ap, x_lo (3 words per DOF).
+    prof = {}
+    for name, kid in (("A", icx.PROF_FUSED_A), ("B", icx.PROF_FUSED_B)):
+        ctx.profile_begin(kid)
+        _, pinfo = job.solve(icx.SOLVE_MIXED, tol=tol, maxit=50000, want_u=False)
+        secs, launches = ctx.profile_end()
+        prof[name] = (secs, launches, pinfo["iterations"])
+    kern_s, launches, executed = prof["A"]
+    bytes_A = (7 * N + Ne) * 4
+    res = {
+        "N": N, "Ne": Ne, "wall": wall, "infos": infos,
+        "ms_step": wall / steps * 1e3,
+        "value": N * sum(i["matvecs"] for i in infos) / wall,
+        "bytes_A": bytes_A,
+        "avg_A": kern_s / max(executed, 1),
+        "launches_A": launches, "executed_A": executed,
+        "achieved_A": bytes_A * executed / kern_s if kern_s > 0 else 0.0,
+        "avg_B": prof["B"][0] / max(prof["B"][2], 1),
+    }
+    inner = infos[-1]["iterations"]
+    refin = infos[-1]["refinements"]
+    # whole-solve algorithmic bytes: 9 N w + N_e w_E per inner iteration (w = 4),
+    # 6 N 8 + N_e 8 per FP64 residual replacement
+    res["bytes_solve"] = inner * (9 * N * 4 + Ne * 4) + refin * (6 * N * 8 + Ne * 8)
+    res["solve_frac"] = res["bytes_solve"] / (res["ms_step"] * 1e-3) / HBM_PEAK
+    res["p50"] = float(np.percentile([i["seconds"] * 1e3 for i in infos], 50))
+    res["p99"] = float(np.percentile([i["seconds"] * 1e3 for i in infos], 99))
+    if plain_too:
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        _, pinfo = job.solve(icx.SOLVE_PLAIN, tol=tol, maxit=50000, want_u=False)
+        ctx.synchronize()
+        pt = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        _, finfo = job.solve(icx.SOLVE_FUSED, tol=tol, maxit=50000, want_u=False)
+        ctx.synchronize()
+        ft = time.perf_counter() - t1
+        res["plain"] = {"ms": pt * 1e3, "iterations": pinfo["iterations"], "relres": pinfo["relres"],
+                        "dof_matvec_per_s": N * pinfo["matvecs"] / pt,
+                        "speedup_mixed_vs_plain": pt * 1e3 / res["ms_step"],
+                        "fused_fp64_ms": ft * 1e3, "fused_fp64_iterations": finfo["iterations"]}
+    else:
+        res["plain"] = None
+    del job
+    return res
+
+
+def summary(size, r, tol):
+    info = r["infos"][-1]
+    return {"workload": "hexa8 cantilever %dx%dx%d, single solve to rel-res %g, mixed precision"
+                        % (tuple(size) + (tol,)),
+            "elements": r["Ne"], "free_dofs": r["N"], "inner_iterations": info["iterations"],
+            "fp64_replacements": info["refinements"], "matvecs": info["matvecs"],
+            "true_relres": info["relres"], "ms_per_solve": r["ms_step"],
+            "dof_matvec_per_s": r["value"],
+            "step_ms_p50": r["p50"], "step_ms_p99": r["p99"],
+            "solve_hbm_frac": r["solve_frac"],
+            "kernel_A_us": r["avg_A"] * 1e6, "kernel_A_GBps": r["achieved_A"] / 1e9,
+            "kernel_B_us": r["avg_B"] * 1e6,
+            "fp64_plain": r["plain"]}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -75,100 +204,48 @@ def main():
               "(the single-solve path would shard by x-slabs with one halo exchange per "
               "iteration; not implemented)", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    if args.pmc_workload:
+        pmc_workload(args)
+        return
 
     import icx
-    nx, ny, nz = args.size
     ctx = icx.Context(0)
-    job = icx.cantilever_3d(ctx, nx, ny, nz)
-    rho = icx.random_density(job.nelx, SEED)
-    job.set_design(rho)
-    N, Ne = job.numeq, job.nelx
-
-    def sync():
-        ctx.synchronize()
-        try:
-            import torch
-            if torch.cuda.is_available():
-                torch.cuda.synchronize()
-        except Exception:
-            pass
-
-    # warmup (mixed)
-    for _ in range(args.warmup):
-        job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=False)
-    sync()
-    infos = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _, info = job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=False)
-        infos.append(info)
-    sync()
-    wall = time.perf_counter() - t0
-    ms_step = wall / args.steps * 1e3
-    inner = infos[-1]["iterations"]
-    mv = infos[-1]["matvecs"]
-    value = N * sum(i["matvecs"] for i in infos) / wall
-    # Dominant kernel: one more solve of the same workload with HIP events around every
-    # launch of the fused FP32 operator kernel (on the library's stream; kernels are
-    # launched one by one while the window is open).  Algorithmic bytes per launch:
-    # reads r, p_old, invdiag, x_lo (4 words per DOF) and E (1 word per element),
-    # writes p, Ap, x_lo (3 words per DOF).
-    ctx.profile_begin(icx.PROF_FUSED_A)
-    _, pinfo_a = job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=False)
-    kern_s, launches = ctx.profile_end()
-    executed = pinfo_a["iterations"]
-    bytes_A = (7 * N + Ne) * 4
-    avg_A = kern_s / max(launches, 1)
-    achieved = bytes_A * executed / kern_s if kern_s > 0 else 0.0
-    # whole-iteration algorithmic bytes (9 N w + N_e w_E per inner iteration,
-    # 6 N 8 + N_e 8 per FP64 replacement)
-    refin = infos[-1]["refinements"]
-    bytes_solve = inner * (9 * N * 4 + Ne * 4) + refin * (6 * N * 8 + Ne * 8)
-    solve_frac = bytes_solve / (ms_step * 1e-3) / HBM_PEAK
-
-    plain = None
-    if not args.no_plain:
-        sync()
-        t1 = time.perf_counter()
-        _, pinfo = job.solve(icx.SOLVE_PLAIN, tol=args.tol, maxit=50000, want_u=False)
-        sync()
-        pt = time.perf_counter() - t1
-        plain = {"ms": pt * 1e3, "iterations": pinfo["iterations"], "relres": pinfo["relres"],
-                 "dof_matvec_per_s": N * pinfo["matvecs"] / pt,
-                 "speedup_mixed_vs_plain": pt * 1e3 / ms_step}
-        t1 = time.perf_counter()
-        _, finfo = job.solve(icx.SOLVE_FUSED, tol=args.tol, maxit=50000, want_u=False)
-        sync()
-        ft = time.perf_counter() - t1
-        plain["fused_fp64_ms"] = ft * 1e3
-        plain["fused_fp64_iterations"] = finfo["iterations"]
+    if args.no_graphs:
+        ctx.set_option(icx.OPT_GRAPHS, 0)
+    r = measure(icx, ctx, args.size, args.tol, args.steps, args.warmup, not args.no_plain)
+    print("headline %dx%dx%d: %.2f ms/solve" % (tuple(args.size) + (r["ms_step"],)), file=sys.stderr,
+          flush=True)
+    large = None
+    if not args.no_large:
+        rl = measure(icx, ctx, args.large, args.tol, 1, 1, not args.no_plain)
+        large = summary(args.large, rl, args.tol)
+        large["roofline_A"] = {"achieved": rl["achieved_A"] / 1e9, "frac": rl["achieved_A"] / HBM_PEAK,
+                               "bytes_per_launch": rl["bytes_A"],
+                               "traffic": measured_traffic(*args.large)}
+        print("large %dx%dx%d: %.2f ms/solve" % (tuple(args.large) + (rl["ms_step"],)), file=sys.stderr,
+              flush=True)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(nx, ny, nz, rho, args.cpu_seconds)
+        cpu = cpu_baseline(*args.size, args.cpu_seconds)
 
+    cfg = summary(args.size, r, args.tol)
+    cfg["graphs"] = not args.no_graphs
+    cfg["large"] = large
     line = {
         "metric": "DOF-matvec/s + % HBM peak, FP32-refined vs FP64 at res 1e-8; p50/p99 ms",
-        "value": value, "unit": "DOF-matvec/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "value": r["value"], "unit": "DOF-matvec/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 operator / f64 residual",
         "data": "synthetic (seeded densities)",
-        "config": {"workload": "hexa8 cantilever %dx%dx%d, single solve to rel-res %g, mixed "
-                               "precision" % (nx, ny, nz, args.tol),
-                   "elements": Ne, "free_dofs": N, "inner_iterations": inner,
-                   "fp64_replacements": refin, "matvecs": mv,
-                   "true_relres": infos[-1]["relres"],
-                   "step_ms_p50": float(np.percentile([i["seconds"] * 1e3 for i in infos], 50)),
-                   "step_ms_p99": float(np.percentile([i["seconds"] * 1e3 for i in infos], 99)),
-                   "solve_hbm_frac": solve_frac, "fp64_plain": plain},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+        "config": cfg,
+        "roofline": {"bound": "hbm", "achieved": r["achieved_A"] / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": r["achieved_A"] / HBM_PEAK,
+                     "traffic": measured_traffic(*args.size),
                      "kernel": "k_hex8<float,15,15,1> (fused direction update + operator)",
-                     "avg_launch_us": avg_A * 1e6, "launches": launches,
-                     "executed_launches": executed,
-                     "avg_executed_launch_us": kern_s / max(executed, 1) * 1e6,
-                     "bytes_per_launch": bytes_A},
+                     "avg_launch_us": r["avg_A"] * 1e6, "launches": r["launches_A"],
+                     "executed_launches": r["executed_A"],
+                     "bytes_per_launch": r["bytes_A"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -85,10 +85,12 @@ template <typename T> struct DBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
-// events around launches of the profiled kernel
+// Event pair for one launch of the profiled kernel, bound to the launch itself
+// (LaunchEvents: the kernel's own start / end timestamps).
 struct ProfScope {
   icx_ctx *c;
   bool on;
+  LaunchEvents le{};
   ProfScope(icx_ctx *ctx, int kernel) : c(ctx), on(ctx->prof_kernel == kernel) {
     if (!on) return;
     if ((int)c->ev_pool.size() < c->ev_used + 2) {
@@ -97,11 +99,12 @@ struct ProfScope {
       c->ev_pool.push_back(a);
       c->ev_pool.push_back(b);
     }
-    (void)hipEventRecord(c->ev_pool[c->ev_used], c->stream);
+    le.start = c->ev_pool[c->ev_used];
+    le.stop = c->ev_pool[c->ev_used + 1];
   }
+  const LaunchEvents *ev() const { return on ? &le : nullptr; }
   ~ProfScope() {
     if (!on) return;
-    (void)hipEventRecord(c->ev_pool[c->ev_used + 1], c->stream);
     c->ev_used += 2;
     c->prof_launches += 1;
   }
@@ -296,6 +299,18 @@ bool build_elastic_op(const BoxElement &el, const std::vector<double> &K, Elasti
           }
           op.blk[chi][c][c2] = v * scale;
         }
+  // the blocks are symmetric (K0 is); make them exactly so -- the kernels read the
+  // upper triangles only
+  for (int chi = 0; chi < nq; ++chi)
+    for (int c = 0; c < nc; ++c)
+      for (int c2 = c + 1; c2 < nc; ++c2) {
+        const double a = op.blk[chi][c][c2], b = op.blk[chi][c2][c];
+        if (std::fabs(a - b) > tol * scale) {
+          why = "element stiffness is not symmetric";
+          return false;
+        }
+        op.blk[chi][c][c2] = op.blk[chi][c2][c] = 0.5 * (a + b);
+      }
   for (int c = 0; c < nc; ++c) {
     op.kd[c] = K[(0 * nc + c) * nd + 0 * nc + c];
     for (int q = 1; q < nq; ++q)
@@ -680,7 +695,8 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   g.nn = (long long)g.sx * g.sy * g.sz;
   g.ne = (long long)g.nx * g.ny * g.nz;
   g.tile_depth = ctx->tile_depth;
-  if (g.nn > (1LL << 31) / 4) { delete j; return fail(ctx, ICX_ERR_ARG, "mesh too large"); }
+  // node vectors are addressed with 32-bit byte offsets (3 components x 8 bytes)
+  if (g.nn > (1LL << 27)) { delete j; return fail(ctx, ICX_ERR_ARG, "mesh too large (more than 2^27 nodes)"); }
   j->nc = ndim;
   j->el.ndim = ndim;
   j->el.nq = ndim == 3 ? 8 : 4;
@@ -1000,7 +1016,7 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
   launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
   auto mv = [&](const double *p, double *t) {
     ProfScope ps(c, ICX_PROF_MATVEC64);
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j));
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
   };
   return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info, j->gr_plain);
 }
@@ -1031,11 +1047,11 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
       {
         ProfScope ps(c, ICX_PROF_FUSED_A);
         launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p,
-                                      w.Ap.p, j->d_u.p, j->d_state.p, P, k, S(j));
+                                      w.Ap.p, j->d_u.p, j->d_state.p, P, k, S(j), ps.ev());
       }
       {
         ProfScope ps(c, ICX_PROF_FUSED_B);
-        launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j));
+        launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j), ps.ev());
       }
     }
   };
@@ -1098,11 +1114,11 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
       {
         ProfScope ps(c, ICX_PROF_FUSED_A);
         launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
-                                     w.xlo.p, j->d_state.p, P, k, S(j));
+                                     w.xlo.p, j->d_state.p, P, k, S(j), ps.ev());
       }
       {
         ProfScope ps(c, ICX_PROF_FUSED_B);
-        launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j));
+        launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j), ps.ev());
       }
     }
   };

@@ -1,5 +1,6 @@
 // Device helpers shared by the libicx kernels (gfx950, wave64).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "icx_internal.h"
@@ -7,6 +8,13 @@
 namespace icx {
 
 #define ICX_CHECK_LAUNCH() (void)hipGetLastError()
+
+// Launch with an optional event pair taking the kernel's own start / end timestamps.
+template <typename F, typename... Args>
+inline void launch_ev(F kernel, int blocks, int threads, hipStream_t s, const LaunchEvents *ev, Args... args) {
+  if (ev) hipExtLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, s, ev->start, ev->stop, 0, args...);
+  else hipLaunchKernelGGL(kernel, dim3(blocks), dim3(threads), 0, s, args...);
+}
 
 // s_waitcnt vmcnt(0): every vector-memory load and store of this wave has completed
 // (gfx9-family encoding; the expcnt and lgkmcnt fields are at their maxima, i.e. not
@@ -48,6 +56,7 @@ __device__ __forceinline__ double block_sum(double v, double *scratch) {
 template <int BS>
 __device__ __forceinline__ double block_total(const double *p, int n, int s, double *scratch) {
   double a = 0.0;
+#pragma unroll 4
   for (int q = threadIdx.x; q < n; q += BS) a += p[(long long)q * s];
   return block_sum<BS>(a, scratch);
 }
@@ -129,7 +138,7 @@ __device__ __forceinline__ void elastic_hex8(T (&v)[3][8], const ElasticOp<T> &o
       T acc = T(0);
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc)
-        if (blk_nz3(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+        if (blk_nz3(chi, c, cc)) acc += op.blk[chi][c < cc ? c : cc][c < cc ? cc : c] * m[cc];
       o[c][chi ^ (1 << c)] = E * acc;
     }
   }
@@ -152,7 +161,7 @@ __device__ __forceinline__ void elastic_quad4(T (&v)[2][4], const ElasticOp<T> &
       T acc = T(0);
 #pragma unroll
       for (int cc = 0; cc < 2; ++cc)
-        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c < cc ? c : cc][c < cc ? cc : c] * m[cc];
       o[c][chi ^ (1 << c)] = E * acc;
     }
   }
@@ -176,7 +185,7 @@ __device__ __forceinline__ T energy_hex8(T (&v)[3][8], const ElasticOp<T> &op) {
       T acc = T(0);
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc)
-        if (blk_nz3(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+        if (blk_nz3(chi, c, cc)) acc += op.blk[chi][c < cc ? c : cc][c < cc ? cc : c] * m[cc];
       e += m[c] * acc;
     }
   }
@@ -194,7 +203,7 @@ __device__ __forceinline__ T energy_quad4(T (&v)[2][4], const ElasticOp<T> &op) 
       T acc = T(0);
 #pragma unroll
       for (int cc = 0; cc < 2; ++cc)
-        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c][cc] * m[cc];
+        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c < cc ? c : cc][c < cc ? cc : c] * m[cc];
       e += m[c] * acc;
     }
   }
@@ -251,23 +260,26 @@ struct IterScalars {
   int stop;  // 0 continue, 2 maxit, 3 tolerance met by the recursive residual, 4 replace
 };
 
-// scratch: 2 BS/64 doubles reserved for this call
+// st: a snapshot of the solver state taken at kernel entry (one scalar round trip;
+// both history slots are read and selected afterwards).  scratch: 2 BS/64 doubles
+// reserved for this call.
 template <int BS>
-__device__ __forceinline__ IterScalars iter_scalars(const SolverState *st, const PcgPartials &P, int i,
+__device__ __forceinline__ IterScalars iter_scalars(const SolverState &st, const PcgPartials &P, int i,
                                                     double *scratch) {
   IterScalars s;
   block_total2<BS>(P.PB + (size_t)((i - 1) & 1) * 2 * RED_MAXBLOCKS, P.nB, scratch, s.rr, s.rz);
   if (i > 0) {
-    s.beta = s.rz / st->rz_hist[(i - 1) & 1];
-    s.alpha_prev = st->alpha_hist[(i - 1) & 1];
+    const bool odd = (i - 1) & 1;
+    s.beta = s.rz / (odd ? st.rz_hist[1] : st.rz_hist[0]);
+    s.alpha_prev = odd ? st.alpha_hist[1] : st.alpha_hist[0];
   } else {
     s.beta = 0.0;
     s.alpha_prev = 0.0;
   }
   s.stop = 0;
-  if (s.rr < st->threshold) s.stop = 3;
-  else if (st->delta2 > 0.0 && s.rr < st->delta2 * st->rr_ref) s.stop = 4;
-  else if (i >= st->maxit) s.stop = 2;
+  if (s.rr < st.threshold) s.stop = 3;
+  else if (st.delta2 > 0.0 && s.rr < st.delta2 * st.rr_ref) s.stop = 4;
+  else if (i >= st.maxit) s.stop = 2;
   return s;
 }
 

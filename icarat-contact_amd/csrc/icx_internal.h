@@ -69,13 +69,20 @@ struct Reduce {
   double *partial;     // [NV][RED_MAXBLOCKS]
 };
 
+// Optional event pair bound to one kernel launch (hipExtLaunchKernelGGL): the events
+// take the kernel's own start and end timestamps.  Null: plain launch.
+struct LaunchEvents {
+  hipEvent_t start, stop;
+};
+
 // ---- structured operators (icx_operator.hip) -------------------------------------
 // y = K(E) x on the node grid (SoA).  mask: per-node bit c = constrained (output
 // forced to zero); may be null.  st may be null; otherwise the kernel exits at once
 // when st->done is set.
 template <typename T>
 void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *x, T *y, const SolverState *st, hipStream_t s);
+                           const T *x, T *y, const SolverState *st, hipStream_t s,
+                           const LaunchEvents *ev = nullptr);
 
 // Fused kernel A of iteration i = st->iter_base + k: scalars from the partials,
 // p_i = invd r + beta_i p_{i-1} (P[i&1] <- ..., P[(i-1)&1] read), Ap_i = K p_i,
@@ -83,7 +90,7 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
                            const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
-                           PcgPartials P, int k, hipStream_t s);
+                           PcgPartials P, int k, hipStream_t s, const LaunchEvents *ev = nullptr);
 // number of blocks of kernel A for this grid (its partial count)
 template <typename T>
 int elastic_fusedA_blocks(const Grid &g);
@@ -102,7 +109,7 @@ void launch_pcg_setup(long long n, SolverState *st, const T *b, T *r, const T *i
 // fused kernel B of iteration i: alpha_i = rz_i / pAp_i; r -= alpha_i Ap; (|r|^2, r.z) -> PB[i&1]
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
-                       int k, hipStream_t s);
+                       int k, hipStream_t s, const LaunchEvents *ev = nullptr);
 // fused FP64 form at the stop: x += alpha_owed p_{stop-1}
 void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0, const double *P1,
                       hipStream_t s);

@@ -24,10 +24,11 @@ template <typename T> struct OpArgs {
 // Block 0 publishes r_i.z_i (and, at a stop, the stop record).  Returns false if the
 // solve stops at this iteration.  scratch: 2 NT/64 doubles for this call only.
 template <typename T, int NT>
-__device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, double *scratch, int i, T &beta, T &alpha_x) {
+__device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverState &ss, double *scratch, int i,
+                                              T &beta, T &alpha_x) {
   SolverState *st = a.st;
-  const IterScalars sc = iter_scalars<NT>(st, a.P, i, scratch);
-  const bool owed = i > 0 && st->flush_iter != i;
+  const IterScalars sc = iter_scalars<NT>(ss, a.P, i, scratch);
+  const bool owed = i > 0 && ss.flush_iter != i;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->rz_hist[i & 1] = sc.rz;
     if (sc.stop) {
@@ -65,22 +66,34 @@ template <typename T> struct OwnedNode {
   uint32_t m;
 };
 
+// load / store at a 32-bit byte offset from a uniform base (global_load with an SGPR
+// base and a 32-bit VGPR offset; node vectors stay below 4 GiB)
+template <typename T> __device__ __forceinline__ T ld_off(const T *base, uint32_t off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T> __device__ __forceinline__ void st_off(T *base, uint32_t off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 template <typename T, int TI, int TJ, int MODE>
 __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   constexpr int NT = (TI + 1) * (TJ + 1);
   constexpr int LW = TI + 2, LN = (TI + 2) * (TJ + 2);
   constexpr int NPOS = (LN + NT - 1) / NT;
+  constexpr uint32_t W = sizeof(T);
   __shared__ T s_in[3][3][LN];
   __shared__ T s_out[24][NT];
   __shared__ double s_red[NT / 64];
   __shared__ double s_sc[2 * (NT / 64)];
   const Grid &g = a.g;
-  if (a.st && a.st->done) return;  // solve already finished (uniform per launch)
+  SolverState ss{};
+  if (a.st) ss = *a.st;  // one scalar round trip
+  if (ss.done) return;   // solve already finished (uniform per launch)
   int it = 0;
   const T *pold = nullptr;
   T *pnew = nullptr;
   if (MODE == 1) {
-    it = a.st->iter_base + a.k;
+    it = ss.iter_base + a.k;
     pold = (it & 1) ? a.P0 : a.P1;
     pnew = (it & 1) ? a.P1 : a.P0;
   }
@@ -91,43 +104,50 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   const int i0 = ti * TI, j0 = tj * TJ, k0 = tk * a.TK;
   const int k1 = min(k0 + a.TK, g.sz);
   const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // element of this thread: (tx, ty) in a (TI+1) x (TJ+1) patch whose corner nodes
+  // span the brick plus a one-node halo
   const int tx = tid % (TI + 1), ty = tid / (TI + 1);
   const int ex = i0 - 1 + tx, ey = j0 - 1 + ty;
   const bool ex_ok = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
-  const long long e_xy = ex_ok ? ex + (long long)g.nx * ey : 0;
-  const long long e_layer = (long long)g.nx * g.ny;
-  const bool node_thread = tid < TI * TJ;
-  const int lx = tid % TI, ly = tid / TI;
+  const uint32_t e_off = ex_ok ? (uint32_t)(ex + g.nx * ey) * W : 0u;
+  const uint32_t e_layer = (uint32_t)(g.nx * g.ny) * W;
+  // owned node of this thread on rows of TI+1 threads (bank-conflict-free node phase)
+  const int lx = tid % (TI + 1), ly = tid / (TI + 1);
+  const bool node_thread = lx < TI && ly < TJ;
   const int ni = i0 + lx, nj = j0 + ly;
   const bool node_ok = node_thread && ni < g.sx && nj < g.sy;
-  const long long nn = g.nn;
-  const long long n_layer = (long long)g.sx * g.sy;
-  const long long n_xy = node_ok ? ni + (long long)g.sx * nj : 0;
+  const uint32_t cstr = (uint32_t)g.nn * W;  // component stride in bytes
+  const uint32_t n_layer = (uint32_t)(g.sx * g.sy) * W;
+  const uint32_t n_off = node_ok ? (uint32_t)(ni + g.sx * nj) * W : 0u;
 
-  // loader positions of this thread
-  long long pos_xy[NPOS];
-  bool pos_ok[NPOS];
+  // Loader positions of this thread.  Nodes outside the grid are loaded from offset 0:
+  // they are corners of elements outside the grid only, whose modulus is zero, so the
+  // (finite) value read never reaches a result.
+  uint32_t pos_off[NPOS];
 #pragma unroll
   for (int p = 0; p < NPOS; ++p) {
     const int idx = tid + p * NT;
     const int li = idx % LW, lj = idx / LW;
     const int gi = i0 - 1 + li, gj = j0 - 1 + lj;
-    pos_ok[p] = idx < LN && gi >= 0 && gi < g.sx && gj >= 0 && gj < g.sy;
-    pos_xy[p] = pos_ok[p] ? gi + (long long)g.sx * gj : 0;
+    const bool ok = idx < LN && gi >= 0 && gi < g.sx && gj >= 0 && gj < g.sy;
+    pos_off[p] = ok ? (uint32_t)(gi + g.sx * gj) * W : 0u;
   }
+  // position p is loaded by the waves that hold a slot of it
+  auto pos_live = [&](int p) { return p * NT + wave * 64 < LN; };
   T pr[NPOS][3], pv[NPOS][3], po[NPOS][3];
   auto prefetch = [&](int k) {
-    const bool kin = k >= 0 && k < g.sz;
+    const uint32_t lo = (k >= 0 && k < g.sz) ? (uint32_t)k * n_layer : 0u;
 #pragma unroll
     for (int p = 0; p < NPOS; ++p) {
-      const bool ok = kin && pos_ok[p];
-      const long long n = pos_xy[p] + n_layer * k;
+      if (!pos_live(p)) continue;
+      const uint32_t o = pos_off[p] + lo;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        pr[p][c] = ok ? a.in[c * nn + n] : T(0);
+        pr[p][c] = ld_off(a.in, o + c * cstr);
         if (MODE == 1) {
-          pv[p][c] = ok ? a.invd[c * nn + n] : T(0);
-          po[p][c] = ok ? pold[c * nn + n] : T(0);
+          pv[p][c] = ld_off(a.invd, o + c * cstr);
+          po[p][c] = ld_off(pold, o + c * cstr);
         }
       }
     }
@@ -136,6 +156,7 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   auto commit = [&](int buf) {
 #pragma unroll
     for (int p = 0; p < NPOS; ++p) {
+      if (!pos_live(p)) continue;
       const int idx = tid + p * NT;
       if (idx < LN) {
 #pragma unroll
@@ -145,18 +166,17 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
     }
   };
   auto load_E = [&](int ez) -> T {
-    return (ex_ok && ez >= 0 && ez < g.nz) ? a.E[e_xy + e_layer * ez] : T(0);
+    const uint32_t lo = (ez >= 0 && ez < g.nz) ? (uint32_t)ez * e_layer : 0u;
+    return ld_off(a.E, e_off + lo);
   };
   auto load_node = [&](int k, OwnedNode<T> &o) {
-    if (node_ok && k >= k0 && k < k1) {
-      const long long n = n_xy + n_layer * k;
-      o.m = a.mask ? a.mask[n] : 0u;
-      if (MODE == 1) {
+    const uint32_t n = n_off + ((k >= k0 && k < k1) ? (uint32_t)k * n_layer : 0u);
+    o.m = a.mask ? (uint32_t)ld_off(a.mask, n / W) : 0u;
+    if (MODE == 1) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          o.x[c] = a.x[c * nn + n];
-          o.po[c] = pold[c * nn + n];
-        }
+      for (int c = 0; c < 3; ++c) {
+        o.x[c] = ld_off(a.x, n + c * cstr);
+        o.po[c] = ld_off(pold, n + c * cstr);
       }
     }
   };
@@ -171,15 +191,15 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   bool have_out = false;
   auto store_out = [&](int ez, int buf, const OwnedNode<T> &o) {
     if (have_out) {
-      const long long n = n_xy + n_layer * ez;
+      const uint32_t n = n_off + (uint32_t)ez * n_layer;
       const int ctr = (lx + 1) + LW * (ly + 1);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        a.y[c * nn + n] = yv[c];
+        st_off(a.y, n + c * cstr, yv[c]);
         if (MODE == 1) {
           const T pv_ = s_in[buf][c][ctr];
-          pnew[c * nn + n] = pv_;
-          a.x[c * nn + n] = o.x[c] + alpha_x * o.po[c];
+          st_off(pnew, n + c * cstr, pv_);
+          st_off(a.x, n + c * cstr, o.x[c] + alpha_x * o.po[c]);
           dot += (double)pv_ * (double)yv[c];
         }
       }
@@ -191,10 +211,10 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   // stored)
   auto layer = [&](int ez, int tt, const T &Ec, T &En, const OwnedNode<T> &oc, OwnedNode<T> &on) {
     const int b0 = tt % 3, b1 = (tt + 1) % 3;
-    wait_vmem();                         // the previous step's loads and stores
-    commit(b1);                          // node layer ez+1
+    wait_vmem();                          // the previous step's loads and stores
+    commit(b1);                           // node layer ez+1
     store_out(ez - 1, (tt + 2) % 3, on);  // node layer ez-1
-    if (ez + 2 <= k1) prefetch(ez + 2);  // committed at the next step
+    if (ez + 2 <= k1) prefetch(ez + 2);   // committed at the next step
     En = load_E(ez + 1);
     load_node(ez + 1, on);
     __syncthreads();
@@ -243,7 +263,8 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   prefetch(k0 - 1);
   T Ea = load_E(k0 - 1), Eb = T(0);
   OwnedNode<T> na{}, nb{};
-  if (MODE == 1 && !fused_scalars<T, NT>(a, s_sc, it, beta, alpha_x)) return;
+  if (MODE == 1 && !fused_scalars<T, NT>(a, ss, s_sc, it, beta, alpha_x)) return;
+  wait_vmem();
   commit(0);
   prefetch(k0);
   int ez = k0 - 1, tt = 0;
@@ -274,16 +295,18 @@ __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
   __shared__ double s_red[(NT + 63) / 64];
   __shared__ double s_sc[2 * ((NT + 63) / 64)];
   const Grid &g = a.g;
-  if (a.st && a.st->done) return;
+  SolverState ss{};
+  if (a.st) ss = *a.st;
+  if (ss.done) return;
   int it = 0;
   T beta = T(0), alpha_x = T(0);
   const T *pold = nullptr;
   T *pnew = nullptr;
   if (MODE == 1) {
-    it = a.st->iter_base + a.k;
+    it = ss.iter_base + a.k;
     pold = (it & 1) ? a.P0 : a.P1;
     pnew = (it & 1) ? a.P1 : a.P0;
-    if (!fused_scalars<T, ((NT + 63) / 64) * 64>(a, s_sc, it, beta, alpha_x)) return;
+    if (!fused_scalars<T, ((NT + 63) / 64) * 64>(a, ss, s_sc, it, beta, alpha_x)) return;
   }
   const int nblocks = a.tiles_x * a.tiles_y;
   const int t = xcd_tile(blockIdx.x, nblocks);
@@ -388,7 +411,7 @@ static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads
     int tk = 16;
     if (g.tile_depth > 0) tk = g.tile_depth;
     else
-      while (tk > 4 && cols * ((g.sz + tk - 1) / tk) < 2048) tk >>= 1;
+      while (tk > 2 && cols * ((g.sz + tk - 1) / tk) < 2048) tk = tk > 4 ? tk >> 1 : tk - 1;
     while (cols * ((g.sz + tk - 1) / tk) > RED_MAXBLOCKS) tk <<= 1;
     a.TK = tk;
     a.tiles_z = (g.sz + tk - 1) / tk;
@@ -417,32 +440,28 @@ int elastic_fusedA_blocks(const Grid &g) {
 
 template <typename T>
 void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *x, T *y, const SolverState *st, hipStream_t s) {
+                           const T *x, T *y, const SolverState *st, hipStream_t s, const LaunchEvents *ev) {
   OpArgs<T> a{};
   a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = x; a.y = y;
   a.st = const_cast<SolverState *>(st);
   int nb, nt;
   setup_tiles(g, a, nb, nt);
-  if (g.ndim == 3)
-    hipLaunchKernelGGL((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 0>), dim3(nb), dim3(nt), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_quad4<T, QUAD_TI, 0>), dim3(nb), dim3(nt), 0, s, a);
+  if (g.ndim == 3) launch_ev((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 0>), nb, nt, s, ev, a);
+  else launch_ev((k_quad4<T, QUAD_TI, 0>), nb, nt, s, ev, a);
   ICX_CHECK_LAUNCH();
 }
 
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
                            const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
-                           PcgPartials P, int k, hipStream_t s) {
+                           PcgPartials P, int k, hipStream_t s, const LaunchEvents *ev) {
   OpArgs<T> a{};
   a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = r; a.invd = invd; a.P0 = P0; a.P1 = P1;
   a.y = Ap; a.x = x; a.st = st; a.P = P; a.k = k;
   int nb, nt;
   setup_tiles(g, a, nb, nt);
-  if (g.ndim == 3)
-    hipLaunchKernelGGL((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), dim3(nb), dim3(nt), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_quad4<T, QUAD_TI, 1>), dim3(nb), dim3(nt), 0, s, a);
+  if (g.ndim == 3) launch_ev((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), nb, nt, s, ev, a);
+  else launch_ev((k_quad4<T, QUAD_TI, 1>), nb, nt, s, ev, a);
   ICX_CHECK_LAUNCH();
 }
 
@@ -450,17 +469,18 @@ template int elastic_fusedA_blocks<double>(const Grid &);
 template int elastic_fusedA_blocks<float>(const Grid &);
 template void launch_elastic_matvec<double>(const Grid &, const ElasticOp<double> &, const double *,
                                             const uint8_t *, const double *, double *,
-                                            const SolverState *, hipStream_t);
+                                            const SolverState *, hipStream_t, const LaunchEvents *);
 template void launch_elastic_matvec<float>(const Grid &, const ElasticOp<float> &, const float *,
                                            const uint8_t *, const float *, float *, const SolverState *,
-                                           hipStream_t);
+                                           hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<double>(const Grid &, const ElasticOp<double> &, const double *,
                                             const uint8_t *, const double *, const double *, double *,
                                             double *, double *, double *, SolverState *, PcgPartials, int,
-                                            hipStream_t);
+                                            hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<float>(const Grid &, const ElasticOp<float> &, const float *,
                                            const uint8_t *, const float *, const float *, float *, float *,
-                                           float *, float *, SolverState *, PcgPartials, int, hipStream_t);
+                                           float *, float *, SolverState *, PcgPartials, int, hipStream_t,
+                                           const LaunchEvents *);
 
 // ---------------------------------------------------------------------------------
 // Helmholtz scalar operator: node-parallel gather over the (up to) 8 elements

@@ -91,8 +91,9 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
   constexpr int W = Vec16<T>::W;
   __shared__ double scr[BS / 64];
   __shared__ double scr2[2 * (BS / 64)];
-  if (st->done) return;
-  const int it = st->iter_base + k;
+  const SolverState ss = *st;  // one scalar round trip
+  if (ss.done) return;
+  const int it = ss.iter_base + k;
   const long long nv = n / W;
   const long long stride = (long long)gridDim.x * BS;
   const long long v0 = blockIdx.x * (long long)BS + threadIdx.x;
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
   };
   load(v0);
   const double pAp = block_total<BS>(P.PA + (size_t)(it & 1) * RED_MAXBLOCKS, P.nA, 1, scr);
-  const double alpha_d = st->rz_hist[it & 1] / pAp;
+  const double alpha_d = ((it & 1) ? ss.rz_hist[1] : ss.rz_hist[0]) / pAp;
   if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha_hist[it & 1] = alpha_d;
   const T alpha = (T)alpha_d;
   double rr = 0.0, rzn = 0.0;
@@ -166,8 +167,8 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
 
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P, int k,
-                       hipStream_t s) {
-  hipLaunchKernelGGL((k_pcg_fusedB<T, 4>), dim3(P.nB), dim3(BS), 0, s, n, st, r, Ap, invd, P, k);
+                       hipStream_t s, const LaunchEvents *ev) {
+  launch_ev((k_pcg_fusedB<T, 4>), P.nB, BS, s, ev, n, st, r, Ap, invd, P, k);
   ICX_CHECK_LAUNCH();
 }
 
@@ -250,7 +251,7 @@ void launch_refine_r(long long n, SolverState *st, const double *b, const double
   template void launch_pcg_setup<T>(long long, SolverState *, const T *, T *, const T *, PcgPartials,    \
                                     double, int, double, hipStream_t);                                   \
   template void launch_pcg_fusedB<T>(long long, SolverState *, T *, const T *, const T *, PcgPartials, int, \
-                                     hipStream_t);
+                                     hipStream_t, const LaunchEvents *);
 ICX_INST_PCG(double)
 ICX_INST_PCG(float)
 
