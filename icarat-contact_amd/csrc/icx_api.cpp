@@ -22,6 +22,10 @@
 
 using namespace icx;
 
+struct BatchRun;
+static void batch_delete(BatchRun *b);
+static void batch_remove(icx_ctx *c, icx_job *j);
+
 struct icx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -38,6 +42,7 @@ struct icx_ctx {
   int ev_used = 0;
   double prof_ms = 0.0;
   long long prof_launches = 0;
+  BatchRun *batch = nullptr;    // submitted solves (icx_submit / icx_poll), created on first use
 };
 
 namespace {
@@ -403,9 +408,12 @@ struct MMAWork {
 };
 struct OptWork {
   bool ready = false;
+  bool in_step = false;  // between icx_opt_step_begin and icx_opt_step_end
   icx_opt_params par;
   int step = 0;
   double V0 = 0.0;
+  double beta = 1.0;     // projection sharpness of the current step
+  std::chrono::steady_clock::time_point t0;
   DBuf<double> x, fil, thr, dthr, dfds, dgds, tmp, tmp2;
 };
 
@@ -444,6 +452,19 @@ struct icx_job {
     gr_helm.reset();
   }
   MixedWork mw;
+  // state of a submitted (batched) mixed solve
+  struct Submitted {
+    bool active = false;     // in the context's batch
+    int was = 0;             // stop code that triggered the pending replacement
+    bool replacing = false;
+    double tol = 0.0;
+    int maxit = 0;
+    int refinements = 0;
+    std::chrono::steady_clock::time_point t0;
+    bool has_info = false;   // info / rc describe the last finished submitted solve
+    icx_solve_info info{};
+    icx_status rc = ICX_OK;
+  } sub;
   FusedWork fw;
   HelmWork hw;
   ConeWork cw;
@@ -509,14 +530,6 @@ icx_status build_rhs(icx_job *j) {
   return ICX_OK;
 }
 
-// sets the index of the first iteration of the next launch chunk
-icx_status set_iter_base(icx_job *j, int base) {
-  icx_ctx *c = j->ctx;
-  c->h_int[0] = base;
-  HIPCHK(c, hipMemcpyAsync(&j->d_state.p->iter_base, c->h_int, sizeof(int), hipMemcpyHostToDevice, S(j)));
-  return ICX_OK;
-}
-
 inline int poll_every(long long N) { return N > 2000000 ? 16 : (N > 100000 ? 12 : 8); }
 
 // Launches one chunk of iterations: replays the job's captured graph, capturing it on
@@ -562,19 +575,16 @@ icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const 
   icx_status rc = read_state(j, st);
   if (rc) return rc;
   const int every = poll_every(N);
-  int base = 0;
   auto chunk = [&]() {
     for (int k = 0; k < every; ++k) {
       matvec(w.p.p, w.t.p);  // tmp = A p
-      launch_eigen_iteration_tail(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, P, k, S(j));
+      launch_eigen_iteration_tail(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, P, S(j));
     }
   };
   while (!st.done) {
     if ((rc = run_chunk(j, graph, chunk))) return rc;
     rc = read_state(j, st);
     if (rc) return rc;
-    base += every;
-    if (!st.done && (rc = set_iter_base(j, base))) return rc;
   }
   const bool conv = st.done == 1;
   info.iterations = conv ? st.stop_iter : st.iter;
@@ -656,6 +666,8 @@ icx_status icx_ctx_destroy(icx_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   while (!ctx->jobs.empty()) icx_job_destroy(ctx->jobs.back());
+  batch_delete(ctx->batch);
+  ctx->batch = nullptr;
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->h_state) (void)hipHostFree(ctx->h_state);
@@ -736,6 +748,7 @@ icx_status icx_job_destroy(icx_job *j) {
   c->jobs.erase(it);
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  batch_remove(c, j);
   delete j;
   return ICX_OK;
 }
@@ -1023,7 +1036,8 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
 
 // Fused forms: kernel A (direction, operator, deferred x update, p.Ap) and kernel B
 // (residual update, |r|^2, r.z) per iteration; the iteration index lives on the device
-// (iter_base + position in the chunk), so the host only launches chunks and polls.
+// (SolverState::iter, advanced by kernel B), so every chunk is the same launch sequence
+// and the host only launches chunks and polls.
 // FP64 fused form:
 static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info &info) {
   icx_ctx *c = j->ctx;
@@ -1041,17 +1055,16 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
   icx_status rc = read_state(j, st);
   if (rc) return rc;
   const int every = poll_every(N);
-  int base = 0;
   auto chunk = [&]() {
     for (int k = 0; k < every; ++k) {
       {
         ProfScope ps(c, ICX_PROF_FUSED_A);
         launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p,
-                                      w.Ap.p, j->d_u.p, j->d_state.p, P, k, S(j), ps.ev());
+                                      w.Ap.p, j->d_u.p, j->d_state.p, P, S(j), ps.ev());
       }
       {
         ProfScope ps(c, ICX_PROF_FUSED_B);
-        launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j), ps.ev());
+        launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, S(j), ps.ev());
       }
     }
   };
@@ -1059,8 +1072,6 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
     if ((rc = run_chunk(j, j->gr_fused, chunk))) return rc;
     rc = read_state(j, st);
     if (rc) return rc;
-    base += every;
-    if (!st.done && (rc = set_iter_base(j, base))) return rc;
   }
   if (st.done != 1) launch_pcg_flush(N, j->d_state.p, j->d_u.p, w.pa.p, w.pb.p, S(j));
   rc = read_state(j, st);
@@ -1078,14 +1089,21 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
 // by a factor 10 since the last replacement (or meets the tolerance), x_hi += x_lo
 // and r = b - K x_hi is recomputed with the FP64 operator; the search direction is
 // kept.  Convergence is declared on the FP64 residual only.
-static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info &info) {
+// Mixed form, in pieces shared by icx_solve and the batched path (icx_submit/icx_poll).
+static PcgPartials mixed_parts(icx_job *j) {
+  return j->parts(elastic_fusedA_blocks<float>(j->g), pcg_stream_blocks(j->nvec()));
+}
+
+// Set-up of a mixed solve: FP32 operands, x = 0, r = b, partials; st receives the state
+// after the set-up (done = 1 at once for a zero right-hand side).
+static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st) {
   icx_ctx *c = j->ctx;
   const long long N = j->nvec();
   MixedWork &w = j->mw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N)); HIPCHK(c, w.Ap.alloc(N));
   HIPCHK(c, w.xlo.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.E.alloc(j->g.ne)); HIPCHK(c, w.Kx.alloc(N));
   HIPCHK(c, w.b32.alloc(N));
-  const PcgPartials P = j->parts(elastic_fusedA_blocks<float>(j->g), pcg_stream_blocks(N));
+  const PcgPartials P = mixed_parts(j);
   launch_convert_f64_f32(j->g.ne, j->d_E.p, w.E.p, S(j));
   launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, nullptr, w.invd.p, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
@@ -1099,7 +1117,6 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
   double bb;
   icx_status rc = read_scalars(j, 1, &bb);
   if (rc) return rc;
-  SolverState st;
   rc = read_state(j, st);
   if (rc) return rc;
   st.rhsNorm2 = bb;
@@ -1107,49 +1124,188 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
   if (bb == 0.0) st.done = 1;
   HIPCHK(c, hipMemcpyAsync(j->d_state.p, &st, sizeof(SolverState), hipMemcpyHostToDevice, S(j)));
   HIPCHK(c, hipStreamSynchronize(S(j)));
-  const int every = poll_every(N);
+  return ICX_OK;
+}
+
+// one iteration of the mixed form for one job: kernel A, kernel B
+static void mixed_iteration(icx_job *j) {
+  icx_ctx *c = j->ctx;
+  MixedWork &w = j->mw;
+  const PcgPartials P = mixed_parts(j);
+  {
+    ProfScope ps(c, ICX_PROF_FUSED_A);
+    launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
+                                 w.xlo.p, j->d_state.p, P, S(j), ps.ev());
+  }
+  {
+    ProfScope ps(c, ICX_PROF_FUSED_B);
+    launch_pcg_fusedB<float>(j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, S(j), ps.ev());
+  }
+}
+
+// residual replacement after a stop of the inner loop: x_hi += x_lo + alpha p;
+// r = b - K x_hi (FP64); the finalisation sets done = 1 (converged), 0 (resume at the
+// stopped iteration) or 2 (iteration limit)
+static void mixed_replace(icx_job *j) {
+  const long long N = j->nvec();
+  MixedWork &w = j->mw;
+  const PcgPartials P = mixed_parts(j);
+  launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pb.p, S(j));
+  launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
+  launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, S(j));
+}
+
+static void mixed_info(const SolverState &st, int refinements, icx_solve_info &info) {
+  info.iterations = st.stop_iter;
+  info.matvecs = st.stop_iter + refinements;
+  info.fp64_matvecs = refinements;
+  info.refinements = refinements;
+  info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.true_rr / st.rhsNorm2) : 0.0;
+}
+
+static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info &info) {
+  SolverState st;
+  icx_status rc = mixed_begin(j, tol, maxit, st);
+  if (rc) return rc;
+  const int every = poll_every(j->nvec());
   int refinements = 0;
   auto chunk = [&]() {
-    for (int k = 0; k < every; ++k) {
-      {
-        ProfScope ps(c, ICX_PROF_FUSED_A);
-        launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
-                                     w.xlo.p, j->d_state.p, P, k, S(j), ps.ev());
-      }
-      {
-        ProfScope ps(c, ICX_PROF_FUSED_B);
-        launch_pcg_fusedB<float>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, k, S(j), ps.ev());
-      }
-    }
+    for (int k = 0; k < every; ++k) mixed_iteration(j);
   };
   while (!st.done || st.done >= 2) {
     if (!st.done) {
       if ((rc = run_chunk(j, j->gr_mixed, chunk))) return rc;
       rc = read_state(j, st);
       if (rc) return rc;
-      if (!st.done) {
-        if ((rc = set_iter_base(j, st.iter_base + every))) return rc;
-        st.iter_base += every;
-        continue;
-      }
+      if (!st.done) continue;
     }
-    // replacement: x_hi += x_lo + alpha p; r = b - K x_hi (FP64)
     const int was = st.done;
-    launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pb.p, S(j));
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
-    launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, S(j));
+    mixed_replace(j);
     ++refinements;
     rc = read_state(j, st);
     if (rc) return rc;
     if (st.done == 1) break;
     if (was == 2 || st.done == 2) { st.done = 2; break; }
   }
-  info.iterations = st.stop_iter;
-  info.matvecs = st.stop_iter + refinements;
-  info.fp64_matvecs = refinements;
-  info.refinements = refinements;
-  info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.true_rr / st.rhsNorm2) : 0.0;
+  mixed_info(st, refinements, info);
   return st.done == 1 ? ICX_OK : ICX_ERR_NOCONV;
+}
+
+// -------------------------------------------------------------------------------------
+// Batched mixed solves.  Submitted solves of any jobs advance together: every chunk is
+// `every` iterations of [kernel A for the hexa8 jobs, kernel A for the quad4 jobs,
+// kernel B for all jobs], one launch each (Batch tables in device memory), captured
+// once per job set as a HIP graph.  Each job's blocks compute exactly what its own
+// launches would (same tiles, same partial sums), so a job's result is bit-identical
+// alone and in any mix; a job whose solve has stopped exits at once in every launch.
+// After each chunk the states of all jobs are gathered with one copy; replacements
+// run per job (FP64 residual), finished jobs leave the set.
+// -------------------------------------------------------------------------------------
+struct BatchRun {
+  std::vector<icx_job *> active;
+  bool dirty = true;
+  DBuf<OpArgs<float>> a3, a2;
+  DBuf<BArgs<float>> ab;
+  DBuf<int> off3, off2, offb;
+  DBuf<SolverState *> sptr;
+  DBuf<SolverState> sdev;
+  SolverState *h_states = nullptr;  // pinned
+  int h_cap = 0;
+  int n3 = 0, n2 = 0, nb3 = 0, nb2 = 0, nbb = 0, every = 8;
+  ChunkGraph graph;
+  ~BatchRun() {
+    if (h_states) (void)hipHostFree(h_states);
+  }
+};
+
+static void batch_delete(BatchRun *b) { delete b; }
+
+static int pad8(int n) { return (n + 7) & ~7; }
+
+static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
+  std::vector<OpArgs<float>> a3, a2;
+  std::vector<BArgs<float>> ab;
+  std::vector<int> o3{0}, o2{0}, ob{0};
+  std::vector<SolverState *> sp;
+  long long maxN = 0;
+  for (icx_job *j : B.active) {
+    MixedWork &w = j->mw;
+    const PcgPartials P = mixed_parts(j);
+    OpArgs<float> a;
+    const int nb = fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
+                                      w.xlo.p, j->d_state.p, P, a);
+    if (j->g.ndim == 3) { a3.push_back(a); o3.push_back(o3.back() + pad8(nb)); }
+    else { a2.push_back(a); o2.push_back(o2.back() + pad8(nb)); }
+    ab.push_back(BArgs<float>{j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P});
+    ob.push_back(ob.back() + pad8(P.nB));
+    sp.push_back(j->d_state.p);
+    maxN = std::max(maxN, j->nvec());
+  }
+  const int n = (int)B.active.size();
+  B.n3 = (int)a3.size(); B.n2 = (int)a2.size();
+  B.nb3 = o3.back(); B.nb2 = o2.back(); B.nbb = ob.back();
+  B.every = poll_every(maxN);
+  auto up = [&](auto &dbuf, const auto &vec) -> icx_status {
+    if (vec.empty()) return ICX_OK;
+    HIPCHK(c, dbuf.alloc(vec.size()));
+    HIPCHK(c, hipMemcpyAsync(dbuf.p, vec.data(), vec.size() * sizeof(vec[0]), hipMemcpyHostToDevice, c->stream));
+    return ICX_OK;
+  };
+  icx_status rc;
+  if ((rc = up(B.a3, a3)) || (rc = up(B.a2, a2)) || (rc = up(B.ab, ab)) || (rc = up(B.off3, o3)) ||
+      (rc = up(B.off2, o2)) || (rc = up(B.offb, ob)) || (rc = up(B.sptr, sp)))
+    return rc;
+  HIPCHK(c, B.sdev.alloc(std::max(n, 1)));
+  if (B.h_cap < n) {
+    if (B.h_states) (void)hipHostFree(B.h_states);
+    B.h_states = nullptr;
+    B.h_cap = 0;
+    HIPCHK(c, hipHostMalloc((void **)&B.h_states, n * sizeof(SolverState), hipHostMallocDefault));
+    B.h_cap = n;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  B.graph.reset();
+  B.dirty = false;
+  return ICX_OK;
+}
+
+static void batch_chunk(icx_ctx *c, BatchRun &B) {
+  for (int k = 0; k < B.every; ++k) {
+    launch_fusedA_batch<float>(3, B.a3.p, B.off3.p, B.n3, B.nb3, c->stream);
+    launch_fusedA_batch<float>(2, B.a2.p, B.off2.p, B.n2, B.nb2, c->stream);
+    launch_pcg_fusedB_batch<float>(B.ab.p, B.offb.p, (int)B.active.size(), B.nbb, c->stream);
+  }
+}
+
+static icx_status batch_states(icx_ctx *c, BatchRun &B) {
+  const int n = (int)B.active.size();
+  launch_gather_states(B.sptr.p, B.sdev.p, n, c->stream);
+  HIPCHK(c, hipMemcpyAsync(B.h_states, B.sdev.p, n * sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ICX_OK;
+}
+
+static void batch_finish(icx_job *j, const SolverState &st, icx_status rc) {
+  auto &b = j->sub;
+  b.info = icx_solve_info{};
+  mixed_info(st, b.refinements, b.info);
+  b.info.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - b.t0).count();
+  b.rc = rc;
+  b.has_info = true;
+  b.active = false;
+  b.replacing = false;
+  j->u_valid = true;
+}
+
+static void batch_remove(icx_ctx *c, icx_job *j) {
+  if (!c->batch) return;
+  auto &v = c->batch->active;
+  auto it = std::find(v.begin(), v.end(), j);
+  if (it != v.end()) {
+    v.erase(it);
+    c->batch->dirty = true;
+  }
+  j->sub.active = false;
 }
 
 icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx_solve_info *info) {
@@ -1598,46 +1754,52 @@ icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0)
 
 extern "C" {
 
-icx_status icx_opt_step(icx_job *j, icx_iter_record *rec) {
+icx_status icx_opt_step_begin(icx_job *j) {
   if (!j) return ICX_ERR_ARG;
   OptWork &o = j->opt;
   if (!o.ready) return fail(j->ctx, ICX_ERR_STATE, "icx_opt_init first");
+  if (o.in_step) return fail(j->ctx, ICX_ERR_STATE, "design step already begun");
   icx_ctx *c = j->ctx;
-  const auto t0 = std::chrono::steady_clock::now();
+  o.t0 = std::chrono::steady_clock::now();
   const icx_opt_params &P = o.par;
   const long long ne = j->g.ne;
   const int k = ++o.step;
-  icx_iter_record r{};
-  r.step = k;
-  icx_status rc = ICX_OK;
   // filter
   if (P.filter == ICX_FILTER_CONE) launch_cone_forward(j->g, stencil(j), j->cw.aa.p, o.x.p, o.fil.p, S(j));
   else if (P.filter == ICX_FILTER_HELMHOLTZ) {
-    rc = helm_apply_dev(j, 0, P.helm_tol, o.x.p, o.fil.p, nullptr);
+    icx_status rc = helm_apply_dev(j, 0, P.helm_tol, o.x.p, o.fil.p, nullptr);
     if (rc) return rc;
   } else HIPCHK(c, hipMemcpyAsync(o.fil.p, o.x.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
-  // projection, beta = min(2^(k/40), 16)
-  double beta = std::min(std::pow(2, (int)(k / 40)), 16.0);
-  if (P.projection) launch_project(ne, beta, 0.5, o.fil.p, o.thr.p, o.dthr.p, S(j));
+  // projection, beta = min(2^(k/40), 16) (opt_macro_oc.cpp:94, opt_mma_Helmholtz.cpp:90)
+  o.beta = std::min(std::pow(2, (int)(k / 40)), 16.0);
+  if (P.projection) launch_project(ne, o.beta, 0.5, o.fil.p, o.thr.p, o.dthr.p, S(j));
   else HIPCHK(c, hipMemcpyAsync(o.thr.p, o.fil.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
   // design -> E
   HIPCHK(c, hipMemcpyAsync(j->d_design.p, o.thr.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
   launch_simp(ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
   j->design_set = true;
-  icx_solve_info info{};
-  rc = build_rhs(j);
-  if (rc) return rc;
-  if (P.form == ICX_SOLVE_PLAIN) rc = solve_plain(j, P.tol, P.maxit, info);
-  else if (P.form == ICX_SOLVE_FUSED) rc = solve_fused(j, P.tol, P.maxit, info);
-  else rc = solve_mixed(j, P.tol, P.maxit, info);
-  if (rc && rc != ICX_ERR_NOCONV) return rc;
+  o.in_step = true;
+  return ICX_OK;
+}
+
+icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_record *rec) {
+  if (!j) return ICX_ERR_ARG;
+  OptWork &o = j->opt;
+  if (!o.in_step) return fail(j->ctx, ICX_ERR_STATE, "icx_opt_step_begin first");
+  o.in_step = false;
+  icx_ctx *c = j->ctx;
+  const icx_opt_params &P = o.par;
+  const long long ne = j->g.ne;
+  const int k = o.step;
+  icx_iter_record r{};
+  r.step = k;
+  r.cg_iterations = info ? info->iterations : 0;
   j->u_valid = true;
-  r.cg_iterations = info.iterations;
   // objective and volume
   launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
   launch_volume(ne, 0, 0.0, 0.5, o.thr.p, j->volume, j->d_scalars.p + 1, j->red(), S(j));
   double sc[2];
-  rc = read_scalars(j, 2, sc);
+  icx_status rc = read_scalars(j, 2, sc);
   if (rc) return rc;
   r.objective = sc[0];
   r.volume = sc[1];
@@ -1672,16 +1834,33 @@ icx_status icx_opt_step(icx_job *j, icx_iter_record *rec) {
   int steps = 0;
   if (P.driver == 0) {
     rc = oc_update_dev(j, o.x.p, o.tmp.p, o.V0, o.tmp2.p, P.xmin, P.xmax, P.move, P.filter,
-                       P.projection ? beta : 0.0, 0.5, o.fil.p, o.dfds.p, &steps);
+                       P.projection ? o.beta : 0.0, 0.5, o.fil.p, o.dfds.p, &steps);
   } else {
     rc = mma_update_dev(j, o.x.p, o.tmp.p, r.constraint, o.tmp2.p, P.xmin, P.xmax, &steps);
   }
   if (rc) return rc;
   r.update_steps = steps;
   HIPCHK(c, hipStreamSynchronize(S(j)));
-  r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  r.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - o.t0).count();
   if (rec) *rec = r;
   return ICX_OK;
+}
+
+icx_status icx_opt_step(icx_job *j, icx_iter_record *rec) {
+  icx_status rc = icx_opt_step_begin(j);
+  if (rc) return rc;
+  const icx_opt_params &P = j->opt.par;
+  icx_solve_info info{};
+  rc = build_rhs(j);
+  if (rc) return rc;
+  if (P.form == ICX_SOLVE_PLAIN) rc = solve_plain(j, P.tol, P.maxit, info);
+  else if (P.form == ICX_SOLVE_FUSED) rc = solve_fused(j, P.tol, P.maxit, info);
+  else rc = solve_mixed(j, P.tol, P.maxit, info);
+  if (rc && rc != ICX_ERR_NOCONV) {
+    j->opt.in_step = false;
+    return rc;
+  }
+  return icx_opt_step_end(j, &info, rec);
 }
 
 icx_status icx_opt_get_design(const icx_job *jc, double *x) {
@@ -1694,12 +1873,151 @@ icx_status icx_opt_get_design(const icx_job *jc, double *x) {
   return ICX_OK;
 }
 
+icx_status icx_submit(icx_job *j, double tol, int maxit) {
+  if (!j) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  if (j->numeq < 0 || !j->design_set)
+    return fail(c, ICX_ERR_STATE, "icx_submit needs icx_make_dof and icx_set_design");
+  if (!(tol > 0.0) || maxit < 1) return fail(c, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
+  if (j->sub.active) return fail(c, ICX_ERR_STATE, "job already has a submitted solve");
+  (void)hipSetDevice(c->device);
+  if (!c->batch) c->batch = new BatchRun();
+  auto &b = j->sub;
+  b.t0 = std::chrono::steady_clock::now();
+  b.tol = tol;
+  b.maxit = maxit;
+  b.refinements = 0;
+  b.replacing = false;
+  b.has_info = false;
+  icx_status rc = build_rhs(j);
+  if (rc) return rc;
+  SolverState st;
+  rc = mixed_begin(j, tol, maxit, st);
+  if (rc) return rc;
+  b.active = true;
+  c->batch->active.push_back(j);
+  c->batch->dirty = true;
+  return ICX_OK;
+}
+
+icx_status icx_poll(icx_ctx *c, int chunks, icx_job **finished, int cap, int *nfinished) {
+  if (!c || !nfinished || (cap > 0 && !finished)) return ICX_ERR_ARG;
+  *nfinished = 0;
+  if (!c->batch || c->batch->active.empty()) return ICX_OK;
+  (void)hipSetDevice(c->device);
+  BatchRun &B = *c->batch;
+  std::vector<icx_job *> done;
+  icx_status rc = ICX_OK;
+  for (int ch = 0; ch < std::max(chunks, 1) && done.empty() && !B.active.empty(); ++ch) {
+    if (B.dirty && (rc = batch_rebuild(c, B))) return rc;
+    // states first: a solve may already be stopped (zero load, converged start)
+    if ((rc = batch_states(c, B))) return rc;
+    bool runnable = false;
+    for (size_t q = 0; q < B.active.size(); ++q) runnable |= B.h_states[q].done == 0;
+    if (runnable) {
+      auto chunk = [&]() { batch_chunk(c, B); };
+      if (!c->use_graphs || c->prof_kernel) {
+        chunk();
+      } else {
+        if (!B.graph.exec) {
+          hipGraph_t graph = nullptr;
+          HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+          chunk();
+          HIPCHK(c, hipStreamEndCapture(c->stream, &graph));
+          const hipError_t e = hipGraphInstantiate(&B.graph.exec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          if (e != hipSuccess) {
+            B.graph.exec = nullptr;
+            c->use_graphs = false;
+          }
+        }
+        if (B.graph.exec) HIPCHK(c, hipGraphLaunch(B.graph.exec, c->stream));
+        else chunk();
+      }
+      if ((rc = batch_states(c, B))) return rc;
+    }
+    // stopped solves: converged at set-up, or a residual replacement is due
+    bool replaced = false;
+    for (size_t q = 0; q < B.active.size(); ++q) {
+      icx_job *j = B.active[q];
+      const SolverState &st = B.h_states[q];
+      if (st.done == 1) {
+        batch_finish(j, st, ICX_OK);
+        done.push_back(j);
+      } else if (st.done >= 2) {
+        j->sub.was = st.done;
+        j->sub.replacing = true;
+        mixed_replace(j);
+        ++j->sub.refinements;
+        replaced = true;
+      }
+    }
+    if (replaced) {
+      if ((rc = batch_states(c, B))) return rc;
+      for (size_t q = 0; q < B.active.size(); ++q) {
+        icx_job *j = B.active[q];
+        if (!j->sub.replacing) continue;
+        j->sub.replacing = false;
+        const SolverState &st = B.h_states[q];
+        if (st.done == 1) {
+          batch_finish(j, st, ICX_OK);
+          done.push_back(j);
+        } else if (j->sub.was == 2 || st.done == 2) {
+          batch_finish(j, st, ICX_ERR_NOCONV);
+          done.push_back(j);
+        }
+      }
+    }
+    for (icx_job *j : done) {
+      auto it = std::find(B.active.begin(), B.active.end(), j);
+      if (it != B.active.end()) B.active.erase(it);
+      B.dirty = true;
+    }
+  }
+  const int n = (int)done.size();
+  for (int q = 0; q < n && q < cap; ++q) finished[q] = done[q];
+  *nfinished = n;
+  if (n > cap) return fail(c, ICX_ERR_ARG, "%d solves finished, room for %d", n, cap);
+  return ICX_OK;
+}
+
+icx_status icx_job_last_info(const icx_job *j, icx_solve_info *info) {
+  if (!j || !info) return ICX_ERR_ARG;
+  if (!j->sub.has_info) return fail(j->ctx, ICX_ERR_STATE, "no finished submitted solve");
+  *info = j->sub.info;
+  return j->sub.rc;
+}
+
 icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, double tol, int maxit,
                            icx_solve_info *infos) {
   if (!ctx || (!jobs && njobs)) return ICX_ERR_ARG;
   icx_status worst = ICX_OK;
+  if (form != ICX_SOLVE_MIXED) {
+    for (int i = 0; i < njobs; ++i) {
+      icx_status rc = icx_solve(jobs[i], form, tol, maxit, nullptr, infos ? &infos[i] : nullptr);
+      if (rc && rc != ICX_ERR_NOCONV) return rc;
+      if (rc) worst = rc;
+    }
+    return worst;
+  }
   for (int i = 0; i < njobs; ++i) {
-    icx_status rc = icx_solve(jobs[i], form, tol, maxit, nullptr, infos ? &infos[i] : nullptr);
+    if (!jobs[i] || jobs[i]->ctx != ctx) return fail(ctx, ICX_ERR_ARG, "job %d is not a job of this context", i);
+    icx_status rc = icx_submit(jobs[i], tol, maxit);
+    if (rc) return rc;
+  }
+  int left = njobs;
+  std::vector<icx_job *> fin(njobs > 0 ? njobs : 1);
+  while (left > 0) {
+    int nf = 0;
+    icx_status rc = icx_poll(ctx, 1 << 20, fin.data(), njobs, &nf);
+    if (rc) return rc;
+    if (nf == 0 && (!ctx->batch || ctx->batch->active.empty())) break;
+    left -= nf;
+  }
+  for (int i = 0; i < njobs; ++i) {
+    icx_solve_info inf{};
+    icx_status rc = icx_job_last_info(jobs[i], &inf);
+    if (infos) infos[i] = inf;
     if (rc && rc != ICX_ERR_NOCONV) return rc;
     if (rc) worst = rc;
   }

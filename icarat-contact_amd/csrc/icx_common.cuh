@@ -22,6 +22,26 @@ inline void launch_ev(F kernel, int blocks, int threads, hipStream_t s, const La
 // it also tells the compiler's wait insertion that nothing older is pending.
 __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Explicit fused multiply-add.  Expressions with two products (a*b + c*d) are written
+// with it, so that every instantiation of a kernel rounds them identically (with
+// contraction on, the compiler may otherwise fuse either product).
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Job and job-local block of a batched launch (see Batch in icx_internal.h); the
+// search is uniform per block.
+__device__ __forceinline__ void batch_block(const int *__restrict__ off, int njobs, int &j, int &blk) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  j = lo;
+  blk = b - off[lo];
+}
+
 // ---------------------------------------------------------------------------------
 // reductions
 //
@@ -91,6 +111,34 @@ __device__ __forceinline__ void haar_inv8(T (&v)[8]) {
 #pragma unroll
   for (int q = 0; q < 8; q += 2) { T a = v[q], b = v[q + 1]; v[q] = a - b; v[q + 1] = a + b; }
 }
+// E * haar_inv8(v), the scale folded into the first stage: two products per pair
+// become one product and two fused multiply-adds
+template <typename T>
+__device__ __forceinline__ void haar_inv8_scaled(T (&v)[8], T E) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T t = E * v[q + 4];
+    const T a = v[q];
+    v[q] = fma_t(E, a, -t);
+    v[q + 4] = fma_t(E, a, t);
+  }
+#pragma unroll
+  for (int q : {0, 1, 4, 5}) { T a = v[q], b = v[q + 2]; v[q] = a - b; v[q + 2] = a + b; }
+#pragma unroll
+  for (int q = 0; q < 8; q += 2) { T a = v[q], b = v[q + 1]; v[q] = a - b; v[q + 1] = a + b; }
+}
+template <typename T>
+__device__ __forceinline__ void haar_inv4_scaled(T (&v)[4], T E) {
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const T t = E * v[q + 2];
+    const T a = v[q];
+    v[q] = fma_t(E, a, -t);
+    v[q + 2] = fma_t(E, a, t);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q += 2) { T a = v[q], b = v[q + 1]; v[q] = a - b; v[q + 1] = a + b; }
+}
 template <typename T>
 __device__ __forceinline__ void haar_fwd4(T (&v)[4]) {
 #pragma unroll
@@ -136,13 +184,18 @@ __device__ __forceinline__ void elastic_hex8(T (&v)[3][8], const ElasticOp<T> &o
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       T acc = T(0);
+      bool first = true;
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc)
-        if (blk_nz3(chi, c, cc)) acc += op.blk[chi][c < cc ? c : cc][c < cc ? cc : c] * m[cc];
-      o[c][chi ^ (1 << c)] = E * acc;
+        if (blk_nz3(chi, c, cc)) {
+          const T k = op.blk[chi][c < cc ? c : cc][c < cc ? cc : c];
+          acc = first ? k * m[cc] : fma_t(k, m[cc], acc);
+          first = false;
+        }
+      o[c][chi ^ (1 << c)] = acc;
     }
   }
-  haar_inv8(o[0]); haar_inv8(o[1]); haar_inv8(o[2]);
+  haar_inv8_scaled(o[0], E); haar_inv8_scaled(o[1], E); haar_inv8_scaled(o[2], E);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -159,13 +212,18 @@ __device__ __forceinline__ void elastic_quad4(T (&v)[2][4], const ElasticOp<T> &
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       T acc = T(0);
+      bool first = true;
 #pragma unroll
       for (int cc = 0; cc < 2; ++cc)
-        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c < cc ? c : cc][c < cc ? cc : c] * m[cc];
-      o[c][chi ^ (1 << c)] = E * acc;
+        if (blk_nz2(chi, c, cc)) {
+          const T k = op.blk[chi][c < cc ? c : cc][c < cc ? cc : c];
+          acc = first ? k * m[cc] : fma_t(k, m[cc], acc);
+          first = false;
+        }
+      o[c][chi ^ (1 << c)] = acc;
     }
   }
-  haar_inv4(o[0]); haar_inv4(o[1]);
+  haar_inv4_scaled(o[0], E); haar_inv4_scaled(o[1], E);
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll

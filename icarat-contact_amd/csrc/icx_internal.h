@@ -47,8 +47,8 @@ struct SolverState {
   double alpha_owed;   // step still owed to x at the stop (fused/mixed forms)
   int done;            // 0 running, 1 converged, 2 maxit, 3 recursive residual met, 4 replace
   int maxit;
-  int iter_base;       // index of the first iteration of the next launch chunk
-  int iter;            // progress: iterations completed
+  int iter_cur;        // iteration in flight: published by the first kernel of an iteration
+  int iter;            // next iteration (= iterations completed), advanced by its last kernel
   int stop_iter;       // iteration index at which `done` was set
   int flush_iter;      // iteration whose owed x update was already applied
   double rz_hist[2];   // fused/mixed forms: r_i.z_i, stored by kernel A of iteration i in slot i & 1
@@ -76,6 +76,32 @@ struct LaunchEvents {
 };
 
 // ---- structured operators (icx_operator.hip) -------------------------------------
+// Arguments of one launch of an elastic operator kernel (one job).
+template <typename T> struct OpArgs {
+  Grid g;
+  int TK, tiles_x, tiles_y, tiles_z;
+  ElasticOp<T> op;
+  const T *E;
+  const uint8_t *mask;
+  const T *in;        // MODE 0: x; MODE 1: r
+  const T *invd;      // MODE 1
+  T *P0, *P1;         // MODE 1: direction buffers, p_i lives in P[i & 1]
+  T *y;               // K x (MODE 0) / K p_i (MODE 1)
+  T *x;               // MODE 1: solution accumulator (deferred update)
+  SolverState *st;    // nullable in MODE 0
+  PcgPartials P;      // MODE 1
+};
+
+// A batched launch runs one kernel phase for several jobs: global block b belongs to
+// job j with off[j] <= b < off[j+1] (ranges padded to multiples of 8, so every job keeps
+// the XCD placement of its own launch); each job computes exactly what its own launch
+// would.
+template <typename A> struct Batch {
+  const A *args;
+  const int *off;     // njobs + 1 entries
+  int njobs;
+};
+
 // y = K(E) x on the node grid (SoA).  mask: per-node bit c = constrained (output
 // forced to zero); may be null.  st may be null; otherwise the kernel exits at once
 // when st->done is set.
@@ -84,13 +110,21 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
                            const T *x, T *y, const SolverState *st, hipStream_t s,
                            const LaunchEvents *ev = nullptr);
 
-// Fused kernel A of iteration i = st->iter_base + k: scalars from the partials,
+// Fused kernel A of iteration i = st->iter (published as st->iter_cur): scalars from the partials,
 // p_i = invd r + beta_i p_{i-1} (P[i&1] <- ..., P[(i-1)&1] read), Ap_i = K p_i,
 // x += alpha_{i-1} p_{i-1}, partial p_i.Ap_i -> PA[i&1].
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
                            const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
-                           PcgPartials P, int k, hipStream_t s, const LaunchEvents *ev = nullptr);
+                           PcgPartials P, hipStream_t s, const LaunchEvents *ev = nullptr);
+// kernel A's arguments for one job; returns its block count
+template <typename T>
+int fusedA_args(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask, const T *r, const T *invd,
+                T *P0, T *P1, T *Ap, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a);
+// kernel A for a batch of jobs of one dimension (args and off in device memory)
+template <typename T>
+void launch_fusedA_batch(int ndim, const OpArgs<T> *args, const int *off, int njobs, int total_blocks,
+                         hipStream_t s);
 // number of blocks of kernel A for this grid (its partial count)
 template <typename T>
 int elastic_fusedA_blocks(const Grid &g);
@@ -106,10 +140,24 @@ int pcg_stream_blocks(long long n);
 template <typename T>
 void launch_pcg_setup(long long n, SolverState *st, const T *b, T *r, const T *invd, PcgPartials P,
                       double tol, int maxit, double delta2, hipStream_t s);
+// Arguments of kernel B for one job (its grid is P.nB blocks).
+template <typename T> struct BArgs {
+  long long n;
+  SolverState *st;
+  T *r;
+  const T *Ap;
+  const T *invd;
+  PcgPartials P;
+};
+// copies the states *src[j] of n jobs to dst[j]
+void launch_gather_states(SolverState *const *src, SolverState *dst, int n, hipStream_t s);
+// kernel B for a batch of jobs (args and off in device memory)
+template <typename T>
+void launch_pcg_fusedB_batch(const BArgs<T> *args, const int *off, int njobs, int total_blocks, hipStream_t s);
 // fused kernel B of iteration i: alpha_i = rz_i / pAp_i; r -= alpha_i Ap; (|r|^2, r.z) -> PB[i&1]
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
-                       int k, hipStream_t s, const LaunchEvents *ev = nullptr);
+                       hipStream_t s, const LaunchEvents *ev = nullptr);
 // fused FP64 form at the stop: x += alpha_owed p_{stop-1}
 void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0, const double *P1,
                       hipStream_t s);
@@ -125,7 +173,7 @@ void launch_refine_r(long long n, SolverState *st, const double *b, const double
 // (p.tmp, x += a p, r -= a tmp, |r|^2, z = M^-1 r, r.z, p = z + beta p)
 void launch_eigen_iteration_tail(long long n, SolverState *st, double *x, double *r, double *p,
                                  const double *t, double *z, const double *invd, PcgPartials P,
-                                 int k, hipStream_t s);
+                                 hipStream_t s);
 // set-up of the Eigen loop: r = b, p = z = invd r; partials (b.b) -> PN[1], (r.z) -> PZ[1]
 void launch_eigen_setup(long long n, SolverState *st, const double *b, double *r, double *p,
                         const double *invd, PcgPartials P, double tol, int maxit, hipStream_t s);

@@ -4,32 +4,17 @@
 
 namespace icx {
 
-template <typename T> struct OpArgs {
-  Grid g;
-  int TK, tiles_x, tiles_y, tiles_z;
-  ElasticOp<T> op;
-  const T *E;
-  const uint8_t *mask;
-  const T *in;        // MODE 0: x; MODE 1: r
-  const T *invd;      // MODE 1
-  T *P0, *P1;         // MODE 1: direction buffers, p_i lives in P[i & 1]
-  T *y;               // K x (MODE 0) / K p_i (MODE 1)
-  T *x;               // MODE 1: solution accumulator (deferred update)
-  SolverState *st;    // nullable in MODE 0
-  PcgPartials P;      // MODE 1
-  int k;              // MODE 1: position of the launch in its chunk
-};
-
 // Iteration scalars of kernel A (MODE 1) for iteration i, formed by every block.
 // Block 0 publishes r_i.z_i (and, at a stop, the stop record).  Returns false if the
 // solve stops at this iteration.  scratch: 2 NT/64 doubles for this call only.
 template <typename T, int NT>
 __device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverState &ss, double *scratch, int i,
-                                              T &beta, T &alpha_x) {
+                                              int blk, T &beta, T &alpha_x) {
   SolverState *st = a.st;
   const IterScalars sc = iter_scalars<NT>(ss, a.P, i, scratch);
   const bool owed = i > 0 && ss.flush_iter != i;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
+    st->iter_cur = i;
     st->rz_hist[i & 1] = sc.rz;
     if (sc.stop) {
       st->done = sc.stop;
@@ -76,7 +61,7 @@ template <typename T> __device__ __forceinline__ void st_off(T *base, uint32_t o
 }
 
 template <typename T, int TI, int TJ, int MODE>
-__global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
+__device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   constexpr int NT = (TI + 1) * (TJ + 1);
   constexpr int LW = TI + 2, LN = (TI + 2) * (TJ + 2);
   constexpr int NPOS = (LN + NT - 1) / NT;
@@ -93,13 +78,14 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   const T *pold = nullptr;
   T *pnew = nullptr;
   if (MODE == 1) {
-    it = ss.iter_base + a.k;
+    it = ss.iter;
     pold = (it & 1) ? a.P0 : a.P1;
     pnew = (it & 1) ? a.P1 : a.P0;
   }
 
   const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
-  const int t = xcd_tile(blockIdx.x, nblocks);
+  if (blk >= nblocks) return;  // padding block of a batched launch
+  const int t = xcd_tile(blk, nblocks);
   const int ti = t % a.tiles_x, tj = (t / a.tiles_x) % a.tiles_y, tk = t / (a.tiles_x * a.tiles_y);
   const int i0 = ti * TI, j0 = tj * TJ, k0 = tk * a.TK;
   const int k1 = min(k0 + a.TK, g.sz);
@@ -161,7 +147,7 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
       if (idx < LN) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          s_in[buf][c][idx] = MODE == 0 ? pr[p][c] : pv[p][c] * pr[p][c] + beta * po[p][c];
+          s_in[buf][c][idx] = MODE == 0 ? pr[p][c] : fma_t(beta, po[p][c], pv[p][c] * pr[p][c]);
       }
     }
   };
@@ -199,8 +185,8 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
         if (MODE == 1) {
           const T pv_ = s_in[buf][c][ctr];
           st_off(pnew, n + c * cstr, pv_);
-          st_off(a.x, n + c * cstr, o.x[c] + alpha_x * o.po[c]);
-          dot += (double)pv_ * (double)yv[c];
+          st_off(a.x, n + c * cstr, fma_t(alpha_x, o.po[c], o.x[c]));
+          dot = fma_t((double)pv_, (double)yv[c], dot);
         }
       }
     }
@@ -263,7 +249,7 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   prefetch(k0 - 1);
   T Ea = load_E(k0 - 1), Eb = T(0);
   OwnedNode<T> na{}, nb{};
-  if (MODE == 1 && !fused_scalars<T, NT>(a, ss, s_sc, it, beta, alpha_x)) return;
+  if (MODE == 1 && !fused_scalars<T, NT>(a, ss, s_sc, it, blk, beta, alpha_x)) return;
   wait_vmem();
   commit(0);
   prefetch(k0);
@@ -280,14 +266,27 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   }
   if (MODE == 1) {
     const double tot = block_sum<NT>(dot, s_red);
-    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blockIdx.x] = tot;
+    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blk] = tot;
   }
+}
+
+template <typename T, int TI, int TJ, int MODE>
+__global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
+  hex8_block<T, TI, TJ, MODE>(a, blockIdx.x);
+}
+// (FP32: four waves per SIMD, as the single-job kernel reaches)
+template <typename T, int TI, int TJ, int MODE>
+__global__ void __launch_bounds__((TI + 1) * (TJ + 1)) __attribute__((amdgpu_waves_per_eu(4)))
+    k_hex8_batch(const OpArgs<T> *__restrict__ args, const int *__restrict__ off, int njobs) {
+  int j, blk;
+  batch_block(off, njobs, j, blk);
+  hex8_block<T, TI, TJ, MODE>(args[j], blk);
 }
 
 // quad4: a block owns TI nodes along x and TK node rows, marching along y (same
 // scheme as hexa8 in one dimension less).
 template <typename T, int TI, int MODE>
-__global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
+__device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   constexpr int NT = TI + 1;
   constexpr int LW = TI + 2;
   __shared__ T s_in[3][2][LW];
@@ -298,18 +297,19 @@ __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
   SolverState ss{};
   if (a.st) ss = *a.st;
   if (ss.done) return;
+  const int nblocks = a.tiles_x * a.tiles_y;
+  if (blk >= nblocks) return;  // padding block of a batched launch
   int it = 0;
   T beta = T(0), alpha_x = T(0);
   const T *pold = nullptr;
   T *pnew = nullptr;
   if (MODE == 1) {
-    it = ss.iter_base + a.k;
+    it = ss.iter;
     pold = (it & 1) ? a.P0 : a.P1;
     pnew = (it & 1) ? a.P1 : a.P0;
-    if (!fused_scalars<T, ((NT + 63) / 64) * 64>(a, ss, s_sc, it, beta, alpha_x)) return;
+    if (!fused_scalars<T, ((NT + 63) / 64) * 64>(a, ss, s_sc, it, blk, beta, alpha_x)) return;
   }
-  const int nblocks = a.tiles_x * a.tiles_y;
-  const int t = xcd_tile(blockIdx.x, nblocks);
+  const int t = xcd_tile(blk, nblocks);
   const int ti = t % a.tiles_x, tj = t / a.tiles_x;
   const int i0 = ti * TI, j0 = tj * a.TK;
   const int j1 = min(j0 + a.TK, g.sy);
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
         T v = T(0);
         if (ok) {
           if (MODE == 0) v = a.in[c * nn + n];
-          else v = a.invd[c * nn + n] * a.in[c * nn + n] + beta * pold[c * nn + n];
+          else v = fma_t(beta, pold[c * nn + n], a.invd[c * nn + n] * a.in[c * nn + n]);
         }
         s_in[buf][c][idx] = v;
       }
@@ -382,8 +382,8 @@ __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
           if (MODE == 1) {
             const T pv_ = s_in[b0][c][tid + 1];
             pnew[c * nn + n] = pv_;
-            a.x[c * nn + n] += alpha_x * pold[c * nn + n];
-            dot += (double)pv_ * (double)yv;
+            a.x[c * nn + n] = fma_t(alpha_x, pold[c * nn + n], a.x[c * nn + n]);
+            dot = fma_t((double)pv_, (double)yv, dot);
           }
         }
       }
@@ -391,14 +391,27 @@ __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
   }
   if (MODE == 1) {
     const double tot = block_sum<((NT + 63) / 64) * 64>(dot, s_red);
-    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blockIdx.x] = tot;
+    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blk] = tot;
   }
+}
+
+template <typename T, int TI, int MODE>
+__global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
+  quad4_block<T, TI, MODE>(a, blockIdx.x);
+}
+template <typename T, int TI, int MODE>
+__global__ void __launch_bounds__(TI + 1)
+    k_quad4_batch(const OpArgs<T> *__restrict__ args, const int *__restrict__ off, int njobs) {
+  int j, blk;
+  batch_block(off, njobs, j, blk);
+  quad4_block<T, TI, MODE>(args[j], blk);
 }
 
 // tile shapes: FP32 15 x 15 (256 threads), FP64 15 x 7 (128 threads, LDS-bound)
 template <typename T> struct HexTile { static constexpr int TI = 15, TJ = 15; };
 template <> struct HexTile<double> { static constexpr int TI = 15, TJ = 7; };
 constexpr int QUAD_TI = 127;
+template <typename T> constexpr int NTHREADS3 = (HexTile<T>::TI + 1) * (HexTile<T>::TJ + 1);
 
 template <typename T>
 static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads) {
@@ -452,18 +465,43 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
 }
 
 template <typename T>
-void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
-                           PcgPartials P, int k, hipStream_t s, const LaunchEvents *ev) {
-  OpArgs<T> a{};
+int fusedA_args(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask, const T *r, const T *invd,
+                T *P0, T *P1, T *Ap, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a) {
+  a = OpArgs<T>{};
   a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = r; a.invd = invd; a.P0 = P0; a.P1 = P1;
-  a.y = Ap; a.x = x; a.st = st; a.P = P; a.k = k;
+  a.y = Ap; a.x = x; a.st = st; a.P = P;
   int nb, nt;
   setup_tiles(g, a, nb, nt);
-  if (g.ndim == 3) launch_ev((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), nb, nt, s, ev, a);
-  else launch_ev((k_quad4<T, QUAD_TI, 1>), nb, nt, s, ev, a);
+  return nb;
+}
+
+template <typename T>
+void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
+                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
+                           PcgPartials P, hipStream_t s, const LaunchEvents *ev) {
+  OpArgs<T> a;
+  const int nb = fusedA_args(g, op, E, mask, r, invd, P0, P1, Ap, x, st, P, a);
+  if (g.ndim == 3) launch_ev((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), nb, (int)NTHREADS3<T>, s, ev, a);
+  else launch_ev((k_quad4<T, QUAD_TI, 1>), nb, QUAD_TI + 1, s, ev, a);
   ICX_CHECK_LAUNCH();
 }
+
+template <typename T>
+void launch_fusedA_batch(int ndim, const OpArgs<T> *args, const int *off, int njobs, int total_blocks,
+                         hipStream_t s) {
+  if (njobs <= 0 || total_blocks <= 0) return;
+  if (ndim == 3)
+    hipLaunchKernelGGL((k_hex8_batch<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), dim3(total_blocks),
+                       dim3(NTHREADS3<T>), 0, s, args, off, njobs);
+  else
+    hipLaunchKernelGGL((k_quad4_batch<T, QUAD_TI, 1>), dim3(total_blocks), dim3(QUAD_TI + 1), 0, s, args, off,
+                       njobs);
+  ICX_CHECK_LAUNCH();
+}
+template int fusedA_args<float>(const Grid &, const ElasticOp<float> &, const float *, const uint8_t *, const float *,
+                                const float *, float *, float *, float *, float *, SolverState *, PcgPartials,
+                                OpArgs<float> &);
+template void launch_fusedA_batch<float>(int, const OpArgs<float> *, const int *, int, int, hipStream_t);
 
 template int elastic_fusedA_blocks<double>(const Grid &);
 template int elastic_fusedA_blocks<float>(const Grid &);
@@ -475,11 +513,11 @@ template void launch_elastic_matvec<float>(const Grid &, const ElasticOp<float> 
                                            hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<double>(const Grid &, const ElasticOp<double> &, const double *,
                                             const uint8_t *, const double *, const double *, double *,
-                                            double *, double *, double *, SolverState *, PcgPartials, int,
+                                            double *, double *, double *, SolverState *, PcgPartials,
                                             hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<float>(const Grid &, const ElasticOp<float> &, const float *,
                                            const uint8_t *, const float *, const float *, float *, float *,
-                                           float *, float *, SolverState *, PcgPartials, int, hipStream_t,
+                                           float *, float *, SolverState *, PcgPartials, hipStream_t,
                                            const LaunchEvents *);
 
 // ---------------------------------------------------------------------------------

@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(BS) k_pcg_setup_finalize(SolverState *st, PcgP
     st->true_rr = bb;
     st->alpha_owed = 0.0;
     st->maxit = maxit;
-    st->iter_base = 0;
+    st->iter_cur = 0;
     st->iter = 0;
     st->stop_iter = 0;
     st->flush_iter = -1;
@@ -85,18 +85,25 @@ __device__ __forceinline__ void set_lane(double2 &v, int w, double x) {
 // per round; the first round's loads are issued before alpha_i is reduced.  Block 0
 // publishes alpha_i.  The vectors are 16-byte aligned (device allocations).
 template <typename T, int U>
-__global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd,
-                                                   PcgPartials P, int k) {
+__device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   using V = typename Vec16<T>::type;
   constexpr int W = Vec16<T>::W;
   __shared__ double scr[BS / 64];
   __shared__ double scr2[2 * (BS / 64)];
+  const long long n = a.n;
+  SolverState *st = a.st;
+  T *r = a.r;
+  const T *Ap = a.Ap;
+  const T *invd = a.invd;
+  const PcgPartials &P = a.P;
+  const int nblk = P.nB;
+  if (blk >= nblk) return;     // padding block of a batched launch
   const SolverState ss = *st;  // one scalar round trip
   if (ss.done) return;
-  const int it = ss.iter_base + k;
+  const int it = ss.iter_cur;
   const long long nv = n / W;
-  const long long stride = (long long)gridDim.x * BS;
-  const long long v0 = blockIdx.x * (long long)BS + threadIdx.x;
+  const long long stride = (long long)nblk * BS;
+  const long long v0 = blk * (long long)BS + threadIdx.x;
   V *rv = reinterpret_cast<V *>(r);
   const V *av = reinterpret_cast<const V *>(Ap);
   const V *dv = reinterpret_cast<const V *>(invd);
@@ -115,7 +122,7 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
   load(v0);
   const double pAp = block_total<BS>(P.PA + (size_t)(it & 1) * RED_MAXBLOCKS, P.nA, 1, scr);
   const double alpha_d = ((it & 1) ? ss.rz_hist[1] : ss.rz_hist[0]) / pAp;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha_hist[it & 1] = alpha_d;
+  if (blk == 0 && threadIdx.x == 0) st->alpha_hist[it & 1] = alpha_d;
   const T alpha = (T)alpha_d;
   double rr = 0.0, rzn = 0.0;
   for (long long base = v0; base < nv; base += U * stride) {
@@ -126,10 +133,10 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
         V out;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          const T ri = lane(R[u], w) - alpha * lane(A[u], w);
+          const T ri = fma_t(-alpha, lane(A[u], w), lane(R[u], w));
           set_lane(out, w, ri);
-          rr += (double)ri * (double)ri;
-          rzn += (double)ri * (double)(lane(D[u], w) * ri);
+          rr = fma_t((double)ri, (double)ri, rr);
+          rzn = fma_t((double)ri, (double)(lane(D[u], w) * ri), rzn);
         }
         rv[v] = out;
       }
@@ -139,10 +146,10 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
   // scalar tail (n not a multiple of W)
   const long long tail = nv * W + v0;
   if (tail < n) {
-    const T ri = r[tail] - alpha * Ap[tail];
+    const T ri = fma_t(-alpha, Ap[tail], r[tail]);
     r[tail] = ri;
-    rr += (double)ri * (double)ri;
-    rzn += (double)ri * (double)(invd[tail] * ri);
+    rr = fma_t((double)ri, (double)ri, rr);
+    rzn = fma_t((double)ri, (double)(invd[tail] * ri), rzn);
   }
   rr = wave_sum(rr);
   rzn = wave_sum(rzn);
@@ -159,16 +166,29 @@ __global__ void __launch_bounds__(BS) k_pcg_fusedB(long long n, SolverState *st,
       s1 += scr2[BS / 64 + w];
     }
     double *slot = pb_slot(P, it);
-    slot[2 * blockIdx.x] = s0;
-    slot[2 * blockIdx.x + 1] = s1;
-    if (blockIdx.x == 0) st->iter = it + 1;
+    slot[2 * blk] = s0;
+    slot[2 * blk + 1] = s1;
+    if (blk == 0) st->iter = it + 1;
   }
 }
 
+template <typename T, int U>
+__global__ void __launch_bounds__(BS) k_pcg_fusedB(BArgs<T> a) {
+  fusedB_block<T, U>(a, blockIdx.x);
+}
+template <typename T, int U>
+__global__ void __launch_bounds__(BS)
+    k_pcg_fusedB_batch(const BArgs<T> *__restrict__ args, const int *__restrict__ off, int njobs) {
+  int j, blk;
+  batch_block(off, njobs, j, blk);
+  fusedB_block<T, U>(args[j], blk);
+}
+
 template <typename T>
-void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P, int k,
+void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
                        hipStream_t s, const LaunchEvents *ev) {
-  launch_ev((k_pcg_fusedB<T, 4>), P.nB, BS, s, ev, n, st, r, Ap, invd, P, k);
+  const BArgs<T> a{n, st, r, Ap, invd, P};
+  launch_ev((k_pcg_fusedB<T, 4>), P.nB, BS, s, ev, a);
   ICX_CHECK_LAUNCH();
 }
 
@@ -235,7 +255,7 @@ __global__ void __launch_bounds__(BS) k_refine_finalize(SolverState *st, PcgPart
       // is already in x_hi
       st->rr_ref = rr;
       st->flush_iter = st->stop_iter;
-      st->iter_base = st->stop_iter;
+      st->iter = st->stop_iter;
       st->done = (st->stop_iter >= st->maxit) ? 2 : 0;
     }
   }
@@ -247,10 +267,18 @@ void launch_refine_r(long long n, SolverState *st, const double *b, const double
   ICX_CHECK_LAUNCH();
 }
 
+template <typename T>
+void launch_pcg_fusedB_batch(const BArgs<T> *args, const int *off, int njobs, int total_blocks, hipStream_t s) {
+  if (njobs <= 0 || total_blocks <= 0) return;
+  hipLaunchKernelGGL((k_pcg_fusedB_batch<T, 4>), dim3(total_blocks), dim3(BS), 0, s, args, off, njobs);
+  ICX_CHECK_LAUNCH();
+}
+template void launch_pcg_fusedB_batch<float>(const BArgs<float> *, const int *, int, int, hipStream_t);
+
 #define ICX_INST_PCG(T)                                                                                  \
   template void launch_pcg_setup<T>(long long, SolverState *, const T *, T *, const T *, PcgPartials,    \
                                     double, int, double, hipStream_t);                                   \
-  template void launch_pcg_fusedB<T>(long long, SolverState *, T *, const T *, const T *, PcgPartials, int, \
+  template void launch_pcg_fusedB<T>(long long, SolverState *, T *, const T *, const T *, PcgPartials, \
                                      hipStream_t, const LaunchEvents *);
 ICX_INST_PCG(double)
 ICX_INST_PCG(float)
@@ -289,7 +317,7 @@ __global__ void __launch_bounds__(BS) k_eig_setup_finalize(SolverState *st, PcgP
     st->threshold = thr;
     st->rr = bb;
     st->maxit = maxit;
-    st->iter_base = 0;
+    st->iter_cur = 0;
     st->iter = 0;
     st->stop_iter = 0;
     st->done = (bb == 0.0 || bb < thr) ? 1 : 0;
@@ -303,11 +331,12 @@ void launch_eigen_setup(long long n, SolverState *st, const double *b, double *r
 }
 
 // p.dot(tmp)
-__global__ void __launch_bounds__(BS) k_eig_dot_pt(long long n, const SolverState *st, const double *p,
-                                                   const double *t, PcgPartials P, int k) {
+__global__ void __launch_bounds__(BS) k_eig_dot_pt(long long n, SolverState *st, const double *p,
+                                                   const double *t, PcgPartials P) {
   __shared__ double scr[BS / 64];
   if (st->done) return;
-  const int it = st->iter_base + k;
+  const int it = st->iter;  // first kernel of the iteration: publish it
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->iter_cur = it;
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) s += p[i] * t[i];
   s = block_sum<BS>(s, scr);
@@ -315,21 +344,20 @@ __global__ void __launch_bounds__(BS) k_eig_dot_pt(long long n, const SolverStat
 }
 // alpha = absNew / p.dot(tmp);  y += sign * alpha * v  (x += alpha p; residual -= alpha tmp)
 __global__ void __launch_bounds__(BS) k_eig_axpy(long long n, const SolverState *st, double sign, double *y,
-                                                 const double *v, PcgPartials P, int k) {
+                                                 const double *v, PcgPartials P) {
   __shared__ double scr[BS / 64];
   if (st->done) return;
-  const int it = st->iter_base + k;
+  const int it = st->iter_cur;
   const double absNew = block_total<BS>(slot1(P.PZ, it - 1), P.nB, 1, scr);
   const double pAp = block_total<BS>(slot1(P.PA, it), P.nB, 1, scr);
   const double a = sign * (absNew / pAp);
   for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) y[i] += a * v[i];
 }
 // residualNorm2 = residual.squaredNorm()
-__global__ void __launch_bounds__(BS) k_eig_norm(long long n, const SolverState *st, const double *r, PcgPartials P,
-                                                 int k) {
+__global__ void __launch_bounds__(BS) k_eig_norm(long long n, const SolverState *st, const double *r, PcgPartials P) {
   __shared__ double scr[BS / 64];
   if (st->done) return;
-  const int it = st->iter_base + k;
+  const int it = st->iter_cur;
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) s += r[i] * r[i];
   s = block_sum<BS>(s, scr);
@@ -337,10 +365,10 @@ __global__ void __launch_bounds__(BS) k_eig_norm(long long n, const SolverState 
 }
 // if (residualNorm2 < threshold) break;  z = precond.solve(residual)
 __global__ void __launch_bounds__(BS) k_eig_precond(long long n, SolverState *st, double *z, const double *invd,
-                                                    const double *r, PcgPartials P, int k) {
+                                                    const double *r, PcgPartials P) {
   __shared__ double scr[BS / 64];
   if (st->done) return;
-  const int it = st->iter_base + k;
+  const int it = st->iter_cur;
   const double rr = block_total<BS>(slot1(P.PN, it), P.nB, 1, scr);
   if (rr < st->threshold) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -356,10 +384,10 @@ __global__ void __launch_bounds__(BS) k_eig_precond(long long n, SolverState *st
 }
 // absNew = residual.dot(z)
 __global__ void __launch_bounds__(BS) k_eig_dot_rz(long long n, const SolverState *st, const double *r,
-                                                   const double *z, PcgPartials P, int k) {
+                                                   const double *z, PcgPartials P) {
   __shared__ double scr[BS / 64];
   if (st->done) return;
-  const int it = st->iter_base + k;
+  const int it = st->iter_cur;
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) s += r[i] * z[i];
   s = block_sum<BS>(s, scr);
@@ -367,10 +395,10 @@ __global__ void __launch_bounds__(BS) k_eig_dot_rz(long long n, const SolverStat
 }
 // beta = absNew / absOld;  p = z + beta * p;  i++
 __global__ void __launch_bounds__(BS) k_eig_pupdate(long long n, SolverState *st, double *p, const double *z,
-                                                    PcgPartials P, int k) {
+                                                    PcgPartials P) {
   __shared__ double scr[BS / 64];
   if (st->done) return;
-  const int it = st->iter_base + k;
+  const int it = st->iter_cur;
   const double absNew = block_total<BS>(slot1(P.PZ, it), P.nB, 1, scr);
   const double absOld = block_total<BS>(slot1(P.PZ, it - 1), P.nB, 1, scr);
   const double beta = absNew / absOld;
@@ -386,15 +414,31 @@ __global__ void __launch_bounds__(BS) k_eig_pupdate(long long n, SolverState *st
 }
 
 void launch_eigen_iteration_tail(long long n, SolverState *st, double *x, double *r, double *p, const double *t,
-                                 double *z, const double *invd, PcgPartials P, int k, hipStream_t s) {
+                                 double *z, const double *invd, PcgPartials P, hipStream_t s) {
   const dim3 nb(P.nB), bs(BS);
-  hipLaunchKernelGGL(k_eig_dot_pt, nb, bs, 0, s, n, st, p, t, P, k);
-  hipLaunchKernelGGL(k_eig_axpy, nb, bs, 0, s, n, st, 1.0, x, p, P, k);
-  hipLaunchKernelGGL(k_eig_axpy, nb, bs, 0, s, n, st, -1.0, r, t, P, k);
-  hipLaunchKernelGGL(k_eig_norm, nb, bs, 0, s, n, st, r, P, k);
-  hipLaunchKernelGGL(k_eig_precond, nb, bs, 0, s, n, st, z, invd, r, P, k);
-  hipLaunchKernelGGL(k_eig_dot_rz, nb, bs, 0, s, n, st, r, z, P, k);
-  hipLaunchKernelGGL(k_eig_pupdate, nb, bs, 0, s, n, st, p, z, P, k);
+  hipLaunchKernelGGL(k_eig_dot_pt, nb, bs, 0, s, n, st, p, t, P);
+  hipLaunchKernelGGL(k_eig_axpy, nb, bs, 0, s, n, st, 1.0, x, p, P);
+  hipLaunchKernelGGL(k_eig_axpy, nb, bs, 0, s, n, st, -1.0, r, t, P);
+  hipLaunchKernelGGL(k_eig_norm, nb, bs, 0, s, n, st, r, P);
+  hipLaunchKernelGGL(k_eig_precond, nb, bs, 0, s, n, st, z, invd, r, P);
+  hipLaunchKernelGGL(k_eig_dot_rz, nb, bs, 0, s, n, st, r, z, P);
+  hipLaunchKernelGGL(k_eig_pupdate, nb, bs, 0, s, n, st, p, z, P);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---- batched solves: gather the states of n jobs into one array (one block per job) ----
+__global__ void k_gather_states(SolverState *const *src, SolverState *dst, int n) {
+  constexpr int W = sizeof(SolverState) / sizeof(int);
+  static_assert(sizeof(SolverState) % sizeof(int) == 0, "state is a whole number of words");
+  const int j = blockIdx.x;
+  if (j >= n) return;
+  const int *s = reinterpret_cast<const int *>(src[j]);
+  int *d = reinterpret_cast<int *>(dst + j);
+  for (int w = threadIdx.x; w < W; w += blockDim.x) d[w] = s[w];
+}
+void launch_gather_states(SolverState *const *src, SolverState *dst, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_states, dim3(n), dim3(64), 0, s, src, dst, n);
   ICX_CHECK_LAUNCH();
 }
 

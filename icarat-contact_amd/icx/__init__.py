@@ -28,8 +28,8 @@ EXPORTS = [
     "icx_get_node_values", "icx_compliance", "icx_sensitivity", "icx_element_volume",
     "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
     "icx_project", "icx_oc_update", "icx_mma_reset", "icx_mma_update",
-    "icx_opt_init", "icx_opt_step", "icx_opt_get_design",
-    "icx_solve_batch", "icx_job_footprint",
+    "icx_opt_init", "icx_opt_step", "icx_opt_step_begin", "icx_opt_step_end", "icx_opt_get_design",
+    "icx_submit", "icx_poll", "icx_job_last_info", "icx_solve_batch", "icx_job_footprint",
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
 ]
 OPT_GRAPHS, OPT_TILE_DEPTH = 1, 2
@@ -121,6 +121,23 @@ class Context:
 
     def profile_begin(self, kernel):
         self.check(lib().icx_profile_begin(self.h, int(kernel)))
+
+    def poll(self, chunks=1 << 20, cap=4096):
+        """Advance every submitted solve (icx_poll); returns the jobs that finished."""
+        arr = (C.c_void_p * cap)()
+        n = C.c_int()
+        self.check(lib().icx_poll(self.h, int(chunks), arr, int(cap), C.byref(n)))
+        by_handle = {j.h.value: j for j in list(self.jobs) if j.h}
+        return [by_handle[arr[i]] for i in range(n.value)]
+
+    def solve_batch(self, jobs, form=None, tol=1e-8, maxit=50000):
+        """icx_solve_batch over jobs; returns one info dict per job."""
+        form = SOLVE_MIXED if form is None else form
+        arr = (C.c_void_p * max(len(jobs), 1))(*[j.h.value for j in jobs])
+        infos = (SolveInfo * max(len(jobs), 1))()
+        rc = lib().icx_solve_batch(self.h, arr, len(jobs), int(form), C.c_double(tol), int(maxit), infos)
+        self.check(rc, allow=(ERR_NOCONV,))
+        return [infos[i].as_dict() for i in range(len(jobs))]
 
     def profile_end(self):
         s, n = C.c_double(), C.c_longlong()
@@ -336,6 +353,32 @@ class Job:
         self.ctx.check(lib().icx_opt_step(self.h, C.byref(r)))
         return r.as_dict()
 
+    def opt_step_begin(self):
+        self.ctx.check(lib().icx_opt_step_begin(self.h))
+
+    def opt_step_end(self, info=None):
+        """info: the solve's info dict (its iteration count goes into the record)."""
+        r = IterRecord()
+        si = SolveInfo()
+        if info:
+            for k, _ in SolveInfo._fields_:
+                setattr(si, k, info[k])
+        self.ctx.check(lib().icx_opt_step_end(self.h, C.byref(si) if info else None, C.byref(r)))
+        return r.as_dict()
+
+    def submit(self, tol=1e-8, maxit=50000):
+        """Start a batched mixed-form solve (icx_submit); it advances in Context.poll."""
+        self.ctx.check(lib().icx_submit(self.h, C.c_double(tol), int(maxit)))
+
+    def last_info(self):
+        """Info dict of the last finished submitted solve (with 'converged')."""
+        si = SolveInfo()
+        rc = lib().icx_job_last_info(self.h, C.byref(si))
+        self.ctx.check(rc, allow=(ERR_NOCONV,))
+        d = si.as_dict()
+        d["converged"] = rc == 0
+        return d
+
     def opt_design(self):
         x = np.zeros(self.nelx)
         self.ctx.check(lib().icx_opt_get_design(self.h, _dp(x)))
@@ -379,4 +422,36 @@ def cantilever_3d(ctx, nx, ny, nz, young=(1e-9, 1.0), nu=0.3, p=3.0):
         if d < j.numeq:
             f[d] += -w / ny
     j.set_fext(f)
+    return j
+
+
+def random_job(ctx, rng, dims=None, size2d=(60, 240), size3d=(32, 96), young=(1e-9, 1.0), nu=0.3, p=3.0):
+    """One job of the batched workload: Q4 d0 x d0/3 (plane stress) or hex8 d0 x d0/2 x
+    d0/2, h = 1, d0 uniform in the given range; the x = 0 or the x = L face clamped; a
+    unit point load (-y in 2D, -z in 3D) at a random node of the opposite face; uniform
+    density = volume fraction ~ U[0.2, 0.5].  rng: numpy Generator.  Returns the job."""
+    ndim = dims if dims is not None else (2 if rng.random() < 0.5 else 3)
+    side = int(rng.integers(0, 2))
+    if ndim == 2:
+        d0 = int(rng.integers(size2d[0], size2d[1] + 1))
+        d1 = max(1, d0 // 3)
+        j = Job(ctx, QUAD4, (d0, d1, 1), (float(d0), float(d1), 1.0), PLANE_STRESS, young, nu, p)
+        x = 0.0 if side == 0 else float(d0)
+        j.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [x, x, 0.0, float(d1), 0.0, 0.0])
+        j.make_dof()
+        node = j.node_index(d0 if side == 0 else 0, int(rng.integers(0, d1 + 1)))
+        j.add_nodal_load(node, 1, -1.0)
+        ne = d0 * d1
+    else:
+        d0 = int(rng.integers(size3d[0], size3d[1] + 1))
+        d1 = max(1, d0 // 2)
+        j = Job(ctx, HEXA8, (d0, d1, d1), (float(d0), float(d1), float(d1)), SOLID3D, young, nu, p)
+        x = 0.0 if side == 0 else float(d0)
+        j.add_dirichlet([1, 1, 1, 0, 0, 0], [0] * 6, [x, x, 0.0, float(d1), 0.0, float(d1)])
+        j.make_dof()
+        node = j.node_index(d0 if side == 0 else 0, int(rng.integers(0, d1 + 1)), int(rng.integers(0, d1 + 1)))
+        j.add_nodal_load(node, 2, -1.0)
+        ne = d0 * d1 * d1
+    j.volfrac = float(rng.uniform(0.2, 0.5))
+    j.set_design(np.full(ne, j.volfrac))
     return j

@@ -197,12 +197,33 @@ typedef struct {
 
 icx_status icx_opt_init(icx_job *job, const icx_opt_params *par, const double *x0);
 icx_status icx_opt_step(icx_job *job, icx_iter_record *rec);
+/* icx_opt_step in two halves around a solve that the caller runs, e.g. through
+ * icx_submit: begin = filter, projection, design -> moduli (opt_macro_oc.cpp:87-101,
+ * opt_mma_Helmholtz.cpp:86-98); end = objective, volume, sensitivity, chain rule,
+ * filter adjoint and the OC / MMA update (opt_macro_oc.cpp:104-165,
+ * opt_mma_Helmholtz.cpp:100-158) on the job's current displacement. */
+icx_status icx_opt_step_begin(icx_job *job);
+icx_status icx_opt_step_end(icx_job *job, const icx_solve_info *solve, icx_iter_record *rec);
 icx_status icx_opt_get_design(const icx_job *job, double *x);
 
 /* ---- batched / serving -------------------------------------------------------------
- * No reference counterpart: the reference runs one process per job.  A batch
- * runs the PCG iterations of many resident jobs together (one launch per
- * kernel for all jobs) until each job meets its tolerance. */
+ * No reference counterpart: the reference runs one process per job.  Submitted
+ * mixed-form solves of any jobs of a context advance together: each launch of a PCG
+ * kernel phase covers every job whose solve is running, so jobs of uneven sizes share
+ * the device; each job's result is bit-identical to its own icx_solve(MIXED).
+ *
+ * icx_submit: start a mixed solve of the job's current design and load (set-up runs
+ *   at once; the iterations run in icx_poll).  One submitted solve per job at a time.
+ * icx_poll: run up to `chunks` chunks of iterations of all submitted solves (with
+ *   their FP64 residual replacements), returning as soon as at least one solve has
+ *   finished; the finished jobs are written to finished[0 .. *nfinished).
+ * icx_job_last_info: the info and status (ICX_OK / ICX_ERR_NOCONV) of the job's last
+ *   finished submitted solve; its displacement is then the job's current one.
+ * icx_solve_batch: submit all, poll until all have finished (form ICX_SOLVE_MIXED;
+ *   other forms solve the jobs one after the other). */
+icx_status icx_submit(icx_job *job, double tol, int maxit);
+icx_status icx_poll(icx_ctx *ctx, int chunks, icx_job **finished, int cap, int *nfinished);
+icx_status icx_job_last_info(const icx_job *job, icx_solve_info *info);
 icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, double tol,
                            int maxit, icx_solve_info *infos);
 /* device bytes a job holds (admission control) */

@@ -1,0 +1,105 @@
+"""Batched mixed-form solves (icx_submit / icx_poll / icx_solve_batch): every job's
+result equals its own icx_solve(MIXED) bit for bit, whatever else is in the batch and
+whenever it was submitted."""
+import numpy as np
+import pytest
+
+import icx
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(ctx, seed, n):
+    rng = np.random.default_rng(seed)
+    return [icx.random_job(ctx, rng, size2d=(12, 60), size3d=(6, 16)) for _ in range(n)]
+
+
+def _alone(jobs, tol=1e-8):
+    out = []
+    for j in jobs:
+        u, info = j.solve(icx.SOLVE_MIXED, tol=tol, maxit=50000)
+        assert info["converged"] and info["relres"] < tol
+        out.append((u, info))
+    return out
+
+
+def test_batch_equals_alone(ctx):
+    jobs = _jobs(ctx, 11, 6)
+    ref = _alone(jobs)
+    infos = ctx.solve_batch(jobs, tol=1e-8)
+    for j, (u0, i0), i1 in zip(jobs, ref, infos):
+        assert i1["iterations"] == i0["iterations"]
+        assert i1["refinements"] == i0["refinements"]
+        assert i1["relres"] == i0["relres"]
+        assert np.array_equal(j.displacement(), u0)
+    for j in jobs:
+        j.close()
+
+
+def test_staggered_submissions(ctx):
+    """Jobs join a running batch at different times (continuous batching)."""
+    jobs = _jobs(ctx, 12, 7)
+    ref = _alone(jobs)
+    pending = list(jobs)
+    running, finished = set(), {}
+    for _ in range(2):
+        j = pending.pop(0)
+        j.submit(tol=1e-8)
+        running.add(j)
+    polls = 0
+    while running or pending:
+        for j in ctx.poll(chunks=1):
+            finished[id(j)] = (j.displacement(), j.last_info())
+            running.discard(j)
+        polls += 1
+        if pending and polls % 2 == 0:
+            j = pending.pop(0)
+            j.submit(tol=1e-8)
+            running.add(j)
+        assert polls < 100000
+    for j, (u0, i0) in zip(jobs, ref):
+        u1, i1 = finished[id(j)]
+        assert i1["converged"] and i1["iterations"] == i0["iterations"]
+        assert np.array_equal(u1, u0)
+    for j in jobs:
+        j.close()
+
+
+def test_batch_zero_load_and_iteration_limit(ctx):
+    rng = np.random.default_rng(5)
+    a = icx.random_job(ctx, rng, dims=3, size3d=(8, 8))
+    a.set_fext(np.zeros(a.numeq))               # zero load: x = 0 at once
+    b = icx.random_job(ctx, rng, dims=2, size2d=(40, 40))
+    b.submit(tol=1e-12, maxit=5)                # stops at the iteration limit
+    a.submit(tol=1e-8)
+    seen = []
+    while len(seen) < 2:
+        seen += ctx.poll()
+    ia, ib = a.last_info(), b.last_info()
+    assert ia["converged"] and ia["iterations"] == 0
+    assert not np.any(a.displacement())
+    assert not ib["converged"] and ib["iterations"] == 5
+    for j in (a, b):
+        j.close()
+
+
+def test_design_step_halves_equal_whole_step(ctx):
+    """icx_opt_step == icx_opt_step_begin + submitted solve + icx_opt_step_end."""
+    x0 = np.full(30 * 10, 0.5)
+    A = icx.cantilever_2d(ctx, 30, 10)
+    B = icx.cantilever_2d(ctx, 30, 10)
+    for j in (A, B):
+        j.opt_init(x0, driver=0, radius=1.5, form=icx.SOLVE_MIXED, tol=1e-9, maxit=20000)
+    for _ in range(3):
+        ra = A.opt_step()
+        B.opt_step_begin()
+        B.submit(tol=1e-9, maxit=20000)
+        done = []
+        while B not in done:
+            done += ctx.poll()
+        rb = B.opt_step_end(B.last_info())
+        for k in ("objective", "constraint", "volume", "cg_iterations", "update_steps"):
+            assert ra[k] == rb[k], k
+    assert np.array_equal(A.opt_design(), B.opt_design())
+    for j in (A, B):
+        j.close()
