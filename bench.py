@@ -13,6 +13,11 @@ value = free DOFs x (operator applications of the step) / step wall time.
 
 --pmc-workload runs a bounded counter workload instead (a 256 MiB device copy of
 known size, then eager mixed-form iterations), for rocprofv3 --pmc passes.
+--batched measures 64 independent jobs (sizes and loads drawn from seed 20261204:
+Q4 60x20 .. 240x80 and hexa8 32x16x16 .. 96x48x48) solved one after another and as
+one batch (icx_solve_batch), and checks that every displacement is bit-identical.
+--serve runs the serving loop (icx.serve): Poisson arrivals, footprint admission,
+OC design iterations with batched solves; p50/p99 design-iteration latency.
 """
 import argparse
 import json
@@ -43,6 +48,14 @@ def parse():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly")
     ap.add_argument("--pmc-workload", action="store_true")
+    ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-iters", type=int, default=10)
+    ap.add_argument("--serve-elems", type=float, nargs=2, default=[5e3, 3e5])
+    ap.add_argument("--serve-budget", type=float, default=240e9)
     ap.add_argument("--pmc-iters", type=int, default=200)
     return ap.parse_args()
 
@@ -112,6 +125,62 @@ def pmc_workload(args):
     ctx.synchronize()
     print("mixed form: %d iterations of %dx%dx%d (N = %d, Ne = %d)" % (
         info["iterations"], nx, ny, nz, job.numeq, job.nelx), flush=True)
+
+
+def batched(args):
+    """64 jobs solved one after another, then as one batch."""
+    import numpy as np
+
+    import icx
+    ctx = icx.Context(0)
+    rng = np.random.default_rng(np.random.PCG64(20261204))
+    jobs = [icx.random_job(ctx, rng) for _ in range(args.batch_jobs)]
+    for j in jobs:  # warm-up: buffers, captured graphs
+        j.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=False)
+    ctx.solve_batch(jobs, tol=args.tol)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    seq = []
+    for j in jobs:
+        seq.append(j.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=True))
+    ctx.synchronize()
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    infos = ctx.solve_batch(jobs, tol=args.tol)
+    ctx.synchronize()
+    t_bat = time.perf_counter() - t0
+    same = all(np.array_equal(j.displacement(), u) and inf["iterations"] == si["iterations"]
+               for j, (u, si), inf in zip(jobs, seq, infos))
+    work = sum(j.numeq * inf["matvecs"] for j, inf in zip(jobs, infos))
+    n2 = sum(1 for j in jobs if j.div[2] == 1)
+    line = {"metric": "DOF-matvec/s, %d batched jobs, FP32-refined PCG to rel-res %g" % (len(jobs), args.tol),
+            "value": work / t_bat, "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32 operator / f64 residual", "data": "synthetic (seeded sizes, loads, densities)",
+            "config": {"workload": "batched jobs: %d Q4 (60x20..240x80) + %d hexa8 (32x16x16..96x48x48), seed "
+                                   "20261204" % (n2, len(jobs) - n2),
+                       "jobs": len(jobs), "free_dofs": sum(j.numeq for j in jobs),
+                       "batched_s": t_bat, "sequential_s": t_seq, "sequential_dof_matvec_per_s": work / t_seq,
+                       "speedup_batched_vs_sequential": t_seq / t_bat,
+                       "bit_identical_to_sequential": same,
+                       "iterations_min_max": [min(i["iterations"] for i in infos),
+                                              max(i["iterations"] for i in infos)],
+                       "all_converged": all(i["relres"] < args.tol for i in infos)}}
+    print(json.dumps(line))
+
+
+def serving(args):
+    import icx
+    from icx import serve
+    ctx = icx.Context(0)
+    out = serve.run(ctx, n_jobs=args.serve_jobs, rate=args.serve_rate, iters=args.serve_iters,
+                    elems=tuple(args.serve_elems), budget=args.serve_budget, tol=args.tol,
+                    log=lambda m: print(m, file=sys.stderr, flush=True))
+    out.pop("designs", None)
+    line = {"metric": "serving: design-iteration latency p50/p99 and DOF-matvec/s",
+            "value": out["dof_matvec_per_s"], "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32 operator / f64 residual", "data": "synthetic (Poisson arrivals, seed 20261205)",
+            "config": out}
+    print(json.dumps(line))
 
 
 def measure(icx, ctx, size, tol, steps, warmup, plain_too):
@@ -206,6 +275,12 @@ def main():
         sys.exit(2)
     if args.pmc_workload:
         pmc_workload(args)
+        return
+    if args.batched:
+        batched(args)
+        return
+    if args.serve:
+        serving(args)
         return
 
     import icx

@@ -2027,14 +2027,22 @@ icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, do
 icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
   if (!j || !bytes) return ICX_ERR_ARG;
   size_t b = 0;
-  auto add = [&](size_t v) { b += v; };
-  add(j->d_fext.bytes()); add(j->d_b.bytes()); add(j->d_u.bytes()); add(j->d_design.bytes()); add(j->d_E.bytes());
-  add(j->d_vec.bytes()); add(j->d_vec2.bytes()); add(j->d_mask.bytes()); add(j->d_dof.bytes()); add(j->d_partial.bytes()); add(j->d_pcgpart.bytes());
-  for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(d->bytes());
-  for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd}) add(d->bytes());
-  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.xlo, &j->mw.invd, &j->mw.E, &j->mw.b32})
-    add(d->bytes());
-  add(j->mw.Kx.bytes());
+  auto add = [&](const auto &d) { b += d.bytes(); };
+  add(j->d_dof); add(j->d_mask); add(j->d_fext); add(j->d_dirich); add(j->d_b); add(j->d_u);
+  add(j->d_design); add(j->d_E); add(j->d_vec); add(j->d_vec2); add(j->d_partial); add(j->d_pcgpart);
+  add(j->d_state); add(j->d_scalars);
+  for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(*d);
+  for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd}) add(*d);
+  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.xlo, &j->mw.invd, &j->mw.E, &j->mw.b32}) add(*d);
+  add(j->mw.Kx);
+  for (auto *d : {&j->hw.invd, &j->hw.b, &j->hw.x, &j->hw.r, &j->hw.p, &j->hw.t, &j->hw.z}) add(*d);
+  add(j->cw.off); add(j->cw.w); add(j->cw.aa);
+  for (auto *d : {&j->mma.low, &j->mma.upp, &j->mma.alpha, &j->mma.beta, &j->mma.p0, &j->mma.q0, &j->mma.p1,
+                  &j->mma.q1, &j->mma.xold1, &j->mma.xold2})
+    add(*d);
+  for (auto *d : {&j->opt.x, &j->opt.fil, &j->opt.thr, &j->opt.dthr, &j->opt.dfds, &j->opt.dgds, &j->opt.tmp,
+                  &j->opt.tmp2})
+    add(*d);
   *bytes = b;
   return ICX_OK;
 }

@@ -46,6 +46,9 @@ struct SolverState {
   double true_rr;      // |b - K x|^2 in FP64 at the last replacement
   double alpha_owed;   // step still owed to x at the stop (fused/mixed forms)
   int done;            // 0 running, 1 converged, 2 maxit, 3 recursive residual met, 4 replace
+  int pending;         // fused/mixed forms: stop code found by kernel A at stop_iter, published
+                       // as `done` by kernel B (no kernel writes a field that it reads itself:
+                       // the waves of a block that starts late must not see a flag flip)
   int maxit;
   int iter_cur;        // iteration in flight: published by the first kernel of an iteration
   int iter;            // next iteration (= iterations completed), advanced by its last kernel

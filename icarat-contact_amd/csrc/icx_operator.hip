@@ -17,7 +17,7 @@ __device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverSt
     st->iter_cur = i;
     st->rz_hist[i & 1] = sc.rz;
     if (sc.stop) {
-      st->done = sc.stop;
+      st->pending = sc.stop;  // kernel B publishes it as `done`
       st->stop_iter = i;
       st->rr = sc.rr;
       st->alpha_owed = owed ? sc.alpha_prev : 0.0;
@@ -72,7 +72,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   __shared__ double s_sc[2 * (NT / 64)];
   const Grid &g = a.g;
   SolverState ss{};
-  if (a.st) ss = *a.st;  // one scalar round trip
+  if (a.st) ss = load_state(a.st);
   if (ss.done) return;   // solve already finished (uniform per launch)
   int it = 0;
   const T *pold = nullptr;
@@ -295,7 +295,7 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   __shared__ double s_sc[2 * ((NT + 63) / 64)];
   const Grid &g = a.g;
   SolverState ss{};
-  if (a.st) ss = *a.st;
+  if (a.st) ss = load_state(a.st);
   if (ss.done) return;
   const int nblocks = a.tiles_x * a.tiles_y;
   if (blk >= nblocks) return;  // padding block of a batched launch

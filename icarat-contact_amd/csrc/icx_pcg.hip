@@ -52,6 +52,7 @@ __global__ void __launch_bounds__(BS) k_pcg_setup_finalize(SolverState *st, PcgP
     st->iter = 0;
     st->stop_iter = 0;
     st->flush_iter = -1;
+    st->pending = 0;
     // Eigen: a zero right-hand side returns x = 0 at once; so does a converged start
     st->done = (bb == 0.0 || bb < thr) ? 1 : 0;
   }
@@ -98,8 +99,12 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   const PcgPartials &P = a.P;
   const int nblk = P.nB;
   if (blk >= nblk) return;     // padding block of a batched launch
-  const SolverState ss = *st;  // one scalar round trip
+  const SolverState ss = load_state(st);
   if (ss.done) return;
+  if (ss.pending) {  // kernel A of this iteration stopped: publish the stop
+    if (blk == 0 && threadIdx.x == 0) st->done = ss.pending;
+    return;
+  }
   const int it = ss.iter_cur;
   const long long nv = n / W;
   const long long stride = (long long)nblk * BS;
@@ -246,6 +251,7 @@ __global__ void __launch_bounds__(BS) k_refine_finalize(SolverState *st, PcgPart
   __shared__ double scr[BS / 64];
   const double rr = block_total<BS>(pb_slot(P, st->stop_iter - 1), P.nB, 2, scr);
   if (threadIdx.x == 0) {
+    st->pending = 0;
     st->true_rr = rr;
     st->rr = rr;
     if (rr < st->threshold) {
@@ -363,11 +369,21 @@ __global__ void __launch_bounds__(BS) k_eig_norm(long long n, const SolverState 
   s = block_sum<BS>(s, scr);
   if (threadIdx.x == 0) slot1(P.PN, it)[blockIdx.x] = s;
 }
+// `done` as one value for the whole block.  The two kernels below set `done` from
+// block 0 while blocks of the same launch may still be starting; a per-wave read
+// could split a block across its barriers.
+__device__ __forceinline__ bool block_done(const SolverState *st) {
+  __shared__ int s_done;
+  if (threadIdx.x == 0) s_done = __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  return s_done != 0;
+}
+
 // if (residualNorm2 < threshold) break;  z = precond.solve(residual)
 __global__ void __launch_bounds__(BS) k_eig_precond(long long n, SolverState *st, double *z, const double *invd,
                                                     const double *r, PcgPartials P) {
   __shared__ double scr[BS / 64];
-  if (st->done) return;
+  if (block_done(st)) return;
   const int it = st->iter_cur;
   const double rr = block_total<BS>(slot1(P.PN, it), P.nB, 1, scr);
   if (rr < st->threshold) {
@@ -397,7 +413,7 @@ __global__ void __launch_bounds__(BS) k_eig_dot_rz(long long n, const SolverStat
 __global__ void __launch_bounds__(BS) k_eig_pupdate(long long n, SolverState *st, double *p, const double *z,
                                                     PcgPartials P) {
   __shared__ double scr[BS / 64];
-  if (st->done) return;
+  if (block_done(st)) return;
   const int it = st->iter_cur;
   const double absNew = block_total<BS>(slot1(P.PZ, it), P.nB, 1, scr);
   const double absOld = block_total<BS>(slot1(P.PZ, it - 1), P.nB, 1, scr);

@@ -1,0 +1,150 @@
+"""Serving loop: topology-optimization jobs arriving over time share one GPU.
+
+Jobs arrive as a Poisson stream.  A job is admitted when its device footprint fits
+the HBM budget (first come, first served); an admitted job's densities, displacements
+and PCG state stay resident until it leaves.  Each design iteration of a job is
+filter + projection (icx_opt_step_begin), a mixed-precision solve submitted to the
+context's batch (icx_submit), and sensitivity + filter adjoint + OC update
+(icx_opt_step_end) once icx_poll reports the solve finished.  The solves of all
+admitted jobs advance together (one launch per PCG kernel phase for all of them), and
+a job that arrives while others are iterating joins at the next chunk: continuous
+batching.  Reported: jobs/s, DOF-matvec/s, and p50/p99 latency of a design iteration
+(begin to end, host clock), plus the admission delay.
+"""
+import collections
+import math
+import time
+
+import numpy as np
+
+from . import HEXA8, PLANE_STRESS, QUAD4, SOLID3D, SOLVE_MIXED, Job
+
+SEED = 20261205
+
+
+def job_spec(rng, elems_lo, elems_hi):
+    """Element count log-uniform in [elems_lo, elems_hi]; 2D / 3D with probability 1/2;
+    2D aspect 3:1 (Q4 plane stress), 3D aspect 2:1:1 (hexa8)."""
+    ne = math.exp(rng.uniform(math.log(elems_lo), math.log(elems_hi)))
+    if rng.random() < 0.5:
+        d0 = max(3, int(round(math.sqrt(3.0 * ne))))
+        return (2, (d0, max(1, int(round(d0 / 3.0))), 1))
+    d0 = max(2, int(round((4.0 * ne) ** (1.0 / 3.0))))
+    d1 = max(1, int(round(d0 / 2.0)))
+    return (3, (d0, d1, d1))
+
+
+def footprint_estimate(ndim, div):
+    """Device bytes of one served job (mixed solve, cone filter, OC design loop):
+    80 B per DOF, 92 B per element, 1 B per node, 0.4 MB of partial sums and state;
+    icx_job_footprint reports the same buffers once they exist."""
+    d0, d1, d2 = div
+    nn = (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
+    ne = d0 * d1 * (d2 if ndim == 3 else 1)
+    n = ndim * nn
+    return 80 * n + 92 * ne + nn + 400 * 1024
+
+
+def make_job(ctx, ndim, div):
+    """Cantilever: x = 0 edge/face clamped, unit load at the far end (2D: node
+    (L, H/2) in -y; 3D: node (L, W/2, 0) in -z)."""
+    d0, d1, d2 = div
+    if ndim == 2:
+        j = Job(ctx, QUAD4, (d0, d1, 1), (float(d0), float(d1), 1.0), PLANE_STRESS)
+        j.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0.0, 0.0, 0.0, float(d1), 0.0, 0.0])
+        j.make_dof()
+        j.add_nodal_load(j.node_index(d0, d1 // 2), 1, -1.0)
+    else:
+        j = Job(ctx, HEXA8, (d0, d1, d2), (float(d0), float(d1), float(d2)), SOLID3D)
+        j.add_dirichlet([1, 1, 1, 0, 0, 0], [0] * 6, [0.0, 0.0, 0.0, float(d1), 0.0, float(d2)])
+        j.make_dof()
+        j.add_nodal_load(j.node_index(d0, d1 // 2, 0), 2, -1.0)
+    return j
+
+
+def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, seed=SEED,
+        tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None):
+    """Serve n_jobs jobs arriving at `rate` per second (exponential gaps), each running
+    `iters` OC design iterations.  specs: optional list of (ndim, div) instead of the
+    drawn sizes; keep_designs: return each job's final design under "designs".
+    Returns a dict of the measurements."""
+    rng = np.random.default_rng(np.random.PCG64(seed))
+    gaps = rng.exponential(1.0 / rate, n_jobs)
+    arrival = np.cumsum(gaps) - gaps[0]
+    drawn = [job_spec(rng, *elems) for _ in range(n_jobs)]
+    specs = list(specs) if specs is not None else drawn
+    n_jobs = len(specs)
+    est = [footprint_estimate(*s) for s in specs]
+    if max(est) > budget:
+        raise ValueError("a job needs %.3g bytes, more than the budget %.3g" % (max(est), budget))
+
+    pending = collections.deque(range(n_jobs))
+    active = {}
+    lat, admit_wait, footprints = [], [], []
+    designs = {}
+    dof_matvecs = 0
+    used = 0.0
+    peak_used = 0.0
+    peak_active = 0
+    finished_jobs = 0
+    t0 = time.perf_counter()
+
+    def begin(rec):
+        rec["t_begin"] = time.perf_counter()
+        rec["job"].opt_step_begin()
+        rec["job"].submit(tol=tol, maxit=maxit)
+
+    while pending or active:
+        now = time.perf_counter() - t0
+        while pending and arrival[pending[0]] <= now and used + est[pending[0]] <= budget:
+            i = pending.popleft()
+            ndim, div = specs[i]
+            job = make_job(ctx, ndim, div)
+            job.opt_init(np.full(job.nelx, x0), driver=0, radius=radius, projection=True,
+                         form=SOLVE_MIXED, tol=tol, maxit=maxit)
+            rec = {"job": job, "spec": i, "step": 0, "numeq": job.numeq}
+            active[id(job)] = rec
+            used += est[i]
+            peak_used = max(peak_used, used)
+            peak_active = max(peak_active, len(active))
+            admit_wait.append(time.perf_counter() - t0 - arrival[i])
+            begin(rec)
+        if active:
+            for job in ctx.poll(chunks=1):
+                rec = active[id(job)]
+                info = job.last_info()
+                job.opt_step_end(info)
+                lat.append(time.perf_counter() - rec["t_begin"])
+                dof_matvecs += rec["numeq"] * info["matvecs"]
+                rec["step"] += 1
+                if rec["step"] < iters:
+                    begin(rec)
+                else:
+                    if len(footprints) < 8:
+                        footprints.append((job.footprint(), est[rec["spec"]]))
+                    if keep_designs:
+                        designs[rec["spec"]] = job.opt_design()
+                    job.close()
+                    del active[id(job)]
+                    used -= est[rec["spec"]]
+                    finished_jobs += 1
+                    if log:
+                        log("job %d done (%d left)" % (rec["spec"], n_jobs - finished_jobs))
+        elif pending:
+            time.sleep(max(0.0, min(0.01, arrival[pending[0]] - (time.perf_counter() - t0))))
+    wall = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1e3
+    return {
+        "jobs": n_jobs, "design_iterations": len(lat), "wall_s": wall,
+        "jobs_per_s": n_jobs / wall, "dof_matvec_per_s": dof_matvecs / wall,
+        "iteration_ms_p50": float(np.percentile(lat_ms, 50)),
+        "iteration_ms_p99": float(np.percentile(lat_ms, 99)),
+        "admission_wait_ms_p50": float(np.percentile(admit_wait, 50) * 1e3),
+        "admission_wait_ms_p99": float(np.percentile(admit_wait, 99) * 1e3),
+        "peak_admitted_jobs": peak_active, "peak_budget_bytes": peak_used,
+        "arrival_rate_per_s": rate, "iterations_per_job": iters, "elements": list(elems),
+        "budget_bytes": budget, "tol": tol,
+        "footprint_checks": footprints,
+        "designs": designs if keep_designs else None,
+        "max_used_within_budget": peak_used <= budget,
+    }

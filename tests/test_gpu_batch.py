@@ -103,3 +103,19 @@ def test_design_step_halves_equal_whole_step(ctx):
     assert np.array_equal(A.opt_design(), B.opt_design())
     for j in (A, B):
         j.close()
+
+
+def test_many_large_jobs_equal_alone(ctx):
+    """Enough large hexa8 jobs that the blocks of one job span several rounds of a
+    launch (a stop found by kernel A must reach every block of the job identically)."""
+    rng = np.random.default_rng(21)
+    jobs = [icx.random_job(ctx, rng, dims=3, size3d=(56, 72)) for _ in range(8)]
+    jobs += [icx.random_job(ctx, rng, dims=2, size2d=(80, 160)) for _ in range(4)]
+    ref = _alone(jobs)
+    for _ in range(2):
+        infos = ctx.solve_batch(jobs, tol=1e-8)
+        for j, (u0, i0), i1 in zip(jobs, ref, infos):
+            assert i1["iterations"] == i0["iterations"]
+            assert np.array_equal(j.displacement(), u0)
+    for j in jobs:
+        j.close()
