@@ -28,29 +28,11 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// The fields of a solver state that the iteration kernels read, loaded with
-// agent-scope relaxed atomic loads (vector memory, not the scalar cache, which is not
-// coherent with the vector stores that write the state).  Every field a kernel reads
-// is written only by other kernels, so all waves of all its blocks see the same values.
-__device__ __forceinline__ SolverState load_state(const SolverState *st) {
-  SolverState s{};
-  auto ld_i = [](const int *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
-  auto ld_d = [](const double *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
-  s.done = ld_i(&st->done);
-  s.pending = ld_i(&st->pending);
-  s.maxit = ld_i(&st->maxit);
-  s.iter = ld_i(&st->iter);
-  s.iter_cur = ld_i(&st->iter_cur);
-  s.flush_iter = ld_i(&st->flush_iter);
-  s.threshold = ld_d(&st->threshold);
-  s.delta2 = ld_d(&st->delta2);
-  s.rr_ref = ld_d(&st->rr_ref);
-  s.rz_hist[0] = ld_d(&st->rz_hist[0]);
-  s.rz_hist[1] = ld_d(&st->rz_hist[1]);
-  s.alpha_hist[0] = ld_d(&st->alpha_hist[0]);
-  s.alpha_hist[1] = ld_d(&st->alpha_hist[1]);
-  return s;
-}
+// Snapshot of a solver state at kernel entry (scalar loads: one round trip).  Every
+// field an iteration kernel reads is written only by other kernels (kernel A records a
+// stop in `pending`, kernel B publishes it as `done`), so all waves of all blocks of a
+// launch read the same values.
+__device__ __forceinline__ SolverState load_state(const SolverState *st) { return *st; }
 
 // Job and job-local block of a batched launch (see Batch in icx_internal.h); the
 // search is uniform per block.
