@@ -324,14 +324,12 @@ struct IterScalars {
   int stop;  // 0 continue, 2 maxit, 3 tolerance met by the recursive residual, 4 replace
 };
 
-// st: a snapshot of the solver state taken at kernel entry (one scalar round trip;
-// both history slots are read and selected afterwards).  scratch: 2 BS/64 doubles
-// reserved for this call.
-template <int BS>
-__device__ __forceinline__ IterScalars iter_scalars(const SolverState &st, const PcgPartials &P, int i,
-                                                    double *scratch) {
+// st: a snapshot of the solver state taken at kernel entry; rr, rz: the totals of the
+// (|r_i|^2, r_i.z_i) partials (block_total2 over P.PB, issued before the state arrives).
+__device__ __forceinline__ IterScalars iter_scalars(const SolverState &st, int i, double rr, double rz) {
   IterScalars s;
-  block_total2<BS>(P.PB + (size_t)((i - 1) & 1) * 2 * RED_MAXBLOCKS, P.nB, scratch, s.rr, s.rz);
+  s.rr = rr;
+  s.rz = rz;
   if (i > 0) {
     const bool odd = (i - 1) & 1;
     s.beta = s.rz / (odd ? st.rz_hist[1] : st.rz_hist[0]);

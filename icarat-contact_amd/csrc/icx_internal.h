@@ -60,8 +60,8 @@ struct SolverState {
 
 // Per-block partial sums of a PCG solve, double-buffered by iteration parity.
 struct PcgPartials {
-  double *PA;          // [2][RED_MAXBLOCKS]     p.Ap of kernel A / p.tmp of the plain loop
-  double *PB;          // [2][RED_MAXBLOCKS][2]  (|r|^2, r.z) of kernel B / set-up / replacement
+  double *PA;          // [2][RED_MAXBLOCKS]     p.Ap of kernel A (slot 0) / p.tmp of the plain loop
+  double *PB;          // [2][RED_MAXBLOCKS][2]  (|r|^2, r.z) of kernel B / set-up / replacement (slot 0)
   double *PN;          // [2][RED_MAXBLOCKS]     plain loop |r|^2
   double *PZ;          // [2][RED_MAXBLOCKS]     plain loop r.z
   int nA, nB;          // partial counts written by the A-type and B-type kernels

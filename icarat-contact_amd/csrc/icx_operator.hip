@@ -4,14 +4,14 @@
 
 namespace icx {
 
-// Iteration scalars of kernel A (MODE 1) for iteration i, formed by every block.
-// Block 0 publishes r_i.z_i (and, at a stop, the stop record).  Returns false if the
-// solve stops at this iteration.  scratch: 2 NT/64 doubles for this call only.
-template <typename T, int NT>
-__device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverState &ss, double *scratch, int i,
-                                              int blk, T &beta, T &alpha_x) {
+// Iteration scalars of kernel A (MODE 1) for iteration i, formed by every block from
+// the (|r_i|^2, r_i.z_i) totals rr, rz.  Block 0 publishes r_i.z_i (and, at a stop, the
+// stop record).  Returns false if the solve stops at this iteration.
+template <typename T>
+__device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverState &ss, int i, int blk, double rr,
+                                              double rz, T &beta, T &alpha_x) {
   SolverState *st = a.st;
-  const IterScalars sc = iter_scalars<NT>(ss, a.P, i, scratch);
+  const IterScalars sc = iter_scalars(ss, i, rr, rz);
   const bool owed = i > 0 && ss.flush_iter != i;
   if (blk == 0 && threadIdx.x == 0) {
     st->iter_cur = i;
@@ -72,16 +72,10 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   __shared__ double s_sc[2 * (NT / 64)];
   const Grid &g = a.g;
   SolverState ss{};
-  if (a.st) ss = load_state(a.st);
-  if (ss.done) return;   // solve already finished (uniform per launch)
+  if (a.st) ss = load_state(a.st);  // used once the first loads are out
   int it = 0;
   const T *pold = nullptr;
   T *pnew = nullptr;
-  if (MODE == 1) {
-    it = ss.iter;
-    pold = (it & 1) ? a.P0 : a.P1;
-    pnew = (it & 1) ? a.P1 : a.P0;
-  }
 
   const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
   if (blk >= nblocks) return;  // padding block of a batched launch
@@ -246,10 +240,43 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     }
   };
 
-  prefetch(k0 - 1);
+  // The first node layer, its moduli and the partial sums go out before the iteration
+  // index is known: p_old is read from both direction buffers and chosen afterwards.
+  T po_b[NPOS][3];
+  {
+    const uint32_t lo = (k0 - 1 >= 0 && k0 - 1 < g.sz) ? (uint32_t)(k0 - 1) * n_layer : 0u;
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p) {
+      if (!pos_live(p)) continue;
+      const uint32_t o = pos_off[p] + lo;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        pr[p][c] = ld_off(a.in, o + c * cstr);
+        if (MODE == 1) {
+          pv[p][c] = ld_off(a.invd, o + c * cstr);
+          po[p][c] = ld_off((const T *)a.P0, o + c * cstr);
+          po_b[p][c] = ld_off((const T *)a.P1, o + c * cstr);
+        }
+      }
+    }
+  }
   T Ea = load_E(k0 - 1), Eb = T(0);
   OwnedNode<T> na{}, nb{};
-  if (MODE == 1 && !fused_scalars<T, NT>(a, ss, s_sc, it, blk, beta, alpha_x)) return;
+  double rr = 0.0, rz = 0.0;
+  if (MODE == 1) block_total2<NT>(a.P.PB, a.P.nB, s_sc, rr, rz);
+  if (ss.done) return;  // solve already finished (uniform: no kernel writes `done` it reads)
+  if (MODE == 1) {
+    it = ss.iter;
+    pold = (it & 1) ? a.P0 : a.P1;
+    pnew = (it & 1) ? a.P1 : a.P0;
+    if (!(it & 1)) {
+#pragma unroll
+      for (int p = 0; p < NPOS; ++p)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) po[p][c] = po_b[p][c];
+    }
+    if (!fused_scalars<T>(a, ss, it, blk, rr, rz, beta, alpha_x)) return;
+  }
   wait_vmem();
   commit(0);
   prefetch(k0);
@@ -266,7 +293,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   }
   if (MODE == 1) {
     const double tot = block_sum<NT>(dot, s_red);
-    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blk] = tot;
+    if (threadIdx.x == 0) a.P.PA[blk] = tot;
   }
 }
 
@@ -307,7 +334,9 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
     it = ss.iter;
     pold = (it & 1) ? a.P0 : a.P1;
     pnew = (it & 1) ? a.P1 : a.P0;
-    if (!fused_scalars<T, ((NT + 63) / 64) * 64>(a, ss, s_sc, it, blk, beta, alpha_x)) return;
+    double rr, rz;
+    block_total2<((NT + 63) / 64) * 64>(a.P.PB, a.P.nB, s_sc, rr, rz);
+    if (!fused_scalars<T>(a, ss, it, blk, rr, rz, beta, alpha_x)) return;
   }
   const int t = xcd_tile(blk, nblocks);
   const int ti = t % a.tiles_x, tj = t / a.tiles_x;
@@ -391,7 +420,7 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   }
   if (MODE == 1) {
     const double tot = block_sum<((NT + 63) / 64) * 64>(dot, s_red);
-    if (threadIdx.x == 0) a.P.PA[(size_t)(it & 1) * RED_MAXBLOCKS + blk] = tot;
+    if (threadIdx.x == 0) a.P.PA[blk] = tot;
   }
 }
 

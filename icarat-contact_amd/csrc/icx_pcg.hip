@@ -7,9 +7,9 @@ namespace icx {
 
 constexpr int BS = 256;
 
-__host__ __device__ static inline double *pb_slot(const PcgPartials &P, int parity) {
-  return P.PB + (size_t)(parity & 1) * 2 * RED_MAXBLOCKS;
-}
+// (|r|^2, r.z) partials of the fused / mixed forms: written by kernel B (or the set-up
+// or a residual replacement), read by the next kernel A only, so one slot suffices
+__host__ __device__ static inline double *pb_slot(const PcgPartials &P) { return P.PB; }
 
 int pcg_stream_blocks(long long n) { return ew_blocks(n, 4, RED_BLOCKS); }
 
@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(BS) k_pcg_setup(long long n, const T *b, T *r,
   bb = block_sum<BS>(bb, scr);
   rz = block_sum<BS>(rz, scr);
   if (threadIdx.x == 0) {
-    double *slot = pb_slot(P, 1);  // the slot of iteration -1
+    double *slot = pb_slot(P);
     slot[2 * blockIdx.x] = bb;
     slot[2 * blockIdx.x + 1] = rz;
   }
@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(BS) k_pcg_setup(long long n, const T *b, T *r,
 __global__ void __launch_bounds__(BS) k_pcg_setup_finalize(SolverState *st, PcgPartials P, double tol, int maxit,
                                                            double delta2) {
   __shared__ double scr[BS / 64];
-  const double bb = block_total<BS>(pb_slot(P, 1), P.nB, 2, scr);
+  const double bb = block_total<BS>(pb_slot(P), P.nB, 2, scr);
   if (threadIdx.x == 0) {
     st->rhsNorm2 = bb;
     double thr = tol * tol * bb;
@@ -99,13 +99,8 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   const PcgPartials &P = a.P;
   const int nblk = P.nB;
   if (blk >= nblk) return;     // padding block of a batched launch
+  // the state, the first round of vectors and the p.Ap partials go out together
   const SolverState ss = load_state(st);
-  if (ss.done) return;
-  if (ss.pending) {  // kernel A of this iteration stopped: publish the stop
-    if (blk == 0 && threadIdx.x == 0) st->done = ss.pending;
-    return;
-  }
-  const int it = ss.iter_cur;
   const long long nv = n / W;
   const long long stride = (long long)nblk * BS;
   const long long v0 = blk * (long long)BS + threadIdx.x;
@@ -125,7 +120,13 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
     }
   };
   load(v0);
-  const double pAp = block_total<BS>(P.PA + (size_t)(it & 1) * RED_MAXBLOCKS, P.nA, 1, scr);
+  const double pAp = block_total<BS>(P.PA, P.nA, 1, scr);
+  if (ss.done) return;
+  if (ss.pending) {  // kernel A of this iteration stopped: publish the stop
+    if (blk == 0 && threadIdx.x == 0) st->done = ss.pending;
+    return;
+  }
+  const int it = ss.iter_cur;
   const double alpha_d = ((it & 1) ? ss.rz_hist[1] : ss.rz_hist[0]) / pAp;
   if (blk == 0 && threadIdx.x == 0) st->alpha_hist[it & 1] = alpha_d;
   const T alpha = (T)alpha_d;
@@ -170,7 +171,7 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
       s0 += scr2[w];
       s1 += scr2[BS / 64 + w];
     }
-    double *slot = pb_slot(P, it);
+    double *slot = pb_slot(P);
     slot[2 * blk] = s0;
     slot[2 * blk + 1] = s1;
     if (blk == 0) st->iter = it + 1;
@@ -242,14 +243,14 @@ __global__ void __launch_bounds__(BS) k_refine_r(long long n, const SolverState 
   rr = block_sum<BS>(rr, scr);
   rz = block_sum<BS>(rz, scr);
   if (threadIdx.x == 0) {
-    double *slot = pb_slot(P, st->stop_iter - 1);  // replaces the residual of iteration stop_iter
+    double *slot = pb_slot(P);  // replaces the residual of iteration stop_iter
     slot[2 * blockIdx.x] = rr;
     slot[2 * blockIdx.x + 1] = rz;
   }
 }
 __global__ void __launch_bounds__(BS) k_refine_finalize(SolverState *st, PcgPartials P) {
   __shared__ double scr[BS / 64];
-  const double rr = block_total<BS>(pb_slot(P, st->stop_iter - 1), P.nB, 2, scr);
+  const double rr = block_total<BS>(pb_slot(P), P.nB, 2, scr);
   if (threadIdx.x == 0) {
     st->pending = 0;
     st->true_rr = rr;
