@@ -382,7 +382,7 @@ struct PcgWork64 {
   DBuf<double> r, p, t, z, invd;
 };
 struct MixedWork {
-  DBuf<float> r, pa, pb, Ap, xlo, invd, E, b32;
+  DBuf<float> r, pa, pb, Ap, Ap2, xlo, invd, E, b32;  // Ap2: seam halves of kernel A
   DBuf<double> Kx;
 };
 struct FusedWork {
@@ -1060,11 +1060,12 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
       {
         ProfScope ps(c, ICX_PROF_FUSED_A);
         launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p,
-                                      w.Ap.p, j->d_u.p, j->d_state.p, P, S(j), ps.ev());
+                                      w.Ap.p, nullptr, j->d_u.p, j->d_state.p, P, S(j), ps.ev());
       }
       {
         ProfScope ps(c, ICX_PROF_FUSED_B);
-        launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, S(j), ps.ev());
+        launch_pcg_fusedB<double>(N, j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, nullptr, SeamGeom{}, S(j),
+                                  ps.ev());
       }
     }
   };
@@ -1093,6 +1094,14 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
 static PcgPartials mixed_parts(icx_job *j) {
   return j->parts(elastic_fusedA_blocks<float>(j->g), pcg_stream_blocks(j->nvec()));
 }
+// seam layers of the job's kernel A (its tiling, as fusedA_args sets it up)
+static SeamGeom mixed_seams(icx_job *j) {
+  OpArgs<float> a;
+  MixedWork &w = j->mw;
+  fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p, w.Ap2.p, w.xlo.p,
+                     j->d_state.p, mixed_parts(j), a);
+  return fusedA_seams(a);
+}
 
 // Set-up of a mixed solve: FP32 operands, x = 0, r = b, partials; st receives the state
 // after the set-up (done = 1 at once for a zero right-hand side).
@@ -1101,6 +1110,7 @@ static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st
   const long long N = j->nvec();
   MixedWork &w = j->mw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N)); HIPCHK(c, w.Ap.alloc(N));
+  HIPCHK(c, w.Ap2.alloc(N));
   HIPCHK(c, w.xlo.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.E.alloc(j->g.ne)); HIPCHK(c, w.Kx.alloc(N));
   HIPCHK(c, w.b32.alloc(N));
   const PcgPartials P = mixed_parts(j);
@@ -1135,11 +1145,12 @@ static void mixed_iteration(icx_job *j) {
   {
     ProfScope ps(c, ICX_PROF_FUSED_A);
     launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
-                                 w.xlo.p, j->d_state.p, P, S(j), ps.ev());
+                                 w.Ap2.p, w.xlo.p, j->d_state.p, P, S(j), ps.ev());
   }
   {
     ProfScope ps(c, ICX_PROF_FUSED_B);
-    launch_pcg_fusedB<float>(j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, S(j), ps.ev());
+    launch_pcg_fusedB<float>(j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, w.Ap2.p, mixed_seams(j), S(j),
+                             ps.ev());
   }
 }
 
@@ -1233,10 +1244,10 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
     const PcgPartials P = mixed_parts(j);
     OpArgs<float> a;
     const int nb = fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
-                                      w.xlo.p, j->d_state.p, P, a);
+                                      w.Ap2.p, w.xlo.p, j->d_state.p, P, a);
     if (j->g.ndim == 3) { a3.push_back(a); o3.push_back(o3.back() + pad8(nb)); }
     else { a2.push_back(a); o2.push_back(o2.back() + pad8(nb)); }
-    ab.push_back(BArgs<float>{j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P});
+    ab.push_back(BArgs<float>{j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, w.Ap2.p, fusedA_seams(a)});
     ob.push_back(ob.back() + pad8(P.nB));
     sp.push_back(j->d_state.p);
     maxN = std::max(maxN, j->nvec());
@@ -2033,7 +2044,9 @@ icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
   add(j->d_state); add(j->d_scalars);
   for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(*d);
   for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd}) add(*d);
-  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.xlo, &j->mw.invd, &j->mw.E, &j->mw.b32}) add(*d);
+  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.Ap2, &j->mw.xlo, &j->mw.invd, &j->mw.E,
+                  &j->mw.b32})
+    add(*d);
   add(j->mw.Kx);
   for (auto *d : {&j->hw.invd, &j->hw.b, &j->hw.x, &j->hw.r, &j->hw.p, &j->hw.t, &j->hw.z}) add(*d);
   add(j->cw.off); add(j->cw.w); add(j->cw.aa);

@@ -28,6 +28,18 @@ __device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70)
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Value held by lane + 1 of the same 16-lane row (DPP row_shl:1); the last lane of
+// each row reads 0.
+__device__ __forceinline__ float row_next(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x101, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double row_next(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xFFFFFFFF), 0x101, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x101, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Snapshot of a solver state at kernel entry (scalar loads: one round trip).  Every
 // field an iteration kernel reads is written only by other kernels (kernel A records a
 // stop in `pending`, kernel B publishes it as `done`), so all waves of all blocks of a

@@ -78,6 +78,56 @@ struct LaunchEvents {
   hipEvent_t start, stop;
 };
 
+// Unsigned division by a run-time constant d >= 1 for dividends n < 2^29 (DOF, node and
+// layer indices: at most 3 * 2^27 DOFs): q = (n * m) >> s with l = ceil(log2 d),
+// s = 29 + l, m = ceil(2^s / d).  n m / 2^s = n/d + n (m d - 2^s) / (d 2^s), and
+// 0 <= m d - 2^s < d with n < 2^29 bounds the second term below 2^-l <= 1/d, which
+// the fractional part of n/d (at most 1 - 1/d) cannot absorb into the next integer:
+// the floor is exact (Granlund-Montgomery).  d > 2^(l-1) gives m <= 2^30: a 32-bit
+// multiplier, and a 32 x 32 -> 64-bit product.
+struct Magic {
+  uint32_t m;
+  int s;
+};
+inline Magic magic_u29(long long d) {
+  int l = 0;
+  while ((1LL << l) < d) ++l;
+  Magic r;
+  r.s = 29 + l;
+  r.m = (uint32_t)(((1ULL << r.s) + (unsigned long long)d - 1) / (unsigned long long)d);
+  return r;
+}
+__host__ __device__ inline uint32_t udiv(uint32_t n, Magic g) {
+  return (uint32_t)(((unsigned long long)n * (unsigned long long)g.m) >> g.s);
+}
+
+// Seam layers of the FP32 hexa8 kernel A (seam split along z): node layers k = TK s,
+// 0 < k <= kmax, receive their lower half in y2.  kmax = 0: no seams.
+struct SeamGeom {
+  Magic nn, L, TK;              // divisors: nodes, nodes per layer, tile depth
+  uint32_t nnodes, layer, tk, kmax;
+};
+
+// Seam flags of the nw consecutive DOFs i0 .. i0+nw-1 (SoA planes of sg.nnodes nodes,
+// node layers of sg.layer >= 4 nodes, nw <= 4): bit w set when DOF i0+w lies on a seam
+// layer k = TK s, 0 < k <= kmax.  The run crosses at most one layer boundary; a DOF in
+// the next component plane is on layer 0.
+__host__ __device__ inline uint32_t seam_flags(uint32_t i0, int nw, const SeamGeom &sg) {
+  const uint32_t c = udiv(i0, sg.nn);
+  const uint32_t n = i0 - c * sg.nnodes;
+  const uint32_t k = udiv(n, sg.L);
+  const uint32_t rem = n - k * sg.layer;
+  const uint32_t kr = k - udiv(k, sg.TK) * sg.tk;
+  const bool s0 = kr == 0 && k > 0 && k <= sg.kmax;
+  const bool s1 = kr + 1 == sg.tk && k + 1 <= sg.kmax;
+  uint32_t f = 0;
+  for (int w = 0; w < nw; ++w) {
+    if (n + w >= sg.nnodes) continue;
+    f |= (uint32_t)(rem + w >= sg.layer ? s1 : s0) << w;
+  }
+  return f;
+}
+
 // ---- structured operators (icx_operator.hip) -------------------------------------
 // Arguments of one launch of an elastic operator kernel (one job).
 template <typename T> struct OpArgs {
@@ -90,6 +140,7 @@ template <typename T> struct OpArgs {
   const T *invd;      // MODE 1
   T *P0, *P1;         // MODE 1: direction buffers, p_i lives in P[i & 1]
   T *y;               // K x (MODE 0) / K p_i (MODE 1)
+  T *y2;              // FP32 hexa8 MODE 1: lower halves of the seam layers of K p_i
   T *x;               // MODE 1: solution accumulator (deferred update)
   SolverState *st;    // nullable in MODE 0
   PcgPartials P;      // MODE 1
@@ -115,15 +166,19 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
 
 // Fused kernel A of iteration i = st->iter (published as st->iter_cur): scalars from the partials,
 // p_i = invd r + beta_i p_{i-1} (P[i&1] <- ..., P[(i-1)&1] read), Ap_i = K p_i,
-// x += alpha_{i-1} p_{i-1}, partial p_i.Ap_i -> PA[i&1].
+// x += alpha_{i-1} p_{i-1}, partial p_i.Ap_i -> PA[i&1].  FP32 hexa8 (seam split): the
+// seam layers of Ap_i are completed by kernel B as Ap + Ap2 (fusedA_seams gives them);
+// Ap2 is touched on the seam layers only.
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
+                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st,
                            PcgPartials P, hipStream_t s, const LaunchEvents *ev = nullptr);
 // kernel A's arguments for one job; returns its block count
 template <typename T>
 int fusedA_args(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask, const T *r, const T *invd,
-                T *P0, T *P1, T *Ap, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a);
+                T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a);
+// the seam layers of a kernel A launch (kmax = 0 when it has none)
+template <typename T> SeamGeom fusedA_seams(const OpArgs<T> &a);
 // kernel A for a batch of jobs of one dimension (args and off in device memory)
 template <typename T>
 void launch_fusedA_batch(int ndim, const OpArgs<T> *args, const int *off, int njobs, int total_blocks,
@@ -151,6 +206,8 @@ template <typename T> struct BArgs {
   const T *Ap;
   const T *invd;
   PcgPartials P;
+  const T *Ap2;       // lower halves of kernel A's seam layers (read where sg says)
+  SeamGeom sg;
 };
 // copies the states *src[j] of n jobs to dst[j]
 void launch_gather_states(SolverState *const *src, SolverState *dst, int n, hipStream_t s);
@@ -158,9 +215,10 @@ void launch_gather_states(SolverState *const *src, SolverState *dst, int n, hipS
 template <typename T>
 void launch_pcg_fusedB_batch(const BArgs<T> *args, const int *off, int njobs, int total_blocks, hipStream_t s);
 // fused kernel B of iteration i: alpha_i = rz_i / pAp_i; r -= alpha_i Ap; (|r|^2, r.z) -> PB[i&1]
+// Ap2 / sg: seam layers of the FP32 hexa8 kernel A (sg.kmax = 0: none).
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
-                       hipStream_t s, const LaunchEvents *ev = nullptr);
+                       const T *Ap2, const SeamGeom &sg, hipStream_t s, const LaunchEvents *ev = nullptr);
 // fused FP64 form at the stop: x += alpha_owed p_{stop-1}
 void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0, const double *P1,
                       hipStream_t s);

@@ -37,14 +37,27 @@ __device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverSt
 // layer ez+1 go out before the element phase of layer ez -- and the loop is unrolled
 // by two with alternating register sets, so no wait on a load in flight is ever
 // forced before its consumer.  Per layer: every thread applies one element (the
-// elements touching the owned nodes), writes its 24 outputs to LDS, and every owned
-// node sums its four in-layer contributions; contributions to the layer above are
-// carried in registers.  Two barriers per layer; summation order fixed.
+// elements touching the owned nodes).  A row of TI+1 = 16 elements is one 16-lane DPP
+// row, so the contributions of element pairs along x are added across lanes
+// (row_shl:1); of the 12 pair sums a thread forms, the 6 that belong to the node row
+// below go through LDS, and every owned node adds its own 6, those 6 and the carry of
+// the layer below (contributions to the layer above are carried in registers).  Two
+// barriers per layer; summation order fixed.
 //
 // MODE 0: y = K x.   MODE 1 (fused kernel A of PCG iteration i): p_i = invd r +
 // beta_i p_{i-1} formed in the LDS loader, y = K p_i, x += alpha_{i-1} p_{i-1} on the
 // owned nodes, partial p_i . y.  The loads of the first node layer are issued before
 // the iteration scalars are reduced.
+//
+// Tiling along z.  MODE 0 and the FP64 MODE 1: a block owns node layers [k0, k1) and
+// applies the element layers k0-1 .. k1-1, one of them also applied by the block
+// below.  FP32 MODE 1 (seam split): a block applies element layers [k0, k1) only, so
+// every element is applied once; node layers k0 .. k1-1 are owned (x, p_i and y
+// stored), and its top node layer k1 receives only the contributions of element layer
+// k1-1: complete in the top tile (which owns it), otherwise the lower half of a seam,
+// stored to y2 at the same index.  Kernel B adds y2 to y on the seam layers.  p_i . y
+// is summed over the node layers k0 .. k1 of every block, which counts each half of a
+// seam once.
 // ---------------------------------------------------------------------------------
 template <typename T> struct OwnedNode {
   T x[3], po[3];
@@ -67,7 +80,8 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   constexpr int NPOS = (LN + NT - 1) / NT;
   constexpr uint32_t W = sizeof(T);
   __shared__ T s_in[3][3][LN];
-  __shared__ T s_out[24][NT];
+  static_assert(TI + 1 == 16, "element rows are 16-lane DPP rows");
+  __shared__ T s_h[6][NT];
   __shared__ double s_red[NT / 64];
   __shared__ double s_sc[2 * (NT / 64)];
   const Grid &g = a.g;
@@ -81,8 +95,11 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   if (blk >= nblocks) return;  // padding block of a batched launch
   const int t = xcd_tile(blk, nblocks);
   const int ti = t % a.tiles_x, tj = (t / a.tiles_x) % a.tiles_y, tk = t / (a.tiles_x * a.tiles_y);
+  constexpr bool SEAM = MODE == 1 && sizeof(T) == 4;
   const int i0 = ti * TI, j0 = tj * TJ, k0 = tk * a.TK;
-  const int k1 = min(k0 + a.TK, g.sz);
+  const int k1 = SEAM ? min(k0 + a.TK, g.nz) : min(k0 + a.TK, g.sz);
+  const bool top_tile = tk == a.tiles_z - 1;
+  const int kfirst = SEAM ? k0 : k0 - 1;  // first node layer and first element layer
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   // element of this thread: (tx, ty) in a (TI+1) x (TJ+1) patch whose corner nodes
@@ -150,7 +167,8 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     return ld_off(a.E, e_off + lo);
   };
   auto load_node = [&](int k, OwnedNode<T> &o) {
-    const uint32_t n = n_off + ((k >= k0 && k < k1) ? (uint32_t)k * n_layer : 0u);
+    const bool in = k >= k0 && (SEAM ? k <= k1 : k < k1);
+    const uint32_t n = n_off + (in ? (uint32_t)k * n_layer : 0u);
     o.m = a.mask ? (uint32_t)ld_off(a.mask, n / W) : 0u;
     if (MODE == 1) {
 #pragma unroll
@@ -198,6 +216,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     En = load_E(ez + 1);
     load_node(ez + 1, on);
     __syncthreads();
+    T hown[2][3];
     {
       T v[3][8];
       const bool ok = ex_ok && ez >= 0 && ez < g.nz;
@@ -209,29 +228,27 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
           v[c][q] = s_in[qc ? b1 : b0][c][(tx + qa) + LW * (ty + qb)];
         }
       elastic_hex8<T>(v, a.op, ok ? Ec : T(0));
+      // corner q = qa + 2 qb + 4 qc of element (tx, ty) is node (tx + qa, ty + qb) of
+      // the halo patch; owned node (lx, ly) is patch node (lx + 1, ly + 1), the corner
+      // qa = qb = 1 of element (lx, ly).  Pair sums along x: this element's qa = 1
+      // corner plus the qa = 0 corner of element tx + 1.
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+      for (int qc = 0; qc < 2; ++qc)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) s_out[c * 8 + q][tid] = v[c][q];
+        for (int c = 0; c < 3; ++c) {
+          s_h[3 * qc + c][tid] = v[c][1 + 4 * qc] + row_next(v[c][4 * qc]);  // qb = 0: node row ty - 1
+          hown[qc][c] = v[c][3 + 4 * qc] + row_next(v[c][2 + 4 * qc]);     // qb = 1: node row ty
+        }
     }
     __syncthreads();
     have_out = false;
     if (node_thread) {
       T yb[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) { yb[c] = acc_top[c]; acc_top[c] = T(0); }
-#pragma unroll
-      for (int db = 0; db < 2; ++db)
-#pragma unroll
-        for (int da = 0; da < 2; ++da) {
-          const int t2 = (lx + da) + (TI + 1) * (ly + db);
-          const int q = (1 - da) + 2 * (1 - db);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            yb[c] += s_out[c * 8 + q][t2];
-            acc_top[c] += s_out[c * 8 + q + 4][t2];
-          }
-        }
+      for (int c = 0; c < 3; ++c) {
+        yb[c] = (acc_top[c] + hown[0][c]) + s_h[c][tid + TI + 1];
+        acc_top[c] = hown[1][c] + s_h[3 + c][tid + TI + 1];
+      }
       if (ez >= k0 && node_ok) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) yv[c] = ((oc.m >> c) & 1) ? T(0) : yb[c];
@@ -244,7 +261,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   // index is known: p_old is read from both direction buffers and chosen afterwards.
   T po_b[NPOS][3];
   {
-    const uint32_t lo = (k0 - 1 >= 0 && k0 - 1 < g.sz) ? (uint32_t)(k0 - 1) * n_layer : 0u;
+    const uint32_t lo = (kfirst >= 0 && kfirst < g.sz) ? (uint32_t)kfirst * n_layer : 0u;
 #pragma unroll
     for (int p = 0; p < NPOS; ++p) {
       if (!pos_live(p)) continue;
@@ -260,7 +277,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
       }
     }
   }
-  T Ea = load_E(k0 - 1), Eb = T(0);
+  T Ea = load_E(kfirst), Eb = T(0);
   OwnedNode<T> na{}, nb{};
   double rr = 0.0, rz = 0.0;
   if (MODE == 1) block_total2<NT>(a.P.PB, a.P.nB, s_sc, rr, rz);
@@ -277,10 +294,31 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     }
     if (!fused_scalars<T>(a, ss, it, blk, rr, rz, beta, alpha_x)) return;
   }
+  if (SEAM) load_node(k0, na);  // the first step's node phase completes node layer k0 (p_old known now)
   wait_vmem();
   commit(0);
-  prefetch(k0);
-  int ez = k0 - 1, tt = 0;
+  prefetch(kfirst + 1);
+  // top node layer k1 of the seam split: the carried contributions of element layer
+  // k1-1 (node o, LDS buffer buf)
+  auto store_top = [&](int buf, const OwnedNode<T> &o) {
+    if (!SEAM || !node_ok) return;
+    const uint32_t n = n_off + (uint32_t)k1 * n_layer;
+    const int ctr = (lx + 1) + LW * (ly + 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T yt = ((o.m >> c) & 1) ? T(0) : acc_top[c];
+      const T pv_ = s_in[buf][c][ctr];
+      dot = fma_t((double)pv_, (double)yt, dot);
+      if (top_tile) {
+        st_off(a.y, n + c * cstr, yt);
+        st_off(pnew, n + c * cstr, pv_);
+        st_off(a.x, n + c * cstr, fma_t(alpha_x, o.po[c], o.x[c]));
+      } else {
+        st_off(a.y2, n + c * cstr, yt);
+      }
+    }
+  };
+  int ez = kfirst, tt = 0;
   for (; ez + 1 < k1; ez += 2, tt += 2) {
     layer(ez, tt, Ea, Eb, na, nb);
     layer(ez + 1, tt + 1, Eb, Ea, nb, na);
@@ -288,8 +326,10 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   if (ez < k1) {
     layer(ez, tt, Ea, Eb, na, nb);
     store_out(ez, tt % 3, na);
+    store_top((tt + 1) % 3, nb);
   } else {
     store_out(ez - 1, (tt + 2) % 3, nb);
+    store_top(tt % 3, na);
   }
   if (MODE == 1) {
     const double tot = block_sum<NT>(dot, s_red);
@@ -442,21 +482,37 @@ template <> struct HexTile<double> { static constexpr int TI = 15, TJ = 7; };
 constexpr int QUAD_TI = 127;
 template <typename T> constexpr int NTHREADS3 = (HexTile<T>::TI + 1) * (HexTile<T>::TJ + 1);
 
+// seam: tile the element layers (FP32 kernel A, seam split) instead of the node layers
 template <typename T>
-static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads) {
+static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads, bool seam = false) {
   if (g.ndim == 3) {
     constexpr int TI = HexTile<T>::TI, TJ = HexTile<T>::TJ;
     a.tiles_x = (g.sx + TI - 1) / TI;
     a.tiles_y = (g.sy + TJ - 1) / TJ;
     // z depth: enough blocks to keep 256 CUs busy, at most one partial per block
     const long long cols = (long long)a.tiles_x * a.tiles_y;
+    const int layers = seam ? g.nz : g.sz;
+    auto blocks = [&](int t) { return cols * ((layers + t - 1) / t); };
     int tk = 16;
-    if (g.tile_depth > 0) tk = g.tile_depth;
-    else
-      while (tk > 2 && cols * ((g.sz + tk - 1) / tk) < 2048) tk = tk > 4 ? tk >> 1 : tk - 1;
-    while (cols * ((g.sz + tk - 1) / tk) > RED_MAXBLOCKS) tk <<= 1;
+    if (g.tile_depth > 0) {
+      tk = g.tile_depth;
+    } else if (seam) {
+      // seam split (no recomputed layer): the deepest brick that still gives about 1.4
+      // rounds of four blocks per CU, else at least 400 blocks, else depth 2 (measured
+      // optimum: 3 at 96x48x48, 6 at 192x96x96)
+      tk = 2;
+      const int cand[] = {6, 4, 3};
+      bool found = false;
+      for (int t : cand)
+        if (!found && blocks(t) >= 1434) { tk = t; found = true; }
+      for (int t : cand)
+        if (!found && blocks(t) >= 400) { tk = t; found = true; }
+    } else {
+      while (tk > 2 && blocks(tk) < 2048) tk = tk > 4 ? tk >> 1 : tk - 1;
+    }
+    while (blocks(tk) > RED_MAXBLOCKS) tk <<= 1;
     a.TK = tk;
-    a.tiles_z = (g.sz + tk - 1) / tk;
+    a.tiles_z = (layers + tk - 1) / tk;
     nblocks = (int)(cols * a.tiles_z);
     nthreads = (TI + 1) * (TJ + 1);
   } else {
@@ -472,11 +528,14 @@ static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads
   }
 }
 
+// kernel A of the fused / mixed forms: the FP32 hexa8 kernel uses the seam split
+template <typename T> constexpr bool fusedA_seam(int ndim) { return ndim == 3 && sizeof(T) == 4; }
+
 template <typename T>
 int elastic_fusedA_blocks(const Grid &g) {
   OpArgs<T> a{};
   int nb, nt;
-  setup_tiles(g, a, nb, nt);
+  setup_tiles(g, a, nb, nt, fusedA_seam<T>(g.ndim));
   return nb;
 }
 
@@ -495,21 +554,34 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
 
 template <typename T>
 int fusedA_args(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask, const T *r, const T *invd,
-                T *P0, T *P1, T *Ap, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a) {
+                T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a) {
   a = OpArgs<T>{};
   a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = r; a.invd = invd; a.P0 = P0; a.P1 = P1;
-  a.y = Ap; a.x = x; a.st = st; a.P = P;
+  a.y = Ap; a.y2 = Ap2; a.x = x; a.st = st; a.P = P;
   int nb, nt;
-  setup_tiles(g, a, nb, nt);
+  setup_tiles(g, a, nb, nt, fusedA_seam<T>(g.ndim));
   return nb;
+}
+
+template <typename T> SeamGeom fusedA_seams(const OpArgs<T> &a) {
+  SeamGeom sg{};
+  if (!fusedA_seam<T>(a.g.ndim) || a.tiles_z < 2) return sg;
+  sg.nn = magic_u29(a.g.nn);
+  sg.L = magic_u29((long long)a.g.sx * a.g.sy);
+  sg.TK = magic_u29(a.TK);
+  sg.nnodes = (uint32_t)a.g.nn;
+  sg.layer = (uint32_t)a.g.sx * a.g.sy;
+  sg.tk = (uint32_t)a.TK;
+  sg.kmax = (uint32_t)(a.tiles_z - 1) * a.TK;
+  return sg;
 }
 
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *x, SolverState *st,
+                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st,
                            PcgPartials P, hipStream_t s, const LaunchEvents *ev) {
   OpArgs<T> a;
-  const int nb = fusedA_args(g, op, E, mask, r, invd, P0, P1, Ap, x, st, P, a);
+  const int nb = fusedA_args(g, op, E, mask, r, invd, P0, P1, Ap, Ap2, x, st, P, a);
   if (g.ndim == 3) launch_ev((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), nb, (int)NTHREADS3<T>, s, ev, a);
   else launch_ev((k_quad4<T, QUAD_TI, 1>), nb, QUAD_TI + 1, s, ev, a);
   ICX_CHECK_LAUNCH();
@@ -528,8 +600,10 @@ void launch_fusedA_batch(int ndim, const OpArgs<T> *args, const int *off, int nj
   ICX_CHECK_LAUNCH();
 }
 template int fusedA_args<float>(const Grid &, const ElasticOp<float> &, const float *, const uint8_t *, const float *,
-                                const float *, float *, float *, float *, float *, SolverState *, PcgPartials,
+                                const float *, float *, float *, float *, float *, float *, SolverState *, PcgPartials,
                                 OpArgs<float> &);
+template SeamGeom fusedA_seams<float>(const OpArgs<float> &);
+template SeamGeom fusedA_seams<double>(const OpArgs<double> &);
 template void launch_fusedA_batch<float>(int, const OpArgs<float> *, const int *, int, int, hipStream_t);
 
 template int elastic_fusedA_blocks<double>(const Grid &);
@@ -542,11 +616,11 @@ template void launch_elastic_matvec<float>(const Grid &, const ElasticOp<float> 
                                            hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<double>(const Grid &, const ElasticOp<double> &, const double *,
                                             const uint8_t *, const double *, const double *, double *,
-                                            double *, double *, double *, SolverState *, PcgPartials,
+                                            double *, double *, double *, double *, SolverState *, PcgPartials,
                                             hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<float>(const Grid &, const ElasticOp<float> &, const float *,
                                            const uint8_t *, const float *, const float *, float *, float *,
-                                           float *, float *, SolverState *, PcgPartials, hipStream_t,
+                                           float *, float *, float *, SolverState *, PcgPartials, hipStream_t,
                                            const LaunchEvents *);
 
 // ---------------------------------------------------------------------------------

@@ -107,8 +107,12 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   V *rv = reinterpret_cast<V *>(r);
   const V *av = reinterpret_cast<const V *>(Ap);
   const V *dv = reinterpret_cast<const V *>(invd);
-  V R[U], A[U], D[U];
+  const V *a2v = reinterpret_cast<const V *>(a.Ap2);
+  const bool seams = a.sg.kmax > 0;
+  V R[U], A[U], D[U], A2[U];
+  uint32_t sf = 0;  // seam flags: bit W u + w for lane w of chunk u
   auto load = [&](long long base) {
+    sf = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const long long v = base + u * stride;
@@ -116,6 +120,11 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
         R[u] = rv[v];
         A[u] = av[v];
         D[u] = dv[v];
+        if (seams) {
+          const uint32_t f = seam_flags((uint32_t)(v * W), W, a.sg);
+          sf |= f << (W * u);
+          if (f) A2[u] = a2v[v];
+        }
       }
     }
   };
@@ -139,7 +148,9 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
         V out;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-          const T ri = fma_t(-alpha, lane(A[u], w), lane(R[u], w));
+          T apw = lane(A[u], w);
+          if ((sf >> (W * u + w)) & 1) apw += lane(A2[u], w);
+          const T ri = fma_t(-alpha, apw, lane(R[u], w));
           set_lane(out, w, ri);
           rr = fma_t((double)ri, (double)ri, rr);
           rzn = fma_t((double)ri, (double)(lane(D[u], w) * ri), rzn);
@@ -152,7 +163,9 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   // scalar tail (n not a multiple of W)
   const long long tail = nv * W + v0;
   if (tail < n) {
-    const T ri = fma_t(-alpha, Ap[tail], r[tail]);
+    T apt = Ap[tail];
+    if (seams && seam_flags((uint32_t)tail, 1, a.sg)) apt += a.Ap2[tail];
+    const T ri = fma_t(-alpha, apt, r[tail]);
     r[tail] = ri;
     rr = fma_t((double)ri, (double)ri, rr);
     rzn = fma_t((double)ri, (double)(invd[tail] * ri), rzn);
@@ -192,8 +205,8 @@ __global__ void __launch_bounds__(BS)
 
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
-                       hipStream_t s, const LaunchEvents *ev) {
-  const BArgs<T> a{n, st, r, Ap, invd, P};
+                       const T *Ap2, const SeamGeom &sg, hipStream_t s, const LaunchEvents *ev) {
+  const BArgs<T> a{n, st, r, Ap, invd, P, Ap2, sg};
   launch_ev((k_pcg_fusedB<T, 4>), P.nB, BS, s, ev, a);
   ICX_CHECK_LAUNCH();
 }
@@ -286,7 +299,7 @@ template void launch_pcg_fusedB_batch<float>(const BArgs<float> *, const int *, 
   template void launch_pcg_setup<T>(long long, SolverState *, const T *, T *, const T *, PcgPartials,    \
                                     double, int, double, hipStream_t);                                   \
   template void launch_pcg_fusedB<T>(long long, SolverState *, T *, const T *, const T *, PcgPartials, \
-                                     hipStream_t, const LaunchEvents *);
+                                     const T *, const SeamGeom &, hipStream_t, const LaunchEvents *);
 ICX_INST_PCG(double)
 ICX_INST_PCG(float)
 

@@ -36,13 +36,13 @@ def job_spec(rng, elems_lo, elems_hi):
 
 def footprint_estimate(ndim, div):
     """Device bytes of one served job (mixed solve, cone filter, OC design loop):
-    80 B per DOF, 92 B per element, 1 B per node, 0.4 MB of partial sums and state;
+    84 B per DOF, 92 B per element, 1 B per node, 0.4 MB of partial sums and state;
     icx_job_footprint reports the same buffers once they exist."""
     d0, d1, d2 = div
     nn = (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
     ne = d0 * d1 * (d2 if ndim == 3 else 1)
     n = ndim * nn
-    return 80 * n + 92 * ne + nn + 400 * 1024
+    return 84 * n + 92 * ne + nn + 400 * 1024
 
 
 def make_job(ctx, ndim, div):

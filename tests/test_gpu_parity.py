@@ -117,6 +117,24 @@ def test_mixed_form_meets_true_residual(ctx):
     assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-7
 
 
+@pytest.mark.parametrize("depth", [0, 1, 2, 3, 5, 40])
+def test_mixed_form_every_tile_depth(ctx, depth):
+    """The FP32 kernel A tiles the element layers along z (seam split: each seam layer
+    is completed by kernel B).  Depth 1 makes every interior node layer a seam, 3 and 5
+    leave a shorter top tile on 7 element layers, 40 a single tile; every depth meets the
+    true residual and the direct solution."""
+    job, pr, rho, ud = _solve_case(ctx, nx=9, ny=5, nz=7, seed=11)
+    ctx.set_option(icx.OPT_TILE_DEPTH, depth)
+    try:
+        u, info = job.solve(icx.SOLVE_MIXED, tol=1e-10, maxit=20000)
+    finally:
+        ctx.set_option(icx.OPT_TILE_DEPTH, 0)
+    assert info["converged"]
+    b = pr.assemble(rho)
+    assert np.linalg.norm(b - pr.csr() @ u) / np.linalg.norm(b) < 1e-10
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-7
+
+
 def test_2d_solve_and_node_values(ctx):
     job = icx.cantilever_2d(ctx, 40, 14)
     pr = O.cantilever_2d(40, 14)
