@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--serve-elems", type=float, nargs=2, default=[5e3, 3e5])
     ap.add_argument("--serve-budget", type=float, default=240e9)
     ap.add_argument("--pmc-iters", type=int, default=200)
+    ap.add_argument("--output", action="store_true", help="time the output step (mises, reactions, VTK)")
     return ap.parse_args()
 
 
@@ -170,6 +171,57 @@ def batched(args):
                        "iterations_min_max": [min(i["iterations"] for i in infos),
                                               max(i["iterations"] for i in infos)],
                        "all_converged": all(i["relres"] < args.tol for i in infos)}}
+    print(json.dumps(line))
+
+
+def output_step(args):
+    """The output step on the large mesh after one mixed solve: element-mean von Mises
+    stress and reactions (device work + copy of the result), and the binary VTK file
+    (queued device work and pinned copies, then the file write)."""
+    import tempfile
+
+    import numpy as np
+
+    import icx
+    nx, ny, nz = args.large
+    ctx = icx.Context(0)
+    job = icx.cantilever_3d(ctx, nx, ny, nz)
+    job.set_design(icx.random_density(job.nelx, SEED))
+    job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=50000, want_u=False)
+    ctx.synchronize()
+
+    def timed(fn, reps=5):
+        fn()
+        ctx.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    mises_ms = timed(job.mises)
+    react_ms = timed(job.reactions)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "res1.vtk")
+        begin_ms = timed(job.output_begin, reps=1)
+        job.output_write(path)
+        job.output_begin()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        job.output_write(path)
+        write_ms = (time.perf_counter() - t0) * 1e3
+        size = os.path.getsize(path)
+    nn, ne = job.numnp, job.nelx
+    line = {"metric": "output step: von Mises, reactions, binary VTK", "value": (begin_ms + write_ms),
+            "unit": "ms per design iteration", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "f64 (file: f32)", "data": "synthetic (seeded densities)",
+            "config": {"workload": "hexa8 cantilever %dx%dx%d after one mixed solve" % (nx, ny, nz),
+                       "nodes": nn, "elements": ne,
+                       "mises_ms_incl_copy": mises_ms, "reactions_ms_incl_copy": react_ms,
+                       "vtk_device_and_copies_ms": begin_ms, "vtk_file_write_ms": write_ms,
+                       "vtk_file_MB": size / 1e6, "vtk_write_MBps": size / 1e6 / (write_ms / 1e3)}}
     print(json.dumps(line))
 
 
@@ -327,6 +379,9 @@ def main():
         return
     if args.serve:
         serving(args)
+        return
+    if args.output:
+        output_step(args)
         return
     if args.design_loop:
         design_loop(args)

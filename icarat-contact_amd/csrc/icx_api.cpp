@@ -90,6 +90,27 @@ template <typename T> struct DBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Pinned host buffer (hipHostMalloc; freed with the owner).
+template <typename T> struct HBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  HBuf() = default;
+  HBuf(const HBuf &) = delete;
+  HBuf &operator=(const HBuf &) = delete;
+  ~HBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count == n && p) return hipSuccess;
+    release();
+    n = count;
+    return hipHostMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T), hipHostMallocDefault);
+  }
+};
+
 // Event pair for one launch of the profiled kernel, bound to the launch itself
 // (LaunchEvents: the kernel's own start / end timestamps).
 struct ProfScope {
@@ -406,6 +427,20 @@ struct MMAWork {
   int iter = 0;
   DBuf<double> low, upp, alpha, beta, p0, q0, p1, q1, xold1, xold2;
 };
+// Output step (icx_output_begin / _write): device temporaries, pinned host copies of
+// the displacement (node-major), density and mises, and the event that completes them.
+struct OutputWork {
+  DBuf<double> ufull, nodes, mises;
+  HBuf<double> h_disp, h_dens, h_mises;
+  std::vector<uint32_t> geo;  // big-endian points, cells and cell types (fixed per job)
+  size_t geo_points = 0, geo_cells = 0;
+  hipEvent_t done = nullptr;
+  bool pending = false;
+  ~OutputWork() {
+    if (done) (void)hipEventDestroy(done);
+  }
+};
+
 struct OptWork {
   bool ready = false;
   bool in_step = false;  // between icx_opt_step_begin and icx_opt_step_end
@@ -468,6 +503,7 @@ struct icx_job {
   FusedWork fw;
   HelmWork hw;
   ConeWork cw;
+  OutputWork ow;
   MMAWork mma;
   OptWork opt;
 
@@ -1388,11 +1424,177 @@ icx_status icx_get_node_values(const icx_job *jc, double *nodeval) {
   return ICX_OK;
 }
 
+// ---- output step ------------------------------------------------------------------
+// node.val of the current solution as a grid vector (free DOFs from u, prescribed
+// values from the Dirichlet grid, flushed) in j->ow.ufull
+static icx_status output_displacement(icx_job *j) {
+  icx_ctx *c = j->ctx;
+  const long long N = j->nvec();
+  HIPCHK(c, j->ow.ufull.alloc(N));
+  launch_full_displacement(N, j->d_u.p, j->any_dirich_value ? j->d_dirich.p : nullptr, j->ow.ufull.p, S(j));
+  return ICX_OK;
+}
+static MisesMat mises_mat(const icx_job *j) {
+  MisesMat m{};
+  m.state = j->prob.state;
+  m.nu = j->prob.poisson;
+  for (int d = 0; d < 3; ++d) m.h[d] = j->el.h[d];
+  return m;
+}
+
+icx_status icx_mises(icx_job *j, double *out) {
+  if (!j || !out) return ICX_ERR_ARG;
+  if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
+  icx_ctx *c = j->ctx;
+  icx_status rc = output_displacement(j);
+  if (rc) return rc;
+  HIPCHK(c, j->ow.mises.alloc(j->g.ne));
+  launch_mises(j->g, mises_mat(j), j->d_E.p, j->ow.ufull.p, j->ow.mises.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(out, j->ow.mises.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_reactions(icx_job *j, double *out) {
+  if (!j || !out) return ICX_ERR_ARG;
+  if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
+  icx_ctx *c = j->ctx;
+  icx_status rc = output_displacement(j);
+  if (rc) return rc;
+  DBuf<double> ku;
+  HIPCHK(c, ku.alloc(j->nvec()));
+  HIPCHK(c, j->ow.nodes.alloc(j->g.nn * 3));
+  // K u over all DOFs: the operator without the constraint mask
+  launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->ow.ufull.p, ku.p, nullptr, S(j));
+  launch_grid_to_nodes(j->g, j->nc, ku.p, j->ow.nodes.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(out, j->ow.nodes.p, j->g.nn * 3 * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_output_begin(icx_job *j) {
+  if (!j) return ICX_ERR_ARG;
+  if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
+  icx_ctx *c = j->ctx;
+  OutputWork &o = j->ow;
+  icx_status rc = output_displacement(j);
+  if (rc) return rc;
+  HIPCHK(c, o.nodes.alloc(j->g.nn * 3));
+  HIPCHK(c, o.mises.alloc(j->g.ne));
+  HIPCHK(c, o.h_disp.alloc(j->g.nn * 3));
+  HIPCHK(c, o.h_dens.alloc(j->g.ne));
+  HIPCHK(c, o.h_mises.alloc(j->g.ne));
+  if (!o.done) HIPCHK(c, hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
+  launch_grid_to_nodes(j->g, j->nc, o.ufull.p, o.nodes.p, S(j));
+  launch_mises(j->g, mises_mat(j), j->d_E.p, o.ufull.p, o.mises.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(o.h_disp.p, o.nodes.p, j->g.nn * 3 * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipMemcpyAsync(o.h_dens.p, j->d_design.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipMemcpyAsync(o.h_mises.p, o.mises.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipEventRecord(o.done, S(j)));
+  o.pending = true;
+  return ICX_OK;
+}
+
+// Legacy VTK, binary (big-endian), with the fields of the reference's output step
+// (src/io/export_vtk.hpp:47-156; opt_macro_oc.cpp:214-238): points, cells in the
+// reference's node order, cell types (9 quad4, 12 hexa8), displacement (node.val),
+// density and mises per element.
+namespace {
+inline uint32_t be_float(double v) {
+  const float x = (float)v;
+  uint32_t w;
+  std::memcpy(&w, &x, 4);
+  return __builtin_bswap32(w);
+}
+inline uint32_t be_int(int v) { return __builtin_bswap32((uint32_t)v); }
+struct BeWriter {
+  FILE *f;
+  std::vector<uint32_t> buf;
+  void text(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vfprintf(f, fmt, ap);
+    va_end(ap);
+  }
+  bool block(const uint32_t *w, size_t n) { return fwrite(w, 4, n, f) == n && fputc('\n', f) != EOF; }
+  bool floats(const double *v, size_t n) {
+    buf.resize(n);
+    for (size_t i = 0; i < n; ++i) buf[i] = be_float(v[i]);
+    return block(buf.data(), n);
+  }
+};
+}  // namespace
+
+icx_status icx_output_write(icx_job *j, const char *path) {
+  if (!j || !path) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  OutputWork &o = j->ow;
+  if (!o.pending) return fail(c, ICX_ERR_STATE, "icx_output_write without icx_output_begin");
+  HIPCHK(c, hipEventSynchronize(o.done));
+  o.pending = false;
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(c, ICX_ERR_ARG, "cannot open %s for writing", path);
+  BeWriter w{f, {}};
+  const long long nn = j->g.nn, ne = j->g.ne;
+  const int nq = j->el.nq;
+  if (o.geo.empty()) {  // points, cells, cell types: built once per job
+    o.geo.reserve(nn * 3 + ne * (nq + 2));
+    for (long long n = 0; n < nn; ++n) {
+      double x[3];
+      node_coord(j, n, x);
+      for (int d = 0; d < 3; ++d) o.geo.push_back(be_float(x[d]));
+    }
+    o.geo_points = o.geo.size();
+    for (long long e = 0; e < ne; ++e) {
+      o.geo.push_back(be_int(nq));
+      for (int k = 0; k < nq; ++k) o.geo.push_back(be_int((int)elem_node(j, e, k)));
+    }
+    o.geo_cells = o.geo.size() - o.geo_points;
+    for (long long e = 0; e < ne; ++e) o.geo.push_back(be_int(j->g.ndim == 3 ? 12 : 9));
+  }
+  const uint32_t *geo = o.geo.data();
+  bool ok = true;
+  w.text("# vtk DataFile Version 4.1\nvtk output\nBINARY\nDATASET UNSTRUCTURED_GRID\n");
+  w.text("POINTS %lld float\n", nn);
+  ok = ok && w.block(geo, o.geo_points);
+  w.text("CELLS %lld %lld\n", ne, ne * (nq + 1));
+  ok = ok && w.block(geo + o.geo_points, o.geo_cells);
+  w.text("CELL_TYPES %lld\n", ne);
+  ok = ok && w.block(geo + o.geo_points + o.geo_cells, ne);
+  w.text("POINT_DATA %lld\nVECTORS displacement float\n", nn);
+  ok = ok && w.floats(o.h_disp.p, nn * 3);
+  w.text("CELL_DATA %lld\nSCALARS density float\nLOOKUP_TABLE default\n", ne);
+  ok = ok && w.floats(o.h_dens.p, ne);
+  w.text("SCALARS mises float\nLOOKUP_TABLE default\n");
+  ok = ok && w.floats(o.h_mises.p, ne);
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) return fail(c, ICX_ERR_ARG, "write to %s failed", path);
+  return ICX_OK;
+}
+
+icx_status icx_write_vtk(icx_job *j, const char *path) {
+  icx_status rc = icx_output_begin(j);
+  if (rc) return rc;
+  return icx_output_write(j, path);
+}
+
 icx_status icx_compliance(icx_job *j, double *cval) {
   if (!j || !cval) return ICX_ERR_ARG;
   if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
   launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
   return read_scalars(j, 1, cval);
+}
+
+// displacement the sensitivity reads: node.val of the driver's sensitivity()
+// (opt_macro_oc.cpp:193-203 gathers node[..].val, which holds prescribed values on
+// constrained DOFs): the solution grid itself, or solution + Dirichlet grid
+static icx_status output_displacement(icx_job *j);
+static icx_status sens_displacement(icx_job *j, const double **u) {
+  *u = j->d_u.p;
+  if (!j->any_dirich_value) return ICX_OK;
+  icx_status rc = output_displacement(j);
+  *u = j->ow.ufull.p;
+  return rc;
 }
 
 icx_status icx_sensitivity(icx_job *j, double vscale, double *dfds, double *dgds) {
@@ -1402,7 +1604,10 @@ icx_status icx_sensitivity(icx_job *j, double vscale, double *dfds, double *dgds
   DBuf<double> a, b;
   HIPCHK(c, a.alloc(j->g.ne));
   HIPCHK(c, b.alloc(j->g.ne));
-  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_u.p,
+  const double *u;
+  icx_status rc = sens_displacement(j, &u);
+  if (rc) return rc;
+  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, u,
                      j->volume, vscale, a.p, b.p, S(j));
   if (dfds) HIPCHK(c, hipMemcpyAsync(dfds, a.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   if (dgds) HIPCHK(c, hipMemcpyAsync(dgds, b.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
@@ -1821,7 +2026,12 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
     r.constraint = P.driver == 0 ? sc[1] : sc[1] / o.V0 - 1.0;
   }
   // sensitivity and chain rule
-  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_u.p,
+  const double *usens;
+  {
+    icx_status rs = sens_displacement(j, &usens);
+    if (rs) return rs;
+  }
+  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, usens,
                      j->volume, P.driver == 0 ? 1.0 : o.V0, o.dfds.p, o.dgds.p, S(j));
   // dfds *= dthr, dgds *= dthr (projection derivative; 1 without projection)
   if (P.projection) {

@@ -273,6 +273,24 @@ void launch_sensitivity(const Grid &g, const ElasticOp<double> &op, double y1, d
 void launch_energy(const Grid &g, const ElasticOp<double> &op, const double *E, const double *u,
                    double *energy, hipStream_t s);
 
+// output step ---------------------------------------------------------------------
+// node.val as a grid vector: u (free DOFs, zero where constrained) + the Dirichlet
+// grid (prescribed values, zero on free DOFs; nullable), |v| < 1e-15 flushed
+void launch_full_displacement(long long n, const double *u, const double *dirich, double *out,
+                              hipStream_t s);
+// material of the element-mean stress: state (ICX_PLANE_STRESS / _STRAIN / SOLID3D),
+// Poisson ratio, element edge lengths
+struct MisesMat {
+  int state;
+  double nu;
+  double h[3];
+};
+// element-mean von Mises stress from the full displacement grid and E per element
+void launch_mises(const Grid &g, const MisesMat &m, const double *E, const double *u, double *out,
+                  hipStream_t s);
+// node-major copy (3 per node, components >= nc zero) of a grid vector, flushed
+void launch_grid_to_nodes(const Grid &g, int nc, const double *grid, double *out, hipStream_t s);
+
 // cone filter on the structured grid: stencil of offsets with weights
 struct Stencil {
   int n;

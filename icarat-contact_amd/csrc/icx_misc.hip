@@ -248,6 +248,105 @@ void launch_energy(const Grid &g, const ElasticOp<double> &op, const double *E, 
 }
 
 // ---------------------------------------------------------------------------------
+// output step: node.val, element-mean von Mises stress, reactions
+// ---------------------------------------------------------------------------------
+__global__ void k_full_disp(long long n, const double *u, const double *dirich, double *out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = flush15(u[i] + (dirich ? dirich[i] : 0.0));
+}
+void launch_full_displacement(long long n, const double *u, const double *dirich, double *out,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_full_disp, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, u, dirich, out);
+  ICX_CHECK_LAUNCH();
+}
+
+// Elastic::makeMisesStress (src/problem/elasticity.cpp:165-208): the strain and stress
+// averaged over the Gauss points, mises = sqrt(s^T V s).  On an axis-aligned box the
+// Gauss-point average of B is B at the element centre (the rule is symmetric and B is
+// linear in each coordinate it depends on), where dN_q/dx = xi_q / (n_e/2 h_x): the
+// strains are face differences, e.g. e_xx = (sum of u_x on the +x face - on the -x
+// face) / (n_e/2 h_x).  De as in src/problem/elasticity.cpp:34-98 with E_e.
+__global__ void k_mises(Grid g, MisesMat m, const double *E, const double *u, double *out) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int ex = (int)(e % g.nx), ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
+    const double Ee = E[e], nu = m.nu;
+    double q;
+    if (g.ndim == 3) {
+      double dx[3] = {0, 0, 0}, dy[3] = {0, 0, 0}, dz[3] = {0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int a = k & 1, b = (k >> 1) & 1, c3 = k >> 2;
+        const long long n = (ex + a) + (long long)g.sx * ((ey + b) + (long long)g.sy * (ez + c3));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double v = u[c * g.nn + n];
+          dx[c] += a ? v : -v;
+          dy[c] += b ? v : -v;
+          dz[c] += c3 ? v : -v;
+        }
+      }
+      const double fx = 0.25 / m.h[0], fy = 0.25 / m.h[1], fz = 0.25 / m.h[2];
+      const double exx = dx[0] * fx, eyy = dy[1] * fy, ezz = dz[2] * fz;
+      const double gxy = dy[0] * fy + dx[1] * fx, gyz = dz[1] * fz + dy[2] * fy, gzx = dz[0] * fz + dx[2] * fx;
+      const double lambda = Ee * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = Ee / (2 * (1.0 + nu));
+      const double sxx = (lambda + 2 * mu) * exx + lambda * (eyy + ezz);
+      const double syy = (lambda + 2 * mu) * eyy + lambda * (exx + ezz);
+      const double szz = (lambda + 2 * mu) * ezz + lambda * (exx + eyy);
+      const double sxy = mu * gxy, syz = mu * gyz, szx = mu * gzx;
+      q = sxx * sxx + syy * syy + szz * szz - sxx * syy - syy * szz - sxx * szz +
+          3.0 * (sxy * sxy + syz * syz + szx * szx);
+    } else {
+      double dx[2] = {0, 0}, dy[2] = {0, 0};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int a = k & 1, b = k >> 1;
+        const long long n = (ex + a) + (long long)g.sx * (ey + b);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const double v = u[c * g.nn + n];
+          dx[c] += a ? v : -v;
+          dy[c] += b ? v : -v;
+        }
+      }
+      const double fx = 0.5 / m.h[0], fy = 0.5 / m.h[1];
+      const double exx = dx[0] * fx, eyy = dy[1] * fy, gxy = dy[0] * fy + dx[1] * fx;
+      double sxx, syy, sxy;
+      if (m.state == 0) {  // plane stress
+        const double f = Ee / (1.0 - nu * nu);
+        sxx = f * (exx + nu * eyy);
+        syy = f * (nu * exx + eyy);
+        sxy = f * (0.5 * (1.0 - nu)) * gxy;
+      } else {             // plane strain
+        const double f = Ee * ((1.0 - nu) / ((1.0 - 2.0 * nu) * (1.0 + nu))), a = nu / (1.0 - nu);
+        sxx = f * (exx + a * eyy);
+        syy = f * (a * exx + eyy);
+        sxy = f * ((1.0 - 2.0 * nu) / (2.0 * (1.0 - nu))) * gxy;
+      }
+      q = sxx * sxx - sxx * syy + syy * syy + 3.0 * sxy * sxy;
+    }
+    out[e] = sqrt(q);
+  }
+}
+void launch_mises(const Grid &g, const MisesMat &m, const double *E, const double *u, double *out,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_mises, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, m, E, u, out);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_grid_to_nodes(long long nn, int nc, const double *grid, double *out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
+       i += (long long)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[3 * i + c] = c < nc ? flush15(grid[c * nn + i]) : 0.0;
+}
+void launch_grid_to_nodes(const Grid &g, int nc, const double *grid, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_grid_to_nodes, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g.nn, nc, grid, out);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------------
 // cone filter (src/optimization/filter.hpp:102-151); stencil offsets are sorted in
 // increasing neighbour id, the order of the reference's neighbour lists.
 // ---------------------------------------------------------------------------------

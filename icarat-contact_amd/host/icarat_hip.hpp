@@ -239,6 +239,15 @@ inline void sensitivity(StructuredModel &model, Optimize &optim, double vscale =
   check(model.ctx(), icx_sensitivity(model.job(), vscale, optim.dfds.data(), optim.dgds.data()), "icx_sensitivity");
 }
 
+// the drivers' output() (opt-macro-oc/opt_macro_oc.cpp:214-238,
+// opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:207-235): res<step>.vtk with the
+// displacement, density and von Mises stress, written binary directly (the reference
+// writes ASCII and converts it with meshio-binary)
+inline void output(StructuredModel &model, const Optimize &optim) {
+  const std::string path = "res" + std::to_string(optim.optstep) + ".vtk";
+  check(model.ctx(), icx_write_vtk(model.job(), path.c_str()), "icx_write_vtk");
+}
+
 // ---------------------------------------------------------------------------------
 inline std::vector<double> threshold_host(StructuredModel &m, const std::vector<double> &v, double beta, double T,
                                           bool derivative) {

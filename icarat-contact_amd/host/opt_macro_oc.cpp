@@ -4,6 +4,7 @@
 //
 // usage: opt_macro_oc <job.toml> [solver=Eigen_CG] [max_steps=1000] [radius=3.0]
 // The OC volume function (filter + threshold of the candidate) runs on the device.
+// Writes function.csv and res<step>.vtk (binary) in the working directory.
 #include <chrono>
 #include <fstream>
 
@@ -63,6 +64,7 @@ int main(int argc, char *argv[]) {
       for (int i = 0; i < fem.nelx; i++) optim.const_h[0] += design_threshold[i] * model.volume()[i];
     }
     csv << optim.optstep << "," << optim.object_f << "," << optim.const_h[0] << "\n";
+    output(model, optim);
     convergence = optim.isConvergence(1.0e-7, 50);
     sensitivity(model, optim);
     for (int i = 0; i < fem.nelx; i++) {

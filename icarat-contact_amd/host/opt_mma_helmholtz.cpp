@@ -2,7 +2,8 @@
 // sensitivity + MMA, step for step as opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:12-171.
 //
 // usage: opt_mma_helmholtz <job.toml> [solver=Eigen_LDLT] [max_steps=1000]
-// Writes function.csv ("step,objective,constraint") in the working directory.
+// Writes function.csv ("step,objective,constraint") and res<step>.vtk (binary) in the
+// working directory.
 #include <chrono>
 #include <fstream>
 
@@ -63,6 +64,7 @@ int main(int argc, char *argv[]) {
     optim.object_f = obj;
     std::cout << "function value = " << optim.object_f << std::endl;
     csv << optim.optstep << "," << optim.object_f << "," << optim.const_h[0] << "\n";
+    output(model, optim);
     convergence = optim.isConvergence(1.0e-7, 50);
     sensitivity(model, optim, V0);
     auto thrediff = filter.thresholdDerivative(fil_s, beta, 0.5);

@@ -26,6 +26,7 @@ EXPORTS = [
     "icx_set_fext", "icx_get_fext", "icx_add_nodal_load",
     "icx_set_design", "icx_matvec", "icx_diagonal", "icx_solve", "icx_get_displacement",
     "icx_get_node_values", "icx_compliance", "icx_sensitivity", "icx_element_volume",
+    "icx_mises", "icx_reactions", "icx_output_begin", "icx_output_write", "icx_write_vtk",
     "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
     "icx_project", "icx_oc_update", "icx_mma_reset", "icx_mma_update",
     "icx_opt_init", "icx_opt_step", "icx_opt_step_begin", "icx_opt_step_end", "icx_opt_get_design",
@@ -288,6 +289,29 @@ class Job:
         v = np.zeros(self.nelx)
         self.ctx.check(lib().icx_element_volume(self.h, _dp(v)))
         return v
+
+    # -- output step ---------------------------------------------------------------
+    def mises(self):
+        """element-mean von Mises stress of the last solve (element order)"""
+        out = np.zeros(self.nelx)
+        self.ctx.check(lib().icx_mises(self.h, _dp(out)))
+        return out
+
+    def reactions(self):
+        """K u over all DOFs, numnp x 3 (flattened), |r| < 1e-15 flushed"""
+        out = np.zeros(self.numnp * 3)
+        self.ctx.check(lib().icx_reactions(self.h, _dp(out)))
+        return out
+
+    def output_begin(self):
+        self.ctx.check(lib().icx_output_begin(self.h))
+
+    def output_write(self, path):
+        self.ctx.check(lib().icx_output_write(self.h, os.fsencode(path)))
+
+    def write_vtk(self, path):
+        """binary legacy VTK: points, cells, displacement, density, mises"""
+        self.ctx.check(lib().icx_write_vtk(self.h, os.fsencode(path)))
 
     # -- filters -----------------------------------------------------------------
     def cone_setup(self, radius):

@@ -130,6 +130,30 @@ icx_status icx_get_displacement(const icx_job *job, double *u);
 icx_status icx_get_node_values(const icx_job *job, double *nodeval);
 /* compliance u . fext (opt-macro-oc/opt_macro_oc.cpp:110) */
 icx_status icx_compliance(icx_job *job, double *c);
+
+/* ---- output step ---------------------------------------------------------------
+ * Element-mean von Mises stress of the last solve and the current design
+ * (Elastic::makeMisesStress, src/problem/elasticity.cpp:165-208: strain and stress
+ * averaged over the Gauss points, sqrt(s^T V s); in 2D the reference's V without s_zz,
+ * also for plane strain).  out: nelx doubles in element-id order. */
+icx_status icx_mises(icx_job *job, double *out);
+/* Reactions: the internal force K u over all DOFs after a solve (the re-assembly in
+ * LinearAnalysis::solve, src/analysis/linear.hpp:90-91, 117-166, assemblingReact
+ * src/misc/assembling.hpp:77-84) scattered to nodes like divideReactionIntoNode
+ * (src/analysis/analysis_utility.hpp:33-51): out numnp x 3, |r| < 1e-15 flushed;
+ * equals fext on free DOFs at convergence. */
+icx_status icx_reactions(icx_job *job, double *out);
+/* The drivers' output() (opt-macro-oc/opt_macro_oc.cpp:214-238,
+ * opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:207-235, src/io/export_vtk.hpp:47-156)
+ * as a binary legacy VTK file, in place of the ASCII file and the meshio-binary
+ * conversion: points, cells (reference node order), cell types, displacement (node.val),
+ * density (the current design) and mises.  icx_output_begin queues the device work and
+ * the copies to pinned host memory on the job's stream and returns at once (later
+ * device work on the stream, e.g. the next design step, does not disturb them);
+ * icx_output_write waits for them and writes `path`.  icx_write_vtk does both. */
+icx_status icx_output_begin(icx_job *job);
+icx_status icx_output_write(icx_job *job, const char *path);
+icx_status icx_write_vtk(icx_job *job, const char *path);
 /* the drivers' sensitivity() (opt-macro-oc/opt_macro_oc.cpp:182-212,
  * opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:173-205):
  * dfds = -(dE/E) u_e^T Ke u_e, dgds = v_e / vscale (vscale = 1 or V0).

@@ -843,6 +843,82 @@ void icxo_sys_element_energy(const icxo_sys *s, const double *nodeval, double *o
   }
 }
 
+/* makeMisesStress: src/problem/elasticity.cpp:165-208.  Element-mean strain and
+ * stress over the Gauss points (fac = 1/ipmax), then mises = sqrt(s^T V s) with
+ * V = [1 -1/2 0; -1/2 1 0; 0 0 3] (2D: xx, yy, xy; plane strain keeps the reference's
+ * 2D form without s_zz) or its 6x6 counterpart (3D: xx, yy, zz, xy, yz, zx).  De is
+ * that of the last assembly (young_ from SIMP in makeKeFint); nodeval is node.val
+ * (3 per node, after the |v| < 1e-15 flush). */
+void icxo_sys_mises(const icxo_sys *s, const double *nodeval, double *mises)
+{
+  const int ne = s->ne, ndim = s->ndim, nd = s->nd;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < s->nelx; e++) {
+    double X[MAXNE * 3], N[MAXNE], dNdX[3 * MAXNE], B[6 * MAXDOF], D[36], d[MAXDOF];
+    double stress[6] = {0, 0, 0, 0, 0, 0};
+    int voigt;
+    element_X(s, e, X);
+    make_De(s->state, s->young[e], s->nu, D, &voigt);
+    for (int k = 0; k < ne; k++)
+      for (int c = 0; c < ndim; c++) d[k * ndim + c] = nodeval[3 * s->conn[ne * e + k] + c];
+    const double fac = 1.0 / ne; /* ipmax = ne for quad4 / hexa8 */
+    for (int ip = 0; ip < ne; ip++) {
+      kinematics(ne, ndim, ip, X, N, dNdX);
+      make_B(ne, ndim, dNdX, B);
+      double strain[6];
+      for (int v = 0; v < voigt; v++) {
+        double t = 0.0;
+        for (int j = 0; j < nd; j++) t += B[v * nd + j] * d[j];
+        strain[v] = t;
+      }
+      for (int v = 0; v < voigt; v++) {
+        double t = 0.0;
+        for (int w = 0; w < voigt; w++) t += D[v * voigt + w] * strain[w];
+        stress[v] += t * fac;
+      }
+    }
+    double q;
+    if (voigt == 3) {
+      q = stress[0] * stress[0] - stress[0] * stress[1] + stress[1] * stress[1] + 3.0 * stress[2] * stress[2];
+    } else {
+      q = stress[0] * stress[0] + stress[1] * stress[1] + stress[2] * stress[2] - stress[0] * stress[1] -
+          stress[1] * stress[2] - stress[0] * stress[2] +
+          3.0 * (stress[3] * stress[3] + stress[4] * stress[4] + stress[5] * stress[5]);
+    }
+    mises[e] = sqrt(q);
+  }
+}
+
+/* Reactions: LinearAnalysis::solve re-assembles after the solve
+ * (src/analysis/linear.hpp:90-91, 117-166): fe = Ke u_e of every element summed
+ * over all DOFs (assemblingReact, src/misc/assembling.hpp:77-84), then
+ * divideReactionIntoNode (src/analysis/analysis_utility.hpp:33-51) with the
+ * |r| < 1e-15 flush.  out: 3 per node (components beyond dofnp are 0). */
+void icxo_sys_reactions(const icxo_sys *s, const double *nodeval, double *out)
+{
+  const int ne = s->ne, nd = s->nd, dofnp = s->dofnp;
+  memset(out, 0, (size_t)s->numnp * 3 * sizeof(double));
+  /* element colours: elements of one colour share no node */
+  for (int col = 0; col < s->ncol; col++) {
+#pragma omp parallel for schedule(static)
+    for (int q = s->cptr[col]; q < s->cptr[col + 1]; q++) {
+      const int e = s->cord[q];
+      double X[MAXNE * 3], Ke[MAXDOF * MAXDOF], vol, d[MAXDOF];
+      element_X(s, e, X);
+      icxo_element_stiffness(ne, s->ndim, s->state, s->young[e], s->nu, X, Ke, &vol);
+      for (int k = 0; k < ne; k++)
+        for (int c = 0; c < dofnp; c++) d[k * dofnp + c] = nodeval[3 * s->conn[ne * e + k] + c];
+      for (int i = 0; i < nd; i++) {
+        double t = 0.0;
+        for (int j = 0; j < nd; j++) t += Ke[i * nd + j] * d[j];
+        out[3 * s->conn[ne * e + i / dofnp] + i % dofnp] += t;
+      }
+    }
+  }
+  for (long long i = 0; i < (long long)s->numnp * 3; i++)
+    if (fabs(out[i]) < 1.0e-15) out[i] = 0.0;
+}
+
 /* Sensitivity: opt_macro_oc.cpp:182-212 (driver A, dgds = v) and
  * opt_mma_Helmholtz.cpp:173-205 (driver B, dgds = v/V0):
  * dfds = -(dE/E) * u^T Ke u.  `energy` is u^T Ke u with Ke = E K0. */

@@ -52,6 +52,10 @@ int icxo_sys_cg(const icxo_sys *s, const double *b, double *x, double tol, int m
                 double *err);
 int icxo_sys_direct(const icxo_sys *s, const double *b, double *x);
 void icxo_sys_element_energy(const icxo_sys *s, const double *nodeval, double *ue_K_ue);
+/* element-mean von Mises stress (Gauss-point average, elasticity.cpp:165-208) */
+void icxo_sys_mises(const icxo_sys *s, const double *nodeval, double *mises);
+/* reactions Ke u_e summed over all DOFs, flushed, 3 per node (linear.hpp:90-91) */
+void icxo_sys_reactions(const icxo_sys *s, const double *nodeval, double *out);
 
 /* ---- Eigen CG restatement on a CSR matrix -------------------------------- */
 int icxo_csr_cg(int n, const int *rowptr, const int *col, const double *val,

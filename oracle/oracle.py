@@ -52,6 +52,8 @@ def lib():
             "icxo_sys_cg": (i, [vp, _dp, _dp, d, i, C.POINTER(d)]),
             "icxo_sys_direct": (i, [vp, _dp, _dp]),
             "icxo_sys_element_energy": (None, [vp, _dp, _dp]),
+            "icxo_sys_mises": (None, [vp, _dp, _dp]),
+            "icxo_sys_reactions": (None, [vp, _dp, _dp]),
             "icxo_csr_cg": (i, [i, _ip, _ip, _dp, _dp, _dp, d, i, C.POINTER(d)]),
             "icxo_divide_val_into_node": (None, [i, i, i, _ip, _dp, vp, _dp]),
             "icxo_sensitivity": (None, [i, i, d, d, d, _dp, _dp, _dp, d, _dp, _dp]),
@@ -207,6 +209,18 @@ class Problem:
     def element_energy(self, nodeval):
         out = np.zeros(self.nelx)
         lib().icxo_sys_element_energy(self.system(), _f64(nodeval), out)
+        return out
+
+    def mises(self, nodeval):
+        """element-mean von Mises stress of the last assembly's design"""
+        out = np.zeros(self.nelx)
+        lib().icxo_sys_mises(self.system(), _f64(nodeval), out)
+        return out
+
+    def reactions(self, nodeval):
+        """K_full u over all DOFs, 3 per node, |r| < 1e-15 flushed"""
+        out = np.zeros(3 * self.numnp)
+        lib().icxo_sys_reactions(self.system(), _f64(nodeval), out)
         return out
 
     def volumes(self):

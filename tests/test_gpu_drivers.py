@@ -56,6 +56,10 @@ def test_cxx_driver_b_executable_matches_oracle():
                              text=True, timeout=300)
         assert out.returncode == 0, out.stderr[-2000:]
         rows = np.loadtxt(os.path.join(d, "function.csv"), delimiter=",", ndmin=2)
+        # the output step: one binary VTK file per step (20 x 10 quad4: 231 points)
+        for k in range(1, steps + 1):
+            head = open(os.path.join(d, "res%d.vtk" % k), "rb").read(120).split(b"\n")
+            assert head[2] == b"BINARY" and head[4] == b"POINTS 231 float"
     pr = O.Problem(4, (20, 10, 1), (20.0, 10.0, 1.0), O.PLANE_STRAIN, young=(0.001, 200.0))
     pr.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, 10, 0, 10])
     pr.make_dof()
