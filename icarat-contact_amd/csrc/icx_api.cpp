@@ -504,6 +504,11 @@ struct icx_job {
   HelmWork hw;
   ConeWork cw;
   OutputWork ow;
+  // heat conduction (prob.state == ICX_HEAT): element matrices of materials 1 and 2,
+  // and K1 - K2 for the sensitivity; d_E holds the weights design^p
+  bool heat() const { return prob.state == ICX_HEAT; }
+  HeatOp heat_op{}, heat_dK{};
+  bool heat_ready = false;
   MMAWork mma;
   OptWork opt;
 
@@ -559,7 +564,8 @@ icx_status build_rhs(icx_job *j) {
   icx_ctx *c = j->ctx;
   HIPCHK(c, hipMemcpyAsync(j->d_b.p, j->d_fext.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
   if (j->any_dirich_value) {
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr, S(j));
+    if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr, S(j));
+    else launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr, S(j));
     launch_axpby(N, -1.0, j->d_vec.p, 1.0, j->d_b.p, S(j));
     launch_mask_zero(j->g, j->nc, j->d_mask.p, j->d_b.p, S(j));
   }
@@ -730,9 +736,10 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   for (int d = 0; d < ndim; ++d)
     if (p->div[d] < 1 || !(p->length[d] > 0.0))
       return fail(ctx, ICX_ERR_ARG, "division and length must be positive");
-  if ((ndim == 3) != (p->state == ICX_SOLID3D))
+  const bool heat = p->state == ICX_HEAT;
+  if (!heat && (ndim == 3) != (p->state == ICX_SOLID3D))
     return fail(ctx, ICX_ERR_ARG, "state must be ICX_SOLID3D exactly for hexa8");
-  if (!(p->poisson > -1.0 && p->poisson < 0.5)) return fail(ctx, ICX_ERR_ARG, "poisson out of range");
+  if (!heat && !(p->poisson > -1.0 && p->poisson < 0.5)) return fail(ctx, ICX_ERR_ARG, "poisson out of range");
   auto *j = new icx_job();
   j->ctx = ctx;
   j->prob = *p;
@@ -745,15 +752,20 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   g.tile_depth = ctx->tile_depth;
   // node vectors are addressed with 32-bit byte offsets (3 components x 8 bytes)
   if (g.nn > (1LL << 27)) { delete j; return fail(ctx, ICX_ERR_ARG, "mesh too large (more than 2^27 nodes)"); }
-  j->nc = ndim;
+  j->nc = heat ? 1 : ndim;
   j->el.ndim = ndim;
   j->el.nq = ndim == 3 ? 8 : 4;
   for (int d = 0; d < 3; ++d) j->el.h[d] = d < ndim ? p->length[d] / p->div[d] : 1.0;
-  std::vector<double> K;
-  j->volume = elastic_K0(j->el, p->state, p->poisson, K);
-  std::string why;
-  if (!build_elastic_op(j->el, K, j->op64, why)) { delete j; return fail(ctx, ICX_ERR_NUMERIC, "%s", why.c_str()); }
-  j->op32 = cast_op<float>(j->op64);
+  if (heat) {
+    j->volume = 1.0;
+    for (int d = 0; d < ndim; ++d) j->volume *= j->el.h[d];
+  } else {
+    std::vector<double> K;
+    j->volume = elastic_K0(j->el, p->state, p->poisson, K);
+    std::string why;
+    if (!build_elastic_op(j->el, K, j->op64, why)) { delete j; return fail(ctx, ICX_ERR_NUMERIC, "%s", why.c_str()); }
+    j->op32 = cast_op<float>(j->op64);
+  }
   j->has.assign(g.nn, 0);
   j->node_flag.assign(g.nn * 6, 0);
   j->node_val.assign(g.nn * 6, 0.0);
@@ -1008,10 +1020,58 @@ icx_status icx_add_nodal_load(icx_job *j, int node, int comp, double value) {
 icx_status icx_set_design(icx_job *j, const double *design) {
   if (!j || !design) return ICX_ERR_ARG;
   icx_ctx *c = j->ctx;
+  if (j->heat() && !j->heat_ready) return fail(c, ICX_ERR_STATE, "icx_set_conductivity first");
   HIPCHK(c, hipMemcpyAsync(j->d_design.p, design, j->g.ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
-  launch_simp(j->g.ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
+  if (j->heat()) launch_heat_weight(j->g.ne, j->prob.penalty, j->d_design.p, j->d_E.p, S(j));
+  else launch_simp(j->g.ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
   HIPCHK(c, hipStreamSynchronize(S(j)));
   j->design_set = true;
+  j->u_valid = false;
+  return ICX_OK;
+}
+
+// conductivity tensor of a [material] row (xx, yy, zz, xy, yz, xz): Heat::setParameter
+static void heat_tensor(int ndim, const double *c6, double *D) {
+  if (ndim == 2) {
+    const double t[4] = {c6[0], c6[3], c6[3], c6[1]};
+    std::memcpy(D, t, sizeof(t));
+  } else {
+    const double t[9] = {c6[0], c6[3], c6[5], c6[3], c6[1], c6[4], c6[5], c6[4], c6[2]};
+    std::memcpy(D, t, sizeof(t));
+  }
+}
+// sum_ip dN^T D dN jac in the corner order of BoxElement (src/problem/heat.cpp:95-106)
+static void heat_element(const BoxElement &el, const double *D, double *K) {
+  const int nq = el.nq, nd = el.ndim;
+  for (int i = 0; i < nq * nq; ++i) K[i] = 0.0;
+  for (int ip = 0; ip < nq; ++ip) {
+    double N[8], dN[3][8], jac;
+    el.eval(ip, N, dN, jac);
+    for (int a = 0; a < nq; ++a)
+      for (int b = 0; b < nq; ++b) {
+        double t = 0.0;
+        for (int x = 0; x < nd; ++x)
+          for (int y = 0; y < nd; ++y) t += dN[x][a] * D[x * nd + y] * dN[y][b];
+        K[a * nq + b] += t * jac;
+      }
+  }
+}
+
+icx_status icx_set_conductivity(icx_job *j, const double *k1, const double *k2) {
+  if (!j || !k1 || !k2) return ICX_ERR_ARG;
+  if (!j->heat()) return fail(j->ctx, ICX_ERR_STATE, "icx_set_conductivity: not a heat job (ICX_HEAT)");
+  if (k1[0] < k2[0])
+    return fail(j->ctx, ICX_ERR_ARG, "the conductivity of material 1 must be at least that of material 2");
+  double D1[9], D2[9], Dd[9];
+  const int nd = j->g.ndim;
+  heat_tensor(nd, k1, D1);
+  heat_tensor(nd, k2, D2);
+  for (int i = 0; i < nd * nd; ++i) Dd[i] = D1[i] - D2[i];
+  j->heat_op.nq = j->heat_dK.nq = j->el.nq;
+  heat_element(j->el, D1, j->heat_op.K1);
+  heat_element(j->el, D2, j->heat_op.K2);
+  heat_element(j->el, Dd, j->heat_dK.K1);
+  j->heat_ready = true;
   j->u_valid = false;
   return ICX_OK;
 }
@@ -1025,7 +1085,10 @@ icx_status icx_matvec(icx_job *j, int prec, const double *x, double *y) {
   HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
   HIPCHK(c, hipMemcpyAsync(tmp.p, x, j->numeq * sizeof(double), hipMemcpyHostToDevice, S(j)));
   launch_dof_to_grid(j->g, j->nc, j->d_dof.p, j->numeq, tmp.p, nullptr, j->d_vec.p, S(j));
-  if (prec == 0) {
+  if (j->heat()) {
+    if (prec != 0) return fail(c, ICX_ERR_ARG, "heat jobs: FP64 operator only");
+    launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
+  } else if (prec == 0) {
     launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
   } else {
     DBuf<float> xf, yf, Ef;
@@ -1045,7 +1108,8 @@ icx_status icx_diagonal(icx_job *j, double *diag) {
   if (!j || !diag) return ICX_ERR_ARG;
   if (j->numeq < 0 || !j->design_set) return ICX_ERR_STATE;
   icx_ctx *c = j->ctx;
-  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, j->d_vec.p, nullptr, S(j));
+  if (j->heat()) launch_heat_invdiag(j->g, j->heat_op, j->d_E.p, j->d_mask.p, j->d_vec.p, S(j));
+  else launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, j->d_vec.p, nullptr, S(j));
   DBuf<double> tmp;
   HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
   launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_vec.p, tmp.p, S(j));
@@ -1062,10 +1126,12 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
   PcgWork64 &w = j->pw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.p.alloc(N)); HIPCHK(c, w.t.alloc(N));
   HIPCHK(c, w.z.alloc(N)); HIPCHK(c, w.invd.alloc(N));
-  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
+  if (j->heat()) launch_heat_invdiag(j->g, j->heat_op, j->d_E.p, j->d_mask.p, w.invd.p, S(j));
+  else launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
   auto mv = [&](const double *p, double *t) {
     ProfScope ps(c, ICX_PROF_MATVEC64);
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
+    if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j));
+    else launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
   };
   return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info, j->gr_plain);
 }
@@ -1360,6 +1426,7 @@ icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx
   if (j->numeq < 0 || !j->design_set)
     return fail(j->ctx, ICX_ERR_STATE, "icx_solve needs icx_make_dof and icx_set_design");
   if (!(tol > 0.0) || maxit < 1) return fail(j->ctx, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
+  if (j->heat() && form != ICX_SOLVE_PLAIN) return fail(j->ctx, ICX_ERR_ARG, "heat jobs: ICX_SOLVE_PLAIN only");
   icx_ctx *c = j->ctx;
   (void)hipSetDevice(c->device);
   icx_solve_info inf{};
@@ -1444,6 +1511,7 @@ static MisesMat mises_mat(const icx_job *j) {
 
 icx_status icx_mises(icx_job *j, double *out) {
   if (!j || !out) return ICX_ERR_ARG;
+  if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "icx_mises: elastic jobs only");
   if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
   icx_ctx *c = j->ctx;
   icx_status rc = output_displacement(j);
@@ -1465,7 +1533,8 @@ icx_status icx_reactions(icx_job *j, double *out) {
   HIPCHK(c, ku.alloc(j->nvec()));
   HIPCHK(c, j->ow.nodes.alloc(j->g.nn * 3));
   // K u over all DOFs: the operator without the constraint mask
-  launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->ow.ufull.p, ku.p, nullptr, S(j));
+  if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, nullptr, j->ow.ufull.p, ku.p, nullptr, S(j));
+  else launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->ow.ufull.p, ku.p, nullptr, S(j));
   launch_grid_to_nodes(j->g, j->nc, ku.p, j->ow.nodes.p, S(j));
   HIPCHK(c, hipMemcpyAsync(out, j->ow.nodes.p, j->g.nn * 3 * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipStreamSynchronize(S(j)));
@@ -1486,10 +1555,11 @@ icx_status icx_output_begin(icx_job *j) {
   HIPCHK(c, o.h_mises.alloc(j->g.ne));
   if (!o.done) HIPCHK(c, hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
   launch_grid_to_nodes(j->g, j->nc, o.ufull.p, o.nodes.p, S(j));
-  launch_mises(j->g, mises_mat(j), j->d_E.p, o.ufull.p, o.mises.p, S(j));
+  if (!j->heat()) launch_mises(j->g, mises_mat(j), j->d_E.p, o.ufull.p, o.mises.p, S(j));
   HIPCHK(c, hipMemcpyAsync(o.h_disp.p, o.nodes.p, j->g.nn * 3 * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipMemcpyAsync(o.h_dens.p, j->d_design.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
-  HIPCHK(c, hipMemcpyAsync(o.h_mises.p, o.mises.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  if (!j->heat())
+    HIPCHK(c, hipMemcpyAsync(o.h_mises.p, o.mises.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipEventRecord(o.done, S(j)));
   o.pending = true;
   return ICX_OK;
@@ -1561,12 +1631,28 @@ icx_status icx_output_write(icx_job *j, const char *path) {
   ok = ok && w.block(geo + o.geo_points, o.geo_cells);
   w.text("CELL_TYPES %lld\n", ne);
   ok = ok && w.block(geo + o.geo_points + o.geo_cells, ne);
-  w.text("POINT_DATA %lld\nVECTORS displacement float\n", nn);
-  ok = ok && w.floats(o.h_disp.p, nn * 3);
-  w.text("CELL_DATA %lld\nSCALARS density float\nLOOKUP_TABLE default\n", ne);
-  ok = ok && w.floats(o.h_dens.p, ne);
-  w.text("SCALARS mises float\nLOOKUP_TABLE default\n");
-  ok = ok && w.floats(o.h_mises.p, ne);
+  if (j->heat()) {
+    // opt-heat/opt_heat.cpp:196-236: temperature, threshold_s (1 where the sum over a
+    // node's elements of design / ne exceeds 0.5), design_s
+    std::vector<double> temp(nn), nds(nn, 0.0), thr(nn);
+    for (long long n = 0; n < nn; ++n) temp[n] = o.h_disp.p[3 * n];
+    for (long long e = 0; e < ne; ++e)
+      for (int k = 0; k < nq; ++k) nds[elem_node(j, e, k)] += o.h_dens.p[e] / nq;
+    for (long long n = 0; n < nn; ++n) thr[n] = nds[n] > 0.5 ? 1.0 : 0.0;
+    w.text("POINT_DATA %lld\nSCALARS temperature float\nLOOKUP_TABLE default\n", nn);
+    ok = ok && w.floats(temp.data(), nn);
+    w.text("SCALARS threshold_s float\nLOOKUP_TABLE default\n");
+    ok = ok && w.floats(thr.data(), nn);
+    w.text("CELL_DATA %lld\nSCALARS design_s float\nLOOKUP_TABLE default\n", ne);
+    ok = ok && w.floats(o.h_dens.p, ne);
+  } else {
+    w.text("POINT_DATA %lld\nVECTORS displacement float\n", nn);
+    ok = ok && w.floats(o.h_disp.p, nn * 3);
+    w.text("CELL_DATA %lld\nSCALARS density float\nLOOKUP_TABLE default\n", ne);
+    ok = ok && w.floats(o.h_dens.p, ne);
+    w.text("SCALARS mises float\nLOOKUP_TABLE default\n");
+    ok = ok && w.floats(o.h_mises.p, ne);
+  }
   ok = (std::fclose(f) == 0) && ok;
   if (!ok) return fail(c, ICX_ERR_ARG, "write to %s failed", path);
   return ICX_OK;
@@ -1607,8 +1693,11 @@ icx_status icx_sensitivity(icx_job *j, double vscale, double *dfds, double *dgds
   const double *u;
   icx_status rc = sens_displacement(j, &u);
   if (rc) return rc;
-  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, u,
-                     j->volume, vscale, a.p, b.p, S(j));
+  if (j->heat())
+    launch_heat_sensitivity(j->g, j->heat_dK, j->prob.penalty, j->d_design.p, u, j->volume, vscale, a.p, b.p, S(j));
+  else
+    launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, u,
+                       j->volume, vscale, a.p, b.p, S(j));
   if (dfds) HIPCHK(c, hipMemcpyAsync(dfds, a.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   if (dgds) HIPCHK(c, hipMemcpyAsync(dgds, b.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipStreamSynchronize(S(j)));
@@ -1944,6 +2033,7 @@ icx_status icx_mma_update(icx_job *j, double *x, const double *dfdx, double gx, 
 icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0) {
   if (!j || !par || !x0) return ICX_ERR_ARG;
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
+  if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
   icx_ctx *c = j->ctx;
   OptWork &o = j->opt;
   o.par = *par;
@@ -2100,6 +2190,7 @@ icx_status icx_submit(icx_job *j, double tol, int maxit) {
   if (j->numeq < 0 || !j->design_set)
     return fail(c, ICX_ERR_STATE, "icx_submit needs icx_make_dof and icx_set_design");
   if (!(tol > 0.0) || maxit < 1) return fail(c, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
+  if (j->heat()) return fail(c, ICX_ERR_ARG, "batched solves: elastic jobs only");
   if (j->sub.active) return fail(c, ICX_ERR_STATE, "job already has a submitted solve");
   (void)hipSetDevice(c->device);
   if (!c->batch) c->batch = new BatchRun();

@@ -291,6 +291,25 @@ void launch_mises(const Grid &g, const MisesMat &m, const double *E, const doubl
 // node-major copy (3 per node, components >= nc zero) of a grid vector, flushed
 void launch_grid_to_nodes(const Grid &g, int nc, const double *grid, double *out, hipStream_t s);
 
+// heat conduction (src/problem/heat.hpp, heat.cpp): one DOF per node, element matrix
+// K_e = w_e K1 + (1 - w_e) K2 with w_e = design_e^p (Heat::makeSIMPDe); K1, K2 are the
+// conductivity matrices of materials 1 and 2, nq x nq row-major in the corner order
+// q = a + 2b + 4c
+struct HeatOp {
+  int nq;
+  double K1[64], K2[64];
+};
+void launch_heat_weight(long long ne, double p, const double *design, double *w, hipStream_t s);
+// y = K x (mask: bit 0 = constrained, output forced to zero; may be null); st may be null
+void launch_heat_matvec(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, const double *x,
+                        double *y, const SolverState *st, hipStream_t s);
+// inverse diagonal (1 where the diagonal is 0, 0 on constrained DOFs)
+void launch_heat_invdiag(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, double *invd,
+                         hipStream_t s);
+// dfds = -p design^(p-1) T_e^T (K1 - K2) T_e (dK holds K1 - K2 in K1), dgds = vol / vscale
+void launch_heat_sensitivity(const Grid &g, const HeatOp &dK, double p, const double *design, const double *u,
+                             double vol, double vscale, double *dfds, double *dgds, hipStream_t s);
+
 // cone filter on the structured grid: stencil of offsets with weights
 struct Stencil {
   int n;

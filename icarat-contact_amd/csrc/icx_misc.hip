@@ -347,6 +347,102 @@ void launch_grid_to_nodes(const Grid &g, int nc, const double *grid, double *out
 }
 
 // ---------------------------------------------------------------------------------
+// heat conduction (src/problem/heat.cpp:62-109): node-parallel gather over the 2^d
+// elements of a node, dense element rows (the anisotropic conductivity tensor has no
+// mirror symmetry to exploit), FP64
+// ---------------------------------------------------------------------------------
+__global__ void k_heat_weight(long long ne, double p, const double *design, double *w) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < ne;
+       e += (long long)gridDim.x * blockDim.x)
+    w[e] = pow(design[e], p);
+}
+void launch_heat_weight(long long ne, double p, const double *design, double *w, hipStream_t s) {
+  hipLaunchKernelGGL(k_heat_weight, dim3(ew_blocks(ne)), dim3(256), 0, s, ne, p, design, w);
+  ICX_CHECK_LAUNCH();
+}
+
+// corners of element (ex, ey, ez) in the order q = a + 2b + 4c
+__device__ __forceinline__ long long corner(const Grid &g, int ex, int ey, int ez, int q) {
+  return (ex + (q & 1)) + (long long)g.sx * ((ey + ((q >> 1) & 1)) + (long long)g.sy * (ez + (q >> 2)));
+}
+
+// y_n = sum over the elements e of node n (local index qn) of row qn of K_e times x_e;
+// DIAG: the diagonal entry instead
+template <bool DIAG>
+__global__ void k_heat_op(Grid g, HeatOp op, const double *w, const uint8_t *mask, const double *x, double *y,
+                          const SolverState *st) {
+  if (st && st->done) return;
+  const int nq = op.nq;
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)((n / g.sx) % g.sy), k = (int)(n / ((long long)g.sx * g.sy));
+    double acc = 0.0;
+    const int nz = g.ndim == 3 ? 2 : 1;
+    for (int dz = 0; dz < nz; ++dz)
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+          const int ex = i - 1 + dx, ey = j - 1 + dy, ez = g.ndim == 3 ? k - 1 + dz : 0;
+          if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+          const long long e = ex + (long long)g.nx * (ey + (long long)g.ny * ez);
+          const int qn = (1 - dx) + 2 * (1 - dy) + (g.ndim == 3 ? 4 * (1 - dz) : 0);
+          const double we = w[e];
+          double a1 = 0.0, a2 = 0.0;
+          if (DIAG) {
+            a1 = op.K1[qn * nq + qn];
+            a2 = op.K2[qn * nq + qn];
+          } else {
+            for (int q = 0; q < nq; ++q) {
+              const double xq = x[corner(g, ex, ey, ez, q)];
+              a1 += op.K1[qn * nq + q] * xq;
+              a2 += op.K2[qn * nq + q] * xq;
+            }
+          }
+          acc += we * a1 + (1.0 - we) * a2;
+        }
+    const bool fixed = mask && (mask[n] & 1);
+    if (DIAG) y[n] = fixed ? 0.0 : (acc != 0.0 ? 1.0 / acc : 1.0);
+    else y[n] = fixed ? 0.0 : acc;
+  }
+}
+void launch_heat_matvec(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, const double *x,
+                        double *y, const SolverState *st, hipStream_t s) {
+  hipLaunchKernelGGL(k_heat_op<false>, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, op, w, mask, x, y, st);
+  ICX_CHECK_LAUNCH();
+}
+void launch_heat_invdiag(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, double *invd,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_heat_op<true>, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, op, w, mask, nullptr, invd,
+                     nullptr);
+  ICX_CHECK_LAUNCH();
+}
+
+// opt-heat/opt_heat.cpp:158-194 (the projection factor is applied by the caller)
+__global__ void k_heat_sens(Grid g, HeatOp dK, double p, const double *design, const double *u, double vol,
+                            double vscale, double *dfds, double *dgds) {
+  const int nq = dK.nq;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int ex = (int)(e % g.nx), ey = (int)((e / g.nx) % g.ny), ez = (int)(e / ((long long)g.nx * g.ny));
+    double t[8];
+    for (int q = 0; q < nq; ++q) t[q] = flush15(u[corner(g, ex, ey, ez, q)]);
+    double en = 0.0;
+    for (int a = 0; a < nq; ++a) {
+      double r = 0.0;
+      for (int b = 0; b < nq; ++b) r += dK.K1[a * nq + b] * t[b];
+      en += t[a] * r;
+    }
+    dfds[e] = -(p * pow(design[e], p - 1.0)) * en;
+    if (dgds) dgds[e] = vol / vscale;
+  }
+}
+void launch_heat_sensitivity(const Grid &g, const HeatOp &dK, double p, const double *design, const double *u,
+                             double vol, double vscale, double *dfds, double *dgds, hipStream_t s) {
+  hipLaunchKernelGGL(k_heat_sens, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, dK, p, design, u, vol, vscale, dfds,
+                     dgds);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------------
 // cone filter (src/optimization/filter.hpp:102-151); stencil offsets are sorted in
 // increasing neighbour id, the order of the reference's neighbour lists.
 // ---------------------------------------------------------------------------------

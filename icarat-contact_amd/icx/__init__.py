@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "_build", "libicx.so"))
 
 QUAD4, HEXA8 = 4, 8
-PLANE_STRESS, PLANE_STRAIN, SOLID3D = 0, 1, 2
+PLANE_STRESS, PLANE_STRAIN, SOLID3D, HEAT = 0, 1, 2, 3
 SOLVE_PLAIN, SOLVE_FUSED, SOLVE_MIXED = 0, 1, 2
 FILTER_NONE, FILTER_CONE, FILTER_HELMHOLTZ = 0, 1, 2
 ERR_NOCONV = 5
@@ -24,7 +24,7 @@ EXPORTS = [
     "icx_job_create", "icx_job_destroy", "icx_job_sizes", "icx_get_mesh",
     "icx_add_dirichlet", "icx_make_dof", "icx_get_dof_map", "icx_add_neumann",
     "icx_set_fext", "icx_get_fext", "icx_add_nodal_load",
-    "icx_set_design", "icx_matvec", "icx_diagonal", "icx_solve", "icx_get_displacement",
+    "icx_set_design", "icx_set_conductivity", "icx_matvec", "icx_diagonal", "icx_solve", "icx_get_displacement",
     "icx_get_node_values", "icx_compliance", "icx_sensitivity", "icx_element_volume",
     "icx_mises", "icx_reactions", "icx_output_begin", "icx_output_write", "icx_write_vtk",
     "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
@@ -181,7 +181,7 @@ class Job:
         lib().icx_job_sizes(self.h, C.byref(a), C.byref(b), C.byref(c))
         self.numnp, self.nelx, self.neq = a.value, b.value, c.value
         self.ndim = 2 if eltype == QUAD4 else 3
-        self.dofnp = self.ndim
+        self.dofnp = 1 if state == HEAT else self.ndim
         self.div = d
         self.numeq = None
 
@@ -243,6 +243,10 @@ class Job:
     def set_design(self, x):
         x = _f64(x)
         self.ctx.check(lib().icx_set_design(self.h, _dp(x)))
+
+    def set_conductivity(self, k1, k2):
+        """heat jobs: conductivity rows (xx, yy, zz, xy, yz, xz) of materials 1 and 2"""
+        self.ctx.check(lib().icx_set_conductivity(self.h, _dp(_f64(k1)), _dp(_f64(k2))))
 
     def matvec(self, x, prec=0):
         x = _f64(x)

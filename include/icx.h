@@ -39,7 +39,7 @@ enum {
 /* element types (src/io/mesh_base.hpp:50-115) */
 enum { ICX_QUAD4 = 4, ICX_HEXA8 = 8 };
 /* material state (src/problem/elasticity.cpp:19-32) */
-enum { ICX_PLANE_STRESS = 0, ICX_PLANE_STRAIN = 1, ICX_SOLID3D = 2 };
+enum { ICX_PLANE_STRESS = 0, ICX_PLANE_STRAIN = 1, ICX_SOLID3D = 2, ICX_HEAT = 3 };
 /* solver forms */
 enum {
   ICX_SOLVE_PLAIN = 0, /* FP64 Jacobi-PCG, Eigen ConjugateGradient loop (parity form) */
@@ -58,7 +58,9 @@ typedef struct {
   int eltype;        /* ICX_QUAD4 or ICX_HEXA8 */
   int div[3];        /* [mesh] division (div[2] ignored for quad4) */
   double length[3];  /* [mesh] length */
-  int state;         /* ICX_PLANE_STRESS / ICX_PLANE_STRAIN / ICX_SOLID3D */
+  int state;         /* ICX_PLANE_STRESS / ICX_PLANE_STRAIN / ICX_SOLID3D, or ICX_HEAT:
+                        heat conduction (LinearHeat, src/problem/heat.hpp), one DOF per
+                        node, quad4 or hexa8; young and poisson unused, penalty = p */
   double young1;     /* [material] young[0] (E1; Emin when young2 > young1) */
   double young2;     /* [material] young[1] (E2; E0) */
   double poisson;    /* [material] poisson */
@@ -110,8 +112,20 @@ icx_status icx_add_nodal_load(icx_job *job, int node, int comp, double value);
 
 /* ---- analysis ------------------------------------------------------------------
  * LinearElastic::setDesignS for every element (src/problem/elasticity.hpp:60):
- * the SIMP modulus of element e is computed from design[e] on the device. */
+ * the SIMP modulus of element e is computed from design[e] on the device.  Heat jobs:
+ * LinearHeat::setDesign (src/problem/heat.hpp:34), conductivity
+ * design^p De1 + (1 - design^p) De2 (Heat::makeSIMPDe, heat.cpp:62-77). */
 icx_status icx_set_design(icx_job *job, const double *design);
+/* Heat jobs: the [material] conductivity rows of materials 1 and 2, six entries each
+ * (xx, yy, zz, xy, yz, xz), as read by Heat::setParameter (src/problem/heat.cpp:19-60;
+ * 2D uses xx, yy, xy).  Material 1 must conduct at least as well as material 2 in xx
+ * (heat.cpp:66-70).  Call before icx_set_design.  (The reference's 3D branch assigns
+ * material 2 to De1_ and leaves De2_ unset; both tensors are taken as given here.)
+ * Heat jobs solve with ICX_SOLVE_PLAIN only; icx_sensitivity gives
+ * -p s^(p-1) T_e^T (K1 - K2) T_e (opt-heat/opt_heat.cpp:158-194 before the projection
+ * factor); icx_write_vtk writes temperature, threshold_s and design_s
+ * (opt_heat.cpp:196-236). */
+icx_status icx_set_conductivity(icx_job *job, const double *k1, const double *k2);
 /* y = K x on the free DOFs (matrix-free; x, y numeq doubles).
  * prec: 0 = FP64, 1 = FP32 operator (inputs rounded to FP32). */
 icx_status icx_matvec(icx_job *job, int prec, const double *x, double *y);

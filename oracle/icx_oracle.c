@@ -286,6 +286,43 @@ void icxo_element_stiffness(int ne, int ndim, int state, double E, double nu,
   *volume = vol;
 }
 
+/* Heat: conductivity tensor from the 6 entries (xx, yy, zz, xy, yz, xz) of a
+ * [material] conductivity row, Heat::setParameter (src/problem/heat.cpp:19-60).  The
+ * reference's 3D branch assigns material 2 to De1_ and leaves De2_ unset; the intended
+ * tensor (the 2D pattern extended) is used here. */
+void icxo_heat_tensor(int ndim, const double *c6, double *D)
+{
+  if (ndim == 2) {
+    D[0] = c6[0]; D[1] = c6[3];
+    D[2] = c6[3]; D[3] = c6[1];
+    return;
+  }
+  D[0] = c6[0]; D[1] = c6[3]; D[2] = c6[5];
+  D[3] = c6[3]; D[4] = c6[1]; D[5] = c6[4];
+  D[6] = c6[5]; D[7] = c6[4]; D[8] = c6[2];
+}
+
+/* LinearHeat::makeKeFint (src/problem/heat.cpp:79-109): Ke = sum_ip B^T De B jac with
+ * B = dN/dX (ndim x ne), volume = sum_ip jac */
+void icxo_heat_element(int ne, int ndim, const double *D, const double *X, double *Ke, double *volume)
+{
+  double N[MAXNE], dNdX[3 * MAXNE];
+  memset(Ke, 0, (size_t)ne * ne * sizeof(double));
+  double vol = 0.0;
+  for (int ip = 0; ip < ne; ip++) {
+    double jac = kinematics(ne, ndim, ip, X, N, dNdX);
+    for (int i = 0; i < ne; i++)
+      for (int j = 0; j < ne; j++) {
+        double t = 0.0;
+        for (int a = 0; a < ndim; a++)
+          for (int b = 0; b < ndim; b++) t += dNdX[a * ne + i] * D[a * ndim + b] * dNdX[b * ne + j];
+        Ke[i * ne + j] += t * jac;
+      }
+    vol += jac;
+  }
+  *volume = vol;
+}
+
 /* Helmholtz element: Ke = sum (N N^T + r_h^2 dN^T dN) jac, r_h = R/(2 sqrt 3);
  * Nint = sum N jac.  src/optimization/filter_helm.hpp:178-222 */
 void icxo_helm_element(int ne, int ndim, double radius, const double *X, double *Ke,
@@ -517,6 +554,7 @@ struct icxo_sys {
   int ne, ndim, dofnp, state, numnp, nelx, numeq, nd;
   const double *x; const int *conn; const int *dof;
   double young1, young2, nu, p;
+  double D1[9], D2[9];   /* heat: conductivity tensors of materials 1 and 2 */
   int *rowptr, *col; double *val; double *invdiag;
   double *volume; double *design; double *young;
   int ncol; int *cptr, *cord;
@@ -526,6 +564,26 @@ static void element_X(const icxo_sys *s, int e, double *X)
 {
   for (int k = 0; k < s->ne; k++)
     for (int d = 0; d < s->ndim; d++) X[k * s->ndim + d] = s->x[3 * s->conn[s->ne * e + k] + d];
+}
+
+/* Ke of element e for the design of the last assembly: elastic E_e K0, or heat with
+ * De = design^p De1 + (1 - design^p) De2 (Heat::makeSIMPDe, src/problem/heat.cpp:62-77) */
+static void element_K(const icxo_sys *s, int e, const double *X, double *Ke, double *vol)
+{
+  if (s->state == ICXO_HEAT) {
+    const double w = pow(s->design[e], s->p);
+    double D[9];
+    for (int i = 0; i < s->ndim * s->ndim; i++) D[i] = w * s->D1[i] + (1.0 - w) * s->D2[i];
+    icxo_heat_element(s->ne, s->ndim, D, X, Ke, vol);
+    return;
+  }
+  icxo_element_stiffness(s->ne, s->ndim, s->state, s->young[e], s->nu, X, Ke, vol);
+}
+
+void icxo_sys_set_conductivity(icxo_sys *s, const double *k1, const double *k2)
+{
+  icxo_heat_tensor(s->ndim, k1, s->D1);
+  icxo_heat_tensor(s->ndim, k2, s->D2);
 }
 
 icxo_sys *icxo_sys_create(int ne, int ndim, int dofnp, int state, int numnp, int nelx,
@@ -576,7 +634,14 @@ icxo_sys *icxo_sys_create(int ne, int ndim, int dofnp, int state, int numnp, int
   for (int e = 0; e < nelx; e++) {
     double X[MAXNE * 3], Ke[MAXDOF * MAXDOF];
     element_X(s, e, X);
-    icxo_element_stiffness(ne, ndim, state, 1.0, nu, X, Ke, &s->volume[e]);
+    if (state == ICXO_HEAT) {
+      const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+      double D[9];
+      for (int i = 0; i < ndim * ndim; i++) D[i] = I[(i / ndim) * 3 + i % ndim];
+      icxo_heat_element(ne, ndim, D, X, Ke, &s->volume[e]);
+    } else {
+      icxo_element_stiffness(ne, ndim, state, 1.0, nu, X, Ke, &s->volume[e]);
+    }
   }
   return s;
 }
@@ -608,8 +673,8 @@ void icxo_sys_assemble(icxo_sys *s, const double *design, const int *has,
     int idof[MAXDOF];
     element_X(s, e, X);
     s->design[e] = design[e];
-    s->young[e] = icxo_simp(s->young1, s->young2, design[e], s->p);
-    icxo_element_stiffness(ne, s->ndim, s->state, s->young[e], s->nu, X, Ke, &vol);
+    s->young[e] = s->state == ICXO_HEAT ? pow(design[e], s->p) : icxo_simp(s->young1, s->young2, design[e], s->p);
+    element_K(s, e, X, Ke, &vol);
     int anyd = 0;
     for (int k = 0; k < ne; k++)
       for (int c = 0; c < dofnp; c++) {
@@ -830,7 +895,7 @@ void icxo_sys_element_energy(const icxo_sys *s, const double *nodeval, double *o
   for (int e = 0; e < s->nelx; e++) {
     double X[MAXNE * 3], Ke[MAXDOF * MAXDOF], vol, d[MAXDOF];
     element_X(s, e, X);
-    icxo_element_stiffness(ne, s->ndim, s->state, s->young[e], s->nu, X, Ke, &vol);
+    element_K(s, e, X, Ke, &vol);
     for (int k = 0; k < ne; k++)
       for (int c = 0; c < dofnp; c++) d[k * dofnp + c] = nodeval[3 * s->conn[ne * e + k] + c];
     double acc = 0.0;
@@ -905,7 +970,7 @@ void icxo_sys_reactions(const icxo_sys *s, const double *nodeval, double *out)
       const int e = s->cord[q];
       double X[MAXNE * 3], Ke[MAXDOF * MAXDOF], vol, d[MAXDOF];
       element_X(s, e, X);
-      icxo_element_stiffness(ne, s->ndim, s->state, s->young[e], s->nu, X, Ke, &vol);
+      element_K(s, e, X, Ke, &vol);
       for (int k = 0; k < ne; k++)
         for (int c = 0; c < dofnp; c++) d[k * dofnp + c] = nodeval[3 * s->conn[ne * e + k] + c];
       for (int i = 0; i < nd; i++) {
@@ -917,6 +982,30 @@ void icxo_sys_reactions(const icxo_sys *s, const double *nodeval, double *out)
   }
   for (long long i = 0; i < (long long)s->numnp * 3; i++)
     if (fabs(out[i]) < 1.0e-15) out[i] = 0.0;
+}
+
+/* Heat sensitivity: opt-heat/opt_heat.cpp:158-194 without the projection factor:
+ * dfds = -T_e^T (p s^(p-1) sum_ip B^T (De1 - De2) B jac) T_e, s = the element design of
+ * the last assembly; nodeval is node.val (3 per node, component 0 used). */
+void icxo_sys_heat_sensitivity(const icxo_sys *s, const double *nodeval, double *dfds)
+{
+  const int ne = s->ne;
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < s->nelx; e++) {
+    double X[MAXNE * 3], Ke[MAXNE * MAXNE], vol, t[MAXNE], D[9];
+    element_X(s, e, X);
+    const double f = s->p * pow(s->design[e], s->p - 1.0);
+    for (int i = 0; i < s->ndim * s->ndim; i++) D[i] = f * (s->D1[i] - s->D2[i]);
+    icxo_heat_element(ne, s->ndim, D, X, Ke, &vol);
+    for (int k = 0; k < ne; k++) t[k] = nodeval[3 * s->conn[ne * e + k]];
+    double acc = 0.0;
+    for (int i = 0; i < ne; i++) {
+      double r = 0.0;
+      for (int j = 0; j < ne; j++) r += Ke[i * ne + j] * t[j];
+      acc += t[i] * r;
+    }
+    dfds[e] = -acc;
+  }
 }
 
 /* Sensitivity: opt_macro_oc.cpp:182-212 (driver A, dgds = v) and

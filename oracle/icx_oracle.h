@@ -17,8 +17,9 @@
 extern "C" {
 #endif
 
-/* material state, elasticity.cpp:19-32 ("plane_stress", "plane_strain", "3D") */
-enum { ICXO_PLANE_STRESS = 0, ICXO_PLANE_STRAIN = 1, ICXO_SOLID3D = 2 };
+/* material state, elasticity.cpp:19-32 ("plane_stress", "plane_strain", "3D"), and
+ * heat conduction (src/problem/heat.hpp, one DOF per node) */
+enum { ICXO_PLANE_STRESS = 0, ICXO_PLANE_STRAIN = 1, ICXO_SOLID3D = 2, ICXO_HEAT = 3 };
 
 /* ---- mesh / boundary conditions (integer, bit-exact) ---------------------- */
 int icxo_mesh_box(int ne, const int *div, const double *len, double *x, int *conn);
@@ -36,6 +37,9 @@ void icxo_element_stiffness(int ne, int ndim, int state, double E, double nu,
                             const double *X, double *Ke, double *volume);
 void icxo_helm_element(int ne, int ndim, double radius, const double *X,
                        double *Ke, double *Nint);
+/* heat: conductivity tensor (ndim x ndim) of a 6-entry row; element conductivity matrix */
+void icxo_heat_tensor(int ndim, const double *c6, double *D);
+void icxo_heat_element(int ne, int ndim, const double *D, const double *X, double *Ke, double *volume);
 
 /* ---- global structural system -------------------------------------------- */
 typedef struct icxo_sys icxo_sys;
@@ -56,6 +60,10 @@ void icxo_sys_element_energy(const icxo_sys *s, const double *nodeval, double *u
 void icxo_sys_mises(const icxo_sys *s, const double *nodeval, double *mises);
 /* reactions Ke u_e summed over all DOFs, flushed, 3 per node (linear.hpp:90-91) */
 void icxo_sys_reactions(const icxo_sys *s, const double *nodeval, double *out);
+/* heat (state ICXO_HEAT): conductivities of materials 1 and 2 (6 entries each), and
+ * the sensitivity -T_e^T dKe/ds T_e (opt-heat/opt_heat.cpp:158-194) */
+void icxo_sys_set_conductivity(icxo_sys *s, const double *k1, const double *k2);
+void icxo_sys_heat_sensitivity(const icxo_sys *s, const double *nodeval, double *dfds);
 
 /* ---- Eigen CG restatement on a CSR matrix -------------------------------- */
 int icxo_csr_cg(int n, const int *rowptr, const int *col, const double *val,

@@ -16,7 +16,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libicx_oracle.so")
 
-PLANE_STRESS, PLANE_STRAIN, SOLID3D = 0, 1, 2
+PLANE_STRESS, PLANE_STRAIN, SOLID3D, HEAT = 0, 1, 2, 3
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -53,6 +53,9 @@ def lib():
             "icxo_sys_direct": (i, [vp, _dp, _dp]),
             "icxo_sys_element_energy": (None, [vp, _dp, _dp]),
             "icxo_sys_mises": (None, [vp, _dp, _dp]),
+            "icxo_sys_set_conductivity": (None, [vp, _dp, _dp]),
+            "icxo_sys_heat_sensitivity": (None, [vp, _dp, _dp]),
+            "icxo_heat_element": (None, [C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
             "icxo_sys_reactions": (None, [vp, _dp, _dp]),
             "icxo_csr_cg": (i, [i, _ip, _ip, _dp, _dp, _dp, d, i, C.POINTER(d)]),
             "icxo_divide_val_into_node": (None, [i, i, i, _ip, _dp, vp, _dp]),
@@ -94,7 +97,7 @@ class Problem:
     def __init__(self, ne, div, length, state, young=(1e-9, 1.0), nu=0.3, penalty=3.0):
         self.ne = int(ne)
         self.ndim = 2 if ne == 4 else 3
-        self.dofnp = self.ndim
+        self.dofnp = 1 if state == HEAT else self.ndim
         self.div = np.array(list(div) + [1] * (3 - len(div)), dtype=np.int32)[:3]
         if self.ndim == 2:
             self.div[2] = 1
@@ -211,6 +214,16 @@ class Problem:
         lib().icxo_sys_element_energy(self.system(), _f64(nodeval), out)
         return out
 
+    def set_conductivity(self, k1, k2):
+        """heat: conductivity rows (xx, yy, zz, xy, yz, xz) of materials 1 and 2"""
+        lib().icxo_sys_set_conductivity(self.system(), _f64(k1), _f64(k2))
+
+    def heat_sensitivity(self, nodeval):
+        """-T_e^T dKe/ds T_e for the design of the last assembly"""
+        out = np.zeros(self.nelx)
+        lib().icxo_sys_heat_sensitivity(self.system(), _f64(nodeval), out)
+        return out
+
     def mises(self, nodeval):
         """element-mean von Mises stress of the last assembly's design"""
         out = np.zeros(self.nelx)
@@ -226,12 +239,16 @@ class Problem:
     def volumes(self):
         X = self.x.reshape(-1, 3)[:, :self.ndim]
         vol = np.zeros(self.nelx)
-        Ke = np.zeros((self.ne * self.dofnp) ** 2)
+        Ke = np.zeros(24 * 24)  # the largest element matrix (hexa8, 3 DOFs per node)
         v = np.zeros(1)
+        eye = np.eye(self.ndim).ravel()
         # all elements of a box mesh share one shape; volumes via quadrature per element
         for e in range(self.nelx):
             Xe = _f64(X[self.conn[e * self.ne:(e + 1) * self.ne]])
-            lib().icxo_element_stiffness(self.ne, self.ndim, self.state, 1.0, self.nu, Xe, Ke, v)
+            if self.state == HEAT:
+                lib().icxo_heat_element(self.ne, self.ndim, eye, Xe, Ke, v)
+            else:
+                lib().icxo_element_stiffness(self.ne, self.ndim, self.state, 1.0, self.nu, Xe, Ke, v)
             vol[e] = v[0]
         return vol
 
@@ -243,6 +260,9 @@ class Problem:
         return dfds, dgds
 
     def element_stiffness(self, E=1.0, e=0):
+        """elastic element stiffness E K0 of element e (elastic problems only)"""
+        if self.state == HEAT:
+            raise ValueError("element_stiffness: elastic problems only (heat: icxo_heat_element)")
         X = _f64(self.x.reshape(-1, 3)[self.conn[e * self.ne:(e + 1) * self.ne], :self.ndim])
         n = self.ne * self.dofnp
         Ke = np.zeros(n * n)
