@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--serve-budget", type=float, default=240e9)
     ap.add_argument("--pmc-iters", type=int, default=200)
     ap.add_argument("--output", action="store_true", help="time the output step (mises, reactions, VTK)")
+    ap.add_argument("--heat", action="store_true", help="heat-conduction solve (FP64 plain form)")
+    ap.add_argument("--heat-div", type=int, default=100)
     return ap.parse_args()
 
 
@@ -225,6 +227,46 @@ def output_step(args):
     print(json.dumps(line))
 
 
+def heat_solve(args):
+    """Heat conduction on a hexa8 box (the 3D example of opt-heat: unit cube, all faces
+    at T = 0, uniform heat source, conductivities 100 / 0.1, design 0.25, p = 3), one
+    FP64 plain-form solve to 1e-8 on the random design of the other workloads; the
+    operator is timed per launch with launch-bound events."""
+    import numpy as np
+
+    import icx
+    n = args.heat_div
+    ctx = icx.Context(0)
+    job = icx.Job(ctx, icx.HEXA8, (n, n, n), (1.0, 1.0, 1.0), icx.HEAT)
+    for face in ([0, 1, 0, 1, 0, 0], [0, 1, 0, 1, 1, 1], [0, 0, 0, 1, 0, 1], [1, 1, 0, 1, 0, 1],
+                 [0, 1, 0, 0, 0, 1], [0, 1, 1, 1, 0, 1]):
+        job.add_dirichlet([1, 0, 0, 0, 0, 0], [0] * 6, face)
+    job.make_dof()
+    job.add_neumann([0.1, 0, 0], [0, 1, 0, 1, 0, 1])
+    job.set_conductivity((100.0, 100.0, 100.0, 0, 0, 0), (0.1, 0.1, 0.1, 0, 0, 0))
+    job.set_design(0.05 + 0.95 * icx.random_density(job.nelx, SEED))
+    job.solve(icx.SOLVE_PLAIN, tol=args.tol, maxit=200000, want_u=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, info = job.solve(icx.SOLVE_PLAIN, tol=args.tol, maxit=200000, want_u=False)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.profile_begin(icx.PROF_MATVEC64)
+    _, pi = job.solve(icx.SOLVE_PLAIN, tol=args.tol, maxit=200000, want_u=False)
+    s, nl = ctx.profile_end()
+    mv_us = s / max(nl, 1) * 1e6
+    N, nn, ne = job.numeq, job.numnp, job.nelx
+    bytes_mv = 2 * nn * 8 + ne * 8 + nn  # x read, y write, weights, mask
+    line = {"metric": "heat conduction: FP64 Jacobi-PCG (Eigen loop) DOF-matvec/s", "value": N * info["matvecs"] / wall,
+            "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "heat hexa8 unit cube %d^3, faces at T = 0, source 0.1, k 100 / 0.1, p = 3"
+                                   % n, "free_dofs": N, "iterations": info["iterations"], "relres": info["relres"],
+                       "ms_per_solve": wall * 1e3, "operator_us": mv_us,
+                       "operator_GBps_algorithmic": bytes_mv / (mv_us * 1e-6) / 1e9,
+                       "operator_bytes_per_launch": bytes_mv}}
+    print(json.dumps(line))
+
+
 def design_loop(args):
     import numpy as np
 
@@ -382,6 +424,9 @@ def main():
         return
     if args.output:
         output_step(args)
+        return
+    if args.heat:
+        heat_solve(args)
         return
     if args.design_loop:
         design_loop(args)

@@ -1130,7 +1130,7 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
   else launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
   auto mv = [&](const double *p, double *t) {
     ProfScope ps(c, ICX_PROF_MATVEC64);
-    if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j));
+    if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
     else launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
   };
   return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info, j->gr_plain);

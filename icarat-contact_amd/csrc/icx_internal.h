@@ -302,7 +302,7 @@ struct HeatOp {
 void launch_heat_weight(long long ne, double p, const double *design, double *w, hipStream_t s);
 // y = K x (mask: bit 0 = constrained, output forced to zero; may be null); st may be null
 void launch_heat_matvec(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, const double *x,
-                        double *y, const SolverState *st, hipStream_t s);
+                        double *y, const SolverState *st, hipStream_t s, const LaunchEvents *ev = nullptr);
 // inverse diagonal (1 where the diagonal is 0, 0 on constrained DOFs)
 void launch_heat_invdiag(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, double *invd,
                          hipStream_t s);

@@ -405,8 +405,8 @@ __global__ void k_heat_op(Grid g, HeatOp op, const double *w, const uint8_t *mas
   }
 }
 void launch_heat_matvec(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, const double *x,
-                        double *y, const SolverState *st, hipStream_t s) {
-  hipLaunchKernelGGL(k_heat_op<false>, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, op, w, mask, x, y, st);
+                        double *y, const SolverState *st, hipStream_t s, const LaunchEvents *ev) {
+  launch_ev(k_heat_op<false>, ew_blocks(g.nn), 256, s, ev, g, op, w, mask, x, y, st);
   ICX_CHECK_LAUNCH();
 }
 void launch_heat_invdiag(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, double *invd,

@@ -102,22 +102,34 @@ class StructuredModel {
       p.length[d] = len[d].as_double();
     }
     const int ndim = config.at("mesh", "dimension").as_int();
-    std::string state = config.at("material", "state").as_string();
-    if (ndim == 3) state = "3D";
-    p.state = state == "plane_stress" ? ICX_PLANE_STRESS : (state == "plane_strain" ? ICX_PLANE_STRAIN : ICX_SOLID3D);
-    const auto &young = config.at("material", "young").as_array();
-    p.young1 = young[0].as_double();
-    p.young2 = young[1].as_double();
-    p.poisson = config.at("material", "poisson").as_double();
+    // [material] type: "Elastic" (Elastic::setParameter, elasticity.cpp:19-32) or "Heat"
+    // (Heat::setParameter, heat.cpp:19-60: two conductivity rows, no young / poisson)
+    heat_ = config.has("material", "type") && config.at("material", "type").as_string() == "Heat";
+    double cond[2][6] = {};
+    if (heat_) {
+      p.state = ICX_HEAT;
+      const auto &rows = config.at("material", "conductivity").as_array();
+      for (int m = 0; m < 2; ++m)
+        for (int k = 0; k < 6; ++k) cond[m][k] = rows[m].as_array()[k].as_double();
+    } else {
+      std::string state = config.at("material", "state").as_string();
+      if (ndim == 3) state = "3D";
+      p.state = state == "plane_stress" ? ICX_PLANE_STRESS : (state == "plane_strain" ? ICX_PLANE_STRAIN : ICX_SOLID3D);
+      const auto &young = config.at("material", "young").as_array();
+      p.young1 = young[0].as_double();
+      p.young2 = young[1].as_double();
+      p.poisson = config.at("material", "poisson").as_double();
+    }
     p.penalty = config.at("material", "penalty").as_double();
     design_s0_ = config.at("material", "designs_0").as_double();
     prob_ = p;
     check(ctx(), icx_job_create(ctx(), &p, &job_), "icx_job_create");
+    if (heat_) check(ctx(), icx_set_conductivity(job_, cond[0], cond[1]), "icx_set_conductivity");
     int numnp, nelx, neq;
     icx_job_sizes(job_, &numnp, &nelx, &neq);
     fem_.ndim = ndim;
-    fem_.dofnp = ndim;
-    fem_.voigt = ndim == 2 ? 3 : 6;
+    fem_.dofnp = heat_ ? 1 : ndim;
+    fem_.voigt = heat_ ? ndim : (ndim == 2 ? 3 : 6);
     fem_.numnp = numnp;
     fem_.nelx = nelx;
     fem_.neq = neq;
@@ -165,6 +177,7 @@ class StructuredModel {
     force.fext_org.assign(fem_.numeq, 0.0);
     check(ctx(), icx_get_fext(job_, force.fext_org.data()), "icx_get_fext");
   }
+  bool heat() const { return heat_; }
   // element[i].setDesignS(d[i]) for all elements
   void setDesignS(const std::vector<double> &d) {
     check(ctx(), icx_set_design(job_, d.data()), "icx_set_design");
@@ -186,6 +199,7 @@ class StructuredModel {
   icx_job *job_ = nullptr;
   icx_problem prob_{};
   double design_s0_ = 0.0;
+  bool heat_ = false;
   std::vector<double> volume_, design_;
 };
 

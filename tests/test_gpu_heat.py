@@ -89,3 +89,57 @@ def test_heat_vtk_fields_and_restrictions(ctx, tmp_path):
             job.solve(form, tol=1e-8)
     with pytest.raises(icx.IcxError):
         job.mises()
+
+
+def _oracle_heat_driver(steps):
+    """opt-heat/opt_heat.cpp:71-150 composed from oracle pieces: Helmholtz filter (CG,
+    1e-10), projection, direct heat solve, T^T K T, sensitivity, MMA."""
+    k1, k2 = (50.0, 40.0, 0.0, 2.0, 0.0, 0.0), (0.05, 0.05, 0.0, 0.0, 0.0, 0.0)
+    pr = O.Problem(QUAD4, (24, 16, 1), (1.5, 1.0, 1.0), O.HEAT, penalty=3.0)
+    pr.add_dirichlet([1, 0, 0, 0, 0, 0], [0] * 6, [0.6, 0.9, 1.0, 1.0, 0, 0])
+    pr.make_dof()
+    pr.add_neumann([0.05, 0, 0], [0.0, 1.5, 0.0, 1.0, 0, 0])
+    pr.set_conductivity(k1, k2)
+    helm = O.Helmholtz(pr, 0.12)
+    mma = O.MMA(pr.nelx, 1)
+    vol = pr.volumes()
+    x = np.full(pr.nelx, 0.4)
+    xmin, xmax = np.full(pr.nelx, 1e-4), np.full(pr.nelx, 1.0)
+    V0, obj, con = 0.0, [], []
+    for step in range(1, steps + 1):
+        fil, _ = helm.apply(x, sens=False, tol=1e-10)
+        beta = min(2.0 ** (step // 40), 16.0)
+        thr = O.threshold(fil, beta, 0.5)
+        b = pr.assemble(thr)
+        T = pr.direct(b)
+        if step == 1:
+            V0 = thr @ vol
+        obj.append(T @ (pr.csr() @ T))
+        con.append(thr @ vol - V0)
+        thd = O.threshold_derivative(fil, beta, 0.5)
+        dfds = pr.heat_sensitivity(pr.node_values(T)) * thd
+        dgds = vol * thd
+        dfds, _ = helm.apply(dfds, sens=True, tol=1e-10)
+        dgds, _ = helm.apply(dgds, sens=True, tol=1e-10)
+        x, _ = mma.update(x, dfds, [con[-1]], dgds, xmin, xmax)
+    return np.array(obj), np.array(con)
+
+
+def test_cxx_heat_driver_matches_oracle():
+    import os
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "icarat-contact_amd", "_build", "opt_heat")
+    toml = os.path.join(root, "icarat-contact_amd", "host", "examples", "heat_q4.toml")
+    steps = 6
+    with tempfile.TemporaryDirectory() as d:
+        out = subprocess.run([exe, toml, str(steps)], cwd=d, capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
+        rows = np.loadtxt(os.path.join(d, "function.csv"), delimiter=",", ndmin=2)
+        head = open(os.path.join(d, "res%d.vtk" % steps), "rb").read(200)
+        assert b"BINARY" in head and b"POINTS 425 float" in head
+    obj, con = _oracle_heat_driver(steps)
+    assert rows.shape[0] == steps
+    assert np.abs(rows[:, 1] - obj).max() <= 1e-7 * np.abs(obj).max(), (rows[:, 1], obj)
+    assert np.abs(rows[:, 2] - con).max() <= 1e-7 * abs(con[0] + 1.0) + 1e-9
