@@ -35,6 +35,7 @@ sys.path.insert(0, os.path.join(ROOT, "icarat-contact_amd"))
 HBM_PEAK = 8.0e12
 SEED = 20261203
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_hex8_fusedA.md")
+TRAFFIC_FILE_SR = os.path.join(ROOT, "profiles", "hbm_traffic_hex8_sr.md")
 
 
 def parse():
@@ -50,6 +51,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch every kernel eagerly")
+    ap.add_argument("--no-pipeline", action="store_true", help="poll the solver state chunk by chunk")
+    ap.add_argument("--single-reduction", action="store_true",
+                    help="mixed form with one kernel per iteration instead of kernels A and B")
     ap.add_argument("--pmc-workload", action="store_true")
     ap.add_argument("--batched", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64)
@@ -61,6 +65,7 @@ def parse():
     ap.add_argument("--serve-iters", type=int, default=10)
     ap.add_argument("--serve-elems", type=float, nargs=2, default=[5e3, 3e5])
     ap.add_argument("--serve-budget", type=float, default=240e9)
+    ap.add_argument("--serve-limit", type=int, default=None, help="serve the first N jobs of the stream")
     ap.add_argument("--pmc-iters", type=int, default=200)
     ap.add_argument("--output", action="store_true", help="time the output step (mises, reactions, VTK)")
     ap.add_argument("--heat", action="store_true", help="heat-conduction solve (FP64 plain form)")
@@ -103,11 +108,11 @@ def cpu_baseline(nx, ny, nz, seconds):
     }
 
 
-def measured_traffic(nx, ny, nz):
-    """HBM bytes per launch of the fused operator kernel from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size)."""
+def measured_traffic(nx, ny, nz, sr=False):
+    """HBM bytes per launch of the iteration kernel from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size)."""
     try:
-        text = open(TRAFFIC_FILE).read()
+        text = open(TRAFFIC_FILE_SR if sr else TRAFFIC_FILE).read()
     except OSError:
         return None
     m = re.search(r"traffic\[%dx%dx%d\]\s*=\s*([0-9.eE+]+)" % (nx, ny, nz), text)
@@ -141,6 +146,8 @@ def batched(args):
 
     import icx
     ctx = icx.Context(0)
+    if args.single_reduction:
+        ctx.set_option(icx.OPT_SINGLE_REDUCTION, 1)
     rng = np.random.default_rng(np.random.PCG64(20261204))
     jobs = [icx.random_job(ctx, rng) for _ in range(args.batch_jobs)]
     for j in jobs:  # warm-up: buffers, captured graphs
@@ -314,7 +321,7 @@ def serving(args):
     ctx = icx.Context(0)
     out = serve.run(ctx, n_jobs=args.serve_jobs, rate=args.serve_rate, iters=args.serve_iters,
                     elems=tuple(args.serve_elems), budget=args.serve_budget, tol=args.tol,
-                    log=lambda m: print(m, file=sys.stderr, flush=True))
+                    log=lambda m: print(m, file=sys.stderr, flush=True), limit=args.serve_limit)
     out.pop("designs", None)
     line = {"metric": "serving: design-iteration latency p50/p99 and DOF-matvec/s",
             "value": out["dof_matvec_per_s"], "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
@@ -323,9 +330,10 @@ def serving(args):
     print(json.dumps(line))
 
 
-def measure(icx, ctx, size, tol, steps, warmup, plain_too):
+def measure(icx, ctx, size, tol, steps, warmup, plain_too, sr=False):
     """Mixed-form solves of one cantilever: graph-replayed steps, then one solve with
-    HIP events around every launch of the fused kernels A and B."""
+    HIP events around every launch of the iteration kernel(s): the single-reduction
+    kernel (sr), else the fused kernels A and B."""
     import numpy as np
     nx, ny, nz = size
     job = icx.cantilever_3d(ctx, nx, ny, nz)
@@ -341,18 +349,23 @@ def measure(icx, ctx, size, tol, steps, warmup, plain_too):
         infos.append(info)
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    # Dominant kernel: HIP events around every launch of the fused FP32 operator
-    # kernel A (the library's stream; launched one by one while the window is open).
-    # Algorithmic bytes per launch: reads r, p_old, invdiag, x_lo (4 words per DOF)
-    # and E (1 word per element); writes p, Ap, x_lo (3 words per DOF).
+    # Dominant kernel: HIP events around every launch of the iteration kernel (the
+    # library's stream; launched one by one while the window is open).
+    # Single-reduction kernel, algorithmic bytes per launch: reads r, w, s, invdiag, p,
+    # x_lo (6 words per DOF) and E (1 word per element); writes r, w, s, p, x_lo (5 words
+    # per DOF).  Kernel A of the two-kernel form: reads r, p_old, invdiag, x_lo, E; writes
+    # p, Ap, x_lo (7 words per DOF + 1 per element).
     prof = {}
     for name, kid in (("A", icx.PROF_FUSED_A), ("B", icx.PROF_FUSED_B)):
+        if sr and name == "B":
+            prof[name] = (0.0, 0, 1)
+            continue
         ctx.profile_begin(kid)
         _, pinfo = job.solve(icx.SOLVE_MIXED, tol=tol, maxit=50000, want_u=False)
         secs, launches = ctx.profile_end()
         prof[name] = (secs, launches, pinfo["iterations"])
     kern_s, launches, executed = prof["A"]
-    bytes_A = (7 * N + Ne) * 4
+    bytes_A = ((11 if sr else 7) * N + Ne) * 4
     res = {
         "N": N, "Ne": Ne, "wall": wall, "infos": infos,
         "ms_step": wall / steps * 1e3,
@@ -365,9 +378,10 @@ def measure(icx, ctx, size, tol, steps, warmup, plain_too):
     }
     inner = infos[-1]["iterations"]
     refin = infos[-1]["refinements"]
-    # whole-solve algorithmic bytes: 9 N w + N_e w_E per inner iteration (w = 4),
+    # whole-solve algorithmic bytes: per inner iteration 11 N w + N_e w_E (single
+    # reduction) or 9 N w + N_e w_E (two kernels, the Ap round trip not counted), w = 4;
     # 6 N 8 + N_e 8 per FP64 residual replacement
-    res["bytes_solve"] = inner * (9 * N * 4 + Ne * 4) + refin * (6 * N * 8 + Ne * 8)
+    res["bytes_solve"] = inner * ((11 if sr else 9) * N * 4 + Ne * 4) + refin * (6 * N * 8 + Ne * 8)
     res["solve_frac"] = res["bytes_solve"] / (res["ms_step"] * 1e-3) / HBM_PEAK
     res["p50"] = float(np.percentile([i["seconds"] * 1e3 for i in infos], 50))
     res["p99"] = float(np.percentile([i["seconds"] * 1e3 for i in infos], 99))
@@ -436,16 +450,21 @@ def main():
     ctx = icx.Context(0)
     if args.no_graphs:
         ctx.set_option(icx.OPT_GRAPHS, 0)
-    r = measure(icx, ctx, args.size, args.tol, args.steps, args.warmup, not args.no_plain)
+    if args.no_pipeline:
+        ctx.set_option(icx.OPT_PIPELINE, 0)
+    sr = args.single_reduction
+    if sr:
+        ctx.set_option(icx.OPT_SINGLE_REDUCTION, 1)
+    r = measure(icx, ctx, args.size, args.tol, args.steps, args.warmup, not args.no_plain, sr)
     print("headline %dx%dx%d: %.2f ms/solve" % (tuple(args.size) + (r["ms_step"],)), file=sys.stderr,
           flush=True)
     large = None
     if not args.no_large:
-        rl = measure(icx, ctx, args.large, args.tol, 1, 1, not args.no_plain)
+        rl = measure(icx, ctx, args.large, args.tol, 1, 1, not args.no_plain, sr)
         large = summary(args.large, rl, args.tol)
         large["roofline_A"] = {"achieved": rl["achieved_A"] / 1e9, "frac": rl["achieved_A"] / HBM_PEAK,
                                "bytes_per_launch": rl["bytes_A"],
-                               "traffic": measured_traffic(*args.large)}
+                               "traffic": measured_traffic(*args.large, sr)}
         print("large %dx%dx%d: %.2f ms/solve" % (tuple(args.large) + (rl["ms_step"],)), file=sys.stderr,
               flush=True)
 
@@ -455,6 +474,8 @@ def main():
 
     cfg = summary(args.size, r, args.tol)
     cfg["graphs"] = not args.no_graphs
+    cfg["pipelined_polling"] = not args.no_pipeline
+    cfg["iteration"] = "single-reduction kernel" if sr else "kernels A + B"
     cfg["large"] = large
     line = {
         "metric": "DOF-matvec/s + % HBM peak, FP32-refined vs FP64 at res 1e-8; p50/p99 ms",
@@ -465,8 +486,10 @@ def main():
         "config": cfg,
         "roofline": {"bound": "hbm", "achieved": r["achieved_A"] / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": r["achieved_A"] / HBM_PEAK,
-                     "traffic": measured_traffic(*args.size),
-                     "kernel": "k_hex8<float,15,15,1> (fused direction update + operator)",
+                     "traffic": measured_traffic(*args.size, sr),
+                     "kernel": ("k_hex8_sr<0> (single-reduction PCG iteration: direction, operator, "
+                                "residual and solution updates)" if sr else
+                                "k_hex8<float,15,15,1> (fused direction update + operator)"),
                      "avg_launch_us": r["avg_A"] * 1e6, "launches": r["launches_A"],
                      "executed_launches": r["executed_A"],
                      "bytes_per_launch": r["bytes_A"]},

@@ -30,10 +30,13 @@ struct icx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  SolverState *h_state = nullptr;  // pinned mirror for polling
+  SolverState *h_state = nullptr;  // pinned mirrors for polling, two slots
+  hipEvent_t ev_state[2] = {nullptr, nullptr};  // state copy into slot k landed
   double *h_scalar = nullptr;      // pinned scalars
   int *h_int = nullptr;            // pinned ints (iteration base)
   bool use_graphs = true;          // replay captured iteration chunks (icx_set_option)
+  bool single_reduction = false;   // hexa8 mixed solves: one kernel per iteration (opt-in)
+  bool pipeline = true;            // keep one chunk queued ahead of the state read
   int tile_depth = 0;              // hexa8 operator z depth per block, 0 = automatic
   // profile window (icx_profile.h)
   std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
@@ -405,6 +408,11 @@ struct PcgWork64 {
 struct MixedWork {
   DBuf<float> r, pa, pb, Ap, Ap2, xlo, invd, E, b32;  // Ap2: seam halves of kernel A
   DBuf<double> Kx;
+  // single-reduction form (hexa8 solves): r = {r, r1}, p = {pa, pb}, Ap = {Ap, ap1};
+  // per-block partials by launch parity
+  bool sr = false;
+  DBuf<float> r1, ap1;
+  DBuf<double> part;
 };
 struct FusedWork {
   DBuf<double> r, pa, pb, Ap, invd;
@@ -531,11 +539,19 @@ namespace {
 
 hipStream_t S(icx_job *j) { return j->ctx->stream; }
 
+// A stop recorded in `pending` is final once the launches before the copy have ended
+// (the single-reduction iteration records its stop there only; the next launch of the
+// two-kernel forms publishes it as `done`).
+inline void settle(SolverState &st) {
+  if (!st.done && st.pending) st.done = st.pending;
+}
+
 icx_status read_state(icx_job *j, SolverState &st) {
   icx_ctx *c = j->ctx;
   HIPCHK(c, hipMemcpyAsync(c->h_state, j->d_state.p, sizeof(SolverState), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipStreamSynchronize(S(j)));
   st = *c->h_state;
+  settle(st);
   if (c->ev_used) prof_collect(c);
   return ICX_OK;
 }
@@ -602,6 +618,40 @@ icx_status run_chunk(icx_job *j, ChunkGraph &g, F launch_chunk) {
   return ICX_OK;
 }
 
+// Runs chunks of a solve until the device state reports a stop and leaves that state in
+// st.  Pipelined (the default): every chunk is followed on the stream by a copy of the
+// state into one of two pinned slots, and the chunk after the one whose state the host
+// waits for is already queued, so the device does not idle while the host polls.  A
+// chunk queued after the stop costs only its launches: every kernel of a chunk returns
+// at once when it reads `done`.  Eager launches under a profile window poll chunk by
+// chunk.
+template <class F>
+icx_status run_to_stop(icx_job *j, ChunkGraph &g, F launch_chunk, SolverState &st) {
+  icx_ctx *c = j->ctx;
+  icx_status rc = ICX_OK;
+  if (st.done) return ICX_OK;
+  if (!c->pipeline || c->prof_kernel) {
+    while (!st.done) {
+      if ((rc = run_chunk(j, g, launch_chunk))) return rc;
+      if ((rc = read_state(j, st))) return rc;
+    }
+    return ICX_OK;
+  }
+  auto post = [&](int slot) -> icx_status {
+    HIPCHK(c, hipMemcpyAsync(c->h_state + slot, j->d_state.p, sizeof(SolverState), hipMemcpyDeviceToHost, S(j)));
+    HIPCHK(c, hipEventRecord(c->ev_state[slot], S(j)));
+    return ICX_OK;
+  };
+  if ((rc = run_chunk(j, g, launch_chunk)) || (rc = post(0))) return rc;
+  for (int slot = 0; !st.done; slot ^= 1) {
+    if ((rc = run_chunk(j, g, launch_chunk)) || (rc = post(slot ^ 1))) return rc;
+    HIPCHK(c, hipEventSynchronize(c->ev_state[slot]));
+    st = c->h_state[slot];
+    settle(st);
+  }
+  return ICX_OK;
+}
+
 // ---- FP64 plain PCG: the Eigen ConjugateGradient loop ------------------------------
 // One kernel per Eigen vector expression (launch_eigen_iteration_tail) after the
 // operator; used for the structural parity form and for the Helmholtz filter solves.
@@ -623,11 +673,7 @@ icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const 
       launch_eigen_iteration_tail(N, j->d_state.p, x, w.r.p, w.p.p, w.t.p, w.z.p, invd, P, S(j));
     }
   };
-  while (!st.done) {
-    if ((rc = run_chunk(j, graph, chunk))) return rc;
-    rc = read_state(j, st);
-    if (rc) return rc;
-  }
+  if ((rc = run_to_stop(j, graph, chunk, st))) return rc;
   const bool conv = st.done == 1;
   info.iterations = conv ? st.stop_iter : st.iter;
   info.matvecs = (conv && st.rhsNorm2 > 0 && st.rr < st.rhsNorm2) ? st.stop_iter + 1 : st.iter;
@@ -653,10 +699,12 @@ icx_status icx_ctx_create(int device, icx_ctx **out) {
   auto *c = new icx_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void **)&c->h_state, sizeof(SolverState), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->h_state, 2 * sizeof(SolverState), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_state[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_state[1], hipEventDisableTiming) != hipSuccess ||
       hipHostMalloc((void **)&c->h_scalar, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void **)&c->h_int, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-    delete c;
+    (void)icx_ctx_destroy(c);
     return ICX_ERR_HIP;
   }
   *out = c;
@@ -678,6 +726,16 @@ icx_status icx_set_option(icx_ctx *ctx, int option, long long value) {
   if (!ctx) return ICX_ERR_ARG;
   if (option == ICX_OPT_GRAPHS) {
     ctx->use_graphs = value != 0;
+    return ICX_OK;
+  }
+  if (option == ICX_OPT_SINGLE_REDUCTION) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->single_reduction = value != 0;
+    for (icx_job *j : ctx->jobs) j->reset_graphs();  // the chunk's launches change
+    return ICX_OK;
+  }
+  if (option == ICX_OPT_PIPELINE) {
+    ctx->pipeline = value != 0;
     return ICX_OK;
   }
   if (option == ICX_OPT_TILE_DEPTH) {
@@ -713,6 +771,8 @@ icx_status icx_ctx_destroy(icx_ctx *ctx) {
   for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   if (ctx->h_state) (void)hipHostFree(ctx->h_state);
+  for (hipEvent_t e : ctx->ev_state)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->h_scalar) (void)hipHostFree(ctx->h_scalar);
   if (ctx->h_int) (void)hipHostFree(ctx->h_int);
   delete ctx;
@@ -1171,11 +1231,7 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
       }
     }
   };
-  while (!st.done) {
-    if ((rc = run_chunk(j, j->gr_fused, chunk))) return rc;
-    rc = read_state(j, st);
-    if (rc) return rc;
-  }
+  if ((rc = run_to_stop(j, j->gr_fused, chunk, st))) return rc;
   if (st.done != 1) launch_pcg_flush(N, j->d_state.p, j->d_u.p, w.pa.p, w.pb.p, S(j));
   rc = read_state(j, st);
   if (rc) return rc;
@@ -1207,21 +1263,49 @@ static SeamGeom mixed_seams(icx_job *j) {
 
 // Set-up of a mixed solve: FP32 operands, x = 0, r = b, partials; st receives the state
 // after the set-up (done = 1 at once for a zero right-hand side).
-static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st) {
+static SrVectors sr_vectors(icx_job *j) {
+  MixedWork &w = j->mw;
+  SrVectors v{};
+  v.r[0] = w.r.p; v.r[1] = w.r1.p;
+  v.p[0] = w.pa.p; v.p[1] = w.pb.p;
+  v.ap[0] = w.Ap.p; v.ap[1] = w.ap1.p;
+  v.x = w.xlo.p;
+  v.invd = w.invd.p;
+  v.E = w.E.p;
+  v.mask = j->d_mask.p;
+  v.part[0] = w.part.p;
+  v.part[1] = w.part.p + 6 * RED_MAXBLOCKS;
+  const PcgPartials P = mixed_parts(j);
+  v.pb = P.PB;
+  v.npb = P.nB;
+  return v;
+}
+
+// hexa8 solves take the single-reduction iteration when it is switched on
+static bool use_sr(const icx_job *j) { return j->g.ndim == 3 && !j->heat() && j->ctx->single_reduction; }
+
+// Set-up of a mixed solve.  sr: the single-reduction iteration (icx_sr.hip) instead of
+// kernels A and B.
+static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st, bool sr) {
   icx_ctx *c = j->ctx;
   const long long N = j->nvec();
   MixedWork &w = j->mw;
+  w.sr = sr;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N)); HIPCHK(c, w.Ap.alloc(N));
-  HIPCHK(c, w.Ap2.alloc(N));
   HIPCHK(c, w.xlo.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.E.alloc(j->g.ne)); HIPCHK(c, w.Kx.alloc(N));
   HIPCHK(c, w.b32.alloc(N));
+  if (sr) {
+    HIPCHK(c, w.r1.alloc(N)); HIPCHK(c, w.ap1.alloc(N)); HIPCHK(c, w.part.alloc(12 * RED_MAXBLOCKS));
+  } else {
+    HIPCHK(c, w.Ap2.alloc(N));
+  }
+  HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(float), S(j)));
   const PcgPartials P = mixed_parts(j);
   launch_convert_f64_f32(j->g.ne, j->d_E.p, w.E.p, S(j));
   launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, nullptr, w.invd.p, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.xlo.p, 0, N * sizeof(float), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(float), S(j)));
-  HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(float), S(j)));
   // r32 = b; the tolerance is tested on the FP64 |b| (the replacement residual is FP64)
   launch_convert_f64_f32(N, j->d_b.p, w.b32.p, S(j));
   launch_pcg_setup<float>(N, j->d_state.p, w.b32.p, w.r.p, w.invd.p, P, tol, maxit, 0.01, S(j));
@@ -1234,7 +1318,12 @@ static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st
   st.rhsNorm2 = bb;
   st.threshold = std::max(tol * tol * bb, 2.2250738585072014e-308);
   if (bb == 0.0) st.done = 1;
+  st.iter2[0] = st.iter2[1] = 0;
+  st.resume_iter = 0;
+  st.stop_slot = 0;
   HIPCHK(c, hipMemcpyAsync(j->d_state.p, &st, sizeof(SolverState), hipMemcpyHostToDevice, S(j)));
+  // w_0 = K invd r_0 and the partials of iteration 0 (nothing when done)
+  if (sr) launch_sr_restart(j->g, j->op32, sr_vectors(j), j->d_state.p, S(j));
   HIPCHK(c, hipStreamSynchronize(S(j)));
   return ICX_OK;
 }
@@ -1263,9 +1352,11 @@ static void mixed_replace(icx_job *j) {
   const long long N = j->nvec();
   MixedWork &w = j->mw;
   const PcgPartials P = mixed_parts(j);
+  // single-reduction form: x is complete at the stop (no step owed)
   launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pb.p, S(j));
   launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
   launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, S(j));
+  if (w.sr) launch_sr_restart(j->g, j->op32, sr_vectors(j), j->d_state.p, S(j));
 }
 
 static void mixed_info(const SolverState &st, int refinements, icx_solve_info &info) {
@@ -1278,20 +1369,25 @@ static void mixed_info(const SolverState &st, int refinements, icx_solve_info &i
 
 static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info &info) {
   SolverState st;
-  icx_status rc = mixed_begin(j, tol, maxit, st);
+  const bool sr = use_sr(j);
+  icx_status rc = mixed_begin(j, tol, maxit, st, sr);
   if (rc) return rc;
-  const int every = poll_every(j->nvec());
+  const int every = poll_every(j->nvec());  // even: launch parities repeat chunk by chunk
   int refinements = 0;
+  const SrVectors sv = sr ? sr_vectors(j) : SrVectors{};
+  icx_ctx *c = j->ctx;
   auto chunk = [&]() {
-    for (int k = 0; k < every; ++k) mixed_iteration(j);
+    for (int k = 0; k < every; ++k) {
+      if (sr) {
+        ProfScope ps(c, ICX_PROF_FUSED_A);
+        launch_sr_iteration(j->g, j->op32, sv, j->d_state.p, k & 1, S(j), ps.ev());
+      } else {
+        mixed_iteration(j);
+      }
+    }
   };
   while (!st.done || st.done >= 2) {
-    if (!st.done) {
-      if ((rc = run_chunk(j, j->gr_mixed, chunk))) return rc;
-      rc = read_state(j, st);
-      if (rc) return rc;
-      if (!st.done) continue;
-    }
+    if ((rc = run_to_stop(j, j->gr_mixed, chunk, st))) return rc;
     const int was = st.done;
     mixed_replace(j);
     ++refinements;
@@ -1319,12 +1415,13 @@ struct BatchRun {
   bool dirty = true;
   DBuf<OpArgs<float>> a3, a2;
   DBuf<BArgs<float>> ab;
-  DBuf<int> off3, off2, offb;
+  DBuf<SrArgs> as0, as1;  // single-reduction jobs, launch parity 0 / 1
+  DBuf<int> off3, off2, offb, offs;
   DBuf<SolverState *> sptr;
   DBuf<SolverState> sdev;
   SolverState *h_states = nullptr;  // pinned
   int h_cap = 0;
-  int n3 = 0, n2 = 0, nb3 = 0, nb2 = 0, nbb = 0, every = 8;
+  int n3 = 0, n2 = 0, nab = 0, ns = 0, nb3 = 0, nb2 = 0, nbb = 0, nbs = 0, every = 8;
   ChunkGraph graph;
   ~BatchRun() {
     if (h_states) (void)hipHostFree(h_states);
@@ -1338,11 +1435,21 @@ static int pad8(int n) { return (n + 7) & ~7; }
 static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
   std::vector<OpArgs<float>> a3, a2;
   std::vector<BArgs<float>> ab;
-  std::vector<int> o3{0}, o2{0}, ob{0};
+  std::vector<SrArgs> as0, as1;
+  std::vector<int> o3{0}, o2{0}, ob{0}, os{0};
   std::vector<SolverState *> sp;
   long long maxN = 0;
   for (icx_job *j : B.active) {
     MixedWork &w = j->mw;
+    sp.push_back(j->d_state.p);
+    maxN = std::max(maxN, j->nvec());
+    if (w.sr) {
+      const SrVectors v = sr_vectors(j);
+      as0.push_back(sr_args(j->g, j->op32, v, j->d_state.p, 0));
+      as1.push_back(sr_args(j->g, j->op32, v, j->d_state.p, 1));
+      os.push_back(os.back() + pad8(sr_blocks(j->g)));
+      continue;
+    }
     const PcgPartials P = mixed_parts(j);
     OpArgs<float> a;
     const int nb = fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
@@ -1351,12 +1458,10 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
     else { a2.push_back(a); o2.push_back(o2.back() + pad8(nb)); }
     ab.push_back(BArgs<float>{j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, w.Ap2.p, fusedA_seams(a)});
     ob.push_back(ob.back() + pad8(P.nB));
-    sp.push_back(j->d_state.p);
-    maxN = std::max(maxN, j->nvec());
   }
   const int n = (int)B.active.size();
-  B.n3 = (int)a3.size(); B.n2 = (int)a2.size();
-  B.nb3 = o3.back(); B.nb2 = o2.back(); B.nbb = ob.back();
+  B.n3 = (int)a3.size(); B.n2 = (int)a2.size(); B.nab = (int)ab.size(); B.ns = (int)as0.size();
+  B.nb3 = o3.back(); B.nb2 = o2.back(); B.nbb = ob.back(); B.nbs = os.back();
   B.every = poll_every(maxN);
   auto up = [&](auto &dbuf, const auto &vec) -> icx_status {
     if (vec.empty()) return ICX_OK;
@@ -1366,7 +1471,8 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
   };
   icx_status rc;
   if ((rc = up(B.a3, a3)) || (rc = up(B.a2, a2)) || (rc = up(B.ab, ab)) || (rc = up(B.off3, o3)) ||
-      (rc = up(B.off2, o2)) || (rc = up(B.offb, ob)) || (rc = up(B.sptr, sp)))
+      (rc = up(B.off2, o2)) || (rc = up(B.offb, ob)) || (rc = up(B.sptr, sp)) || (rc = up(B.as0, as0)) ||
+      (rc = up(B.as1, as1)) || (rc = up(B.offs, os)))
     return rc;
   HIPCHK(c, B.sdev.alloc(std::max(n, 1)));
   if (B.h_cap < n) {
@@ -1382,11 +1488,13 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
   return ICX_OK;
 }
 
+// `every` is even: the launch parities of the single-reduction jobs repeat chunk by chunk
 static void batch_chunk(icx_ctx *c, BatchRun &B) {
   for (int k = 0; k < B.every; ++k) {
+    launch_sr_batch((k & 1) ? B.as1.p : B.as0.p, B.offs.p, B.ns, B.nbs, c->stream);
     launch_fusedA_batch<float>(3, B.a3.p, B.off3.p, B.n3, B.nb3, c->stream);
     launch_fusedA_batch<float>(2, B.a2.p, B.off2.p, B.n2, B.nb2, c->stream);
-    launch_pcg_fusedB_batch<float>(B.ab.p, B.offb.p, (int)B.active.size(), B.nbb, c->stream);
+    launch_pcg_fusedB_batch<float>(B.ab.p, B.offb.p, B.nab, B.nbb, c->stream);
   }
 }
 
@@ -1395,6 +1503,7 @@ static icx_status batch_states(icx_ctx *c, BatchRun &B) {
   launch_gather_states(B.sptr.p, B.sdev.p, n, c->stream);
   HIPCHK(c, hipMemcpyAsync(B.h_states, B.sdev.p, n * sizeof(SolverState), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < n; ++q) settle(B.h_states[q]);
   return ICX_OK;
 }
 
@@ -2204,7 +2313,7 @@ icx_status icx_submit(icx_job *j, double tol, int maxit) {
   icx_status rc = build_rhs(j);
   if (rc) return rc;
   SolverState st;
-  rc = mixed_begin(j, tol, maxit, st);
+  rc = mixed_begin(j, tol, maxit, st, use_sr(j));
   if (rc) return rc;
   b.active = true;
   c->batch->active.push_back(j);
@@ -2346,9 +2455,9 @@ icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
   for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(*d);
   for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd}) add(*d);
   for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.Ap2, &j->mw.xlo, &j->mw.invd, &j->mw.E,
-                  &j->mw.b32})
+                  &j->mw.b32, &j->mw.r1, &j->mw.ap1})
     add(*d);
-  add(j->mw.Kx);
+  add(j->mw.Kx); add(j->mw.part);
   for (auto *d : {&j->hw.invd, &j->hw.b, &j->hw.x, &j->hw.r, &j->hw.p, &j->hw.t, &j->hw.z}) add(*d);
   add(j->cw.off); add(j->cw.w); add(j->cw.aa);
   for (auto *d : {&j->mma.low, &j->mma.upp, &j->mma.alpha, &j->mma.beta, &j->mma.p0, &j->mma.q0, &j->mma.p1,

@@ -40,6 +40,15 @@ __device__ __forceinline__ double row_next(double v) {
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
+// load / store at a 32-bit byte offset from a uniform base (global_load with an SGPR
+// base and a 32-bit VGPR offset; node vectors stay below 4 GiB)
+template <typename T> __device__ __forceinline__ T ld_off(const T *base, uint32_t off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T> __device__ __forceinline__ void st_off(T *base, uint32_t off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 // Snapshot of a solver state at kernel entry (scalar loads: one round trip).  Every
 // field an iteration kernel reads is written only by other kernels (kernel A records a
 // stop in `pending`, kernel B publishes it as `done`), so all waves of all blocks of a

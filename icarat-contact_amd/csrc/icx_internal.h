@@ -56,6 +56,12 @@ struct SolverState {
   int flush_iter;      // iteration whose owed x update was already applied
   double rz_hist[2];   // fused/mixed forms: r_i.z_i, stored by kernel A of iteration i in slot i & 1
   double alpha_hist[2];// fused/mixed forms: alpha_i, stored by kernel B of iteration i in slot i & 1
+                       // (single-reduction form: r_i.z_i and alpha_i, stored by launch i + 1)
+  // single-reduction form (FP32 hexa8 mixed solves, one kernel per iteration)
+  int iter2[2];        // iteration index read by the launches of parity k from slot k; a launch
+                       // of parity k that continues writes the next index to slot k ^ 1
+  int resume_iter;     // first launch after a restart (set-up or residual replacement)
+  int stop_slot;       // launch parity of the stopping launch
 };
 
 // Per-block partial sums of a PCG solve, double-buffered by iteration parity.
@@ -186,6 +192,56 @@ void launch_fusedA_batch(int ndim, const OpArgs<T> *args, const int *off, int nj
 // number of blocks of kernel A for this grid (its partial count)
 template <typename T>
 int elastic_fusedA_blocks(const Grid &g);
+
+// ---- single-reduction PCG iteration, FP32 hexa8 (icx_sr.hip) -------------------------
+// One kernel per iteration of the mixed form's Jacobi-PCG: launch k takes the step of
+// iteration k-1 (alpha = rz/pi, r -= alpha Ap, x += alpha p), forms z = invd r and
+// p = z + beta p with beta from the expansion of the new r.z in quantities the previous
+// launch reduced (rz - 2 alpha Ap.z + alpha^2 Ap.invd Ap), applies the operator and
+// writes the partials (r.z, |r|^2, p.Ap, Ap.z, Ap.invd Ap) of iteration k.  r, p and Ap
+// alternate between two buffers by launch parity (neighbouring blocks read them at the
+// halo); x is updated in place.
+struct SrVectors {
+  float *r[2], *p[2], *ap[2];
+  float *x;
+  const float *invd, *E;
+  const uint8_t *mask;
+  double *part[2];     // [RED_MAXBLOCKS][6]: the five partials per block, by launch parity
+  const double *pb;    // (|r|^2, r.z) partials of the set-up or the replacement residual
+  int npb;             // their count
+};
+// Arguments of one launch of the iteration kernel (one job, one launch parity).
+struct SrArgs {
+  Grid g;
+  int TK, tiles_x, tiles_y, tiles_z;
+  ElasticOp<float> op;
+  const float *E;
+  const uint8_t *mask;
+  const float *invd;
+  const float *r_in, *ap_in, *p_in;
+  float *r_out, *ap_out, *p_out;
+  float *x;
+  SolverState *st;
+  const double *part_in;
+  double *part_out;
+  const double *pb;
+  int npb;
+  int slot;
+};
+SrArgs sr_args(const Grid &g, const ElasticOp<float> &op, const SrVectors &v, SolverState *st, int slot);
+// block count of the iteration kernel (its partial count)
+int sr_blocks(const Grid &g);
+// iteration of launch parity slot (chunks of iterations are an even number of launches)
+void launch_sr_iteration(const Grid &g, const ElasticOp<float> &op, const SrVectors &v, SolverState *st,
+                         int slot, hipStream_t s, const LaunchEvents *ev = nullptr);
+// the iteration kernel for a batch of jobs (args of one launch parity and off in device memory)
+void launch_sr_batch(const SrArgs *args, const int *off, int njobs, int total_blocks, hipStream_t s);
+// restart after the set-up or a residual replacement (r[0] holds the new residual, pb its
+// partials): iterate stop_iter formed again, p[0] = invd r + beta' p_{stop_iter-1},
+// Ap[0] = K p[0], partials into part[0]; state: iter2[0] = resume_iter = stop_iter + 1.
+// Does nothing when the state says done.
+void launch_sr_restart(const Grid &g, const ElasticOp<float> &op, const SrVectors &v, SolverState *st,
+                       hipStream_t s);
 
 void launch_helm_matvec(const Grid &g, const HelmOp<double> &op, const double *x, double *y,
                         const SolverState *st, hipStream_t s);

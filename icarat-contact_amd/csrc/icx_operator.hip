@@ -64,15 +64,6 @@ template <typename T> struct OwnedNode {
   uint32_t m;
 };
 
-// load / store at a 32-bit byte offset from a uniform base (global_load with an SGPR
-// base and a 32-bit VGPR offset; node vectors stay below 4 GiB)
-template <typename T> __device__ __forceinline__ T ld_off(const T *base, uint32_t off) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
-}
-template <typename T> __device__ __forceinline__ void st_off(T *base, uint32_t off, T v) {
-  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
-}
-
 template <typename T, int TI, int TJ, int MODE>
 __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   constexpr int NT = (TI + 1) * (TJ + 1);

@@ -37,11 +37,15 @@ def job_spec(rng, elems_lo, elems_hi):
 def footprint_estimate(ndim, div):
     """Device bytes of one served job (mixed solve, cone filter, OC design loop):
     84 B per DOF, 92 B per element, 1 B per node, 0.4 MB of partial sums and state;
-    icx_job_footprint reports the same buffers once they exist."""
+    hexa8 jobs (single-reduction iteration) hold four more FP32 vectors (100 B per DOF)
+    and 0.375 MB more partial sums.  icx_job_footprint reports the same buffers once
+    they exist."""
     d0, d1, d2 = div
     nn = (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
     ne = d0 * d1 * (d2 if ndim == 3 else 1)
     n = ndim * nn
+    if ndim == 3:
+        return 100 * n + 92 * ne + nn + 784 * 1024
     return 84 * n + 92 * ne + nn + 400 * 1024
 
 
@@ -63,16 +67,19 @@ def make_job(ctx, ndim, div):
 
 
 def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, seed=SEED,
-        tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None):
+        tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None, limit=None):
     """Serve n_jobs jobs arriving at `rate` per second (exponential gaps), each running
     `iters` OC design iterations.  specs: optional list of (ndim, div) instead of the
-    drawn sizes; keep_designs: return each job's final design under "designs".
-    Returns a dict of the measurements."""
+    drawn sizes; keep_designs: return each job's final design under "designs"; limit:
+    serve only the first `limit` jobs of the drawn stream.  Returns a dict of the
+    measurements."""
     rng = np.random.default_rng(np.random.PCG64(seed))
     gaps = rng.exponential(1.0 / rate, n_jobs)
     arrival = np.cumsum(gaps) - gaps[0]
     drawn = [job_spec(rng, *elems) for _ in range(n_jobs)]
     specs = list(specs) if specs is not None else drawn
+    if limit is not None:
+        specs = specs[:limit]
     n_jobs = len(specs)
     est = [footprint_estimate(*s) for s in specs]
     if max(est) > budget:
@@ -143,6 +150,7 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         "admission_wait_ms_p99": float(np.percentile(admit_wait, 99) * 1e3),
         "peak_admitted_jobs": peak_active, "peak_budget_bytes": peak_used,
         "arrival_rate_per_s": rate, "iterations_per_job": iters, "elements": list(elems),
+        "stream_jobs_drawn": len(drawn),
         "budget_bytes": budget, "tol": tol,
         "footprint_checks": footprints,
         "designs": designs if keep_designs else None,

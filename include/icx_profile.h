@@ -29,8 +29,16 @@ icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
  * ICX_OPT_TILE_DEPTH (default 0 = automatic): node layers along z per block of the
  *   hexa8 operator kernels, 1..64 (raised as needed to keep the block count within
  *   the partial-sum buffers).  Applies to every job of the context; results of the
- *   operator are identical, PCG dot products change only in rounding. */
-enum { ICX_OPT_GRAPHS = 1, ICX_OPT_TILE_DEPTH = 2 };
+ *   operator are identical, PCG dot products change only in rounding.
+ * ICX_OPT_PIPELINE (default 1): while the host waits for the solver state after a chunk
+ *   of PCG iterations, the next chunk is already queued (it does nothing if the solve
+ *   has stopped); 0 polls chunk by chunk.  Results are identical.
+ * ICX_OPT_SINGLE_REDUCTION (default 0): 1 runs hexa8 mixed-precision solves (icx_solve and
+ *   submitted solves) with one kernel and one reduction phase per PCG iteration instead
+ *   of two (the next r.z is formed from an expansion in dot products reduced with the
+ *   previous iteration's).  Same iterates up to rounding, same stopping tests; on one
+ *   GPU the two-kernel iteration is faster.  Set it before submitting solves. */
+enum { ICX_OPT_GRAPHS = 1, ICX_OPT_TILE_DEPTH = 2, ICX_OPT_PIPELINE = 3, ICX_OPT_SINGLE_REDUCTION = 4 };
 icx_status icx_set_option(icx_ctx *ctx, int option, long long value);
 
 #ifdef __cplusplus

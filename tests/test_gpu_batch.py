@@ -119,3 +119,24 @@ def test_many_large_jobs_equal_alone(ctx):
             assert np.array_equal(j.displacement(), u0)
     for j in jobs:
         j.close()
+
+
+def test_single_reduction_batch_equals_alone(ctx):
+    """With the single-reduction iteration switched on, hexa8 jobs run it in the batch
+    (one launch of the iteration kernel for all of them per iteration) and quad4 jobs
+    keep kernels A and B; every job still equals its own solve bit for bit."""
+    ctx.set_option(icx.OPT_SINGLE_REDUCTION, 1)
+    try:
+        rng = np.random.default_rng(31)
+        jobs = [icx.random_job(ctx, rng, dims=3, size3d=(6, 20)) for _ in range(4)]
+        jobs += [icx.random_job(ctx, rng, dims=2, size2d=(12, 60)) for _ in range(2)]
+        ref = _alone(jobs)
+        infos = ctx.solve_batch(jobs, tol=1e-8)
+        for j, (u0, i0), i1 in zip(jobs, ref, infos):
+            assert i1["iterations"] == i0["iterations"]
+            assert i1["relres"] == i0["relres"]
+            assert np.array_equal(j.displacement(), u0)
+    finally:
+        ctx.set_option(icx.OPT_SINGLE_REDUCTION, 0)
+    for j in jobs:
+        j.close()

@@ -135,6 +135,53 @@ def test_mixed_form_every_tile_depth(ctx, depth):
     assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-7
 
 
+@pytest.mark.parametrize("depth", [0, 1, 3])
+@pytest.mark.parametrize("dims", [(9, 5, 7), (20, 9, 8)])
+def test_single_reduction_matches_two_kernel_form(ctx, depth, dims):
+    """Switched on, the hexa8 mixed form runs one kernel per iteration (the next r.z from
+    an expansion reduced with the previous iteration's dot products); the iterates are
+    those of the two-kernel iteration up to rounding.  Both meet the true residual
+    after several replacements, agree with the direct solution and take nearly the same
+    number of iterations, at every tile depth."""
+    job, pr, rho, ud = _solve_case(ctx, *dims, seed=13)
+    ctx.set_option(icx.OPT_TILE_DEPTH, depth)
+    try:
+        u2, i2 = job.solve(icx.SOLVE_MIXED, tol=1e-10, maxit=20000)
+        ctx.set_option(icx.OPT_SINGLE_REDUCTION, 1)
+        u1, i1 = job.solve(icx.SOLVE_MIXED, tol=1e-10, maxit=20000)
+    finally:
+        ctx.set_option(icx.OPT_TILE_DEPTH, 0)
+        ctx.set_option(icx.OPT_SINGLE_REDUCTION, 0)
+    b = pr.assemble(rho)
+    K = pr.csr()
+    for u, info in ((u1, i1), (u2, i2)):
+        assert info["converged"] and info["refinements"] >= 2
+        assert np.linalg.norm(b - K @ u) / np.linalg.norm(b) < 1e-10
+        assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-7
+    assert abs(i1["iterations"] - i2["iterations"]) <= max(3, int(0.05 * i2["iterations"]))
+
+
+def test_single_reduction_iteration_limit_and_zero_load(ctx):
+    job, pr, rho, ud = _solve_case(ctx, 10, 6, 5, seed=17)
+    ctx.set_option(icx.OPT_SINGLE_REDUCTION, 1)
+    try:
+        _single_reduction_limits(job, pr, rho, ud)
+    finally:
+        ctx.set_option(icx.OPT_SINGLE_REDUCTION, 0)
+
+
+def _single_reduction_limits(job, pr, rho, ud):
+    u, info = job.solve(icx.SOLVE_MIXED, tol=1e-12, maxit=9)
+    assert not info["converged"] and info["iterations"] == 9
+    K = pr.csr()
+    # CG from x = 0 lowers the energy norm of the error at every step
+    e = u - ud
+    assert np.isfinite(u).all() and e @ (K @ e) < 0.5 * (ud @ (K @ ud))
+    job.set_fext(np.zeros(job.numeq))
+    u, info = job.solve(icx.SOLVE_MIXED, tol=1e-8)
+    assert info["converged"] and info["iterations"] == 0 and not np.any(u)
+
+
 def test_2d_solve_and_node_values(ctx):
     job = icx.cantilever_2d(ctx, 40, 14)
     pr = O.cantilever_2d(40, 14)
