@@ -17,7 +17,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace icx;
@@ -460,6 +462,103 @@ struct OptWork {
   DBuf<double> x, fil, thr, dthr, dfds, dgds, tmp, tmp2;
 };
 
+// ---- element stiffness of a general quad4 / hexa8 -------------------------------------
+// LinearElastic::makeKeFint (src/problem/elasticity.cpp:210-254) on an element of any
+// shape: Ke = sum_ip B^T D B det J at the 2 x 2 (x 2) Gauss points +-1/sqrt(3), weight 1,
+// local nodes in the reference's order (src/problem/shape_function.cpp:97-119, 403-447),
+// engineering shears ordered xy, yz, zx (src/problem/bmatrix.hpp:62-69, 124-140).
+// Returns false when det J <= 0 at a Gauss point (inverted or degenerate element).
+static bool general_Ke(int nq, int ndim, int state, double nu, const double *X, double *Ke, double &vol) {
+  static const int corner[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                   {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  const int nd = nq * ndim;
+  double D[36];
+  int voigt;
+  material_matrix(state, 1.0, nu, D, voigt);
+  std::fill(Ke, Ke + nd * nd, 0.0);
+  vol = 0.0;
+  const double g = 1.0 / std::sqrt(3.0);
+  double B[6 * 24], DB[6 * 24];
+  for (int ip = 0; ip < nq; ++ip) {
+    double r[3] = {0, 0, 0};
+    for (int d = 0; d < ndim; ++d) r[d] = corner[ip][d] * g;
+    // dN/dr in natural coordinates
+    double dr[3][8];
+    const double sc = ndim == 3 ? 0.125 : 0.25;
+    for (int a = 0; a < nq; ++a)
+      for (int d = 0; d < ndim; ++d) {
+        double v = sc * corner[a][d];
+        for (int e = 0; e < ndim; ++e)
+          if (e != d) v *= 1.0 + corner[a][e] * r[e];
+        dr[d][a] = v;
+      }
+    // J[d][e] = dx_e / dr_d
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int d = 0; d < ndim; ++d)
+      for (int e = 0; e < ndim; ++e)
+        for (int a = 0; a < nq; ++a) J[d][e] += dr[d][a] * X[3 * a + e];
+    double det, inv[3][3];
+    if (ndim == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      inv[0][0] = J[1][1] / det; inv[0][1] = -J[0][1] / det;
+      inv[1][0] = -J[1][0] / det; inv[1][1] = J[0][0] / det;
+    } else {
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      inv[0][0] = c00 / det;
+      inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      inv[1][0] = c01 / det;
+      inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      inv[2][0] = c02 / det;
+      inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    if (!(det > 0.0)) return false;
+    // dN/dX_e = sum_d inv(J)[e][d] dN/dr_d
+    double dX[3][8];
+    for (int e = 0; e < ndim; ++e)
+      for (int a = 0; a < nq; ++a) {
+        double v = 0.0;
+        for (int d = 0; d < ndim; ++d) v += inv[e][d] * dr[d][a];
+        dX[e][a] = v;
+      }
+    std::fill(B, B + voigt * nd, 0.0);
+    for (int a = 0; a < nq; ++a) {
+      if (ndim == 2) {
+        B[0 * nd + 2 * a] = dX[0][a];
+        B[1 * nd + 2 * a + 1] = dX[1][a];
+        B[2 * nd + 2 * a] = dX[1][a];
+        B[2 * nd + 2 * a + 1] = dX[0][a];
+      } else {
+        B[0 * nd + 3 * a] = dX[0][a];
+        B[1 * nd + 3 * a + 1] = dX[1][a];
+        B[2 * nd + 3 * a + 2] = dX[2][a];
+        B[3 * nd + 3 * a] = dX[1][a]; B[3 * nd + 3 * a + 1] = dX[0][a];
+        B[4 * nd + 3 * a + 1] = dX[2][a]; B[4 * nd + 3 * a + 2] = dX[1][a];
+        B[5 * nd + 3 * a] = dX[2][a]; B[5 * nd + 3 * a + 2] = dX[0][a];
+      }
+    }
+    for (int v = 0; v < voigt; ++v)
+      for (int i = 0; i < nd; ++i) {
+        double t = 0.0;
+        for (int w = 0; w < voigt; ++w) t += D[v * voigt + w] * B[w * nd + i];
+        DB[v * nd + i] = t;
+      }
+    for (int i = 0; i < nd; ++i)
+      for (int k = 0; k < nd; ++k) {
+        double t = 0.0;
+        for (int v = 0; v < voigt; ++v) t += B[v * nd + i] * DB[v * nd + k];
+        Ke[i * nd + k] += t * det;
+      }
+    vol += det;
+  }
+  return true;
+}
+
 struct icx_job {
   icx_ctx *ctx = nullptr;
   icx_problem prob{};
@@ -495,6 +594,14 @@ struct icx_job {
     gr_helm.reset();
   }
   MixedWork mw;
+  // unstructured mesh (icx_job_create_mesh): coordinates (numnp x 3), connectivity
+  // (nelx x ne, the reference's local node order), element volumes; device operator
+  bool mesh = false;
+  std::vector<double> mx, mvol;
+  std::vector<int> mconn;
+  DBuf<int> d_conn, d_adj_ptr, d_adj;
+  DBuf<double> d_ke, d_ye, d_vol;
+  MeshOp mop{};
   // state of a submitted (batched) mixed solve
   struct Submitted {
     bool active = false;     // in the context's batch
@@ -581,6 +688,7 @@ icx_status build_rhs(icx_job *j) {
   HIPCHK(c, hipMemcpyAsync(j->d_b.p, j->d_fext.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
   if (j->any_dirich_value) {
     if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr, S(j));
+    else if (j->mesh) launch_mesh_matvec(j->mop, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr, S(j));
     else launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->d_dirich.p, j->d_vec.p, nullptr, S(j));
     launch_axpby(N, -1.0, j->d_vec.p, 1.0, j->d_b.p, S(j));
     launch_mask_zero(j->g, j->nc, j->d_mask.p, j->d_b.p, S(j));
@@ -787,6 +895,31 @@ icx_status icx_ctx_synchronize(icx_ctx *ctx) {
   return ICX_OK;
 }
 
+// host BC state and the device buffers every job holds; registers the job
+static icx_status job_finish(icx_ctx *ctx, icx_job *j, icx_job **out) {
+  const Grid &g = j->g;
+  j->has.assign(g.nn, 0);
+  j->node_flag.assign(g.nn * 6, 0);
+  j->node_val.assign(g.nn * 6, 0.0);
+  (void)hipSetDevice(ctx->device);
+  const long long N = j->nvec();
+  if (j->d_partial.alloc(2 * RED_MAXBLOCKS) || j->d_pcgpart.alloc(10 * RED_MAXBLOCKS) || j->d_state.alloc(1) ||
+      j->d_scalars.alloc(64) || j->d_fext.alloc(N) || j->d_b.alloc(N) || j->d_u.alloc(N) ||
+      j->d_design.alloc(g.ne) || j->d_E.alloc(g.ne) || j->d_vec.alloc(N) || j->d_vec2.alloc(N) ||
+      j->d_mask.alloc(g.nn)) {
+    delete j;
+    return fail(ctx, ICX_ERR_NOMEM, "device allocation failed");
+  }
+  (void)hipMemsetAsync(j->d_pcgpart.p, 0, 10 * RED_MAXBLOCKS * sizeof(double), ctx->stream);
+  (void)hipMemsetAsync(j->d_state.p, 0, sizeof(SolverState), ctx->stream);
+  (void)hipMemsetAsync(j->d_fext.p, 0, N * sizeof(double), ctx->stream);
+  (void)hipMemsetAsync(j->d_mask.p, 0, g.nn, ctx->stream);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) { delete j; return fail(ctx, ICX_ERR_HIP, "stream sync failed"); }
+  ctx->jobs.push_back(j);
+  *out = j;
+  return ICX_OK;
+}
+
 icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   if (!ctx || !p || !out) return ICX_ERR_ARG;
   *out = nullptr;
@@ -826,25 +959,173 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
     if (!build_elastic_op(j->el, K, j->op64, why)) { delete j; return fail(ctx, ICX_ERR_NUMERIC, "%s", why.c_str()); }
     j->op32 = cast_op<float>(j->op64);
   }
-  j->has.assign(g.nn, 0);
-  j->node_flag.assign(g.nn * 6, 0);
-  j->node_val.assign(g.nn * 6, 0.0);
+  return job_finish(ctx, j, out);
+}
+
+// Element matrices of a mesh job, in parallel over host threads: Ke0 entry-major
+// ([k][e]) and the volumes; false with the first bad element in `bad`.
+static bool mesh_elements(const icx_job *j, std::vector<double> &ke, std::vector<double> &vol, long long &bad) {
+  const int nq = j->el.nq, ndim = j->g.ndim, nd = nq * ndim;
+  const long long ne = j->g.ne;
+  ke.assign((size_t)nd * nd * ne, 0.0);
+  vol.assign(ne, 0.0);
+  const unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<long long> first_bad(nt, -1);
+  auto work = [&](unsigned t) {
+    std::vector<double> K(nd * nd);
+    double X[24];
+    for (long long e = t; e < ne; e += nt) {
+      for (int a = 0; a < nq; ++a)
+        for (int d = 0; d < 3; ++d) X[3 * a + d] = j->mx[3 * (size_t)j->mconn[e * nq + a] + d];
+      double v;
+      if (!general_Ke(nq, ndim, j->prob.state, j->prob.poisson, X, K.data(), v)) {
+        if (first_bad[t] < 0) first_bad[t] = e;
+        continue;
+      }
+      vol[e] = v;
+      for (int k = 0; k < nd * nd; ++k) ke[(size_t)k * ne + e] = K[k];
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto &t : th) t.join();
+  bad = -1;
+  for (long long b : first_bad)
+    if (b >= 0 && (bad < 0 || b < bad)) bad = b;
+  return bad < 0;
+}
+
+icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
+  if (!ctx || !m || !out || !m->x || !m->conn) return ICX_ERR_ARG;
+  *out = nullptr;
+  if (m->eltype != ICX_QUAD4 && m->eltype != ICX_HEXA8)
+    return fail(ctx, ICX_ERR_ARG, "eltype must be ICX_QUAD4 or ICX_HEXA8");
+  const int ndim = m->eltype == ICX_QUAD4 ? 2 : 3, nq = m->eltype;
+  if (m->numnp < 1 || m->nelx < 1) return fail(ctx, ICX_ERR_ARG, "empty mesh");
+  if ((long long)m->numnp > (1LL << 27)) return fail(ctx, ICX_ERR_ARG, "mesh too large (more than 2^27 nodes)");
+  if (m->state == ICX_HEAT) return fail(ctx, ICX_ERR_ARG, "mesh jobs: elastic states only");
+  if ((ndim == 3) != (m->state == ICX_SOLID3D))
+    return fail(ctx, ICX_ERR_ARG, "state must be ICX_SOLID3D exactly for hexa8");
+  if (!(m->poisson > -1.0 && m->poisson < 0.5)) return fail(ctx, ICX_ERR_ARG, "poisson out of range");
+  for (long long k = 0; k < (long long)m->nelx * nq; ++k)
+    if (m->conn[k] < 0 || m->conn[k] >= m->numnp) return fail(ctx, ICX_ERR_ARG, "connectivity entry %lld out of range", k);
+  auto *j = new icx_job();
+  j->ctx = ctx;
+  j->mesh = true;
+  icx_problem &p = j->prob;
+  p.eltype = m->eltype;
+  p.state = m->state;
+  p.young1 = m->young1;
+  p.young2 = m->young2;
+  p.poisson = m->poisson;
+  p.penalty = m->penalty;
+  Grid &g = j->g;
+  g.ndim = ndim;
+  g.nn = m->numnp;
+  g.ne = m->nelx;
+  g.nx = g.ny = g.nz = 1;
+  g.sx = m->numnp; g.sy = g.sz = 1;
+  g.tile_depth = ctx->tile_depth;
+  j->nc = ndim;
+  j->el.ndim = ndim;
+  j->el.nq = nq;
+  j->mx.assign(m->x, m->x + 3 * (size_t)m->numnp);
+  j->mconn.assign(m->conn, m->conn + (size_t)m->nelx * nq);
+  std::vector<double> ke;
+  long long bad;
+  if (!mesh_elements(j, ke, j->mvol, bad)) {
+    delete j;
+    return fail(ctx, ICX_ERR_NUMERIC, "element %lld has a non-positive Jacobian determinant", bad);
+  }
+  // connectivity node-of-element-major, node adjacency (e * nq + a, increasing e)
+  const long long ne = g.ne, nn = g.nn;
+  std::vector<int> conn_t((size_t)nq * ne), ptr(nn + 1, 0), adj((size_t)nq * ne);
+  for (long long e = 0; e < ne; ++e)
+    for (int a = 0; a < nq; ++a) {
+      conn_t[(size_t)a * ne + e] = j->mconn[e * nq + a];
+      ++ptr[j->mconn[e * nq + a] + 1];
+    }
+  for (long long n = 0; n < nn; ++n) ptr[n + 1] += ptr[n];
+  {
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (long long e = 0; e < ne; ++e)
+      for (int a = 0; a < nq; ++a) adj[fill[j->mconn[e * nq + a]]++] = (int)(e * nq + a);
+  }
   (void)hipSetDevice(ctx->device);
-  const long long N = j->nvec();
-  if (j->d_partial.alloc(2 * RED_MAXBLOCKS) || j->d_pcgpart.alloc(10 * RED_MAXBLOCKS) || j->d_state.alloc(1) ||
-      j->d_scalars.alloc(64) || j->d_fext.alloc(N) || j->d_b.alloc(N) || j->d_u.alloc(N) ||
-      j->d_design.alloc(g.ne) || j->d_E.alloc(g.ne) || j->d_vec.alloc(N) || j->d_vec2.alloc(N) ||
-      j->d_mask.alloc(g.nn)) {
+  const int nd = nq * ndim;
+  hipStream_t s = ctx->stream;
+  if (j->d_conn.alloc(conn_t.size()) || j->d_adj_ptr.alloc(ptr.size()) || j->d_adj.alloc(adj.size()) ||
+      j->d_ke.alloc(ke.size()) || j->d_ye.alloc((size_t)ne * nd) || j->d_vol.alloc(ne)) {
     delete j;
     return fail(ctx, ICX_ERR_NOMEM, "device allocation failed");
   }
-  (void)hipMemsetAsync(j->d_pcgpart.p, 0, 10 * RED_MAXBLOCKS * sizeof(double), ctx->stream);
-  (void)hipMemsetAsync(j->d_state.p, 0, sizeof(SolverState), ctx->stream);
-  (void)hipMemsetAsync(j->d_fext.p, 0, N * sizeof(double), ctx->stream);
-  (void)hipMemsetAsync(j->d_mask.p, 0, g.nn, ctx->stream);
-  if (hipStreamSynchronize(ctx->stream) != hipSuccess) { delete j; return fail(ctx, ICX_ERR_HIP, "stream sync failed"); }
-  ctx->jobs.push_back(j);
-  *out = j;
+  if (hipMemcpyAsync(j->d_conn.p, conn_t.data(), conn_t.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(j->d_adj_ptr.p, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(j->d_adj.p, adj.data(), adj.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(j->d_ke.p, ke.data(), ke.size() * sizeof(double), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(j->d_vol.p, j->mvol.data(), ne * sizeof(double), hipMemcpyHostToDevice, s) ||
+      hipStreamSynchronize(s)) {
+    delete j;
+    return fail(ctx, ICX_ERR_HIP, "upload of the mesh failed");
+  }
+  j->mop = MeshOp{nq, ndim, nn, ne, j->d_conn.p, j->d_ke.p, j->d_adj_ptr.p, j->d_adj.p, j->d_ye.p};
+  return job_finish(ctx, j, out);
+}
+
+// MeshMDPA::setParameter / generate (src/io/mesh_mdpa.hpp:84-216): the nodes of the
+// "Begin Nodes" block in file order (conn refers to them by id - 1, as the reference's
+// node array does), the elements of every "Begin Elements" block whose name carries the
+// element tag ("2D4" quad4, "3D8" hexa8) in file order, "id property n1 .. nne" per line.
+icx_status icx_mdpa_read(const char *path, int eltype, int *numnp, int *nelx, double *x, int *conn) {
+  if (!path || !numnp || !nelx) return ICX_ERR_ARG;
+  if (eltype != ICX_QUAD4 && eltype != ICX_HEXA8) return ICX_ERR_ARG;
+  std::FILE *f = std::fopen(path, "r");
+  if (!f) return ICX_ERR_ARG;
+  const char *tag = eltype == ICX_QUAD4 ? "2D4" : "3D8";
+  const int nq = eltype;
+  std::vector<double> xs;
+  std::vector<int> cs;
+  int nnodes = 0, block = 0;  // block: 0 none, 1 nodes, 2 elements of the type, 3 other elements
+  bool ok = true;
+  char buf[4096];
+  while (ok && std::fgets(buf, sizeof buf, f)) {
+    std::string line(buf);
+    while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
+    if (block == 0) {
+      if (line.rfind("Begin Nodes", 0) == 0) block = 1;
+      else if (line.rfind("Begin Elements", 0) == 0) block = line.find(tag) != std::string::npos ? 2 : 3;
+      continue;
+    }
+    if ((block == 1 && line == "End Nodes") || (block >= 2 && line == "End Elements")) {
+      block = 0;
+      continue;
+    }
+    if (block == 1) {
+      long id;
+      double v[3];
+      if (std::sscanf(line.c_str(), "%ld %lf %lf %lf", &id, &v[0], &v[1], &v[2]) != 4) { ok = false; break; }
+      ++nnodes;
+      xs.insert(xs.end(), v, v + 3);
+    } else if (block == 2) {
+      std::istringstream is(line);
+      long id, prop;
+      if (!(is >> id >> prop)) { ok = false; break; }
+      for (int a = 0; a < nq; ++a) {
+        long n;
+        if (!(is >> n)) { ok = false; break; }
+        cs.push_back((int)(n - 1));
+      }
+    }
+  }
+  std::fclose(f);
+  if (!ok || block != 0 || nnodes == 0 || cs.empty()) return ICX_ERR_ARG;
+  for (int v : cs)
+    if (v < 0 || v >= nnodes) return ICX_ERR_ARG;
+  *numnp = nnodes;
+  *nelx = (int)(cs.size() / nq);
+  if (x) std::memcpy(x, xs.data(), xs.size() * sizeof(double));
+  if (conn) std::memcpy(conn, cs.data(), cs.size() * sizeof(int));
   return ICX_OK;
 }
 
@@ -871,6 +1152,10 @@ icx_status icx_job_sizes(const icx_job *j, int *numnp, int *nelx, int *neq) {
 
 // node coordinates x = (L/div) * index (src/io/mesh_icarat.hpp:325-335, 440-455)
 static void node_coord(const icx_job *j, long long n, double *x) {
+  if (j->mesh) {
+    for (int d = 0; d < 3; ++d) x[d] = j->mx[3 * n + d];
+    return;
+  }
   const Grid &g = j->g;
   const long long i = n % g.sx, jj = (n / g.sx) % g.sy, k = n / ((long long)g.sx * g.sy);
   x[0] = (j->prob.length[0] / j->prob.div[0]) * (double)i;
@@ -883,6 +1168,7 @@ static const int kRefCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                      {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
 
 static long long elem_node(const icx_job *j, long long e, int k) {
+  if (j->mesh) return j->mconn[e * j->el.nq + k];
   const Grid &g = j->g;
   const long long ex = e % g.nx, ey = (e / g.nx) % g.ny, ez = e / ((long long)g.nx * g.ny);
   const int *cc = kRefCorner[k];
@@ -1145,7 +1431,10 @@ icx_status icx_matvec(icx_job *j, int prec, const double *x, double *y) {
   HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
   HIPCHK(c, hipMemcpyAsync(tmp.p, x, j->numeq * sizeof(double), hipMemcpyHostToDevice, S(j)));
   launch_dof_to_grid(j->g, j->nc, j->d_dof.p, j->numeq, tmp.p, nullptr, j->d_vec.p, S(j));
-  if (j->heat()) {
+  if (j->mesh) {
+    if (prec != 0) return fail(c, ICX_ERR_ARG, "mesh jobs: FP64 operator only");
+    launch_mesh_matvec(j->mop, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
+  } else if (j->heat()) {
     if (prec != 0) return fail(c, ICX_ERR_ARG, "heat jobs: FP64 operator only");
     launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
   } else if (prec == 0) {
@@ -1169,6 +1458,7 @@ icx_status icx_diagonal(icx_job *j, double *diag) {
   if (j->numeq < 0 || !j->design_set) return ICX_ERR_STATE;
   icx_ctx *c = j->ctx;
   if (j->heat()) launch_heat_invdiag(j->g, j->heat_op, j->d_E.p, j->d_mask.p, j->d_vec.p, S(j));
+  else if (j->mesh) launch_mesh_diag(j->mop, j->d_E.p, j->d_mask.p, j->d_vec.p, true, S(j));
   else launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, j->d_vec.p, nullptr, S(j));
   DBuf<double> tmp;
   HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
@@ -1187,10 +1477,12 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.p.alloc(N)); HIPCHK(c, w.t.alloc(N));
   HIPCHK(c, w.z.alloc(N)); HIPCHK(c, w.invd.alloc(N));
   if (j->heat()) launch_heat_invdiag(j->g, j->heat_op, j->d_E.p, j->d_mask.p, w.invd.p, S(j));
+  else if (j->mesh) launch_mesh_diag(j->mop, j->d_E.p, j->d_mask.p, w.invd.p, true, S(j));
   else launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
   auto mv = [&](const double *p, double *t) {
     ProfScope ps(c, ICX_PROF_MATVEC64);
     if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
+    else if (j->mesh) launch_mesh_matvec(j->mop, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
     else launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, p, t, j->d_state.p, S(j), ps.ev());
   };
   return pcg_plain(j, N, j->d_b.p, j->d_u.p, w.invd.p, w, tol, maxit, mv, info, j->gr_plain);
@@ -1536,6 +1828,7 @@ icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx
     return fail(j->ctx, ICX_ERR_STATE, "icx_solve needs icx_make_dof and icx_set_design");
   if (!(tol > 0.0) || maxit < 1) return fail(j->ctx, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
   if (j->heat() && form != ICX_SOLVE_PLAIN) return fail(j->ctx, ICX_ERR_ARG, "heat jobs: ICX_SOLVE_PLAIN only");
+  if (j->mesh && form != ICX_SOLVE_PLAIN) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: ICX_SOLVE_PLAIN only");
   icx_ctx *c = j->ctx;
   (void)hipSetDevice(c->device);
   icx_solve_info inf{};
@@ -1619,6 +1912,7 @@ static MisesMat mises_mat(const icx_job *j) {
 }
 
 icx_status icx_mises(icx_job *j, double *out) {
+  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "von Mises stress");
   if (!j || !out) return ICX_ERR_ARG;
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "icx_mises: elastic jobs only");
   if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
@@ -1643,6 +1937,7 @@ icx_status icx_reactions(icx_job *j, double *out) {
   HIPCHK(c, j->ow.nodes.alloc(j->g.nn * 3));
   // K u over all DOFs: the operator without the constraint mask
   if (j->heat()) launch_heat_matvec(j->g, j->heat_op, j->d_E.p, nullptr, j->ow.ufull.p, ku.p, nullptr, S(j));
+  else if (j->mesh) launch_mesh_matvec(j->mop, j->d_E.p, nullptr, j->ow.ufull.p, ku.p, nullptr, S(j));
   else launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, nullptr, j->ow.ufull.p, ku.p, nullptr, S(j));
   launch_grid_to_nodes(j->g, j->nc, ku.p, j->ow.nodes.p, S(j));
   HIPCHK(c, hipMemcpyAsync(out, j->ow.nodes.p, j->g.nn * 3 * sizeof(double), hipMemcpyDeviceToHost, S(j)));
@@ -1651,6 +1946,7 @@ icx_status icx_reactions(icx_job *j, double *out) {
 }
 
 icx_status icx_output_begin(icx_job *j) {
+  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the VTK output step");
   if (!j) return ICX_ERR_ARG;
   if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
   icx_ctx *c = j->ctx;
@@ -1804,6 +2100,9 @@ icx_status icx_sensitivity(icx_job *j, double vscale, double *dfds, double *dgds
   if (rc) return rc;
   if (j->heat())
     launch_heat_sensitivity(j->g, j->heat_dK, j->prob.penalty, j->d_design.p, u, j->volume, vscale, a.p, b.p, S(j));
+  else if (j->mesh)
+    launch_mesh_sensitivity(j->mop, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, u, j->d_vol.p,
+                            vscale, a.p, b.p, S(j));
   else
     launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, u,
                        j->volume, vscale, a.p, b.p, S(j));
@@ -1815,7 +2114,9 @@ icx_status icx_sensitivity(icx_job *j, double vscale, double *dfds, double *dgds
 
 icx_status icx_element_volume(const icx_job *j, double *vol) {
   if (!j || !vol) return ICX_ERR_ARG;
-  for (long long e = 0; e < j->g.ne; ++e) vol[e] = j->volume;
+  if (j->mesh) std::memcpy(vol, j->mvol.data(), j->g.ne * sizeof(double));
+  else
+    for (long long e = 0; e < j->g.ne; ++e) vol[e] = j->volume;
   return ICX_OK;
 }
 
@@ -1905,6 +2206,7 @@ icx_status helm_apply_dev(icx_job *j, int sens, double tol, const double *din, d
 extern "C" {
 
 icx_status icx_cone_filter_setup(icx_job *j, double radius) {
+  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the cone filter");
   if (!j || !(radius > 0.0)) return ICX_ERR_ARG;
   return cone_setup(j, radius);
 }
@@ -1925,6 +2227,7 @@ icx_status icx_cone_filter(icx_job *j, int adjoint, const double *in, double *ou
 }
 
 icx_status icx_helmholtz_setup(icx_job *j, double radius) {
+  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the Helmholtz filter");
   if (!j || !(radius > 0.0)) return ICX_ERR_ARG;
   return helm_setup(j, radius);
 }
@@ -2097,6 +2400,7 @@ extern "C" {
 
 icx_status icx_oc_update(icx_job *j, double *x, const double *dfdx, double g, const double *dgdx, double xmin,
                          double xmax, double move, int filter, double beta, double T, int *bisections) {
+  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the OC update");
   if (!j || !x || !dfdx || !dgdx) return ICX_ERR_ARG;
   if (filter == ICX_FILTER_CONE && !j->cw.ready) return fail(j->ctx, ICX_ERR_STATE, "cone filter not set up");
   if (filter == ICX_FILTER_HELMHOLTZ && !j->hw.ready) return fail(j->ctx, ICX_ERR_STATE, "Helmholtz filter not set up");
@@ -2140,6 +2444,7 @@ icx_status icx_mma_update(icx_job *j, double *x, const double *dfdx, double gx, 
 
 // ---- device-resident design loop ------------------------------------------------------------
 icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0) {
+  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the device design loop");
   if (!j || !par || !x0) return ICX_ERR_ARG;
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
@@ -2294,6 +2599,7 @@ icx_status icx_opt_get_design(const icx_job *jc, double *x) {
 }
 
 icx_status icx_submit(icx_job *j, double tol, int maxit) {
+  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "submitted solves");
   if (!j) return ICX_ERR_ARG;
   icx_ctx *c = j->ctx;
   if (j->numeq < 0 || !j->design_set)

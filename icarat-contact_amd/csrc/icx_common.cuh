@@ -49,6 +49,10 @@ template <typename T> __device__ __forceinline__ void st_off(T *base, uint32_t o
   *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
 }
 
+// |v| < 1e-15 flushed to 0, as divideValIntoNode does before the sensitivity reads
+// node values (src/analysis/analysis_utility.hpp:27-28)
+__device__ __forceinline__ double flush15(double v) { return fabs(v) < 1.0e-15 ? 0.0 : v; }
+
 // Snapshot of a solver state at kernel entry (scalar loads: one round trip).  Every
 // field an iteration kernel reads is written only by other kernels (kernel A records a
 // stop in `pending`, kernel B publishes it as `done`), so all waves of all blocks of a

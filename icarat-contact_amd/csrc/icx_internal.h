@@ -193,6 +193,33 @@ void launch_fusedA_batch(int ndim, const OpArgs<T> *args, const int *off, int nj
 template <typename T>
 int elastic_fusedA_blocks(const Grid &g);
 
+// ---- unstructured-mesh elastic operator (icx_mesh.hip) -------------------------------
+// Mesh of one element type (ne = 4 quad4 / 8 hexa8, nc = 2 / 3 components): connectivity
+// node-of-element-major ([a][e]), unit-modulus element matrices entry-major ([k][e],
+// k = i * nd + j, nd = ne * nc, node-major local DOFs a * nc + c), node adjacency in CSR
+// (entries e * ne + a, increasing e), and a scratch of nd doubles per element.
+struct MeshOp {
+  int ne, nc;
+  long long nn, nelx;
+  const int *conn;
+  const double *ke;
+  const int *adj_ptr, *adj;
+  double *ye;
+};
+// y = K(E) x on the node vectors (SoA, nc components), zero on constrained components;
+// st may be null, otherwise both passes exit at once when st->done is set
+void launch_mesh_matvec(const MeshOp &m, const double *E, const uint8_t *mask, const double *x, double *y,
+                        const SolverState *st, hipStream_t s, const LaunchEvents *ev = nullptr);
+// diagonal of K(E), or its Jacobi inverse (1 where the diagonal is zero or constrained)
+void launch_mesh_diag(const MeshOp &m, const double *E, const uint8_t *mask, double *out, bool inv,
+                      hipStream_t s);
+// compliance sensitivity -dE/drho u_e^T Ke0 u_e and dv = vol_e / vscale per element
+void launch_mesh_sensitivity(const MeshOp &m, double y1, double y2, double p, const double *design,
+                             const double *u, const double *vol, double vscale, double *dfds, double *dgds,
+                             hipStream_t s);
+// E_e u_e^T Ke0_e u_e
+void launch_mesh_energy(const MeshOp &m, const double *E, const double *u, double *energy, hipStream_t s);
+
 // ---- single-reduction PCG iteration, FP32 hexa8 (icx_sr.hip) -------------------------
 // One kernel per iteration of the mixed form's Jacobi-PCG: launch k takes the step of
 // iteration k-1 (alpha = rz/pi, r -= alpha Ap, x += alpha p), forms z = invd r and

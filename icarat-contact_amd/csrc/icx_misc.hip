@@ -192,7 +192,6 @@ void launch_mask_zero(const Grid &g, int nc, const uint8_t *mask, double *v, hip
 // ---------------------------------------------------------------------------------
 // sensitivity (opt_macro_oc.cpp:182-212, opt_mma_Helmholtz.cpp:173-205)
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ double flush15(double v) { return fabs(v) < 1.0e-15 ? 0.0 : v; }
 
 template <bool SENS>
 __global__ void k_element_energy(Grid g, ElasticOp<double> op, double y1, double y2, double p,

@@ -21,7 +21,7 @@ ERR_NOCONV = 5
 
 EXPORTS = [
     "icx_ctx_create", "icx_ctx_destroy", "icx_last_error", "icx_ctx_synchronize",
-    "icx_job_create", "icx_job_destroy", "icx_job_sizes", "icx_get_mesh",
+    "icx_job_create", "icx_job_create_mesh", "icx_mdpa_read", "icx_job_destroy", "icx_job_sizes", "icx_get_mesh",
     "icx_add_dirichlet", "icx_make_dof", "icx_get_dof_map", "icx_add_neumann",
     "icx_set_fext", "icx_get_fext", "icx_add_nodal_load",
     "icx_set_design", "icx_set_conductivity", "icx_matvec", "icx_diagonal", "icx_solve", "icx_get_displacement",
@@ -41,6 +41,29 @@ class Problem(C.Structure):
     _fields_ = [("eltype", C.c_int), ("div", C.c_int * 3), ("length", C.c_double * 3),
                 ("state", C.c_int), ("young1", C.c_double), ("young2", C.c_double),
                 ("poisson", C.c_double), ("penalty", C.c_double)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("eltype", C.c_int), ("numnp", C.c_int), ("nelx", C.c_int),
+                ("x", C.POINTER(C.c_double)), ("conn", C.POINTER(C.c_int)), ("state", C.c_int),
+                ("young1", C.c_double), ("young2", C.c_double), ("poisson", C.c_double),
+                ("penalty", C.c_double)]
+
+
+def read_mdpa(path, eltype):
+    """Nodes (numnp x 3, flat) and 0-based connectivity (flat) of an MDPA file's
+    elements of one type (MeshMDPA, src/io/mesh_mdpa.hpp)."""
+    n, m = C.c_int(), C.c_int()
+    p = str(path).encode()
+    rc = lib().icx_mdpa_read(p, int(eltype), C.byref(n), C.byref(m), None, None)
+    if rc:
+        raise IcxError(rc, "cannot read %s as an MDPA mesh of element type %d" % (path, eltype))
+    x = np.zeros(3 * n.value)
+    conn = np.zeros(m.value * int(eltype), dtype=np.int32)
+    rc = lib().icx_mdpa_read(p, int(eltype), C.byref(n), C.byref(m), _dp(x), _ip(conn))
+    if rc:
+        raise IcxError(rc, "cannot read %s" % path)
+    return x, conn
 
 
 class SolveInfo(C.Structure):
@@ -184,6 +207,31 @@ class Job:
         self.dofnp = 1 if state == HEAT else self.ndim
         self.div = d
         self.numeq = None
+
+    @classmethod
+    def from_mesh(cls, ctx, eltype, x, conn, state, young=(1e-9, 1.0), nu=0.3, penalty=3.0):
+        """A job on an unstructured mesh (icx_job_create_mesh): x numnp x 3, conn nelx x ne
+        (0-based, the reference's local node order)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1)
+        m = MeshDesc()
+        m.eltype, m.numnp, m.nelx = int(eltype), len(x) // 3, len(conn) // int(eltype)
+        m.x, m.conn = _dp(x), _ip(conn)
+        m.state, m.young1, m.young2 = state, float(young[0]), float(young[1])
+        m.poisson, m.penalty = float(nu), float(penalty)
+        self.h = C.c_void_p()
+        ctx.check(lib().icx_job_create_mesh(ctx.h, C.byref(m), C.byref(self.h)))
+        ctx.jobs.add(self)
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        lib().icx_job_sizes(self.h, C.byref(a), C.byref(b), C.byref(c))
+        self.numnp, self.nelx, self.neq = a.value, b.value, c.value
+        self.ndim = 2 if eltype == QUAD4 else 3
+        self.dofnp = self.ndim
+        self.div = None
+        self.numeq = None
+        return self
 
     def close(self):
         if self.h and self.ctx.h:

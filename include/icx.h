@@ -87,6 +87,29 @@ icx_status icx_ctx_synchronize(icx_ctx *ctx);
  * LinearElastic::setParameter (src/problem/elasticity.cpp:19-32). */
 icx_status icx_job_create(icx_ctx *ctx, const icx_problem *prob, icx_job **out);
 icx_status icx_job_destroy(icx_job *job);
+
+/* Unstructured mesh of one element type (MeshMDPA, src/io/mesh_mdpa.hpp:84-238, with
+ * the elements of LinearElastic, src/problem/elasticity.cpp:210-254): node coordinates
+ * and 0-based connectivity in the reference's local node order (quad4: counter-clockwise;
+ * hexa8: the z-low face counter-clockwise, then the z-high face).  Element matrices are
+ * integrated per element (2 x 2 (x 2) Gauss points); an element with det J <= 0 at a
+ * Gauss point is rejected (ICX_ERR_NUMERIC).  Mesh jobs take the FP64 plain solve form
+ * and the cone filter; the box-only entry points (FP32 operator, fused and mixed forms,
+ * Helmholtz filter, von Mises stress, submitted solves) return ICX_ERR_ARG. */
+typedef struct {
+  int eltype;          /* ICX_QUAD4 or ICX_HEXA8 */
+  int numnp, nelx;
+  const double *x;     /* numnp x 3 (z ignored for quad4) */
+  const int *conn;     /* nelx x ne, 0-based */
+  int state;           /* ICX_PLANE_STRESS / ICX_PLANE_STRAIN (quad4), ICX_SOLID3D (hexa8) */
+  double young1, young2, poisson, penalty;
+} icx_mesh;
+icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *mesh, icx_job **out);
+/* MDPA file (MeshMDPA::setParameter / generate, src/io/mesh_mdpa.hpp:84-216): the nodes
+ * of the "Begin Nodes" block in file order and the elements of every "Begin Elements"
+ * block whose name carries the element's tag ("2D4" quad4, "3D8" hexa8), in file order;
+ * connectivity as node id - 1.  x and conn may be NULL to query the sizes first. */
+icx_status icx_mdpa_read(const char *path, int eltype, int *numnp, int *nelx, double *x, int *conn);
 /* numnp, nelx, neq (= numnp * dofnp) */
 icx_status icx_job_sizes(const icx_job *job, int *numnp, int *nelx, int *neq);
 /* node coordinates (numnp x 3) and connectivity (nelx x ne), as MeshIcarat builds them */

@@ -121,6 +121,31 @@ class Problem:
         self.fext = None
         self._sys = None
 
+    @classmethod
+    def from_mesh(cls, ne, x, conn, state, young=(1e-9, 1.0), nu=0.3, penalty=3.0):
+        """Problem on an explicit mesh (MeshMDPA, src/io/mesh_mdpa.hpp:84-238): x numnp x 3,
+        conn nelx x ne (0-based, the reference's local node order)."""
+        self = cls.__new__(cls)
+        self.ne = int(ne)
+        self.ndim = 2 if ne == 4 else 3
+        self.dofnp = 1 if state == HEAT else self.ndim
+        self.div = None
+        self.length = None
+        self.state = state
+        self.young1, self.young2 = float(young[0]), float(young[1])
+        self.nu, self.p = float(nu), float(penalty)
+        self.x = _f64(x).reshape(-1).copy()
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1).copy()
+        self.numnp, self.nelx = len(self.x) // 3, len(self.conn) // self.ne
+        self.has = np.zeros(self.numnp, dtype=np.int32)
+        self.node_flag = np.zeros(self.numnp * 6, dtype=np.int32)
+        self.node_val = np.zeros(self.numnp * 6)
+        self.dof = None
+        self.numeq = None
+        self.fext = None
+        self._sys = None
+        return self
+
     # -- boundary conditions ------------------------------------------------
     def add_dirichlet(self, flag6, val6, range6):
         assert self.dof is None, "Dirichlet ranges must precede make_dof"

@@ -1,0 +1,137 @@
+"""Unstructured-mesh jobs (icx_job_create_mesh, MeshMDPA meshes) against the oracle on
+the same explicit meshes: the FP64 operator and its diagonal to rounding, the plain
+solve to 1e-10 of the direct solution, compliance, sensitivities and volumes; a box
+mesh given explicitly equals the structured job."""
+import numpy as np
+import pytest
+
+import icx
+import oracle as O
+from meshgen import annulus_quad, box_mesh, distort, write_mdpa
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    (4, O.PLANE_STRESS, (12, 6), (12.0, 6.0)),
+    (4, O.PLANE_STRAIN, (9, 7), (9.0, 7.0)),
+    (8, O.SOLID3D, (7, 4, 5), (7.0, 4.0, 5.0)),
+]
+
+
+def _pair(ctx, ne, state, x, conn, L):
+    job = icx.Job.from_mesh(ctx, ne, x, conn, state)
+    pr = O.Problem.from_mesh(ne, x, conn, state)
+    nd = 2 if ne == 4 else 3
+    clamp = [0.0, 0.0, 0.0, L[1], 0.0, L[2] if nd == 3 else 0.0]
+    flags = [1] * nd + [0] * (6 - nd)
+    for o in (job, pr):
+        o.add_dirichlet(flags, [0] * 6, clamp)
+    assert job.make_dof() == pr.make_dof()
+    load = [L[0], L[0], 0.0, L[1], 0.0, L[2] if nd == 3 else 0.0]
+    val = [0.0, -0.1, 0.0] if nd == 2 else [0.0, 0.0, -0.1]
+    for o in (job, pr):
+        o.add_neumann(val, load)
+    assert np.abs(job.fext() - pr.fext).max() <= 1e-15 * np.abs(pr.fext).max()
+    rho = O.random_density(pr.nelx, 9, lo=0.05, hi=1.0)
+    job.set_design(rho)
+    b = pr.assemble(rho)
+    return job, pr, rho, b
+
+
+@pytest.mark.parametrize("ne,state,div,L", CASES)
+def test_mesh_operator_solve_sensitivity(ctx, ne, state, div, L):
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 3, L)
+    job, pr, rho, b = _pair(ctx, ne, state, xd, conn, L)
+    v = np.random.default_rng(1).standard_normal(pr.numeq)
+    y, yo = job.matvec(v), pr.spmv(v)
+    assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
+    K = pr.csr()
+    assert np.abs(job.diagonal() - K.diagonal()).max() <= 1e-14 * np.abs(K.diagonal()).max()
+    u, info = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=50000)
+    ud = pr.direct(b)
+    assert info["converged"]
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-10
+    c = job.compliance()
+    assert abs(c - ud @ pr.fext) / abs(ud @ pr.fext) < 1e-10
+    dc, dv = job.sensitivity()
+    vol = pr.volumes()
+    dco, dvo = pr.sensitivity(rho, pr.element_energy(pr.node_values(ud)), vol)
+    assert np.abs(dc - dco).max() <= 1e-8 * np.abs(dco).max()
+    assert np.abs(dv - vol).max() <= 1e-14 * vol.max()
+    assert np.abs(job.volumes() - vol).max() <= 1e-14 * vol.max()
+    # K u over all DOFs: the applied load on the free DOFs, and zero in total (K 1 = 0:
+    # the supports carry the load)
+    R = job.reactions().reshape(-1, 3)
+    assert np.abs(R.sum(axis=0)).max() <= 1e-12 * np.abs(R).max()
+    free = [(n, c, pr.dof[n * pr.dofnp + c]) for n in range(pr.numnp) for c in range(pr.dofnp)
+            if pr.dof[n * pr.dofnp + c] < pr.numeq]
+    Rf = np.array([R[n, c] for n, c, _ in free])
+    F = np.array([pr.fext[d] for _, _, d in free])
+    assert np.abs(Rf - F).max() <= 1e-9 * np.abs(F).max()
+
+
+@pytest.mark.parametrize("ne,div,L", [(4, (10, 5), (10.0, 5.0)), (8, (6, 3, 4), (6.0, 3.0, 4.0))])
+def test_box_mesh_equals_structured_job(ctx, ne, div, L):
+    """The structured box given as an explicit mesh: same operator, same solve."""
+    state = O.PLANE_STRESS if ne == 4 else O.SOLID3D
+    x, conn = box_mesh(ne, div, L)
+    a = icx.Job(ctx, ne, div, L, state)
+    b = icx.Job.from_mesh(ctx, ne, x, conn, state)
+    nd = 2 if ne == 4 else 3
+    for j in (a, b):
+        j.add_dirichlet([1] * nd + [0] * (6 - nd), [0] * 6, [0, 0, 0, L[1], 0, L[2] if nd == 3 else 0])
+        j.make_dof()
+        j.add_neumann([0.0, -1.0, 0.0] if nd == 2 else [0.0, 0.0, -1.0],
+                      [L[0], L[0], 0, L[1], 0, L[2] if nd == 3 else 0])
+        j.set_design(O.random_density(j.nelx, 4))
+    assert np.array_equal(a.fext(), b.fext())
+    v = np.random.default_rng(2).standard_normal(a.numeq)
+    ya, yb = a.matvec(v), b.matvec(v)
+    assert np.abs(ya - yb).max() <= 1e-13 * np.abs(ya).max()
+    ua, _ = a.solve(icx.SOLVE_PLAIN, tol=1e-12, maxit=50000)
+    ub, _ = b.solve(icx.SOLVE_PLAIN, tol=1e-12, maxit=50000)
+    assert np.linalg.norm(ua - ub) / np.linalg.norm(ua) < 1e-9
+    assert np.abs(a.volumes() - b.volumes()).max() <= 1e-14
+
+
+def test_mdpa_annulus_with_curved_traction(ctx, tmp_path):
+    """A quarter annulus read from an MDPA file, clamped on its x = 0 edge, loaded on the
+    y = 0 edge: tractions integrated on the element sides the box selects (judgeLine)."""
+    x, conn = annulus_quad(6, 12, 1.0, 2.0)
+    path = tmp_path / "ring.mdpa"
+    write_mdpa(path, x, conn, 4)
+    xr, cr = icx.read_mdpa(path, icx.QUAD4)
+    job = icx.Job.from_mesh(ctx, icx.QUAD4, xr, cr, icx.PLANE_STRAIN)
+    pr = O.Problem.from_mesh(4, xr, cr, O.PLANE_STRAIN)
+    for o in (job, pr):
+        o.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [-1e-9, 1e-9, 0.0, 3.0, 0, 0])
+    assert job.make_dof() == pr.make_dof()
+    for o in (job, pr):
+        o.add_neumann([0.0, 0.3, 0.0], [1.0, 2.0, -1e-9, 1e-9, 0, 0])
+    assert np.abs(job.fext() - pr.fext).max() <= 1e-15
+    rho = O.random_density(pr.nelx, 2, lo=0.1, hi=1.0)
+    job.set_design(rho)
+    b = pr.assemble(rho)
+    u, info = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=50000)
+    ud = pr.direct(b)
+    assert info["converged"] and np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-10
+
+
+def test_mesh_job_rejects_box_only_calls(ctx):
+    x, conn = box_mesh(4, (4, 2), (4.0, 2.0))
+    job = icx.Job.from_mesh(ctx, icx.QUAD4, x, conn, icx.PLANE_STRESS)
+    job.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, 2, 0, 0])
+    job.make_dof()
+    job.set_design(np.full(job.nelx, 0.5))
+    v = np.ones(job.numeq)
+    for call in (lambda: job.matvec(v, prec=1), lambda: job.solve(icx.SOLVE_MIXED),
+                 lambda: job.solve(icx.SOLVE_FUSED), lambda: job.helmholtz_setup(1.5),
+                 lambda: job.submit(tol=1e-8)):
+        with pytest.raises(icx.IcxError) as e:
+            call()
+        assert e.value.code == 1
+    bad = conn.reshape(-1, 4)[:, ::-1].reshape(-1).copy()  # clockwise: det J < 0
+    with pytest.raises(icx.IcxError) as e:
+        icx.Job.from_mesh(ctx, icx.QUAD4, x, bad, icx.PLANE_STRESS)
+    assert e.value.code == 6
