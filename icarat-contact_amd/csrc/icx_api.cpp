@@ -431,6 +431,9 @@ struct ConeWork {
   int n = 0;
   DBuf<int> off;
   DBuf<double> w, aa;
+  // mesh jobs: neighbour lists in CSR (ptr, id, weights r - d)
+  DBuf<int> ptr, id;
+  DBuf<double> wt;
 };
 struct MMAWork {
   bool ready = false;
@@ -2127,10 +2130,127 @@ icx_status icx_element_volume(const icx_job *j, double *vol) {
 // =====================================================================================
 namespace {
 
+// Filter::preNeighbour (src/optimization/filter.hpp:191-234) on an explicit mesh: the
+// neighbours of element i are the elements whose centres (the mean of their node
+// coordinates, summed as x / ne in node order) lie within the radius, in increasing id,
+// with weight r - d; candidates come from a cell grid of cell size r.  aa_i = sum of the
+// weights in that order (Filter::densityFilter's normalisation).
+static void mesh_neighbours(const icx_job *j, double radius, std::vector<int> &ptr, std::vector<int> &id,
+                            std::vector<double> &wt, std::vector<double> &aa) {
+  const int nq = j->el.nq, nd = j->g.ndim;
+  const long long ne = j->g.ne;
+  std::vector<double> cen(3 * ne, 0.0);
+  for (long long e = 0; e < ne; ++e)
+    for (int d = 0; d < nd; ++d) {
+      double v = 0.0;
+      for (int a = 0; a < nq; ++a) v += j->mx[3 * (size_t)j->mconn[e * nq + a] + d] / nq;
+      cen[3 * e + d] = v;
+    }
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < nd; ++d) {
+    lo[d] = hi[d] = cen[d];
+    for (long long e = 0; e < ne; ++e) {
+      lo[d] = std::min(lo[d], cen[3 * e + d]);
+      hi[d] = std::max(hi[d], cen[3 * e + d]);
+    }
+  }
+  int ncell[3] = {1, 1, 1};
+  for (int d = 0; d < nd; ++d) ncell[d] = std::max(1, std::min(1 << 20, (int)((hi[d] - lo[d]) / radius) + 1));
+  auto cell_of = [&](long long e, int *q) {
+    for (int d = 0; d < 3; ++d) {
+      q[d] = d < nd ? (int)((cen[3 * e + d] - lo[d]) / radius) : 0;
+      q[d] = std::max(0, std::min(ncell[d] - 1, q[d]));
+    }
+  };
+  const long long nc = (long long)ncell[0] * ncell[1] * ncell[2];
+  std::vector<int> cptr(nc + 1, 0), cid(ne);
+  std::vector<long long> ecell(ne);
+  for (long long e = 0; e < ne; ++e) {
+    int q[3];
+    cell_of(e, q);
+    ecell[e] = q[0] + (long long)ncell[0] * (q[1] + (long long)ncell[1] * q[2]);
+    ++cptr[ecell[e] + 1];
+  }
+  for (long long k = 0; k < nc; ++k) cptr[k + 1] += cptr[k];
+  {
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+    for (long long e = 0; e < ne; ++e) cid[fill[ecell[e]]++] = (int)e;  // increasing id per cell
+  }
+  const unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+  std::vector<std::vector<int>> rid(ne);
+  std::vector<std::vector<double>> rdist(ne);
+  auto work = [&](unsigned t) {
+    std::vector<std::pair<int, double>> cand;
+    for (long long i = t; i < ne; i += nt) {
+      int q[3];
+      cell_of(i, q);
+      cand.clear();
+      for (int dz = (nd == 3 ? -1 : 0); dz <= (nd == 3 ? 1 : 0); ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int x = q[0] + dx, y = q[1] + dy, z = q[2] + dz;
+            if (x < 0 || y < 0 || z < 0 || x >= ncell[0] || y >= ncell[1] || z >= ncell[2]) continue;
+            const long long k = x + (long long)ncell[0] * (y + (long long)ncell[1] * z);
+            for (int m = cptr[k]; m < cptr[k + 1]; ++m) {
+              const int jj = cid[m];
+              double s2 = 0.0;
+              for (int d = 0; d < nd; ++d) {
+                const double t2 = cen[3 * i + d] - cen[3 * (size_t)jj + d];
+                s2 += t2 * t2;
+              }
+              const double dist = std::sqrt(s2);
+              if (dist <= radius) cand.emplace_back(jj, dist);
+            }
+          }
+      std::sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+      for (const auto &pr : cand) {
+        rid[i].push_back(pr.first);
+        rdist[i].push_back(pr.second);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto &t : th) t.join();
+  ptr.assign(ne + 1, 0);
+  for (long long i = 0; i < ne; ++i) ptr[i + 1] = ptr[i] + (int)rid[i].size();
+  id.resize(ptr[ne]);
+  wt.resize(ptr[ne]);
+  aa.assign(ne, 0.0);
+  for (long long i = 0; i < ne; ++i) {
+    double a = 0.0;
+    for (size_t k = 0; k < rid[i].size(); ++k) {
+      id[ptr[i] + k] = rid[i][k];
+      wt[ptr[i] + k] = radius - rdist[i][k];
+      a += radius - rdist[i][k];
+    }
+    aa[i] = a;
+  }
+}
+
 icx_status cone_setup(icx_job *j, double radius) {
   icx_ctx *c = j->ctx;
   ConeWork &w = j->cw;
   if (w.ready && w.radius == radius) return ICX_OK;
+  if (j->mesh) {
+    std::vector<int> ptr, id;
+    std::vector<double> wt, aa;
+    mesh_neighbours(j, radius, ptr, id, wt, aa);
+    for (double a : aa)
+      if (a < 1.0e-10) return fail(c, ICX_ERR_NUMERIC, "cone filter: an element has no neighbour weight (radius too small)");
+    HIPCHK(c, w.ptr.alloc(ptr.size())); HIPCHK(c, w.id.alloc(std::max<size_t>(id.size(), 1)));
+    HIPCHK(c, w.wt.alloc(std::max<size_t>(wt.size(), 1))); HIPCHK(c, w.aa.alloc(aa.size()));
+    HIPCHK(c, hipMemcpyAsync(w.ptr.p, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, S(j)));
+    HIPCHK(c, hipMemcpyAsync(w.id.p, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice, S(j)));
+    HIPCHK(c, hipMemcpyAsync(w.wt.p, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, S(j)));
+    HIPCHK(c, hipMemcpyAsync(w.aa.p, aa.data(), aa.size() * sizeof(double), hipMemcpyHostToDevice, S(j)));
+    HIPCHK(c, hipStreamSynchronize(S(j)));
+    w.n = (int)id.size();
+    w.ready = true;
+    w.radius = radius;
+    return ICX_OK;
+  }
   const Grid &g = j->g;
   const double *h = j->el.h;
   const int R[3] = {(int)std::ceil(radius / h[0]) + 1, (int)std::ceil(radius / h[1]) + 1,
@@ -2164,6 +2284,14 @@ icx_status cone_setup(icx_job *j, double radius) {
 }
 
 Stencil stencil(icx_job *j) { return Stencil{j->cw.n, j->cw.off.p, j->cw.w.p}; }
+ConeCsr cone_csr(icx_job *j) { return ConeCsr{j->cw.ptr.p, j->cw.id.p, j->cw.wt.p}; }
+
+// forward (adjoint = 0) or adjoint cone filter of device vectors
+void cone_apply(icx_job *j, int adjoint, const double *in, double *out) {
+  if (j->mesh) launch_cone_csr(j->g.ne, cone_csr(j), j->cw.aa.p, in, out, adjoint, S(j));
+  else if (adjoint) launch_cone_adjoint(j->g, stencil(j), j->cw.aa.p, in, out, S(j));
+  else launch_cone_forward(j->g, stencil(j), j->cw.aa.p, in, out, S(j));
+}
 
 icx_status helm_setup(icx_job *j, double radius) {
   icx_ctx *c = j->ctx;
@@ -2206,7 +2334,6 @@ icx_status helm_apply_dev(icx_job *j, int sens, double tol, const double *din, d
 extern "C" {
 
 icx_status icx_cone_filter_setup(icx_job *j, double radius) {
-  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the cone filter");
   if (!j || !(radius > 0.0)) return ICX_ERR_ARG;
   return cone_setup(j, radius);
 }
@@ -2219,8 +2346,7 @@ icx_status icx_cone_filter(icx_job *j, int adjoint, const double *in, double *ou
   DBuf<double> a, b;
   HIPCHK(c, a.alloc(ne)); HIPCHK(c, b.alloc(ne));
   HIPCHK(c, hipMemcpyAsync(a.p, in, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
-  if (adjoint) launch_cone_adjoint(j->g, stencil(j), j->cw.aa.p, a.p, b.p, S(j));
-  else launch_cone_forward(j->g, stencil(j), j->cw.aa.p, a.p, b.p, S(j));
+  cone_apply(j, adjoint, a.p, b.p);
   HIPCHK(c, hipMemcpyAsync(out, b.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipStreamSynchronize(S(j)));
   return ICX_OK;
@@ -2275,7 +2401,10 @@ icx_status oc_update_dev(icx_job *j, double *dx, const double *dfdx, double g0, 
   const int project = beta > 0.0;
   while ((l1 - l0) / (l1 + l0) > 1.0e-5 && 100 > counter) {
     lambda = 0.5 * (l1 + l0);
-    if (filter == ICX_FILTER_CONE) {
+    if (filter == ICX_FILTER_CONE && j->mesh) {
+      launch_oc_volume_csr(ne, cone_csr(j), j->cw.aa.p, lambda, move, xmin, xmax, dx, dfdx, dgdx, project, beta, T,
+                           j->d_vol.p, j->d_scalars.p, j->red(), S(j));
+    } else if (filter == ICX_FILTER_CONE) {
       launch_oc_volume_cone(j->g, stencil(j), j->cw.aa.p, lambda, move, xmin, xmax, dx, dfdx, dgdx, project,
                             beta, T, j->volume, j->d_scalars.p, j->red(), S(j));
     } else {
@@ -2286,7 +2415,8 @@ icx_status oc_update_dev(icx_job *j, double *dx, const double *dfdx, double g0, 
         if (rc) return rc;
         src = dtmp;
       }
-      launch_volume(ne, project, beta, T, src, j->volume, j->d_scalars.p, j->red(), S(j));
+      launch_volume(ne, project, beta, T, src, j->volume, j->mesh ? j->d_vol.p : nullptr, j->d_scalars.p, j->red(),
+                    S(j));
     }
     double gk;
     icx_status rc = read_scalars(j, 1, &gk);
@@ -2400,7 +2530,6 @@ extern "C" {
 
 icx_status icx_oc_update(icx_job *j, double *x, const double *dfdx, double g, const double *dgdx, double xmin,
                          double xmax, double move, int filter, double beta, double T, int *bisections) {
-  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the OC update");
   if (!j || !x || !dfdx || !dgdx) return ICX_ERR_ARG;
   if (filter == ICX_FILTER_CONE && !j->cw.ready) return fail(j->ctx, ICX_ERR_STATE, "cone filter not set up");
   if (filter == ICX_FILTER_HELMHOLTZ && !j->hw.ready) return fail(j->ctx, ICX_ERR_STATE, "Helmholtz filter not set up");
@@ -2444,10 +2573,11 @@ icx_status icx_mma_update(icx_job *j, double *x, const double *dfdx, double gx, 
 
 // ---- device-resident design loop ------------------------------------------------------------
 icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0) {
-  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the device design loop");
   if (!j || !par || !x0) return ICX_ERR_ARG;
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
+  if (j->mesh && (par->filter == ICX_FILTER_HELMHOLTZ || par->form != ICX_SOLVE_PLAIN))
+    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: the design loop takes the cone filter and ICX_SOLVE_PLAIN");
   icx_ctx *c = j->ctx;
   OptWork &o = j->opt;
   o.par = *par;
@@ -2485,7 +2615,7 @@ icx_status icx_opt_step_begin(icx_job *j) {
   const long long ne = j->g.ne;
   const int k = ++o.step;
   // filter
-  if (P.filter == ICX_FILTER_CONE) launch_cone_forward(j->g, stencil(j), j->cw.aa.p, o.x.p, o.fil.p, S(j));
+  if (P.filter == ICX_FILTER_CONE) cone_apply(j, 0, o.x.p, o.fil.p);
   else if (P.filter == ICX_FILTER_HELMHOLTZ) {
     icx_status rc = helm_apply_dev(j, 0, P.helm_tol, o.x.p, o.fil.p, nullptr);
     if (rc) return rc;
@@ -2517,7 +2647,8 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
   j->u_valid = true;
   // objective and volume
   launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
-  launch_volume(ne, 0, 0.0, 0.5, o.thr.p, j->volume, j->d_scalars.p + 1, j->red(), S(j));
+  launch_volume(ne, 0, 0.0, 0.5, o.thr.p, j->volume, j->mesh ? j->d_vol.p : nullptr, j->d_scalars.p + 1, j->red(),
+                S(j));
   double sc[2];
   icx_status rc = read_scalars(j, 2, sc);
   if (rc) return rc;
@@ -2535,8 +2666,12 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
     icx_status rs = sens_displacement(j, &usens);
     if (rs) return rs;
   }
-  launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, usens,
-                     j->volume, P.driver == 0 ? 1.0 : o.V0, o.dfds.p, o.dgds.p, S(j));
+  if (j->mesh)
+    launch_mesh_sensitivity(j->mop, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, usens,
+                            j->d_vol.p, P.driver == 0 ? 1.0 : o.V0, o.dfds.p, o.dgds.p, S(j));
+  else
+    launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, usens,
+                       j->volume, P.driver == 0 ? 1.0 : o.V0, o.dfds.p, o.dgds.p, S(j));
   // dfds *= dthr, dgds *= dthr (projection derivative; 1 without projection)
   if (P.projection) {
     launch_mul(ne, o.dthr.p, o.dfds.p, S(j));
@@ -2544,8 +2679,8 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
   }
   // filter adjoint
   if (P.filter == ICX_FILTER_CONE) {
-    launch_cone_adjoint(j->g, stencil(j), j->cw.aa.p, o.dfds.p, o.tmp.p, S(j));
-    launch_cone_adjoint(j->g, stencil(j), j->cw.aa.p, o.dgds.p, o.tmp2.p, S(j));
+    cone_apply(j, 1, o.dfds.p, o.tmp.p);
+    cone_apply(j, 1, o.dgds.p, o.tmp2.p);
   } else if (P.filter == ICX_FILTER_HELMHOLTZ) {
     rc = helm_apply_dev(j, 1, P.helm_tol, o.dfds.p, o.tmp.p, nullptr);
     if (rc) return rc;

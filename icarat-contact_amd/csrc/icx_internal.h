@@ -417,8 +417,19 @@ void launch_oc_candidate(long long n, double lambda, double move, double xmin, d
                          const double *x, const double *dfdx, const double *dgdx, double *xnew,
                          hipStream_t s);
 // sum_i proj(rho_i) * vol -> out[0]
-void launch_volume(long long n, int project, double beta, double T, const double *rho,
-                   double vol, double *out, Reduce red, hipStream_t s);
+// sum_e (projected) rho_e v_e with v_e = volv[e] when volv is given, else vol
+void launch_volume(long long n, int project, double beta, double T, const double *rho, double vol,
+                   const double *volv, double *out, Reduce red, hipStream_t s);
+// cone filter over explicit neighbour lists (CSR rows in increasing neighbour id, w = r - d)
+struct ConeCsr {
+  const int *ptr, *id;
+  const double *w;
+};
+void launch_cone_csr(long long n, ConeCsr f, const double *aa, const double *in, double *out, int adjoint,
+                     hipStream_t s);
+void launch_oc_volume_csr(long long n, ConeCsr f, const double *aa, double lambda, double move, double xmin,
+                          double xmax, const double *x, const double *dfdx, const double *dgdx, int project,
+                          double beta, double T, const double *vol, double *out, Reduce red, hipStream_t s);
 void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double lambda, double move,
                            double xmin, double xmax, const double *x, const double *dfdx,
                            const double *dgdx, int project, double beta, double T, double vol,

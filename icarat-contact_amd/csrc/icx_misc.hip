@@ -589,20 +589,40 @@ void launch_oc_candidate(long long n, double lambda, double move, double xmin, d
 }
 
 __global__ void __launch_bounds__(256) k_volume(long long n, int project, double beta, double T,
-                                                const double *rho, double vol, Reduce red) {
+                                                const double *rho, double vol, const double *volv, Reduce red) {
   __shared__ double scr[4];
   double s = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    s += (project ? proj(rho[i], beta, T) : rho[i]) * vol;
+    s += (project ? proj(rho[i], beta, T) : rho[i]) * (volv ? volv[i] : vol);
   s = block_sum<256>(s, scr);
   if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
 }
 void launch_volume(long long n, int project, double beta, double T, const double *rho, double vol,
-                   double *out, Reduce red, hipStream_t s) {
+                   const double *volv, double *out, Reduce red, hipStream_t s) {
   const int nb = ew_blocks(n, 4, RED_BLOCKS);
-  hipLaunchKernelGGL(k_volume, dim3(nb), dim3(256), 0, s, n, project, beta, T, rho, vol, red);
+  hipLaunchKernelGGL(k_volume, dim3(nb), dim3(256), 0, s, n, project, beta, T, rho, vol, volv, red);
   launch_finalize(red, nb, 1, out, s);
+}
+
+// Cone filter on explicit neighbour lists (unstructured meshes): the same sums as the
+// stencil kernels, over the CSR rows (neighbours in increasing element id)
+__global__ void k_cone_csr(long long n, ConeCsr f, const double *aa, const double *in, double *out, int adjoint) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int k = f.ptr[e]; k < f.ptr[e + 1]; ++k) {
+      const int j = f.id[k];
+      if (adjoint) s += f.w[k] * in[j] / aa[j];
+      else s += f.w[k] * in[j];
+    }
+    out[e] = adjoint ? s : s / aa[e];
+  }
+}
+void launch_cone_csr(long long n, ConeCsr f, const double *aa, const double *in, double *out, int adjoint,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_cone_csr, dim3(ew_blocks(n)), dim3(256), 0, s, n, f, aa, in, out, adjoint);
+  ICX_CHECK_LAUNCH();
 }
 
 // one bisection step of the OC volume lambda with the cone filter, fused:
@@ -627,6 +647,35 @@ __global__ void __launch_bounds__(256) k_oc_volume_cone(Grid g, Stencil st, cons
   s = block_sum<256>(s, scr);
   if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
 }
+// the fused OC bisection step on explicit neighbour lists, per-element volumes
+__global__ void __launch_bounds__(256) k_oc_volume_csr(long long n, ConeCsr f, const double *aa, double lambda,
+                                                       double move, double xmin, double xmax, const double *x,
+                                                       const double *dfdx, const double *dgdx, int project,
+                                                       double beta, double T, const double *vol, Reduce red) {
+  __shared__ double scr[4];
+  double s = 0.0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    double bb = 0.0;
+    for (int k = f.ptr[e]; k < f.ptr[e + 1]; ++k) {
+      const int j = f.id[k];
+      bb += f.w[k] * oc_x(lambda, move, xmin, xmax, x[j], dfdx[j], dgdx[j]);
+    }
+    const double v = bb / aa[e];
+    s += (project ? proj(v, beta, T) : v) * vol[e];
+  }
+  s = block_sum<256>(s, scr);
+  if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
+}
+void launch_oc_volume_csr(long long n, ConeCsr f, const double *aa, double lambda, double move, double xmin,
+                          double xmax, const double *x, const double *dfdx, const double *dgdx, int project,
+                          double beta, double T, const double *vol, double *out, Reduce red, hipStream_t s) {
+  const int nb = ew_blocks(n, 1, RED_BLOCKS);
+  hipLaunchKernelGGL(k_oc_volume_csr, dim3(nb), dim3(256), 0, s, n, f, aa, lambda, move, xmin, xmax, x, dfdx,
+                     dgdx, project, beta, T, vol, red);
+  launch_finalize(red, nb, 1, out, s);
+}
+
 void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double lambda, double move,
                            double xmin, double xmax, const double *x, const double *dfdx,
                            const double *dgdx, int project, double beta, double T, double vol,

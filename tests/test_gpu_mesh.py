@@ -127,7 +127,9 @@ def test_mesh_job_rejects_box_only_calls(ctx):
     v = np.ones(job.numeq)
     for call in (lambda: job.matvec(v, prec=1), lambda: job.solve(icx.SOLVE_MIXED),
                  lambda: job.solve(icx.SOLVE_FUSED), lambda: job.helmholtz_setup(1.5),
-                 lambda: job.submit(tol=1e-8)):
+                 lambda: job.submit(tol=1e-8), lambda: job.mises(),
+                 lambda: job.opt_init(np.full(job.nelx, 0.5), filter=icx.FILTER_HELMHOLTZ),
+                 lambda: job.opt_init(np.full(job.nelx, 0.5), form=icx.SOLVE_MIXED)):
         with pytest.raises(icx.IcxError) as e:
             call()
         assert e.value.code == 1
@@ -135,3 +137,47 @@ def test_mesh_job_rejects_box_only_calls(ctx):
     with pytest.raises(icx.IcxError) as e:
         icx.Job.from_mesh(ctx, icx.QUAD4, x, bad, icx.PLANE_STRESS)
     assert e.value.code == 6
+
+
+@pytest.mark.parametrize("ne,state,div,L,r", [(4, O.PLANE_STRESS, (14, 7), (14.0, 7.0), 1.5),
+                                              (4, O.PLANE_STRAIN, (10, 6), (10.0, 6.0), 3.0),
+                                              (8, O.SOLID3D, (6, 4, 4), (6.0, 4.0, 4.0), 1.6)])
+def test_mesh_cone_filter_matches_brute_force(ctx, ne, state, div, L, r):
+    """Neighbours by centre distance on a distorted mesh (cell grid on the device side,
+    the O(n^2) scan of Filter::preNeighbour on the oracle side): forward and adjoint."""
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 6, L)
+    job = icx.Job.from_mesh(ctx, ne, xd, conn, state)
+    f = O.Problem.from_mesh(ne, xd, conn, state).cone_filter(r, brute_force=True)
+    job.cone_setup(r)
+    v = np.random.default_rng(4).uniform(0.0, 1.0, job.nelx)
+    fwd = f.apply(v)
+    assert np.abs(job.cone(v) - fwd).max() <= 1e-14
+    s = np.random.default_rng(5).standard_normal(job.nelx)
+    assert np.abs(job.cone(s, adjoint=True) - f.apply(s, sens=True)).max() <= 1e-14 * np.abs(s).max()
+
+
+def test_mesh_driver_a_matches_oracle(ctx):
+    """Driver A (cone filter, projection, OC) on a distorted quad4 mesh: the device design
+    loop against the oracle's driver loop, step by step."""
+    ne, div, L, steps = 4, (30, 10), (30.0, 10.0), 10
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 8, L)
+    job = icx.Job.from_mesh(ctx, ne, xd, conn, icx.PLANE_STRESS)
+    pr = O.Problem.from_mesh(ne, xd, conn, O.PLANE_STRESS)
+    for o in (job, pr):
+        o.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, L[1], 0, 0])
+    assert job.make_dof() == pr.make_dof()
+    for o in (job, pr):
+        o.add_neumann([0.0, -0.05, 0.0], [L[0], L[0], 0.0, 2.0, 0, 0])
+    x0 = np.full(pr.nelx, 0.5)
+    ref = pr.run_driver(0, x0, steps, 1.5, projection=True, direct=True)
+    job.opt_init(x0, driver=0, filter=icx.FILTER_CONE, radius=1.5, form=icx.SOLVE_PLAIN,
+                 tol=1e-13, maxit=50000)
+    recs = [job.opt_step() for _ in range(steps)]
+    obj = np.array([r["objective"] for r in recs])
+    rel = np.abs(obj - ref["obj"]) / ref["obj"]
+    assert rel.max() < 1e-8, rel
+    vol = np.array([r["volume"] for r in recs])
+    assert np.abs(vol - ref["vol"]).max() <= 1e-8 * np.abs(ref["vol"]).max()
+    assert np.abs(job.opt_design() - ref["design"]).max() < 1e-7
