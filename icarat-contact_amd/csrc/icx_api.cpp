@@ -965,12 +965,13 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   return job_finish(ctx, j, out);
 }
 
-// Element matrices of a mesh job, in parallel over host threads: Ke0 entry-major
-// ([k][e]) and the volumes; false with the first bad element in `bad`.
+// Element matrices of a mesh job, in parallel over host threads: the upper triangle of
+// Ke0, row-major packed, entry-major ([p][e]), and the volumes; false with the first bad
+// element in `bad`.
 static bool mesh_elements(const icx_job *j, std::vector<double> &ke, std::vector<double> &vol, long long &bad) {
   const int nq = j->el.nq, ndim = j->g.ndim, nd = nq * ndim;
   const long long ne = j->g.ne;
-  ke.assign((size_t)nd * nd * ne, 0.0);
+  ke.assign((size_t)nd * (nd + 1) / 2 * ne, 0.0);
   vol.assign(ne, 0.0);
   const unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
   std::vector<long long> first_bad(nt, -1);
@@ -986,7 +987,9 @@ static bool mesh_elements(const icx_job *j, std::vector<double> &ke, std::vector
         continue;
       }
       vol[e] = v;
-      for (int k = 0; k < nd * nd; ++k) ke[(size_t)k * ne + e] = K[k];
+      size_t p = 0;
+      for (int r = 0; r < nd; ++r)
+        for (int c = r; c < nd; ++c, ++p) ke[p * ne + e] = K[r * nd + c];
     }
   };
   std::vector<std::thread> th;

@@ -195,9 +195,10 @@ int elastic_fusedA_blocks(const Grid &g);
 
 // ---- unstructured-mesh elastic operator (icx_mesh.hip) -------------------------------
 // Mesh of one element type (ne = 4 quad4 / 8 hexa8, nc = 2 / 3 components): connectivity
-// node-of-element-major ([a][e]), unit-modulus element matrices entry-major ([k][e],
-// k = i * nd + j, nd = ne * nc, node-major local DOFs a * nc + c), node adjacency in CSR
-// (entries e * ne + a, increasing e), and a scratch of nd doubles per element.
+// node-of-element-major ([a][e]), unit-modulus element matrices as their upper triangles,
+// entry-major ([p][e], p = i nd - i (i - 1) / 2 + j - i for i <= j, nd = ne * nc,
+// node-major local DOFs a * nc + c), node adjacency in CSR (entries e * ne + a,
+// increasing e), and a scratch of nd doubles per element.
 struct MeshOp {
   int ne, nc;
   long long nn, nelx;

@@ -3,8 +3,9 @@
 // reference stores Ke per element as well, src/problem/elasticity.cpp:210-254).
 //
 // y = K(E) x in two passes, deterministic (no atomics):
-//   element pass: y_e = E_e Ke0_e x_e, one thread per element; Ke0 is stored entry-major
-//     ([k][e]), so the threads of a wave stream consecutive addresses; y_e is written
+//   element pass: y_e = E_e Ke0_e x_e, one thread per element; Ke0 is symmetric and its
+//     upper triangle is stored entry-major ([p][e], p the row-major packed index of
+//     i <= j), so the threads of a wave stream consecutive addresses; y_e is written
 //     element-major ([e][i]) for the node pass;
 //   node pass: y_n = sum over the (element, local node) pairs of node n, in increasing
 //     element order (node adjacency in CSR), zero on constrained components.
@@ -14,6 +15,28 @@
 namespace icx {
 
 namespace {
+
+// packed index of (i, j), i <= j, in the row-major upper triangle of an nd x nd matrix
+__host__ __device__ constexpr int tri(int nd, int i, int j) { return i * nd - i * (i - 1) / 2 + (j - i); }
+
+// y = Ke0_e x over the packed upper triangle (each off-diagonal entry used twice)
+template <int ND>
+__device__ __forceinline__ void sym_apply(const double *ke, long long ne, long long e, const double (&x)[ND],
+                                          double (&y)[ND]) {
+#pragma unroll
+  for (int i = 0; i < ND; ++i) y[i] = 0.0;
+  // running pointer over the packed entries (one stride register, not one offset per entry)
+  const double *kp = ke + e;
+#pragma unroll
+  for (int i = 0; i < ND; ++i)
+#pragma unroll
+    for (int j = i; j < ND; ++j) {
+      const double k = *kp;
+      kp += ne;
+      y[i] = fma_t(k, x[j], y[i]);
+      if (j != i) y[j] = fma_t(k, x[i], y[j]);
+    }
+}
 
 template <int NE, int NC>
 __global__ void __launch_bounds__(256) k_mesh_element(MeshOp m, const double *E, const double *x, double *ye,
@@ -29,16 +52,12 @@ __global__ void __launch_bounds__(256) k_mesh_element(MeshOp m, const double *E,
 #pragma unroll
       for (int c = 0; c < NC; ++c) xe[a * NC + c] = x[c * m.nn + n];
     }
+    double y[ND];
+    sym_apply<ND>(m.ke, ne, e, xe, y);
     const double Ee = E[e];
     double *out = ye + e * ND;
-#pragma unroll 4
-    for (int i = 0; i < ND; ++i) {
-      const double *row = m.ke + (long long)i * ND * ne + e;
-      double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < ND; ++k) acc = fma_t(row[(long long)k * ne], xe[k], acc);
-      out[i] = Ee * acc;
-    }
+    for (int i = 0; i < ND; ++i) out[i] = Ee * y[i];
   }
 }
 
@@ -81,7 +100,7 @@ __global__ void __launch_bounds__(256) k_mesh_diag(MeshOp m, const double *E, co
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
         const int i = a * NC + c;
-        acc[c] += E[e] * m.ke[((long long)i * ND + i) * ne + e];
+        acc[c] += E[e] * m.ke[(long long)tri(ND, i, i) * ne + e];
       }
     }
     const uint32_t msk = mask ? mask[n] : 0u;
@@ -102,6 +121,16 @@ __global__ void __launch_bounds__(256) k_mesh_energy(MeshOp m, double y1, double
   constexpr int ND = NE * NC;
   const long long ne = m.nelx;
   for (long long e = blockIdx.x * 256LL + threadIdx.x; e < ne; e += (long long)gridDim.x * 256) {
+    // the SIMP derivative first: pow's registers are free again before the element loads
+    double scale;
+    if (SENS) {
+      const double dens = design[e];
+      if (y2 >= y1) scale = -(p * pow(dens, p - 1.0) * (y2 - y1));
+      else scale = -((y2 - y1) * p * pow(1.0 - dens, p - 1.0));
+      if (dgds) dgds[e] = vol[e] / vscale;
+    } else {
+      scale = E[e];
+    }
     double ue[ND];
 #pragma unroll
     for (int a = 0; a < NE; ++a) {
@@ -109,25 +138,12 @@ __global__ void __launch_bounds__(256) k_mesh_energy(MeshOp m, double y1, double
 #pragma unroll
       for (int c = 0; c < NC; ++c) ue[a * NC + c] = flush15(u[c * m.nn + n]);
     }
+    double y[ND];
+    sym_apply<ND>(m.ke, ne, e, ue, y);
     double en = 0.0;
-#pragma unroll 4
-    for (int i = 0; i < ND; ++i) {
-      const double *row = m.ke + (long long)i * ND * ne + e;
-      double acc = 0.0;
 #pragma unroll
-      for (int k = 0; k < ND; ++k) acc = fma_t(row[(long long)k * ne], ue[k], acc);
-      en = fma_t(ue[i], acc, en);
-    }
-    if (SENS) {
-      const double dens = design[e];
-      double dyoung;
-      if (y2 >= y1) dyoung = p * pow(dens, p - 1.0) * (y2 - y1);
-      else dyoung = (y2 - y1) * p * pow(1.0 - dens, p - 1.0);
-      if (dfds) dfds[e] = -dyoung * en;
-      if (dgds) dgds[e] = vol[e] / vscale;
-    } else {
-      dfds[e] = E[e] * en;
-    }
+    for (int i = 0; i < ND; ++i) en = fma_t(ue[i], y[i], en);
+    if (dfds) dfds[e] = scale * en;
   }
 }
 
