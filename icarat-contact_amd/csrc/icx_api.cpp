@@ -465,98 +465,33 @@ struct OptWork {
   DBuf<double> x, fil, thr, dthr, dfds, dgds, tmp, tmp2;
 };
 
-// ---- element stiffness of a general quad4 / hexa8 -------------------------------------
-// LinearElastic::makeKeFint (src/problem/elasticity.cpp:210-254) on an element of any
-// shape: Ke = sum_ip B^T D B det J at the 2 x 2 (x 2) Gauss points +-1/sqrt(3), weight 1,
-// local nodes in the reference's order (src/problem/shape_function.cpp:97-119, 403-447),
-// engineering shears ordered xy, yz, zx (src/problem/bmatrix.hpp:62-69, 124-140).
-// Returns false when det J <= 0 at a Gauss point (inverted or degenerate element).
-static bool general_Ke(int nq, int ndim, int state, double nu, const double *X, double *Ke, double &vol) {
+// ---- volume of a general quad4 / hexa8 ----------------------------------------------
+// sum_ip det J at the 2 x 2 (x 2) Gauss points +-1/sqrt(3), weight 1 (the volume
+// LinearElastic::makeKeFint accumulates, src/problem/elasticity.cpp:249), local nodes in
+// the reference's order (src/problem/shape_function.cpp:97-119, 403-447).  Returns false
+// when det J <= 0 at a Gauss point (inverted or degenerate element).
+static bool general_volume(int nq, int ndim, const double *X, double &vol) {
   static const int corner[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                                    {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
-  const int nd = nq * ndim;
-  double D[36];
-  int voigt;
-  material_matrix(state, 1.0, nu, D, voigt);
-  std::fill(Ke, Ke + nd * nd, 0.0);
-  vol = 0.0;
   const double g = 1.0 / std::sqrt(3.0);
-  double B[6 * 24], DB[6 * 24];
+  vol = 0.0;
   for (int ip = 0; ip < nq; ++ip) {
     double r[3] = {0, 0, 0};
     for (int d = 0; d < ndim; ++d) r[d] = corner[ip][d] * g;
-    // dN/dr in natural coordinates
-    double dr[3][8];
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     const double sc = ndim == 3 ? 0.125 : 0.25;
     for (int a = 0; a < nq; ++a)
       for (int d = 0; d < ndim; ++d) {
         double v = sc * corner[a][d];
         for (int e = 0; e < ndim; ++e)
           if (e != d) v *= 1.0 + corner[a][e] * r[e];
-        dr[d][a] = v;
+        for (int k = 0; k < ndim; ++k) J[d][k] += v * X[3 * a + k];
       }
-    // J[d][e] = dx_e / dr_d
-    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    for (int d = 0; d < ndim; ++d)
-      for (int e = 0; e < ndim; ++e)
-        for (int a = 0; a < nq; ++a) J[d][e] += dr[d][a] * X[3 * a + e];
-    double det, inv[3][3];
-    if (ndim == 2) {
-      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      inv[0][0] = J[1][1] / det; inv[0][1] = -J[0][1] / det;
-      inv[1][0] = -J[1][0] / det; inv[1][1] = J[0][0] / det;
-    } else {
-      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-      inv[0][0] = c00 / det;
-      inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
-      inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
-      inv[1][0] = c01 / det;
-      inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
-      inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
-      inv[2][0] = c02 / det;
-      inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
-      inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
-    }
+    const double det = ndim == 2 ? J[0][0] * J[1][1] - J[0][1] * J[1][0]
+                                 : J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                                       J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
     if (!(det > 0.0)) return false;
-    // dN/dX_e = sum_d inv(J)[e][d] dN/dr_d
-    double dX[3][8];
-    for (int e = 0; e < ndim; ++e)
-      for (int a = 0; a < nq; ++a) {
-        double v = 0.0;
-        for (int d = 0; d < ndim; ++d) v += inv[e][d] * dr[d][a];
-        dX[e][a] = v;
-      }
-    std::fill(B, B + voigt * nd, 0.0);
-    for (int a = 0; a < nq; ++a) {
-      if (ndim == 2) {
-        B[0 * nd + 2 * a] = dX[0][a];
-        B[1 * nd + 2 * a + 1] = dX[1][a];
-        B[2 * nd + 2 * a] = dX[1][a];
-        B[2 * nd + 2 * a + 1] = dX[0][a];
-      } else {
-        B[0 * nd + 3 * a] = dX[0][a];
-        B[1 * nd + 3 * a + 1] = dX[1][a];
-        B[2 * nd + 3 * a + 2] = dX[2][a];
-        B[3 * nd + 3 * a] = dX[1][a]; B[3 * nd + 3 * a + 1] = dX[0][a];
-        B[4 * nd + 3 * a + 1] = dX[2][a]; B[4 * nd + 3 * a + 2] = dX[1][a];
-        B[5 * nd + 3 * a] = dX[2][a]; B[5 * nd + 3 * a + 2] = dX[0][a];
-      }
-    }
-    for (int v = 0; v < voigt; ++v)
-      for (int i = 0; i < nd; ++i) {
-        double t = 0.0;
-        for (int w = 0; w < voigt; ++w) t += D[v * voigt + w] * B[w * nd + i];
-        DB[v * nd + i] = t;
-      }
-    for (int i = 0; i < nd; ++i)
-      for (int k = 0; k < nd; ++k) {
-        double t = 0.0;
-        for (int v = 0; v < voigt; ++v) t += B[v * nd + i] * DB[v * nd + k];
-        Ke[i * nd + k] += t * det;
-      }
     vol += det;
   }
   return true;
@@ -603,7 +538,7 @@ struct icx_job {
   std::vector<double> mx, mvol;
   std::vector<int> mconn;
   DBuf<int> d_conn, d_adj_ptr, d_adj;
-  DBuf<double> d_ke, d_ye, d_vol;
+  DBuf<double> d_xyz, d_ye, d_vol;
   MeshOp mop{};
   // state of a submitted (batched) mixed solve
   struct Submitted {
@@ -965,31 +900,25 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
   return job_finish(ctx, j, out);
 }
 
-// Element matrices of a mesh job, in parallel over host threads: the upper triangle of
-// Ke0, row-major packed, entry-major ([p][e]), and the volumes; false with the first bad
-// element in `bad`.
-static bool mesh_elements(const icx_job *j, std::vector<double> &ke, std::vector<double> &vol, long long &bad) {
-  const int nq = j->el.nq, ndim = j->g.ndim, nd = nq * ndim;
+// Element volumes of a mesh job, in parallel over host threads; false with the first
+// element whose Jacobian determinant is not positive in `bad`.
+static bool mesh_volumes(const icx_job *j, std::vector<double> &vol, long long &bad) {
+  const int nq = j->el.nq, ndim = j->g.ndim;
   const long long ne = j->g.ne;
-  ke.assign((size_t)nd * (nd + 1) / 2 * ne, 0.0);
   vol.assign(ne, 0.0);
   const unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
   std::vector<long long> first_bad(nt, -1);
   auto work = [&](unsigned t) {
-    std::vector<double> K(nd * nd);
     double X[24];
     for (long long e = t; e < ne; e += nt) {
       for (int a = 0; a < nq; ++a)
         for (int d = 0; d < 3; ++d) X[3 * a + d] = j->mx[3 * (size_t)j->mconn[e * nq + a] + d];
       double v;
-      if (!general_Ke(nq, ndim, j->prob.state, j->prob.poisson, X, K.data(), v)) {
+      if (!general_volume(nq, ndim, X, v)) {
         if (first_bad[t] < 0) first_bad[t] = e;
         continue;
       }
       vol[e] = v;
-      size_t p = 0;
-      for (int r = 0; r < nd; ++r)
-        for (int c = r; c < nd; ++c, ++p) ke[p * ne + e] = K[r * nd + c];
     }
   };
   std::vector<std::thread> th;
@@ -1038,9 +967,8 @@ icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
   j->el.nq = nq;
   j->mx.assign(m->x, m->x + 3 * (size_t)m->numnp);
   j->mconn.assign(m->conn, m->conn + (size_t)m->nelx * nq);
-  std::vector<double> ke;
   long long bad;
-  if (!mesh_elements(j, ke, j->mvol, bad)) {
+  if (!mesh_volumes(j, j->mvol, bad)) {
     delete j;
     return fail(ctx, ICX_ERR_NUMERIC, "element %lld has a non-positive Jacobian determinant", bad);
   }
@@ -1061,21 +989,28 @@ icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
   (void)hipSetDevice(ctx->device);
   const int nd = nq * ndim;
   hipStream_t s = ctx->stream;
+  std::vector<double> xyz((size_t)ndim * nn);
+  for (long long n = 0; n < nn; ++n)
+    for (int d = 0; d < ndim; ++d) xyz[(size_t)d * nn + n] = j->mx[3 * n + d];
   if (j->d_conn.alloc(conn_t.size()) || j->d_adj_ptr.alloc(ptr.size()) || j->d_adj.alloc(adj.size()) ||
-      j->d_ke.alloc(ke.size()) || j->d_ye.alloc((size_t)ne * nd) || j->d_vol.alloc(ne)) {
+      j->d_xyz.alloc(xyz.size()) || j->d_ye.alloc((size_t)ne * nd) || j->d_vol.alloc(ne)) {
     delete j;
     return fail(ctx, ICX_ERR_NOMEM, "device allocation failed");
   }
   if (hipMemcpyAsync(j->d_conn.p, conn_t.data(), conn_t.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
       hipMemcpyAsync(j->d_adj_ptr.p, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
       hipMemcpyAsync(j->d_adj.p, adj.data(), adj.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
-      hipMemcpyAsync(j->d_ke.p, ke.data(), ke.size() * sizeof(double), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(j->d_xyz.p, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice, s) ||
       hipMemcpyAsync(j->d_vol.p, j->mvol.data(), ne * sizeof(double), hipMemcpyHostToDevice, s) ||
       hipStreamSynchronize(s)) {
     delete j;
     return fail(ctx, ICX_ERR_HIP, "upload of the mesh failed");
   }
-  j->mop = MeshOp{nq, ndim, nn, ne, j->d_conn.p, j->d_ke.p, j->d_adj_ptr.p, j->d_adj.p, j->d_ye.p};
+  double D[36];
+  int voigt;
+  material_matrix(p.state, 1.0, p.poisson, D, voigt);
+  j->mop = MeshOp{nq, ndim, nn, ne, j->d_conn.p, j->d_xyz.p, D[0], D[1], D[(voigt - 1) * voigt + voigt - 1],
+                  j->d_adj_ptr.p, j->d_adj.p, j->d_ye.p};
   return job_finish(ctx, j, out);
 }
 

@@ -195,15 +195,16 @@ int elastic_fusedA_blocks(const Grid &g);
 
 // ---- unstructured-mesh elastic operator (icx_mesh.hip) -------------------------------
 // Mesh of one element type (ne = 4 quad4 / 8 hexa8, nc = 2 / 3 components): connectivity
-// node-of-element-major ([a][e]), unit-modulus element matrices as their upper triangles,
-// entry-major ([p][e], p = i nd - i (i - 1) / 2 + j - i for i <= j, nd = ne * nc,
-// node-major local DOFs a * nc + c), node adjacency in CSR (entries e * ne + a,
-// increasing e), and a scratch of nd doubles per element.
+// node-of-element-major ([a][e]), node coordinates ([d][node]), the unit-modulus material
+// (normal diagonal d0, normal off-diagonal d1, shear d2), node adjacency in CSR (entries
+// e * ne + a, increasing e), and a scratch of ne * nc doubles per element.  Element
+// matrices are integrated at the Gauss points in every pass, never stored.
 struct MeshOp {
   int ne, nc;
   long long nn, nelx;
   const int *conn;
-  const double *ke;
+  const double *xyz;
+  double d0, d1, d2;
   const int *adj_ptr, *adj;
   double *ye;
 };

@@ -1,69 +1,240 @@
-// Unstructured-mesh elastic operator of libicx (gfx950): explicit connectivity and one
-// unit-modulus element matrix per element (MeshMDPA meshes, src/io/mesh_mdpa.hpp; the
-// reference stores Ke per element as well, src/problem/elasticity.cpp:210-254).
+// Unstructured-mesh elastic operator of libicx (gfx950): explicit connectivity, element
+// matrices never stored (MeshMDPA meshes, src/io/mesh_mdpa.hpp; the element of
+// LinearElastic::makeKeFint, src/problem/elasticity.cpp:210-254).
 //
-// y = K(E) x in two passes, deterministic (no atomics):
-//   element pass: y_e = E_e Ke0_e x_e, one thread per element; Ke0 is symmetric and its
-//     upper triangle is stored entry-major ([p][e], p the row-major packed index of
-//     i <= j), so the threads of a wave stream consecutive addresses; y_e is written
-//     element-major ([e][i]) for the node pass;
+// Every element pass integrates at the Gauss points from the element's node coordinates:
+//   y_e = E_e sum_ip B_ip^T D (B_ip x_e) det J_ip
+// (2 x 2 (x 2) points +-1/sqrt(3), weight 1), which is Ke0_e x_e without forming Ke0_e:
+// per element it reads 8 node coordinates and values instead of a 300-word matrix, and
+// spends ~5k FP64 operations on them (the part has FP64 arithmetic to spare, not
+// bandwidth).  y = K(E) x is then two passes, deterministic (no atomics):
+//   element pass: one thread per element, y_e written element-major ([e][i]);
 //   node pass: y_n = sum over the (element, local node) pairs of node n, in increasing
 //     element order (node adjacency in CSR), zero on constrained components.
-// The Jacobi diagonal and the element energies u_e^T Ke0_e u_e use the same layouts.
+// The Jacobi diagonal (the element pass writes diag Ke0_e) and the element energies
+// u_e^T Ke0_e u_e = sum_ip eps^T D eps det J use the same integration.
 #include "icx_common.cuh"
 
 namespace icx {
 
 namespace {
 
-// packed index of (i, j), i <= j, in the row-major upper triangle of an nd x nd matrix
-__host__ __device__ constexpr int tri(int nd, int i, int j) { return i * nd - i * (i - 1) / 2 + (j - i); }
-
-// y = Ke0_e x over the packed upper triangle (each off-diagonal entry used twice)
-template <int ND>
-__device__ __forceinline__ void sym_apply(const double *ke, long long ne, long long e, const double (&x)[ND],
-                                          double (&y)[ND]) {
-#pragma unroll
-  for (int i = 0; i < ND; ++i) y[i] = 0.0;
-  // running pointer over the packed entries (one stride register, not one offset per entry)
-  const double *kp = ke + e;
-#pragma unroll
-  for (int i = 0; i < ND; ++i)
-#pragma unroll
-    for (int j = i; j < ND; ++j) {
-      const double k = *kp;
-      kp += ne;
-      y[i] = fma_t(k, x[j], y[i]);
-      if (j != i) y[j] = fma_t(k, x[i], y[j]);
-    }
+// the reference's local node order: corner signs of node a along x, y, z
+__host__ __device__ constexpr int csign(int a, int d) {
+  return d == 0 ? ((((a + 1) >> 1) & 1) ? 1 : -1) : d == 1 ? (((a >> 1) & 1) ? 1 : -1) : (((a >> 2) & 1) ? 1 : -1);
 }
 
+// Gauss point ip of the element and the Jacobian there: inv(J) and det(J).  X[a][k] are
+// the node coordinates.  dN_a/dr_d = s_ad / 2^NC prod_{e != d} (1 + s_ae r_e).
+template <int NE, int NC>
+struct GaussPoint {
+  double r[NC];
+  double inv[NC][NC];
+  double det;
+  __device__ __forceinline__ double dr(int a, int d) const {
+    double v = (NC == 3 ? 0.125 : 0.25) * csign(a, d);
+#pragma unroll
+    for (int e = 0; e < NC; ++e)
+      if (e != d) v *= 1.0 + csign(a, e) * r[e];
+    return v;
+  }
+  // dN_a / dX_k
+  __device__ __forceinline__ double dx(int a, int k) const {
+    double v = 0.0;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) v = fma_t(inv[k][d], dr(a, d), v);
+    return v;
+  }
+  __device__ __forceinline__ GaussPoint(int ip, const double (&X)[NE][NC]) {
+    const double g = 0.57735026918962576451;  // 1 / sqrt(3)
+#pragma unroll
+    for (int d = 0; d < NC; ++d) r[d] = csign(ip, d) * g;
+    double J[NC][NC];
+#pragma unroll
+    for (int d = 0; d < NC; ++d)
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NE; ++a) v = fma_t(dr(a, d), X[a][k], v);
+        J[d][k] = v;
+      }
+    if (NC == 2) {
+      det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      const double id = 1.0 / det;
+      inv[0][0] = J[1][1] * id; inv[0][1] = -J[0][1] * id;
+      inv[1][0] = -J[1][0] * id; inv[1][1] = J[0][0] * id;
+    } else {
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      const double id = 1.0 / det;
+      inv[0][0] = c00 * id;
+      inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+      inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+      inv[1][0] = c01 * id;
+      inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+      inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+      inv[2][0] = c02 * id;
+      inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+      inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    }
+  }
+};
+
+constexpr int voigt(int nc) { return nc == 3 ? 6 : 3; }
+
+// engineering strain B u (xx, yy, zz, xy, yz, zx / xx, yy, xy)
+template <int NE, int NC>
+__device__ __forceinline__ void strain(const GaussPoint<NE, NC> &gp, const double (&u)[NE * NC],
+                                       double (&eps)[voigt(NC)]) {
+#pragma unroll
+  for (int v = 0; v < voigt(NC); ++v) eps[v] = 0.0;
+#pragma unroll
+  for (int a = 0; a < NE; ++a) {
+    const double gx = gp.dx(a, 0), gy = gp.dx(a, 1);
+    if (NC == 2) {
+      const double ux = u[2 * a], uy = u[2 * a + 1];
+      eps[0] = fma_t(gx, ux, eps[0]);
+      eps[1] = fma_t(gy, uy, eps[1]);
+      eps[2] = fma_t(gy, ux, fma_t(gx, uy, eps[2]));
+    } else {
+      const double gz = gp.dx(a, 2);
+      const double ux = u[3 * a], uy = u[3 * a + 1], uz = u[3 * a + 2];
+      eps[0] = fma_t(gx, ux, eps[0]);
+      eps[1] = fma_t(gy, uy, eps[1]);
+      eps[2] = fma_t(gz, uz, eps[2]);
+      eps[3] = fma_t(gy, ux, fma_t(gx, uy, eps[3]));
+      eps[4] = fma_t(gz, uy, fma_t(gy, uz, eps[4]));
+      eps[5] = fma_t(gz, ux, fma_t(gx, uz, eps[5]));
+    }
+  }
+}
+
+// stress D eps of the unit-modulus material: normal diagonal d0, normal off-diagonal d1,
+// shear d2 (isotropic 3D, plane stress and plane strain all have this pattern)
+template <int NC>
+__device__ __forceinline__ void stress(const MeshOp &m, const double (&eps)[voigt(NC)], double (&sig)[voigt(NC)]) {
+  if (NC == 2) {
+    sig[0] = fma_t(m.d0, eps[0], m.d1 * eps[1]);
+    sig[1] = fma_t(m.d1, eps[0], m.d0 * eps[1]);
+    sig[2] = m.d2 * eps[2];
+  } else {
+    const double t = m.d1 * (eps[0] + eps[1] + eps[2]);
+    const double dd = m.d0 - m.d1;
+    sig[0] = fma_t(dd, eps[0], t);
+    sig[1] = fma_t(dd, eps[1], t);
+    sig[2] = fma_t(dd, eps[2], t);
+    sig[3] = m.d2 * eps[3];
+    sig[4] = m.d2 * eps[4];
+    sig[5] = m.d2 * eps[5];
+  }
+}
+
+template <int NE, int NC>
+__device__ __forceinline__ void gather_coords(const MeshOp &m, long long e, double (&X)[NE][NC]) {
+#pragma unroll
+  for (int a = 0; a < NE; ++a) {
+    const long long n = m.conn[a * m.nelx + e];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) X[a][c] = m.xyz[c * m.nn + n];
+  }
+}
+template <int NE, int NC>
+__device__ __forceinline__ void gather_values(const MeshOp &m, long long e, const double *x, double (&u)[NE * NC],
+                                              bool flush) {
+#pragma unroll
+  for (int a = 0; a < NE; ++a) {
+    const long long n = m.conn[a * m.nelx + e];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const double v = x[c * m.nn + n];
+      u[a * NC + c] = flush ? flush15(v) : v;
+    }
+  }
+}
+
+// y_e = E_e Ke0_e x_e
 template <int NE, int NC>
 __global__ void __launch_bounds__(256) k_mesh_element(MeshOp m, const double *E, const double *x, double *ye,
                                                       const SolverState *st) {
   constexpr int ND = NE * NC;
   if (st && st->done) return;
-  const long long ne = m.nelx;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < ne; e += (long long)gridDim.x * 256) {
-    double xe[ND];
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
+    double X[NE][NC], u[ND], f[ND];
+    gather_coords<NE, NC>(m, e, X);
+    gather_values<NE, NC>(m, e, x, u, false);
 #pragma unroll
-    for (int a = 0; a < NE; ++a) {
-      const long long n = m.conn[a * ne + e];
+    for (int i = 0; i < ND; ++i) f[i] = 0.0;
+#pragma unroll 1  // one Gauss point at a time: the element's values stay in registers
+    for (int ip = 0; ip < NE; ++ip) {
+      const GaussPoint<NE, NC> gp(ip, X);
+      double eps[voigt(NC)], sig[voigt(NC)];
+      strain<NE, NC>(gp, u, eps);
+      stress<NC>(m, eps, sig);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) xe[a * NC + c] = x[c * m.nn + n];
+      for (int v = 0; v < voigt(NC); ++v) sig[v] *= gp.det;
+      // B^T sigma
+#pragma unroll
+      for (int a = 0; a < NE; ++a) {
+        const double gx = gp.dx(a, 0), gy = gp.dx(a, 1);
+        if (NC == 2) {
+          f[2 * a] = fma_t(gx, sig[0], fma_t(gy, sig[2], f[2 * a]));
+          f[2 * a + 1] = fma_t(gy, sig[1], fma_t(gx, sig[2], f[2 * a + 1]));
+        } else {
+          const double gz = gp.dx(a, 2);
+          f[3 * a] = fma_t(gx, sig[0], fma_t(gy, sig[3], fma_t(gz, sig[5], f[3 * a])));
+          f[3 * a + 1] = fma_t(gy, sig[1], fma_t(gx, sig[3], fma_t(gz, sig[4], f[3 * a + 1])));
+          f[3 * a + 2] = fma_t(gz, sig[2], fma_t(gy, sig[4], fma_t(gx, sig[5], f[3 * a + 2])));
+        }
+      }
     }
-    double y[ND];
-    sym_apply<ND>(m.ke, ne, e, xe, y);
     const double Ee = E[e];
     double *out = ye + e * ND;
 #pragma unroll
-    for (int i = 0; i < ND; ++i) out[i] = Ee * y[i];
+    for (int i = 0; i < ND; ++i) out[i] = Ee * f[i];
+  }
+}
+
+// E_e diag(Ke0_e) into ye: DOF (a, c) has B column entries g_c in the normal row of c and
+// the other two g in the shear rows of c, so Ke_ii = sum_ip det J (d0 g_c^2 + d2 (|g|^2 - g_c^2))
+template <int NE, int NC>
+__global__ void __launch_bounds__(256) k_mesh_elem_diag(MeshOp m, const double *E, double *ye) {
+  constexpr int ND = NE * NC;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
+    double X[NE][NC], dg[ND];
+    gather_coords<NE, NC>(m, e, X);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) dg[i] = 0.0;
+#pragma unroll 1  // one Gauss point at a time: the element's values stay in registers
+    for (int ip = 0; ip < NE; ++ip) {
+      const GaussPoint<NE, NC> gp(ip, X);
+#pragma unroll
+      for (int a = 0; a < NE; ++a) {
+        double g[NC], s2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+          g[k] = gp.dx(a, k);
+          s2 = fma_t(g[k], g[k], s2);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const double gc2 = g[c] * g[c];
+          dg[a * NC + c] = fma_t(gp.det, fma_t(m.d0, gc2, m.d2 * (s2 - gc2)), dg[a * NC + c]);
+        }
+      }
+    }
+    const double Ee = E[e];
+    double *out = ye + e * ND;
+#pragma unroll
+    for (int i = 0; i < ND; ++i) out[i] = Ee * dg[i];
   }
 }
 
 template <int NE, int NC>
 __global__ void __launch_bounds__(256) k_mesh_gather(MeshOp m, const double *ye, const uint8_t *mask, double *y,
-                                                     const SolverState *st) {
+                                                     const SolverState *st, int invert) {
   constexpr int ND = NE * NC;
   if (st && st->done) return;
   for (long long n = blockIdx.x * 256LL + threadIdx.x; n < m.nn; n += (long long)gridDim.x * 256) {
@@ -79,49 +250,21 @@ __global__ void __launch_bounds__(256) k_mesh_gather(MeshOp m, const double *ye,
     }
     const uint32_t msk = mask ? mask[n] : 0u;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) y[c * m.nn + n] = ((msk >> c) & 1) ? 0.0 : acc[c];
-  }
-}
-
-// diag (inv = false) or Jacobi inverse (inv = true: 1/d, 1 where d = 0 or constrained)
-template <int NE, int NC>
-__global__ void __launch_bounds__(256) k_mesh_diag(MeshOp m, const double *E, const uint8_t *mask, double *out,
-                                                   bool inv) {
-  constexpr int ND = NE * NC;
-  const long long ne = m.nelx;
-  for (long long n = blockIdx.x * 256LL + threadIdx.x; n < m.nn; n += (long long)gridDim.x * 256) {
-    double acc[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-    for (int k = m.adj_ptr[n]; k < m.adj_ptr[n + 1]; ++k) {
-      const long long v = m.adj[k];
-      const long long e = v / NE;
-      const int a = (int)(v - e * NE);
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int i = a * NC + c;
-        acc[c] += E[e] * m.ke[(long long)tri(ND, i, i) * ne + e];
-      }
-    }
-    const uint32_t msk = mask ? mask[n] : 0u;
-#pragma unroll
     for (int c = 0; c < NC; ++c) {
       const double d = ((msk >> c) & 1) ? 0.0 : acc[c];
-      out[c * m.nn + n] = inv ? (d != 0.0 ? 1.0 / d : 1.0) : d;
+      y[c * m.nn + n] = invert ? (d != 0.0 ? 1.0 / d : 1.0) : d;
     }
   }
 }
 
-// u_e^T Ke0_e u_e with |u| < 1e-15 flushed (the displacement the drivers' sensitivity
-// reads), then the compliance sensitivity or E_e times the energy
+// u_e^T Ke0_e u_e = sum_ip eps^T D eps det J with |u| < 1e-15 flushed (the displacement
+// the drivers' sensitivity reads), then the compliance sensitivity or E_e times it
 template <int NE, int NC, bool SENS>
 __global__ void __launch_bounds__(256) k_mesh_energy(MeshOp m, double y1, double y2, double p, const double *design,
                                                      const double *E, const double *u, const double *vol,
                                                      double vscale, double *dfds, double *dgds) {
   constexpr int ND = NE * NC;
-  const long long ne = m.nelx;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < ne; e += (long long)gridDim.x * 256) {
-    // the SIMP derivative first: pow's registers are free again before the element loads
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
     double scale;
     if (SENS) {
       const double dens = design[e];
@@ -131,18 +274,21 @@ __global__ void __launch_bounds__(256) k_mesh_energy(MeshOp m, double y1, double
     } else {
       scale = E[e];
     }
-    double ue[ND];
-#pragma unroll
-    for (int a = 0; a < NE; ++a) {
-      const long long n = m.conn[a * ne + e];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) ue[a * NC + c] = flush15(u[c * m.nn + n]);
-    }
-    double y[ND];
-    sym_apply<ND>(m.ke, ne, e, ue, y);
+    double X[NE][NC], ue[ND];
+    gather_coords<NE, NC>(m, e, X);
+    gather_values<NE, NC>(m, e, u, ue, true);
     double en = 0.0;
+#pragma unroll 1  // one Gauss point at a time: the element's values stay in registers
+    for (int ip = 0; ip < NE; ++ip) {
+      const GaussPoint<NE, NC> gp(ip, X);
+      double eps[voigt(NC)], sig[voigt(NC)];
+      strain<NE, NC>(gp, ue, eps);
+      stress<NC>(m, eps, sig);
+      double w = 0.0;
 #pragma unroll
-    for (int i = 0; i < ND; ++i) en = fma_t(ue[i], y[i], en);
+      for (int v = 0; v < voigt(NC); ++v) w = fma_t(eps[v], sig[v], w);
+      en = fma_t(w, gp.det, en);
+    }
     if (dfds) dfds[e] = scale * en;
   }
 }
@@ -156,21 +302,26 @@ void launch_mesh_matvec(const MeshOp &m, const double *E, const uint8_t *mask, c
   if (m.ne == 8) {
     launch_ev(k_mesh_element<8, 3>, mesh_blocks(m.nelx), 256, s, ev, m, E, x, m.ye, st);
     hipLaunchKernelGGL((k_mesh_gather<8, 3>), dim3(mesh_blocks(m.nn)), dim3(256), 0, s, m, (const double *)m.ye,
-                       mask, y, st);
+                       mask, y, st, 0);
   } else {
     launch_ev(k_mesh_element<4, 2>, mesh_blocks(m.nelx), 256, s, ev, m, E, x, m.ye, st);
     hipLaunchKernelGGL((k_mesh_gather<4, 2>), dim3(mesh_blocks(m.nn)), dim3(256), 0, s, m, (const double *)m.ye,
-                       mask, y, st);
+                       mask, y, st, 0);
   }
   ICX_CHECK_LAUNCH();
 }
 
 void launch_mesh_diag(const MeshOp &m, const double *E, const uint8_t *mask, double *out, bool inv,
                       hipStream_t s) {
-  if (m.ne == 8)
-    hipLaunchKernelGGL((k_mesh_diag<8, 3>), dim3(mesh_blocks(m.nn)), dim3(256), 0, s, m, E, mask, out, inv);
-  else
-    hipLaunchKernelGGL((k_mesh_diag<4, 2>), dim3(mesh_blocks(m.nn)), dim3(256), 0, s, m, E, mask, out, inv);
+  if (m.ne == 8) {
+    hipLaunchKernelGGL((k_mesh_elem_diag<8, 3>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, m.ye);
+    hipLaunchKernelGGL((k_mesh_gather<8, 3>), dim3(mesh_blocks(m.nn)), dim3(256), 0, s, m, (const double *)m.ye,
+                       mask, out, nullptr, (int)inv);
+  } else {
+    hipLaunchKernelGGL((k_mesh_elem_diag<4, 2>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, m.ye);
+    hipLaunchKernelGGL((k_mesh_gather<4, 2>), dim3(mesh_blocks(m.nn)), dim3(256), 0, s, m, (const double *)m.ye,
+                       mask, out, nullptr, (int)inv);
+  }
   ICX_CHECK_LAUNCH();
 }
 
