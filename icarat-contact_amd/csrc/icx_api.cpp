@@ -1853,7 +1853,6 @@ static MisesMat mises_mat(const icx_job *j) {
 }
 
 icx_status icx_mises(icx_job *j, double *out) {
-  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "von Mises stress");
   if (!j || !out) return ICX_ERR_ARG;
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "icx_mises: elastic jobs only");
   if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
@@ -1861,7 +1860,8 @@ icx_status icx_mises(icx_job *j, double *out) {
   icx_status rc = output_displacement(j);
   if (rc) return rc;
   HIPCHK(c, j->ow.mises.alloc(j->g.ne));
-  launch_mises(j->g, mises_mat(j), j->d_E.p, j->ow.ufull.p, j->ow.mises.p, S(j));
+  if (j->mesh) launch_mesh_mises(j->mop, j->d_E.p, j->ow.ufull.p, j->ow.mises.p, S(j));
+  else launch_mises(j->g, mises_mat(j), j->d_E.p, j->ow.ufull.p, j->ow.mises.p, S(j));
   HIPCHK(c, hipMemcpyAsync(out, j->ow.mises.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipStreamSynchronize(S(j)));
   return ICX_OK;
@@ -1887,7 +1887,6 @@ icx_status icx_reactions(icx_job *j, double *out) {
 }
 
 icx_status icx_output_begin(icx_job *j) {
-  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the VTK output step");
   if (!j) return ICX_ERR_ARG;
   if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
   icx_ctx *c = j->ctx;
@@ -1901,7 +1900,8 @@ icx_status icx_output_begin(icx_job *j) {
   HIPCHK(c, o.h_mises.alloc(j->g.ne));
   if (!o.done) HIPCHK(c, hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
   launch_grid_to_nodes(j->g, j->nc, o.ufull.p, o.nodes.p, S(j));
-  if (!j->heat()) launch_mises(j->g, mises_mat(j), j->d_E.p, o.ufull.p, o.mises.p, S(j));
+  if (j->mesh) launch_mesh_mises(j->mop, j->d_E.p, o.ufull.p, o.mises.p, S(j));
+  else if (!j->heat()) launch_mises(j->g, mises_mat(j), j->d_E.p, o.ufull.p, o.mises.p, S(j));
   HIPCHK(c, hipMemcpyAsync(o.h_disp.p, o.nodes.p, j->g.nn * 3 * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   HIPCHK(c, hipMemcpyAsync(o.h_dens.p, j->d_design.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
   if (!j->heat())

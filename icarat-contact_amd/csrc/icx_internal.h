@@ -221,6 +221,8 @@ void launch_mesh_sensitivity(const MeshOp &m, double y1, double y2, double p, co
                              hipStream_t s);
 // E_e u_e^T Ke0_e u_e
 void launch_mesh_energy(const MeshOp &m, const double *E, const double *u, double *energy, hipStream_t s);
+// element von Mises stress of the Gauss-point mean strain (u: node values, SoA)
+void launch_mesh_mises(const MeshOp &m, const double *E, const double *u, double *out, hipStream_t s);
 
 // ---- single-reduction PCG iteration, FP32 hexa8 (icx_sr.hip) -------------------------
 // One kernel per iteration of the mixed form's Jacobi-PCG: launch k takes the step of

@@ -293,6 +293,42 @@ __global__ void __launch_bounds__(256) k_mesh_energy(MeshOp m, double y1, double
   }
 }
 
+// Elastic::makeMisesStress (src/problem/elasticity.cpp:165-208): strain averaged over the
+// Gauss points (weight 1 / ipmax), stress E_e D of it, mises = sqrt(s^T V s) with V the
+// deviatoric form (shear terms weighted 3).  u: node values (flushed, Dirichlet values in).
+template <int NE, int NC>
+__global__ void __launch_bounds__(256) k_mesh_mises(MeshOp m, const double *E, const double *u, double *out) {
+  constexpr int ND = NE * NC;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
+    double X[NE][NC], ue[ND], eb[voigt(NC)];
+    gather_coords<NE, NC>(m, e, X);
+    gather_values<NE, NC>(m, e, u, ue, false);
+#pragma unroll
+    for (int v = 0; v < voigt(NC); ++v) eb[v] = 0.0;
+#pragma unroll 1
+    for (int ip = 0; ip < NE; ++ip) {
+      const GaussPoint<NE, NC> gp(ip, X);
+      double eps[voigt(NC)];
+      strain<NE, NC>(gp, ue, eps);
+#pragma unroll
+      for (int v = 0; v < voigt(NC); ++v) eb[v] = fma_t(eps[v], 1.0 / NE, eb[v]);
+    }
+    double sg[voigt(NC)];
+    stress<NC>(m, eb, sg);
+    const double Ee = E[e];
+#pragma unroll
+    for (int v = 0; v < voigt(NC); ++v) sg[v] *= Ee;
+    double q;
+    if (NC == 2) {
+      q = sg[0] * sg[0] + sg[1] * sg[1] - sg[0] * sg[1] + 3.0 * sg[2] * sg[2];
+    } else {
+      q = sg[0] * sg[0] + sg[1] * sg[1] + sg[2] * sg[2] - sg[0] * sg[1] - sg[1] * sg[2] - sg[0] * sg[2] +
+          3.0 * (sg[3] * sg[3] + sg[4] * sg[4] + sg[5] * sg[5]);
+    }
+    out[e] = sqrt(q);
+  }
+}
+
 int mesh_blocks(long long n) { return ew_blocks(n, 1, 4096); }
 
 }  // namespace
@@ -344,6 +380,14 @@ void launch_mesh_energy(const MeshOp &m, const double *E, const double *u, doubl
   else
     hipLaunchKernelGGL((k_mesh_energy<4, 2, false>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, 0.0, 0.0,
                        0.0, nullptr, E, u, nullptr, 1.0, energy, nullptr);
+  ICX_CHECK_LAUNCH();
+}
+
+void launch_mesh_mises(const MeshOp &m, const double *E, const double *u, double *out, hipStream_t s) {
+  if (m.ne == 8)
+    hipLaunchKernelGGL((k_mesh_mises<8, 3>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, u, out);
+  else
+    hipLaunchKernelGGL((k_mesh_mises<4, 2>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, u, out);
   ICX_CHECK_LAUNCH();
 }
 

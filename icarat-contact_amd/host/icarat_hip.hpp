@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -125,16 +126,70 @@ class StructuredModel {
     prob_ = p;
     check(ctx(), icx_job_create(ctx(), &p, &job_), "icx_job_create");
     if (heat_) check(ctx(), icx_set_conductivity(job_, cond[0], cond[1]), "icx_set_conductivity");
-    int numnp, nelx, neq;
-    icx_job_sizes(job_, &numnp, &nelx, &neq);
-    fem_.ndim = ndim;
-    fem_.dofnp = heat_ ? 1 : ndim;
-    fem_.voigt = heat_ ? ndim : (ndim == 2 ? 3 : 6);
-    fem_.numnp = numnp;
-    fem_.nelx = nelx;
-    fem_.neq = neq;
-    volume_.assign(nelx, 0.0);
-    icx_element_volume(job_, volume_.data());
+    finish(ndim);
+  }
+
+  // MeshMDPA::setParameter / generate (src/io/mesh_mdpa.hpp:84-216): the mesh of an .mdpa
+  // file.  [mesh] type is a string or a list of element types; the GPU path takes meshes
+  // of one type, quad4 or hexa8, and rejects a file that holds elements of another
+  // listed type.  [material] as for the box (elastic only).
+  StructuredModel(const toml_lite::Table &config, FEM &fem, const std::string &mdpa) : fem_(fem) {
+    static const char *kTags[][2] = {{"tria3", "2D3"},   {"tria6", "2D6"},   {"quad4", "2D4"}, {"quad8", "2D8"},
+                                     {"tetra4", "3D4"},  {"tetra10", "3D10"}, {"prism6", "3D6"},
+                                     {"prism15", "3D15"}, {"hexa8", "3D8"},   {"hexa20", "3D20"}};
+    std::vector<std::string> types;
+    const auto &tv = config.at("mesh", "type");
+    if (tv.kind == toml_lite::Value::ARR)
+      for (const auto &t : tv.as_array()) types.push_back(t.as_string());
+    else
+      types.push_back(tv.as_string());
+    int eltype = 0;
+    for (const auto &t : types)
+      if (t == "quad4" || t == "hexa8") eltype = t == "quad4" ? ICX_QUAD4 : ICX_HEXA8;
+    if (!eltype) fatal("no element type of the GPU path (quad4, hexa8) in [mesh] type");
+    {
+      std::ifstream in(mdpa);
+      if (!in) fatal("cannot open " + mdpa);
+      std::string line;
+      while (std::getline(in, line)) {
+        if (line.rfind("Begin Elements", 0) != 0) continue;
+        for (const auto &t : types) {
+          if (t == "quad4" || t == "hexa8") continue;
+          for (const auto &kt : kTags)
+            if (t == kt[0] && line.find(kt[1]) != std::string::npos)
+              fatal("element type " + t + " is not supported by the GPU path (quad4, hexa8)");
+        }
+      }
+    }
+    const int ndim = config.at("mesh", "dimension").as_int();
+    if ((ndim == 2) != (eltype == ICX_QUAD4)) fatal("[mesh] dimension does not match the element type");
+    std::string state = config.at("material", "state").as_string();
+    if (ndim == 3) state = "3D";
+    icx_mesh m{};
+    m.eltype = eltype;
+    m.state = state == "plane_stress" ? ICX_PLANE_STRESS : (state == "plane_strain" ? ICX_PLANE_STRAIN : ICX_SOLID3D);
+    const auto &young = config.at("material", "young").as_array();
+    m.young1 = young[0].as_double();
+    m.young2 = young[1].as_double();
+    m.poisson = config.at("material", "poisson").as_double();
+    m.penalty = config.at("material", "penalty").as_double();
+    design_s0_ = config.at("material", "designs_0").as_double();
+    if (icx_mdpa_read(mdpa.c_str(), eltype, &m.numnp, &m.nelx, nullptr, nullptr) != ICX_OK)
+      fatal("cannot read the " + std::string(eltype == ICX_QUAD4 ? "quad4" : "hexa8") + " mesh of " + mdpa);
+    std::vector<double> x(3 * (size_t)m.numnp);
+    std::vector<int> conn((size_t)m.nelx * eltype);
+    if (icx_mdpa_read(mdpa.c_str(), eltype, &m.numnp, &m.nelx, x.data(), conn.data()) != ICX_OK)
+      fatal("cannot read " + mdpa);
+    m.x = x.data();
+    m.conn = conn.data();
+    prob_.eltype = eltype;
+    prob_.state = m.state;
+    prob_.young1 = m.young1;
+    prob_.young2 = m.young2;
+    prob_.poisson = m.poisson;
+    prob_.penalty = m.penalty;
+    check(ctx(), icx_job_create_mesh(ctx(), &m, &job_), "icx_job_create_mesh");
+    finish(ndim);
   }
   ~StructuredModel() {
     if (job_) icx_job_destroy(job_);
@@ -195,6 +250,19 @@ class StructuredModel {
   }
 
  private:
+  void finish(int ndim) {
+    int numnp, nelx, neq;
+    icx_job_sizes(job_, &numnp, &nelx, &neq);
+    fem_.ndim = ndim;
+    fem_.dofnp = heat_ ? 1 : ndim;
+    fem_.voigt = heat_ ? ndim : (ndim == 2 ? 3 : 6);
+    fem_.numnp = numnp;
+    fem_.nelx = nelx;
+    fem_.neq = neq;
+    volume_.assign(nelx, 0.0);
+    icx_element_volume(job_, volume_.data());
+  }
+
   FEM &fem_;
   icx_job *job_ = nullptr;
   icx_problem prob_{};

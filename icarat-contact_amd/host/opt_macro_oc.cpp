@@ -1,12 +1,14 @@
 // Driver A on the GPU path: cone density filter + projection + linear solve +
-// sensitivity + OC, step for step as opt-macro-oc/opt_macro_oc.cpp:13-180, on a
-// structured [mesh] box (the reference reads an unstructured .mdpa mesh instead).
+// sensitivity + OC, step for step as opt-macro-oc/opt_macro_oc.cpp:13-180, on the
+// unstructured mesh of an .mdpa file (as the reference: MeshMDPA) or on the structured
+// [mesh] box of the TOML file when no .mdpa is given.
 //
-// usage: opt_macro_oc <job.toml> [solver=Eigen_CG] [max_steps=1000] [radius=3.0]
+// usage: opt_macro_oc <job.toml> [mesh.mdpa] [solver=Eigen_CG] [max_steps=1000] [radius=3.0]
 // The OC volume function (filter + threshold of the candidate) runs on the device.
 // Writes function.csv and res<step>.vtk (binary) in the working directory.
 #include <chrono>
 #include <fstream>
+#include <memory>
 
 #include "icarat_hip.hpp"
 
@@ -14,19 +16,30 @@ using namespace icarat;
 
 int main(int argc, char *argv[]) {
   if (argc < 2) {
-    std::cout << "usage: " << argv[0] << " <.toml file> [solver] [max_steps] [radius]\n";
+    std::cout << "usage: " << argv[0] << " <.toml file> [.mdpa file] [solver] [max_steps] [radius]\n";
     return 1;
   }
-  const std::string solver = argc > 2 ? argv[2] : "Eigen_CG";
-  const int max_steps = argc > 3 ? std::atoi(argv[3]) : 1000;
-  const double filter_radious = argc > 4 ? std::atof(argv[4]) : 3.0;  // opt_macro_oc.cpp:74
+  std::string mdpa;
+  int a = 2;
+  if (argc > 2) {
+    const std::string s2 = argv[2];
+    if (s2.size() > 5 && s2.compare(s2.size() - 5, 5, ".mdpa") == 0) {
+      mdpa = s2;
+      a = 3;
+    }
+  }
+  const std::string solver = argc > a ? argv[a] : "Eigen_CG";
+  const int max_steps = argc > a + 1 ? std::atoi(argv[a + 1]) : 1000;
+  const double filter_radious = argc > a + 2 ? std::atof(argv[a + 2]) : 3.0;  // opt_macro_oc.cpp:74
   auto t_start = std::chrono::system_clock::now();
   int convergence = 1;
   double V0 = 0.0, T = 0.5;
   const auto config = toml_lite::parse(argv[1]);
 
   FEM fem;
-  StructuredModel model(config, fem);
+  std::unique_ptr<StructuredModel> mp(mdpa.empty() ? new StructuredModel(config, fem)
+                                                   : new StructuredModel(config, fem, mdpa));
+  StructuredModel &model = *mp;
   model.input_dirich(config);
   fem.numeq = model.makeDOF();
   Force force(fem.numeq);

@@ -60,6 +60,9 @@ def test_mesh_operator_solve_sensitivity(ctx, ne, state, div, L):
     assert np.abs(dc - dco).max() <= 1e-8 * np.abs(dco).max()
     assert np.abs(dv - vol).max() <= 1e-14 * vol.max()
     assert np.abs(job.volumes() - vol).max() <= 1e-14 * vol.max()
+    # element von Mises stress of the Gauss-point mean strain
+    mo = pr.mises(pr.node_values(ud))
+    assert np.abs(job.mises() - mo).max() <= 1e-8 * np.abs(mo).max()
     # K u over all DOFs: the applied load on the free DOFs, and zero in total (K 1 = 0:
     # the supports carry the load)
     R = job.reactions().reshape(-1, 3)
@@ -127,7 +130,7 @@ def test_mesh_job_rejects_box_only_calls(ctx):
     v = np.ones(job.numeq)
     for call in (lambda: job.matvec(v, prec=1), lambda: job.solve(icx.SOLVE_MIXED),
                  lambda: job.solve(icx.SOLVE_FUSED), lambda: job.helmholtz_setup(1.5),
-                 lambda: job.submit(tol=1e-8), lambda: job.mises(),
+                 lambda: job.submit(tol=1e-8),
                  lambda: job.opt_init(np.full(job.nelx, 0.5), filter=icx.FILTER_HELMHOLTZ),
                  lambda: job.opt_init(np.full(job.nelx, 0.5), form=icx.SOLVE_MIXED)):
         with pytest.raises(icx.IcxError) as e:
@@ -181,3 +184,60 @@ def test_mesh_driver_a_matches_oracle(ctx):
     vol = np.array([r["volume"] for r in recs])
     assert np.abs(vol - ref["vol"]).max() <= 1e-8 * np.abs(ref["vol"]).max()
     assert np.abs(job.opt_design() - ref["design"]).max() < 1e-7
+
+
+ROOT = __import__("os").path.dirname(__import__("os").path.dirname(__import__("os").path.abspath(__file__)))
+
+
+def test_cxx_driver_a_reads_mdpa_and_matches_oracle(tmp_path):
+    """The C++ driver A (opt_macro_oc <toml> <mdpa>, as the reference's opt-macro-oc)
+    on a distorted quad4 mesh: its function.csv against the oracle's driver loop, and
+    a binary VTK file per step with the mesh's points and cells."""
+    import os
+    import subprocess
+    ne, div, L, steps = 4, (24, 8), (24.0, 8.0), 6
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 12, L)
+    write_mdpa(tmp_path / "plate.mdpa", xd, conn, ne)
+    (tmp_path / "job.toml").write_text("""[mesh]
+dimension = 2
+type = ["quad4", "tria3"]
+
+[boundary_condition]
+dirichlet = [
+  [[1, 1, 0, 0, 0, 0], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0], [[0.0, 0.0], [0.0, 8.0], [0.0, 0.0]]],
+]
+neumann = [
+  [[0.0, -0.01, 0.0], [[24.0, 24.0], [0.0, 2.0], [0.0, 0.0]]],
+]
+
+[material]
+designs_0 = 0.4
+penalty = 3.0
+poisson = 0.3
+state = "plane_strain"
+type = "Elastic"
+young = [0.001, 100.0]
+""")
+    exe = os.path.join(ROOT, "icarat-contact_amd", "_build", "opt_macro_oc")
+    out = subprocess.run([exe, "job.toml", "plate.mdpa", "Eigen_CG", str(steps), "1.5"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    rows = np.loadtxt(tmp_path / "function.csv", delimiter=",", ndmin=2)
+    assert rows.shape[0] == steps
+    head = open(tmp_path / "res1.vtk", "rb").read(120).split(b"\n")
+    assert head[2] == b"BINARY" and head[4] == ("POINTS %d float" % (len(xd) // 3)).encode()
+    pr = O.Problem.from_mesh(ne, xd, conn, O.PLANE_STRAIN, young=(0.001, 100.0))
+    pr.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, 8, 0, 0])
+    pr.make_dof()
+    pr.add_neumann([0.0, -0.01, 0.0], [24, 24, 0, 2, 0, 0])
+    ref = pr.run_driver(0, np.full(pr.nelx, 0.4), steps, 1.5, projection=True, direct=True)
+    # the driver solves with CG to the reference's 1e-10, the oracle directly
+    rel = np.abs(rows[:, 1] - ref["obj"]) / np.abs(ref["obj"])
+    assert rel.max() <= 1e-7, rel
+    # with a tria3 block in the file, the GPU path refuses the mesh
+    with open(tmp_path / "plate.mdpa", "a") as f:
+        f.write("\nBegin Elements SmallDisplacementElement2D3N\n1 0 1 2 3\nEnd Elements\n")
+    out = subprocess.run([exe, "job.toml", "plate.mdpa", "Eigen_CG", "1"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode != 0 and "tria3" in out.stderr
