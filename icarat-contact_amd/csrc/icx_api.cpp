@@ -440,6 +440,13 @@ struct MMAWork {
   int iter = 0;
   DBuf<double> low, upp, alpha, beta, p0, q0, p1, q1, xold1, xold2;
 };
+// GCMMASolver state with m = 1 (GCMMASolver.hpp:91-114): the MMA vectors (iter is
+// outeriter) plus the conservative factors raa0 / raa, the constants r0 / r of the
+// approximations and their values f0app / fapp at the last returned point.
+struct GcmmaWork {
+  MMAWork v;
+  double raa0 = 1.0, raa = 1.0, r0 = 0.0, r = 0.0, f0app = 0.0, fapp = 0.0;
+};
 // Output step (icx_output_begin / _write): device temporaries, pinned host copies of
 // the displacement (node-major), density and mises, and the event that completes them.
 struct OutputWork {
@@ -463,6 +470,7 @@ struct OptWork {
   double beta = 1.0;     // projection sharpness of the current step
   std::chrono::steady_clock::time_point t0;
   DBuf<double> x, fil, thr, dthr, dfds, dgds, tmp, tmp2;
+  DBuf<double> xnew;     // GCMMA driver: the trial design
 };
 
 // ---- volume of a general quad4 / hexa8 ----------------------------------------------
@@ -517,7 +525,7 @@ struct icx_job {
   DBuf<int> d_dof;
   DBuf<uint8_t> d_mask;
   DBuf<double> d_fext, d_dirich, d_b, d_u, d_design, d_E, d_vec, d_vec2;
-  DBuf<double> d_partial;   // generic reductions [2][RED_MAXBLOCKS]
+  DBuf<double> d_partial;   // generic reductions [3][RED_MAXBLOCKS]
   DBuf<double> d_pcgpart;   // PCG partials: PA, PB (x2), PN, PZ
   DBuf<SolverState> d_state;
   DBuf<double> d_scalars;
@@ -563,6 +571,7 @@ struct icx_job {
   HeatOp heat_op{}, heat_dK{};
   bool heat_ready = false;
   MMAWork mma;
+  GcmmaWork gc;
   OptWork opt;
 
   Reduce red() const { return Reduce{d_partial.p}; }
@@ -841,7 +850,7 @@ static icx_status job_finish(icx_ctx *ctx, icx_job *j, icx_job **out) {
   j->node_val.assign(g.nn * 6, 0.0);
   (void)hipSetDevice(ctx->device);
   const long long N = j->nvec();
-  if (j->d_partial.alloc(2 * RED_MAXBLOCKS) || j->d_pcgpart.alloc(10 * RED_MAXBLOCKS) || j->d_state.alloc(1) ||
+  if (j->d_partial.alloc(3 * RED_MAXBLOCKS) || j->d_pcgpart.alloc(10 * RED_MAXBLOCKS) || j->d_state.alloc(1) ||
       j->d_scalars.alloc(64) || j->d_fext.alloc(N) || j->d_b.alloc(N) || j->d_u.alloc(N) ||
       j->d_design.alloc(g.ne) || j->d_E.alloc(g.ne) || j->d_vec.alloc(N) || j->d_vec2.alloc(N) ||
       j->d_mask.alloc(g.nn)) {
@@ -2370,13 +2379,11 @@ icx_status oc_update_dev(icx_job *j, double *dx, const double *dfdx, double g0, 
   return ICX_OK;
 }
 
-MMAVecs mma_vecs(icx_job *j) {
-  MMAWork &m = j->mma;
+MMAVecs mma_vecs(MMAWork &m) {
   return MMAVecs{m.low.p, m.upp.p, m.alpha.p, m.beta.p, m.p0.p, m.q0.p, m.p1.p, m.q1.p, m.xold1.p, m.xold2.p};
 }
 
-icx_status mma_alloc(icx_job *j) {
-  MMAWork &m = j->mma;
+icx_status mma_alloc(icx_job *j, MMAWork &m) {
   if (m.ready) return ICX_OK;
   icx_ctx *c = j->ctx;
   const long long n = j->g.ne;
@@ -2389,26 +2396,13 @@ icx_status mma_alloc(icx_job *j) {
   return ICX_OK;
 }
 
-// MMASolver::Update with m = 1 (MMASolver.cpp:66-154, 184-398): per-element work on the
-// device, the scalar dual Newton iteration on the host.
-icx_status mma_update_dev(icx_job *j, double *dx, const double *dfdx, double gx, const double *dgdx,
-                          double xmin, double xmax, int *steps) {
-  icx_status rc = mma_alloc(j);
-  if (rc) return rc;
-  MMAWork &m = j->mma;
+// SolveDIP with m = 1, a = 0 (MMASolver.cpp:102-154, 184-398; GCMMASolver.cpp:135-197,
+// 227-429): the scalar dual Newton iteration on the host; x(lambda) into dx and its two
+// sums on the device.  b is the constraint's constant, cc the y penalty c.
+icx_status mma_dual(icx_job *j, const MMAVecs &v, double *dx, double b, double cc, double epsimin, int *steps) {
   const long long n = j->g.ne;
-  const double move = 0.15, albefa = 0.2, asyminit = 0.2, asymdec = 0.7, asyminc = 1.2, raa0 = 0.00001,
-               xmamieps = 1.0e-5;
-  const double a = 0.0, cc = 100000.0;
-  const double epsimin = std::sqrt((double)(n + 1)) * 1e-9;
-  m.iter++;
-  MMAVecs v = mma_vecs(j);
-  launch_mma_gensub(n, m.iter, v, dx, dfdx, dgdx, xmin, xmax, move, albefa, asyminit, asymdec, asyminc, raa0,
-                    xmamieps, j->d_scalars.p, j->red(), S(j));
-  double bsum;
-  rc = read_scalars(j, 1, &bsum);
-  if (rc) return rc;
-  const double b = -gx + bsum;
+  const double a = 0.0;
+  icx_status rc = ICX_OK;
   double lam = cc / 2.0, mu = 1.0;
   double epsi = 1.0, err = 1.0;
   int outloop = 0, total = 0;
@@ -2462,6 +2456,80 @@ icx_status mma_update_dev(icx_job *j, double *dx, const double *dfdx, double gx,
   return ICX_OK;
 }
 
+// MMASolver::Update with m = 1 (MMASolver.cpp:66-154, 400-512): GenSub on the device,
+// then the dual solve with c = 1e5.
+icx_status mma_update_dev(icx_job *j, double *dx, const double *dfdx, double gx, const double *dgdx,
+                          double xmin, double xmax, int *steps) {
+  MMAWork &m = j->mma;
+  icx_status rc = mma_alloc(j, m);
+  if (rc) return rc;
+  const long long n = j->g.ne;
+  const double move = 0.15, albefa = 0.2, asyminit = 0.2, asymdec = 0.7, asyminc = 1.2, raa0 = 0.00001,
+               xmamieps = 1.0e-5;
+  const double epsimin = std::sqrt((double)(n + 1)) * 1e-9;
+  m.iter++;
+  MMAVecs v = mma_vecs(m);
+  launch_mma_gensub(n, m.iter, 1, 1, v, dx, dfdx, dgdx, xmin, xmax, move, albefa, asyminit, asymdec, asyminc,
+                    raa0, raa0, xmamieps, j->d_scalars.p, j->red(), S(j));
+  double sums[2];
+  rc = read_scalars(j, 2, sums);
+  if (rc) return rc;
+  return mma_dual(j, v, dx, -gx + sums[1], 100000.0, epsimin, steps);
+}
+
+// GCMMASolver with m = 1, c = 1000 (GCMMASolver.cpp:41-129, 432-620).  outer = 1:
+// OuterUpdate (new asymptotes and x history, raa0 / raa decay to max(1e-6, 0.1 raa));
+// outer = 0: InnerUpdate, whose RaaUpdate grows raa0 / raa from the true values
+// f0new / fnew at the last returned point, which xmma holds on entry.  Both then build
+// the subproblem at xval, solve its dual into xmma and evaluate the approximations there.
+icx_status gcmma_update_dev(icx_job *j, bool outer, double *xmma, double f0new, double fnew, const double *xval,
+                            double f0x, const double *df0dx, double fx, const double *dfdx, double xmin,
+                            double xmax, int *steps) {
+  GcmmaWork &g = j->gc;
+  icx_status rc = mma_alloc(j, g.v);
+  if (rc) return rc;
+  const long long n = j->g.ne;
+  const double move = 0.15, albefa = 0.2, asyminit = 0.2, asymdec = 0.7, asyminc = 1.2, xmamieps = 1.0e-5;
+  const double raaeps = 1e-6, epsimin = std::sqrt((double)(n + 1)) * 1e-9;
+  const double xmami = std::max(xmamieps, xmax - xmin);
+  MMAVecs v = mma_vecs(g.v);
+  if (outer) {
+    g.v.iter++;
+    g.raa0 = std::max(raaeps, 0.1 * g.raa0);
+    g.raa = std::max(raaeps, 0.1 * g.raa);
+  } else {
+    launch_gcmma_approx(n, v, xmma, xval, xmami, j->d_scalars.p, j->red(), S(j));
+    double a[3];
+    rc = read_scalars(j, 3, a);
+    if (rc) return rc;
+    const double raacof = std::max(1e-12, a[2]);
+    if (f0new > g.f0app + 0.5 * epsimin) g.raa0 = std::min(1.1 * (g.raa0 + (1.0 / raacof) * (f0new - g.f0app)), 10.0 * g.raa0);
+    if (fnew > g.fapp + 0.5 * epsimin) g.raa = std::min(1.1 * (g.raa + (1.0 / raacof) * (fnew - g.fapp)), 10.0 * g.raa);
+  }
+  launch_mma_gensub(n, g.v.iter, outer ? 1 : 0, 0, v, xval, df0dx, dfdx, xmin, xmax, move, albefa, asyminit,
+                    asymdec, asyminc, g.raa0, g.raa, xmamieps, j->d_scalars.p, j->red(), S(j));
+  double sums[2];
+  rc = read_scalars(j, 2, sums);
+  if (rc) return rc;
+  g.r0 = f0x - sums[0];
+  g.r = fx - sums[1];
+  rc = mma_dual(j, v, xmma, -g.r, 1000.0, epsimin, steps);
+  if (rc) return rc;
+  launch_gcmma_approx(n, v, xmma, xval, xmami, j->d_scalars.p, j->red(), S(j));
+  double a[3];
+  rc = read_scalars(j, 3, a);
+  if (rc) return rc;
+  g.f0app = a[0] + g.r0;
+  g.fapp = a[1] + g.r;
+  return ICX_OK;
+}
+
+// ConCheck (GCMMASolver.cpp:115-129)
+bool gcmma_conservative(icx_job *j, double f0new, double fnew) {
+  const double epsimin = std::sqrt((double)(j->g.ne + 1)) * 1e-9;
+  return !(j->gc.f0app + epsimin < f0new) && !(j->gc.fapp + epsimin < fnew);
+}
+
 }  // namespace
 
 extern "C" {
@@ -2509,11 +2577,57 @@ icx_status icx_mma_update(icx_job *j, double *x, const double *dfdx, double gx, 
   return ICX_OK;
 }
 
+icx_status icx_gcmma_reset(icx_job *j) {
+  if (!j) return ICX_ERR_ARG;
+  j->gc.v.iter = 0;
+  return ICX_OK;
+}
+
+static icx_status gcmma_call(icx_job *j, bool outer, double *xmma, double f0new, double fnew, const double *xval,
+                             double f0x, const double *df0dx, double fx, const double *dfdx, double xmin,
+                             double xmax, int *newton_steps) {
+  if (!j || !xmma || !xval || !df0dx || !dfdx) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  if (!outer && j->gc.v.iter == 0) return fail(c, ICX_ERR_STATE, "GCMMA inner update before an outer update");
+  const long long ne = j->g.ne;
+  DBuf<double> xm, xv, df, dg;
+  HIPCHK(c, xm.alloc(ne)); HIPCHK(c, xv.alloc(ne)); HIPCHK(c, df.alloc(ne)); HIPCHK(c, dg.alloc(ne));
+  if (!outer) HIPCHK(c, hipMemcpyAsync(xm.p, xmma, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(xv.p, xval, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(df.p, df0dx, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(dg.p, dfdx, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  icx_status rc = gcmma_update_dev(j, outer, xm.p, f0new, fnew, xv.p, f0x, df.p, fx, dg.p, xmin, xmax, newton_steps);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(xmma, xm.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+icx_status icx_gcmma_outer_update(icx_job *j, double *xmma, const double *xval, double f0x, const double *df0dx,
+                                  double fx, const double *dfdx, double xmin, double xmax, int *newton_steps) {
+  return gcmma_call(j, true, xmma, 0.0, 0.0, xval, f0x, df0dx, fx, dfdx, xmin, xmax, newton_steps);
+}
+
+icx_status icx_gcmma_inner_update(icx_job *j, double *xmma, double f0xnew, double fxnew, const double *xval,
+                                  double f0x, const double *df0dx, double fx, const double *dfdx, double xmin,
+                                  double xmax, int *newton_steps) {
+  return gcmma_call(j, false, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax, newton_steps);
+}
+
+icx_status icx_gcmma_concheck(const icx_job *jc, double f0xnew, double fxnew, int *conservative) {
+  icx_job *j = const_cast<icx_job *>(jc);
+  if (!j || !conservative) return ICX_ERR_ARG;
+  if (j->gc.v.iter == 0) return fail(j->ctx, ICX_ERR_STATE, "GCMMA check before an outer update");
+  *conservative = gcmma_conservative(j, f0xnew, fxnew) ? 1 : 0;
+  return ICX_OK;
+}
+
 // ---- device-resident design loop ------------------------------------------------------------
 icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0) {
   if (!j || !par || !x0) return ICX_ERR_ARG;
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
+  if (par->driver < 0 || par->driver > 2) return fail(j->ctx, ICX_ERR_ARG, "driver must be 0 (OC), 1 (MMA) or 2 (GCMMA)");
   if (j->mesh && (par->filter == ICX_FILTER_HELMHOLTZ || par->form != ICX_SOLVE_PLAIN))
     return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: the design loop takes the cone filter and ICX_SOLVE_PLAIN");
   icx_ctx *c = j->ctx;
@@ -2529,9 +2643,17 @@ icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0)
   else if (par->filter == ICX_FILTER_HELMHOLTZ) rc = helm_setup(j, par->filter_radius);
   if (rc) return rc;
   if (par->driver == 1) {
-    rc = mma_alloc(j);
+    rc = mma_alloc(j, j->mma);
     if (rc) return rc;
     j->mma.iter = 0;
+  } else if (par->driver == 2) {
+    rc = mma_alloc(j, j->gc.v);
+    if (rc) return rc;
+    GcmmaWork &g = j->gc;  // a new GCMMASolver (GCMMASolver.cpp:41-56)
+    g.v.iter = 0;
+    g.raa0 = g.raa = 1.0;
+    g.r0 = g.r = g.f0app = g.fapp = 0.0;
+    HIPCHK(c, o.xnew.alloc(ne));
   }
   HIPCHK(c, hipStreamSynchronize(S(j)));
   o.ready = true;
@@ -2540,6 +2662,92 @@ icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0)
 
 }  // extern "C"
 
+namespace {
+
+// filter, projection (beta of the current step), design -> moduli: the design of x
+// (opt_macro_oc.cpp:87-101, opt_mma_Helmholtz.cpp:86-98, opt_macro_gcmma.cpp:96-101)
+icx_status opt_design(icx_job *j, const double *x) {
+  OptWork &o = j->opt;
+  const icx_opt_params &P = o.par;
+  icx_ctx *c = j->ctx;
+  const long long ne = j->g.ne;
+  if (P.filter == ICX_FILTER_CONE) cone_apply(j, 0, x, o.fil.p);
+  else if (P.filter == ICX_FILTER_HELMHOLTZ) {
+    icx_status rc = helm_apply_dev(j, 0, P.helm_tol, x, o.fil.p, nullptr);
+    if (rc) return rc;
+  } else HIPCHK(c, hipMemcpyAsync(o.fil.p, x, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  if (P.projection) launch_project(ne, o.beta, 0.5, o.fil.p, o.thr.p, o.dthr.p, S(j));
+  else HIPCHK(c, hipMemcpyAsync(o.thr.p, o.fil.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(j->d_design.p, o.thr.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  launch_simp(ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
+  j->design_set = true;
+  return ICX_OK;
+}
+
+// the structural solve of the design loop in its configured form
+icx_status opt_solve(icx_job *j, icx_solve_info &info) {
+  const icx_opt_params &P = j->opt.par;
+  icx_status rc = build_rhs(j);
+  if (rc) return rc;
+  if (P.form == ICX_SOLVE_PLAIN) rc = solve_plain(j, P.tol, P.maxit, info);
+  else if (P.form == ICX_SOLVE_FUSED) rc = solve_fused(j, P.tol, P.maxit, info);
+  else rc = solve_mixed(j, P.tol, P.maxit, info);
+  return rc == ICX_ERR_NOCONV ? ICX_OK : rc;
+}
+
+// objective u.fext and volume sum rho_hat v of the job's current solve and design
+icx_status opt_objective_volume(icx_job *j, double *obj, double *vol) {
+  launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
+  launch_volume(j->g.ne, 0, 0.0, 0.5, j->opt.thr.p, j->volume, j->mesh ? j->d_vol.p : nullptr, j->d_scalars.p + 1,
+                j->red(), S(j));
+  double sc[2];
+  icx_status rc = read_scalars(j, 2, sc);
+  *obj = sc[0];
+  *vol = sc[1];
+  return rc;
+}
+
+// The GCMMA step of opt_macro_gcmma.cpp:150-215: outer update from x with (f, df, g, dg),
+// a solve of the trial design, and up to 15 inner updates while the approximations are
+// not conservative; x takes the last trial design.
+icx_status gcmma_step(icx_job *j, double f, const double *df, double g, const double *dg, icx_iter_record &r) {
+  OptWork &o = j->opt;
+  const icx_opt_params &P = o.par;
+  int steps = 0, total = 0;
+  double fnew = 0.0, gnew = 0.0;
+  auto trial = [&]() -> icx_status {
+    icx_status rc = opt_design(j, o.xnew.p);
+    if (rc) return rc;
+    icx_solve_info info{};
+    rc = opt_solve(j, info);
+    if (rc) return rc;
+    r.trial_cg_iterations += info.iterations;
+    double vol;
+    rc = opt_objective_volume(j, &fnew, &vol);
+    gnew = vol - o.V0;
+    return rc;
+  };
+  icx_status rc = gcmma_update_dev(j, true, o.xnew.p, 0.0, 0.0, o.x.p, f, df, g, dg, P.xmin, P.xmax, &steps);
+  if (rc) return rc;
+  total += steps;
+  if ((rc = trial())) return rc;
+  bool conserv = gcmma_conservative(j, fnew, gnew);
+  int inner = 0;
+  for (; !conserv && inner < 15; ++inner) {
+    rc = gcmma_update_dev(j, false, o.xnew.p, fnew, gnew, o.x.p, f, df, g, dg, P.xmin, P.xmax, &steps);
+    if (rc) return rc;
+    total += steps;
+    if ((rc = trial())) return rc;
+    conserv = gcmma_conservative(j, fnew, gnew);
+  }
+  r.inner_loops = inner;
+  r.update_steps = total;
+  HIPCHK(j->ctx, hipMemcpyAsync(o.x.p, o.xnew.p, j->g.ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  return ICX_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 icx_status icx_opt_step_begin(icx_job *j) {
@@ -2547,25 +2755,14 @@ icx_status icx_opt_step_begin(icx_job *j) {
   OptWork &o = j->opt;
   if (!o.ready) return fail(j->ctx, ICX_ERR_STATE, "icx_opt_init first");
   if (o.in_step) return fail(j->ctx, ICX_ERR_STATE, "design step already begun");
-  icx_ctx *c = j->ctx;
   o.t0 = std::chrono::steady_clock::now();
-  const icx_opt_params &P = o.par;
-  const long long ne = j->g.ne;
   const int k = ++o.step;
-  // filter
-  if (P.filter == ICX_FILTER_CONE) cone_apply(j, 0, o.x.p, o.fil.p);
-  else if (P.filter == ICX_FILTER_HELMHOLTZ) {
-    icx_status rc = helm_apply_dev(j, 0, P.helm_tol, o.x.p, o.fil.p, nullptr);
-    if (rc) return rc;
-  } else HIPCHK(c, hipMemcpyAsync(o.fil.p, o.x.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
-  // projection, beta = min(2^(k/40), 16) (opt_macro_oc.cpp:94, opt_mma_Helmholtz.cpp:90)
-  o.beta = std::min(std::pow(2, (int)(k / 40)), 16.0);
-  if (P.projection) launch_project(ne, o.beta, 0.5, o.fil.p, o.thr.p, o.dthr.p, S(j));
-  else HIPCHK(c, hipMemcpyAsync(o.thr.p, o.fil.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
-  // design -> E
-  HIPCHK(c, hipMemcpyAsync(j->d_design.p, o.thr.p, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
-  launch_simp(ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
-  j->design_set = true;
+  // projection, beta = min(2^(k/40), 16) (opt_macro_oc.cpp:94, opt_mma_Helmholtz.cpp:90);
+  // the GCMMA driver: min(int(max((k - 100) / 40, 1)), 8) (opt_macro_gcmma.cpp:91-92)
+  if (o.par.driver == 2) o.beta = std::min((double)(int)std::max((k - 100) / 40.0, 1.0), 8.0);
+  else o.beta = std::min(std::pow(2, (int)(k / 40)), 16.0);
+  icx_status rc = opt_design(j, o.x.p);
+  if (rc) return rc;
   o.in_step = true;
   return ICX_OK;
 }
@@ -2583,21 +2780,14 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
   r.step = k;
   r.cg_iterations = info ? info->iterations : 0;
   j->u_valid = true;
-  // objective and volume
-  launch_dot(j->nvec(), j->d_u.p, j->d_fext.p, j->d_scalars.p, j->red(), S(j));
-  launch_volume(ne, 0, 0.0, 0.5, o.thr.p, j->volume, j->mesh ? j->d_vol.p : nullptr, j->d_scalars.p + 1, j->red(),
-                S(j));
-  double sc[2];
-  icx_status rc = read_scalars(j, 2, sc);
+  // objective and volume; const_h: V (A), V / V0 - 1 (B), V - V0 (GCMMA,
+  // opt_macro_gcmma.cpp:109-125)
+  icx_status rc = opt_objective_volume(j, &r.objective, &r.volume);
   if (rc) return rc;
-  r.objective = sc[0];
-  r.volume = sc[1];
-  if (k == 1) {
-    o.V0 = sc[1];
-    r.constraint = P.driver == 0 ? o.V0 : 0.0;
-  } else {
-    r.constraint = P.driver == 0 ? sc[1] : sc[1] / o.V0 - 1.0;
-  }
+  if (k == 1) o.V0 = r.volume;
+  if (P.driver == 0) r.constraint = r.volume;
+  else if (P.driver == 1) r.constraint = k == 1 ? 0.0 : r.volume / o.V0 - 1.0;
+  else r.constraint = r.volume - o.V0;
   // sensitivity and chain rule
   const double *usens;
   {
@@ -2606,10 +2796,10 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
   }
   if (j->mesh)
     launch_mesh_sensitivity(j->mop, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, usens,
-                            j->d_vol.p, P.driver == 0 ? 1.0 : o.V0, o.dfds.p, o.dgds.p, S(j));
+                            j->d_vol.p, P.driver == 1 ? o.V0 : 1.0, o.dfds.p, o.dgds.p, S(j));
   else
     launch_sensitivity(j->g, j->op64, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, usens,
-                       j->volume, P.driver == 0 ? 1.0 : o.V0, o.dfds.p, o.dgds.p, S(j));
+                       j->volume, P.driver == 1 ? o.V0 : 1.0, o.dfds.p, o.dgds.p, S(j));
   // dfds *= dthr, dgds *= dthr (projection derivative; 1 without projection)
   if (P.projection) {
     launch_mul(ne, o.dthr.p, o.dfds.p, S(j));
@@ -2633,8 +2823,11 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
   if (P.driver == 0) {
     rc = oc_update_dev(j, o.x.p, o.tmp.p, o.V0, o.tmp2.p, P.xmin, P.xmax, P.move, P.filter,
                        P.projection ? o.beta : 0.0, 0.5, o.fil.p, o.dfds.p, &steps);
-  } else {
+  } else if (P.driver == 1) {
     rc = mma_update_dev(j, o.x.p, o.tmp.p, r.constraint, o.tmp2.p, P.xmin, P.xmax, &steps);
+  } else {
+    rc = gcmma_step(j, r.objective, o.tmp.p, r.constraint, o.tmp2.p, r);
+    steps = r.update_steps;
   }
   if (rc) return rc;
   r.update_steps = steps;
@@ -2647,14 +2840,9 @@ icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_rec
 icx_status icx_opt_step(icx_job *j, icx_iter_record *rec) {
   icx_status rc = icx_opt_step_begin(j);
   if (rc) return rc;
-  const icx_opt_params &P = j->opt.par;
   icx_solve_info info{};
-  rc = build_rhs(j);
-  if (rc) return rc;
-  if (P.form == ICX_SOLVE_PLAIN) rc = solve_plain(j, P.tol, P.maxit, info);
-  else if (P.form == ICX_SOLVE_FUSED) rc = solve_fused(j, P.tol, P.maxit, info);
-  else rc = solve_mixed(j, P.tol, P.maxit, info);
-  if (rc && rc != ICX_ERR_NOCONV) {
+  rc = opt_solve(j, info);
+  if (rc) {
     j->opt.in_step = false;
     return rc;
   }

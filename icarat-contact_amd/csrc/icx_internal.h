@@ -443,10 +443,15 @@ void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double l
 struct MMAVecs {
   double *low, *upp, *alpha, *beta, *p0, *q0, *p1, *q1, *xold1, *xold2;
 };
-void launch_mma_gensub(long long n, int iter, const MMAVecs &m, const double *xval,
+// asymp = 1: new asymptotes and x history (MMA, GCMMA outer); 0: keep them (GCMMA inner).
+// clamp = 1: MMA's alpha <= xmax, beta >= xmin. sums[0..1] = objective / constraint sums.
+void launch_mma_gensub(long long n, int iter, int asymp, int clamp, const MMAVecs &m, const double *xval,
                        const double *dfdx, const double *dgdx, double xmin, double xmax,
                        double move, double albefa, double asyminit, double asymdec, double asyminc,
-                       double raa0, double xmamieps, double *bsum, Reduce red, hipStream_t s);
+                       double raa0, double raa1, double xmamieps, double *sums, Reduce red, hipStream_t s);
+// GCMMA approximations at xmma and raacof: out[0..2]
+void launch_gcmma_approx(long long n, const MMAVecs &m, const double *xmma, const double *xval, double xmami,
+                         double *out, Reduce red, hipStream_t s);
 // x(lambda) and sums: out[0] = sum p1/(upp-x) + q1/(x-low), out[1] = sum PQ^2 df2
 void launch_mma_dual(long long n, double lam, const MMAVecs &m, double *x, double *out, Reduce red,
                      hipStream_t s);

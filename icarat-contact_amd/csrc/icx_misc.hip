@@ -688,25 +688,31 @@ void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double l
 
 // ---------------------------------------------------------------------------------
 // MMA (m = 1), src/optimization/opt-solver/MMASolver.cpp:400-512 (GenSub) and
-// 184-398 (XYZofLAMBDA, DualGrad, DualHess, DualResidual sums)
+// 184-398 (XYZofLAMBDA, DualGrad, DualHess, DualResidual sums); GCMMA's Asymp and GenSub
+// (GCMMASolver.cpp:432-489, 546-596) are the same pass with raa0 / raa in place of MMA's
+// one raa0, no clamp of alpha to xmax and of beta to xmin, and, in inner updates, the
+// asymptotes and x history left as they are (asymp = 0).
+// sums: [0] = sum p0/(U-x) + q0/(x-L), [1] = sum p1/(U-x) + q1/(x-L), both at xval.
 // ---------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_mma_gensub(long long n, int iter, MMAVecs m, const double *xval,
-                                                    const double *dfdx, const double *dgdx, double xmin,
-                                                    double xmax, double move, double albefa, double asyminit,
-                                                    double asymdec, double asyminc, double raa0,
-                                                    double xmamieps, Reduce red) {
+__global__ void __launch_bounds__(256) k_mma_gensub(long long n, int iter, int asymp, int clamp, MMAVecs m,
+                                                    const double *xval, const double *dfdx, const double *dgdx,
+                                                    double xmin, double xmax, double move, double albefa,
+                                                    double asyminit, double asymdec, double asyminc, double raa0,
+                                                    double raa1, double xmamieps, Reduce red) {
   __shared__ double scr[4];
-  double s = 0.0;
+  double s = 0.0, s0 = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const double xv = xval[i];
     double low, upp;
-    const double xo1 = m.xold1[i], xo2 = m.xold2[i];
-    if (iter < 3) {
+    if (!asymp) {
+      low = m.low[i]; upp = m.upp[i];
+    } else if (iter < 3) {
       low = xv - asyminit * (xmax - xmin);
       upp = xv + asyminit * (xmax - xmin);
     } else {
       low = m.low[i]; upp = m.upp[i];
+      const double xo1 = m.xold1[i], xo2 = m.xold2[i];
       const double zzz = (xv - xo1) * (xo1 - xo2);
       const double gamma = zzz < 0.0 ? asymdec : (zzz > 0.0 ? asyminc : 1.0);
       low = xv - gamma * (xo1 - low);
@@ -720,38 +726,77 @@ __global__ void __launch_bounds__(256) k_mma_gensub(long long n, int iter, MMAVe
       if (xv < xmi) { low = xv - (xma - xv) / 0.9; upp = xv + (xma - xv) / 0.9; }
       if (xv > xma) { low = xv - (xv - xmi) / 0.9; upp = xv + (xv - xmi) / 0.9; }
     }
-    m.low[i] = low; m.upp[i] = upp;
+    if (asymp) {
+      m.low[i] = low; m.upp[i] = upp;
+      m.xold2[i] = m.xold1[i];
+      m.xold1[i] = xv;
+    }
     double al = fmax(xmin, low + albefa * (xv - low));
     al = fmax(al, xv - move * (xmax - xmin));
-    m.alpha[i] = fmin(al, xmax);
+    m.alpha[i] = clamp ? fmin(al, xmax) : al;
     double be = fmin(xmax, upp - albefa * (upp - xv));
     be = fmin(be, xv + move * (xmax - xmin));
-    m.beta[i] = fmax(be, xmin);
+    m.beta[i] = clamp ? fmax(be, xmin) : be;
     const double xmamiinv = 1.0 / fmax(xmamieps, xmax - xmin);
     const double df = dfdx[i];
     const double pq = 0.001 * fabs(df) + raa0 * xmamiinv;
     const double u2 = pow(upp - xv, 2.0), l2 = pow(xv - low, 2.0);
-    m.p0[i] = u2 * (fmax(0.0, df) + pq);
-    m.q0[i] = l2 * (fmax(0.0, -1.0 * df) + pq);
+    const double p0 = u2 * (fmax(0.0, df) + pq), q0 = l2 * (fmax(0.0, -1.0 * df) + pq);
+    m.p0[i] = p0; m.q0[i] = q0;
     const double dg = dgdx[i];
-    const double pqj = 0.001 * fabs(dg) + raa0 * xmamiinv;
+    const double pqj = 0.001 * fabs(dg) + raa1 * xmamiinv;
     const double p1 = u2 * (fmax(0.0, dg) + pqj), q1 = l2 * (fmax(0.0, -1.0 * dg) + pqj);
     m.p1[i] = p1; m.q1[i] = q1;
+    s0 += p0 / (upp - xv) + q0 / (xv - low);
     s += p1 / (upp - xv) + q1 / (xv - low);
-    m.xold2[i] = xo1;
-    m.xold1[i] = xv;
   }
+  s0 = block_sum<256>(s0, scr);
   s = block_sum<256>(s, scr);
-  if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) {
+    red.partial[blockIdx.x] = s0;
+    red.partial[RED_MAXBLOCKS + blockIdx.x] = s;
+  }
 }
-void launch_mma_gensub(long long n, int iter, const MMAVecs &m, const double *xval,
+void launch_mma_gensub(long long n, int iter, int asymp, int clamp, const MMAVecs &m, const double *xval,
                        const double *dfdx, const double *dgdx, double xmin, double xmax,
                        double move, double albefa, double asyminit, double asymdec, double asyminc,
-                       double raa0, double xmamieps, double *bsum, Reduce red, hipStream_t s) {
+                       double raa0, double raa1, double xmamieps, double *sums, Reduce red, hipStream_t s) {
   const int nb = ew_blocks(n, 4, RED_BLOCKS);
-  hipLaunchKernelGGL(k_mma_gensub, dim3(nb), dim3(256), 0, s, n, iter, m, xval, dfdx, dgdx, xmin, xmax, move,
-                     albefa, asyminit, asymdec, asyminc, raa0, xmamieps, red);
-  launch_finalize(red, nb, 1, bsum, s);
+  hipLaunchKernelGGL(k_mma_gensub, dim3(nb), dim3(256), 0, s, n, iter, asymp, clamp, m, xval, dfdx, dgdx, xmin,
+                     xmax, move, albefa, asyminit, asymdec, asyminc, raa0, raa1, xmamieps, red);
+  launch_finalize(red, nb, 2, sums, s);
+}
+
+// GCMMA ComputeApprox and RaaUpdate's raacof (GCMMASolver.cpp:598-620, 510-525) at xmma:
+// out[0] = sum p0/(U-x) + q0/(x-L), out[1] = the same with p1 / q1,
+// out[2] = sum (x-xval)^2 / ((U-x)(x-L)) * (U-L) / max(xmamieps, xmax-xmin)
+__global__ void __launch_bounds__(256) k_gcmma_approx(long long n, MMAVecs m, const double *xmma,
+                                                      const double *xval, double xmami, Reduce red) {
+  __shared__ double scr[4];
+  double a0 = 0.0, a1 = 0.0, rc = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double x = xmma[i], xv = xval[i], low = m.low[i], upp = m.upp[i];
+    const double uxinv = 1.0 / (upp - x), xlinv = 1.0 / (x - low);
+    a0 += m.p0[i] * uxinv + m.q0[i] * xlinv;
+    a1 += m.p1[i] * uxinv + m.q1[i] * xlinv;
+    const double xxux = (x - xv) / (upp - x), xxxl = (x - xv) / (x - low);
+    rc += xxux * xxxl * ((upp - low) / xmami);
+  }
+  a0 = block_sum<256>(a0, scr);
+  a1 = block_sum<256>(a1, scr);
+  rc = block_sum<256>(rc, scr);
+  if (threadIdx.x == 0) {
+    red.partial[blockIdx.x] = a0;
+    red.partial[RED_MAXBLOCKS + blockIdx.x] = a1;
+    red.partial[2 * RED_MAXBLOCKS + blockIdx.x] = rc;
+  }
+}
+void launch_gcmma_approx(long long n, const MMAVecs &m, const double *xmma, const double *xval, double xmami,
+                         double *out, Reduce red, hipStream_t s) {
+  const int nb = ew_blocks(n, 4, RED_BLOCKS);
+  hipLaunchKernelGGL(k_gcmma_approx, dim3(nb), dim3(256), 0, s, n, m, xmma, xval, xmami, red);
+  launch_finalize(red, nb, 3, out, s);
 }
 
 __global__ void __launch_bounds__(256) k_mma_dual(long long n, double lam, MMAVecs m, double *x, Reduce red) {

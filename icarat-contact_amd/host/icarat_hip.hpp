@@ -457,4 +457,45 @@ class MMASolver {
   int n_;
 };
 
+// GCMMASolver (src/optimization/opt-solver/GCMMASolver.hpp:61-145) with m = 1, a = 0,
+// c = 1000 (the driver's use, opt_macro_gcmma.cpp:69); the solver state (asymptotes,
+// x history, raa0 / raa, the approximations' values) lives with the model's job.
+class GCMMASolver {
+ public:
+  GCMMASolver(StructuredModel &model, int n, int m = 1, double a = 0.0, double c = 1000.0, double d = 0.0)
+      : model_(model), n_(n) {
+    if (m != 1 || a != 0.0 || c != 1000.0 || d != 0.0)
+      StructuredModel::fatal("GCMMASolver: the GPU path implements m = 1, a = 0, c = 1000");
+  }
+  void OuterUpdate(double *xmma, const double *xval, double f0x, const double *df0dx, const double *fx,
+                   const double *dfdx, const double *xmin, const double *xmax) {
+    uniform(xmin, xmax);
+    int steps = 0;
+    check(model_.ctx(), icx_gcmma_outer_update(model_.job(), xmma, xval, f0x, df0dx, fx[0], dfdx, xmin[0], xmax[0],
+                                               &steps), "icx_gcmma_outer_update");
+  }
+  void InnerUpdate(double *xmma, double f0xnew, const double *fxnew, const double *xval, double f0x,
+                   const double *df0dx, const double *fx, const double *dfdx, const double *xmin,
+                   const double *xmax) {
+    uniform(xmin, xmax);
+    int steps = 0;
+    check(model_.ctx(), icx_gcmma_inner_update(model_.job(), xmma, f0xnew, fxnew[0], xval, f0x, df0dx, fx[0], dfdx,
+                                               xmin[0], xmax[0], &steps), "icx_gcmma_inner_update");
+  }
+  bool ConCheck(double f0xnew, const double *fxnew) const {
+    int ok = 0;
+    check(model_.ctx(), icx_gcmma_concheck(model_.job(), f0xnew, fxnew[0], &ok), "icx_gcmma_concheck");
+    return ok != 0;
+  }
+  void Reset() { icx_gcmma_reset(model_.job()); }
+
+ private:
+  void uniform(const double *xmin, const double *xmax) const {
+    for (int i = 1; i < n_; ++i)
+      if (xmin[i] != xmin[0] || xmax[i] != xmax[0]) StructuredModel::fatal("GCMMASolver: xmin/xmax must be uniform");
+  }
+  StructuredModel &model_;
+  int n_;
+};
+
 }  // namespace icarat

@@ -29,6 +29,7 @@ EXPORTS = [
     "icx_mises", "icx_reactions", "icx_output_begin", "icx_output_write", "icx_write_vtk",
     "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
     "icx_project", "icx_oc_update", "icx_mma_reset", "icx_mma_update",
+    "icx_gcmma_reset", "icx_gcmma_outer_update", "icx_gcmma_inner_update", "icx_gcmma_concheck",
     "icx_opt_init", "icx_opt_step", "icx_opt_step_begin", "icx_opt_step_end", "icx_opt_get_design",
     "icx_submit", "icx_poll", "icx_job_last_info", "icx_solve_batch", "icx_job_footprint",
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
@@ -84,7 +85,8 @@ class OptParams(C.Structure):
 class IterRecord(C.Structure):
     _fields_ = [("step", C.c_int), ("objective", C.c_double), ("constraint", C.c_double),
                 ("volume", C.c_double), ("cg_iterations", C.c_int), ("update_steps", C.c_int),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("inner_loops", C.c_int),
+                ("trial_cg_iterations", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -414,6 +416,35 @@ class Job:
         self.ctx.check(lib().icx_mma_update(self.h, _dp(x), _dp(df), C.c_double(gx), _dp(dg),
                                             C.c_double(xmin), C.c_double(xmax), C.byref(n)))
         return x, n.value
+
+    def gcmma_reset(self):
+        self.ctx.check(lib().icx_gcmma_reset(self.h))
+
+    def gcmma_outer(self, xval, f0x, df0dx, fx, dfdx, xmin=1e-4, xmax=1.0):
+        """GCMMASolver::OuterUpdate (m = 1): returns (xmma, dual Newton steps)."""
+        xmma = np.zeros(self.nelx)
+        xv, df, dg = _f64(xval), _f64(df0dx), _f64(dfdx)
+        n = C.c_int()
+        self.ctx.check(lib().icx_gcmma_outer_update(
+            self.h, _dp(xmma), _dp(xv), C.c_double(f0x), _dp(df), C.c_double(fx), _dp(dg),
+            C.c_double(xmin), C.c_double(xmax), C.byref(n)))
+        return xmma, n.value
+
+    def gcmma_inner(self, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin=1e-4, xmax=1.0):
+        """GCMMASolver::InnerUpdate (m = 1): xmma is the last returned point."""
+        xm = _f64(xmma).copy()
+        xv, df, dg = _f64(xval), _f64(df0dx), _f64(dfdx)
+        n = C.c_int()
+        self.ctx.check(lib().icx_gcmma_inner_update(
+            self.h, _dp(xm), C.c_double(f0xnew), C.c_double(fxnew), _dp(xv), C.c_double(f0x),
+            _dp(df), C.c_double(fx), _dp(dg), C.c_double(xmin), C.c_double(xmax), C.byref(n)))
+        return xm, n.value
+
+    def gcmma_concheck(self, f0xnew, fxnew):
+        ok = C.c_int()
+        self.ctx.check(lib().icx_gcmma_concheck(self.h, C.c_double(f0xnew), C.c_double(fxnew),
+                                                C.byref(ok)))
+        return bool(ok.value)
 
     # -- device-resident design loop ------------------------------------------------
     def opt_init(self, x0, driver=0, filter=FILTER_CONE, radius=1.5, projection=True,

@@ -229,13 +229,32 @@ icx_status icx_oc_update(icx_job *job, double *x, const double *dfdx, double g,
 icx_status icx_mma_reset(icx_job *job);
 icx_status icx_mma_update(icx_job *job, double *x, const double *dfdx, double gx,
                           const double *dgdx, double xmin, double xmax, int *newton_steps);
+/* GCMMASolver(n, m = 1, a = 0, c = 1000, d = 0) (src/optimization/opt-solver/
+ * GCMMASolver.hpp:61-145, GCMMASolver.cpp:41-661) with its state on the device; xmin /
+ * xmax are scalars.  A job's first use starts from raa0 = raa = 1; icx_gcmma_reset is
+ * Reset() (the outer counter only, :88).
+ * outer: OuterUpdate(xmma, xval, f0x, df0dx, fx, dfdx, xmin, xmax) (:66-90).
+ * inner: InnerUpdate(xmma, f0xnew, fxnew, xval, ...) (:92-113); xmma holds the point
+ *   the last update returned (where f0xnew / fxnew were evaluated) and is overwritten.
+ * concheck: ConCheck(f0xnew, fxnew) (:115-129), 1 = conservative. */
+icx_status icx_gcmma_reset(icx_job *job);
+icx_status icx_gcmma_outer_update(icx_job *job, double *xmma, const double *xval, double f0x,
+                                  const double *df0dx, double fx, const double *dfdx,
+                                  double xmin, double xmax, int *newton_steps);
+icx_status icx_gcmma_inner_update(icx_job *job, double *xmma, double f0xnew, double fxnew,
+                                  const double *xval, double f0x, const double *df0dx,
+                                  double fx, const double *dfdx, double xmin, double xmax,
+                                  int *newton_steps);
+icx_status icx_gcmma_concheck(const icx_job *job, double f0xnew, double fxnew, int *conservative);
 
-/* ---- device-resident design loops (drivers A and B) ---------------------------
+/* ---- device-resident design loops ------------------------------------------------
  * One design iteration of opt-macro-oc/opt_macro_oc.cpp:84-172 (driver 0: cone
- * filter + OC) or opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163
- * (driver 1: Helmholtz + MMA), with every vector kept on the device. */
+ * filter + OC), opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163 (driver 1:
+ * Helmholtz + MMA) or opt_macro_gcmma.cpp:83-216 (driver 2: cone filter + GCMMA, with
+ * beta = min(int(max((k-100)/40, 1)), 8), const_h = V - V0 and up to 15 inner loops,
+ * each solving a trial design), with every vector kept on the device. */
 typedef struct {
-  int driver;          /* 0 = OC driver (A), 1 = MMA + Helmholtz driver (B) */
+  int driver;          /* 0 = OC driver (A), 1 = MMA + Helmholtz driver (B), 2 = GCMMA */
   int filter;          /* ICX_FILTER_CONE / ICX_FILTER_HELMHOLTZ */
   double filter_radius;
   int projection;      /* 1: beta = min(2^(k/40), 16) with tanh projection */
@@ -252,8 +271,10 @@ typedef struct {
   double constraint;   /* const_h[0] as the driver stores it */
   double volume;       /* sum rho_hat v */
   int cg_iterations;
-  int update_steps;    /* OC bisections or MMA Newton steps */
+  int update_steps;    /* OC bisections or MMA / GCMMA dual Newton steps */
   double seconds;      /* wall time of the design iteration */
+  int inner_loops;     /* GCMMA inner updates this step */
+  int trial_cg_iterations; /* GCMMA: PCG iterations of the trial-design solves */
 } icx_iter_record;
 
 icx_status icx_opt_init(icx_job *job, const icx_opt_params *par, const double *x0);

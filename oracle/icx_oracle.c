@@ -1349,6 +1349,10 @@ struct icxo_mma {
   double *low, *upp, *alpha, *beta, *p0, *q0, *pij, *qij, *b, *grad, *hess;
   double *xold1, *xold2;
   int last_outloop;
+  /* GCMMA only (GCMMASolver.hpp:91-114) */
+  int gcmma;
+  double raa0eps, raaeps, r0, f0app;
+  double *raa, *r, *fapp;
 };
 
 icxo_mma *icxo_mma_create(int n, int m, double ai, double ci, double di)
@@ -1376,14 +1380,13 @@ void icxo_mma_destroy(icxo_mma *M)
   free(M->a); free(M->c); free(M->d); free(M->y); free(M->lam); free(M->mu); free(M->s);
   free(M->low); free(M->upp); free(M->alpha); free(M->beta); free(M->p0); free(M->q0);
   free(M->pij); free(M->qij); free(M->b); free(M->grad); free(M->hess);
-  free(M->xold1); free(M->xold2); free(M);
+  free(M->xold1); free(M->xold2); free(M->raa); free(M->r); free(M->fapp); free(M);
 }
 
-static void mma_gensub(icxo_mma *M, const double *xval, const double *dfdx,
-                       const double *gx, const double *dgdx, const double *xmin,
-                       const double *xmax)
+/* asymptotes L, U from the x history (MMASolver.cpp:408-463, GCMMASolver.cpp:432-489) */
+static void mma_asymptotes(icxo_mma *M, const double *xval, const double *xmin, const double *xmax)
 {
-  const int n = M->n, m = M->m;
+  const int n = M->n;
   M->iter++;
   if (M->iter < 3) {
     for (int i = 0; i < n; i++) {
@@ -1412,6 +1415,14 @@ static void mma_gensub(icxo_mma *M, const double *xval, const double *dfdx,
       }
     }
   }
+}
+
+static void mma_gensub(icxo_mma *M, const double *xval, const double *dfdx,
+                       const double *gx, const double *dgdx, const double *xmin,
+                       const double *xmax)
+{
+  const int n = M->n, m = M->m;
+  mma_asymptotes(M, xval, xmin, xmax);
   for (int i = 0; i < n; i++) {
     double al = fmax(xmin[i], M->low[i] + M->albefa * (xval[i] - M->low[i]));
     al = fmax(al, xval[i] - M->move * (xmax[i] - xmin[i]));
@@ -1605,6 +1616,155 @@ int icxo_mma_update(icxo_mma *M, double *x, const double *dfdx, const double *gx
 }
 
 /* ------------------------------------------------------------------------ */
+/* GCMMA (m constraints): src/optimization/opt-solver/GCMMASolver.cpp:41-620. */
+/* The subproblem and its dual interior point solve (SolveDIP and helpers,   */
+/* :135-429) are those of MMA above; GCMMA adds the conservative raa0 / raa  */
+/* terms, the constants r0 / r of the approximations and their values at the */
+/* new point (ComputeApprox), which ConCheck compares with the true values.  */
+/* ------------------------------------------------------------------------ */
+icxo_mma *icxo_gcmma_create(int n, int m, double ai, double ci, double di)
+{
+  icxo_mma *M = icxo_mma_create(n, m, ai, ci, di); /* move 0.15, albefa 0.2, asym 0.2/0.7/1.2 */
+  M->gcmma = 1;
+  M->raa0eps = 1e-6; M->raaeps = 1e-6; M->raa0 = 1.0;
+  M->raa = malloc(m * sizeof(double)); M->r = calloc(m, sizeof(double));
+  M->fapp = calloc(m, sizeof(double));
+  for (int j = 0; j < m; j++) M->raa[j] = 1.0;
+  return M;
+}
+
+/* GenSub (GCMMASolver.cpp:546-596): alpha / beta without the MMA clamps, raa0 and raa[j]
+ * in the p / q coefficients, r0 = f0x - sum(p0/(U-x) + q0/(x-L)), r = fx - ..., b = -r */
+static void gcmma_gensub(icxo_mma *M, const double *xval, double f0x, const double *df0dx,
+                         const double *fx, const double *dfdx, const double *xmin,
+                         const double *xmax)
+{
+  const int n = M->n, m = M->m;
+  M->r0 = 0.0;
+  for (int j = 0; j < m; j++) M->r[j] = 0.0;
+  for (int i = 0; i < n; i++) {
+    M->alpha[i] = fmax(xmin[i], M->low[i] + M->albefa * (xval[i] - M->low[i]));
+    M->alpha[i] = fmax(M->alpha[i], xval[i] - M->move * (xmax[i] - xmin[i]));
+    M->beta[i] = fmin(xmax[i], M->upp[i] - M->albefa * (M->upp[i] - xval[i]));
+    M->beta[i] = fmin(M->beta[i], xval[i] + M->move * (xmax[i] - xmin[i]));
+    const double uxinv = 1.0 / (M->upp[i] - xval[i]), xlinv = 1.0 / (xval[i] - M->low[i]);
+    const double xmamiinv = 1.0 / fmax(M->xmamieps, xmax[i] - xmin[i]);
+    const double pq = 0.001 * fabs(df0dx[i]) + M->raa0 * xmamiinv;
+    M->p0[i] = pow(M->upp[i] - xval[i], 2.0) * (fmax(0.0, df0dx[i]) + pq);
+    M->q0[i] = pow(xval[i] - M->low[i], 2.0) * (fmax(0.0, -1.0 * df0dx[i]) + pq);
+    M->r0 += M->p0[i] * uxinv + M->q0[i] * xlinv;
+    for (int j = 0; j < m; j++) {
+      const double g = dfdx[i * m + j];
+      const double pqj = 0.001 * fabs(g) + M->raa[j] * xmamiinv;
+      M->pij[i * m + j] = pow(M->upp[i] - xval[i], 2.0) * (fmax(0.0, g) + pqj);
+      M->qij[i * m + j] = pow(xval[i] - M->low[i], 2.0) * (fmax(0.0, -1.0 * g) + pqj);
+      M->r[j] += M->pij[i * m + j] * uxinv + M->qij[i * m + j] * xlinv;
+    }
+  }
+  M->r0 = f0x - M->r0;
+  for (int j = 0; j < m; j++) {
+    M->r[j] = fx[j] - M->r[j];
+    M->b[j] = -M->r[j];
+  }
+}
+
+/* ComputeApprox (GCMMASolver.cpp:598-620) */
+static void gcmma_approx(icxo_mma *M, const double *xmma)
+{
+  const int n = M->n, m = M->m;
+  M->f0app = 0.0;
+  for (int j = 0; j < m; j++) M->fapp[j] = 0.0;
+  for (int i = 0; i < n; i++) {
+    const double uxinv = 1.0 / (M->upp[i] - xmma[i]), xlinv = 1.0 / (xmma[i] - M->low[i]);
+    M->f0app += M->p0[i] * uxinv + M->q0[i] * xlinv;
+    for (int j = 0; j < m; j++) M->fapp[j] += M->pij[i * m + j] * uxinv + M->qij[i * m + j] * xlinv;
+  }
+  M->f0app += M->r0;
+  for (int j = 0; j < m; j++) M->fapp[j] += M->r[j];
+}
+
+/* OuterUpdate (GCMMASolver.cpp:66-90): Asymp (asymptotes, then raa0 / raa decay to
+ * max(eps, 0.1 raa), :432-507), GenSub, x history, dual solve from xval, ComputeApprox */
+int icxo_gcmma_outer(icxo_mma *M, double *xmma, const double *xval, double f0x,
+                     const double *df0dx, const double *fx, const double *dfdx,
+                     const double *xmin, const double *xmax)
+{
+  mma_asymptotes(M, xval, xmin, xmax);
+  M->raa0 = fmax(M->raa0eps, 0.1 * M->raa0);
+  for (int j = 0; j < M->m; j++) M->raa[j] = fmax(M->raaeps, 0.1 * M->raa[j]);
+  gcmma_gensub(M, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+  memcpy(M->xold2, M->xold1, (size_t)M->n * 8);
+  memcpy(M->xold1, xval, (size_t)M->n * 8);
+  memcpy(xmma, xval, (size_t)M->n * 8);
+  const int it = mma_solve_dip(M, xmma);
+  gcmma_approx(M, xmma);
+  return it;
+}
+
+/* InnerUpdate (GCMMASolver.cpp:92-113) with RaaUpdate (:510-544): raa0 / raa grow where
+ * the approximation at the last xmma fell below the true value; asymptotes and the x
+ * history stay */
+int icxo_gcmma_inner(icxo_mma *M, double *xmma, double f0xnew, const double *fxnew,
+                     const double *xval, double f0x, const double *df0dx, const double *fx,
+                     const double *dfdx, const double *xmin, const double *xmax)
+{
+  double raacof = 0.0;
+  for (int i = 0; i < M->n; i++) {
+    const double xmami = fmax(M->xmamieps, xmax[i] - xmin[i]);
+    const double xxux = (xmma[i] - xval[i]) / (M->upp[i] - xmma[i]);
+    const double xxxl = (xmma[i] - xval[i]) / (xmma[i] - M->low[i]);
+    raacof += xxux * xxxl * ((M->upp[i] - M->low[i]) / xmami);
+  }
+  raacof = fmax(1e-12, raacof);
+  if (f0xnew > M->f0app + 0.5 * M->epsimin) {
+    const double d0 = (1.0 / raacof) * (f0xnew - M->f0app);
+    M->raa0 = fmin(1.1 * (M->raa0 + d0), 10.0 * M->raa0);
+  }
+  for (int j = 0; j < M->m; j++)
+    if (fxnew[j] > M->fapp[j] + 0.5 * M->epsimin) {
+      const double dj = (1.0 / raacof) * (fxnew[j] - M->fapp[j]);
+      M->raa[j] = fmin(1.1 * (M->raa[j] + dj), 10.0 * M->raa[j]);
+    }
+  gcmma_gensub(M, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+  memcpy(xmma, xval, (size_t)M->n * 8);
+  const int it = mma_solve_dip(M, xmma);
+  gcmma_approx(M, xmma);
+  return it;
+}
+
+/* ConCheck (GCMMASolver.cpp:115-129) */
+int icxo_gcmma_concheck(const icxo_mma *M, double f0xnew, const double *fxnew)
+{
+  if (M->f0app + M->epsimin < f0xnew) return 0;
+  for (int j = 0; j < M->m; j++)
+    if (M->fapp[j] + M->epsimin < fxnew[j]) return 0;
+  return 1;
+}
+
+/* the GCMMA approximations (the subproblem's functions) at any x (test access) */
+void icxo_gcmma_approx_at(const icxo_mma *M, const double *x, double *f0, double *fj)
+{
+  double s0 = 0.0;
+  for (int j = 0; j < M->m; j++) fj[j] = M->r[j];
+  for (int i = 0; i < M->n; i++) {
+    const double uxinv = 1.0 / (M->upp[i] - x[i]), xlinv = 1.0 / (x[i] - M->low[i]);
+    s0 += M->p0[i] * uxinv + M->q0[i] * xlinv;
+    for (int j = 0; j < M->m; j++) fj[j] += M->pij[i * M->m + j] * uxinv + M->qij[i * M->m + j] * xlinv;
+  }
+  *f0 = s0 + M->r0;
+}
+
+void icxo_gcmma_state(const icxo_mma *M, double *f0app, double *fapp, double *raa0, double *raa)
+{
+  if (f0app) *f0app = M->f0app;
+  if (raa0) *raa0 = M->raa0;
+  for (int j = 0; j < M->m; j++) {
+    if (fapp) fapp[j] = M->fapp[j];
+    if (raa) raa[j] = M->raa[j];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* driver loops                                                               */
 /* ------------------------------------------------------------------------ */
 typedef struct {
@@ -1625,6 +1785,29 @@ static double oc_volume(const double *x, void *user)
   return g;
 }
 
+/* objective and constraint of a trial design of the GCMMA driver
+ * (opt_macro_gcmma.cpp:158-175, 192-207): filter, threshold, solve, f = u.fext,
+ * g = sum rho_hat v - V0 */
+static int gcmma_trial(const icxo_driver_cfg *g, icxo_sys *s, icxo_filter *flt, const double *xnew,
+                       double beta, double V0, double *fil, double *thr, double *fint, double *rhs,
+                       double *u, double *f, double *gv)
+{
+  const int nelx = g->nelx, numeq = g->numeq;
+  icxo_filter_apply(flt, xnew, fil, 0);
+  if (g->projection) icxo_threshold(nelx, fil, beta, 0.5, thr);
+  else memcpy(thr, fil, (size_t)nelx * 8);
+  icxo_sys_assemble(s, thr, g->has, g->node_flag, g->node_val, fint);
+  for (int i = 0; i < numeq; i++) rhs[i] = g->fext[i] - fint[i];
+  if (g->direct) { if (icxo_sys_direct(s, rhs, u) != 0) return -1; }
+  else { double err; icxo_sys_cg(s, rhs, u, g->cg_tol, g->cg_maxit, &err); }
+  double o = 0.0, v = 0.0;
+  for (int i = 0; i < numeq; i++) o += u[i] * g->fext[i];
+  for (int i = 0; i < nelx; i++) v += thr[i] * s->volume[i];
+  *f = o;
+  *gv = v - V0;
+  return 0;
+}
+
 int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
                     double *vol_hist, double *last_dfds, double *last_u, int *cg_hist)
 {
@@ -1635,25 +1818,32 @@ int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
   icxo_filter *flt = NULL;
   icxo_helm *hlm = NULL;
   icxo_mma *mma = NULL;
-  if (g->driver == 0) flt = icxo_filter_create(g->ne, g->ndim, nelx, g->x, g->conn, g->filter_radius, 0);
+  const int cone = g->driver != 1;
+  if (cone) flt = icxo_filter_create(g->ne, g->ndim, nelx, g->x, g->conn, g->filter_radius, 0);
   else {
     hlm = icxo_helm_create(g->ne, g->ndim, g->numnp, nelx, g->x, g->conn, g->filter_radius);
     mma = icxo_mma_create(nelx, 1, 0.0, 100000.0, 0.0);
   }
+  if (g->driver == 2) mma = icxo_gcmma_create(nelx, 1, 0.0, 1000.0, 0.0);
   size_t E = (size_t)nelx * sizeof(double);
   double *fil = malloc(E), *thr = malloc(E), *dthr = malloc(E), *dfds = malloc(E),
          *dgds = malloc(E), *energy = malloc(E), *tmp = malloc(E), *tmp2 = malloc(E),
          *xmin = malloc(E), *xmax = malloc(E);
   double *fint = malloc((size_t)numeq * 8), *rhs = malloc((size_t)numeq * 8),
          *u = malloc((size_t)numeq * 8), *nodeval = malloc((size_t)g->numnp * 3 * 8);
+  double *xnew = malloc(E), *u2 = malloc((size_t)numeq * 8);
   for (int i = 0; i < nelx; i++) { xmin[i] = g->xmin; xmax[i] = g->xmax; }
   double V0 = 0.0, gcon = 0.0;
   int rc = 0;
   for (int k = 1; k <= g->iterations; k++) {
-    if (g->driver == 0) icxo_filter_apply(flt, design, fil, 0);
+    if (cone) icxo_filter_apply(flt, design, fil, 0);
     else icxo_helm_apply(hlm, design, fil, 0, g->helm_tol, 10000, g->helm_direct);
     double beta = pow(2, (int)(k / 40));
     if (beta > 16.0) beta = 16.0;
+    if (g->driver == 2) { /* opt_macro_gcmma.cpp:91-92 */
+      const int bt = (int)fmax((k - 100) / 40.0, 1.0);
+      beta = fmin((double)bt, 8.0);
+    }
     if (g->projection) {
       icxo_threshold(nelx, fil, beta, T, thr);
       icxo_threshold_derivative(nelx, fil, beta, T, dthr);
@@ -1676,8 +1866,10 @@ int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
       gcon = (g->driver == 0) ? V0 : 0.0;
     } else {
       gcon = 0.0;
-      if (g->driver == 0) for (int i = 0; i < nelx; i++) gcon += thr[i] * s->volume[i];
-      else {
+      if (g->driver != 1) {
+        for (int i = 0; i < nelx; i++) gcon += thr[i] * s->volume[i];
+        if (g->driver == 2) gcon -= V0;
+      } else {
         for (int i = 0; i < nelx; i++) gcon += thr[i] * s->volume[i] / V0;
         gcon -= 1;
       }
@@ -1685,10 +1877,10 @@ int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
     obj_hist[k - 1] = obj;
     if (vol_hist) vol_hist[k - 1] = gcon;
     icxo_sys_element_energy(s, nodeval, energy);
-    icxo_sensitivity(nelx, g->driver, g->young1, g->young2, g->p, thr, energy, s->volume,
+    icxo_sensitivity(nelx, g->driver == 1, g->young1, g->young2, g->p, thr, energy, s->volume,
                      V0, dfds, dgds);
     for (int i = 0; i < nelx; i++) { dfds[i] *= dthr[i]; dgds[i] *= dthr[i]; }
-    if (g->driver == 0) {
+    if (cone) {
       icxo_filter_apply(flt, dfds, tmp, 1); memcpy(dfds, tmp, E);
       icxo_filter_apply(flt, dgds, tmp, 1); memcpy(dgds, tmp, E);
     } else {
@@ -1702,13 +1894,27 @@ int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
     if (g->driver == 0) {
       oc_vol_ctx c = {flt, nelx, beta, T, g->projection, s->volume, tmp, tmp2};
       icxo_oc_update(nelx, design, dfds, V0, dgds, xmin, xmax, g->move, oc_volume, &c);
-    } else {
+    } else if (g->driver == 1) {
       double gx[1] = {gcon};
       icxo_mma_update(mma, design, dfds, gx, dgds, xmin, xmax);
+    } else { /* opt_macro_gcmma.cpp:150-215 */
+      double gx[1] = {gcon}, fnew, gnew[1];
+      icxo_gcmma_outer(mma, xnew, design, obj, dfds, gx, dgds, xmin, xmax);
+      if (gcmma_trial(g, s, flt, xnew, beta, V0, tmp, tmp2, fint, rhs, u2, &fnew, gnew)) { rc = -1; break; }
+      int conserv = icxo_gcmma_concheck(mma, fnew, gnew), inner = 0;
+      for (; !conserv && inner < 15; ++inner) {
+        icxo_gcmma_inner(mma, xnew, fnew, gnew, design, obj, dfds, gx, dgds, xmin, xmax);
+        if (gcmma_trial(g, s, flt, xnew, beta, V0, tmp, tmp2, fint, rhs, u2, &fnew, gnew)) { rc = -1; break; }
+        conserv = icxo_gcmma_concheck(mma, fnew, gnew);
+      }
+      if (rc) break;
+      if (g->inner_hist) g->inner_hist[k - 1] = inner;
+      memcpy(design, xnew, E);
     }
   }
   free(fil); free(thr); free(dthr); free(dfds); free(dgds); free(energy); free(tmp);
   free(tmp2); free(xmin); free(xmax); free(fint); free(rhs); free(u); free(nodeval);
+  free(xnew); free(u2);
   icxo_sys_destroy(s); icxo_filter_destroy(flt); icxo_helm_destroy(hlm); icxo_mma_destroy(mma);
   return rc;
 }

@@ -107,12 +107,26 @@ icxo_mma *icxo_mma_create(int n, int m, double a, double c, double d);
 void icxo_mma_destroy(icxo_mma *m);
 int icxo_mma_update(icxo_mma *m, double *x, const double *dfdx, const double *gx,
                     const double *dgdx, const double *xmin, const double *xmax);
+/* GCMMASolver(n, m, a=0, c=1000, d=0): OuterUpdate, InnerUpdate, ConCheck
+ * (src/optimization/opt-solver/GCMMASolver.cpp:41-129); destroy with icxo_mma_destroy */
+icxo_mma *icxo_gcmma_create(int n, int m, double a, double c, double d);
+int icxo_gcmma_outer(icxo_mma *m, double *xmma, const double *xval, double f0x,
+                     const double *df0dx, const double *fx, const double *dfdx,
+                     const double *xmin, const double *xmax);
+int icxo_gcmma_inner(icxo_mma *m, double *xmma, double f0xnew, const double *fxnew,
+                     const double *xval, double f0x, const double *df0dx, const double *fx,
+                     const double *dfdx, const double *xmin, const double *xmax);
+int icxo_gcmma_concheck(const icxo_mma *m, double f0xnew, const double *fxnew);
+void icxo_gcmma_approx_at(const icxo_mma *m, const double *x, double *f0, double *fj);
+void icxo_gcmma_state(const icxo_mma *m, double *f0app, double *fapp, double *raa0, double *raa);
 
 /* ---- drivers ----------------------------------------------------------------
  * A: opt-macro-oc/opt_macro_oc.cpp:84-172  (cone filter, projection, CG, OC)
- * B: opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163 (Helmholtz, MMA) */
+ * B: opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163 (Helmholtz, MMA)
+ * 2: opt_macro_gcmma.cpp:83-216 (cone filter r, beta = min(int(max((k-100)/40, 1)), 8),
+ *    GCMMA with up to 15 inner loops, each re-solving a trial design) */
 typedef struct {
-  int driver;          /* 0 = A (OC + cone), 1 = B (MMA + Helmholtz) */
+  int driver;          /* 0 = A (OC + cone), 1 = B (MMA + Helmholtz), 2 = GCMMA + cone */
   int ne, ndim, dofnp, state;
   int numnp, nelx, numeq;
   const double *x; const int *conn; const int *dof;
@@ -127,6 +141,7 @@ typedef struct {
   double cg_tol; int cg_maxit;
   double helm_tol;     /* Helmholtz CG tolerance (reference 1e-10) */
   int helm_direct;
+  int *inner_hist;     /* driver 2: GCMMA inner loops per step (may be NULL) */
 } icxo_driver_cfg;
 
 int icxo_run_driver(const icxo_driver_cfg *cfg, double *design /* in: x0, out: final */,

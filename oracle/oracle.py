@@ -318,13 +318,15 @@ class Problem:
                         ("xmin", C.c_double), ("xmax", C.c_double), ("move", C.c_double),
                         ("iterations", C.c_int), ("projection", C.c_int), ("direct", C.c_int),
                         ("cg_tol", C.c_double), ("cg_maxit", C.c_int),
-                        ("helm_tol", C.c_double), ("helm_direct", C.c_int)]
+                        ("helm_tol", C.c_double), ("helm_direct", C.c_int),
+                        ("inner_hist", C.c_void_p)]
         h, f, v = self._bc_ptrs()
+        inner = np.zeros(iterations, dtype=np.int32)
         cfg = Cfg(driver, self.ne, self.ndim, self.dofnp, self.state, self.numnp, self.nelx,
                   self.numeq, self.x.ctypes.data, self.conn.ctypes.data, self.dof.ctypes.data,
                   h, f, v, self.fext.ctypes.data, self.young1, self.young2, self.nu, self.p,
                   radius, xmin, xmax, move, iterations, int(projection), int(direct), cg_tol,
-                  cg_maxit, helm_tol, int(helm_direct))
+                  cg_maxit, helm_tol, int(helm_direct), inner.ctypes.data)
         x = _f64(x0).copy()
         obj = np.zeros(iterations)
         vol = np.zeros(iterations)
@@ -336,7 +338,7 @@ class Problem:
         rc = L.icxo_run_driver(C.byref(cfg), x, obj, vol, dfds, u, its)
         if rc != 0:
             raise RuntimeError("driver failed %d" % rc)
-        return dict(design=x, obj=obj, vol=vol, dfds=dfds, u=u, cg_iters=its)
+        return dict(design=x, obj=obj, vol=vol, dfds=dfds, u=u, cg_iters=its, inner=inner)
 
 
 class ConeFilter:
@@ -422,6 +424,55 @@ class MMA:
         it = lib().icxo_mma_update(self.h, x, _f64(dfdx), _f64(np.atleast_1d(gx)), _f64(dgdx),
                                    _f64(xmin), _f64(xmax))
         return x, it
+
+
+class GCMMA:
+    """GCMMASolver(n, m, a=0, c=1000, d=0) (src/optimization/opt-solver/GCMMASolver.cpp:41-129)."""
+
+    def __init__(self, n, m=1, a=0.0, c=1000.0, d=0.0):
+        self.n, self.m = n, m
+        L = lib()
+        for name, res, args in (
+                ("icxo_gcmma_create", C.c_void_p, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]),
+                ("icxo_gcmma_outer", C.c_int, [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+                ("icxo_gcmma_inner", C.c_int, [C.c_void_p, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp,
+                                               _dp, _dp, _dp]),
+                ("icxo_gcmma_concheck", C.c_int, [C.c_void_p, C.c_double, _dp]),
+                ("icxo_gcmma_state", None, [C.c_void_p, _dp, _dp, _dp, _dp]),
+                ("icxo_gcmma_approx_at", None, [C.c_void_p, _dp, _dp, _dp])):
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        self.h = L.icxo_gcmma_create(n, m, a, c, d)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.icxo_mma_destroy(self.h)
+
+    def outer(self, xval, f0x, df0dx, fx, dfdx, xmin, xmax):
+        xmma = np.zeros(self.n)
+        it = lib().icxo_gcmma_outer(self.h, xmma, _f64(xval), float(f0x), _f64(df0dx),
+                                    _f64(np.atleast_1d(fx)), _f64(dfdx), _f64(xmin), _f64(xmax))
+        return xmma, it
+
+    def inner(self, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax):
+        xmma = _f64(xmma).copy()
+        it = lib().icxo_gcmma_inner(self.h, xmma, float(f0xnew), _f64(np.atleast_1d(fxnew)), _f64(xval),
+                                    float(f0x), _f64(df0dx), _f64(np.atleast_1d(fx)), _f64(dfdx),
+                                    _f64(xmin), _f64(xmax))
+        return xmma, it
+
+    def concheck(self, f0xnew, fxnew):
+        return bool(lib().icxo_gcmma_concheck(self.h, float(f0xnew), _f64(np.atleast_1d(fxnew))))
+
+    def approx_at(self, x):
+        f0, fj = np.zeros(1), np.zeros(self.m)
+        lib().icxo_gcmma_approx_at(self.h, _f64(x), f0, fj)
+        return f0[0], fj
+
+    def state(self):
+        f0, fa, r0, ra = np.zeros(1), np.zeros(self.m), np.zeros(1), np.zeros(self.m)
+        lib().icxo_gcmma_state(self.h, f0, fa, r0, ra)
+        return dict(f0app=f0[0], fapp=fa, raa0=r0[0], raa=ra)
 
 
 # -- the benchmark problems ----------------------------------------------------
