@@ -438,6 +438,7 @@ struct ConeWork {
 struct MMAWork {
   bool ready = false;
   int iter = 0;
+  double asyminit = 0.2, asymdec = 0.7, asyminc = 1.2;  // SetAsymptotes (MMASolver.hpp:71-76)
   DBuf<double> low, upp, alpha, beta, p0, q0, p1, q1, xold1, xold2;
 };
 // GCMMASolver state with m = 1 (GCMMASolver.hpp:91-114): the MMA vectors (iter is
@@ -1876,6 +1877,65 @@ icx_status icx_mises(icx_job *j, double *out) {
   return ICX_OK;
 }
 
+// The stress constraint of constraint/opt-stress-constraint/sensitivity.cpp:8-155 on the job's
+// current design and solve (quad4 box jobs): g, and with dgds one adjoint solve (plain FP64
+// PCG at tol) K lambda = sum_e Fe.  lambda takes the Dirichlet values at constrained DOFs
+// (numeqToElement, src/misc/assembling.hpp:271-298); the job's displacement is restored.
+static icx_status stress_constraint_dev(icx_job *j, double slim, double pn, double qp, double tol, int maxit,
+                                        double *g, double *dgds, int *adjoint_iterations) {
+  icx_ctx *c = j->ctx;
+  if (j->heat() || j->mesh || j->g.ndim != 2)
+    return fail(c, ICX_ERR_ARG, "the stress constraint is for 2D box jobs (the reference exits in 3D)");
+  if (!(j->prob.young2 > j->prob.young1)) return fail(c, ICX_ERR_ARG, "young2 need larger parameter than young1");
+  if (!j->u_valid) return fail(c, ICX_ERR_STATE, "no solution available");
+  if (!(slim > 0.0) || !(pn > 0.0)) return fail(c, ICX_ERR_ARG, "stress_limit and p_norm must be > 0");
+  const long long ne = j->g.ne, N = j->nvec();
+  DBuf<double> U, sig, vm, sqp, f12, fe, usave;
+  HIPCHK(c, U.alloc(N)); HIPCHK(c, sig.alloc(3 * ne)); HIPCHK(c, vm.alloc(ne)); HIPCHK(c, sqp.alloc(ne));
+  icx_status rc = output_displacement(j);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(U.p, j->ow.ufull.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  const MisesMat mm = mises_mat(j);
+  const StressPar sp{qp, pn, slim, j->prob.young1, j->prob.young2, j->prob.penalty};
+  launch_stress_eval(j->g, mm, sp, j->d_design.p, j->d_E.p, U.p, sig.p, vm.p, sqp.p, j->d_scalars.p, j->red(), S(j));
+  double psum;
+  rc = read_scalars(j, 1, &psum);
+  if (rc) return rc;
+  *g = std::pow(psum, 1.0 / pn) - 1.0;
+  if (!dgds) return ICX_OK;
+  HIPCHK(c, f12.alloc(ne)); HIPCHK(c, fe.alloc(8 * ne)); HIPCHK(c, usave.alloc(N));
+  HIPCHK(c, hipMemcpyAsync(usave.p, j->d_u.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  launch_stress_adjoint_load(j->g, mm, sp, std::pow(psum, (1.0 / pn) - 1.0), j->d_design.p, j->d_E.p, sig.p, vm.p,
+                             sqp.p, f12.p, fe.p, j->d_mask.p, j->d_b.p, S(j));
+  icx_solve_info info{};
+  rc = solve_plain(j, tol, maxit, info);
+  if (rc && rc != ICX_ERR_NOCONV) return rc;
+  if (adjoint_iterations) *adjoint_iterations = info.iterations;
+  rc = output_displacement(j);  // lambda with the Dirichlet values
+  if (rc) return rc;
+  launch_stress_sens(j->g, j->op64, sp, j->d_design.p, U.p, j->ow.ufull.p, f12.p, dgds, S(j));
+  HIPCHK(c, hipMemcpyAsync(j->d_u.p, usave.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  j->u_valid = true;
+  return ICX_OK;
+}
+
+icx_status icx_stress_constraint(icx_job *j, double stress_limit, double p_norm, double qp, double tol, int maxit,
+                                 double *g, double *dgds, int *adjoint_iterations) {
+  if (!j || !g) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  DBuf<double> dg;
+  if (dgds) HIPCHK(c, dg.alloc(j->g.ne));
+  icx_status rc = stress_constraint_dev(j, stress_limit, p_norm, qp, tol, maxit, g, dgds ? dg.p : nullptr,
+                                        adjoint_iterations);
+  if (rc) return rc;
+  if (dgds) {
+    HIPCHK(c, hipMemcpyAsync(dgds, dg.p, j->g.ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+    HIPCHK(c, hipStreamSynchronize(S(j)));
+  }
+  return ICX_OK;
+}
+
 icx_status icx_reactions(icx_job *j, double *out) {
   if (!j || !out) return ICX_ERR_ARG;
   if (!j->u_valid) return fail(j->ctx, ICX_ERR_STATE, "no solution available");
@@ -2464,8 +2524,8 @@ icx_status mma_update_dev(icx_job *j, double *dx, const double *dfdx, double gx,
   icx_status rc = mma_alloc(j, m);
   if (rc) return rc;
   const long long n = j->g.ne;
-  const double move = 0.15, albefa = 0.2, asyminit = 0.2, asymdec = 0.7, asyminc = 1.2, raa0 = 0.00001,
-               xmamieps = 1.0e-5;
+  const double move = 0.15, albefa = 0.2, raa0 = 0.00001, xmamieps = 1.0e-5;
+  const double asyminit = m.asyminit, asymdec = m.asymdec, asyminc = m.asyminc;
   const double epsimin = std::sqrt((double)(n + 1)) * 1e-9;
   m.iter++;
   MMAVecs v = mma_vecs(m);
@@ -2557,6 +2617,14 @@ icx_status icx_oc_update(icx_job *j, double *x, const double *dfdx, double g, co
 icx_status icx_mma_reset(icx_job *j) {
   if (!j) return ICX_ERR_ARG;
   j->mma.iter = 0;
+  return ICX_OK;
+}
+
+icx_status icx_mma_set_asymptotes(icx_job *j, double init, double decrease, double increase) {
+  if (!j) return ICX_ERR_ARG;
+  j->mma.asyminit = init;
+  j->mma.asymdec = decrease;
+  j->mma.asyminc = increase;
   return ICX_OK;
 }
 

@@ -299,6 +299,27 @@ __device__ __forceinline__ T energy_quad4(T (&v)[2][4], const ElasticOp<T> &op) 
   return e;
 }
 
+// u^T K0 w of one quad4 element (K0 symmetric, block diagonal in the Haar basis)
+template <typename T>
+__device__ __forceinline__ T bilinear_quad4(T (&u)[2][4], T (&w)[2][4], const ElasticOp<T> &op) {
+  haar_fwd4(u[0]); haar_fwd4(u[1]);
+  haar_fwd4(w[0]); haar_fwd4(w[1]);
+  T e = T(0);
+#pragma unroll
+  for (int chi = 0; chi < 4; ++chi) {
+    const T mu[2] = {u[0][chi ^ 1], u[1][chi ^ 2]}, mw[2] = {w[0][chi ^ 1], w[1][chi ^ 2]};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      T acc = T(0);
+#pragma unroll
+      for (int cc = 0; cc < 2; ++cc)
+        if (blk_nz2(chi, c, cc)) acc += op.blk[chi][c < cc ? c : cc][c < cc ? cc : c] * mw[cc];
+      e += mu[c] * acc;
+    }
+  }
+  return e;
+}
+
 // ---------------------------------------------------------------------------------
 // XCD-aware tile order: consecutive hardware block ids go round-robin over the 8
 // XCDs; remap so each XCD works on a contiguous run of tiles (shared halo lines

@@ -372,6 +372,21 @@ struct MisesMat {
   double nu;
   double h[3];
 };
+// stress constraint of quad4 box jobs: qp relaxation, p-norm, SIMP (y1, y2, p)
+struct StressPar {
+  double qp, pn, slim, y1, y2, p;
+};
+// sig [3][ne], vm, sqp per element; psum = sum (sqp / slim)^pn
+void launch_stress_eval(const Grid &g, const MisesMat &m, const StressPar &sp, const double *design,
+                        const double *E, const double *u, double *sig, double *vm, double *sqp, double *psum,
+                        Reduce red, hipStream_t s);
+// f12 per element, fe [8][ne] scratch, load = the gathered adjoint load ([2][nn], masked)
+void launch_stress_adjoint_load(const Grid &g, const MisesMat &m, const StressPar &sp, double dpndvm,
+                                const double *design, const double *E, const double *sig, const double *vm,
+                                const double *sqp, double *f12, double *fe, const uint8_t *mask, double *load,
+                                hipStream_t s);
+void launch_stress_sens(const Grid &g, const ElasticOp<double> &op, const StressPar &sp, const double *design,
+                        const double *u, const double *lam, const double *f12, double *dgds, hipStream_t s);
 // element-mean von Mises stress from the full displacement grid and E per element
 void launch_mises(const Grid &g, const MisesMat &m, const double *E, const double *u, double *out,
                   hipStream_t s);

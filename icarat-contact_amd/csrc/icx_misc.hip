@@ -334,6 +334,162 @@ void launch_mises(const Grid &g, const MisesMat &m, const double *E, const doubl
   ICX_CHECK_LAUNCH();
 }
 
+// ---------------------------------------------------------------------------------
+// stress constraint, quad4 box jobs (constraint/opt-stress-constraint/sensitivity.cpp:8-155).
+// Element-mean strain of a rectangle = strain at its centre; dN_k/dx = +-1/(2 hx),
+// dN_k/dy = +-1/(2 hy) for local node k = a + 2b at (ex + a, ey + b).
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ void q4_mean_stress(const Grid &g, const MisesMat &m, double Ee, const double *u,
+                                               int ex, int ey, double (&sg)[3]) {
+  double dx[2] = {0, 0}, dy[2] = {0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int a = k & 1, b = k >> 1;
+    const long long n = (ex + a) + (long long)g.sx * (ey + b);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double v = flush15(u[c * g.nn + n]);
+      dx[c] += a ? v : -v;
+      dy[c] += b ? v : -v;
+    }
+  }
+  const double fx = 0.5 / m.h[0], fy = 0.5 / m.h[1], nu = m.nu;
+  const double exx = dx[0] * fx, eyy = dy[1] * fy, gxy = dy[0] * fy + dx[1] * fx;
+  if (m.state == 0) {
+    const double f = Ee / (1.0 - nu * nu);
+    sg[0] = f * (exx + nu * eyy); sg[1] = f * (nu * exx + eyy); sg[2] = f * (0.5 * (1.0 - nu)) * gxy;
+  } else {
+    const double f = Ee * ((1.0 - nu) / ((1.0 - 2.0 * nu) * (1.0 + nu))), a = nu / (1.0 - nu);
+    sg[0] = f * (exx + a * eyy); sg[1] = f * (a * exx + eyy); sg[2] = f * ((1.0 - 2.0 * nu) / (2.0 * (1.0 - nu))) * gxy;
+  }
+}
+
+// sigma_e, vm_e, s_e = rho^qp E2 vm_e / E_e; partials of sum (s_e / slim)^P
+__global__ void __launch_bounds__(256) k_stress_eval(Grid g, MisesMat m, StressPar sp, const double *design,
+                                                     const double *E, const double *u, double *sig, double *vm,
+                                                     double *sqp, Reduce red) {
+  __shared__ double scr[4];
+  double acc = 0.0;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int ex = (int)(e % g.nx), ey = (int)(e / g.nx);
+    const double Ee = E[e];
+    double sg[3];
+    q4_mean_stress(g, m, Ee, u, ex, ey, sg);
+    const double v = sqrt(sg[0] * sg[0] - sg[0] * sg[1] + sg[1] * sg[1] + 3.0 * sg[2] * sg[2]);
+    const double s = pow(design[e], sp.qp) * sp.y2 * v / Ee;
+    sig[e] = sg[0]; sig[g.ne + e] = sg[1]; sig[2 * g.ne + e] = sg[2];
+    vm[e] = v;
+    sqp[e] = s;
+    acc += pow(s / sp.slim, sp.pn);
+  }
+  acc = block_sum<256>(acc, scr);
+  if (threadIdx.x == 0) red.partial[blockIdx.x] = acc;
+}
+
+// factor1 . factor2 into f12, and the element loads of the adjoint problem
+// Fe = (rho^qp E2 / E_e) factor1^T De B_mean into fe[(2k + c) ne + e]
+__global__ void __launch_bounds__(256) k_stress_dual(Grid g, MisesMat m, StressPar sp, double dpndvm,
+                                                     const double *design, const double *E, const double *sig,
+                                                     const double *vm, const double *sqp, double *f12, double *fe) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const double Ee = E[e], rho = design[e], v = vm[e], nu = m.nu;
+    const double s0 = sig[e], s1 = sig[g.ne + e], s2 = sig[2 * g.ne + e];
+    const double dvm[3] = {(2.0 * s0 - s1) / (2.0 * v), (2.0 * s1 - s0) / (2.0 * v), 3.0 * s2 / v};
+    const double c1 = dpndvm * pow(sqp[e] / sp.slim, sp.pn - 1);
+    const double dvds = sp.qp * pow(rho, sp.qp - 1.0) * sp.y2;
+    double f1[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f1[i] = c1 * dvm[i] / sp.slim;
+    f12[e] = f1[0] * (dvds * s0 / Ee) + f1[1] * (dvds * s1 / Ee) + f1[2] * (dvds * s2 / Ee);
+    // t = De f1 (De = E_e D symmetric), scaled by rho^qp E2 / E_e
+    double d00, d01, d22;
+    if (m.state == 0) {
+      const double f = Ee / (1.0 - nu * nu);
+      d00 = f; d01 = f * nu; d22 = f * (0.5 * (1.0 - nu));
+    } else {
+      const double f = Ee * ((1.0 - nu) / ((1.0 - 2.0 * nu) * (1.0 + nu)));
+      d00 = f; d01 = f * (nu / (1.0 - nu)); d22 = f * ((1.0 - 2.0 * nu) / (2.0 * (1.0 - nu)));
+    }
+    const double coef = pow(rho, sp.qp) * sp.y2 / Ee;
+    const double t0 = coef * (d00 * f1[0] + d01 * f1[1]), t1 = coef * (d01 * f1[0] + d00 * f1[1]),
+                 t2 = coef * d22 * f1[2];
+    const double hx = 0.5 / m.h[0], hy = 0.5 / m.h[1];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const double nx = (k & 1) ? hx : -hx, ny = (k >> 1) ? hy : -hy;
+      fe[(2 * k) * g.ne + e] = nx * t0 + ny * t2;
+      fe[(2 * k + 1) * g.ne + e] = ny * t1 + nx * t2;
+    }
+  }
+}
+
+// node gather of the element loads (free DOFs; constrained ones are zero)
+__global__ void k_q4_gather(Grid g, const double *fe, const uint8_t *mask, double *out) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)(n / g.sx);
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int a = k & 1, b = k >> 1, ex = i - a, ey = j - b;
+      if (ex < 0 || ey < 0 || ex >= g.nx || ey >= g.ny) continue;
+      const long long e = ex + (long long)g.nx * ey;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) acc[c] += fe[(2 * k + c) * g.ne + e];
+    }
+    const uint8_t mk = mask[n];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) out[c * g.nn + n] = ((mk >> c) & 1) ? 0.0 : acc[c];
+  }
+}
+
+// dg_e = factor1 . factor2 - p rho^(p-1) (E2 - E1) u_e^T K0 lambda_e
+__global__ void k_stress_sens(Grid g, ElasticOp<double> op, StressPar sp, const double *design, const double *u,
+                              const double *lam, const double *f12, double *dgds) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int ex = (int)(e % g.nx), ey = (int)(e / g.nx);
+    double vu[2][4], vl[2][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long n = (ex + (q & 1)) + (long long)g.sx * (ey + (q >> 1));
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        vu[c][q] = flush15(u[c * g.nn + n]);
+        vl[c][q] = lam[c * g.nn + n];
+      }
+    }
+    const double b = bilinear_quad4<double>(vu, vl, op);
+    const double dvpds = sp.p * pow(design[e], sp.p - 1.0) * (sp.y2 - sp.y1);
+    dgds[e] = f12[e] - dvpds * b;
+  }
+}
+
+void launch_stress_eval(const Grid &g, const MisesMat &m, const StressPar &sp, const double *design,
+                        const double *E, const double *u, double *sig, double *vm, double *sqp, double *psum,
+                        Reduce red, hipStream_t s) {
+  const int nb = ew_blocks(g.ne, 4, RED_BLOCKS);
+  hipLaunchKernelGGL(k_stress_eval, dim3(nb), dim3(256), 0, s, g, m, sp, design, E, u, sig, vm, sqp, red);
+  launch_finalize(red, nb, 1, psum, s);
+}
+void launch_stress_adjoint_load(const Grid &g, const MisesMat &m, const StressPar &sp, double dpndvm,
+                                const double *design, const double *E, const double *sig, const double *vm,
+                                const double *sqp, double *f12, double *fe, const uint8_t *mask, double *load,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_stress_dual, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, m, sp, dpndvm, design, E, sig, vm,
+                     sqp, f12, fe);
+  ICX_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_q4_gather, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, fe, mask, load);
+  ICX_CHECK_LAUNCH();
+}
+void launch_stress_sens(const Grid &g, const ElasticOp<double> &op, const StressPar &sp, const double *design,
+                        const double *u, const double *lam, const double *f12, double *dgds, hipStream_t s) {
+  hipLaunchKernelGGL(k_stress_sens, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, op, sp, design, u, lam, f12, dgds);
+  ICX_CHECK_LAUNCH();
+}
+
 __global__ void k_grid_to_nodes(long long nn, int nc, const double *grid, double *out) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nn;
        i += (long long)gridDim.x * blockDim.x)

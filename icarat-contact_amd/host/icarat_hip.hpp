@@ -451,11 +451,35 @@ class MMASolver {
     std::cout << "MMA loop: " << steps << std::endl;
   }
   void Reset() { icx_mma_reset(model_.job()); }
+  void SetAsymptotes(double init, double decrease, double increase) {
+    check(model_.ctx(), icx_mma_set_asymptotes(model_.job(), init, decrease, increase), "icx_mma_set_asymptotes");
+  }
 
  private:
   StructuredModel &model_;
   int n_;
 };
+
+// The stress-constrained driver's functions (constraint/opt-stress-constraint/
+// sensitivity.cpp:8-155) over icx_stress_constraint; FEMStressConst's parameters.
+namespace stress {
+struct StressParams {
+  double stressLim = 1.0, pNorm = 8.0, qp = 0.7;
+};
+inline double calConstraintValue(StructuredModel &model, const StressParams &sp) {
+  double g = 0.0;
+  check(model.ctx(), icx_stress_constraint(model.job(), sp.stressLim, sp.pNorm, sp.qp, 1e-10, 10000, &g, nullptr,
+                                           nullptr), "icx_stress_constraint");
+  return g;
+}
+// optim.dgds = dg/drho_hat (before the threshold derivative); the adjoint solve is the
+// Eigen-CG restatement at 1e-10 (Solvers(...).solve("Eigen_CG"), sensitivity.cpp:124-126)
+inline void sensitivity(StructuredModel &model, const StressParams &sp, Optimize &optim) {
+  double g = 0.0;
+  check(model.ctx(), icx_stress_constraint(model.job(), sp.stressLim, sp.pNorm, sp.qp, 1e-10, 10000, &g,
+                                           optim.dgds.data(), nullptr), "icx_stress_constraint");
+}
+}  // namespace stress
 
 // GCMMASolver (src/optimization/opt-solver/GCMMASolver.hpp:61-145) with m = 1, a = 0,
 // c = 1000 (the driver's use, opt_macro_gcmma.cpp:69); the solver state (asymptotes,

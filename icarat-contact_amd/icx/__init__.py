@@ -30,6 +30,7 @@ EXPORTS = [
     "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
     "icx_project", "icx_oc_update", "icx_mma_reset", "icx_mma_update",
     "icx_gcmma_reset", "icx_gcmma_outer_update", "icx_gcmma_inner_update", "icx_gcmma_concheck",
+    "icx_mma_set_asymptotes", "icx_stress_constraint",
     "icx_opt_init", "icx_opt_step", "icx_opt_step_begin", "icx_opt_step_end", "icx_opt_get_design",
     "icx_submit", "icx_poll", "icx_job_last_info", "icx_solve_batch", "icx_job_footprint",
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
@@ -416,6 +417,20 @@ class Job:
         self.ctx.check(lib().icx_mma_update(self.h, _dp(x), _dp(df), C.c_double(gx), _dp(dg),
                                             C.c_double(xmin), C.c_double(xmax), C.byref(n)))
         return x, n.value
+
+    def mma_set_asymptotes(self, init, decrease=0.7, increase=1.2):
+        self.ctx.check(lib().icx_mma_set_asymptotes(self.h, C.c_double(init), C.c_double(decrease),
+                                                    C.c_double(increase)))
+
+    def stress_constraint(self, stress_limit, p_norm, qp, tol=1e-12, maxit=100000, sens=True):
+        """(g, dg/drho_hat or None, adjoint PCG iterations) of the stress-constrained driver"""
+        g = C.c_double()
+        it = C.c_int()
+        dg = np.zeros(self.nelx) if sens else None
+        self.ctx.check(lib().icx_stress_constraint(self.h, C.c_double(stress_limit), C.c_double(p_norm),
+                                                   C.c_double(qp), C.c_double(tol), int(maxit), C.byref(g),
+                                                   _dp(dg) if sens else None, C.byref(it)))
+        return g.value, dg, it.value
 
     def gcmma_reset(self):
         self.ctx.check(lib().icx_gcmma_reset(self.h))

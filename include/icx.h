@@ -227,6 +227,9 @@ icx_status icx_oc_update(icx_job *job, double *x, const double *dfdx, double g,
 /* MMASolver(n, m=1) + Update (src/optimization/opt-solver/MMASolver.hpp:62-116,
  * MMASolver.cpp:41-553); the optimiser history lives on the device. */
 icx_status icx_mma_reset(icx_job *job);
+/* MMASolver::SetAsymptotes(init, decrease, increase) (MMASolver.hpp:71-76; defaults
+ * 0.2, 0.7, 1.2; the stress driver sets 0.1, 0.7, 1.2, opt_stress_constraint.cpp:72) */
+icx_status icx_mma_set_asymptotes(icx_job *job, double init, double decrease, double increase);
 icx_status icx_mma_update(icx_job *job, double *x, const double *dfdx, double gx,
                           const double *dgdx, double xmin, double xmax, int *newton_steps);
 /* GCMMASolver(n, m = 1, a = 0, c = 1000, d = 0) (src/optimization/opt-solver/
@@ -246,6 +249,17 @@ icx_status icx_gcmma_inner_update(icx_job *job, double *xmma, double f0xnew, dou
                                   double fx, const double *dfdx, double xmin, double xmax,
                                   int *newton_steps);
 icx_status icx_gcmma_concheck(const icx_job *job, double f0xnew, double fxnew, int *conservative);
+
+/* Stress constraint of the stress-constrained driver (constraint/opt-stress-constraint/
+ * sensitivity.cpp:8-155: calConstraintValue and sensitivity) on the job's current design
+ * and solve; 2D box jobs (the reference exits in 3D), young2 > young1.
+ *   g = (sum_e (rho_e^qp E2 vm_e / (E_e stress_limit))^p_norm)^(1/p_norm) - 1
+ * with vm_e the element-mean von Mises stress.  dgds (nelx doubles, NULL: g only) is
+ * dg/drho_hat by one adjoint solve (plain FP64 PCG at tol, maxit); the job's
+ * displacement stays.  adjoint_iterations may be NULL. */
+icx_status icx_stress_constraint(icx_job *job, double stress_limit, double p_norm, double qp,
+                                 double tol, int maxit, double *g, double *dgds,
+                                 int *adjoint_iterations);
 
 /* ---- device-resident design loops ------------------------------------------------
  * One design iteration of opt-macro-oc/opt_macro_oc.cpp:84-172 (driver 0: cone

@@ -954,6 +954,109 @@ void icxo_sys_mises(const icxo_sys *s, const double *nodeval, double *mises)
   }
 }
 
+/* Stress constraint, 2D elastic jobs (constraint/opt-stress-constraint/sensitivity.cpp:8-155)
+ * on the last assembly (design = rho_hat, young_e) and its solution u (nodeval: node.val
+ * after the flush):
+ *   s_e = rho_e^qp E2 vm_e / E_e (qp relaxation), g = (sum (s_e/slim)^P)^(1/P) - 1
+ *   factor1_e = psum^(1/P-1) (s_e/slim)^(P-1) dvm/dsigma / slim, factor2_e = qp rho^(qp-1) E2 sigma_e / E_e
+ *   K lambda = sum_e Fe, Fe = sum_ip (rho^qp E2 / E_e) factor1^T De Be / ipmax  (free DOFs)
+ *   dg_e = factor1.factor2 - (p rho^(p-1) (E2 - E1) / E_e) (Ke u_e).lambda_e
+ * lambda_e takes the Dirichlet value at constrained DOFs (numeqToElement, src/misc/
+ * assembling.hpp:271-298).  The adjoint solve is direct or CG (cg_tol).  Returns -1 for
+ * 3D jobs or E2 <= E1, as the reference exits there. */
+int icxo_stress_constraint(const icxo_sys *s, const double *nodeval, const double *node_val_dirich,
+                           double qp, double pnorm, double slim, int direct, double cg_tol,
+                           double *g, double *dgds, double *sqp_out)
+{
+  const int ne = s->ne, ndim = s->ndim, nd = s->nd, nelx = s->nelx, numeq = s->numeq;
+  if (ndim != 2 || !(s->young2 > s->young1)) return -1;
+  double *sig = malloc((size_t)nelx * 3 * 8), *vm = malloc((size_t)nelx * 8), *sqp = malloc((size_t)nelx * 8);
+  double psum = 0.0;
+  for (int e = 0; e < nelx; e++) {
+    double X[MAXNE * 3], N[MAXNE], dNdX[3 * MAXNE], B[6 * MAXDOF], D[36], d[MAXDOF], st[3] = {0, 0, 0};
+    int voigt;
+    element_X(s, e, X);
+    make_De(s->state, s->young[e], s->nu, D, &voigt);
+    for (int k = 0; k < ne; k++)
+      for (int c = 0; c < 2; c++) d[k * 2 + c] = nodeval[3 * s->conn[ne * e + k] + c];
+    for (int ip = 0; ip < ne; ip++) {
+      kinematics(ne, ndim, ip, X, N, dNdX);
+      make_B(ne, ndim, dNdX, B);
+      double eps[3];
+      for (int v = 0; v < 3; v++) { double t = 0.0; for (int j = 0; j < nd; j++) t += B[v * nd + j] * d[j]; eps[v] = t; }
+      for (int v = 0; v < 3; v++) { double t = 0.0; for (int w = 0; w < 3; w++) t += D[v * 3 + w] * eps[w]; st[v] += t / ne; }
+    }
+    for (int v = 0; v < 3; v++) sig[3 * e + v] = st[v];
+    vm[e] = sqrt(st[0] * st[0] - st[0] * st[1] + st[1] * st[1] + 3.0 * st[2] * st[2]);
+    sqp[e] = pow(s->design[e], qp) * s->young2 * vm[e] / s->young[e];
+    psum += pow(sqp[e] / slim, pnorm);
+  }
+  *g = pow(psum, 1.0 / pnorm) - 1.0;
+  if (sqp_out) memcpy(sqp_out, sqp, (size_t)nelx * 8);
+  const double dpndvm = pow(psum, (1.0 / pnorm) - 1.0);
+  double *f1 = malloc((size_t)nelx * 3 * 8), *f12 = malloc((size_t)nelx * 8);
+  double *load = calloc((size_t)numeq, 8), *lam = calloc((size_t)numeq, 8);
+  for (int e = 0; e < nelx; e++) {
+    const double *st = sig + 3 * e;
+    const double dvm[3] = {(2.0 * st[0] - st[1]) / (2.0 * vm[e]), (2.0 * st[1] - st[0]) / (2.0 * vm[e]),
+                           3.0 * st[2] / vm[e]};
+    const double dvds = qp * pow(s->design[e], qp - 1.0) * s->young2;
+    double acc = 0.0;
+    for (int v = 0; v < 3; v++) {
+      f1[3 * e + v] = dpndvm * pow(sqp[e] / slim, pnorm - 1) * dvm[v] / slim;
+      acc += f1[3 * e + v] * (dvds * st[v] / s->young[e]);
+    }
+    f12[e] = acc;
+    double X[MAXNE * 3], N[MAXNE], dNdX[3 * MAXNE], B[6 * MAXDOF], D[36], Fe[MAXDOF] = {0};
+    int voigt;
+    element_X(s, e, X);
+    make_De(s->state, s->young[e], s->nu, D, &voigt);
+    const double coef = pow(s->design[e], qp) * s->young2 / s->young[e];
+    for (int ip = 0; ip < ne; ip++) {
+      kinematics(ne, ndim, ip, X, N, dNdX);
+      make_B(ne, ndim, dNdX, B);
+      double t[3];
+      for (int w = 0; w < 3; w++) { t[w] = 0.0; for (int v = 0; v < 3; v++) t[w] += f1[3 * e + v] * D[v * 3 + w]; }
+      for (int j = 0; j < nd; j++) {
+        double a = 0.0;
+        for (int w = 0; w < 3; w++) a += t[w] * B[w * nd + j];
+        Fe[j] += coef * a / ne;
+      }
+    }
+    for (int k = 0; k < ne; k++)
+      for (int c = 0; c < 2; c++) {
+        const int dof = s->dof[2 * s->conn[ne * e + k] + c];
+        if (dof < numeq) load[dof] += Fe[2 * k + c];
+      }
+  }
+  int rc = 0;
+  if (direct) rc = icxo_sys_direct(s, load, lam);
+  else { double err; icxo_sys_cg(s, load, lam, cg_tol, 100000, &err); }
+  if (rc == 0) {
+    for (int e = 0; e < nelx; e++) {
+      double X[MAXNE * 3], Ke[MAXDOF * MAXDOF], vol, d[MAXDOF], l[MAXDOF];
+      element_X(s, e, X);
+      element_K(s, e, X, Ke, &vol);
+      for (int k = 0; k < ne; k++)
+        for (int c = 0; c < 2; c++) {
+          const int n = s->conn[ne * e + k], dof = s->dof[2 * n + c];
+          d[2 * k + c] = nodeval[3 * n + c];
+          l[2 * k + c] = dof < numeq ? lam[dof] : (node_val_dirich ? node_val_dirich[6 * n + c] : 0.0);
+        }
+      double f34 = 0.0;
+      for (int i = 0; i < nd; i++) {
+        double t = 0.0;
+        for (int j = 0; j < nd; j++) t += Ke[i * nd + j] * d[j];
+        f34 += t * l[i];
+      }
+      const double dvpds = s->p * pow(s->design[e], s->p - 1.0) * (s->young2 - s->young1);
+      dgds[e] = f12[e] - (dvpds / s->young[e]) * f34;
+    }
+  }
+  free(sig); free(vm); free(sqp); free(f1); free(f12); free(load); free(lam);
+  return rc;
+}
+
 /* Reactions: LinearAnalysis::solve re-assembles after the solve
  * (src/analysis/linear.hpp:90-91, 117-166): fe = Ke u_e of every element summed
  * over all DOFs (assemblingReact, src/misc/assembling.hpp:77-84), then
@@ -1819,12 +1922,17 @@ int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
   icxo_helm *hlm = NULL;
   icxo_mma *mma = NULL;
   const int cone = g->driver != 1;
+  if (g->driver == 3 && g->ndim != 2) return -1;
   if (cone) flt = icxo_filter_create(g->ne, g->ndim, nelx, g->x, g->conn, g->filter_radius, 0);
   else {
     hlm = icxo_helm_create(g->ne, g->ndim, g->numnp, nelx, g->x, g->conn, g->filter_radius);
     mma = icxo_mma_create(nelx, 1, 0.0, 100000.0, 0.0);
   }
   if (g->driver == 2) mma = icxo_gcmma_create(nelx, 1, 0.0, 1000.0, 0.0);
+  if (g->driver == 3) { /* MMASolver(n, 1) with SetAsymptotes(0.1, 0.7, 1.2) */
+    mma = icxo_mma_create(nelx, 1, 0.0, 100000.0, 0.0);
+    mma->asyminit = 0.1;
+  }
   size_t E = (size_t)nelx * sizeof(double);
   double *fil = malloc(E), *thr = malloc(E), *dthr = malloc(E), *dfds = malloc(E),
          *dgds = malloc(E), *energy = malloc(E), *tmp = malloc(E), *tmp2 = malloc(E),
@@ -1844,6 +1952,7 @@ int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
       const int bt = (int)fmax((k - 100) / 40.0, 1.0);
       beta = fmin((double)bt, 8.0);
     }
+    if (g->driver == 3) beta = fmin(pow(2.0, (int)(k / 40)), 8.0); /* opt_stress_constraint.cpp:94 */
     if (g->projection) {
       icxo_threshold(nelx, fil, beta, T, thr);
       icxo_threshold_derivative(nelx, fil, beta, T, dthr);
@@ -1858,6 +1967,24 @@ int icxo_run_driver(const icxo_driver_cfg *g, double *design, double *obj_hist,
     else { double err; it = icxo_sys_cg(s, rhs, u, g->cg_tol, g->cg_maxit, &err); }
     if (cg_hist) cg_hist[k - 1] = it;
     icxo_divide_val_into_node(g->numnp, g->dofnp, numeq, g->dof, u, g->node_val, nodeval);
+    if (g->driver == 3) { /* opt_stress_constraint.cpp:103-150: f = volume, g = stress p-norm */
+      double vol = 0.0;
+      for (int i = 0; i < nelx; i++) vol += thr[i] * s->volume[i];
+      if (icxo_stress_constraint(s, nodeval, g->node_val, g->qp, g->p_norm, g->stress_limit, g->direct,
+                                 g->cg_tol, &gcon, dgds, NULL)) { rc = -1; break; }
+      obj_hist[k - 1] = vol;
+      if (vol_hist) vol_hist[k - 1] = gcon;
+      for (int i = 0; i < nelx; i++) { dfds[i] = s->volume[i] * dthr[i]; dgds[i] *= dthr[i]; }
+      icxo_filter_apply(flt, dfds, tmp, 1); memcpy(dfds, tmp, E);
+      icxo_filter_apply(flt, dgds, tmp, 1); memcpy(dgds, tmp, E);
+      if (k == g->iterations) {
+        if (last_dfds) memcpy(last_dfds, dgds, E);
+        if (last_u) memcpy(last_u, u, (size_t)numeq * 8);
+      }
+      double gx[1] = {gcon};
+      icxo_mma_update(mma, design, dfds, gx, dgds, xmin, xmax);
+      continue;
+    }
     double obj = 0.0;
     for (int i = 0; i < numeq; i++) obj += u[i] * g->fext[i];
     if (k == 1) {

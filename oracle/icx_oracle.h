@@ -117,6 +117,9 @@ int icxo_gcmma_inner(icxo_mma *m, double *xmma, double f0xnew, const double *fxn
                      const double *xval, double f0x, const double *df0dx, const double *fx,
                      const double *dfdx, const double *xmin, const double *xmax);
 int icxo_gcmma_concheck(const icxo_mma *m, double f0xnew, const double *fxnew);
+int icxo_stress_constraint(const icxo_sys *s, const double *nodeval, const double *node_val_dirich,
+                           double qp, double pnorm, double slim, int direct, double cg_tol,
+                           double *g, double *dgds, double *sqp_out);
 void icxo_gcmma_approx_at(const icxo_mma *m, const double *x, double *f0, double *fj);
 void icxo_gcmma_state(const icxo_mma *m, double *f0app, double *fapp, double *raa0, double *raa);
 
@@ -124,7 +127,10 @@ void icxo_gcmma_state(const icxo_mma *m, double *f0app, double *fapp, double *ra
  * A: opt-macro-oc/opt_macro_oc.cpp:84-172  (cone filter, projection, CG, OC)
  * B: opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:81-163 (Helmholtz, MMA)
  * 2: opt_macro_gcmma.cpp:83-216 (cone filter r, beta = min(int(max((k-100)/40, 1)), 8),
- *    GCMMA with up to 15 inner loops, each re-solving a trial design) */
+ *    GCMMA with up to 15 inner loops, each re-solving a trial design)
+ * 3: constraint/opt-stress-constraint/opt_stress_constraint.cpp:85-158 (2D: cone filter r,
+ *    beta = min(2^(k/40), 8), f = volume, g = stress p-norm, MMA with asyminit 0.1);
+ *    obj_hist = volume, vol_hist = g, last_dfds = the filtered dg */
 typedef struct {
   int driver;          /* 0 = A (OC + cone), 1 = B (MMA + Helmholtz), 2 = GCMMA + cone */
   int ne, ndim, dofnp, state;
@@ -142,6 +148,7 @@ typedef struct {
   double helm_tol;     /* Helmholtz CG tolerance (reference 1e-10) */
   int helm_direct;
   int *inner_hist;     /* driver 2: GCMMA inner loops per step (may be NULL) */
+  double stress_limit, p_norm, qp;  /* driver 3 ([this_problem] of the stress driver) */
 } icxo_driver_cfg;
 
 int icxo_run_driver(const icxo_driver_cfg *cfg, double *design /* in: x0, out: final */,

@@ -57,6 +57,7 @@ def lib():
             "icxo_sys_heat_sensitivity": (None, [vp, _dp, _dp]),
             "icxo_heat_element": (None, [C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
             "icxo_sys_reactions": (None, [vp, _dp, _dp]),
+            "icxo_stress_constraint": (i, [vp, _dp, vp, d, d, d, i, d, C.POINTER(d), _dp, _dp]),
             "icxo_csr_cg": (i, [i, _ip, _ip, _dp, _dp, _dp, d, i, C.POINTER(d)]),
             "icxo_divide_val_into_node": (None, [i, i, i, _ip, _dp, vp, _dp]),
             "icxo_sensitivity": (None, [i, i, d, d, d, _dp, _dp, _dp, d, _dp, _dp]),
@@ -255,6 +256,18 @@ class Problem:
         lib().icxo_sys_mises(self.system(), _f64(nodeval), out)
         return out
 
+    def stress_constraint(self, nodeval, qp, p_norm, stress_limit, direct=True, cg_tol=1e-12):
+        """g and dg/drho_hat of the stress driver (constraint/opt-stress-constraint/
+        sensitivity.cpp:8-155) for the last assembly's design and its solution; also the
+        qp-relaxed stresses"""
+        g = C.c_double()
+        dg, sqp = np.zeros(self.nelx), np.zeros(self.nelx)
+        rc = lib().icxo_stress_constraint(self.system(), _f64(nodeval), self.node_val.ctypes.data, qp,
+                                          p_norm, stress_limit, int(direct), cg_tol, C.byref(g), dg, sqp)
+        if rc != 0:
+            raise RuntimeError("stress constraint failed (%d)" % rc)
+        return g.value, dg, sqp
+
     def reactions(self, nodeval):
         """K_full u over all DOFs, 3 per node, |r| < 1e-15 flushed"""
         out = np.zeros(3 * self.numnp)
@@ -305,7 +318,9 @@ class Problem:
     # -- drivers ---------------------------------------------------------------
     def run_driver(self, driver, x0, iterations, radius, projection=True, direct=False,
                    cg_tol=1e-10, cg_maxit=10000, helm_tol=1e-10, helm_direct=False,
-                   xmin=1e-4, xmax=1.0, move=0.15):
+                   xmin=1e-4, xmax=1.0, move=0.15, stress=(1.0, 8.0, 0.7)):
+        """driver 0 (OC), 1 (MMA + Helmholtz), 2 (GCMMA), 3 (stress constraint; stress =
+        (stress_limit, p_norm, qp), obj = volume and vol = the stress constraint)"""
         class Cfg(C.Structure):
             _fields_ = [("driver", C.c_int), ("ne", C.c_int), ("ndim", C.c_int),
                         ("dofnp", C.c_int), ("state", C.c_int), ("numnp", C.c_int),
@@ -319,14 +334,16 @@ class Problem:
                         ("iterations", C.c_int), ("projection", C.c_int), ("direct", C.c_int),
                         ("cg_tol", C.c_double), ("cg_maxit", C.c_int),
                         ("helm_tol", C.c_double), ("helm_direct", C.c_int),
-                        ("inner_hist", C.c_void_p)]
+                        ("inner_hist", C.c_void_p), ("stress_limit", C.c_double),
+                        ("p_norm", C.c_double), ("qp", C.c_double)]
         h, f, v = self._bc_ptrs()
         inner = np.zeros(iterations, dtype=np.int32)
         cfg = Cfg(driver, self.ne, self.ndim, self.dofnp, self.state, self.numnp, self.nelx,
                   self.numeq, self.x.ctypes.data, self.conn.ctypes.data, self.dof.ctypes.data,
                   h, f, v, self.fext.ctypes.data, self.young1, self.young2, self.nu, self.p,
                   radius, xmin, xmax, move, iterations, int(projection), int(direct), cg_tol,
-                  cg_maxit, helm_tol, int(helm_direct), inner.ctypes.data)
+                  cg_maxit, helm_tol, int(helm_direct), inner.ctypes.data, stress[0], stress[1],
+                  stress[2])
         x = _f64(x0).copy()
         obj = np.zeros(iterations)
         vol = np.zeros(iterations)
