@@ -36,7 +36,7 @@ def job_spec(rng, elems_lo, elems_hi):
 
 def footprint_estimate(ndim, div, single_reduction=False):
     """Device bytes of one served job (mixed solve, cone filter, OC design loop):
-    84 B per DOF, 92 B per element, 1 B per node, 0.4 MB of partial sums and state;
+    84 B per DOF, 92 B per element, 1 B per node, 432 KB of partial sums and state;
     a hexa8 job on the single-reduction iteration (ICX_OPT_SINGLE_REDUCTION) holds one
     more FP32 vector (88 B per DOF) and 0.375 MB more partial sums.  icx_job_footprint
     reports the same buffers once they exist."""
@@ -45,8 +45,8 @@ def footprint_estimate(ndim, div, single_reduction=False):
     ne = d0 * d1 * (d2 if ndim == 3 else 1)
     n = ndim * nn
     if ndim == 3 and single_reduction:
-        return 88 * n + 92 * ne + nn + 784 * 1024
-    return 84 * n + 92 * ne + nn + 400 * 1024
+        return 88 * n + 92 * ne + nn + 816 * 1024
+    return 84 * n + 92 * ne + nn + 432 * 1024
 
 
 def make_job(ctx, ndim, div):
