@@ -71,6 +71,10 @@ def parse():
     ap.add_argument("--heat", action="store_true", help="heat-conduction solve (FP64 plain form)")
     ap.add_argument("--mdpa", action="store_true", help="unstructured hexa8 mesh from an MDPA file (FP64 plain form)")
     ap.add_argument("--heat-div", type=int, default=100)
+    ap.add_argument("--gcmma", action="store_true", help="GCMMA design loop (driver 2) on the C1 cantilever")
+    ap.add_argument("--stress", action="store_true",
+                    help="stress-constrained driver (C++ executable) on the reference's 200 x 100 job")
+    ap.add_argument("--opt-steps", type=int, default=30)
     return ap.parse_args()
 
 
@@ -272,6 +276,131 @@ def heat_solve(args):
                        "ms_per_solve": wall * 1e3, "operator_us": mv_us,
                        "operator_GBps_algorithmic": bytes_mv / (mv_us * 1e-6) / 1e9,
                        "operator_bytes_per_launch": bytes_mv}}
+    print(json.dumps(line))
+
+
+def gcmma_loop(args):
+    """The GCMMA driver (opt_macro_gcmma.cpp:83-216) as the device-resident design loop on
+    the C1 cantilever (Q4 120 x 40, plane stress, point load, cone filter r = 2 h, x0 = 0.5),
+    FP64 fused-form solves (the Eigen CG loop's algorithm) to 1e-10 (the reference's
+    Eigen CG tolerance).  Every solve of a
+    step (the design's and each trial design's) is in the timed region.  CPU baseline: the
+    oracle's loop with its CG at the same tolerance, same steps."""
+    import numpy as np
+
+    import icx
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    steps = args.opt_steps
+    ctx = icx.Context(0)
+    job = icx.cantilever_2d(ctx, 120, 40)
+    x0 = np.full(job.nelx, 0.5)
+    job.opt_init(x0, driver=2, filter=icx.FILTER_CONE, radius=2.0, form=icx.SOLVE_FUSED, tol=1e-10,
+                 maxit=100000)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    recs = [job.opt_step() for _ in range(steps)]
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    solves = steps + sum(r["inner_loops"] for r in recs) + steps
+    cg = sum(r["cg_iterations"] + r["trial_cg_iterations"] for r in recs)
+    cpu = None
+    if not args.no_cpu:
+        pr = O.cantilever_2d(120, 40)
+        t1 = time.perf_counter()
+        ref = pr.run_driver(2, x0, steps, 2.0, projection=True, direct=False, cg_tol=1e-10, cg_maxit=100000)
+        tc = time.perf_counter() - t1
+        cpu = {"value": steps / tc, "unit": "design iterations/s", "cores": os.cpu_count() or 1, "kind": "port",
+               "sample": "oracle GCMMA loop, %d steps on the same job (CG 1e-10)" % steps,
+               "objective_last_rel_diff": abs(ref["obj"][-1] - recs[-1]["objective"]) / ref["obj"][-1]}
+    line = {"metric": "GCMMA design iterations/s", "value": steps / wall, "unit": "design iterations/s",
+            "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "GCMMA driver, Q4 120 x 40 cantilever, cone r = 2, %d steps" % steps,
+                       "ms_per_step": wall / steps * 1e3, "solves": solves, "pcg_iterations": cg,
+                       "inner_loops": [r["inner_loops"] for r in recs],
+                       "objective_first_last": [recs[0]["objective"], recs[-1]["objective"]]},
+            "cpu_baseline": cpu}
+    print(json.dumps(line))
+
+
+STRESS_TOML = """[mesh]
+dimension = 2
+division = [200, 100, 1]
+length = [200.0, 100.0, 0.25]
+type = "quad4"
+
+[boundary_condition]
+dirichlet = [
+  [[1, 1, 0, 0, 0, 0], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0], [[0.0, 0.0], [0.0, 100.0], [0.0, 0.0]]],
+]
+neumann = [
+  [[0.0, -0.1, 0.0], [[200.0, 200.0], [40.0, 60.0], [0.0, 0.0]]],
+]
+
+[material]
+designs_0 = 0.5
+penalty = 3.0
+poisson = 0.3
+state = "plane_stress"
+type = "Elastic"
+young = [0.001, 100.0]
+
+[this_problem]
+fdm = false
+p_norm = 8.0
+qp = 0.7
+stress_limit = 1.0
+"""
+
+
+def stress_driver(args):
+    """The stress-constrained driver (constraint/opt-stress-constraint, its init.toml:
+    Q4 200 x 100, plane stress) run as the C++ executable over libicx for --opt-steps
+    steps (structural solve, stress p-norm, adjoint solve, filter, MMA, binary VTK per
+    step); value = design iterations per second of the executable's whole run.  CPU
+    baseline: the oracle's loop (CG 1e-10 for both solves) on the same job."""
+    import subprocess
+    import tempfile
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    steps = args.opt_steps
+    exe = os.path.join(ROOT, "icarat-contact_amd", "_build", "opt_stress_constraint")
+    with tempfile.TemporaryDirectory() as d:
+        toml = os.path.join(d, "init.toml")
+        open(toml, "w").write(STRESS_TOML)
+        t0 = time.perf_counter()
+        out = subprocess.run([exe, toml, "Eigen_CG", str(steps)], cwd=d, capture_output=True, text=True,
+                             timeout=600)
+        wall = time.perf_counter() - t0
+        if out.returncode != 0:
+            raise RuntimeError(out.stderr[-2000:])
+        rows = np.loadtxt(os.path.join(d, "function.csv"), delimiter=",", ndmin=2)
+    cpu = None
+    if not args.no_cpu:
+        pr = O.Problem(4, (200, 100, 1), (200.0, 100.0, 0.25), O.PLANE_STRESS, young=(0.001, 100.0))
+        pr.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, 100, 0, 0])
+        pr.make_dof()
+        pr.add_neumann([0.0, -0.1, 0.0], [200, 200, 40, 60, 0, 0])
+        k = min(steps, 5)
+        t1 = time.perf_counter()
+        ref = pr.run_driver(3, np.full(pr.nelx, 0.5), k, 2.0, projection=True, direct=False, cg_tol=1e-10,
+                            cg_maxit=100000, stress=(1.0, 8.0, 0.7))
+        tc = time.perf_counter() - t1
+        cpu = {"value": k / tc, "unit": "design iterations/s", "cores": os.cpu_count() or 1, "kind": "port",
+               "sample": "oracle stress-constrained loop, first %d steps of the same job (CG 1e-10)" % k,
+               "constraint_rel_diff_first_steps": float(np.max(np.abs(ref["vol"] - rows[:k, 2]) /
+                                                              np.maximum(1e-12, np.abs(ref["vol"]))))}
+    line = {"metric": "stress-constrained design iterations/s", "value": steps / wall,
+            "unit": "design iterations/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "stress-constrained driver, Q4 200 x 100 plane stress (init.toml), %d steps"
+                                   % steps, "ms_per_step": wall / steps * 1e3,
+                       "volume_first_last": [rows[0, 1], rows[-1, 1]],
+                       "constraint_first_last": [rows[0, 2], rows[-1, 2]]},
+            "cpu_baseline": cpu}
     print(json.dumps(line))
 
 
@@ -538,6 +667,10 @@ def main():
     if args.output:
         output_step(args)
         return
+    if args.gcmma:
+        return gcmma_loop(args)
+    if args.stress:
+        return stress_driver(args)
     if args.heat:
         heat_solve(args)
         return
