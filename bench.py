@@ -357,7 +357,8 @@ def stress_driver(args):
     """The stress-constrained driver (constraint/opt-stress-constraint, its init.toml:
     Q4 200 x 100, plane stress) run as the C++ executable over libicx for --opt-steps
     steps (structural solve, stress p-norm, adjoint solve, filter, MMA, binary VTK per
-    step); value = design iterations per second of the executable's whole run.  CPU
+    step; structural and adjoint solves in the FP64 fused form of the Eigen CG loop at
+    1e-10); value = design iterations per second of the executable's whole run.  CPU
     baseline: the oracle's loop (CG 1e-10 for both solves) on the same job."""
     import subprocess
     import tempfile
@@ -372,7 +373,7 @@ def stress_driver(args):
         toml = os.path.join(d, "init.toml")
         open(toml, "w").write(STRESS_TOML)
         t0 = time.perf_counter()
-        out = subprocess.run([exe, toml, "Eigen_CG", str(steps)], cwd=d, capture_output=True, text=True,
+        out = subprocess.run([exe, toml, "HIP_PCG_FUSED", str(steps)], cwd=d, capture_output=True, text=True,
                              timeout=600)
         wall = time.perf_counter() - t0
         if out.returncode != 0:

@@ -1878,8 +1878,8 @@ icx_status icx_mises(icx_job *j, double *out) {
 }
 
 // The stress constraint of constraint/opt-stress-constraint/sensitivity.cpp:8-155 on the job's
-// current design and solve (quad4 box jobs): g, and with dgds one adjoint solve (plain FP64
-// PCG at tol) K lambda = sum_e Fe.  lambda takes the Dirichlet values at constrained DOFs
+// current design and solve (quad4 box jobs): g, and with dgds one adjoint solve (the FP64
+// fused form of the Eigen CG loop, at tol) K lambda = sum_e Fe.  lambda takes the Dirichlet values at constrained DOFs
 // (numeqToElement, src/misc/assembling.hpp:271-298); the job's displacement is restored.
 static icx_status stress_constraint_dev(icx_job *j, double slim, double pn, double qp, double tol, int maxit,
                                         double *g, double *dgds, int *adjoint_iterations) {
@@ -1908,7 +1908,7 @@ static icx_status stress_constraint_dev(icx_job *j, double slim, double pn, doub
   launch_stress_adjoint_load(j->g, mm, sp, std::pow(psum, (1.0 / pn) - 1.0), j->d_design.p, j->d_E.p, sig.p, vm.p,
                              sqp.p, f12.p, fe.p, j->d_mask.p, j->d_b.p, S(j));
   icx_solve_info info{};
-  rc = solve_plain(j, tol, maxit, info);
+  rc = solve_fused(j, tol, maxit, info);
   if (rc && rc != ICX_ERR_NOCONV) return rc;
   if (adjoint_iterations) *adjoint_iterations = info.iterations;
   rc = output_displacement(j);  // lambda with the Dirichlet values

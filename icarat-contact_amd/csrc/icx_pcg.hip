@@ -340,6 +340,12 @@ __global__ void __launch_bounds__(BS) k_eig_setup_finalize(SolverState *st, PcgP
     st->iter_cur = 0;
     st->iter = 0;
     st->stop_iter = 0;
+    // fields the fused / mixed forms leave behind: the host maps a non-zero `pending`
+    // to `done`, so a stale stop code would end this solve before its first iteration
+    st->pending = 0;
+    st->flush_iter = -1;
+    st->alpha_owed = 0.0;
+    st->delta2 = 0.0;
     st->done = (bb == 0.0 || bb < thr) ? 1 : 0;
   }
 }

@@ -255,7 +255,8 @@ icx_status icx_gcmma_concheck(const icx_job *job, double f0xnew, double fxnew, i
  * and solve; 2D box jobs (the reference exits in 3D), young2 > young1.
  *   g = (sum_e (rho_e^qp E2 vm_e / (E_e stress_limit))^p_norm)^(1/p_norm) - 1
  * with vm_e the element-mean von Mises stress.  dgds (nelx doubles, NULL: g only) is
- * dg/drho_hat by one adjoint solve (plain FP64 PCG at tol, maxit); the job's
+ * dg/drho_hat by one adjoint solve (the FP64 fused form of the Eigen CG loop at tol,
+ * maxit); the job's
  * displacement stays.  adjoint_iterations may be NULL. */
 icx_status icx_stress_constraint(icx_job *job, double stress_limit, double p_norm, double qp,
                                  double tol, int maxit, double *g, double *dgds,

@@ -313,3 +313,26 @@ def test_gpu_matches_golden(ctx, name):
     job.helmholtz_setup(2.0)
     h, _ = job.helmholtz(g["rho"], tol=1e-13)
     assert np.abs(h - g["helm"]).max() < 1e-10
+
+
+@pytest.mark.parametrize("dims", [(24, 10), (10, 5, 5)])
+def test_solve_forms_in_any_order_on_one_job(ctx, dims):
+    """Plain, fused and mixed solves interleaved on one job (each form's captured graphs
+    and polling state reused across the others), a new design before every solve; each
+    result against the oracle's direct solve."""
+    if len(dims) == 2:
+        job, pr = icx.cantilever_2d(ctx, *dims), O.cantilever_2d(*dims)
+    else:
+        job, pr = icx.cantilever_3d(ctx, *dims), O.cantilever_3d(*dims)
+    rng = np.random.default_rng(11)
+    order = [icx.SOLVE_FUSED, icx.SOLVE_PLAIN, icx.SOLVE_FUSED, icx.SOLVE_MIXED, icx.SOLVE_PLAIN,
+             icx.SOLVE_PLAIN, icx.SOLVE_FUSED, icx.SOLVE_MIXED, icx.SOLVE_FUSED]
+    for k, form in enumerate(order):
+        rho = rng.uniform(0.2, 1.0, pr.nelx)
+        job.set_design(rho)
+        u_ref = pr.direct(pr.assemble(rho))
+        tol = 1e-10 if form == icx.SOLVE_MIXED else 1e-12
+        u, info = job.solve(form, tol=tol, maxit=50000)
+        assert info["converged"], (k, form)
+        err = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
+        assert err < (1e-7 if form == icx.SOLVE_MIXED else 1e-9), (k, form, err)
