@@ -40,6 +40,7 @@ struct icx_ctx {
   bool single_reduction = false;   // hexa8 mixed solves: one kernel per iteration (opt-in)
   bool pipeline = true;            // keep one chunk queued ahead of the state read
   int tile_depth = 0;              // hexa8 operator z depth per block, 0 = automatic
+  bool heat_dense = false;         // heat jobs: dense element rows even where Haar-diagonal
   // profile window (icx_profile.h)
   std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
   int prof_kernel = 0;
@@ -571,6 +572,7 @@ struct icx_job {
   bool heat() const { return prob.state == ICX_HEAT; }
   HeatOp heat_op{}, heat_dK{};
   bool heat_ready = false;
+  bool heat_haar_ok = false;  // both conductivity element matrices Haar-diagonal
   MMAWork mma;
   GcmmaWork gc;
   OptWork opt;
@@ -792,6 +794,15 @@ icx_status icx_set_option(icx_ctx *ctx, int option, long long value) {
   }
   if (option == ICX_OPT_PIPELINE) {
     ctx->pipeline = value != 0;
+    return ICX_OK;
+  }
+  if (option == ICX_OPT_HEAT_DENSE) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->heat_dense = value != 0;
+    for (icx_job *j : ctx->jobs) {
+      j->heat_op.haar = (j->heat_haar_ok && !ctx->heat_dense) ? 1 : 0;
+      j->reset_graphs();
+    }
     return ICX_OK;
   }
   if (option == ICX_OPT_TILE_DEPTH) {
@@ -1368,6 +1379,21 @@ icx_status icx_set_conductivity(icx_job *j, const double *k1, const double *k2) 
   heat_element(j->el, D1, j->heat_op.K1);
   heat_element(j->el, D2, j->heat_op.K2);
   heat_element(j->el, Dd, j->heat_dK.K1);
+  // diagonal tensors on a box element: both element matrices Haar-diagonal (fast operator)
+  {
+    const int nq = j->el.nq;
+    HelmOp<double> h1, h2;
+    std::string why;
+    const bool d1 = build_helm_op(j->el, std::vector<double>(j->heat_op.K1, j->heat_op.K1 + nq * nq), h1, why);
+    const bool d2 = build_helm_op(j->el, std::vector<double>(j->heat_op.K2, j->heat_op.K2 + nq * nq), h2, why);
+    j->heat_haar_ok = d1 && d2;
+    j->heat_op.haar = (j->heat_haar_ok && !j->ctx->heat_dense) ? 1 : 0;
+    j->reset_graphs();
+    for (int q = 0; q < 8; ++q) {
+      j->heat_op.kf1[q] = q < nq ? h1.kf[q] : 0.0;
+      j->heat_op.kf2[q] = q < nq ? h2.kf[q] : 0.0;
+    }
+  }
   j->heat_ready = true;
   j->u_valid = false;
   return ICX_OK;

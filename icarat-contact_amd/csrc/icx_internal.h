@@ -400,6 +400,10 @@ void launch_grid_to_nodes(const Grid &g, int nc, const double *grid, double *out
 struct HeatOp {
   int nq;
   double K1[64], K2[64];
+  // haar = 1: K1 and K2 are diagonal in the element's Haar basis (diagonal tensors on a
+  // box element), with eigenvalues kf1 / kf2 scaled as the Helmholtz operator's kf
+  int haar;
+  double kf1[8], kf2[8];
 };
 void launch_heat_weight(long long ne, double p, const double *design, double *w, hipStream_t s);
 // y = K x (mask: bit 0 = constrained, output forced to zero; may be null); st may be null

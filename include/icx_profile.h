@@ -29,7 +29,8 @@ icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
  * ICX_OPT_TILE_DEPTH (default 0 = automatic): node layers along z per block of the
  *   hexa8 operator kernels, 1..64 (raised as needed to keep the block count within
  *   the partial-sum buffers).  Applies to every job of the context; results of the
- *   operator are identical, PCG dot products change only in rounding.
+ *   operator are identical, PCG dot products change only in rounding.  For Haar-diagonal
+ *   hexa8 heat jobs: the nodes each thread marches along z (2, 4, 8 or 16; else 4).
  * ICX_OPT_PIPELINE (default 1): while the host waits for the solver state after a chunk
  *   of PCG iterations, the next chunk is already queued (it does nothing if the solve
  *   has stopped); 0 polls chunk by chunk.  Results are identical.
@@ -37,8 +38,13 @@ icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
  *   submitted solves) with one kernel and one reduction phase per PCG iteration instead
  *   of two (the next r.z is formed from an expansion in dot products reduced with the
  *   previous iteration's).  Same iterates up to rounding, same stopping tests; on one
- *   GPU the two-kernel iteration is faster.  Set it before submitting solves. */
-enum { ICX_OPT_GRAPHS = 1, ICX_OPT_TILE_DEPTH = 2, ICX_OPT_PIPELINE = 3, ICX_OPT_SINGLE_REDUCTION = 4 };
+ *   GPU the two-kernel iteration is faster.  Set it before submitting solves.
+ * ICX_OPT_HEAT_DENSE (default 0): 1 makes heat jobs apply their element matrices as dense
+ *   rows even where both are diagonal in the element's Haar basis (diagonal conductivity
+ *   tensors), the path taken otherwise; for checking one operator against the other.
+ */
+enum { ICX_OPT_GRAPHS = 1, ICX_OPT_TILE_DEPTH = 2, ICX_OPT_PIPELINE = 3, ICX_OPT_SINGLE_REDUCTION = 4,
+       ICX_OPT_HEAT_DENSE = 5 };
 icx_status icx_set_option(icx_ctx *ctx, int option, long long value);
 
 #ifdef __cplusplus

@@ -143,3 +143,40 @@ def test_cxx_heat_driver_matches_oracle():
     assert rows.shape[0] == steps
     assert np.abs(rows[:, 1] - obj).max() <= 1e-7 * np.abs(obj).max(), (rows[:, 1], obj)
     assert np.abs(rows[:, 2] - con).max() <= 1e-7 * abs(con[0] + 1.0) + 1e-9
+
+
+@pytest.mark.parametrize("ne,div", [(QUAD4, (12, 8)), (HEXA8, (6, 4, 5)), (HEXA8, (9, 7, 19))])
+def test_heat_haar_operator_for_diagonal_tensors(ctx, ne, div):
+    """Diagonal conductivity tensors (as in opt-heat's examples) make both element
+    matrices diagonal in the Haar basis: that operator against the oracle, against the
+    dense-row operator (ICX_OPT_HEAT_DENSE) on the same job, and through a solve."""
+    L = tuple(float(d) for d in div)
+    k1, k2 = (100.0, 70.0, 40.0, 0, 0, 0), (0.1, 0.3, 0.2, 0, 0, 0)
+    job = icx.Job(ctx, ne, div, L, icx.HEAT)
+    pr = O.Problem(ne, div, L, O.HEAT)
+    bc = ([1, 0, 0, 0, 0, 0], [0] * 6, [0, 0, 0, div[1], 0, div[2] if ne == HEXA8 else 0])
+    nm = ([0.1, 0, 0], [0, div[0], 0, div[1], 0, div[2] if ne == HEXA8 else 0])
+    for o in (job, pr):
+        o.add_dirichlet(*bc)
+        o.make_dof()
+        o.add_neumann(*nm)
+        o.set_conductivity(k1, k2)
+    rho = O.random_density(pr.nelx, 5, lo=0.05, hi=1.0)
+    job.set_design(rho)
+    b = pr.assemble(rho)
+    x = np.random.default_rng(4).standard_normal(pr.numeq)
+    y = job.matvec(x)
+    yo = pr.spmv(x)
+    assert np.abs(y - yo).max() <= 1e-13 * np.abs(yo).max()
+    ctx.set_option(icx.OPT_HEAT_DENSE, 1)
+    try:
+        yd = job.matvec(x)
+        Tdense, _ = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=20000)
+    finally:
+        ctx.set_option(icx.OPT_HEAT_DENSE, 0)
+    assert np.abs(y - yd).max() <= 1e-13 * np.abs(yd).max()
+    T, info = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=20000)
+    Td = pr.direct(b)
+    assert info["converged"]
+    assert np.linalg.norm(T - Td) / np.linalg.norm(Td) < 1e-10
+    assert np.linalg.norm(T - Tdense) / np.linalg.norm(Td) < 1e-11
