@@ -111,3 +111,48 @@ def test_cxx_gcmma_driver_executable_matches_oracle():
     assert rel.max() <= 1e-8, rel
     assert np.abs(rows[:, 2] - ref["vol"]).max() <= 1e-8 * pr.nelx
     assert np.array_equal(rows[:, 3].astype(int), ref["inner"])
+
+
+def test_gcmma_driver_through_the_split_step(ctx):
+    """Driver 2 through icx_opt_step_begin / _end with the caller's own solve between
+    them (the serving loop's pattern); the trial solves run inside _end."""
+    nx, ny, steps = 40, 14, 18
+    job = icx.cantilever_2d(ctx, nx, ny)
+    pr = O.cantilever_2d(nx, ny)
+    x0 = np.full(pr.nelx, 0.3)
+    ref = pr.run_driver(2, x0, steps, 2.0, projection=True, direct=True)
+    job.opt_init(x0, driver=2, filter=icx.FILTER_CONE, radius=2.0, form=icx.SOLVE_PLAIN, tol=1e-13, maxit=50000)
+    recs = []
+    for _ in range(steps):
+        job.opt_step_begin()
+        _, info = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=50000, want_u=False)
+        recs.append(job.opt_step_end(info))
+    obj = np.array([r["objective"] for r in recs])
+    assert (np.abs(obj - ref["obj"]) / ref["obj"]).max() < 1e-7
+    assert np.array_equal(np.array([r["inner_loops"] for r in recs]), ref["inner"])
+
+
+def test_gcmma_driver_on_a_distorted_mesh(ctx):
+    """Driver 2 on an unstructured quad4 mesh (per-element volumes in V0, the constraint
+    and dg, cone filter on the mesh's neighbour lists) against the oracle."""
+    from meshgen import box_mesh, distort
+    ne, div, L, steps = 4, (30, 10), (30.0, 10.0), 12
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 8, L)
+    job = icx.Job.from_mesh(ctx, ne, xd, conn, icx.PLANE_STRESS)
+    pr = O.Problem.from_mesh(ne, xd, conn, O.PLANE_STRESS)
+    for o in (job, pr):
+        o.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, L[1], 0, 0])
+    assert job.make_dof() == pr.make_dof()
+    for o in (job, pr):
+        o.add_neumann([0.0, -0.05, 0.0], [L[0], L[0], 0.0, 2.0, 0, 0])
+    x0 = np.full(pr.nelx, 0.4)
+    ref = pr.run_driver(2, x0, steps, 2.0, projection=True, direct=True)
+    job.opt_init(x0, driver=2, filter=icx.FILTER_CONE, radius=2.0, form=icx.SOLVE_PLAIN, tol=1e-13, maxit=50000)
+    recs = [job.opt_step() for _ in range(steps)]
+    obj = np.array([r["objective"] for r in recs])
+    con = np.array([r["constraint"] for r in recs])
+    assert (np.abs(obj - ref["obj"]) / ref["obj"]).max() < 1e-7
+    assert np.abs(con - ref["vol"]).max() <= 1e-8 * L[0] * L[1]
+    assert np.array_equal(np.array([r["inner_loops"] for r in recs]), ref["inner"])
+    assert np.abs(job.opt_design() - ref["design"]).max() < 1e-6
