@@ -670,8 +670,89 @@ __global__ void __launch_bounds__(256) k_heat_haar_col(Grid g, HeatOp op, const 
   }
 }
 
+// hexa8, Haar-diagonal, elements transformed once: a 16 x 16 block owns the element
+// columns (ex, ey) = (i0 - 1 + tx, j0 - 1 + ty) around its 15 x 15 node tile and marches
+// the node layers k0 .. k0 + HZ - 1.  At layer k each thread transforms its element
+// (ex, ey, k) once; the element's lower-face outputs plus the upper-face outputs kept
+// from (ex, ey, k - 1) are its column's share of the four nodes at layer k, exchanged
+// through LDS (double-buffered: one barrier per layer).
+constexpr int HT = 15, HZ = 8;
+__global__ void __launch_bounds__(256) k_heat_haar_tile(Grid g, HeatOp op, const double *w, const uint8_t *mask,
+                                                        const double *x, double *y, const SolverState *st) {
+  if (st && st->done) return;
+  __shared__ double sh[2][4][256];
+  const int tiles_x = (g.sx + HT - 1) / HT, tiles_y = (g.sy + HT - 1) / HT;
+  const int b = blockIdx.x;
+  const int ti = b % tiles_x, tj = (b / tiles_x) % tiles_y, tk = b / (tiles_x * tiles_y);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int i0 = ti * HT, j0 = tj * HT, k0 = tk * HZ, k1 = min(k0 + HZ, g.sz);
+  const int ex = i0 - 1 + tx, ey = j0 - 1 + ty;
+  const bool col = ex >= 0 && ey >= 0 && ex < g.nx && ey < g.ny;
+  double dkf[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) dkf[q] = op.kf1[q] - op.kf2[q];
+  // x on the element column's four corner lines at node layer kk (0 outside the grid)
+  auto face = [&](int kk, double (&f)[4]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int ii = ex + (c & 1), jj = ey + (c >> 1);
+      const bool in = col && kk >= 0 && kk < g.sz;
+      f[c] = in ? x[ii + (long long)g.sx * (jj + (long long)g.sy * kk)] : 0.0;
+    }
+  };
+  // y_e of element (ex, ey, ez) from its lower / upper faces; false: no element there
+  auto element = [&](int ez, const double (&lo)[4], const double (&hi)[4], double (&v)[8]) -> bool {
+    if (!col || ez < 0 || ez >= g.nz) return false;
+    const double we = w[ex + (long long)g.nx * (ey + (long long)g.ny * ez)];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { v[c] = lo[c]; v[c + 4] = hi[c]; }
+    haar_fwd8(v);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] *= fma(we, dkf[q], op.kf2[q]);
+    haar_inv8(v);
+    return true;
+  };
+  double lo[4], hi[4], up[4] = {0.0, 0.0, 0.0, 0.0}, v[8];
+  face(k0 - 1, lo);
+  face(k0, hi);
+  if (element(k0 - 1, lo, hi, v))
+#pragma unroll
+    for (int c = 0; c < 4; ++c) up[c] = v[c + 4];
+  for (int k = k0; k < k1; ++k) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) lo[c] = hi[c];
+    face(k + 1, hi);
+    double cur[4] = {0.0, 0.0, 0.0, 0.0};
+    if (element(k, lo, hi, v)) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cur[c] = v[c];
+    }
+    const int buf = (k - k0) & 1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      sh[buf][c][threadIdx.x] = cur[c] + up[c];
+      up[c] = (col && k < g.nz) ? v[c + 4] : 0.0;
+    }
+    __syncthreads();
+    // node (i0 + tx, j0 + ty): corner 3 of column (tx, ty), 2 of (tx+1, ty), 1 of (tx, ty+1), 0 of (tx+1, ty+1)
+    const int i = i0 + tx, j = j0 + ty;
+    if (tx < HT && ty < HT && i < g.sx && j < g.sy) {
+      const int t = threadIdx.x;
+      const double acc = sh[buf][3][t] + sh[buf][2][t + 1] + sh[buf][1][t + 16] + sh[buf][0][t + 17];
+      const long long n = i + (long long)g.sx * (j + (long long)g.sy * k);
+      y[n] = (mask && (mask[n] & 1)) ? 0.0 : acc;
+    }
+  }
+}
+
 void launch_heat_matvec(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, const double *x,
                         double *y, const SolverState *st, hipStream_t s, const LaunchEvents *ev) {
+  if (op.haar && g.ndim == 3 && g.tile_depth != 2 && g.tile_depth != 4 && g.tile_depth != 8 && g.tile_depth != 16) {
+    const int nb = ((g.sx + HT - 1) / HT) * ((g.sy + HT - 1) / HT) * ((g.sz + HZ - 1) / HZ);
+    launch_ev(k_heat_haar_tile, nb, 256, s, ev, g, op, w, mask, x, y, st);
+    ICX_CHECK_LAUNCH();
+    return;
+  }
   // node layers per thread: the context's tile depth when it is 2, 4, 8 or 16, else 4
   // (measured at 100^3: 30.7, 28.2, 29.9, 36.7 us for 2, 4, 8, 16)
   const int zc = (g.tile_depth == 2 || g.tile_depth == 8 || g.tile_depth == 16) ? g.tile_depth : 4;

@@ -145,11 +145,13 @@ def test_cxx_heat_driver_matches_oracle():
     assert np.abs(rows[:, 2] - con).max() <= 1e-7 * abs(con[0] + 1.0) + 1e-9
 
 
-@pytest.mark.parametrize("ne,div", [(QUAD4, (12, 8)), (HEXA8, (6, 4, 5)), (HEXA8, (9, 7, 19))])
+@pytest.mark.parametrize("ne,div", [(QUAD4, (12, 8)), (HEXA8, (6, 4, 5)), (HEXA8, (9, 7, 19)),
+                                    (HEXA8, (33, 17, 9))])
 def test_heat_haar_operator_for_diagonal_tensors(ctx, ne, div):
     """Diagonal conductivity tensors (as in opt-heat's examples) make both element
     matrices diagonal in the Haar basis: that operator against the oracle, against the
-    dense-row operator (ICX_OPT_HEAT_DENSE) on the same job, and through a solve."""
+    dense-row operator (ICX_OPT_HEAT_DENSE) on the same job, and through a solve; the
+    grids cover one and several 15 x 15 node tiles and z chunks that do not divide."""
     L = tuple(float(d) for d in div)
     k1, k2 = (100.0, 70.0, 40.0, 0, 0, 0), (0.1, 0.3, 0.2, 0, 0, 0)
     job = icx.Job(ctx, ne, div, L, icx.HEAT)
