@@ -274,6 +274,9 @@ void launch_sr_batch(const SrArgs *args, const int *off, int njobs, int total_bl
 void launch_sr_restart(const Grid &g, const ElasticOp<float> &op, const SrVectors &v, SolverState *st,
                        hipStream_t s);
 
+// hexa8 Haar-diagonal operator with the same eigenvalues kf[8] in every element (tile kernel)
+void launch_helm_tile(const Grid &g, const double *kf, const double *x, double *y, const SolverState *st,
+                      hipStream_t s);
 void launch_helm_matvec(const Grid &g, const HelmOp<double> &op, const double *x, double *y,
                         const SolverState *st, hipStream_t s);
 

@@ -675,7 +675,8 @@ __global__ void __launch_bounds__(256) k_heat_haar_col(Grid g, HeatOp op, const 
 // the node layers k0 .. k0 + HZ - 1.  At layer k each thread transforms its element
 // (ex, ey, k) once; the element's lower-face outputs plus the upper-face outputs kept
 // from (ex, ey, k - 1) are its column's share of the four nodes at layer k, exchanged
-// through LDS (double-buffered: one barrier per layer).
+// through LDS (double-buffered: one barrier per layer).  w = null: every element has the
+// eigenvalues kf2 (the Helmholtz filter's operator).
 constexpr int HT = 15, HZ = 8;
 __global__ void __launch_bounds__(256) k_heat_haar_tile(Grid g, HeatOp op, const double *w, const uint8_t *mask,
                                                         const double *x, double *y, const SolverState *st) {
@@ -703,7 +704,7 @@ __global__ void __launch_bounds__(256) k_heat_haar_tile(Grid g, HeatOp op, const
   // y_e of element (ex, ey, ez) from its lower / upper faces; false: no element there
   auto element = [&](int ez, const double (&lo)[4], const double (&hi)[4], double (&v)[8]) -> bool {
     if (!col || ez < 0 || ez >= g.nz) return false;
-    const double we = w[ex + (long long)g.nx * (ey + (long long)g.ny * ez)];
+    const double we = w ? w[ex + (long long)g.nx * (ey + (long long)g.ny * ez)] : 0.0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) { v[c] = lo[c]; v[c + 4] = hi[c]; }
     haar_fwd8(v);
@@ -743,6 +744,16 @@ __global__ void __launch_bounds__(256) k_heat_haar_tile(Grid g, HeatOp op, const
       y[n] = (mask && (mask[n] & 1)) ? 0.0 : acc;
     }
   }
+}
+
+void launch_helm_tile(const Grid &g, const double *kf, const double *x, double *y, const SolverState *st,
+                      hipStream_t s) {
+  HeatOp op{};
+  for (int q = 0; q < 8; ++q) op.kf1[q] = op.kf2[q] = kf[q];
+  op.haar = 1;
+  const int nb = ((g.sx + HT - 1) / HT) * ((g.sy + HT - 1) / HT) * ((g.sz + HZ - 1) / HZ);
+  hipLaunchKernelGGL(k_heat_haar_tile, dim3(nb), dim3(256), 0, s, g, op, nullptr, nullptr, x, y, st);
+  ICX_CHECK_LAUNCH();
 }
 
 void launch_heat_matvec(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, const double *x,
