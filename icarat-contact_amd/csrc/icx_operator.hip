@@ -271,8 +271,10 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   T Ea = load_E(kfirst), Eb = T(0);
   OwnedNode<T> na{}, nb{};
   double rr = 0.0, rz = 0.0;
+  // solve already finished (uniform: no kernel writes `done` it reads): leave before the
+  // partials reduction, whose results only a running iteration uses
+  if (ss.done) return;
   if (MODE == 1) block_total2<NT>(a.P.PB, a.P.nB, s_sc, rr, rz);
-  if (ss.done) return;  // solve already finished (uniform: no kernel writes `done` it reads)
   if (MODE == 1) {
     it = ss.iter;
     pold = (it & 1) ? a.P0 : a.P1;
