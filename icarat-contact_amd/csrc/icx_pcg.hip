@@ -129,12 +129,14 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
     }
   };
   load(v0);
-  const double pAp = block_total<BS>(P.PA, P.nA, 1, scr);
+  // both exits are uniform and leave before the partials reduction, which only a running
+  // iteration uses
   if (ss.done) return;
   if (ss.pending) {  // kernel A of this iteration stopped: publish the stop
     if (blk == 0 && threadIdx.x == 0) st->done = ss.pending;
     return;
   }
+  const double pAp = block_total<BS>(P.PA, P.nA, 1, scr);
   const int it = ss.iter_cur;
   const double alpha_d = ((it & 1) ? ss.rz_hist[1] : ss.rz_hist[0]) / pAp;
   if (blk == 0 && threadIdx.x == 0) st->alpha_hist[it & 1] = alpha_d;
