@@ -9,6 +9,17 @@
  *
  * Every function names the reference file:line it restates.  Paths are
  * relative to the reference repository root.
+ *
+ * Pins (tests/test_oracle_*.py, tests/test_golden.py): mesh, DOF map and loads by
+ * closed forms and hand-checked grids; element matrices by closed forms, rigid modes
+ * and the patch test; solves by direct solutions and scipy; sensitivities, projection
+ * and the stress constraint's adjoint derivative by finite differences; filters by
+ * brute force, invariants and adjoint identities; OC by a replayed bisection; MMA and
+ * GCMMA by closed-form optima, SLSQP on the subproblem, first-order exactness and the
+ * RaaUpdate identity.  Parity unpinned: the exact Eigen CG iterates (the algorithm is
+ * restated, iteration counts agree within about 1%), SimplicialLDLT's rounding, and the
+ * exact multi-step MMASolver / GCMMASolver trajectories (each update is pinned by the
+ * checks above; the history-dependent sequence only by this restatement).
  */
 #ifndef ICX_ORACLE_H
 #define ICX_ORACLE_H
