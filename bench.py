@@ -249,6 +249,8 @@ def heat_solve(args):
     import icx
     n = args.heat_div
     ctx = icx.Context(0)
+    if args.no_graphs:
+        ctx.set_option(icx.OPT_GRAPHS, 0)
     job = icx.Job(ctx, icx.HEXA8, (n, n, n), (1.0, 1.0, 1.0), icx.HEAT)
     for face in ([0, 1, 0, 1, 0, 0], [0, 1, 0, 1, 1, 1], [0, 0, 0, 1, 0, 1], [1, 1, 0, 1, 0, 1],
                  [0, 1, 0, 0, 0, 1], [0, 1, 1, 1, 0, 1]):
@@ -293,6 +295,8 @@ def gcmma_loop(args):
     import oracle as O
     steps = args.opt_steps
     ctx = icx.Context(0)
+    if args.no_graphs:
+        ctx.set_option(icx.OPT_GRAPHS, 0)
     job = icx.cantilever_2d(ctx, 120, 40)
     x0 = np.full(job.nelx, 0.5)
     job.opt_init(x0, driver=2, filter=icx.FILTER_CONE, radius=2.0, form=icx.SOLVE_FUSED, tol=1e-10,
@@ -449,6 +453,8 @@ def mesh_solve(args):
         xr, cr = icx.read_mdpa(path, icx.HEXA8)
         t_read = time.perf_counter() - t0
     ctx = icx.Context(0)
+    if args.no_graphs:
+        ctx.set_option(icx.OPT_GRAPHS, 0)
     t0 = time.perf_counter()
     job = icx.Job.from_mesh(ctx, icx.HEXA8, xr, cr, icx.SOLID3D)
     t_create = time.perf_counter() - t0
