@@ -547,7 +547,7 @@ struct icx_job {
   bool mesh = false;
   std::vector<double> mx, mvol;
   std::vector<int> mconn;
-  DBuf<int> d_conn, d_adj_ptr, d_adj;
+  DBuf<int> d_conn, d_adj_ptr, d_slot;
   DBuf<double> d_xyz, d_ye, d_vol;
   MeshOp mop{};
   // state of a submitted (batched) mixed solve
@@ -995,7 +995,7 @@ icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
   }
   // connectivity node-of-element-major, node adjacency (e * nq + a, increasing e)
   const long long ne = g.ne, nn = g.nn;
-  std::vector<int> conn_t((size_t)nq * ne), ptr(nn + 1, 0), adj((size_t)nq * ne);
+  std::vector<int> conn_t((size_t)nq * ne), ptr(nn + 1, 0), adj((size_t)nq * ne), slot((size_t)nq * ne);
   for (long long e = 0; e < ne; ++e)
     for (int a = 0; a < nq; ++a) {
       conn_t[(size_t)a * ne + e] = j->mconn[e * nq + a];
@@ -1007,20 +1007,22 @@ icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
     for (long long e = 0; e < ne; ++e)
       for (int a = 0; a < nq; ++a) adj[fill[j->mconn[e * nq + a]]++] = (int)(e * nq + a);
   }
+  // the device takes the inverse: the row of (e, a) in node order
+  for (size_t k = 0; k < adj.size(); ++k) slot[adj[k]] = (int)k;
   (void)hipSetDevice(ctx->device);
   const int nd = nq * ndim;
   hipStream_t s = ctx->stream;
   std::vector<double> xyz((size_t)ndim * nn);
   for (long long n = 0; n < nn; ++n)
     for (int d = 0; d < ndim; ++d) xyz[(size_t)d * nn + n] = j->mx[3 * n + d];
-  if (j->d_conn.alloc(conn_t.size()) || j->d_adj_ptr.alloc(ptr.size()) || j->d_adj.alloc(adj.size()) ||
+  if (j->d_conn.alloc(conn_t.size()) || j->d_adj_ptr.alloc(ptr.size()) || j->d_slot.alloc(slot.size()) ||
       j->d_xyz.alloc(xyz.size()) || j->d_ye.alloc((size_t)ne * nd) || j->d_vol.alloc(ne)) {
     delete j;
     return fail(ctx, ICX_ERR_NOMEM, "device allocation failed");
   }
   if (hipMemcpyAsync(j->d_conn.p, conn_t.data(), conn_t.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
       hipMemcpyAsync(j->d_adj_ptr.p, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
-      hipMemcpyAsync(j->d_adj.p, adj.data(), adj.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
+      hipMemcpyAsync(j->d_slot.p, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, s) ||
       hipMemcpyAsync(j->d_xyz.p, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice, s) ||
       hipMemcpyAsync(j->d_vol.p, j->mvol.data(), ne * sizeof(double), hipMemcpyHostToDevice, s) ||
       hipStreamSynchronize(s)) {
@@ -1031,7 +1033,7 @@ icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
   int voigt;
   material_matrix(p.state, 1.0, p.poisson, D, voigt);
   j->mop = MeshOp{nq, ndim, nn, ne, j->d_conn.p, j->d_xyz.p, D[0], D[1], D[(voigt - 1) * voigt + voigt - 1],
-                  j->d_adj_ptr.p, j->d_adj.p, j->d_ye.p};
+                  j->d_adj_ptr.p, j->d_slot.p, j->d_ye.p};
   return job_finish(ctx, j, out);
 }
 

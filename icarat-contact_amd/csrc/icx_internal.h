@@ -205,7 +205,10 @@ struct MeshOp {
   const int *conn;
   const double *xyz;
   double d0, d1, d2;
-  const int *adj_ptr, *adj;
+  // element outputs in node order: node n's (element, corner) outputs are the rows
+  // adj_ptr[n] .. adj_ptr[n + 1] - 1 of ye (nc values each); slot[e * ne + a] is the row
+  // of element e's corner a
+  const int *adj_ptr, *slot;
   double *ye;
 };
 // y = K(E) x on the node vectors (SoA, nc components), zero on constrained components;

@@ -191,9 +191,12 @@ __global__ void __launch_bounds__(256) k_mesh_element(MeshOp m, const double *E,
       }
     }
     const double Ee = E[e];
-    double *out = ye + e * ND;
 #pragma unroll
-    for (int i = 0; i < ND; ++i) out[i] = Ee * f[i];
+    for (int a = 0; a < NE; ++a) {
+      double *out = ye + (long long)m.slot[e * NE + a] * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) out[c] = Ee * f[a * NC + c];
+    }
   }
 }
 
@@ -226,9 +229,12 @@ __global__ void __launch_bounds__(256) k_mesh_elem_diag(MeshOp m, const double *
       }
     }
     const double Ee = E[e];
-    double *out = ye + e * ND;
 #pragma unroll
-    for (int i = 0; i < ND; ++i) out[i] = Ee * dg[i];
+    for (int a = 0; a < NE; ++a) {
+      double *out = ye + (long long)m.slot[e * NE + a] * NC;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) out[c] = Ee * dg[a * NC + c];
+    }
   }
 }
 
@@ -241,13 +247,9 @@ __global__ void __launch_bounds__(256) k_mesh_gather(MeshOp m, const double *ye,
     double acc[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-    for (int k = m.adj_ptr[n]; k < m.adj_ptr[n + 1]; ++k) {
-      const long long v = m.adj[k];
-      const long long e = v / NE;
-      const int a = (int)(v - e * NE);
+    for (int k = m.adj_ptr[n]; k < m.adj_ptr[n + 1]; ++k)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc[c] += ye[e * ND + a * NC + c];
-    }
+      for (int c = 0; c < NC; ++c) acc[c] += ye[(long long)k * NC + c];
     const uint32_t msk = mask ? mask[n] : 0u;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
