@@ -72,6 +72,7 @@ def parse():
     ap.add_argument("--mdpa", action="store_true", help="unstructured hexa8 mesh from an MDPA file (FP64 plain form)")
     ap.add_argument("--heat-div", type=int, default=100)
     ap.add_argument("--gcmma", action="store_true", help="GCMMA design loop (driver 2) on the C1 cantilever")
+    ap.add_argument("--c1", action="store_true", help="the C1 design loop: OC driver, Q4 120 x 40, 50 iterations")
     ap.add_argument("--stress", action="store_true",
                     help="stress-constrained driver (C++ executable) on the reference's 200 x 100 job")
     ap.add_argument("--opt-steps", type=int, default=30)
@@ -322,6 +323,51 @@ def gcmma_loop(args):
             "config": {"workload": "GCMMA driver, Q4 120 x 40 cantilever, cone r = 2, %d steps" % steps,
                        "ms_per_step": wall / steps * 1e3, "solves": solves, "pcg_iterations": cg,
                        "inner_loops": [r["inner_loops"] for r in recs],
+                       "objective_first_last": [recs[0]["objective"], recs[-1]["objective"]]},
+            "cpu_baseline": cpu}
+    print(json.dumps(line))
+
+
+def c1_loop(args):
+    """The reference's CPU-runnable case: the OC driver (opt_macro_oc.cpp:84-172) on the
+    Q4 120 x 40 plane-stress cantilever (E0 = 1, Emin = 1e-9, nu = 0.3, p = 3, left edge
+    clamped, unit point load at the right mid-edge), cone filter r = 1.5, the reference's
+    projection schedule, x0 = 0.5, 50 iterations, FP64 fused-form solves to 1e-10 (the
+    reference's Eigen CG tolerance), the OC bisection on the device.  CPU baseline: the
+    oracle's loop (CG 1e-10) on the same job."""
+    import numpy as np
+
+    import icx
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    steps = 50
+    ctx = icx.Context(0)
+    if args.no_graphs:
+        ctx.set_option(icx.OPT_GRAPHS, 0)
+    job = icx.cantilever_2d(ctx, 120, 40)
+    x0 = np.full(job.nelx, 0.5)
+    job.opt_init(x0, driver=0, filter=icx.FILTER_CONE, radius=1.5, form=icx.SOLVE_FUSED, tol=1e-10,
+                 maxit=100000)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    recs = [job.opt_step() for _ in range(steps)]
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    cpu = None
+    if not args.no_cpu:
+        pr = O.cantilever_2d(120, 40)
+        t1 = time.perf_counter()
+        ref = pr.run_driver(0, x0, steps, 1.5, projection=True, direct=False, cg_tol=1e-10, cg_maxit=100000)
+        tc = time.perf_counter() - t1
+        cpu = {"value": steps / tc, "unit": "design iterations/s", "cores": os.cpu_count() or 1, "kind": "port",
+               "sample": "oracle OC loop, the same 50 iterations (CG 1e-10)",
+               "objective_max_rel_diff": float(np.max(np.abs(ref["obj"] - np.array([r["objective"] for r in recs]))
+                                                      / ref["obj"]))}
+    line = {"metric": "C1 design iterations/s", "value": steps / wall, "unit": "design iterations/s",
+            "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "OC driver, Q4 120 x 40 cantilever, cone r = 1.5, projection, 50 iterations",
+                       "ms_per_step": wall / steps * 1e3,
+                       "pcg_iterations": int(sum(r["cg_iterations"] for r in recs)),
                        "objective_first_last": [recs[0]["objective"], recs[-1]["objective"]]},
             "cpu_baseline": cpu}
     print(json.dumps(line))
@@ -676,6 +722,8 @@ def main():
         return
     if args.gcmma:
         return gcmma_loop(args)
+    if args.c1:
+        return c1_loop(args)
     if args.stress:
         return stress_driver(args)
     if args.heat:
