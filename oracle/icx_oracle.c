@@ -1718,6 +1718,30 @@ int icxo_mma_update(icxo_mma *M, double *x, const double *dfdx, const double *gx
   return mma_solve_dip(M, x);
 }
 
+static void copy_opt(double *dst, const double *src, size_t n)
+{
+  if (dst) memcpy(dst, src, n * sizeof(double));
+}
+
+void icxo_mma_subproblem(const icxo_mma *M, double *low, double *upp, double *alpha,
+                         double *beta, double *p0, double *q0, double *pij, double *qij,
+                         double *b, double *lam, double *y, double *z)
+{
+  size_t n = M->n, m = M->m;
+  copy_opt(low, M->low, n); copy_opt(upp, M->upp, n);
+  copy_opt(alpha, M->alpha, n); copy_opt(beta, M->beta, n);
+  copy_opt(p0, M->p0, n); copy_opt(q0, M->q0, n);
+  copy_opt(pij, M->pij, n * m); copy_opt(qij, M->qij, n * m);
+  copy_opt(b, M->b, m); copy_opt(lam, M->lam, m); copy_opt(y, M->y, m);
+  if (z) *z = M->z;
+}
+
+/* MMASolver::SetAsymptotes (src/optimization/opt-solver/MMASolver.hpp:91-97) */
+void icxo_mma_set_asymptotes(icxo_mma *M, double init, double decrease, double increase)
+{
+  M->asyminit = init; M->asymdec = decrease; M->asyminc = increase;
+}
+
 /* ------------------------------------------------------------------------ */
 /* GCMMA (m constraints): src/optimization/opt-solver/GCMMASolver.cpp:41-620. */
 /* The subproblem and its dual interior point solve (SolveDIP and helpers,   */

@@ -14,12 +14,15 @@
  * closed forms and hand-checked grids; element matrices by closed forms, rigid modes
  * and the patch test; solves by direct solutions and scipy; sensitivities, projection
  * and the stress constraint's adjoint derivative by finite differences; filters by
- * brute force, invariants and adjoint identities; OC by a replayed bisection; MMA and
+ * brute force, invariants and adjoint identities; OC by the closed-form limit lambda*
+ * of a separable problem (exponent, move limits, xmin, bracket) and a replayed
+ * bisection; MMA by the asymptote history worked by hand over designed x sequences
+ * (init rule, gamma 0.7 / 1.2 / 1, both clamps, the out-of-box rule), literal GenSub
+ * coefficients (0.001 and raa0 terms, alpha/beta with albefa and the move limit),
+ * first-order exactness at xval and the KKT conditions of the m = 1 subproblem;
  * GCMMA by closed-form optima, SLSQP on the subproblem, first-order exactness and the
  * RaaUpdate identity.  Parity unpinned: the exact Eigen CG iterates (the algorithm is
- * restated, iteration counts agree within about 1%), SimplicialLDLT's rounding, and the
- * exact multi-step MMASolver / GCMMASolver trajectories (each update is pinned by the
- * checks above; the history-dependent sequence only by this restatement).
+ * restated, iteration counts agree within about 1%) and SimplicialLDLT's rounding.
  */
 #ifndef ICX_ORACLE_H
 #define ICX_ORACLE_H
@@ -118,6 +121,14 @@ icxo_mma *icxo_mma_create(int n, int m, double a, double c, double d);
 void icxo_mma_destroy(icxo_mma *m);
 int icxo_mma_update(icxo_mma *m, double *x, const double *dfdx, const double *gx,
                     const double *dgdx, const double *xmin, const double *xmax);
+/* the subproblem of the last update (any pointer may be NULL): asymptotes low/upp, move
+ * bounds alpha/beta (n each), p0/q0 (n), pij/qij (n*m, row i*m+j), b (m), and the dual
+ * solution lam, y (m) and z */
+void icxo_mma_subproblem(const icxo_mma *m, double *low, double *upp, double *alpha,
+                         double *beta, double *p0, double *q0, double *pij, double *qij,
+                         double *b, double *lam, double *y, double *z);
+/* SetAsymptotes(init, decrease, increase) (MMASolver.hpp:91-97) */
+void icxo_mma_set_asymptotes(icxo_mma *m, double init, double decrease, double increase);
 /* GCMMASolver(n, m, a=0, c=1000, d=0): OuterUpdate, InnerUpdate, ConCheck
  * (src/optimization/opt-solver/GCMMASolver.cpp:41-129); destroy with icxo_mma_destroy */
 icxo_mma *icxo_gcmma_create(int n, int m, double a, double c, double d);

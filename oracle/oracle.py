@@ -442,6 +442,31 @@ class MMA:
                                    _f64(xmin), _f64(xmax))
         return x, it
 
+    def set_asymptotes(self, init, decrease=0.7, increase=1.2):
+        L = lib()
+        L.icxo_mma_set_asymptotes.restype = None
+        L.icxo_mma_set_asymptotes.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.icxo_mma_set_asymptotes(self.h, init, decrease, increase)
+
+    def subproblem(self):
+        """the last update's subproblem: low, upp, alpha, beta, p0, q0, pij, qij (n x m),
+        b, and the dual solution lam, y, z"""
+        n, m = self.n, self.m
+        out = {k: np.zeros(n) for k in ("low", "upp", "alpha", "beta", "p0", "q0")}
+        out.update(pij=np.zeros(n * m), qij=np.zeros(n * m), b=np.zeros(m), lam=np.zeros(m),
+                   y=np.zeros(m))
+        z = C.c_double()
+        L = lib()
+        L.icxo_mma_subproblem.restype = None
+        L.icxo_mma_subproblem.argtypes = [C.c_void_p] + [_dp] * 11 + [C.POINTER(C.c_double)]
+        L.icxo_mma_subproblem(self.h, out["low"], out["upp"], out["alpha"], out["beta"],
+                              out["p0"], out["q0"], out["pij"], out["qij"], out["b"],
+                              out["lam"], out["y"], C.byref(z))
+        out["pij"] = out["pij"].reshape(n, m)
+        out["qij"] = out["qij"].reshape(n, m)
+        out["z"] = z.value
+        return out
+
 
 class GCMMA:
     """GCMMASolver(n, m, a=0, c=1000, d=0) (src/optimization/opt-solver/GCMMASolver.cpp:41-129)."""
