@@ -551,7 +551,8 @@ def random_job(ctx, rng, dims=None, size2d=(60, 240), size3d=(32, 96), young=(1e
     """One job of the batched workload: Q4 d0 x d0/3 (plane stress) or hex8 d0 x d0/2 x
     d0/2, h = 1, d0 uniform in the given range; the x = 0 or the x = L face clamped; a
     unit point load (-y in 2D, -z in 3D) at a random node of the opposite face; uniform
-    density = volume fraction ~ U[0.2, 0.5].  rng: numpy Generator.  Returns the job."""
+    density = volume fraction ~ U[0.2, 0.5].  rng: numpy Generator.  Returns the job, with
+    its clamped box and load recorded in `spec` (flag, clamp range, (node, comp, value))."""
     ndim = dims if dims is not None else (2 if rng.random() < 0.5 else 3)
     side = int(rng.integers(0, 2))
     if ndim == 2:
@@ -563,6 +564,8 @@ def random_job(ctx, rng, dims=None, size2d=(60, 240), size3d=(32, 96), young=(1e
         j.make_dof()
         node = j.node_index(d0 if side == 0 else 0, int(rng.integers(0, d1 + 1)))
         j.add_nodal_load(node, 1, -1.0)
+        j.spec = {"clamp": [x, x, 0.0, float(d1), 0.0, 0.0], "flag": [1, 1, 0, 0, 0, 0],
+                  "load": (node, 1, -1.0)}
         ne = d0 * d1
     else:
         d0 = int(rng.integers(size3d[0], size3d[1] + 1))
@@ -573,6 +576,8 @@ def random_job(ctx, rng, dims=None, size2d=(60, 240), size3d=(32, 96), young=(1e
         j.make_dof()
         node = j.node_index(d0 if side == 0 else 0, int(rng.integers(0, d1 + 1)), int(rng.integers(0, d1 + 1)))
         j.add_nodal_load(node, 2, -1.0)
+        j.spec = {"clamp": [x, x, 0.0, float(d1), 0.0, float(d1)], "flag": [1, 1, 1, 0, 0, 0],
+                  "load": (node, 2, -1.0)}
         ne = d0 * d1 * d1
     j.volfrac = float(rng.uniform(0.2, 0.5))
     j.set_design(np.full(ne, j.volfrac))
