@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <thread>
@@ -28,8 +29,17 @@ struct BatchRun;
 static void batch_delete(BatchRun *b);
 static void batch_remove(icx_ctx *c, icx_job *j);
 
+// Scratch of icx_matvec_noalloc (the entry point for callers that cannot allocate
+// device memory): one array per GPU, held until the process ends.  Host code never
+// uses the symbol itself (on the host it names a stand-in, not device memory); each
+// device's address comes from hipGetSymbolAddress (scratch_address, called by
+// icx_ctx_create with that device current) and is kept per device.
+constexpr size_t ICX_NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[ICX_NOALLOC_SCRATCH_BYTES];
+
 struct icx_ctx {
   int device = 0;
+  unsigned char *scratch = nullptr;  // this device's copy of `scratch` (device address)
   hipStream_t stream = nullptr;
   std::string err;
   SolverState *h_state = nullptr;  // pinned mirrors for polling, two slots
@@ -64,6 +74,23 @@ icx_status fail(icx_ctx *ctx, icx_status code, const char *fmt, ...) {
   va_end(ap);
   if (ctx) ctx->err = buf;
   return code;
+}
+
+// The device address of `scratch` on `device`, looked up once per device (the caller
+// has made `device` current) and kept for every later context on it.
+hipError_t scratch_address(int device, unsigned char **out) {
+  static std::mutex mu;
+  static std::vector<unsigned char *> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)addr.size() <= device) addr.resize(device + 1, nullptr);
+  if (!addr[device]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = static_cast<unsigned char *>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
 }
 
 #define HIPCHK(ctx, call)                                                                     \
@@ -744,6 +771,57 @@ icx_status pcg_plain(icx_job *j, long long N, const double *b, double *x, const 
 }  // namespace
 
 // =====================================================================================
+// matvec staging (icx_matvec allocates it per call, icx_matvec_noalloc takes it from the
+// device's scratch array)
+// =====================================================================================
+namespace {
+
+struct MatvecStaging {
+  double *dof = nullptr;                           // numeq doubles: x in, y out
+  float *xf = nullptr, *yf = nullptr, *Ef = nullptr;  // FP32 operator: grid vectors, moduli
+};
+
+size_t noalloc_round(size_t b) { return (b + 255) & ~size_t(255); }
+
+icx_status matvec_check(icx_job *j, int prec, const double *x, double *y) {
+  if (!j || !x || !y) return ICX_ERR_ARG;
+  if (j->numeq < 0 || !j->design_set)
+    return fail(j->ctx, ICX_ERR_STATE, "icx_matvec needs icx_make_dof and icx_set_design");
+  if (prec != 0 && prec != 1) return fail(j->ctx, ICX_ERR_ARG, "icx_matvec: precision 0 (FP64) or 1 (FP32)");
+  if (prec != 0 && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: FP64 operator only");
+  if (prec != 0 && j->heat()) return fail(j->ctx, ICX_ERR_ARG, "heat jobs: FP64 operator only");
+  return ICX_OK;
+}
+
+// y = K x through the staging buffers: every staged word is written before it is read
+// (x by the copy, the FP32 operands by the conversions, y by the operator), so nothing
+// has to start at zero
+icx_status matvec_staged(icx_job *j, int prec, const double *x, double *y, const MatvecStaging &sg) {
+  icx_ctx *c = j->ctx;
+  const long long N = j->nvec();
+  HIPCHK(c, hipMemcpyAsync(sg.dof, x, j->numeq * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  launch_dof_to_grid(j->g, j->nc, j->d_dof.p, j->numeq, sg.dof, nullptr, j->d_vec.p, S(j));
+  if (j->mesh) {
+    launch_mesh_matvec(j->mop, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
+  } else if (j->heat()) {
+    launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
+  } else if (prec == 0) {
+    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
+  } else {
+    launch_convert_f64_f32(N, j->d_vec.p, sg.xf, S(j));
+    launch_convert_f64_f32(j->g.ne, j->d_E.p, sg.Ef, S(j));
+    launch_elastic_matvec<float>(j->g, j->op32, sg.Ef, j->d_mask.p, sg.xf, sg.yf, nullptr, S(j));
+    launch_convert_f32_f64(N, sg.yf, j->d_vec2.p, S(j));
+  }
+  launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_vec2.p, sg.dof, S(j));
+  HIPCHK(c, hipMemcpyAsync(y, sg.dof, j->numeq * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
+}  // namespace
+
+// =====================================================================================
 // C ABI
 // =====================================================================================
 extern "C" {
@@ -756,7 +834,8 @@ icx_status icx_ctx_create(int device, icx_ctx **out) {
   if (device < 0 || device >= ndev) return ICX_ERR_ARG;
   auto *c = new icx_ctx();
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipSetDevice(device) != hipSuccess || scratch_address(device, &c->scratch) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipHostMalloc((void **)&c->h_state, 2 * sizeof(SolverState), hipHostMallocDefault) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_state[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_state[1], hipEventDisableTiming) != hipSuccess ||
@@ -1402,34 +1481,43 @@ icx_status icx_set_conductivity(icx_job *j, const double *k1, const double *k2) 
 }
 
 icx_status icx_matvec(icx_job *j, int prec, const double *x, double *y) {
-  if (!j || !x || !y) return ICX_ERR_ARG;
-  if (j->numeq < 0 || !j->design_set) return fail(j->ctx, ICX_ERR_STATE, "icx_matvec needs icx_make_dof and icx_set_design");
+  icx_status st = matvec_check(j, prec, x, y);
+  if (st != ICX_OK) return st;
   icx_ctx *c = j->ctx;
-  const long long N = j->nvec();
+  MatvecStaging sg;
   DBuf<double> tmp;
+  DBuf<float> xf, yf, Ef;
   HIPCHK(c, tmp.alloc(std::max(j->numeq, 1)));
-  HIPCHK(c, hipMemcpyAsync(tmp.p, x, j->numeq * sizeof(double), hipMemcpyHostToDevice, S(j)));
-  launch_dof_to_grid(j->g, j->nc, j->d_dof.p, j->numeq, tmp.p, nullptr, j->d_vec.p, S(j));
-  if (j->mesh) {
-    if (prec != 0) return fail(c, ICX_ERR_ARG, "mesh jobs: FP64 operator only");
-    launch_mesh_matvec(j->mop, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
-  } else if (j->heat()) {
-    if (prec != 0) return fail(c, ICX_ERR_ARG, "heat jobs: FP64 operator only");
-    launch_heat_matvec(j->g, j->heat_op, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
-  } else if (prec == 0) {
-    launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_vec.p, j->d_vec2.p, nullptr, S(j));
-  } else {
-    DBuf<float> xf, yf, Ef;
-    HIPCHK(c, xf.alloc(N)); HIPCHK(c, yf.alloc(N)); HIPCHK(c, Ef.alloc(j->g.ne));
-    launch_convert_f64_f32(N, j->d_vec.p, xf.p, S(j));
-    launch_convert_f64_f32(j->g.ne, j->d_E.p, Ef.p, S(j));
-    launch_elastic_matvec<float>(j->g, j->op32, Ef.p, j->d_mask.p, xf.p, yf.p, nullptr, S(j));
-    launch_convert_f32_f64(N, yf.p, j->d_vec2.p, S(j));
+  sg.dof = tmp.p;
+  if (prec != 0) {
+    HIPCHK(c, xf.alloc(j->nvec())); HIPCHK(c, yf.alloc(j->nvec())); HIPCHK(c, Ef.alloc(j->g.ne));
+    sg.xf = xf.p; sg.yf = yf.p; sg.Ef = Ef.p;
   }
-  launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_vec2.p, tmp.p, S(j));
-  HIPCHK(c, hipMemcpyAsync(y, tmp.p, j->numeq * sizeof(double), hipMemcpyDeviceToHost, S(j)));
-  HIPCHK(c, hipStreamSynchronize(S(j)));
-  return ICX_OK;
+  return matvec_staged(j, prec, x, y, sg);
+}
+
+icx_status icx_matvec_noalloc(icx_job *j, int prec, const double *x, double *y) {
+  icx_status st = matvec_check(j, prec, x, y);
+  if (st != ICX_OK) return st;
+  icx_ctx *c = j->ctx;
+  if (!c->scratch) return fail(c, ICX_ERR_STATE, "icx_matvec_noalloc: no scratch address for device %d", c->device);
+  // carve the staging buffers out of the device's scratch array, 256-byte aligned
+  const size_t sz_dof = noalloc_round(sizeof(double) * (size_t)std::max(j->numeq, 1));
+  const size_t sz_vec = noalloc_round(sizeof(float) * (size_t)j->nvec());
+  const size_t sz_el = noalloc_round(sizeof(float) * (size_t)j->g.ne);
+  const size_t need = sz_dof + (prec != 0 ? 2 * sz_vec + sz_el : 0);
+  if (need > ICX_NOALLOC_SCRATCH_BYTES)
+    return fail(c, ICX_ERR_ARG,
+                "icx_matvec_noalloc: this job needs %zu bytes of scratch (precision %d), the maximum is %zu",
+                need, prec, (size_t)ICX_NOALLOC_SCRATCH_BYTES);
+  MatvecStaging sg;
+  sg.dof = reinterpret_cast<double *>(c->scratch);
+  if (prec != 0) {
+    sg.xf = reinterpret_cast<float *>(c->scratch + sz_dof);
+    sg.yf = reinterpret_cast<float *>(c->scratch + sz_dof + sz_vec);
+    sg.Ef = reinterpret_cast<float *>(c->scratch + sz_dof + 2 * sz_vec);
+  }
+  return matvec_staged(j, prec, x, y, sg);
 }
 
 icx_status icx_diagonal(icx_job *j, double *diag) {

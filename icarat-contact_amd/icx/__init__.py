@@ -24,7 +24,7 @@ EXPORTS = [
     "icx_job_create", "icx_job_create_mesh", "icx_mdpa_read", "icx_job_destroy", "icx_job_sizes", "icx_get_mesh",
     "icx_add_dirichlet", "icx_make_dof", "icx_get_dof_map", "icx_add_neumann",
     "icx_set_fext", "icx_get_fext", "icx_add_nodal_load",
-    "icx_set_design", "icx_set_conductivity", "icx_matvec", "icx_diagonal", "icx_solve", "icx_get_displacement",
+    "icx_set_design", "icx_set_conductivity", "icx_matvec", "icx_matvec_noalloc", "icx_diagonal", "icx_solve", "icx_get_displacement",
     "icx_get_node_values", "icx_compliance", "icx_sensitivity", "icx_element_volume",
     "icx_mises", "icx_reactions", "icx_output_begin", "icx_output_write", "icx_write_vtk",
     "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
@@ -303,6 +303,14 @@ class Job:
         x = _f64(x)
         y = np.zeros(self.numeq)
         self.ctx.check(lib().icx_matvec(self.h, int(prec), _dp(x), _dp(y)))
+        return y
+
+    def matvec_noalloc(self, x, prec=0):
+        """icx_matvec_noalloc: the same product with the library's per-GPU scratch array
+        as device staging (nothing allocated)"""
+        x = _f64(x)
+        y = np.zeros(self.numeq)
+        self.ctx.check(lib().icx_matvec_noalloc(self.h, int(prec), _dp(x), _dp(y)))
         return y
 
     def diagonal(self):

@@ -152,6 +152,17 @@ icx_status icx_set_conductivity(icx_job *job, const double *k1, const double *k2
 /* y = K x on the free DOFs (matrix-free; x, y numeq doubles).
  * prec: 0 = FP64, 1 = FP32 operator (inputs rounded to FP32). */
 icx_status icx_matvec(icx_job *job, int prec, const double *x, double *y);
+/* No counterpart in the reference.  icx_matvec for callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc): the same kernels, with the
+ * device staging (x and y on the DOF axis; for prec 1 also the FP32 operand, result and
+ * moduli) taken from a 64 MiB array that the library holds on each GPU until the process
+ * ends, instead of buffers allocated per call.  Calls no allocating routine.
+ * Limit: 8 numeq bytes (prec 0), plus 8 N_grid + 4 nelx bytes for prec 1, each rounded
+ * up to 256, at most 64 MiB (N_grid = nodes x DOFs per node); larger jobs get
+ * ICX_ERR_ARG.  Every call on one GPU uses the same array: calls that overlap on one GPU
+ * (two contexts or threads at once) race; each call waits for its work before it
+ * returns, so calls made one after another are safe. */
+icx_status icx_matvec_noalloc(icx_job *job, int prec, const double *x, double *y);
 /* Jacobi diagonal of K (numeq doubles) */
 icx_status icx_diagonal(icx_job *job, double *diag);
 /* LinearAnalysis::solve (src/analysis/linear.hpp:54-104) with the solver
