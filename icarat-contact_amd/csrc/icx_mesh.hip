@@ -13,7 +13,7 @@
 //     element order (node adjacency in CSR), zero on constrained components.
 // The Jacobi diagonal (the element pass writes diag Ke0_e) and the element energies
 // u_e^T Ke0_e u_e = sum_ip eps^T D eps det J use the same integration.
-#include "icx_common.cuh"
+#include "icx_device.h"
 
 namespace icx {
 

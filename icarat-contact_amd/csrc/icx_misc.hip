@@ -1,7 +1,7 @@
 // Element-level and streaming kernels of libicx (gfx950): SIMP moduli, Jacobi
 // diagonals, layout conversions, sensitivity, cone filter, projection, Helmholtz
 // right-hand side, OC and MMA pieces.
-#include "icx_common.cuh"
+#include "icx_device.h"
 
 #include <math.h>
 

@@ -1,6 +1,6 @@
 // Matrix-free structured operators of libicx (gfx950): elastic hexa8 / quad4 and the
 // scalar Helmholtz operator.
-#include "icx_common.cuh"
+#include "icx_device.h"
 
 namespace icx {
 

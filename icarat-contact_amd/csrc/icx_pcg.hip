@@ -1,7 +1,7 @@
 // PCG vector kernels of libicx (gfx950).  Every reducing kernel writes one partial
 // per block; the totals are formed by the kernels that consume them (see
-// icx_common.cuh), indexed by the parity of the iteration that produced them.
-#include "icx_common.cuh"
+// icx_device.h), indexed by the parity of the iteration that produced them.
+#include "icx_device.h"
 
 namespace icx {
 

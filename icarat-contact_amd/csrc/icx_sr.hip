@@ -28,7 +28,7 @@
 // iterate k-1 is formed again from the new r: p = z + beta' p_{k-2}, beta' = rz' / rz_{k-2}
 // (0 at the set-up), Ap = K p and its partials, as the two-kernel form continues the
 // direction after a replacement.
-#include "icx_common.cuh"
+#include "icx_device.h"
 
 namespace icx {
 
