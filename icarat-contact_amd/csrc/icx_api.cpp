@@ -377,6 +377,11 @@ bool build_elastic_op(const BoxElement &el, const std::vector<double> &K, Elasti
         why = "element diagonal differs between nodes";
         return false;
       }
+    if (!(op.kd[c] > 0.0)) {
+      why = "element diagonal is not positive";
+      return false;
+    }
+    op.ikd[c] = 1.0 / op.kd[c];
   }
   return true;
 }
@@ -408,7 +413,10 @@ template <typename T> ElasticOp<T> cast_op(const ElasticOp<double> &a) {
   for (int i = 0; i < 8; ++i)
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k) b.blk[i][j][k] = (T)a.blk[i][j][k];
-  for (int j = 0; j < 3; ++j) b.kd[j] = (T)a.kd[j];
+  for (int j = 0; j < 3; ++j) {
+    b.kd[j] = (T)a.kd[j];
+    b.ikd[j] = (T)a.ikd[j];
+  }
   return b;
 }
 
@@ -436,7 +444,7 @@ struct PcgWork64 {
   DBuf<double> r, p, t, z, invd;
 };
 struct MixedWork {
-  DBuf<float> r, pa, pb, Ap, Ap2, xlo, invd, E, b32;  // Ap2: seam halves of kernel A
+  DBuf<float> r, pa, pb, Ap, Ap2, xlo, invd, invs, E, b32;  // Ap2: seam halves of kernel A
   DBuf<double> Kx;
   // single-reduction form (hexa8 solves): r = {r, r1}, p = {pa, pb}, Ap = {Ap, ap1};
   // per-block partials by launch parity
@@ -445,7 +453,7 @@ struct MixedWork {
   DBuf<double> part;
 };
 struct FusedWork {
-  DBuf<double> r, pa, pb, Ap, invd;
+  DBuf<double> r, pa, pb, Ap, invd, invs;
 };
 struct HelmWork {
   bool ready = false;
@@ -1565,9 +1573,9 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
   const long long N = j->nvec();
   FusedWork &w = j->fw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N));
-  HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N));
+  HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.invs.alloc(j->g.nn));
   const PcgPartials P = j->parts(elastic_fusedA_blocks<double>(j->g), pcg_stream_blocks(N));
-  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, w.invd.p, nullptr, S(j));
+  launch_elastic_invdiag_node<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(double), S(j)));
@@ -1580,8 +1588,8 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
     for (int k = 0; k < every; ++k) {
       {
         ProfScope ps(c, ICX_PROF_FUSED_A);
-        launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p,
-                                      w.Ap.p, nullptr, j->d_u.p, j->d_state.p, P, S(j), ps.ev());
+        launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p,
+                                      w.pb.p, w.Ap.p, nullptr, j->d_u.p, j->d_state.p, P, S(j), ps.ev());
       }
       {
         ProfScope ps(c, ICX_PROF_FUSED_B);
@@ -1615,7 +1623,8 @@ static PcgPartials mixed_parts(icx_job *j) {
 static SeamGeom mixed_seams(icx_job *j) {
   OpArgs<float> a;
   MixedWork &w = j->mw;
-  fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p, w.Ap2.p, w.xlo.p,
+  fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p, w.pb.p, w.Ap.p, w.Ap2.p,
+                     w.xlo.p,
                      j->d_state.p, mixed_parts(j), a);
   return fusedA_seams(a);
 }
@@ -1652,6 +1661,7 @@ static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st
   w.sr = sr;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N)); HIPCHK(c, w.Ap.alloc(N));
   HIPCHK(c, w.xlo.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.E.alloc(j->g.ne)); HIPCHK(c, w.Kx.alloc(N));
+  HIPCHK(c, w.invs.alloc(j->g.nn));
   HIPCHK(c, w.b32.alloc(N));
   if (sr) {
     HIPCHK(c, w.r1.alloc(N)); HIPCHK(c, w.ap1.alloc(N)); HIPCHK(c, w.part.alloc(12 * RED_MAXBLOCKS));
@@ -1661,7 +1671,7 @@ static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st
   HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(float), S(j)));
   const PcgPartials P = mixed_parts(j);
   launch_convert_f64_f32(j->g.ne, j->d_E.p, w.E.p, S(j));
-  launch_elastic_invdiag(j->g, j->op64.kd, j->d_E.p, j->d_mask.p, nullptr, w.invd.p, S(j));
+  launch_elastic_invdiag_node<float>(j->g, j->op32, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.xlo.p, 0, N * sizeof(float), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(float), S(j)));
@@ -1694,7 +1704,8 @@ static void mixed_iteration(icx_job *j) {
   const PcgPartials P = mixed_parts(j);
   {
     ProfScope ps(c, ICX_PROF_FUSED_A);
-    launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
+    launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p, w.pb.p,
+                                 w.Ap.p,
                                  w.Ap2.p, w.xlo.p, j->d_state.p, P, S(j), ps.ev());
   }
   {
@@ -1811,7 +1822,8 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
     }
     const PcgPartials P = mixed_parts(j);
     OpArgs<float> a;
-    const int nb = fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.pb.p, w.Ap.p,
+    const int nb = fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p, w.pb.p,
+                                      w.Ap.p,
                                       w.Ap2.p, w.xlo.p, j->d_state.p, P, a);
     if (j->g.ndim == 3) { a3.push_back(a); o3.push_back(o3.back() + pad8(nb)); }
     else { a2.push_back(a); o2.push_back(o2.back() + pad8(nb)); }
@@ -3204,8 +3216,8 @@ icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
   add(j->d_design); add(j->d_E); add(j->d_vec); add(j->d_vec2); add(j->d_partial); add(j->d_pcgpart);
   add(j->d_state); add(j->d_scalars);
   for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(*d);
-  for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd}) add(*d);
-  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.Ap2, &j->mw.xlo, &j->mw.invd, &j->mw.E,
+  for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd, &j->fw.invs}) add(*d);
+  for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.Ap2, &j->mw.xlo, &j->mw.invd, &j->mw.invs, &j->mw.E,
                   &j->mw.b32, &j->mw.r1, &j->mw.ap1})
     add(*d);
   add(j->mw.Kx); add(j->mw.part);

@@ -25,7 +25,13 @@ struct Grid {
 template <typename T> struct ElasticOp {
   T blk[8][3][3];
   T kd[3];             // diagonal of K0 per displacement component
+  T ikd[3];            // 1 / kd, rounded once to T (the per-node Jacobi factor)
 };
+
+// Jacobi factor of a box job's DOF from its node's inverse modulus sum s (s < 0 marks a
+// zero diagonal, where the inverse is 1 as in Eigen's DiagonalPreconditioner) and the
+// component's 1 / kd: one multiply, so every kernel that forms it gets the same bits.
+template <typename T> __host__ __device__ inline T jacobi_inv(T s, T ik) { return s < T(0) ? T(1) : s * ik; }
 
 // Helmholtz element matrix (scalar), diagonal in the Haar basis.
 template <typename T> struct HelmOp {
@@ -143,7 +149,8 @@ template <typename T> struct OpArgs {
   const T *E;
   const uint8_t *mask;
   const T *in;        // MODE 0: x; MODE 1: r
-  const T *invd;      // MODE 1
+  const T *invd;      // MODE 1 quad4: Jacobi inverse per DOF
+  const T *invs;      // MODE 1 hexa8: per node 1 / (sum of adjacent moduli), -1 for 0
   T *P0, *P1;         // MODE 1: direction buffers, p_i lives in P[i & 1]
   T *y;               // K x (MODE 0) / K p_i (MODE 1)
   T *y2;              // FP32 hexa8 MODE 1: lower halves of the seam layers of K p_i
@@ -177,12 +184,12 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
 // Ap2 is touched on the seam layers only.
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st,
-                           PcgPartials P, hipStream_t s, const LaunchEvents *ev = nullptr);
+                           const T *r, const T *invd, const T *invs, T *P0, T *P1, T *Ap, T *Ap2, T *x,
+                           SolverState *st, PcgPartials P, hipStream_t s, const LaunchEvents *ev = nullptr);
 // kernel A's arguments for one job; returns its block count
 template <typename T>
 int fusedA_args(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask, const T *r, const T *invd,
-                T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a);
+                const T *invs, T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a);
 // the seam layers of a kernel A launch (kmax = 0 when it has none)
 template <typename T> SeamGeom fusedA_seams(const OpArgs<T> &a);
 // kernel A for a batch of jobs of one dimension (args and off in device memory)
@@ -344,6 +351,14 @@ void launch_simp(long long ne, double y1, double y2, double p, const double *des
 // Jacobi inverse diagonal from E: invd[c][n] = 1/(kd_c * sum_{e ni n} E_e) (0 at constrained)
 void launch_elastic_invdiag(const Grid &g, const double *kd, const double *E, const uint8_t *mask,
                             double *invd64, float *invd32, hipStream_t s);
+// Jacobi preconditioner of the fused and mixed forms on a box job, in T: per node
+// invs[n] = 1 / S_n (S_n = sum of the moduli of the elements around node n; -1 where
+// S_n = 0) and per DOF invd[c][n] = jacobi_inv(invs[n], op.ikd[c]) (0 at constrained
+// DOFs).  Kernel A of a hexa8 job forms the same factor from invs (one word per node
+// instead of one per DOF); kernel B and the set-up read invd.
+template <typename T>
+void launch_elastic_invdiag_node(const Grid &g, const ElasticOp<T> &op, const double *E, const uint8_t *mask,
+                                 T *invs, T *invd, hipStream_t s);
 void launch_helm_invdiag(const Grid &g, double kdiag, double *invd, hipStream_t s);
 void launch_dof_to_grid(const Grid &g, int nc, const int *dof, int numeq, const double *v,
                         const double *dirich /* nullable: per node*6 */, double *grid,

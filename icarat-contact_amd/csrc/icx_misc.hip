@@ -64,6 +64,40 @@ void launch_elastic_invdiag(const Grid &g, const double *kd, const double *E, co
   ICX_CHECK_LAUNCH();
 }
 
+template <typename T>
+__global__ void k_elastic_invdiag_node(Grid g, T ik0, T ik1, T ik2, const double *E, const uint8_t *mask, T *invs,
+                                       T *invd) {
+  const int nc = g.ndim;
+  const T ik[3] = {ik0, ik1, ik2};
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
+       n += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.sx), j = (int)((n / g.sx) % g.sy), k = (int)(n / ((long long)g.sx * g.sy));
+    double S = 0.0;
+    for (int dz = (g.ndim == 3 ? -1 : 0); dz <= 0; ++dz)
+      for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+          const int ex = i + dx, ey = j + dy, ez = k + dz;
+          if (ex < 0 || ey < 0 || ez < 0 || ex >= g.nx || ey >= g.ny || ez >= g.nz) continue;
+          S += E[ex + (long long)g.nx * (ey + (long long)g.ny * ez)];
+        }
+    const T s = S != 0.0 ? (T)(1.0 / S) : T(-1);
+    invs[n] = s;
+    const uint8_t m = mask ? mask[n] : 0;
+    for (int c = 0; c < nc; ++c) invd[c * g.nn + n] = ((m >> c) & 1) ? T(0) : jacobi_inv(s, ik[c]);
+  }
+}
+template <typename T>
+void launch_elastic_invdiag_node(const Grid &g, const ElasticOp<T> &op, const double *E, const uint8_t *mask,
+                                 T *invs, T *invd, hipStream_t s) {
+  hipLaunchKernelGGL(k_elastic_invdiag_node<T>, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, op.ikd[0], op.ikd[1],
+                     op.ikd[2], E, mask, invs, invd);
+  ICX_CHECK_LAUNCH();
+}
+template void launch_elastic_invdiag_node<float>(const Grid &, const ElasticOp<float> &, const double *,
+                                                 const uint8_t *, float *, float *, hipStream_t);
+template void launch_elastic_invdiag_node<double>(const Grid &, const ElasticOp<double> &, const double *,
+                                                  const uint8_t *, double *, double *, hipStream_t);
+
 __global__ void k_helm_invdiag(Grid g, double kdiag, double *invd) {
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
        n += (long long)gridDim.x * blockDim.x) {

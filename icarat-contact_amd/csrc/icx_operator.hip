@@ -123,20 +123,18 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   }
   // position p is loaded by the waves that hold a slot of it
   auto pos_live = [&](int p) { return p * NT + wave * 64 < LN; };
-  T pr[NPOS][3], pv[NPOS][3], po[NPOS][3];
+  T pr[NPOS][3], pv[NPOS], po[NPOS][3];
   auto prefetch = [&](int k) {
     const uint32_t lo = (k >= 0 && k < g.sz) ? (uint32_t)k * n_layer : 0u;
 #pragma unroll
     for (int p = 0; p < NPOS; ++p) {
       if (!pos_live(p)) continue;
       const uint32_t o = pos_off[p] + lo;
+      if (MODE == 1) pv[p] = ld_off(a.invs, o);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         pr[p][c] = ld_off(a.in, o + c * cstr);
-        if (MODE == 1) {
-          pv[p][c] = ld_off(a.invd, o + c * cstr);
-          po[p][c] = ld_off(pold, o + c * cstr);
-        }
+        if (MODE == 1) po[p][c] = ld_off(pold, o + c * cstr);
       }
     }
   };
@@ -149,7 +147,8 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
       if (idx < LN) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          s_in[buf][c][idx] = MODE == 0 ? pr[p][c] : fma_t(beta, po[p][c], pv[p][c] * pr[p][c]);
+          s_in[buf][c][idx] =
+              MODE == 0 ? pr[p][c] : fma_t(beta, po[p][c], jacobi_inv(pv[p], a.op.ikd[c]) * pr[p][c]);
       }
     }
   };
@@ -257,11 +256,11 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     for (int p = 0; p < NPOS; ++p) {
       if (!pos_live(p)) continue;
       const uint32_t o = pos_off[p] + lo;
+      if (MODE == 1) pv[p] = ld_off(a.invs, o);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         pr[p][c] = ld_off(a.in, o + c * cstr);
         if (MODE == 1) {
-          pv[p][c] = ld_off(a.invd, o + c * cstr);
           po[p][c] = ld_off((const T *)a.P0, o + c * cstr);
           po_b[p][c] = ld_off((const T *)a.P1, o + c * cstr);
         }
@@ -547,9 +546,9 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
 
 template <typename T>
 int fusedA_args(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask, const T *r, const T *invd,
-                T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a) {
+                const T *invs, T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st, PcgPartials P, OpArgs<T> &a) {
   a = OpArgs<T>{};
-  a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = r; a.invd = invd; a.P0 = P0; a.P1 = P1;
+  a.g = g; a.op = op; a.E = E; a.mask = mask; a.in = r; a.invd = invd; a.invs = invs; a.P0 = P0; a.P1 = P1;
   a.y = Ap; a.y2 = Ap2; a.x = x; a.st = st; a.P = P;
   int nb, nt;
   setup_tiles(g, a, nb, nt, fusedA_seam<T>(g.ndim));
@@ -571,10 +570,10 @@ template <typename T> SeamGeom fusedA_seams(const OpArgs<T> &a) {
 
 template <typename T>
 void launch_elastic_fusedA(const Grid &g, const ElasticOp<T> &op, const T *E, const uint8_t *mask,
-                           const T *r, const T *invd, T *P0, T *P1, T *Ap, T *Ap2, T *x, SolverState *st,
-                           PcgPartials P, hipStream_t s, const LaunchEvents *ev) {
+                           const T *r, const T *invd, const T *invs, T *P0, T *P1, T *Ap, T *Ap2, T *x,
+                           SolverState *st, PcgPartials P, hipStream_t s, const LaunchEvents *ev) {
   OpArgs<T> a;
-  const int nb = fusedA_args(g, op, E, mask, r, invd, P0, P1, Ap, Ap2, x, st, P, a);
+  const int nb = fusedA_args(g, op, E, mask, r, invd, invs, P0, P1, Ap, Ap2, x, st, P, a);
   if (g.ndim == 3) launch_ev((k_hex8<T, HexTile<T>::TI, HexTile<T>::TJ, 1>), nb, (int)NTHREADS3<T>, s, ev, a);
   else launch_ev((k_quad4<T, QUAD_TI, 1>), nb, QUAD_TI + 1, s, ev, a);
   ICX_CHECK_LAUNCH();
@@ -593,8 +592,8 @@ void launch_fusedA_batch(int ndim, const OpArgs<T> *args, const int *off, int nj
   ICX_CHECK_LAUNCH();
 }
 template int fusedA_args<float>(const Grid &, const ElasticOp<float> &, const float *, const uint8_t *, const float *,
-                                const float *, float *, float *, float *, float *, float *, SolverState *, PcgPartials,
-                                OpArgs<float> &);
+                                const float *, const float *, float *, float *, float *, float *, float *,
+                                SolverState *, PcgPartials, OpArgs<float> &);
 template SeamGeom fusedA_seams<float>(const OpArgs<float> &);
 template SeamGeom fusedA_seams<double>(const OpArgs<double> &);
 template void launch_fusedA_batch<float>(int, const OpArgs<float> *, const int *, int, int, hipStream_t);
@@ -608,11 +607,13 @@ template void launch_elastic_matvec<float>(const Grid &, const ElasticOp<float> 
                                            const uint8_t *, const float *, float *, const SolverState *,
                                            hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<double>(const Grid &, const ElasticOp<double> &, const double *,
-                                            const uint8_t *, const double *, const double *, double *,
+                                            const uint8_t *, const double *, const double *, const double *,
+                                            double *,
                                             double *, double *, double *, double *, SolverState *, PcgPartials,
                                             hipStream_t, const LaunchEvents *);
 template void launch_elastic_fusedA<float>(const Grid &, const ElasticOp<float> &, const float *,
-                                           const uint8_t *, const float *, const float *, float *, float *,
+                                           const uint8_t *, const float *, const float *, const float *,
+                                           float *, float *,
                                            float *, float *, float *, SolverState *, PcgPartials, hipStream_t,
                                            const LaunchEvents *);
 
