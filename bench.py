@@ -634,8 +634,9 @@ def measure(icx, ctx, size, tol, steps, warmup, plain_too, sr=False):
     # library's stream; launched one by one while the window is open).
     # Single-reduction kernel, algorithmic bytes per launch: reads r, w, s, invdiag, p,
     # x_lo (6 words per DOF) and E (1 word per element); writes r, w, s, p, x_lo (5 words
-    # per DOF).  Kernel A of the two-kernel form: reads r, p_old, invdiag, x_lo, E; writes
-    # p, Ap, x_lo (7 words per DOF + 1 per element).
+    # per DOF).  Kernel A of the two-kernel form: reads r, p_old, x_lo (3 words per DOF),
+    # the inverse modulus sum of every node (1 word per node) and E (1 word per element);
+    # writes p, Ap, x_lo (3 words per DOF).
     prof = {}
     for name, kid in (("A", icx.PROF_FUSED_A), ("B", icx.PROF_FUSED_B)):
         if sr and name == "B":
@@ -646,7 +647,8 @@ def measure(icx, ctx, size, tol, steps, warmup, plain_too, sr=False):
         secs, launches = ctx.profile_end()
         prof[name] = (secs, launches, pinfo["iterations"])
     kern_s, launches, executed = prof["A"]
-    bytes_A = ((11 if sr else 7) * N + Ne) * 4
+    nodes = (size[0] + 1) * (size[1] + 1) * (size[2] + 1)
+    bytes_A = ((11 * N + Ne) if sr else (6 * N + nodes + Ne)) * 4
     res = {
         "N": N, "Ne": Ne, "wall": wall, "infos": infos,
         "ms_step": wall / steps * 1e3,
