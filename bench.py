@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--serve-elems", type=float, nargs=2, default=[5e3, 3e5])
     ap.add_argument("--serve-budget", type=float, default=240e9)
     ap.add_argument("--serve-limit", type=int, default=None, help="serve the first N jobs of the stream")
+    ap.add_argument("--serve-max-dofs", type=float, default=None,
+                    help="admit jobs while the resident DOFs stay within this many")
     ap.add_argument("--pmc-iters", type=int, default=200)
     ap.add_argument("--output", action="store_true", help="time the output step (mises, reactions, VTK)")
     ap.add_argument("--heat", action="store_true", help="heat-conduction solve (FP64 plain form)")
@@ -602,7 +604,8 @@ def serving(args):
     ctx = icx.Context(0)
     out = serve.run(ctx, n_jobs=args.serve_jobs, rate=args.serve_rate, iters=args.serve_iters,
                     elems=tuple(args.serve_elems), budget=args.serve_budget, tol=args.tol,
-                    log=lambda m: print(m, file=sys.stderr, flush=True), limit=args.serve_limit)
+                    log=lambda m: print(m, file=sys.stderr, flush=True), limit=args.serve_limit,
+                    max_resident_dofs=int(args.serve_max_dofs) if args.serve_max_dofs else None)
     out.pop("designs", None)
     line = {"metric": "serving: design-iteration latency p50/p99 and DOF-matvec/s",
             "value": out["dof_matvec_per_s"], "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,

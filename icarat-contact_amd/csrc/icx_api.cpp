@@ -8,6 +8,7 @@
 #include "../../include/icx.h"
 #include "../../include/icx_profile.h"
 #include "icx_internal.h"
+#include "../host/toml_job.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -566,6 +567,7 @@ struct icx_job {
   DBuf<double> d_pcgpart;   // PCG partials: PA, PB (x2), PN, PZ
   DBuf<SolverState> d_state;
   DBuf<double> d_scalars;
+  DBuf<OcBisect> d_oc;      // device-side OC bisection state
   bool design_set = false;
   bool u_valid = false;
   PcgWork64 pw;
@@ -680,6 +682,15 @@ icx_status build_rhs(icx_job *j) {
   }
   return ICX_OK;
 }
+
+// Calls that change what a running submitted solve reads (its design and moduli, its
+// load, the solve buffers themselves) are refused until icx_poll has finished it.
+#define ICX_REFUSE_IF_SUBMITTED(j)                                                            \
+  do {                                                                                        \
+    if ((j)->sub.active)                                                                      \
+      return fail((j)->ctx, ICX_ERR_STATE, "%s: the job has a submitted solve running (poll "  \
+                  "until it finishes)", __func__);                                            \
+  } while (0)
 
 inline int poll_every(long long N) { return N > 2000000 ? 16 : (N > 100000 ? 12 : 8); }
 
@@ -1039,6 +1050,32 @@ static bool mesh_volumes(const icx_job *j, std::vector<double> &vol, long long &
   return bad < 0;
 }
 
+icx_status icx_job_get_problem(const icx_job *j, icx_problem *prob) {
+  if (!j || !prob) return ICX_ERR_ARG;
+  *prob = j->prob;
+  return ICX_OK;
+}
+
+icx_status icx_job_create_toml(icx_ctx *ctx, const char *text, icx_job **out, int *numeq, double *design0) {
+  if (!ctx || !text || !out) return ICX_ERR_ARG;
+  *out = nullptr;
+  icarat::TomlJob tj;
+  std::string why;
+  icx_status rc;
+  try {
+    const icarat::toml_lite::Table cfg = icarat::toml_lite::parse_string(text);
+    rc = icarat::box_job_from_toml(ctx, cfg, tj, why);
+  } catch (const std::exception &e) {
+    rc = ICX_ERR_ARG;
+    why = e.what();
+  }
+  if (rc) return fail(ctx, rc, "icx_job_create_toml: %s", why.c_str());
+  *out = tj.job;
+  if (numeq) *numeq = tj.numeq;
+  if (design0) *design0 = tj.design0;
+  return ICX_OK;
+}
+
 icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
   if (!ctx || !m || !out || !m->x || !m->conn) return ICX_ERR_ARG;
   *out = nullptr;
@@ -1047,6 +1084,9 @@ icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *m, icx_job **out) {
   const int ndim = m->eltype == ICX_QUAD4 ? 2 : 3, nq = m->eltype;
   if (m->numnp < 1 || m->nelx < 1) return fail(ctx, ICX_ERR_ARG, "empty mesh");
   if ((long long)m->numnp > (1LL << 27)) return fail(ctx, ICX_ERR_ARG, "mesh too large (more than 2^27 nodes)");
+  // element-corner rows (e * nq + a) and their outputs (row * ndim) are 32-bit indices
+  if ((long long)m->nelx * nq * ndim >= (1LL << 31))
+    return fail(ctx, ICX_ERR_ARG, "mesh too large (nelx * nodes per element * dimension >= 2^31)");
   if (m->state == ICX_HEAT) return fail(ctx, ICX_ERR_ARG, "mesh jobs: elastic states only");
   if ((ndim == 3) != (m->state == ICX_SOLID3D))
     return fail(ctx, ICX_ERR_ARG, "state must be ICX_SOLID3D exactly for hexa8");
@@ -1240,6 +1280,7 @@ icx_status icx_get_mesh(const icx_job *j, double *x, int *conn) {
 
 icx_status icx_add_dirichlet(icx_job *j, const int *flag6, const double *val6, const double *range6) {
   if (!j || !flag6 || !val6 || !range6) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   if (j->numeq >= 0) return fail(j->ctx, ICX_ERR_STATE, "Dirichlet ranges must precede icx_make_dof");
   const double tol = 1.0e-7;
   const double lo[3] = {range6[0] - tol, range6[2] - tol, range6[4] - tol};
@@ -1264,6 +1305,7 @@ icx_status icx_add_dirichlet(icx_job *j, const int *flag6, const double *val6, c
 
 icx_status icx_make_dof(icx_job *j, int *numeq) {
   if (!j) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   const int nc = j->nc;
   const long long nn = j->g.nn;
   j->dof.assign(nn * nc, 0);
@@ -1323,6 +1365,7 @@ icx_status icx_get_dof_map(const icx_job *j, int *dof) {
 // side is chosen by judgeLine and the side's consistent load sum N^T t |J| w is added.
 icx_status icx_add_neumann(icx_job *j, const double *val3, const double *range6) {
   if (!j || !val3 || !range6) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_add_neumann needs icx_make_dof first");
   const double tol = 1.0e-7;
   const double lo[3] = {range6[0] - tol, range6[2] - tol, range6[4] - tol};
@@ -1393,6 +1436,7 @@ icx_status icx_add_neumann(icx_job *j, const double *val3, const double *range6)
 
 icx_status icx_set_fext(icx_job *j, const double *fext) {
   if (!j || !fext) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_set_fext needs icx_make_dof first");
   std::memcpy(j->fext.data(), fext, j->numeq * sizeof(double));
   return upload_fext(j);
@@ -1407,6 +1451,7 @@ icx_status icx_get_fext(const icx_job *j, double *fext) {
 
 icx_status icx_add_nodal_load(icx_job *j, int node, int comp, double value) {
   if (!j) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_add_nodal_load needs icx_make_dof first");
   if (node < 0 || node >= j->g.nn || comp < 0 || comp >= j->nc) return fail(j->ctx, ICX_ERR_ARG, "node/comp out of range");
   const int id = j->dof[(long long)node * j->nc + comp];
@@ -1416,8 +1461,14 @@ icx_status icx_add_nodal_load(icx_job *j, int node, int comp, double value) {
 
 icx_status icx_set_design(icx_job *j, const double *design) {
   if (!j || !design) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   icx_ctx *c = j->ctx;
   if (j->heat() && !j->heat_ready) return fail(c, ICX_ERR_STATE, "icx_set_conductivity first");
+  // densities live in [0, 1]; NaN, infinities and values outside are refused before any
+  // modulus is formed from them
+  for (long long e = 0; e < j->g.ne; ++e)
+    if (!(design[e] >= 0.0 && design[e] <= 1.0))
+      return fail(c, ICX_ERR_NUMERIC, "icx_set_design: design[%lld] = %g is not in [0, 1]", e, design[e]);
   HIPCHK(c, hipMemcpyAsync(j->d_design.p, design, j->g.ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
   if (j->heat()) launch_heat_weight(j->g.ne, j->prob.penalty, j->d_design.p, j->d_E.p, S(j));
   else launch_simp(j->g.ne, j->prob.young1, j->prob.young2, j->prob.penalty, j->d_design.p, j->d_E.p, nullptr, S(j));
@@ -1456,6 +1507,7 @@ static void heat_element(const BoxElement &el, const double *D, double *K) {
 
 icx_status icx_set_conductivity(icx_job *j, const double *k1, const double *k2) {
   if (!j || !k1 || !k2) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   if (!j->heat()) return fail(j->ctx, ICX_ERR_STATE, "icx_set_conductivity: not a heat job (ICX_HEAT)");
   if (k1[0] < k2[0])
     return fail(j->ctx, ICX_ERR_ARG, "the conductivity of material 1 must be at least that of material 2");
@@ -1903,6 +1955,7 @@ static void batch_remove(icx_ctx *c, icx_job *j) {
 
 icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx_solve_info *info) {
   if (!j) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   if (j->numeq < 0 || !j->design_set)
     return fail(j->ctx, ICX_ERR_STATE, "icx_solve needs icx_make_dof and icx_set_design");
   if (!(tol > 0.0) || maxit < 1) return fail(j->ctx, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
@@ -2036,21 +2089,27 @@ static icx_status stress_constraint_dev(icx_job *j, double slim, double pn, doub
   launch_stress_adjoint_load(j->g, mm, sp, std::pow(psum, (1.0 / pn) - 1.0), j->d_design.p, j->d_E.p, sig.p, vm.p,
                              sqp.p, f12.p, fe.p, j->d_mask.p, j->d_b.p, S(j));
   icx_solve_info info{};
-  rc = solve_fused(j, tol, maxit, info);
-  if (rc && rc != ICX_ERR_NOCONV) return rc;
-  if (adjoint_iterations) *adjoint_iterations = info.iterations;
-  rc = output_displacement(j);  // lambda with the Dirichlet values
-  if (rc) return rc;
-  launch_stress_sens(j->g, j->op64, sp, j->d_design.p, U.p, j->ow.ufull.p, f12.p, dgds, S(j));
-  HIPCHK(c, hipMemcpyAsync(j->d_u.p, usave.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
-  HIPCHK(c, hipStreamSynchronize(S(j)));
+  rc = solve_fused(j, tol, maxit, info);  // lambda in d_u (an adjoint at maxit is kept)
+  if (rc == ICX_ERR_NOCONV) rc = ICX_OK;
+  if (!rc && adjoint_iterations) *adjoint_iterations = info.iterations;
+  if (!rc) rc = output_displacement(j);  // lambda with the Dirichlet values
+  if (!rc) launch_stress_sens(j->g, j->op64, sp, j->d_design.p, U.p, j->ow.ufull.p, f12.p, dgds, S(j));
+  // the job's displacement comes back on every path: lambda never stands in for u
+  const hipError_t e1 = hipMemcpyAsync(j->d_u.p, usave.p, N * sizeof(double), hipMemcpyDeviceToDevice, S(j));
+  const hipError_t e2 = hipStreamSynchronize(S(j));
+  if (e1 != hipSuccess || e2 != hipSuccess) {
+    j->u_valid = false;
+    return fail(c, ICX_ERR_HIP, "restoring the displacement after the adjoint solve failed: %s",
+                hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+  }
   j->u_valid = true;
-  return ICX_OK;
+  return rc;
 }
 
 icx_status icx_stress_constraint(icx_job *j, double stress_limit, double p_norm, double qp, double tol, int maxit,
                                  double *g, double *dgds, int *adjoint_iterations) {
   if (!j || !g) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   icx_ctx *c = j->ctx;
   DBuf<double> dg;
   if (dgds) HIPCHK(c, dg.alloc(j->g.ne));
@@ -2526,33 +2585,56 @@ icx_status icx_project(icx_job *j, double beta, double T, const double *in, doub
 
 namespace {
 
-// OC bisection with the volume evaluated on the device (OCSolver.hpp:20-68)
+// OC bisection with the volume evaluated on the device (OCSolver.hpp:20-68).  Cone filter
+// and no filter: the bisection itself runs on the device (OcBisect: each step is the
+// fused volume kernel, its reduction and the bracket update), launched in rounds of
+// steps with one read of the state per round; a step queued past the stop exits at once.
+// Helmholtz filter: a PDE solve per step, the bracket on the host.
+constexpr int OC_ROUND = 16;
 icx_status oc_update_dev(icx_job *j, double *dx, const double *dfdx, double g0, const double *dgdx,
                          double xmin, double xmax, double move, int filter, double beta, double T,
                          double *dxnew, double *dtmp, int *bisections) {
   const long long ne = j->g.ne;
+  icx_ctx *c = j->ctx;
+  const int project = beta > 0.0;
+  if (filter != ICX_FILTER_HELMHOLTZ) {
+    HIPCHK(c, j->d_oc.alloc(1));
+    OcBisect *bs = j->d_oc.p;
+    launch_oc_init(bs, g0, S(j));
+    OcBisect *h = reinterpret_cast<OcBisect *>(c->h_scalar);  // pinned, 64 doubles
+    for (;;) {
+      for (int r = 0; r < OC_ROUND; ++r) {
+        if (filter == ICX_FILTER_CONE && j->mesh)
+          launch_oc_volume_csr(ne, cone_csr(j), j->cw.aa.p, 0.0, move, xmin, xmax, dx, dfdx, dgdx, project, beta, T,
+                               j->d_vol.p, j->d_scalars.p, j->red(), S(j), bs);
+        else if (filter == ICX_FILTER_CONE)
+          launch_oc_volume_cone(j->g, stencil(j), j->cw.aa.p, 0.0, move, xmin, xmax, dx, dfdx, dgdx, project, beta,
+                                T, j->volume, j->d_scalars.p, j->red(), S(j), bs);
+        else {
+          launch_oc_candidate(ne, 0.0, move, xmin, xmax, dx, dfdx, dgdx, dxnew, S(j), bs, true);
+          launch_volume(ne, project, beta, T, dxnew, j->volume, j->mesh ? j->d_vol.p : nullptr, j->d_scalars.p,
+                        j->red(), S(j));
+        }
+        launch_oc_decide(bs, j->d_scalars.p, S(j));
+      }
+      HIPCHK(c, hipMemcpyAsync(h, bs, sizeof(OcBisect), hipMemcpyDeviceToHost, S(j)));
+      HIPCHK(c, hipStreamSynchronize(S(j)));
+      if (h->done) break;
+    }
+    launch_oc_candidate(ne, 0.0, move, xmin, xmax, dx, dfdx, dgdx, dxnew, S(j), bs, false);
+    HIPCHK(c, hipMemcpyAsync(dx, dxnew, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
+    if (bisections) *bisections = h->counter;
+    return ICX_OK;
+  }
   double l0 = 0.0, l1 = 1.0e4, lambda = 0.0;
   int counter = 0;
-  const int project = beta > 0.0;
   while ((l1 - l0) / (l1 + l0) > 1.0e-5 && 100 > counter) {
     lambda = 0.5 * (l1 + l0);
-    if (filter == ICX_FILTER_CONE && j->mesh) {
-      launch_oc_volume_csr(ne, cone_csr(j), j->cw.aa.p, lambda, move, xmin, xmax, dx, dfdx, dgdx, project, beta, T,
-                           j->d_vol.p, j->d_scalars.p, j->red(), S(j));
-    } else if (filter == ICX_FILTER_CONE) {
-      launch_oc_volume_cone(j->g, stencil(j), j->cw.aa.p, lambda, move, xmin, xmax, dx, dfdx, dgdx, project,
-                            beta, T, j->volume, j->d_scalars.p, j->red(), S(j));
-    } else {
-      launch_oc_candidate(ne, lambda, move, xmin, xmax, dx, dfdx, dgdx, dxnew, S(j));
-      const double *src = dxnew;
-      if (filter == ICX_FILTER_HELMHOLTZ) {
-        icx_status rc = helm_apply_dev(j, 0, j->opt.ready ? j->opt.par.helm_tol : 1e-10, dxnew, dtmp, nullptr);
-        if (rc) return rc;
-        src = dtmp;
-      }
-      launch_volume(ne, project, beta, T, src, j->volume, j->mesh ? j->d_vol.p : nullptr, j->d_scalars.p, j->red(),
-                    S(j));
-    }
+    launch_oc_candidate(ne, lambda, move, xmin, xmax, dx, dfdx, dgdx, dxnew, S(j));
+    icx_status hrc = helm_apply_dev(j, 0, j->opt.ready ? j->opt.par.helm_tol : 1e-10, dxnew, dtmp, nullptr);
+    if (hrc) return hrc;
+    launch_volume(ne, project, beta, T, dtmp, j->volume, j->mesh ? j->d_vol.p : nullptr, j->d_scalars.p, j->red(),
+                  S(j));
     double gk;
     icx_status rc = read_scalars(j, 1, &gk);
     if (rc) return rc;
@@ -2561,7 +2643,6 @@ icx_status oc_update_dev(icx_job *j, double *dx, const double *dfdx, double g0, 
     counter++;
   }
   launch_oc_candidate(ne, lambda, move, xmin, xmax, dx, dfdx, dgdx, dxnew, S(j));
-  icx_ctx *c = j->ctx;
   HIPCHK(c, hipMemcpyAsync(dx, dxnew, ne * sizeof(double), hipMemcpyDeviceToDevice, S(j)));
   if (bisections) *bisections = counter;
   return ICX_OK;
@@ -2821,6 +2902,7 @@ icx_status icx_gcmma_concheck(const icx_job *jc, double f0xnew, double fxnew, in
 // ---- device-resident design loop ------------------------------------------------------------
 icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0) {
   if (!j || !par || !x0) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
   if (par->driver < 0 || par->driver > 2) return fail(j->ctx, ICX_ERR_ARG, "driver must be 0 (OC), 1 (MMA) or 2 (GCMMA)");
@@ -2948,6 +3030,7 @@ extern "C" {
 
 icx_status icx_opt_step_begin(icx_job *j) {
   if (!j) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   OptWork &o = j->opt;
   if (!o.ready) return fail(j->ctx, ICX_ERR_STATE, "icx_opt_init first");
   if (o.in_step) return fail(j->ctx, ICX_ERR_STATE, "design step already begun");
@@ -2965,6 +3048,7 @@ icx_status icx_opt_step_begin(icx_job *j) {
 
 icx_status icx_opt_step_end(icx_job *j, const icx_solve_info *info, icx_iter_record *rec) {
   if (!j) return ICX_ERR_ARG;
+  ICX_REFUSE_IF_SUBMITTED(j);
   OptWork &o = j->opt;
   if (!o.in_step) return fail(j->ctx, ICX_ERR_STATE, "icx_opt_step_begin first");
   o.in_step = false;
@@ -3081,6 +3165,17 @@ icx_status icx_submit(icx_job *j, double tol, int maxit) {
   b.active = true;
   c->batch->active.push_back(j);
   c->batch->dirty = true;
+  return ICX_OK;
+}
+
+icx_status icx_cancel(icx_job *j) {
+  if (!j) return ICX_ERR_ARG;
+  icx_ctx *c = j->ctx;
+  if (!j->sub.active) return fail(c, ICX_ERR_STATE, "icx_cancel: the job has no submitted solve");
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // queued chunks that touch the job end first
+  batch_remove(c, j);
+  j->sub.has_info = false;
+  j->u_valid = false;
   return ICX_OK;
 }
 
@@ -3222,13 +3317,16 @@ icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
     add(*d);
   add(j->mw.Kx); add(j->mw.part);
   for (auto *d : {&j->hw.invd, &j->hw.b, &j->hw.x, &j->hw.r, &j->hw.p, &j->hw.t, &j->hw.z}) add(*d);
-  add(j->cw.off); add(j->cw.w); add(j->cw.aa);
-  for (auto *d : {&j->mma.low, &j->mma.upp, &j->mma.alpha, &j->mma.beta, &j->mma.p0, &j->mma.q0, &j->mma.p1,
-                  &j->mma.q1, &j->mma.xold1, &j->mma.xold2})
-    add(*d);
+  add(j->cw.off); add(j->cw.w); add(j->cw.aa); add(j->cw.ptr); add(j->cw.id); add(j->cw.wt);
+  for (const MMAWork *m : {&j->mma, &j->gc.v})
+    for (auto *d : {&m->low, &m->upp, &m->alpha, &m->beta, &m->p0, &m->q0, &m->p1, &m->q1, &m->xold1, &m->xold2})
+      add(*d);
   for (auto *d : {&j->opt.x, &j->opt.fil, &j->opt.thr, &j->opt.dthr, &j->opt.dfds, &j->opt.dgds, &j->opt.tmp,
-                  &j->opt.tmp2})
+                  &j->opt.tmp2, &j->opt.xnew})
     add(*d);
+  // mesh jobs and the output step
+  add(j->d_conn); add(j->d_adj_ptr); add(j->d_slot); add(j->d_xyz); add(j->d_ye); add(j->d_vol);
+  add(j->ow.ufull); add(j->ow.nodes); add(j->ow.mises); add(j->d_oc);
   *bytes = b;
   return ICX_OK;
 }

@@ -28,6 +28,13 @@ template <typename T> struct ElasticOp {
   T ikd[3];            // 1 / kd, rounded once to T (the per-node Jacobi factor)
 };
 
+// hexa8 scalar operators (heat and Helmholtz): the shared-element tile kernel, unless the
+// context's tile depth (ICX_OPT_TILE_DEPTH) names a column depth of the per-node kernels
+// (2, 4, 8 or 16)
+inline bool scalar_tile_kernel(const Grid &g) {
+  return g.ndim == 3 && g.tile_depth != 2 && g.tile_depth != 4 && g.tile_depth != 8 && g.tile_depth != 16;
+}
+
 // Jacobi factor of a box job's DOF from its node's inverse modulus sum s (s < 0 marks a
 // zero diagonal, where the inverse is 1 as in Eigen's DiagonalPreconditioner) and the
 // component's 1 / kd: one multiply, so every kernel that forms it gets the same bits.
@@ -456,10 +463,26 @@ void launch_helm_rhs(const Grid &g, double scale, const double *rho, double *f, 
 void launch_helm_to_element(const Grid &g, double scale, const double *phi, double *out,
                             hipStream_t s);
 
+// OC bisection state on the device (OCSolver.hpp:20-68): the bracket [l0, l1], the
+// lambda of the last evaluated step, the volume target g0, the step count and the stop
+// flag.  The OC kernels below take it (nullable): with it, a kernel evaluates
+// lambda = (l1 + l0) / 2 and leaves at once when done is set, so the host launches the
+// steps in rounds and reads the state once per round instead of once per step.
+struct OcBisect {
+  double l0, l1, lambda, g0;
+  int counter, done;
+};
+void launch_oc_init(OcBisect *b, double g0, hipStream_t s);
+// one step's decision on the volume gk[0]: the reference's bracket update, counter and
+// loop condition ((l1 - l0) / (l1 + l0) > 1e-5 and counter < 100)
+void launch_oc_decide(OcBisect *b, const double *gk, hipStream_t s);
+
 // OC: x_new(lambda) and, for the cone filter, the fused filter+projection+volume
+// (lambda taken from bs when bs is given: the last evaluated step's, bs->lambda, for
+// the candidate; the step's midpoint for the volume kernels)
 void launch_oc_candidate(long long n, double lambda, double move, double xmin, double xmax,
                          const double *x, const double *dfdx, const double *dgdx, double *xnew,
-                         hipStream_t s);
+                         hipStream_t s, const OcBisect *bs = nullptr, bool mid = false);
 // sum_i proj(rho_i) * vol -> out[0]
 // sum_e (projected) rho_e v_e with v_e = volv[e] when volv is given, else vol
 void launch_volume(long long n, int project, double beta, double T, const double *rho, double vol,
@@ -473,11 +496,12 @@ void launch_cone_csr(long long n, ConeCsr f, const double *aa, const double *in,
                      hipStream_t s);
 void launch_oc_volume_csr(long long n, ConeCsr f, const double *aa, double lambda, double move, double xmin,
                           double xmax, const double *x, const double *dfdx, const double *dgdx, int project,
-                          double beta, double T, const double *vol, double *out, Reduce red, hipStream_t s);
+                          double beta, double T, const double *vol, double *out, Reduce red, hipStream_t s,
+                          const OcBisect *bs = nullptr);
 void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double lambda, double move,
                            double xmin, double xmax, const double *x, const double *dfdx,
                            const double *dgdx, int project, double beta, double T, double vol,
-                           double *out, Reduce red, hipStream_t s);
+                           double *out, Reduce red, hipStream_t s, const OcBisect *bs = nullptr);
 
 // MMA (m = 1)
 struct MMAVecs {

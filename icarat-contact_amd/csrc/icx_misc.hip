@@ -792,7 +792,7 @@ void launch_helm_tile(const Grid &g, const double *kf, const double *x, double *
 
 void launch_heat_matvec(const Grid &g, const HeatOp &op, const double *w, const uint8_t *mask, const double *x,
                         double *y, const SolverState *st, hipStream_t s, const LaunchEvents *ev) {
-  if (op.haar && g.ndim == 3 && g.tile_depth != 2 && g.tile_depth != 4 && g.tile_depth != 8 && g.tile_depth != 16) {
+  if (op.haar && scalar_tile_kernel(g)) {
     const int nb = ((g.sx + HT - 1) / HT) * ((g.sy + HT - 1) / HT) * ((g.sz + HZ - 1) / HZ);
     launch_ev(k_heat_haar_tile, nb, 256, s, ev, g, op, w, mask, x, y, st);
     ICX_CHECK_LAUNCH();
@@ -979,17 +979,54 @@ __device__ __forceinline__ double oc_x(double lambda, double move, double xmin, 
   else if (v > hi) v = hi;
   return v;
 }
-__global__ void k_oc_candidate(long long n, double lambda, double move, double xmin, double xmax,
-                               const double *x, const double *dfdx, const double *dgdx, double *xnew) {
+__global__ void k_oc_init(OcBisect *b, double g0) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    b->l0 = 0.0;
+    b->l1 = 1.0e4;
+    b->lambda = 0.0;
+    b->g0 = g0;
+    b->counter = 0;
+    b->done = 0;
+  }
+}
+void launch_oc_init(OcBisect *b, double g0, hipStream_t s) {
+  hipLaunchKernelGGL(k_oc_init, dim3(1), dim3(64), 0, s, b, g0);
+  ICX_CHECK_LAUNCH();
+}
+__global__ void k_oc_decide(OcBisect *b, const double *gk) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || b->done) return;
+  const double lambda = 0.5 * (b->l1 + b->l0);
+  if (gk[0] - b->g0 > 0.0) b->l0 = lambda;
+  else b->l1 = lambda;
+  b->lambda = lambda;
+  b->counter += 1;
+  if (!((b->l1 - b->l0) / (b->l1 + b->l0) > 1.0e-5 && 100 > b->counter)) b->done = 1;
+}
+void launch_oc_decide(OcBisect *b, const double *gk, hipStream_t s) {
+  hipLaunchKernelGGL(k_oc_decide, dim3(1), dim3(64), 0, s, b, gk);
+  ICX_CHECK_LAUNCH();
+}
+// lambda of an OC kernel: the argument, or from the device bisection state (the step's
+// midpoint, or the last evaluated step's lambda for the final candidate)
+__device__ __forceinline__ double oc_lambda(double lambda, const OcBisect *bs, bool mid) {
+  if (!bs) return lambda;
+  return mid ? 0.5 * (bs->l1 + bs->l0) : bs->lambda;
+}
+
+__global__ void k_oc_candidate(long long n, double lambda_arg, double move, double xmin, double xmax,
+                               const double *x, const double *dfdx, const double *dgdx, double *xnew,
+                               const OcBisect *bs, bool mid) {
+  if (bs && mid && bs->done) return;  // a bisection step queued past the stop
+  const double lambda = oc_lambda(lambda_arg, bs, mid);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     xnew[i] = oc_x(lambda, move, xmin, xmax, x[i], dfdx[i], dgdx[i]);
 }
 void launch_oc_candidate(long long n, double lambda, double move, double xmin, double xmax,
                          const double *x, const double *dfdx, const double *dgdx, double *xnew,
-                         hipStream_t s) {
+                         hipStream_t s, const OcBisect *bs, bool mid) {
   hipLaunchKernelGGL(k_oc_candidate, dim3(ew_blocks(n)), dim3(256), 0, s, n, lambda, move, xmin, xmax, x,
-                     dfdx, dgdx, xnew);
+                     dfdx, dgdx, xnew, bs, mid);
   ICX_CHECK_LAUNCH();
 }
 
@@ -1032,11 +1069,14 @@ void launch_cone_csr(long long n, ConeCsr f, const double *aa, const double *in,
 
 // one bisection step of the OC volume lambda with the cone filter, fused:
 // sum_i threshold( sum_j w_ij x_new_j(lambda) / aa_i ) v
-__global__ void __launch_bounds__(256) k_oc_volume_cone(Grid g, Stencil st, const double *aa, double lambda,
+__global__ void __launch_bounds__(256) k_oc_volume_cone(Grid g, Stencil st, const double *aa, double lambda_arg,
                                                         double move, double xmin, double xmax, const double *x,
                                                         const double *dfdx, const double *dgdx, int project,
-                                                        double beta, double T, double vol, Reduce red) {
+                                                        double beta, double T, double vol, Reduce red,
+                                                        const OcBisect *bs) {
   __shared__ double scr[4];
+  if (bs && bs->done) return;  // a bisection step queued past the stop (uniform exit)
+  const double lambda = oc_lambda(lambda_arg, bs, true);
   double s = 0.0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
        e += (long long)gridDim.x * blockDim.x) {
@@ -1053,11 +1093,14 @@ __global__ void __launch_bounds__(256) k_oc_volume_cone(Grid g, Stencil st, cons
   if (threadIdx.x == 0) red.partial[blockIdx.x] = s;
 }
 // the fused OC bisection step on explicit neighbour lists, per-element volumes
-__global__ void __launch_bounds__(256) k_oc_volume_csr(long long n, ConeCsr f, const double *aa, double lambda,
-                                                       double move, double xmin, double xmax, const double *x,
-                                                       const double *dfdx, const double *dgdx, int project,
-                                                       double beta, double T, const double *vol, Reduce red) {
+__global__ void __launch_bounds__(256) k_oc_volume_csr(long long n, ConeCsr f, const double *aa,
+                                                       double lambda_arg, double move, double xmin, double xmax,
+                                                       const double *x, const double *dfdx, const double *dgdx,
+                                                       int project, double beta, double T, const double *vol,
+                                                       Reduce red, const OcBisect *bs) {
   __shared__ double scr[4];
+  if (bs && bs->done) return;  // a bisection step queued past the stop (uniform exit)
+  const double lambda = oc_lambda(lambda_arg, bs, true);
   double s = 0.0;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
@@ -1074,20 +1117,21 @@ __global__ void __launch_bounds__(256) k_oc_volume_csr(long long n, ConeCsr f, c
 }
 void launch_oc_volume_csr(long long n, ConeCsr f, const double *aa, double lambda, double move, double xmin,
                           double xmax, const double *x, const double *dfdx, const double *dgdx, int project,
-                          double beta, double T, const double *vol, double *out, Reduce red, hipStream_t s) {
+                          double beta, double T, const double *vol, double *out, Reduce red, hipStream_t s,
+                          const OcBisect *bs) {
   const int nb = ew_blocks(n, 1, RED_BLOCKS);
   hipLaunchKernelGGL(k_oc_volume_csr, dim3(nb), dim3(256), 0, s, n, f, aa, lambda, move, xmin, xmax, x, dfdx,
-                     dgdx, project, beta, T, vol, red);
+                     dgdx, project, beta, T, vol, red, bs);
   launch_finalize(red, nb, 1, out, s);
 }
 
 void launch_oc_volume_cone(const Grid &g, Stencil st, const double *aa, double lambda, double move,
                            double xmin, double xmax, const double *x, const double *dfdx,
                            const double *dgdx, int project, double beta, double T, double vol,
-                           double *out, Reduce red, hipStream_t s) {
+                           double *out, Reduce red, hipStream_t s, const OcBisect *bs) {
   const int nb = ew_blocks(g.ne, 1, RED_BLOCKS);
   hipLaunchKernelGGL(k_oc_volume_cone, dim3(nb), dim3(256), 0, s, g, st, aa, lambda, move, xmin, xmax, x,
-                     dfdx, dgdx, project, beta, T, vol, red);
+                     dfdx, dgdx, project, beta, T, vol, red, bs);
   launch_finalize(red, nb, 1, out, s);
 }
 

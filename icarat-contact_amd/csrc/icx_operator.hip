@@ -678,8 +678,8 @@ void launch_helm_matvec(const Grid &g, const HelmOp<double> &op, const double *x
                         const SolverState *st, hipStream_t s) {
   const int nb = ew_blocks(g.nn);
   // hexa8: element transforms shared through LDS (the heat operator's tile kernel); the
-  // per-node gather stays for a tile depth set through ICX_OPT_TILE_DEPTH
-  if (g.ndim == 3 && g.tile_depth == 0) launch_helm_tile(g, op.kf, x, y, st, s);
+  // per-node gather under the heat operator's rule for its per-node kernels
+  if (scalar_tile_kernel(g)) launch_helm_tile(g, op.kf, x, y, st, s);
   else if (g.ndim == 3) hipLaunchKernelGGL(k_helm_hex8, dim3(nb), dim3(256), 0, s, g, op, x, y, st);
   else hipLaunchKernelGGL(k_helm_quad4, dim3(nb), dim3(256), 0, s, g, op, x, y, st);
   ICX_CHECK_LAUNCH();

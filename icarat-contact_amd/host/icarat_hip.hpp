@@ -32,6 +32,7 @@
 
 #include "../../include/icx.h"
 #include "toml_lite.hpp"
+#include "toml_job.hpp"
 
 namespace icarat {
 
@@ -90,43 +91,19 @@ class Optimize {
 // structured box model on the device
 class StructuredModel {
  public:
+  // The box job of a reference job file: mesh, material, Dirichlet ranges, DOF map and
+  // tractions in the reference's order, by the same front end as icx_job_create_toml.
   StructuredModel(const toml_lite::Table &config, FEM &fem) : fem_(fem) {
-    const std::string type = config.at("mesh", "type").as_string();
-    if (type != "quad4" && type != "hexa8")
-      fatal("element type " + type + " is not supported by the GPU path (quad4, hexa8)");
-    icx_problem p{};
-    p.eltype = type == "quad4" ? ICX_QUAD4 : ICX_HEXA8;
-    const auto &div = config.at("mesh", "division").as_array();
-    const auto &len = config.at("mesh", "length").as_array();
-    for (int d = 0; d < 3; ++d) {
-      p.div[d] = div[d].as_int();
-      p.length[d] = len[d].as_double();
-    }
-    const int ndim = config.at("mesh", "dimension").as_int();
-    // [material] type: "Elastic" (Elastic::setParameter, elasticity.cpp:19-32) or "Heat"
-    // (Heat::setParameter, heat.cpp:19-60: two conductivity rows, no young / poisson)
-    heat_ = config.has("material", "type") && config.at("material", "type").as_string() == "Heat";
-    double cond[2][6] = {};
-    if (heat_) {
-      p.state = ICX_HEAT;
-      const auto &rows = config.at("material", "conductivity").as_array();
-      for (int m = 0; m < 2; ++m)
-        for (int k = 0; k < 6; ++k) cond[m][k] = rows[m].as_array()[k].as_double();
-    } else {
-      std::string state = config.at("material", "state").as_string();
-      if (ndim == 3) state = "3D";
-      p.state = state == "plane_stress" ? ICX_PLANE_STRESS : (state == "plane_strain" ? ICX_PLANE_STRAIN : ICX_SOLID3D);
-      const auto &young = config.at("material", "young").as_array();
-      p.young1 = young[0].as_double();
-      p.young2 = young[1].as_double();
-      p.poisson = config.at("material", "poisson").as_double();
-    }
-    p.penalty = config.at("material", "penalty").as_double();
-    design_s0_ = config.at("material", "designs_0").as_double();
-    prob_ = p;
-    check(ctx(), icx_job_create(ctx(), &p, &job_), "icx_job_create");
-    if (heat_) check(ctx(), icx_set_conductivity(job_, cond[0], cond[1]), "icx_set_conductivity");
-    finish(ndim);
+    TomlJob tj;
+    std::string why;
+    if (box_job_from_toml(ctx(), config, tj, why) != ICX_OK) fatal(why);
+    job_ = tj.job;
+    heat_ = tj.heat;
+    prob_ = tj.prob;
+    design_s0_ = tj.design0;
+    numeq_ = tj.numeq;
+    from_spec_ = true;
+    finish(tj.ndim);
   }
 
   // MeshMDPA::setParameter / generate (src/io/mesh_mdpa.hpp:84-216): the mesh of an .mdpa
@@ -197,8 +174,9 @@ class StructuredModel {
   StructuredModel(const StructuredModel &) = delete;
   StructuredModel &operator=(const StructuredModel &) = delete;
 
-  // BCIcarat::input_dirich
+  // BCIcarat::input_dirich (a box job built from its job file has its ranges already)
   void input_dirich(const toml_lite::Table &config) {
+    if (from_spec_) return;
     for (const auto &entry : config.at("boundary_condition", "dirichlet").as_array()) {
       const auto &e = entry.as_array();
       int flag[6];
@@ -214,13 +192,14 @@ class StructuredModel {
   }
   // BCIcarat::makeDOF
   int makeDOF() {
+    if (from_spec_) return numeq_;
     int numeq = 0;
     check(ctx(), icx_make_dof(job_, &numeq), "icx_make_dof");
     return numeq;
   }
   // BCIcarat::input_neumann (accumulates into force.fext_org)
   void input_neumann(const toml_lite::Table &config, Force &force) {
-    if (config.has("boundary_condition", "neumann"))
+    if (!from_spec_ && config.has("boundary_condition", "neumann"))
       for (const auto &entry : config.at("boundary_condition", "neumann").as_array()) {
         const auto &e = entry.as_array();
         double val[3], range[6];
@@ -268,6 +247,8 @@ class StructuredModel {
   icx_problem prob_{};
   double design_s0_ = 0.0;
   bool heat_ = false;
+  bool from_spec_ = false;  // box job built by box_job_from_toml (ranges, DOF map, loads done)
+  int numeq_ = 0;
   std::vector<double> volume_, design_;
 };
 

@@ -27,7 +27,11 @@ struct Value {
     if (kind != NUM) throw std::runtime_error("toml: number expected");
     return num;
   }
-  int as_int() const { return (int)as_double(); }
+  int as_int() const {
+    const double v = as_double();
+    if (!(v >= -2147483648.0 && v <= 2147483647.0)) throw std::runtime_error("toml: integer out of range");
+    return (int)v;
+  }
   const std::string &as_string() const {
     if (kind != STR) throw std::runtime_error("toml: string expected");
     return str;
@@ -73,25 +77,31 @@ struct Parser {
       break;
     }
   }
-  Value value() {
+  // the next character after white space and comments; throws at the end of the text
+  char peek() {
     skip_ws();
     if (i >= s.size()) throw std::runtime_error("toml: unexpected end");
+    return s[i];
+  }
+  Value value(int depth = 0) {
+    if (depth > 32) throw std::runtime_error("toml: arrays nested too deeply");
+    const char c0 = peek();
     Value v;
-    if (s[i] == '[') {
+    if (c0 == '[') {
       v.kind = Value::ARR;
       ++i;
       for (;;) {
-        skip_ws();
-        if (s[i] == ']') { ++i; break; }
-        v.arr.push_back(value());
-        skip_ws();
-        if (s[i] == ',') { ++i; continue; }
-        if (s[i] == ']') { ++i; break; }
+        if (peek() == ']') { ++i; break; }
+        v.arr.push_back(value(depth + 1));
+        const char c = peek();
+        if (c == ',') { ++i; continue; }
+        if (c == ']') { ++i; break; }
         throw std::runtime_error("toml: ',' or ']' expected");
       }
-    } else if (s[i] == '"') {
+    } else if (c0 == '"') {
       v.kind = Value::STR;
-      size_t j = s.find('"', i + 1);
+      const size_t j = s.find('"', i + 1);
+      if (j == std::string::npos) throw std::runtime_error("toml: unterminated string");
       v.str = s.substr(i + 1, j - i - 1);
       i = j + 1;
     } else if (s.compare(i, 4, "true") == 0) {
@@ -118,13 +128,17 @@ inline Table parse_string(const std::string &src) {
     p.skip_ws();
     if (p.i >= src.size()) break;
     if (src[p.i] == '[') {
-      size_t j = src.find(']', p.i);
+      const size_t j = src.find(']', p.i);
+      if (j == std::string::npos) throw std::runtime_error("toml: unterminated section header");
       section = src.substr(p.i + 1, j - p.i - 1);
+      if (section.find('\n') != std::string::npos) throw std::runtime_error("toml: bad section header");
       p.i = j + 1;
       continue;
     }
-    size_t eq = src.find('=', p.i);
+    const size_t eq = src.find('=', p.i);
+    if (eq == std::string::npos) throw std::runtime_error("toml: key = value expected");
     std::string key = src.substr(p.i, eq - p.i);
+    if (key.find('\n') != std::string::npos) throw std::runtime_error("toml: key = value expected near " + key);
     while (!key.empty() && std::isspace((unsigned char)key.back())) key.pop_back();
     p.i = eq + 1;
     t.data_[section][key] = p.value();

@@ -21,7 +21,7 @@ ERR_NOCONV = 5
 
 EXPORTS = [
     "icx_ctx_create", "icx_ctx_destroy", "icx_last_error", "icx_ctx_synchronize",
-    "icx_job_create", "icx_job_create_mesh", "icx_mdpa_read", "icx_job_destroy", "icx_job_sizes", "icx_get_mesh",
+    "icx_job_create", "icx_job_create_toml", "icx_job_create_mesh", "icx_mdpa_read", "icx_job_destroy", "icx_job_sizes", "icx_job_get_problem", "icx_get_mesh",
     "icx_add_dirichlet", "icx_make_dof", "icx_get_dof_map", "icx_add_neumann",
     "icx_set_fext", "icx_get_fext", "icx_add_nodal_load",
     "icx_set_design", "icx_set_conductivity", "icx_matvec", "icx_matvec_noalloc", "icx_diagonal", "icx_solve", "icx_get_displacement",
@@ -32,7 +32,7 @@ EXPORTS = [
     "icx_gcmma_reset", "icx_gcmma_outer_update", "icx_gcmma_inner_update", "icx_gcmma_concheck",
     "icx_mma_set_asymptotes", "icx_stress_constraint",
     "icx_opt_init", "icx_opt_step", "icx_opt_step_begin", "icx_opt_step_end", "icx_opt_get_design",
-    "icx_submit", "icx_poll", "icx_job_last_info", "icx_solve_batch", "icx_job_footprint",
+    "icx_submit", "icx_poll", "icx_job_last_info", "icx_cancel", "icx_solve_batch", "icx_job_footprint",
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
 ]
 OPT_GRAPHS, OPT_TILE_DEPTH, OPT_PIPELINE, OPT_SINGLE_REDUCTION, OPT_HEAT_DENSE = 1, 2, 3, 4, 5
@@ -131,6 +131,7 @@ class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
         self.jobs = weakref.WeakSet()
+        self.options = {}  # values set through set_option
         rc = lib().icx_ctx_create(device, C.byref(self.h))
         if rc != 0:
             raise IcxError(rc, "icx_ctx_create failed (no GPU?)")
@@ -145,6 +146,7 @@ class Context:
 
     def set_option(self, option, value):
         self.check(lib().icx_set_option(self.h, int(option), C.c_longlong(int(value))))
+        self.options[int(option)] = int(value)
 
     def profile_begin(self, kernel):
         self.check(lib().icx_profile_begin(self.h, int(kernel)))
@@ -234,6 +236,28 @@ class Job:
         self.dofnp = self.ndim
         self.div = None
         self.numeq = None
+        return self
+
+    @classmethod
+    def from_toml(cls, ctx, text):
+        """A box job from the text of a reference job file (icx_job_create_toml): mesh,
+        material, Dirichlet ranges, DOF map and tractions; `design0` holds designs_0."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        n, d0 = C.c_int(), C.c_double()
+        ctx.check(lib().icx_job_create_toml(ctx.h, text.encode(), C.byref(self.h), C.byref(n), C.byref(d0)))
+        ctx.jobs.add(self)
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        lib().icx_job_sizes(self.h, C.byref(a), C.byref(b), C.byref(c))
+        self.numnp, self.nelx, self.neq = a.value, b.value, c.value
+        p = Problem()
+        ctx.check(lib().icx_job_get_problem(self.h, C.byref(p)))
+        self.ndim = 2 if p.eltype == QUAD4 else 3
+        self.dofnp = 1 if p.state == HEAT else self.ndim
+        self.div = [p.div[0], p.div[1], p.div[2] if self.ndim == 3 else 1]
+        self.numeq = n.value
+        self.design0 = d0.value
         return self
 
     def close(self):
@@ -499,6 +523,10 @@ class Job:
     def submit(self, tol=1e-8, maxit=50000):
         """Start a batched mixed-form solve (icx_submit); it advances in Context.poll."""
         self.ctx.check(lib().icx_submit(self.h, C.c_double(tol), int(maxit)))
+
+    def cancel(self):
+        """icx_cancel: stop this job's submitted solve"""
+        self.ctx.check(lib().icx_cancel(self.h))
 
     def last_info(self):
         """Info dict of the last finished submitted solve (with 'converged')."""

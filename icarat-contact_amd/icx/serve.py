@@ -13,11 +13,12 @@ batching.  Reported: jobs/s, DOF-matvec/s, and p50/p99 latency of a design itera
 """
 import collections
 import math
+import re
 import time
 
 import numpy as np
 
-from . import HEXA8, PLANE_STRESS, QUAD4, SOLID3D, SOLVE_MIXED, Job
+from . import HEXA8, OPT_SINGLE_REDUCTION, PLANE_STRESS, QUAD4, SOLID3D, SOLVE_MIXED, IcxError, Job
 
 SEED = 20261205
 
@@ -36,17 +37,43 @@ def job_spec(rng, elems_lo, elems_hi):
 
 def footprint_estimate(ndim, div, single_reduction=False):
     """Device bytes of one served job (mixed solve, cone filter, OC design loop):
-    84 B per DOF, 92 B per element, 1 B per node, 432 KB of partial sums and state;
-    a hexa8 job on the single-reduction iteration (ICX_OPT_SINGLE_REDUCTION) holds one
-    more FP32 vector (88 B per DOF) and 0.375 MB more partial sums.  icx_job_footprint
-    reports the same buffers once they exist."""
+    84 B per DOF, 92 B per element, 5 B per node (constraint mask and the FP32 inverse
+    modulus sums), 432 KB of partial sums and state; a hexa8 job on the
+    single-reduction iteration (ICX_OPT_SINGLE_REDUCTION) holds one more FP32 vector
+    (88 B per DOF) and 0.375 MB more partial sums.  icx_job_footprint reports the same
+    buffers once they exist."""
     d0, d1, d2 = div
     nn = (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
     ne = d0 * d1 * (d2 if ndim == 3 else 1)
     n = ndim * nn
     if ndim == 3 and single_reduction:
-        return 88 * n + 92 * ne + nn + 816 * 1024
-    return 84 * n + 92 * ne + nn + 432 * 1024
+        return 88 * n + 92 * ne + 5 * nn + 816 * 1024
+    return 84 * n + 92 * ne + 5 * nn + 432 * 1024
+
+
+def toml_mesh(text):
+    """(ndim, div) of a job file's [mesh] section, read only for the footprint estimate
+    before admission; the job itself is built by the library's TOML front end."""
+    sec = re.search(r"^\s*\[mesh\]\s*$(.*?)(?=^\s*\[|\Z)", text, re.M | re.S)
+    if not sec:
+        return None
+    dim = re.search(r"^\s*dimension\s*=\s*(\d+)", sec.group(1), re.M)
+    div = re.search(r"^\s*division\s*=\s*\[([^\]]*)\]", sec.group(1), re.M)
+    try:
+        d = [int(float(v)) for v in div.group(1).split(",")[:3]]
+        return int(dim.group(1)), (d[0], d[1], d[2] if int(dim.group(1)) == 3 else 1)
+    except (AttributeError, ValueError, IndexError):
+        return None
+
+
+def create(ctx, spec):
+    """The job of a request and its initial design value: spec is (ndim, div) for the
+    drawn cantilevers, or the text of a reference job file (TOML) -- mesh, material,
+    Dirichlet ranges and tractions read by icx_job_create_toml, x0 = designs_0."""
+    if isinstance(spec, str):
+        job = Job.from_toml(ctx, spec)
+        return job, job.design0
+    return make_job(ctx, *spec), None
 
 
 def make_job(ctx, ndim, div):
@@ -67,12 +94,25 @@ def make_job(ctx, ndim, div):
 
 
 def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, seed=SEED,
-        tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None, limit=None):
+        tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None, limit=None,
+        cancel=None, max_resident_dofs=None):
     """Serve n_jobs jobs arriving at `rate` per second (exponential gaps), each running
-    `iters` OC design iterations.  specs: optional list of (ndim, div) instead of the
-    drawn sizes; keep_designs: return each job's final design under "designs"; limit:
-    serve only the first `limit` jobs of the drawn stream.  Returns a dict of the
-    measurements."""
+    `iters` OC design iterations.
+    specs: optional list of requests instead of the drawn sizes -- (ndim, div)
+      cantilevers or job-file texts (TOML, the reference's keys; x0 = its designs_0); a
+      request the library refuses is recorded under "rejected" and the loop goes on.
+    keep_designs: return each job's final design under "designs" and its objective
+      history under "objectives".
+    limit: serve only the first `limit` jobs of the drawn stream.
+    cancel: {job: design iteration} -- the job is withdrawn while that iteration's solve
+      runs (icx_cancel), its budget freed.
+    max_resident_dofs: admit a job only while the resident jobs' DOFs, its own included,
+      stay within this many (a job larger than the cap is admitted alone).  The PCG
+      iterations of all resident jobs share each launch, so this bounds the work of one
+      batched iteration and with it a design iteration's latency; the wait moves to
+      admission (first come, first served: a job never overtakes an earlier one).
+    Returns a dict of the measurements."""
+    cancel = dict(cancel or {})
     rng = np.random.default_rng(np.random.PCG64(seed))
     gaps = rng.exponential(1.0 / rate, n_jobs)
     arrival = np.cumsum(gaps) - gaps[0]
@@ -81,19 +121,37 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
     if limit is not None:
         specs = specs[:limit]
     n_jobs = len(specs)
-    est = [footprint_estimate(*s) for s in specs]
+    sr = bool(getattr(ctx, "options", {}).get(OPT_SINGLE_REDUCTION, 0))
+    def estimate(spec):
+        if isinstance(spec, str):
+            m = toml_mesh(spec)
+            return footprint_estimate(*m, single_reduction=sr) if m else 0.0
+        return footprint_estimate(*spec, single_reduction=sr)
+
+    est = [estimate(s) for s in specs]
+
+    def dofs(spec):
+        m = toml_mesh(spec) if isinstance(spec, str) else spec
+        if not m:
+            return 0
+        ndim, (d0, d1, d2) = m
+        return ndim * (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
+
+    ndofs = [dofs(s) for s in specs]
+    resident = peak_resident = 0
     if max(est) > budget:
         raise ValueError("a job needs %.3g bytes, more than the budget %.3g" % (max(est), budget))
 
     pending = collections.deque(range(n_jobs))
     active = {}
     lat, admit_wait, footprints = [], [], []
-    designs = {}
+    designs, histories = {}, {}
     dof_matvecs = 0
     used = 0.0
     peak_used = 0.0
     peak_active = 0
     finished_jobs = 0
+    cancelled, rejected = [], []
     t0 = time.perf_counter()
 
     def begin(rec):
@@ -103,24 +161,44 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
 
     while pending or active:
         now = time.perf_counter() - t0
-        while pending and arrival[pending[0]] <= now and used + est[pending[0]] <= budget:
+        while (pending and arrival[pending[0]] <= now and used + est[pending[0]] <= budget and
+               (max_resident_dofs is None or not active or resident + ndofs[pending[0]] <= max_resident_dofs)):
             i = pending.popleft()
-            ndim, div = specs[i]
-            job = make_job(ctx, ndim, div)
-            job.opt_init(np.full(job.nelx, x0), driver=0, radius=radius, projection=True,
-                         form=SOLVE_MIXED, tol=tol, maxit=maxit)
+            try:
+                job, x_init = create(ctx, specs[i])
+            except IcxError as e:
+                rejected.append((i, str(e)))
+                finished_jobs += 1
+                continue
+            job.opt_init(np.full(job.nelx, x0 if x_init is None else x_init), driver=0, radius=radius,
+                         projection=True, form=SOLVE_MIXED, tol=tol, maxit=maxit)
             rec = {"job": job, "spec": i, "step": 0, "numeq": job.numeq}
             active[id(job)] = rec
             used += est[i]
+            resident += ndofs[i]
+            peak_resident = max(peak_resident, resident)
             peak_used = max(peak_used, used)
             peak_active = max(peak_active, len(active))
             admit_wait.append(time.perf_counter() - t0 - arrival[i])
             begin(rec)
         if active:
-            for job in ctx.poll(chunks=1):
+            finished = ctx.poll(chunks=1)
+            for rec in [r for r in active.values() if cancel.get(r["spec"]) == r["step"]]:
+                if rec["job"] in finished:
+                    continue
+                rec["job"].cancel()
+                rec["job"].close()
+                del active[id(rec["job"])]
+                used -= est[rec["spec"]]
+                resident -= ndofs[rec["spec"]]
+                cancelled.append(rec["spec"])
+                finished_jobs += 1
+            for job in finished:
                 rec = active[id(job)]
                 info = job.last_info()
-                job.opt_step_end(info)
+                step_rec = job.opt_step_end(info)
+                if keep_designs:
+                    histories.setdefault(rec["spec"], []).append(step_rec["objective"])
                 lat.append(time.perf_counter() - rec["t_begin"])
                 dof_matvecs += rec["numeq"] * info["matvecs"]
                 rec["step"] += 1
@@ -134,6 +212,7 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
                     job.close()
                     del active[id(job)]
                     used -= est[rec["spec"]]
+                    resident -= ndofs[rec["spec"]]
                     finished_jobs += 1
                     if log:
                         log("job %d done (%d left)" % (rec["spec"], n_jobs - finished_jobs))
@@ -153,6 +232,10 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         "stream_jobs_drawn": len(drawn),
         "budget_bytes": budget, "tol": tol,
         "footprint_checks": footprints,
+        "cancelled": cancelled,
+        "rejected": rejected,
         "designs": designs if keep_designs else None,
+        "objectives": histories if keep_designs else None,
         "max_used_within_budget": peak_used <= budget,
+        "max_resident_dofs": max_resident_dofs, "peak_resident_dofs": peak_resident,
     }

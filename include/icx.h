@@ -87,6 +87,19 @@ icx_status icx_ctx_synchronize(icx_ctx *ctx);
  * LinearElastic::setParameter (src/problem/elasticity.cpp:19-32). */
 icx_status icx_job_create(icx_ctx *ctx, const icx_problem *prob, icx_job **out);
 icx_status icx_job_destroy(icx_job *job);
+/* A box job from the text of a reference job file (TOML), as the drivers set one up:
+ * [mesh] dimension / type / division / length (MeshIcarat::setParameter,
+ * src/io/mesh_icarat.hpp:69-128), [material] (Elastic::setParameter,
+ * src/problem/elasticity.cpp:19-32; Heat::setParameter, heat.cpp:19-60), then the
+ * [boundary_condition] dirichlet ranges, countDOF and the neumann tractions
+ * (BCIcarat::input_dirich / makeDOF / input_neumann,
+ * src/io/boundary_condition_icarat.hpp:68-396; boundary_condition_base.cpp:143-180).  The
+ * job comes back with its DOF map and load; numeq and design0 (either may be NULL)
+ * receive the free DOF count and designs_0.  A malformed or unsupported spec returns
+ * ICX_ERR_ARG with the reason in icx_last_error.  The C++ drivers' StructuredModel builds
+ * its jobs with the same code. */
+icx_status icx_job_create_toml(icx_ctx *ctx, const char *toml_text, icx_job **out, int *numeq,
+                               double *design0);
 
 /* Unstructured mesh of one element type (MeshMDPA, src/io/mesh_mdpa.hpp:84-238, with
  * the elements of LinearElastic, src/problem/elasticity.cpp:210-254): node coordinates
@@ -112,6 +125,9 @@ icx_status icx_job_create_mesh(icx_ctx *ctx, const icx_mesh *mesh, icx_job **out
 icx_status icx_mdpa_read(const char *path, int eltype, int *numnp, int *nelx, double *x, int *conn);
 /* numnp, nelx, neq (= numnp * dofnp) */
 icx_status icx_job_sizes(const icx_job *job, int *numnp, int *nelx, int *neq);
+/* the job's problem descriptor (mesh jobs: eltype, state and material; div and length
+ * zero) -- what FEM / the [mesh] and [material] keys hold for the reference's drivers */
+icx_status icx_job_get_problem(const icx_job *job, icx_problem *prob);
 /* node coordinates (numnp x 3) and connectivity (nelx x ne), as MeshIcarat builds them */
 icx_status icx_get_mesh(const icx_job *job, double *x, int *conn);
 
@@ -137,7 +153,11 @@ icx_status icx_add_nodal_load(icx_job *job, int node, int comp, double value);
  * LinearElastic::setDesignS for every element (src/problem/elasticity.hpp:60):
  * the SIMP modulus of element e is computed from design[e] on the device.  Heat jobs:
  * LinearHeat::setDesign (src/problem/heat.hpp:34), conductivity
- * design^p De1 + (1 - design^p) De2 (Heat::makeSIMPDe, heat.cpp:62-77). */
+ * design^p De1 + (1 - design^p) De2 (Heat::makeSIMPDe, heat.cpp:62-77).
+ * A design outside [0, 1] (negative, above 1, NaN or infinite) is refused with
+ * ICX_ERR_NUMERIC and leaves the job's design unchanged (the reference does not check).
+ * While a submitted solve of the job runs, this call and the others that change what
+ * it reads (loads, Dirichlet ranges, solves, design-loop steps) return ICX_ERR_STATE. */
 icx_status icx_set_design(icx_job *job, const double *design);
 /* Heat jobs: the [material] conductivity rows of materials 1 and 2, six entries each
  * (xx, yy, zz, xy, yz, xz), as read by Heat::setParameter (src/problem/heat.cpp:19-60;
@@ -332,6 +352,10 @@ icx_status icx_opt_get_design(const icx_job *job, double *x);
 icx_status icx_submit(icx_job *job, double tol, int maxit);
 icx_status icx_poll(icx_ctx *ctx, int chunks, icx_job **finished, int cap, int *nfinished);
 icx_status icx_job_last_info(const icx_job *job, icx_solve_info *info);
+/* icx_cancel: stop the job's submitted solve and take it out of the batch (the other
+ *   solves go on unchanged); its displacement is then undefined (no solution until the
+ *   next solve) and icx_poll never returns it.  ICX_ERR_STATE if it has none running. */
+icx_status icx_cancel(icx_job *job);
 icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, double tol,
                            int maxit, icx_solve_info *infos);
 /* device bytes a job holds (admission control) */

@@ -1,0 +1,159 @@
+"""Failure handling on the GPU path: invalid designs, zero moduli (zero Jacobi diagonal),
+calls refused while a submitted solve runs, cancelling a submitted or served solve, and
+the stress adjoint leaving the job's displacement in place when its solve stops at maxit.
+Every error is a returned status; no kernel is made to fault."""
+import numpy as np
+import pytest
+
+import icx
+import oracle as O
+from icx import serve
+
+pytestmark = pytest.mark.gpu
+
+ERR_STATE, ERR_NUMERIC = 2, 6
+
+
+def test_set_design_refuses_nan_negative_and_out_of_range(ctx):
+    job = icx.cantilever_3d(ctx, 8, 4, 4)
+    rho = O.random_density(job.nelx, 1)
+    job.set_design(rho)
+    u0, _ = job.solve(icx.SOLVE_PLAIN, tol=1e-12)
+    for bad in (np.nan, -0.1, 1.5, np.inf, -np.inf):
+        x = rho.copy()
+        x[17] = bad
+        with pytest.raises(icx.IcxError) as e:
+            job.set_design(x)
+        assert e.value.code == ERR_NUMERIC and "design[17]" in str(e.value)
+    # the refused calls left the design as it was
+    u1, _ = job.solve(icx.SOLVE_PLAIN, tol=1e-12)
+    assert np.array_equal(u0, u1)
+    # both ends of [0, 1] are valid
+    x = rho.copy()
+    x[0], x[1] = 0.0, 1.0
+    job.set_design(x)
+    job.close()
+
+
+def _void_block(nx, ny, nz):
+    """densities with a void block (rho = 0) whose interior nodes touch only void
+    elements"""
+    x = O.random_density(nx * ny * nz, 4, lo=0.3)
+    for k in range(1, 4):
+        for j in range(1, 4):
+            for i in range(3, 7):
+                x[i + nx * (j + ny * k)] = 0.0
+    return x
+
+
+@pytest.mark.parametrize("form", [icx.SOLVE_PLAIN, icx.SOLVE_FUSED, icx.SOLVE_MIXED])
+def test_zero_modulus_gives_zero_diagonal_rows(ctx, form):
+    """Emin = 0 and a void block: the interior nodes of the block have a zero diagonal
+    (Jacobi factor 1, Eigen's rule) and zero rows; every form converges to the oracle's
+    Eigen-loop solution, and those nodes stay at zero."""
+    nx, ny, nz = 12, 5, 5
+    young = (0.0, 1.0)
+    job = icx.cantilever_3d(ctx, nx, ny, nz, young=young)
+    pr = O.cantilever_3d(nx, ny, nz, young=young)
+    x = _void_block(nx, ny, nz)
+    job.set_design(x)
+    b = pr.assemble(x)
+    K = pr.csr()
+    diag = K.diagonal()
+    zero_rows = np.nonzero(diag == 0.0)[0]
+    assert len(zero_rows) == 3 * 2 * 2 * 3   # nodes i = 4..6, j = 2..3, k = 2..3
+    uo, _, _ = pr.cg(b, tol=1e-12, maxit=100000)
+    tol = 1e-8 if form == icx.SOLVE_MIXED else 1e-12
+    u, info = job.solve(form, tol=tol, maxit=100000)
+    assert info["converged"] and np.all(np.isfinite(u))
+    assert np.all(u[zero_rows] == 0.0)
+    assert np.linalg.norm(b - K @ u) / np.linalg.norm(b) <= tol * (1 + 1e-6)
+    bar = 1e-6 if form == icx.SOLVE_MIXED else 1e-9
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) <= bar
+    job.close()
+
+
+def test_calls_refused_while_a_submitted_solve_runs(ctx):
+    job = icx.cantilever_3d(ctx, 10, 5, 5)
+    rho = O.random_density(job.nelx, 2)
+    job.set_design(rho)
+    u_alone, _ = job.solve(icx.SOLVE_MIXED, tol=1e-8)
+    job.submit(tol=1e-8)
+    calls = [lambda: job.set_design(rho), lambda: job.solve(icx.SOLVE_PLAIN),
+             lambda: job.set_fext(np.zeros(job.numeq)), lambda: job.add_nodal_load(0, 0, 1.0),
+             lambda: job.add_neumann([0.0, 0.0, -1.0], [10, 10, 0, 5, 0, 5]),
+             lambda: job.submit(tol=1e-8)]
+    for call in calls:
+        with pytest.raises(icx.IcxError) as e:
+            call()
+        assert e.value.code == ERR_STATE
+    done = []
+    while not done:
+        done = ctx.poll()
+    assert done == [job]
+    assert np.array_equal(job.displacement(), u_alone)
+    job.close()
+
+
+def test_cancel_a_submitted_solve(ctx):
+    """Three submitted solves, one cancelled after a chunk: the other two finish with
+    their solo results, the cancelled job is never returned, and it solves normally
+    afterwards."""
+    jobs = [icx.cantilever_3d(ctx, 12, 6, 6), icx.cantilever_2d(ctx, 60, 20), icx.cantilever_3d(ctx, 10, 5, 5)]
+    alone = []
+    for s, j in enumerate(jobs):
+        j.set_design(O.random_density(j.nelx, 10 + s))
+        alone.append(j.solve(icx.SOLVE_MIXED, tol=1e-8)[0])
+    for j in jobs:
+        j.submit(tol=1e-8)
+    first = ctx.poll(chunks=1)
+    assert jobs[0] not in first
+    jobs[0].cancel()
+    with pytest.raises(icx.IcxError) as e:
+        jobs[0].cancel()
+    assert e.value.code == ERR_STATE
+    with pytest.raises(icx.IcxError):
+        jobs[0].displacement()          # no solution after a cancel
+    got = list(first)
+    while len(got) < 2:
+        got += ctx.poll()
+    assert set(got) == {jobs[1], jobs[2]}
+    assert np.array_equal(jobs[1].displacement(), alone[1])
+    assert np.array_equal(jobs[2].displacement(), alone[2])
+    u, info = jobs[0].solve(icx.SOLVE_MIXED, tol=1e-8)
+    assert info["converged"] and np.array_equal(u, alone[0])
+    for j in jobs:
+        j.close()
+
+
+def test_cancel_a_served_job(ctx):
+    """A served job withdrawn during its second design iteration frees its budget; the
+    others' designs equal their solo runs."""
+    specs = [(2, (30, 10, 1)), (3, (10, 5, 5)), (2, (24, 8, 1))]
+    out = serve.run(ctx, rate=500.0, iters=3, specs=specs, tol=1e-8, keep_designs=True, cancel={1: 1})
+    assert out["cancelled"] == [1]
+    assert sorted(out["designs"]) == [0, 2]
+    assert out["design_iterations"] == 3 + 1 + 3
+    for i in (0, 2):
+        ndim, div = specs[i]
+        j = serve.make_job(ctx, ndim, div)
+        j.opt_init(np.full(j.nelx, 0.4), driver=0, radius=1.5, projection=True, form=icx.SOLVE_MIXED,
+                   tol=1e-8, maxit=50000)
+        for _ in range(3):
+            j.opt_step()
+        assert np.array_equal(out["designs"][i], j.opt_design()), i
+        j.close()
+
+
+def test_stress_adjoint_at_maxit_keeps_the_displacement(ctx):
+    """The stress constraint's adjoint solve stopped at maxit = 1: g and dg come back and
+    the job's displacement (and compliance) are the ones of the structural solve."""
+    job = icx.cantilever_2d(ctx, 30, 10)
+    job.set_design(O.random_density(job.nelx, 6, lo=0.3))
+    u0, _ = job.solve(icx.SOLVE_PLAIN, tol=1e-12)
+    c0 = job.compliance()
+    g, dg, _ = job.stress_constraint(1.0, 8.0, 0.7, tol=1e-12, maxit=1)
+    assert np.isfinite(g) and dg.shape == (job.nelx,)
+    assert np.array_equal(job.displacement(), u0)
+    assert job.compliance() == c0
+    job.close()

@@ -240,10 +240,23 @@ def test_projection(ctx):
         assert np.abs(dh - O.threshold_derivative(x, beta)).max() <= 1e-14 * beta
 
 
-@pytest.mark.parametrize("ne,div,R", [(4, (24, 10, 1), 2.0), (8, (9, 6, 5), 2.0), (8, (10, 7, 6), 3.0)])
-def test_helmholtz_filter(ctx, ne, div, R):
+@pytest.mark.parametrize("ne,div,R,depth", [(4, (24, 10, 1), 2.0, 0), (8, (9, 6, 5), 2.0, 0), (8, (10, 7, 6), 3.0, 0),
+                                            (8, (20, 17, 11), 2.0, 0), (8, (20, 17, 11), 2.0, 4),
+                                            (8, (10, 7, 6), 3.0, 8)])
+def test_helmholtz_filter(ctx, ne, div, R, depth):
+    """Helmholtz filter against the oracle.  hexa8 at tile depth 0 takes the shared-element
+    tile kernel (20 x 17 x 11: several 15 x 15 node tiles and 8-layer z chunks); depths
+    4 and 8 take the per-node gather."""
     st = O.PLANE_STRESS if ne == 4 else O.SOLID3D
     L = tuple(float(d) for d in div)
+    ctx.set_option(icx.OPT_TILE_DEPTH, depth)
+    try:
+        _helmholtz_case(ctx, ne, div, R, st, L)
+    finally:
+        ctx.set_option(icx.OPT_TILE_DEPTH, 0)
+
+
+def _helmholtz_case(ctx, ne, div, R, st, L):
     job = icx.Job(ctx, ne, div, L, st)
     pr = O.Problem(ne, div, L, st)
     job.helmholtz_setup(R)

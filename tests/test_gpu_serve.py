@@ -40,3 +40,65 @@ def test_admission_by_footprint(ctx):
     assert out["max_used_within_budget"]
     for real, estimate in out["footprint_checks"]:
         assert real <= estimate, (real, estimate)
+
+
+TOML_Q4 = """# 40 x 12 quad4 cantilever, left edge clamped, traction strip at the free end
+[mesh]
+dimension = 2
+type = "quad4"
+division = [40, 12, 1]
+length = [40.0, 12.0, 1.0]
+
+[boundary_condition]
+dirichlet = [
+  [[1, 1, 0, 0, 0, 0], [0.0, 0.0, 0.0, 0.0, 0.0, 0.0], [[0.0, 0.0], [0.0, 12.0], [0.0, 0.0]]],
+]
+neumann = [
+  [[0.0, -0.01, 0.0], [[40.0, 40.0], [4.0, 8.0], [0.0, 0.0]]],
+]
+
+[material]
+designs_0 = 0.45
+penalty = 3.0
+poisson = 0.3
+state = "plane_stress"
+type = "Elastic"
+young = [0.001, 1.0]
+"""
+
+
+def test_served_toml_job_equals_the_cxx_driver(ctx, tmp_path):
+    """A job given as the text of a reference job file is served (icx_job_create_toml,
+    the front end the C++ drivers use) and follows the C++ driver A on the same file step
+    by step (same mixed-form solves at the reference's 1e-10, cone r = 3, projection);
+    a malformed request is refused and the stream goes on."""
+    import os
+    import subprocess
+    steps = 4
+    bad = TOML_Q4.replace('type = "quad4"', 'type = "tria3"')
+    out = serve.run(ctx, rate=500.0, iters=steps, specs=[TOML_Q4, bad, (2, (30, 10, 1))], tol=1e-10, radius=3.0,
+                    keep_designs=True)
+    assert [r[0] for r in out["rejected"]] == [1] and "tria3" in out["rejected"][0][1]
+    assert sorted(out["designs"]) == [0, 2]
+    f = tmp_path / "job.toml"
+    f.write_text(TOML_Q4)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "icarat-contact_amd", "_build", "opt_macro_oc")
+    res = subprocess.run([exe, str(f), "HIP_PCG_MIXED", str(steps), "3.0"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=300)
+    assert res.returncode == 0, res.stderr[-2000:]
+    rows = np.loadtxt(tmp_path / "function.csv", delimiter=",", ndmin=2)
+    served = np.array(out["objectives"][0])
+    assert rows.shape[0] == steps and len(served) == steps
+    assert np.abs(served - rows[:, 1]).max() <= 1e-9 * np.abs(rows[:, 1]).max(), (served, rows[:, 1])
+
+
+def test_resident_dof_cap(ctx):
+    """With the resident DOFs capped the jobs queue at admission, one batch never holds
+    more than the cap, and every design still equals its solo run."""
+    iters, tol = 2, 1e-8
+    cap = 2 * 31 * 11 + 3 * 11 * 6 * 6   # DOFs of the first two jobs: all four never fit
+    out = serve.run(ctx, rate=1000.0, iters=iters, specs=SPECS, tol=tol, keep_designs=True, max_resident_dofs=cap)
+    assert out["peak_resident_dofs"] <= cap and out["peak_admitted_jobs"] < len(SPECS)
+    for i, (ndim, div) in enumerate(SPECS):
+        assert np.array_equal(out["designs"][i], _alone(ctx, ndim, div, iters, tol)), i
