@@ -2546,6 +2546,22 @@ icx_status icx_cone_filter(icx_job *j, int adjoint, const double *in, double *ou
   return ICX_OK;
 }
 
+icx_status icx_sensitivity_filter(icx_job *j, const double *design, const double *dfds, double *out) {
+  if (!j || !design || !dfds || !out) return ICX_ERR_ARG;
+  if (!j->cw.ready) return fail(j->ctx, ICX_ERR_STATE, "icx_cone_filter_setup first");
+  icx_ctx *c = j->ctx;
+  const long long ne = j->g.ne;
+  DBuf<double> x, df, o;
+  HIPCHK(c, x.alloc(ne)); HIPCHK(c, df.alloc(ne)); HIPCHK(c, o.alloc(ne));
+  HIPCHK(c, hipMemcpyAsync(x.p, design, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  HIPCHK(c, hipMemcpyAsync(df.p, dfds, ne * sizeof(double), hipMemcpyHostToDevice, S(j)));
+  if (j->mesh) launch_cone_sensitivity_csr(ne, cone_csr(j), j->cw.aa.p, x.p, df.p, j->d_vol.p, o.p, S(j));
+  else launch_cone_sensitivity(j->g, stencil(j), j->cw.aa.p, x.p, df.p, j->volume, o.p, S(j));
+  HIPCHK(c, hipMemcpyAsync(out, o.p, ne * sizeof(double), hipMemcpyDeviceToHost, S(j)));
+  HIPCHK(c, hipStreamSynchronize(S(j)));
+  return ICX_OK;
+}
+
 icx_status icx_helmholtz_setup(icx_job *j, double radius) {
   if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the Helmholtz filter");
   if (!j || !(radius > 0.0)) return ICX_ERR_ARG;

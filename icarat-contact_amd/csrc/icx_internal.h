@@ -494,6 +494,12 @@ struct ConeCsr {
 };
 void launch_cone_csr(long long n, ConeCsr f, const double *aa, const double *in, double *out, int adjoint,
                      hipStream_t s);
+// Filter::sensitivityFilter over the cone stencil (box volume vol) or the CSR rows
+// (per-element volumes volv)
+void launch_cone_sensitivity(const Grid &g, Stencil st, const double *aa, const double *x, const double *df,
+                             double vol, double *out, hipStream_t s);
+void launch_cone_sensitivity_csr(long long n, ConeCsr f, const double *aa, const double *x, const double *df,
+                                 const double *volv, double *out, hipStream_t s);
 void launch_oc_volume_csr(long long n, ConeCsr f, const double *aa, double lambda, double move, double xmin,
                           double xmax, const double *x, const double *dfdx, const double *dgdx, int project,
                           double beta, double T, const double *vol, double *out, Reduce red, hipStream_t s,

@@ -906,6 +906,48 @@ void launch_cone_adjoint(const Grid &g, Stencil st, const double *aa, const doub
   ICX_CHECK_LAUNCH();
 }
 
+// Filter::sensitivityFilter (src/optimization/filter.hpp:72-100), the classic sensitivity
+// filter, as the reference writes it: bb = sum_j (w x_j df_j) / (x_j v_j) in neighbour
+// order, out = ((x_i v_i) / (x_i aa_i)) bb (an element with x = 0 gives 0 / 0, as there).
+// v: per-element volumes (volv) or the box volume (vol).
+__device__ __forceinline__ double sens_term(double w, double x, double df, double v) {
+  return (w * x * df) / (x * v);
+}
+__global__ void k_cone_sens(Grid g, Stencil st, const double *aa, const double *x, const double *df, double vol,
+                            double *out) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < g.ne;
+       e += (long long)gridDim.x * blockDim.x) {
+    double bb = 0.0;
+    for (int q = 0; q < st.n; ++q) {
+      long long j;
+      if (nb_elem(g, e, st.off + 3 * q, j)) bb += sens_term(st.w[q], x[j], df[j], vol);
+    }
+    out[e] = ((x[e] * vol) / (x[e] * aa[e])) * bb;
+  }
+}
+__global__ void k_cone_sens_csr(long long n, ConeCsr f, const double *aa, const double *x, const double *df,
+                                const double *volv, double *out) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    double bb = 0.0;
+    for (int k = f.ptr[e]; k < f.ptr[e + 1]; ++k) {
+      const int j = f.id[k];
+      bb += sens_term(f.w[k], x[j], df[j], volv[j]);
+    }
+    out[e] = ((x[e] * volv[e]) / (x[e] * aa[e])) * bb;
+  }
+}
+void launch_cone_sensitivity(const Grid &g, Stencil st, const double *aa, const double *x, const double *df,
+                             double vol, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_cone_sens, dim3(ew_blocks(g.ne)), dim3(256), 0, s, g, st, aa, x, df, vol, out);
+  ICX_CHECK_LAUNCH();
+}
+void launch_cone_sensitivity_csr(long long n, ConeCsr f, const double *aa, const double *x, const double *df,
+                                 const double *volv, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_cone_sens_csr, dim3(ew_blocks(n)), dim3(256), 0, s, n, f, aa, x, df, volv, out);
+  ICX_CHECK_LAUNCH();
+}
+
 // threshold / thresholdDerivative, src/optimization/filter.hpp:153-185
 __device__ __forceinline__ double proj(double x, double beta, double T) {
   return 0.5 * (tanh(T * beta) + tanh(beta * (x - T))) / tanh(T * beta);

@@ -27,7 +27,7 @@ EXPORTS = [
     "icx_set_design", "icx_set_conductivity", "icx_matvec", "icx_matvec_noalloc", "icx_diagonal", "icx_solve", "icx_get_displacement",
     "icx_get_node_values", "icx_compliance", "icx_sensitivity", "icx_element_volume",
     "icx_mises", "icx_reactions", "icx_output_begin", "icx_output_write", "icx_write_vtk",
-    "icx_cone_filter_setup", "icx_cone_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
+    "icx_cone_filter_setup", "icx_cone_filter", "icx_sensitivity_filter", "icx_helmholtz_setup", "icx_helmholtz_filter",
     "icx_project", "icx_oc_update", "icx_mma_reset", "icx_mma_update",
     "icx_gcmma_reset", "icx_gcmma_outer_update", "icx_gcmma_inner_update", "icx_gcmma_concheck",
     "icx_mma_set_asymptotes", "icx_stress_constraint",
@@ -408,6 +408,13 @@ class Job:
         v = _f64(v)
         out = np.zeros(self.nelx)
         self.ctx.check(lib().icx_cone_filter(self.h, int(adjoint), _dp(v), _dp(out)))
+        return out
+
+    def sensitivity_filter(self, design, dfds):
+        """Filter::sensitivityFilter (icx_sensitivity_filter; cone_setup first)"""
+        x, df = _f64(design), _f64(dfds)
+        out = np.zeros(self.nelx)
+        self.ctx.check(lib().icx_sensitivity_filter(self.h, _dp(x), _dp(df), _dp(out)))
         return out
 
     def helmholtz_setup(self, radius):

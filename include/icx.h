@@ -235,6 +235,12 @@ icx_status icx_element_volume(const icx_job *job, double *vol);
  * (src/optimization/filter.hpp:17-151, 191-234); adjoint = isSens. */
 icx_status icx_cone_filter_setup(icx_job *job, double radius);
 icx_status icx_cone_filter(icx_job *job, int adjoint, const double *in, double *out);
+/* Filter::sensitivityFilter(design_s, dfds) (src/optimization/filter.hpp:72-100): the
+ * classic sensitivity filter over the same neighbours (icx_cone_filter_setup first),
+ * out_i = ((x_i v_i) / (x_i a_i)) sum_j (w_ij x_j dfds_j) / (x_j v_j), a_i = sum_j w_ij,
+ * v the element volumes; as in the reference an element with x = 0 gives a non-finite
+ * value.  Neither reference driver calls it. */
+icx_status icx_sensitivity_filter(icx_job *job, const double *design, const double *dfds, double *out);
 /* FilterHelmholtz(fem, element, node, R) + densityFilter(v, "Eigen_CG", log, sensFlag)
  * (src/optimization/filter_helm.hpp:23-137, 178-256); solves to tol (1e-10 in
  * the reference).  iters may be NULL. */

@@ -427,10 +427,10 @@ void icxo_neumann(int ndim, int dofnp, int ne, int nelx, const double *x,
           double cz = t1[0] * t2[1] - t1[1] * t2[0];
           double jac = sqrt(cx * cx + cy * cy + cz * cz) * 1.0;
           for (int a = 0; a < 4; a++)
-            for (int c = 0; c < 3; c++) fe[a * 3 + c] += Nq[a] * val3[c] * jac;
+            for (int c = 0; c < dofnp && c < 3; c++) fe[a * 3 + c] += Nq[a] * val3[c] * jac;
         }
         for (int a = 0; a < 4; a++)
-          for (int c = 0; c < 3; c++) {
+          for (int c = 0; c < dofnp && c < 3; c++) {
             int id = dof[dofnp * conn[ne * e + sn[a]] + c];
             if (id < numeq) fext[id] += fe[a * 3 + c];
           }
@@ -821,6 +821,7 @@ int icxo_sys_cg(const icxo_sys *s, const double *b, double *x, double tol, int m
 static int banded_cholesky_solve(int n, const int *rowptr, const int *col,
                                  const double *val, const double *b, double *x)
 {
+  if (n <= 0) return -1;
   int bw = 0;
   for (int r = 0; r < n; r++)
     for (int q = rowptr[r]; q < rowptr[r + 1]; q++)
@@ -1292,6 +1293,26 @@ void icxo_filter_apply(icxo_filter *f, const double *in, double *out, int is_sen
       }
       out[i] = s;
     }
+  }
+}
+
+/* Filter::sensitivityFilter: src/optimization/filter.hpp:72-100 (aa summed afresh, the
+ * neighbour terms as written there) */
+void icxo_filter_sensitivity(const icxo_filter *f, const double *design, const double *dfds,
+                             const double *volume, double *out)
+{
+  const double r = f->radius;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < f->nelx; i++) {
+    double aa = 0.0, bb = 0.0;
+    for (int q = f->ptr[i]; q < f->ptr[i + 1]; q++) {
+      const int j = f->id[q];
+      const double w = r - f->dist[q];
+      aa += w;
+      bb += (w * design[j] * dfds[j]) / (design[j] * volume[j]);
+    }
+    const double cc = (design[i] * volume[i]) / (design[i] * aa);
+    out[i] = cc * bb;
   }
 }
 

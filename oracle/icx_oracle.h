@@ -99,6 +99,9 @@ void icxo_filter_destroy(icxo_filter *f);
 long long icxo_filter_nnz(const icxo_filter *f);
 void icxo_filter_neighbours(const icxo_filter *f, int *ptr, int *id, double *dist);
 void icxo_filter_apply(icxo_filter *f, const double *in, double *out, int is_sens);
+/* Filter::sensitivityFilter (filter.hpp:72-100): volume per element */
+void icxo_filter_sensitivity(const icxo_filter *f, const double *design, const double *dfds,
+                             const double *volume, double *out);
 void icxo_threshold(int n, const double *in, double beta, double T, double *out);
 void icxo_threshold_derivative(int n, const double *in, double beta, double T, double *out);
 

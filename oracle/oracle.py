@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libicx_oracle.so")
+# ICX_ORACLE_LIB selects another build of the same source (the sanitizer build)
+_LIB_PATH = os.environ.get("ICX_ORACLE_LIB") or os.path.join(_HERE, "_build", "libicx_oracle.so")
 
 PLANE_STRESS, PLANE_STRAIN, SOLID3D, HEAT = 0, 1, 2, 3
 
@@ -66,6 +67,7 @@ def lib():
             "icxo_filter_nnz": (C.c_longlong, [vp]),
             "icxo_filter_neighbours": (None, [vp, _ip, _ip, _dp]),
             "icxo_filter_apply": (None, [vp, _dp, _dp, i]),
+            "icxo_filter_sensitivity": (None, [vp, _dp, _dp, _dp, _dp]),
             "icxo_threshold": (None, [i, _dp, d, d, _dp]),
             "icxo_threshold_derivative": (None, [i, _dp, d, d, _dp]),
             "icxo_helm_create": (vp, [i, i, i, i, _dp, _ip, d]),
@@ -379,6 +381,12 @@ class ConeFilter:
     def apply(self, v, sens=False):
         out = np.zeros(self.n)
         lib().icxo_filter_apply(self.h, _f64(v), out, int(sens))
+        return out
+
+    def sensitivity(self, design, dfds, volume):
+        """Filter::sensitivityFilter (filter.hpp:72-100)"""
+        out = np.zeros(self.n)
+        lib().icxo_filter_sensitivity(self.h, _f64(design), _f64(dfds), _f64(volume), out)
         return out
 
 

@@ -27,7 +27,7 @@ def test_oracle_matches_golden(name):
     b = pr.assemble(g["rho"])
     u = pr.direct(b)
     assert np.linalg.norm(u - g["u"]) / np.linalg.norm(g["u"]) < 1e-10
-    assert abs(u @ pr.fext - g["c"]) / g["c"] < 1e-12
+    assert abs(u @ pr.fext - g["c"]) / g["c"] < 1e-11
     dc, _ = pr.sensitivity(g["rho"], pr.element_energy(pr.node_values(u)), pr.volumes())
     assert np.abs(dc - g["dc"]).max() <= 1e-9 * np.abs(g["dc"]).max()
     assert np.abs(pr.cone_filter(1.5).apply(g["rho"]) - g["cone"]).max() < 1e-14

@@ -72,3 +72,27 @@ def test_cxx_driver_b_executable_matches_oracle():
     assert rows.shape[0] == steps
     rel = np.abs(rows[:, 1] - ref["obj"]) / np.abs(ref["obj"])
     assert rel.max() <= 1e-7, rel
+
+
+def test_eigen_ldlt_substitute_on_a_design_with_voids(ctx):
+    """Driver B's "Eigen_LDLT" (SimplicialLDLT, solvers.cpp:178-200) runs as the FP64 plain
+    form at rel-res 1e-14.  On the C1 cantilever's design after 60 OC iterations (862
+    elements at xmin = 1e-4, i.e. E = Emin + 1e-12 ~ 1e-9: contrast 1e9) that solve meets
+    the direct solution (the oracle's banded Cholesky) within 1e-10 in u and c."""
+    nx, ny = 120, 40
+    pr = O.cantilever_2d(nx, ny)
+    ref = pr.run_driver(0, np.full(pr.nelx, 0.5), 60, 1.5, projection=True, direct=True)
+    rho = ref["design"]
+    assert (rho <= 1.0001e-4).sum() > 500 and rho.max() == 1.0   # voids and solid
+    b = pr.assemble(rho)
+    ud = pr.direct(b)
+    job = icx.cantilever_2d(ctx, nx, ny)
+    job.set_design(rho)
+    u, info = job.solve(icx.SOLVE_PLAIN, tol=1e-14, maxit=200000)
+    assert info["converged"], info
+    cd = ud @ pr.fext
+    print("LDLT substitute: %d iterations, |du|/|u| %.3e, dc %.3e"
+          % (info["iterations"], np.linalg.norm(u - ud) / np.linalg.norm(ud), abs(job.compliance() - cd) / cd))
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-10
+    assert abs(job.compliance() - cd) / cd <= 1e-10
+    job.close()

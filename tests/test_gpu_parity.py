@@ -231,6 +231,32 @@ def test_cone_filter_forward_adjoint(ctx, ne, div, r):
     assert np.abs(job.cone(s, adjoint=True) - f.apply(s, sens=True)).max() <= 1e-14
 
 
+def test_sensitivity_filter(ctx):
+    """Filter::sensitivityFilter (filter.hpp:72-100) on box jobs and a distorted mesh job
+    against the oracle."""
+    from meshgen import box_mesh, distort
+    rng = np.random.default_rng(12)
+    for ne, div, r in [(4, (20, 9, 1), 1.5), (8, (7, 6, 5), 1.8)]:
+        st = O.PLANE_STRESS if ne == 4 else O.SOLID3D
+        L = tuple(float(d) for d in div)
+        job = icx.Job(ctx, ne, div, L, st)
+        pr = O.Problem(ne, div, L, st)
+        job.cone_setup(r)
+        x, df = O.random_density(pr.nelx, 5), -rng.random(pr.nelx)
+        ref = pr.cone_filter(r).sensitivity(x, df, pr.volumes())
+        assert np.abs(job.sensitivity_filter(x, df) - ref).max() <= 1e-14 * np.abs(ref).max()
+        job.close()
+    x0, conn = box_mesh(8, (6, 5, 4), (6.0, 5.0, 4.0))
+    xd = distort(x0, conn, 8, 0.2, 3, (6.0, 5.0, 4.0))
+    job = icx.Job.from_mesh(ctx, 8, xd, conn, icx.SOLID3D)
+    pr = O.Problem.from_mesh(8, xd, conn, O.SOLID3D)
+    job.cone_setup(1.7)
+    x, df = O.random_density(pr.nelx, 6), -rng.random(pr.nelx)
+    ref = pr.cone_filter(1.7).sensitivity(x, df, pr.volumes())
+    assert np.abs(job.sensitivity_filter(x, df) - ref).max() <= 1e-13 * np.abs(ref).max()
+    job.close()
+
+
 def test_projection(ctx):
     job = icx.Job(ctx, 8, (5, 4, 3), (5.0, 4.0, 3.0), O.SOLID3D)
     x = np.linspace(0, 1, job.nelx)

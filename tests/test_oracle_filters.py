@@ -90,3 +90,38 @@ def test_helmholtz_element_matrix_closed_form():
     L = np.array([[4, -1, -2, -1], [-1, 4, -1, -2], [-2, -1, 4, -1], [-1, -2, -1, 4]]) / 6.0
     assert np.abs(K.reshape(4, 4) - (M + L)).max() < 1e-15
     assert np.allclose(N, 0.25, rtol=0, atol=1e-16)
+
+
+def test_sensitivity_filter_pins():
+    """Filter::sensitivityFilter (filter.hpp:72-100): with uniform densities and volumes it
+    is the density filter of dfds; multiplying x by a constant changes nothing; and on a
+    distorted 2D mesh it equals an O(n^2) evaluation with element centres and distances
+    computed here."""
+    pr = O.Problem(4, (9, 6, 1), (9.0, 6.0, 1.0), O.PLANE_STRESS)
+    f = pr.cone_filter(1.7)
+    rng = np.random.default_rng(4)
+    df = -rng.random(pr.nelx)
+    vol = pr.volumes()
+    x = np.full(pr.nelx, 0.4)
+    assert np.allclose(f.sensitivity(x, df, vol), f.apply(df), rtol=1e-14, atol=0)
+    x = rng.uniform(0.1, 1.0, pr.nelx)
+    s1 = f.sensitivity(x, df, vol)
+    assert np.allclose(f.sensitivity(3.0 * x, df, vol), s1, rtol=1e-14, atol=0)
+    # brute force on a distorted mesh
+    from meshgen import box_mesh, distort
+    x0, conn = box_mesh(4, (7, 5, 1), (7.0, 5.0, 1.0))
+    X = distort(x0, conn, 4, 0.2, 2, (7.0, 5.0))
+    pm = O.Problem.from_mesh(4, X, conn, O.PLANE_STRESS)
+    r = 1.6
+    fm = pm.cone_filter(r)
+    vm = pm.volumes()
+    xm = rng.uniform(0.1, 1.0, pm.nelx)
+    dm = -rng.random(pm.nelx)
+    Xn = np.asarray(X).reshape(-1, 3)
+    cen = Xn[np.asarray(conn).reshape(-1, 4)].mean(axis=1)[:, :2]
+    d = np.sqrt(((cen[:, None, :] - cen[None, :, :]) ** 2).sum(-1))
+    w = np.where(d <= r, r - d, 0.0)
+    aa = w.sum(axis=1)
+    bb = (w * xm[None, :] * dm[None, :] / (xm * vm)[None, :]).sum(axis=1)
+    ref = (xm * vm) / (xm * aa) * bb
+    assert np.allclose(fm.sensitivity(xm, dm, vm), ref, rtol=1e-12, atol=0)

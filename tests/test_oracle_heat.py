@@ -103,4 +103,4 @@ def test_heat_problem_volumes():
         pr.make_dof()
         h = np.array(L) / np.array(div)
         ref = h[0] * h[1] * (h[2] if ne == 8 else 1.0)
-        assert np.allclose(pr.volumes(), ref, rtol=1e-15, atol=0)
+        assert np.allclose(pr.volumes(), ref, rtol=1e-14, atol=0)
