@@ -25,7 +25,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
+#include <functional>
 #include <iostream>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -56,15 +58,48 @@ struct FEM {
   int neq = 0, numeq = 0, ndim = 0, nelx = 0, voigt = 0, dofnp = 0, numnp = 0;
 };
 
+// A std::vector<double> with the few Eigen::VectorXd calls the reference's drivers make
+// on Force / Value (dot, setZero, norm), so that e.g. dis.val.dot(force.fext) reads as
+// there.
+struct DVec : std::vector<double> {
+  using std::vector<double>::vector;
+  double dot(const std::vector<double> &o) const {
+    double s = 0.0;
+    for (size_t i = 0; i < size(); ++i) s += (*this)[i] * o[i];
+    return s;
+  }
+  void setZero() { std::fill(begin(), end(), 0.0); }
+  double norm() const { return std::sqrt(dot(*this)); }
+};
+
 struct Force {
   explicit Force(int numeq) : fext_org(numeq, 0.0), fext(numeq, 0.0), fint(numeq, 0.0), fres(numeq, 0.0) {}
-  std::vector<double> fext_org, fext, fint, fres;
+  DVec fext_org, fext, fint, fres;
 };
 
 struct Value {
   explicit Value(int numeq) : val(numeq, 0.0) {}
-  std::vector<double> val;
+  DVec val;
 };
+
+// Node and element records with the reference's field names (src/problem/base.hpp:37-106).
+// For a GPU job the element record is a handle: its design, volume, connectivity and von
+// Mises value mirror the job on the device (see LinearElastic below).
+struct Node {
+  int ID = 0;
+  int dof[6] = {0, 0, 0, 0, 0, 0};
+  double x[3] = {0, 0, 0};
+  double val[3] = {0, 0, 0};
+  double reaction[3] = {0, 0, 0};
+};
+struct Element {
+  int numdof = 0, ne = 0, ipmax = 0;
+  std::string eType;
+  int ID = 0;
+  std::vector<int> nodeID;
+  double volume = 0.0;
+};
+class StructuredModel;
 
 class Optimize {
  public:
@@ -104,6 +139,7 @@ class StructuredModel {
     numeq_ = tj.numeq;
     from_spec_ = true;
     finish(tj.ndim);
+    current() = this;
   }
 
   // MeshMDPA::setParameter / generate (src/io/mesh_mdpa.hpp:84-216): the mesh of an .mdpa
@@ -167,8 +203,10 @@ class StructuredModel {
     prob_.penalty = m.penalty;
     check(ctx(), icx_job_create_mesh(ctx(), &m, &job_), "icx_job_create_mesh");
     finish(ndim);
+    current() = this;
   }
   ~StructuredModel() {
+    if (current() == this) current() = nullptr;
     if (job_) icx_job_destroy(job_);
   }
   StructuredModel(const StructuredModel &) = delete;
@@ -216,6 +254,32 @@ class StructuredModel {
   void setDesignS(const std::vector<double> &d) {
     check(ctx(), icx_set_design(job_, d.data()), "icx_set_design");
     design_ = d;
+    design_dirty_ = false;
+  }
+  // one element's design, from an element handle (LinearElastic::setDesignS): kept on
+  // the host and sent with the next solve
+  void setElementDesign(int e, double d) {
+    if (design_.size() != (size_t)fem_.nelx) design_.assign(fem_.nelx, design_s0_);
+    design_[e] = d;
+    design_dirty_ = true;
+  }
+  void flushDesign() {
+    if (design_dirty_) setDesignS(design_);
+  }
+  // the element-mean von Mises stresses of the last solve (icx_mises, once per solve)
+  const std::vector<double> &mises() {
+    if (mises_solve_ != solves_) {
+      mises_.assign(fem_.nelx, 0.0);
+      check(ctx(), icx_mises(job_, mises_.data()), "icx_mises");
+      mises_solve_ = solves_;
+    }
+    return mises_;
+  }
+  void solved() { ++solves_; }
+  // the model the latest StructuredModel built (the reference's MMASolver(n, m) names no model)
+  static StructuredModel *&current() {
+    static StructuredModel *m = nullptr;
+    return m;
   }
   const std::vector<double> &design_s() const { return design_; }
   double design_s0() const { return design_s0_; }
@@ -249,14 +313,176 @@ class StructuredModel {
   bool heat_ = false;
   bool from_spec_ = false;  // box job built by box_job_from_toml (ranges, DOF map, loads done)
   int numeq_ = 0;
+  bool design_dirty_ = false;
+  long long solves_ = 0, mises_solve_ = -1;
+  std::vector<double> mises_;
   std::vector<double> volume_, design_;
 };
 
 // ---------------------------------------------------------------------------------
+// The reference's set-up objects over a GPU job, so that a driver written against
+// MeshIcarat / LinearElastic / Node / BCIcarat (src/io/mesh_icarat.hpp:69-136,
+// src/problem/elasticity.hpp:15-100, src/problem/base.hpp:37-106,
+// src/io/boundary_condition_icarat.hpp:68-396) keeps its set-up code:
+//   MeshIcarat<LinearElastic, Node> mesh(fem, config);  mesh.setParameter("structure");
+//   vector<LinearElastic> element (one per element, setParameter(config));
+//   vector<Node> node(fem.numnp);  mesh.generate(element, node);
+//   BCIcarat<LinearElastic, Node> BC(fem, element, node);  BC.input_dirich(config);
+//   fem.numeq = BC.makeDOF();  BC.input_neumann(config, force);
+// generate() builds the job from the job file (box mesh, material, Dirichlet ranges, DOF
+// map and tractions, by box_job_from_toml); the element records then hold handles: an
+// element's setDesignS goes to the job with the next solve, makeMisesStress reads the
+// device's von Mises value.  The analysis and filter classes below take (fem, element,
+// node, ...) as the reference's do.
+class LinearElastic : public Element {
+ public:
+  LinearElastic(int voigt, int ne_, int ipmax_, int numdof_) : voigt_(voigt) {
+    ne = ne_;
+    ipmax = ipmax_;
+    numdof = numdof_;
+  }
+  // Elastic::setParameter (src/problem/elasticity.cpp:19-32): [material] young, poisson,
+  // penalty, state and the initial design designs_0
+  void setParameter(const toml_lite::Table &config) {
+    const auto &young = config.at("material", "young").as_array();
+    young1_ = young.at(0).as_double();
+    young2_ = young.at(1).as_double();
+    poisson_ = config.at("material", "poisson").as_double();
+    pp_ = config.at("material", "penalty").as_double();
+    mattype_ = config.has("material", "state") ? config.at("material", "state").as_string() : "3D";
+    design_s_ = config.at("material", "designs_0").as_double();
+  }
+  void setDesignS(const double &d) {
+    design_s_ = d;
+    if (model_) model_->setElementDesign(ID, d);
+  }
+  // Elastic::makeMisesStress: the element-mean von Mises stress of the last solve
+  void makeMisesStress(FEM &, std::vector<Node> &) {
+    if (model_) mises_ = model_->mises()[ID];
+  }
+  const double &design_s() const { return design_s_; }
+  const double &young1() const { return young1_; }
+  const double &young2() const { return young2_; }
+  const double &poisson() const { return poisson_; }
+  const double &pp() const { return pp_; }
+  const double &mises() const { return mises_; }
+  const std::string &mattype() const { return mattype_; }
+  // SIMP modulus of the current design (src/problem/base.hpp SIMP)
+  double young() const {
+    if (young2_ - young1_ >= 0.0) return (1.0 - std::pow(design_s_, pp_)) * young1_ + std::pow(design_s_, pp_) * young2_;
+    return std::pow(1.0 - design_s_, pp_) * young1_ + (1.0 - std::pow(1.0 - design_s_, pp_)) * young2_;
+  }
+  StructuredModel &model() const {
+    if (!model_) StructuredModel::fatal("element not bound to a mesh: call MeshIcarat::generate first");
+    return *model_;
+  }
+  void bind(StructuredModel *m) { model_ = m; }
+
+ private:
+  int voigt_;
+  StructuredModel *model_ = nullptr;
+  std::string mattype_;
+  double design_s_ = 0.0, young1_ = 0.0, young2_ = 0.0, poisson_ = 0.0, pp_ = 0.0, mises_ = 0.0;
+};
+
+template <class E, class N>
+class MeshIcarat {
+ public:
+  MeshIcarat(FEM &fem, const toml_lite::Table &config) : fem_(fem), config_(config) {}
+  // MeshIcarat::setParameter: the [mesh] sizes (mesh_icarat.hpp:69-128)
+  void setParameter(const std::string &) {
+    const std::string type = config_.at("mesh", "type").as_string();
+    if (type != "quad4" && type != "hexa8")
+      StructuredModel::fatal("element type " + type + " is not supported by the GPU path (quad4, hexa8)");
+    ne_ = type == "quad4" ? 4 : 8;
+    const auto &div = config_.at("mesh", "division").as_array();
+    fem_.ndim = config_.at("mesh", "dimension").as_int();
+    const bool heat = config_.has("material", "type") && config_.at("material", "type").as_string() == "Heat";
+    fem_.dofnp = heat ? 1 : fem_.ndim;
+    fem_.voigt = heat ? fem_.ndim : (fem_.ndim == 2 ? 3 : 6);
+    const int d0 = div.at(0).as_int(), d1 = div.at(1).as_int(), d2 = fem_.ndim == 3 ? div.at(2).as_int() : 1;
+    fem_.nelx = d0 * d1 * d2;
+    fem_.numnp = (d0 + 1) * (d1 + 1) * (fem_.ndim == 3 ? d2 + 1 : 1);
+    fem_.neq = fem_.numnp * fem_.dofnp;
+  }
+  int numeleTypes() const { return 1; }
+  int ne(int) const { return ne_; }
+  int ipmax(int) const { return ne_; }
+  int nelx(int) const { return fem_.nelx; }
+  // MeshIcarat::generate: the job, and the element / node records bound to it
+  void generate(std::vector<E> &element, std::vector<N> &node) {
+    model_.reset(new StructuredModel(config_, fem_));
+    if ((int)element.size() != fem_.nelx || (int)node.size() != fem_.numnp)
+      StructuredModel::fatal("MeshIcarat::generate: element / node vectors do not match the mesh");
+    std::vector<double> x(3 * (size_t)fem_.numnp);
+    std::vector<int> conn((size_t)fem_.nelx * ne_);
+    check(model_->ctx(), icx_get_mesh(model_->job(), x.data(), conn.data()), "icx_get_mesh");
+    for (int n = 0; n < fem_.numnp; ++n) {
+      node[n].ID = n;
+      for (int d = 0; d < 3; ++d) node[n].x[d] = x[3 * (size_t)n + d];
+    }
+    const std::vector<double> &vol = model_->volume();
+    for (int e = 0; e < fem_.nelx; ++e) {
+      E &el = element[e];
+      el.ID = e;
+      el.eType = ne_ == 4 ? "quad4" : "hexa8";
+      el.nodeID.assign(conn.begin() + (size_t)e * ne_, conn.begin() + (size_t)(e + 1) * ne_);
+      el.volume = vol[e];
+      el.bind(model_.get());
+      el.setDesignS(el.design_s());
+    }
+  }
+  StructuredModel &model() { return *model_; }
+
+ private:
+  FEM &fem_;
+  const toml_lite::Table &config_;
+  int ne_ = 0;
+  std::unique_ptr<StructuredModel> model_;
+};
+
+template <class E, class N>
+class BCIcarat {
+ public:
+  BCIcarat(FEM &fem, std::vector<E> &element, std::vector<N> &node) : fem_(fem), element_(element), node_(node) {}
+  void input_dirich(const toml_lite::Table &config) { element_.front().model().input_dirich(config); }
+  // BCIcarat::makeDOF: numeq, and node.dof as countDOF numbers it
+  int makeDOF() {
+    StructuredModel &m = element_.front().model();
+    const int numeq = m.makeDOF();
+    std::vector<int> dof((size_t)fem_.numnp * fem_.dofnp);
+    check(m.ctx(), icx_get_dof_map(m.job(), dof.data()), "icx_get_dof_map");
+    for (int n = 0; n < fem_.numnp; ++n)
+      for (int c = 0; c < fem_.dofnp; ++c) node_[n].dof[c] = dof[(size_t)n * fem_.dofnp + c];
+    return numeq;
+  }
+  void input_neumann(const toml_lite::Table &config, Force &force) {
+    element_.front().model().input_neumann(config, force);
+  }
+
+ private:
+  FEM &fem_;
+  std::vector<E> &element_;
+  std::vector<N> &node_;
+};
+
+// ---------------------------------------------------------------------------------
+template <class E = void, class N = void>
 class LinearAnalysis {
  public:
   LinearAnalysis(FEM &fem, StructuredModel &model, Force &force, Value &value)
       : fem_(fem), model_(model), force_(force), value_(value) {}
+  // LinearAnalysis<E, N>(fem, element, node, force, value) (src/analysis/linear.hpp:18-19):
+  // the solve also writes node.val (divideValIntoNode)
+  LinearAnalysis(FEM &fem, std::vector<E> &element, std::vector<N> &node, Force &force, Value &value)
+      : fem_(fem), model_(element.front().model()), force_(force), value_(value) {
+    node_vals_ = [this, &node]() {
+      std::vector<double> v(3 * (size_t)fem_.numnp);
+      check(model_.ctx(), icx_get_node_values(model_.job(), v.data()), "icx_get_node_values");
+      for (int n = 0; n < fem_.numnp; ++n)
+        for (int c = 0; c < 3; ++c) node[n].val[c] = v[3 * (size_t)n + c];
+    };
+  }
 
   // LinearAnalysis::solve with the Solvers facade dispatch on the name
   void solve(const std::string &solvName, bool log) {
@@ -273,9 +499,12 @@ class LinearAnalysis {
     if (!force_.fext_org.empty()) check(model_.ctx(), icx_set_fext(model_.job(), force_.fext_org.data()), "icx_set_fext");
     force_.fext = force_.fext_org;
     value_.val.assign(fem_.numeq, 0.0);
+    model_.flushDesign();
     icx_solve_info info{};
     icx_status rc = icx_solve(model_.job(), form, tol, maxit_, value_.val.data(), &info);
     check(model_.ctx(), rc, "icx_solve");
+    model_.solved();
+    if (node_vals_) node_vals_();
     info_ = info;
     if (log) {
       std::cout << "#iterations:     " << info.iterations << std::endl;
@@ -295,6 +524,7 @@ class LinearAnalysis {
   double tol_ = 1.0e-10;   // src/analysis/solver/solvers.hpp:90
   int maxit_ = 10000;      // src/analysis/solver/solvers.hpp:91
   icx_solve_info info_{};
+  std::function<void()> node_vals_;
 };
 
 // the drivers' sensitivity(): dfds = -(dE/E) u^T Ke u, dgds = v / vscale
@@ -311,6 +541,28 @@ inline void output(StructuredModel &model, const Optimize &optim) {
   check(model.ctx(), icx_write_vtk(model.job(), path.c_str()), "icx_write_vtk");
 }
 
+// the reference driver's sensitivity(fem, element, node, optim, V0) and output(fem,
+// element, node, optim) (opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:173-235) on the
+// element handles' job: dfds = -(dE/E) u^T Ke u, dgds = v / V0; res<step>.vtk
+inline void sensitivity(FEM &, std::vector<LinearElastic> &element, std::vector<Node> &, Optimize &optim,
+                        double V0) {
+  sensitivity(element.front().model(), optim, V0);
+}
+inline void output(FEM &, std::vector<LinearElastic> &element, std::vector<Node> &, Optimize &optim) {
+  output(element.front().model(), optim);
+}
+// src/io/export_graph.hpp: addPlot(x, y, file, isFirst) and exportGraph(y, file), in the
+// reference's text format
+template <typename T1, typename T2>
+void addPlot(T1 xx, T2 yy, const std::string &filename, int isFirst) {
+  std::ofstream f(filename, isFirst == 1 ? std::ios::out : std::ios::app);
+  f << xx << " " << yy << std::endl;
+}
+template <typename T> void exportGraph(T &yy, const std::string &filename) {
+  std::ofstream f(filename, std::ios::out);
+  for (int i = 0; i < (int)yy.size(); i++) f << i << " " << yy[i] << std::endl;
+}
+
 // ---------------------------------------------------------------------------------
 inline std::vector<double> threshold_host(StructuredModel &m, const std::vector<double> &v, double beta, double T,
                                           bool derivative) {
@@ -320,10 +572,21 @@ inline std::vector<double> threshold_host(StructuredModel &m, const std::vector<
   return out;
 }
 
+template <class E = void, class N = void>
 class Filter {
  public:
   Filter(FEM &, StructuredModel &model, double radius) : model_(model) {
     check(model.ctx(), icx_cone_filter_setup(model.job(), radius), "icx_cone_filter_setup");
+  }
+  // Filter<E, N>(fem, element, node, r) (src/optimization/filter.hpp:20-21)
+  Filter(FEM &fem, std::vector<E> &element, std::vector<N> &, double radius)
+      : Filter(fem, element.front().model(), radius) {}
+  // Filter::sensitivityFilter (filter.hpp:72-100)
+  std::vector<double> sensitivityFilter(const std::vector<double> &design_s, const std::vector<double> &dfds) {
+    std::vector<double> out(dfds.size());
+    check(model_.ctx(), icx_sensitivity_filter(model_.job(), design_s.data(), dfds.data(), out.data()),
+          "icx_sensitivity_filter");
+    return out;
   }
   std::vector<double> densityFilter(const std::vector<double> &design_s, bool isSens) {
     std::vector<double> out(design_s.size());
@@ -341,11 +604,15 @@ class Filter {
   StructuredModel &model_;
 };
 
+template <class E = void, class N = void>
 class FilterHelmholtz {
  public:
   FilterHelmholtz(FEM &, StructuredModel &model, double radius) : model_(model) {
     check(model.ctx(), icx_helmholtz_setup(model.job(), radius), "icx_helmholtz_setup");
   }
+  // FilterHelmholtz<E, N>(fem, element, node, R) (src/optimization/filter_helm.hpp:23-30)
+  FilterHelmholtz(FEM &fem, std::vector<E> &element, std::vector<N> &, double radius)
+      : FilterHelmholtz(fem, element.front().model(), radius) {}
   std::vector<double> densityFilter(const std::vector<double> &design_s, const std::string &solvName, int log,
                                     bool sensFlag) {
     if (solvName != "Eigen_CG" && solvName != "HIP_PCG") {
@@ -416,6 +683,10 @@ class MMASolver {
  public:
   // m = 1, a = 0, c = 1e5 (the drivers' use, MMASolver.hpp:67); the optimiser history
   // lives with the model's job on the device
+  // MMASolver(n, m, a, c, d) as the reference constructs it (MMASolver.hpp:67): the model
+  // is the one the latest mesh generated
+  explicit MMASolver(int n, int m = 1, double a = 0.0, double c = 100000.0, double d = 0.0)
+      : MMASolver(current_model(), n, m, a, c, d) {}
   MMASolver(StructuredModel &model, int n, int m = 1, double a = 0.0, double c = 100000.0, double d = 0.0)
       : model_(model), n_(n) {
     if (m != 1 || a != 0.0 || c != 100000.0 || d != 0.0)
@@ -432,6 +703,10 @@ class MMASolver {
     std::cout << "MMA loop: " << steps << std::endl;
   }
   void Reset() { icx_mma_reset(model_.job()); }
+  static StructuredModel &current_model() {
+    if (!StructuredModel::current()) StructuredModel::fatal("MMASolver: no model (generate a mesh first)");
+    return *StructuredModel::current();
+  }
   void SetAsymptotes(double init, double decrease, double increase) {
     check(model_.ctx(), icx_mma_set_asymptotes(model_.job(), init, decrease, increase), "icx_mma_set_asymptotes");
   }

@@ -96,3 +96,27 @@ def test_eigen_ldlt_substitute_on_a_design_with_voids(ctx):
     assert np.linalg.norm(u - ud) / np.linalg.norm(ud) <= 1e-10
     assert abs(job.compliance() - cd) / cd <= 1e-10
     job.close()
+
+
+def test_reference_shaped_driver_b_equals_the_model_driver():
+    """Driver B written with the reference's constructors (MeshIcarat / BCIcarat /
+    LinearAnalysis<E, N>(fem, element, node, force, dis) / FilterHelmholtz<E, N>(fem,
+    element, node, R) / MMASolver(n, m) over LinearElastic and Node records) follows the
+    model-based C++ driver B step for step on the same job file (its addPlot output keeps
+    the reference's six significant digits)."""
+    build = os.path.join(ROOT, "icarat-contact_amd", "_build")
+    toml = os.path.join(ROOT, "icarat-contact_amd", "host", "examples", "mbb_fdm_q4.toml")
+    steps = 6
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
+        r1 = subprocess.run([os.path.join(build, "opt_mma_helmholtz_ref"), toml, str(steps)], cwd=d1,
+                            capture_output=True, text=True, timeout=300)
+        assert r1.returncode == 0, r1.stderr[-2000:]
+        r2 = subprocess.run([os.path.join(build, "opt_mma_helmholtz"), toml, "Eigen_LDLT", str(steps)], cwd=d2,
+                            capture_output=True, text=True, timeout=300)
+        assert r2.returncode == 0, r2.stderr[-2000:]
+        ref_shaped = np.loadtxt(os.path.join(d1, "function.csv"), ndmin=2)
+        model = np.loadtxt(os.path.join(d2, "function.csv"), delimiter=",", ndmin=2)
+        assert os.path.exists(os.path.join(d1, "res%d.vtk" % steps))
+        assert os.path.exists(os.path.join(d1, "objective.csv"))
+    assert ref_shaped.shape == (steps, 2) and np.array_equal(ref_shaped[:, 0], np.arange(1, steps + 1))
+    assert np.all(np.abs(ref_shaped[:, 1] - model[:, 1]) <= 5e-6 * np.abs(model[:, 1]))
