@@ -459,7 +459,8 @@ struct FusedWork {
 struct HelmWork {
   bool ready = false;
   double radius = 0;
-  HelmOp<double> op;
+  HelmOp<double> op;   // box jobs
+  double rh2 = 0;      // mesh jobs: (R / (2 sqrt 3))^2
   DBuf<double> invd, b, x, r, p, t, z;
 };
 struct ConeWork {
@@ -2491,14 +2492,22 @@ icx_status helm_setup(icx_job *j, double radius) {
   icx_ctx *c = j->ctx;
   HelmWork &w = j->hw;
   if (w.ready && w.radius == radius) return ICX_OK;
-  std::vector<double> K;
-  helm_K0(j->el, radius, K);
-  std::string why;
-  if (!build_helm_op(j->el, K, w.op, why)) return fail(c, ICX_ERR_NUMERIC, "%s", why.c_str());
+  j->gr_helm.reset();  // captured launches carry the operator of the previous radius
+  if (!j->mesh) {
+    std::vector<double> K;
+    helm_K0(j->el, radius, K);
+    std::string why;
+    if (!build_helm_op(j->el, K, w.op, why)) return fail(c, ICX_ERR_NUMERIC, "%s", why.c_str());
+  }
   const long long nn = j->g.nn;
   HIPCHK(c, w.invd.alloc(nn)); HIPCHK(c, w.b.alloc(nn)); HIPCHK(c, w.x.alloc(nn));
   HIPCHK(c, w.r.alloc(nn)); HIPCHK(c, w.p.alloc(nn)); HIPCHK(c, w.t.alloc(nn)); HIPCHK(c, w.z.alloc(nn));
-  launch_helm_invdiag(j->g, w.op.kdiag, w.invd.p, S(j));
+  if (j->mesh) {
+    w.rh2 = std::pow(radius / (2.0 * std::sqrt(3.0)), 2);  // filter_helm.hpp:204
+    launch_mesh_helm_invdiag(j->mop, w.rh2, w.invd.p, S(j));
+  } else {
+    launch_helm_invdiag(j->g, w.op.kdiag, w.invd.p, S(j));
+  }
   HIPCHK(c, hipStreamSynchronize(S(j)));
   w.ready = true;
   w.radius = radius;
@@ -2509,16 +2518,24 @@ icx_status helm_setup(icx_job *j, double radius) {
 icx_status helm_apply_dev(icx_job *j, int sens, double tol, const double *din, double *dout, int *iters) {
   HelmWork &w = j->hw;
   const double nq = j->el.nq;
-  const double rhs_scale = sens ? j->volume / (nq * nq) : j->volume / nq;
-  launch_helm_rhs(j->g, rhs_scale, din, w.b.p, S(j));
+  if (j->mesh) {
+    launch_mesh_helm_rhs(j->mop, sens ? nq : 1.0, din, w.b.p, S(j));
+  } else {
+    const double rhs_scale = sens ? j->volume / (nq * nq) : j->volume / nq;
+    launch_helm_rhs(j->g, rhs_scale, din, w.b.p, S(j));
+  }
   PcgWork64 pw;  // borrow the Helmholtz buffers
   pw.r.p = w.r.p; pw.p.p = w.p.p; pw.t.p = w.t.p; pw.z.p = w.z.p;
-  auto mv = [&](const double *p, double *t) { launch_helm_matvec(j->g, w.op, p, t, j->d_state.p, S(j)); };
+  auto mv = [&](const double *p, double *t) {
+    if (j->mesh) launch_mesh_helm_matvec(j->mop, w.rh2, p, t, j->d_state.p, S(j));
+    else launch_helm_matvec(j->g, w.op, p, t, j->d_state.p, S(j));
+  };
   icx_solve_info info{};
   icx_status rc = pcg_plain(j, j->g.nn, w.b.p, w.x.p, w.invd.p, pw, tol, 10000, mv, info, j->gr_helm);
   pw.r.p = pw.p.p = pw.t.p = pw.z.p = nullptr;  // not owned
   if (rc && rc != ICX_ERR_NOCONV) return rc;
-  launch_helm_to_element(j->g, sens ? 1.0 : 1.0 / nq, w.x.p, dout, S(j));
+  if (j->mesh) launch_mesh_helm_to_element(j->mop, sens ? 1.0 : nq, w.x.p, dout, S(j));
+  else launch_helm_to_element(j->g, sens ? 1.0 : 1.0 / nq, w.x.p, dout, S(j));
   if (iters) *iters = info.iterations;
   return ICX_OK;
 }
@@ -2563,7 +2580,6 @@ icx_status icx_sensitivity_filter(icx_job *j, const double *design, const double
 }
 
 icx_status icx_helmholtz_setup(icx_job *j, double radius) {
-  if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "the Helmholtz filter");
   if (!j || !(radius > 0.0)) return ICX_ERR_ARG;
   return helm_setup(j, radius);
 }
@@ -2922,8 +2938,8 @@ icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0)
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
   if (par->driver < 0 || par->driver > 2) return fail(j->ctx, ICX_ERR_ARG, "driver must be 0 (OC), 1 (MMA) or 2 (GCMMA)");
-  if (j->mesh && (par->filter == ICX_FILTER_HELMHOLTZ || par->form != ICX_SOLVE_PLAIN))
-    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: the design loop takes the cone filter and ICX_SOLVE_PLAIN");
+  if (j->mesh && par->form != ICX_SOLVE_PLAIN)
+    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: the design loop takes ICX_SOLVE_PLAIN");
   icx_ctx *c = j->ctx;
   OptWork &o = j->opt;
   o.par = *par;

@@ -240,6 +240,15 @@ void launch_mesh_sensitivity(const MeshOp &m, double y1, double y2, double p, co
 void launch_mesh_energy(const MeshOp &m, const double *E, const double *u, double *energy, hipStream_t s);
 // element von Mises stress of the Gauss-point mean strain (u: node values, SoA)
 void launch_mesh_mises(const MeshOp &m, const double *E, const double *u, double *out, hipStream_t s);
+// Helmholtz filter on a mesh job, one scalar per node (rh2 = (R / (2 sqrt 3))^2):
+// y = K x (both passes exit at once when st->done is set; st may be null), the Jacobi
+// inverse of diag K (1 on a zero diagonal), the right-hand side sum_e rho_e int N / div,
+// and the element values sum_ip N(ip) . phi_e / div
+void launch_mesh_helm_matvec(const MeshOp &m, double rh2, const double *x, double *y, const SolverState *st,
+                             hipStream_t s);
+void launch_mesh_helm_invdiag(const MeshOp &m, double rh2, double *invd, hipStream_t s);
+void launch_mesh_helm_rhs(const MeshOp &m, double div, const double *rho, double *f, hipStream_t s);
+void launch_mesh_helm_to_element(const MeshOp &m, double div, const double *phi, double *out, hipStream_t s);
 
 // ---- single-reduction PCG iteration, FP32 hexa8 (icx_sr.hip) -------------------------
 // One kernel per iteration of the mixed form's Jacobi-PCG: launch k takes the step of

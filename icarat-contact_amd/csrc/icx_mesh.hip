@@ -38,6 +38,13 @@ struct GaussPoint {
       if (e != d) v *= 1.0 + csign(a, e) * r[e];
     return v;
   }
+  // N_a = prod_d (1 + s_ad r_d) / 2^NC
+  __device__ __forceinline__ double N(int a) const {
+    double v = NC == 3 ? 0.125 : 0.25;
+#pragma unroll
+    for (int d = 0; d < NC; ++d) v *= 1.0 + csign(a, d) * r[d];
+    return v;
+  }
   // dN_a / dX_k
   __device__ __forceinline__ double dx(int a, int k) const {
     double v = 0.0;
@@ -331,7 +338,132 @@ __global__ void __launch_bounds__(256) k_mesh_mises(MeshOp m, const double *E, c
   }
 }
 
+// ---- Helmholtz filter on the mesh (FilterHelmholtz, src/optimization/filter_helm.hpp:75-256):
+// one scalar per node, element matrix sum_ip (N N^T + rh2 dN^T dN) det J with
+// rh2 = (R / (2 sqrt 3))^2, never stored.  The outputs go through ye in node order and
+// the scalar gather, as the elastic operator's.
+
+// y_e = Ke_e x_e
+template <int NE, int NC>
+__global__ void __launch_bounds__(256) k_mesh_helm_element(MeshOp m, double rh2, const double *x, double *ye,
+                                                           const SolverState *st) {
+  if (st && st->done) return;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
+    double X[NE][NC], v[NE], f[NE];
+    gather_coords<NE, NC>(m, e, X);
+#pragma unroll
+    for (int a = 0; a < NE; ++a) {
+      v[a] = x[m.conn[a * m.nelx + e]];
+      f[a] = 0.0;
+    }
+#pragma unroll 1
+    for (int ip = 0; ip < NE; ++ip) {
+      const GaussPoint<NE, NC> gp(ip, X);
+      double nv = 0.0, gv[NC];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) gv[k] = 0.0;
+#pragma unroll
+      for (int a = 0; a < NE; ++a) {
+        nv = fma_t(gp.N(a), v[a], nv);
+#pragma unroll
+        for (int k = 0; k < NC; ++k) gv[k] = fma_t(gp.dx(a, k), v[a], gv[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < NC; ++k) gv[k] *= rh2;
+#pragma unroll
+      for (int a = 0; a < NE; ++a) {
+        double t = gp.N(a) * nv;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) t = fma_t(gp.dx(a, k), gv[k], t);
+        f[a] = fma_t(t, gp.det, f[a]);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NE; ++a) ye[m.slot[e * NE + a]] = f[a];
+  }
+}
+
+// diag Ke_e = sum_ip (N_a^2 + rh2 |dN_a|^2) det J
+template <int NE, int NC>
+__global__ void __launch_bounds__(256) k_mesh_helm_diag(MeshOp m, double rh2, double *ye) {
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
+    double X[NE][NC], f[NE];
+    gather_coords<NE, NC>(m, e, X);
+#pragma unroll
+    for (int a = 0; a < NE; ++a) f[a] = 0.0;
+#pragma unroll 1
+    for (int ip = 0; ip < NE; ++ip) {
+      const GaussPoint<NE, NC> gp(ip, X);
+#pragma unroll
+      for (int a = 0; a < NE; ++a) {
+        double g2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) g2 = fma_t(gp.dx(a, k), gp.dx(a, k), g2);
+        const double n = gp.N(a);
+        f[a] = fma_t(fma_t(rh2, g2, n * n), gp.det, f[a]);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NE; ++a) ye[m.slot[e * NE + a]] = f[a];
+  }
+}
+
+// right-hand side f_n = sum over the elements of n of rho_e int N_a det J (divided by
+// ipmax in the sensitivity mode: div = NE, else 1), filter_helm.hpp:216-219
+template <int NE, int NC>
+__global__ void __launch_bounds__(256) k_mesh_helm_rhs(MeshOp m, double div, const double *rho, double *ye) {
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
+    double X[NE][NC], f[NE];
+    gather_coords<NE, NC>(m, e, X);
+#pragma unroll
+    for (int a = 0; a < NE; ++a) f[a] = 0.0;
+#pragma unroll 1
+    for (int ip = 0; ip < NE; ++ip) {
+      const GaussPoint<NE, NC> gp(ip, X);
+#pragma unroll
+      for (int a = 0; a < NE; ++a) f[a] = fma_t(gp.N(a), gp.det, f[a]);
+    }
+    const double r = rho[e];
+#pragma unroll
+    for (int a = 0; a < NE; ++a) ye[m.slot[e * NE + a]] = (r * f[a]) / div;
+  }
+}
+
+// divideToElement (filter_helm.hpp:224-255): out_e = sum_ip N(ip) . phi_e, divided by
+// ipmax (div = NE) in the forward mode
+template <int NE, int NC>
+__global__ void __launch_bounds__(256) k_mesh_helm_to_element(MeshOp m, double div, const double *phi, double *out) {
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
+    double v[NE];
+#pragma unroll
+    for (int a = 0; a < NE; ++a) v[a] = phi[m.conn[a * m.nelx + e]];
+    const double g = 0.57735026918962576451;
+    double s = 0.0;
+#pragma unroll 1
+    for (int ip = 0; ip < NE; ++ip) {
+      double t = 0.0;
+#pragma unroll
+      for (int a = 0; a < NE; ++a) {
+        double n = NC == 3 ? 0.125 : 0.25;
+#pragma unroll
+        for (int d = 0; d < NC; ++d) n *= 1.0 + csign(a, d) * csign(ip, d) * g;
+        t = fma_t(n, v[a], t);
+      }
+      s += t;
+    }
+    out[e] = s / div;
+  }
+}
+
 int mesh_blocks(long long n) { return ew_blocks(n, 1, 4096); }
+
+template <int NE, int NC, class K, class... A>
+void mesh_scalar_pass(const MeshOp &m, K kern, double *y, bool invert, const SolverState *st, hipStream_t s,
+                      const LaunchEvents *ev, A... args) {
+  launch_ev(kern, mesh_blocks(m.nelx), 256, s, ev, m, args...);
+  hipLaunchKernelGGL((k_mesh_gather<NE, 1>), dim3(mesh_blocks(m.nn)), dim3(256), 0, s, m, (const double *)m.ye,
+                     (const uint8_t *)nullptr, y, st, (int)invert);
+}
 
 }  // namespace
 
@@ -390,6 +522,33 @@ void launch_mesh_mises(const MeshOp &m, const double *E, const double *u, double
     hipLaunchKernelGGL((k_mesh_mises<8, 3>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, u, out);
   else
     hipLaunchKernelGGL((k_mesh_mises<4, 2>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, u, out);
+  ICX_CHECK_LAUNCH();
+}
+
+void launch_mesh_helm_matvec(const MeshOp &m, double rh2, const double *x, double *y, const SolverState *st,
+                             hipStream_t s) {
+  if (m.ne == 8) mesh_scalar_pass<8, 3>(m, k_mesh_helm_element<8, 3>, y, false, st, s, nullptr, rh2, x, m.ye, st);
+  else mesh_scalar_pass<4, 2>(m, k_mesh_helm_element<4, 2>, y, false, st, s, nullptr, rh2, x, m.ye, st);
+  ICX_CHECK_LAUNCH();
+}
+
+void launch_mesh_helm_invdiag(const MeshOp &m, double rh2, double *invd, hipStream_t s) {
+  if (m.ne == 8) mesh_scalar_pass<8, 3>(m, k_mesh_helm_diag<8, 3>, invd, true, nullptr, s, nullptr, rh2, m.ye);
+  else mesh_scalar_pass<4, 2>(m, k_mesh_helm_diag<4, 2>, invd, true, nullptr, s, nullptr, rh2, m.ye);
+  ICX_CHECK_LAUNCH();
+}
+
+void launch_mesh_helm_rhs(const MeshOp &m, double div, const double *rho, double *f, hipStream_t s) {
+  if (m.ne == 8) mesh_scalar_pass<8, 3>(m, k_mesh_helm_rhs<8, 3>, f, false, nullptr, s, nullptr, div, rho, m.ye);
+  else mesh_scalar_pass<4, 2>(m, k_mesh_helm_rhs<4, 2>, f, false, nullptr, s, nullptr, div, rho, m.ye);
+  ICX_CHECK_LAUNCH();
+}
+
+void launch_mesh_helm_to_element(const MeshOp &m, double div, const double *phi, double *out, hipStream_t s) {
+  if (m.ne == 8)
+    hipLaunchKernelGGL((k_mesh_helm_to_element<8, 3>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, div, phi, out);
+  else
+    hipLaunchKernelGGL((k_mesh_helm_to_element<4, 2>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, div, phi, out);
   ICX_CHECK_LAUNCH();
 }
 

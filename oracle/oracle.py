@@ -393,6 +393,7 @@ class ConeFilter:
 class Helmholtz:
     def __init__(self, prob, radius):
         self.n = prob.nelx
+        self.prob = prob  # icxo_helm keeps a pointer to prob.conn
         self.h = lib().icxo_helm_create(prob.ne, prob.ndim, prob.numnp, prob.nelx, prob.x,
                                         prob.conn, float(radius))
 

@@ -298,6 +298,21 @@ def _helmholtz_case(ctx, ne, div, R, st, L):
     assert abs(it - ito) <= 2
 
 
+def test_helmholtz_new_radius_on_one_job(ctx):
+    """A second radius on the same job: the filter solves replayed from HIP graphs must
+    take the new operator (the captured launches of the first radius are dropped)."""
+    ne, div = 8, (10, 6, 5)
+    L = tuple(float(d) for d in div)
+    job = icx.Job(ctx, ne, div, L, O.SOLID3D)
+    pr = O.Problem(ne, div, L, O.SOLID3D)
+    x = O.random_density(pr.nelx, 12)
+    for R in (1.5, 3.0, 1.5):
+        job.helmholtz_setup(R)
+        g, _ = job.helmholtz(x, tol=1e-13)
+        o, _ = pr.helmholtz(R).apply(x, tol=1e-13)
+        assert np.abs(g - o).max() <= 1e-10 * np.abs(o).max(), R
+
+
 def test_oc_update_matches(ctx):
     job = icx.Job(ctx, 8, (12, 6, 6), (12.0, 6.0, 6.0), O.SOLID3D)
     pr = O.Problem(8, (12, 6, 6), (12.0, 6.0, 6.0), O.SOLID3D)
