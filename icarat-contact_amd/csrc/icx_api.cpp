@@ -52,6 +52,7 @@ struct icx_ctx {
   bool pipeline = true;            // keep one chunk queued ahead of the state read
   int tile_depth = 0;              // hexa8 operator z depth per block, 0 = automatic
   bool heat_dense = false;         // heat jobs: dense element rows even where Haar-diagonal
+  bool mma_host_dual = false;      // MMA / GCMMA dual: Newton loop on the host
   // profile window (icx_profile.h)
   std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
   int prof_kernel = 0;
@@ -569,6 +570,7 @@ struct icx_job {
   DBuf<SolverState> d_state;
   DBuf<double> d_scalars;
   DBuf<OcBisect> d_oc;      // device-side OC bisection state
+  DBuf<MmaDual> d_mdual;    // device-side MMA dual loop state
   bool design_set = false;
   bool u_valid = false;
   PcgWork64 pw;
@@ -893,6 +895,10 @@ icx_status icx_set_option(icx_ctx *ctx, int option, long long value) {
   }
   if (option == ICX_OPT_PIPELINE) {
     ctx->pipeline = value != 0;
+    return ICX_OK;
+  }
+  if (option == ICX_OPT_MMA_HOST_DUAL) {
+    ctx->mma_host_dual = value != 0;
     return ICX_OK;
   }
   if (option == ICX_OPT_HEAT_DENSE) {
@@ -2700,7 +2706,8 @@ icx_status mma_alloc(icx_job *j, MMAWork &m) {
 // SolveDIP with m = 1, a = 0 (MMASolver.cpp:102-154, 184-398; GCMMASolver.cpp:135-197,
 // 227-429): the scalar dual Newton iteration on the host; x(lambda) into dx and its two
 // sums on the device.  b is the constraint's constant, cc the y penalty c.
-icx_status mma_dual(icx_job *j, const MMAVecs &v, double *dx, double b, double cc, double epsimin, int *steps) {
+icx_status mma_dual_host(icx_job *j, const MMAVecs &v, double *dx, double b, double cc, double epsimin,
+                         int *steps) {
   const long long n = j->g.ne;
   const double a = 0.0;
   icx_status rc = ICX_OK;
@@ -2754,6 +2761,33 @@ icx_status mma_dual(icx_job *j, const MMAVecs &v, double *dx, double b, double c
     epsi = epsi * 0.1;
   }
   if (steps) *steps = total;
+  return ICX_OK;
+}
+
+// The same loop on the device (MmaDual): after the first evaluation, rounds of
+// (advance, evaluate) pairs with one read of the state per round; pairs queued past the
+// end leave at once, so x holds x(lambda) of the last Newton step, as in the host loop.
+constexpr int MMA_DUAL_ROUND = 16;
+icx_status mma_dual(icx_job *j, const MMAVecs &v, double *dx, double b, double cc, double epsimin, int *steps) {
+  icx_ctx *c = j->ctx;
+  if (c->mma_host_dual) return mma_dual_host(j, v, dx, b, cc, epsimin, steps);
+  const long long n = j->g.ne;
+  HIPCHK(c, j->d_mdual.alloc(1));
+  MmaDual *d = j->d_mdual.p;
+  launch_mma_dual_init(d, b, cc, epsimin, S(j));
+  launch_mma_dual(n, 0.0, v, dx, j->d_scalars.p, j->red(), S(j), d);
+  static_assert(sizeof(MmaDual) <= 64 * sizeof(double), "pinned scalar buffer");
+  MmaDual *h = reinterpret_cast<MmaDual *>(c->h_scalar);  // pinned, 64 doubles
+  for (;;) {
+    for (int r = 0; r < MMA_DUAL_ROUND; ++r) {
+      launch_mma_dual_advance(d, j->d_scalars.p, S(j));
+      launch_mma_dual(n, 0.0, v, dx, j->d_scalars.p, j->red(), S(j), d);
+    }
+    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(MmaDual), hipMemcpyDeviceToHost, S(j)));
+    HIPCHK(c, hipStreamSynchronize(S(j)));
+    if (h->done) break;
+  }
+  if (steps) *steps = h->total;
   return ICX_OK;
 }
 

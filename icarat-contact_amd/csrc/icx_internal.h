@@ -531,8 +531,21 @@ void launch_mma_gensub(long long n, int iter, int asymp, int clamp, const MMAVec
 // GCMMA approximations at xmma and raacof: out[0..2]
 void launch_gcmma_approx(long long n, const MMAVecs &m, const double *xmma, const double *xval, double xmami,
                          double *out, Reduce red, hipStream_t s);
-// x(lambda) and sums: out[0] = sum p1/(upp-x) + q1/(x-low), out[1] = sum PQ^2 df2
+// x(lambda) and sums: out[0] = sum p1/(upp-x) + q1/(x-low), out[1] = sum PQ^2 df2.
+// With d given, lambda is d->lam and the kernel leaves at once when d->done is set.
 void launch_mma_dual(long long n, double lam, const MMAVecs &m, double *x, double *out, Reduce red,
-                     hipStream_t s);
+                     hipStream_t s, const struct MmaDual *d = nullptr);
+
+// The dual's interior-point Newton loop (MMASolver::DualSolve with m = 1, a = 0,
+// MMASolver.cpp:400-512) kept on the device: launch_mma_dual evaluates x(lambda) and the
+// two sums at d->lam, launch_mma_dual_advance takes them and runs the loop's control flow
+// up to its next Newton step (or to the end: done), so the host queues (advance, eval)
+// pairs in rounds and reads the state once per round instead of once per Newton step.
+struct MmaDual {
+  double lam, mu, epsi, err, b, cc, epsimin, y, z;
+  int outloop, loop, total, in_inner, need_err, done;
+};
+void launch_mma_dual_init(MmaDual *d, double b, double cc, double epsimin, hipStream_t s);
+void launch_mma_dual_advance(MmaDual *d, const double *sums, hipStream_t s);
 
 }  // namespace icx

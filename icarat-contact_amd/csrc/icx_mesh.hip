@@ -248,7 +248,6 @@ __global__ void __launch_bounds__(256) k_mesh_elem_diag(MeshOp m, const double *
 template <int NE, int NC>
 __global__ void __launch_bounds__(256) k_mesh_gather(MeshOp m, const double *ye, const uint8_t *mask, double *y,
                                                      const SolverState *st, int invert) {
-  constexpr int ND = NE * NC;
   if (st && st->done) return;
   for (long long n = blockIdx.x * 256LL + threadIdx.x; n < m.nn; n += (long long)gridDim.x * 256) {
     double acc[NC];

@@ -1290,8 +1290,13 @@ void launch_gcmma_approx(long long n, const MMAVecs &m, const double *xmma, cons
   launch_finalize(red, nb, 3, out, s);
 }
 
-__global__ void __launch_bounds__(256) k_mma_dual(long long n, double lam, MMAVecs m, double *x, Reduce red) {
+__global__ void __launch_bounds__(256) k_mma_dual(long long n, double lam, MMAVecs m, double *x, Reduce red,
+                                                  const MmaDual *d) {
   __shared__ double scr[4];
+  if (d) {
+    if (d->done) return;
+    lam = d->lam;
+  }
   double s1 = 0.0, s2 = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
@@ -1317,10 +1322,102 @@ __global__ void __launch_bounds__(256) k_mma_dual(long long n, double lam, MMAVe
   }
 }
 void launch_mma_dual(long long n, double lam, const MMAVecs &m, double *x, double *out, Reduce red,
-                     hipStream_t s) {
+                     hipStream_t s, const MmaDual *d) {
   const int nb = ew_blocks(n, 4, RED_BLOCKS);
-  hipLaunchKernelGGL(k_mma_dual, dim3(nb), dim3(256), 0, s, n, lam, m, x, red);
+  hipLaunchKernelGGL(k_mma_dual, dim3(nb), dim3(256), 0, s, n, lam, m, x, red, d);
   launch_finalize(red, nb, 2, out, s);
+}
+
+// The dual loop's scalar steps, in the reference's operation order (this library is
+// built without contraction), so that every value is the one the host loop computes
+// from the same sums.  xyz's prologue: lambda >= 0, y and z of lambda.
+__device__ __forceinline__ void mma_dual_yz(MmaDual &s) {
+  const double a = 0.0;
+  if (s.lam < 0.0) s.lam = 0;
+  const double ly = s.lam - s.cc;
+  s.y = 0.0 < ly ? ly : 0.0;
+  const double lz = 10.0 * (s.lam * a - 1.0);
+  s.z = 0.0 < lz ? lz : 0.0;
+}
+__global__ void k_mma_dual_init(MmaDual *d, double b, double cc, double epsimin) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  MmaDual s;
+  s.lam = cc / 2.0;
+  s.mu = 1.0;
+  s.epsi = 1.0;
+  s.err = 1.0;
+  s.b = b;
+  s.cc = cc;
+  s.epsimin = epsimin;
+  s.outloop = s.loop = s.total = 0;
+  s.in_inner = s.need_err = s.done = 0;
+  mma_dual_yz(s);
+  *d = s;
+}
+void launch_mma_dual_init(MmaDual *d, double b, double cc, double epsimin, hipStream_t s) {
+  hipLaunchKernelGGL(k_mma_dual_init, dim3(1), dim3(64), 0, s, d, b, cc, epsimin);
+  ICX_CHECK_LAUNCH();
+}
+// Takes the sums at d->lam: the residual of the last Newton step if one is pending, then
+// the loops' tests (inner: err > 0.9 epsi and loop < 100; outer: epsi > epsimin and
+// outloop < 100, epsi /= 10 after each inner loop) until the next Newton step, which it
+// takes, or the end.  Every pass of the for loop either takes a step and leaves, or
+// ends an inner loop, or starts one of at most 100 outer loops, so it ends.
+__global__ void k_mma_dual_advance(MmaDual *d, const double *sums) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || d->done) return;
+  MmaDual s = *d;
+  const double a = 0.0;
+  const double S1 = sums[0], S2 = sums[1];
+  if (s.need_err) {
+    const double r0 = -s.b - a * s.z - s.y + s.mu + S1;
+    const double r1 = s.mu * s.lam - s.epsi;
+    const double e0 = fabs(r0), e1 = fabs(r1);
+    s.err = e0 < e1 ? e1 : e0;
+    s.need_err = 0;
+  }
+  for (;;) {
+    if (s.in_inner) {
+      if (s.err > 0.9 * s.epsi && s.loop < 100) {
+        s.loop++;
+        s.total++;
+        double grad = -s.b - a * s.z - s.y + S1;
+        grad = -1.0 * grad - s.epsi / s.lam;
+        if (s.lam < 0.0) s.lam = 0.0;
+        double hess = S2;
+        if (s.lam > s.cc) hess += -1.0;
+        hess += -s.mu / s.lam;
+        double corr = 1e-4 * hess / 1;
+        if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+        hess += corr;
+        const double s0 = grad / hess;
+        const double s1 = -s.mu + s.epsi / s.lam - s0 * s.mu / s.lam;
+        double theta = 1.005;
+        if (theta < -1.01 * s0 / s.lam) theta = -1.01 * s0 / s.lam;
+        if (theta < -1.01 * s1 / s.mu) theta = -1.01 * s1 / s.mu;
+        theta = 1.0 / theta;
+        s.lam = s.lam + theta * s0;
+        s.mu = s.mu + theta * s1;
+        mma_dual_yz(s);
+        s.need_err = 1;
+        break;
+      }
+      s.in_inner = 0;
+      s.epsi = s.epsi * 0.1;
+    }
+    if (s.epsi > s.epsimin && s.outloop < 100) {
+      s.outloop++;
+      s.loop = 0;
+      s.in_inner = 1;
+      continue;
+    }
+    s.done = 1;
+    break;
+  }
+  *d = s;
+}
+void launch_mma_dual_advance(MmaDual *d, const double *sums, hipStream_t s) {
+  hipLaunchKernelGGL(k_mma_dual_advance, dim3(1), dim3(64), 0, s, d, sums);
+  ICX_CHECK_LAUNCH();
 }
 
 }  // namespace icx

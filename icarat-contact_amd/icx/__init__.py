@@ -35,7 +35,7 @@ EXPORTS = [
     "icx_submit", "icx_poll", "icx_job_last_info", "icx_cancel", "icx_solve_batch", "icx_job_footprint",
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
 ]
-OPT_GRAPHS, OPT_TILE_DEPTH, OPT_PIPELINE, OPT_SINGLE_REDUCTION, OPT_HEAT_DENSE = 1, 2, 3, 4, 5
+OPT_GRAPHS, OPT_TILE_DEPTH, OPT_PIPELINE, OPT_SINGLE_REDUCTION, OPT_HEAT_DENSE, OPT_MMA_HOST_DUAL = 1, 2, 3, 4, 5, 6
 PROF_MATVEC64, PROF_FUSED_A, PROF_FUSED_B = 1, 2, 3
 
 

@@ -42,9 +42,12 @@ icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
  * ICX_OPT_HEAT_DENSE (default 0): 1 makes heat jobs apply their element matrices as dense
  *   rows even where both are diagonal in the element's Haar basis (diagonal conductivity
  *   tensors), the path taken otherwise; for checking one operator against the other.
+ * ICX_OPT_MMA_HOST_DUAL (default 0): 1 runs the Newton loop of the MMA / GCMMA dual on
+ *   the host, reading the two sums of every step, instead of on the device with one read
+ *   of its state per 16 steps; the same arithmetic, for checking one against the other.
  */
 enum { ICX_OPT_GRAPHS = 1, ICX_OPT_TILE_DEPTH = 2, ICX_OPT_PIPELINE = 3, ICX_OPT_SINGLE_REDUCTION = 4,
-       ICX_OPT_HEAT_DENSE = 5 };
+       ICX_OPT_HEAT_DENSE = 5, ICX_OPT_MMA_HOST_DUAL = 6 };
 icx_status icx_set_option(icx_ctx *ctx, int option, long long value);
 
 #ifdef __cplusplus

@@ -350,6 +350,39 @@ def test_mma_updates_match(ctx):
         xg = xo.copy()   # teacher forcing: continue both from the oracle's iterate
 
 
+def test_mma_dual_on_device_equals_host_loop(ctx):
+    """The dual's Newton loop run on the device (one read of its state per 16 steps)
+    against the same loop run on the host (one read per step): the same Newton step
+    count and the same bits, for MMA updates and for driver B's device design loop."""
+    n = 300
+    rng = np.random.default_rng(17)
+    c = rng.uniform(0.5, 2.0, n)
+    runs = {}
+    for host in (1, 0):
+        ctx.set_option(icx.OPT_MMA_HOST_DUAL, host)
+        try:
+            job = icx.Job(ctx, 8, (10, 6, 5), (10.0, 6.0, 5.0), O.SOLID3D)
+            x = np.random.default_rng(7).uniform(0.2, 0.6, n)
+            hist = []
+            for it in range(5):
+                df = -c / x ** 2
+                g = x.sum() / (0.4 * n) - 1.0
+                x, steps = job.mma_update(x, df, g, np.full(n, 1.0 / (0.4 * n)), 1e-4, 1.0)
+                hist.append((x.copy(), steps))
+            drv = icx.cantilever_2d(ctx, 30, 10)
+            drv.opt_init(np.full(drv.nelx, 0.4), driver=1, filter=icx.FILTER_HELMHOLTZ, radius=2.0,
+                         form=icx.SOLVE_PLAIN, tol=1e-12, maxit=50000, helm_tol=1e-12)
+            objs = [drv.opt_step()["objective"] for _ in range(4)]
+            runs[host] = (hist, objs, drv.opt_design())
+        finally:
+            ctx.set_option(icx.OPT_MMA_HOST_DUAL, 0)
+    for (xh, sh), (xd, sd) in zip(runs[1][0], runs[0][0]):
+        assert sh == sd and sh > 0
+        assert np.array_equal(xh, xd)
+    assert runs[1][1] == runs[0][1]
+    assert np.array_equal(runs[1][2], runs[0][2])
+
+
 @pytest.mark.parametrize("name", ["q4_cantilever_24x8.npz", "hex8_cantilever_10x4x4.npz"])
 def test_gpu_matches_golden(ctx, name):
     import os
