@@ -530,6 +530,17 @@ def mesh_solve(args):
     print("mesh profile window: %.6f s over %d launches, %d matvecs" % (s, nl, pi["matvecs"]), file=sys.stderr,
           flush=True)
     el_us = s / max(nl, 1) * 1e6
+    # the mixed form on the same job: FP32 Gauss-point element pass, FP64 replacement
+    job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=200000, want_u=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, mi = job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=200000, want_u=False)
+    ctx.synchronize()
+    mixed_wall = time.perf_counter() - t0
+    ctx.profile_begin(icx.PROF_FUSED_A)
+    job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=200000, want_u=False)
+    s32, nl32 = ctx.profile_end()
+    el32_us = s32 / max(nl32, 1) * 1e6
     # FilterHelmholtz on the same mesh (R = 2 element widths, the reference's 1e-10): the
     # scalar operator integrated per element in the same way
     job.helmholtz_setup(2.0)
@@ -560,7 +571,12 @@ def mesh_solve(args):
                        "element_pass_bytes_per_launch": bytes_el,
                        "element_pass_GBps_algorithmic": bytes_el / (el_us * 1e-6) / 1e9 if el_us else None,
                        "element_pass_flops_per_launch": flops_el,
-                       "helmholtz_filter": {"R": 2.0, "tol": 1e-10, "iterations": h_it, "ms": h_ms}},
+                       "helmholtz_filter": {"R": 2.0, "tol": 1e-10, "iterations": h_it, "ms": h_ms},
+                       "mixed": {"ms_per_solve": mixed_wall * 1e3, "iterations": mi["iterations"],
+                                 "refinements": mi["refinements"], "relres": mi["relres"],
+                                 "dof_matvec_per_s": N * mi["matvecs"] / mixed_wall,
+                                 "element_pass_fp32_us": el32_us,
+                                 "speedup_vs_plain": wall / mixed_wall}},
             "roofline": {"bound": "fp64", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s",
                          "frac": tf / 78.6 if tf else None, "traffic": None,
                          "kernel": "k_mesh_element<8,3> (Gauss-point element pass)"}}

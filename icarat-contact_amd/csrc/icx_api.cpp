@@ -1676,10 +1676,13 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
 // kept.  Convergence is declared on the FP64 residual only.
 // Mixed form, in pieces shared by icx_solve and the batched path (icx_submit/icx_poll).
 static PcgPartials mixed_parts(icx_job *j) {
+  if (j->mesh) return j->parts(mesh_gather_blocks(j->mop), pcg_stream_blocks(j->nvec()));
   return j->parts(elastic_fusedA_blocks<float>(j->g), pcg_stream_blocks(j->nvec()));
 }
-// seam layers of the job's kernel A (its tiling, as fusedA_args sets it up)
+// seam layers of the job's kernel A (its tiling, as fusedA_args sets it up); none for
+// mesh jobs
 static SeamGeom mixed_seams(icx_job *j) {
+  if (j->mesh) return SeamGeom{};
   OpArgs<float> a;
   MixedWork &w = j->mw;
   fusedA_args<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p, w.pb.p, w.Ap.p, w.Ap2.p,
@@ -1709,7 +1712,9 @@ static SrVectors sr_vectors(icx_job *j) {
 }
 
 // hexa8 solves take the single-reduction iteration when it is switched on
-static bool use_sr(const icx_job *j) { return j->g.ndim == 3 && !j->heat() && j->ctx->single_reduction; }
+static bool use_sr(const icx_job *j) {
+  return j->g.ndim == 3 && !j->heat() && !j->mesh && j->ctx->single_reduction;
+}
 
 // Set-up of a mixed solve.  sr: the single-reduction iteration (icx_sr.hip) instead of
 // kernels A and B.
@@ -1724,13 +1729,18 @@ static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st
   HIPCHK(c, w.b32.alloc(N));
   if (sr) {
     HIPCHK(c, w.r1.alloc(N)); HIPCHK(c, w.ap1.alloc(N)); HIPCHK(c, w.part.alloc(12 * RED_MAXBLOCKS));
-  } else {
+  } else if (!j->mesh) {
     HIPCHK(c, w.Ap2.alloc(N));
   }
   HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(float), S(j)));
   const PcgPartials P = mixed_parts(j);
   launch_convert_f64_f32(j->g.ne, j->d_E.p, w.E.p, S(j));
-  launch_elastic_invdiag_node<float>(j->g, j->op32, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
+  if (j->mesh) {  // the FP64 Jacobi inverse (1 on constrained components), rounded
+    launch_mesh_diag(j->mop, j->d_E.p, j->d_mask.p, w.Kx.p, true, S(j));
+    launch_convert_f64_f32(N, w.Kx.p, w.invd.p, S(j));
+  } else {
+    launch_elastic_invdiag_node<float>(j->g, j->op32, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
+  }
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.xlo.p, 0, N * sizeof(float), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(float), S(j)));
@@ -1761,6 +1771,17 @@ static void mixed_iteration(icx_job *j) {
   icx_ctx *c = j->ctx;
   MixedWork &w = j->mw;
   const PcgPartials P = mixed_parts(j);
+  if (j->mesh) {  // one buffer holds p (updated in place)
+    {
+      ProfScope ps(c, ICX_PROF_FUSED_A);
+      launch_mesh_mixed_iteration(j->mop, j->d_state.p, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.Ap.p,
+                                  w.xlo.p, P, S(j), ps.ev());
+    }
+    ProfScope ps(c, ICX_PROF_FUSED_B);
+    launch_pcg_fusedB<float>(j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, nullptr, SeamGeom{}, S(j),
+                             ps.ev());
+    return;
+  }
   {
     ProfScope ps(c, ICX_PROF_FUSED_A);
     launch_elastic_fusedA<float>(j->g, j->op32, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p, w.pb.p,
@@ -1782,6 +1803,12 @@ static void mixed_replace(icx_job *j) {
   MixedWork &w = j->mw;
   const PcgPartials P = mixed_parts(j);
   // single-reduction form: x is complete at the stop (no step owed)
+  if (j->mesh) {  // p in place: both parities read the one buffer
+    launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pa.p, S(j));
+    launch_mesh_matvec(j->mop, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
+    launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, S(j));
+    return;
+  }
   launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pb.p, S(j));
   launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
   launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, S(j));
@@ -1967,7 +1994,8 @@ icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx
     return fail(j->ctx, ICX_ERR_STATE, "icx_solve needs icx_make_dof and icx_set_design");
   if (!(tol > 0.0) || maxit < 1) return fail(j->ctx, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
   if (j->heat() && form != ICX_SOLVE_PLAIN) return fail(j->ctx, ICX_ERR_ARG, "heat jobs: ICX_SOLVE_PLAIN only");
-  if (j->mesh && form != ICX_SOLVE_PLAIN) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: ICX_SOLVE_PLAIN only");
+  if (j->mesh && form == ICX_SOLVE_FUSED)
+    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: ICX_SOLVE_PLAIN or ICX_SOLVE_MIXED");
   icx_ctx *c = j->ctx;
   (void)hipSetDevice(c->device);
   icx_solve_info inf{};
@@ -2972,8 +3000,8 @@ icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0)
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
   if (par->driver < 0 || par->driver > 2) return fail(j->ctx, ICX_ERR_ARG, "driver must be 0 (OC), 1 (MMA) or 2 (GCMMA)");
-  if (j->mesh && par->form != ICX_SOLVE_PLAIN)
-    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: the design loop takes ICX_SOLVE_PLAIN");
+  if (j->mesh && par->form == ICX_SOLVE_FUSED)
+    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: the design loop takes ICX_SOLVE_PLAIN or ICX_SOLVE_MIXED");
   icx_ctx *c = j->ctx;
   OptWork &o = j->opt;
   o.par = *par;

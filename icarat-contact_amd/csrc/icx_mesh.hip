@@ -26,57 +26,57 @@ __host__ __device__ constexpr int csign(int a, int d) {
 
 // Gauss point ip of the element and the Jacobian there: inv(J) and det(J).  X[a][k] are
 // the node coordinates.  dN_a/dr_d = s_ad / 2^NC prod_{e != d} (1 + s_ae r_e).
-template <int NE, int NC>
+template <int NE, int NC, typename T = double>
 struct GaussPoint {
-  double r[NC];
-  double inv[NC][NC];
-  double det;
-  __device__ __forceinline__ double dr(int a, int d) const {
-    double v = (NC == 3 ? 0.125 : 0.25) * csign(a, d);
+  T r[NC];
+  T inv[NC][NC];
+  T det;
+  __device__ __forceinline__ T dr(int a, int d) const {
+    T v = T(NC == 3 ? 0.125 : 0.25) * T(csign(a, d));
 #pragma unroll
     for (int e = 0; e < NC; ++e)
-      if (e != d) v *= 1.0 + csign(a, e) * r[e];
+      if (e != d) v *= T(1) + T(csign(a, e)) * r[e];
     return v;
   }
   // N_a = prod_d (1 + s_ad r_d) / 2^NC
-  __device__ __forceinline__ double N(int a) const {
-    double v = NC == 3 ? 0.125 : 0.25;
+  __device__ __forceinline__ T N(int a) const {
+    T v = T(NC == 3 ? 0.125 : 0.25);
 #pragma unroll
-    for (int d = 0; d < NC; ++d) v *= 1.0 + csign(a, d) * r[d];
+    for (int d = 0; d < NC; ++d) v *= T(1) + T(csign(a, d)) * r[d];
     return v;
   }
   // dN_a / dX_k
-  __device__ __forceinline__ double dx(int a, int k) const {
-    double v = 0.0;
+  __device__ __forceinline__ T dx(int a, int k) const {
+    T v = T(0);
 #pragma unroll
     for (int d = 0; d < NC; ++d) v = fma_t(inv[k][d], dr(a, d), v);
     return v;
   }
-  __device__ __forceinline__ GaussPoint(int ip, const double (&X)[NE][NC]) {
-    const double g = 0.57735026918962576451;  // 1 / sqrt(3)
+  __device__ __forceinline__ GaussPoint(int ip, const T (&X)[NE][NC]) {
+    const T g = T(0.57735026918962576451);  // 1 / sqrt(3)
 #pragma unroll
-    for (int d = 0; d < NC; ++d) r[d] = csign(ip, d) * g;
-    double J[NC][NC];
+    for (int d = 0; d < NC; ++d) r[d] = T(csign(ip, d)) * g;
+    T J[NC][NC];
 #pragma unroll
     for (int d = 0; d < NC; ++d)
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
-        double v = 0.0;
+        T v = T(0);
 #pragma unroll
         for (int a = 0; a < NE; ++a) v = fma_t(dr(a, d), X[a][k], v);
         J[d][k] = v;
       }
     if (NC == 2) {
       det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      const double id = 1.0 / det;
+      const T id = T(1) / det;
       inv[0][0] = J[1][1] * id; inv[0][1] = -J[0][1] * id;
       inv[1][0] = -J[1][0] * id; inv[1][1] = J[0][0] * id;
     } else {
-      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const T c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const T c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const T c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
       det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-      const double id = 1.0 / det;
+      const T id = T(1) / det;
       inv[0][0] = c00 * id;
       inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
       inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
@@ -93,22 +93,22 @@ struct GaussPoint {
 constexpr int voigt(int nc) { return nc == 3 ? 6 : 3; }
 
 // engineering strain B u (xx, yy, zz, xy, yz, zx / xx, yy, xy)
-template <int NE, int NC>
-__device__ __forceinline__ void strain(const GaussPoint<NE, NC> &gp, const double (&u)[NE * NC],
-                                       double (&eps)[voigt(NC)]) {
+template <int NE, int NC, typename T = double>
+__device__ __forceinline__ void strain(const GaussPoint<NE, NC, T> &gp, const T (&u)[NE * NC],
+                                       T (&eps)[voigt(NC)]) {
 #pragma unroll
-  for (int v = 0; v < voigt(NC); ++v) eps[v] = 0.0;
+  for (int v = 0; v < voigt(NC); ++v) eps[v] = T(0);
 #pragma unroll
   for (int a = 0; a < NE; ++a) {
-    const double gx = gp.dx(a, 0), gy = gp.dx(a, 1);
+    const T gx = gp.dx(a, 0), gy = gp.dx(a, 1);
     if (NC == 2) {
-      const double ux = u[2 * a], uy = u[2 * a + 1];
+      const T ux = u[2 * a], uy = u[2 * a + 1];
       eps[0] = fma_t(gx, ux, eps[0]);
       eps[1] = fma_t(gy, uy, eps[1]);
       eps[2] = fma_t(gy, ux, fma_t(gx, uy, eps[2]));
     } else {
-      const double gz = gp.dx(a, 2);
-      const double ux = u[3 * a], uy = u[3 * a + 1], uz = u[3 * a + 2];
+      const T gz = gp.dx(a, 2);
+      const T ux = u[3 * a], uy = u[3 * a + 1], uz = u[3 * a + 2];
       eps[0] = fma_t(gx, ux, eps[0]);
       eps[1] = fma_t(gy, uy, eps[1]);
       eps[2] = fma_t(gz, uz, eps[2]);
@@ -121,63 +121,68 @@ __device__ __forceinline__ void strain(const GaussPoint<NE, NC> &gp, const doubl
 
 // stress D eps of the unit-modulus material: normal diagonal d0, normal off-diagonal d1,
 // shear d2 (isotropic 3D, plane stress and plane strain all have this pattern)
-template <int NC>
-__device__ __forceinline__ void stress(const MeshOp &m, const double (&eps)[voigt(NC)], double (&sig)[voigt(NC)]) {
+template <int NC, typename T = double>
+__device__ __forceinline__ void stress(const MeshOp &m, const T (&eps)[voigt(NC)], T (&sig)[voigt(NC)]) {
+  const T d0 = T(m.d0), d1 = T(m.d1), d2 = T(m.d2);
   if (NC == 2) {
-    sig[0] = fma_t(m.d0, eps[0], m.d1 * eps[1]);
-    sig[1] = fma_t(m.d1, eps[0], m.d0 * eps[1]);
-    sig[2] = m.d2 * eps[2];
+    sig[0] = fma_t(d0, eps[0], d1 * eps[1]);
+    sig[1] = fma_t(d1, eps[0], d0 * eps[1]);
+    sig[2] = d2 * eps[2];
   } else {
-    const double t = m.d1 * (eps[0] + eps[1] + eps[2]);
-    const double dd = m.d0 - m.d1;
+    const T t = d1 * (eps[0] + eps[1] + eps[2]);
+    const T dd = d0 - d1;
     sig[0] = fma_t(dd, eps[0], t);
     sig[1] = fma_t(dd, eps[1], t);
     sig[2] = fma_t(dd, eps[2], t);
-    sig[3] = m.d2 * eps[3];
-    sig[4] = m.d2 * eps[4];
-    sig[5] = m.d2 * eps[5];
+    sig[3] = d2 * eps[3];
+    sig[4] = d2 * eps[4];
+    sig[5] = d2 * eps[5];
   }
 }
 
-template <int NE, int NC>
-__device__ __forceinline__ void gather_coords(const MeshOp &m, long long e, double (&X)[NE][NC]) {
+// node coordinates of element e; in FP32 relative to its first node (formed in FP64), so
+// the Jacobian, which depends on coordinate differences only, keeps FP32 precision
+template <int NE, int NC, typename T = double>
+__device__ __forceinline__ void gather_coords(const MeshOp &m, long long e, T (&X)[NE][NC]) {
+  double base[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) base[c] = sizeof(T) == 4 ? m.xyz[c * m.nn + m.conn[e]] : 0.0;
 #pragma unroll
   for (int a = 0; a < NE; ++a) {
     const long long n = m.conn[a * m.nelx + e];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) X[a][c] = m.xyz[c * m.nn + n];
+    for (int c = 0; c < NC; ++c) X[a][c] = sizeof(T) == 4 ? T(m.xyz[c * m.nn + n] - base[c]) : T(m.xyz[c * m.nn + n]);
   }
 }
-template <int NE, int NC>
-__device__ __forceinline__ void gather_values(const MeshOp &m, long long e, const double *x, double (&u)[NE * NC],
-                                              bool flush) {
+template <int NE, int NC, typename T = double>
+__device__ __forceinline__ void gather_values(const MeshOp &m, long long e, const T *x, T (&u)[NE * NC], bool flush) {
 #pragma unroll
   for (int a = 0; a < NE; ++a) {
     const long long n = m.conn[a * m.nelx + e];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const double v = x[c * m.nn + n];
-      u[a * NC + c] = flush ? flush15(v) : v;
+      const T v = x[c * m.nn + n];
+      u[a * NC + c] = flush ? T(flush15(v)) : v;
     }
   }
 }
 
-// y_e = E_e Ke0_e x_e
-template <int NE, int NC>
-__global__ void __launch_bounds__(256) k_mesh_element(MeshOp m, const double *E, const double *x, double *ye,
+// y_e = E_e Ke0_e x_e (T = float: the mixed form's FP32 operator)
+template <int NE, int NC, typename T = double>
+__global__ void __launch_bounds__(256) k_mesh_element(MeshOp m, const T *E, const T *x, T *ye,
                                                       const SolverState *st) {
   constexpr int ND = NE * NC;
   if (st && st->done) return;
   for (long long e = blockIdx.x * 256LL + threadIdx.x; e < m.nelx; e += (long long)gridDim.x * 256) {
-    double X[NE][NC], u[ND], f[ND];
+    T X[NE][NC], u[ND], f[ND];
     gather_coords<NE, NC>(m, e, X);
     gather_values<NE, NC>(m, e, x, u, false);
 #pragma unroll
-    for (int i = 0; i < ND; ++i) f[i] = 0.0;
+    for (int i = 0; i < ND; ++i) f[i] = T(0);
 #pragma unroll 1  // one Gauss point at a time: the element's values stay in registers
     for (int ip = 0; ip < NE; ++ip) {
-      const GaussPoint<NE, NC> gp(ip, X);
-      double eps[voigt(NC)], sig[voigt(NC)];
+      const GaussPoint<NE, NC, T> gp(ip, X);
+      T eps[voigt(NC)], sig[voigt(NC)];
       strain<NE, NC>(gp, u, eps);
       stress<NC>(m, eps, sig);
 #pragma unroll
@@ -185,22 +190,22 @@ __global__ void __launch_bounds__(256) k_mesh_element(MeshOp m, const double *E,
       // B^T sigma
 #pragma unroll
       for (int a = 0; a < NE; ++a) {
-        const double gx = gp.dx(a, 0), gy = gp.dx(a, 1);
+        const T gx = gp.dx(a, 0), gy = gp.dx(a, 1);
         if (NC == 2) {
           f[2 * a] = fma_t(gx, sig[0], fma_t(gy, sig[2], f[2 * a]));
           f[2 * a + 1] = fma_t(gy, sig[1], fma_t(gx, sig[2], f[2 * a + 1]));
         } else {
-          const double gz = gp.dx(a, 2);
+          const T gz = gp.dx(a, 2);
           f[3 * a] = fma_t(gx, sig[0], fma_t(gy, sig[3], fma_t(gz, sig[5], f[3 * a])));
           f[3 * a + 1] = fma_t(gy, sig[1], fma_t(gx, sig[3], fma_t(gz, sig[4], f[3 * a + 1])));
           f[3 * a + 2] = fma_t(gz, sig[2], fma_t(gy, sig[4], fma_t(gx, sig[5], f[3 * a + 2])));
         }
       }
     }
-    const double Ee = E[e];
+    const T Ee = E[e];
 #pragma unroll
     for (int a = 0; a < NE; ++a) {
-      double *out = ye + (long long)m.slot[e * NE + a] * NC;
+      T *out = ye + (long long)m.slot[e * NE + a] * NC;
 #pragma unroll
       for (int c = 0; c < NC; ++c) out[c] = Ee * f[a * NC + c];
     }
@@ -263,6 +268,68 @@ __global__ void __launch_bounds__(256) k_mesh_gather(MeshOp m, const double *ye,
       y[c * m.nn + n] = invert ? (d != 0.0 ? 1.0 / d : 1.0) : d;
     }
   }
+}
+
+// ---- mixed form on mesh jobs: FP32 inner Jacobi-PCG, FP64 residual replacement ------
+// Per iteration: k_mesh_dir (the iteration scalars from kernel B's partials and the stop
+// protocol of the structured kernel A, then p = invd r + beta p and x += alpha_{i-1}
+// p_{i-1} in place: one thread per vector entry reads and writes it), the FP32 element
+// pass, k_mesh_gather_dot (Ap = node sums, zero on constrained components, and the
+// block partials of p.Ap), then kernel B (icx_pcg.hip).  A stop found by k_mesh_dir
+// leaves p unchanged (p_{i-1}, the direction the replacement's owed x step reads).
+template <typename T>
+__global__ void __launch_bounds__(256) k_mesh_dir(long long n, SolverState *st, const T *r, const T *invd, T *p,
+                                                  T *x, PcgPartials P) {
+  __shared__ double s_sc[2 * 4];
+  const SolverState ss = load_state(st);
+  if (ss.done) return;  // uniform
+  double rr, rz;
+  block_total2<256>(P.PB, P.nB, s_sc, rr, rz);
+  const int it = ss.iter;
+  const IterScalars sc = iter_scalars(ss, it, rr, rz);
+  const bool owed = it > 0 && ss.flush_iter != it;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter_cur = it;
+    st->rz_hist[it & 1] = sc.rz;
+    if (sc.stop) {
+      st->pending = sc.stop;  // kernel B publishes it as `done`
+      st->stop_iter = it;
+      st->rr = sc.rr;
+      st->alpha_owed = owed ? sc.alpha_prev : 0.0;
+    }
+  }
+  if (sc.stop) return;
+  const T beta = (T)sc.beta, ax = owed ? (T)sc.alpha_prev : T(0);
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+    const T po = p[i];
+    x[i] = fma_t(ax, po, x[i]);
+    p[i] = fma_t(beta, po, invd[i] * r[i]);
+  }
+}
+
+template <int NC, typename T>
+__global__ void __launch_bounds__(256) k_mesh_gather_dot(MeshOp m, const T *ye, const uint8_t *mask, const T *p, T *y,
+                                                         const SolverState *st, double *PA) {
+  __shared__ double s_red[4];
+  if (st->done) return;  // uniform
+  double dot = 0.0;
+  for (long long n = blockIdx.x * 256LL + threadIdx.x; n < m.nn; n += (long long)gridDim.x * 256) {
+    T acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = T(0);
+    for (int k = m.adj_ptr[n]; k < m.adj_ptr[n + 1]; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) acc[c] += ye[(long long)k * NC + c];
+    const uint32_t msk = mask ? mask[n] : 0u;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const T d = ((msk >> c) & 1) ? T(0) : acc[c];
+      y[c * m.nn + n] = d;
+      dot = fma_t((double)p[c * m.nn + n], (double)d, dot);
+    }
+  }
+  dot = block_sum<256>(dot, s_red);
+  if (threadIdx.x == 0) PA[blockIdx.x] = dot;
 }
 
 // u_e^T Ke0_e u_e = sum_ip eps^T D eps det J with |u| < 1e-15 flushed (the displacement
@@ -521,6 +588,28 @@ void launch_mesh_mises(const MeshOp &m, const double *E, const double *u, double
     hipLaunchKernelGGL((k_mesh_mises<8, 3>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, u, out);
   else
     hipLaunchKernelGGL((k_mesh_mises<4, 2>), dim3(mesh_blocks(m.nelx)), dim3(256), 0, s, m, E, u, out);
+  ICX_CHECK_LAUNCH();
+}
+
+int mesh_gather_blocks(const MeshOp &m) { return mesh_blocks(m.nn); }
+
+void launch_mesh_mixed_iteration(const MeshOp &m, SolverState *st, const float *E, const uint8_t *mask,
+                                 const float *r, const float *invd, float *p, float *Ap, float *x,
+                                 const PcgPartials &P, hipStream_t s, const LaunchEvents *ev) {
+  const long long n = (long long)m.nc * m.nn;
+  hipLaunchKernelGGL(k_mesh_dir<float>, dim3(ew_blocks(n, 4, 4096)), dim3(256), 0, s, n, st, r, invd, p, x, P);
+  float *ye = reinterpret_cast<float *>(m.ye);
+  if (m.ne == 8) {
+    launch_ev(k_mesh_element<8, 3, float>, mesh_blocks(m.nelx), 256, s, ev, m, E, (const float *)p, ye,
+              (const SolverState *)st);
+    hipLaunchKernelGGL((k_mesh_gather_dot<3, float>), dim3(mesh_gather_blocks(m)), dim3(256), 0, s, m,
+                       (const float *)ye, mask, (const float *)p, Ap, (const SolverState *)st, P.PA);
+  } else {
+    launch_ev(k_mesh_element<4, 2, float>, mesh_blocks(m.nelx), 256, s, ev, m, E, (const float *)p, ye,
+              (const SolverState *)st);
+    hipLaunchKernelGGL((k_mesh_gather_dot<2, float>), dim3(mesh_gather_blocks(m)), dim3(256), 0, s, m,
+                       (const float *)ye, mask, (const float *)p, Ap, (const SolverState *)st, P.PA);
+  }
   ICX_CHECK_LAUNCH();
 }
 

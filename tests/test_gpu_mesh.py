@@ -74,6 +74,49 @@ def test_mesh_operator_solve_sensitivity(ctx, ne, state, div, L):
     assert np.abs(Rf - F).max() <= 1e-9 * np.abs(F).max()
 
 
+@pytest.mark.parametrize("ne,state,div,L", CASES)
+def test_mesh_mixed_form_meets_true_residual(ctx, ne, state, div, L):
+    """The mixed form on a distorted mesh (FP32 Gauss-point element pass, FP32 inner
+    Jacobi-PCG, FP64 residual replacement): the true residual through the oracle's CSR
+    matrix meets the tolerance, and u matches the direct solution."""
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 3, L)
+    job, pr, rho, b = _pair(ctx, ne, state, xd, conn, L)
+    ud = pr.direct(b)
+    K = pr.csr()
+    for tol in (1e-8, 1e-11):
+        u, info = job.solve(icx.SOLVE_MIXED, tol=tol, maxit=50000)
+        assert info["converged"] and info["refinements"] >= 1, (tol, info)
+        assert np.linalg.norm(b - K @ u) / np.linalg.norm(b) < tol, tol
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-8
+    # the plain form on the same job afterwards still follows the direct solution
+    u2, _ = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=50000)
+    assert np.linalg.norm(u2 - ud) / np.linalg.norm(ud) < 1e-10
+
+
+def test_mesh_driver_a_mixed_form(ctx):
+    """Driver A on a distorted hexa8 mesh with the mixed form at 1e-11: objectives and
+    design follow the oracle's loop with direct solves."""
+    ne, div, L, steps = 8, (8, 4, 4), (8.0, 4.0, 4.0), 6
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 14, L)
+    job = icx.Job.from_mesh(ctx, ne, xd, conn, icx.SOLID3D)
+    pr = O.Problem.from_mesh(ne, xd, conn, O.SOLID3D)
+    for o in (job, pr):
+        o.add_dirichlet([1, 1, 1, 0, 0, 0], [0] * 6, [0, 0, 0, L[1], 0, L[2]])
+    assert job.make_dof() == pr.make_dof()
+    for o in (job, pr):
+        o.add_neumann([0.0, 0.0, -0.1], [L[0], L[0], 0.0, L[1], 0.0, L[2]])
+    x0 = np.full(pr.nelx, 0.5)
+    ref = pr.run_driver(0, x0, steps, 1.5, projection=True, direct=True)
+    job.opt_init(x0, driver=0, filter=icx.FILTER_CONE, radius=1.5, form=icx.SOLVE_MIXED,
+                 tol=1e-11, maxit=50000)
+    obj = np.array([job.opt_step()["objective"] for _ in range(steps)])
+    rel = np.abs(obj - ref["obj"]) / ref["obj"]
+    assert rel.max() < 1e-8, rel
+    assert np.abs(job.opt_design() - ref["design"]).max() < 1e-6
+
+
 @pytest.mark.parametrize("ne,div,L", [(4, (10, 5), (10.0, 5.0)), (8, (6, 3, 4), (6.0, 3.0, 4.0))])
 def test_box_mesh_equals_structured_job(ctx, ne, div, L):
     """The structured box given as an explicit mesh: same operator, same solve."""
@@ -128,9 +171,9 @@ def test_mesh_job_rejects_box_only_calls(ctx):
     job.make_dof()
     job.set_design(np.full(job.nelx, 0.5))
     v = np.ones(job.numeq)
-    for call in (lambda: job.matvec(v, prec=1), lambda: job.solve(icx.SOLVE_MIXED),
-                 lambda: job.solve(icx.SOLVE_FUSED), lambda: job.submit(tol=1e-8),
-                 lambda: job.opt_init(np.full(job.nelx, 0.5), form=icx.SOLVE_MIXED)):
+    for call in (lambda: job.matvec(v, prec=1), lambda: job.solve(icx.SOLVE_FUSED),
+                 lambda: job.submit(tol=1e-8),
+                 lambda: job.opt_init(np.full(job.nelx, 0.5), form=icx.SOLVE_FUSED)):
         with pytest.raises(icx.IcxError) as e:
             call()
         assert e.value.code == 1
