@@ -152,18 +152,25 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
     peak_active = 0
     finished_jobs = 0
     cancelled, rejected = [], []
+    # wall time by phase: batched PCG chunks (poll), per-job work after a solve (step_end:
+    # sensitivity, adjoint filter, OC) and before one (step_begin + submit: filter,
+    # projection, set-up), admission (create + opt_init)
+    split = {"poll": 0.0, "step_end": 0.0, "step_begin_submit": 0.0, "admit": 0.0}
     t0 = time.perf_counter()
 
     def begin(rec):
-        rec["t_begin"] = time.perf_counter()
+        tb = time.perf_counter()
+        rec["t_begin"] = tb
         rec["job"].opt_step_begin()
         rec["job"].submit(tol=tol, maxit=maxit)
+        split["step_begin_submit"] += time.perf_counter() - tb
 
     while pending or active:
         now = time.perf_counter() - t0
         while (pending and arrival[pending[0]] <= now and used + est[pending[0]] <= budget and
                (max_resident_dofs is None or not active or resident + ndofs[pending[0]] <= max_resident_dofs)):
             i = pending.popleft()
+            ta = time.perf_counter()
             try:
                 job, x_init = create(ctx, specs[i])
             except IcxError as e:
@@ -172,6 +179,7 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
                 continue
             job.opt_init(np.full(job.nelx, x0 if x_init is None else x_init), driver=0, radius=radius,
                          projection=True, form=SOLVE_MIXED, tol=tol, maxit=maxit)
+            split["admit"] += time.perf_counter() - ta
             rec = {"job": job, "spec": i, "step": 0, "numeq": job.numeq}
             active[id(job)] = rec
             used += est[i]
@@ -182,7 +190,9 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
             admit_wait.append(time.perf_counter() - t0 - arrival[i])
             begin(rec)
         if active:
+            tp = time.perf_counter()
             finished = ctx.poll(chunks=1)
+            split["poll"] += time.perf_counter() - tp
             for rec in [r for r in active.values() if cancel.get(r["spec"]) == r["step"]]:
                 if rec["job"] in finished:
                     continue
@@ -196,7 +206,9 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
             for job in finished:
                 rec = active[id(job)]
                 info = job.last_info()
+                te = time.perf_counter()
                 step_rec = job.opt_step_end(info)
+                split["step_end"] += time.perf_counter() - te
                 if keep_designs:
                     histories.setdefault(rec["spec"], []).append(step_rec["objective"])
                 lat.append(time.perf_counter() - rec["t_begin"])
@@ -234,6 +246,7 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         "footprint_checks": footprints,
         "cancelled": cancelled,
         "rejected": rejected,
+        "wall_split_s": split,
         "designs": designs if keep_designs else None,
         "objectives": histories if keep_designs else None,
         "max_used_within_budget": peak_used <= budget,
