@@ -1791,7 +1791,8 @@ static void mixed_iteration(icx_job *j) {
   {
     ProfScope ps(c, ICX_PROF_FUSED_B);
     launch_pcg_fusedB<float>(j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, w.Ap2.p, mixed_seams(j), S(j),
-                             ps.ev());
+                             ps.ev(), (j->g.ndim == 3 && j->g.nn >= B_NODE_MIN_NODES) ? w.invs.p : nullptr,
+                             j->op32.ikd);
   }
 }
 
@@ -1913,7 +1914,13 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
                                       w.Ap2.p, w.xlo.p, j->d_state.p, P, a);
     if (j->g.ndim == 3) { a3.push_back(a); o3.push_back(o3.back() + pad8(nb)); }
     else { a2.push_back(a); o2.push_back(o2.back() + pad8(nb)); }
-    ab.push_back(BArgs<float>{j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, w.Ap2.p, fusedA_seams(a)});
+    BArgs<float> ba{j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, w.Ap2.p, fusedA_seams(a), nullptr,
+                    {0.f, 0.f, 0.f}, j->g.nn};
+    if (j->g.ndim == 3 && j->g.nn >= B_NODE_MIN_NODES) {  // the per-node Jacobi factor, as kernel A forms it
+      ba.invs = w.invs.p;
+      for (int c = 0; c < 3; ++c) ba.ikd[c] = j->op32.ikd[c];
+    }
+    ab.push_back(ba);
     ob.push_back(ob.back() + pad8(P.nB));
   }
   const int n = (int)B.active.size();

@@ -331,6 +331,11 @@ template <typename T> struct BArgs {
   PcgPartials P;
   const T *Ap2;       // lower halves of kernel A's seam layers (read where sg says)
   SeamGeom sg;
+  // box elastic jobs: the Jacobi factor per node (jacobi_inv(invs[node], ikd[c]), as
+  // kernel A forms it) instead of invd; nn nodes, n = 3 nn.  Null: invd per entry.
+  const T *invs;
+  T ikd[3];
+  long long nn;
 };
 // copies the states *src[j] of n jobs to dst[j]
 void launch_gather_states(SolverState *const *src, SolverState *dst, int n, hipStream_t s);
@@ -339,9 +344,16 @@ template <typename T>
 void launch_pcg_fusedB_batch(const BArgs<T> *args, const int *off, int njobs, int total_blocks, hipStream_t s);
 // fused kernel B of iteration i: alpha_i = rz_i / pAp_i; r -= alpha_i Ap; (|r|^2, r.z) -> PB[i&1]
 // Ap2 / sg: seam layers of the FP32 hexa8 kernel A (sg.kmax = 0: none).
+// invs / ikd (box elastic jobs, n = 3 nn): the Jacobi factor per node, read once for the
+// three components of each node, instead of invd.  Worth it where the iteration's
+// vectors outgrow the Infinity Cache and kernel B streams from HBM (measured: -1.6 us
+// per launch at 192x96x96, +0.1 us at 96x48x48), so the callers pass invs for jobs of
+// at least B_NODE_MIN_NODES nodes.
+constexpr long long B_NODE_MIN_NODES = 1LL << 20;
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
-                       const T *Ap2, const SeamGeom &sg, hipStream_t s, const LaunchEvents *ev = nullptr);
+                       const T *Ap2, const SeamGeom &sg, hipStream_t s, const LaunchEvents *ev = nullptr,
+                       const T *invs = nullptr, const T *ikd = nullptr);
 // fused FP64 form at the stop: x += alpha_owed p_{stop-1}
 void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0, const double *P1,
                       hipStream_t s);

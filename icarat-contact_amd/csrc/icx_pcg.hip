@@ -3,6 +3,7 @@
 // icx_device.h), indexed by the parity of the iteration that produced them.
 #include "icx_device.h"
 
+
 namespace icx {
 
 constexpr int BS = 256;
@@ -193,22 +194,152 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   }
 }
 
+// Kernel B on a box elastic job with the per-node Jacobi factor: each thread takes
+// chunks of 4 consecutive nodes with their 3 components (3 vectors of r and Ap, one of
+// invs read once for the three), so the preconditioner costs one word per node instead
+// of one per entry.  Component planes start at c nn, not 16-byte aligned in general:
+// the r and Ap vectors are dword-aligned 16-byte accesses.  Nodes past the last full
+// chunk (nn mod 4) are taken by thread 0 of block 0.
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+template <int U>
+__device__ __forceinline__ void fusedB_node_block(const BArgs<float> &a, int blk) {
+  __shared__ double scr[BS / 64];
+  __shared__ double scr2[2 * (BS / 64)];
+  SolverState *st = a.st;
+  const PcgPartials &P = a.P;
+  const int nblk = P.nB;
+  if (blk >= nblk) return;     // padding block of a batched launch
+  const SolverState ss = load_state(st);
+  const long long nn = a.nn, nq = nn / 4;
+  const long long stride = (long long)nblk * BS;
+  const long long q0 = blk * (long long)BS + threadIdx.x;
+  const bool seams = a.sg.kmax > 0;
+  f4u R[U][3], A[U][3], A2[U][3];
+  float4 S[U];
+  uint32_t sf[U];
+  auto load = [&](long long base) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long q = base + u * stride;
+      sf[u] = 0;
+      if (q < nq) {
+        const long long n0 = 4 * q;
+        S[u] = *reinterpret_cast<const float4 *>(a.invs + n0);
+        if (seams) sf[u] = seam_flags((uint32_t)n0, 4, a.sg);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          R[u][c] = *reinterpret_cast<const f4u *>(a.r + c * nn + n0);
+          A[u][c] = *reinterpret_cast<const f4u *>(a.Ap + c * nn + n0);
+          if (sf[u]) A2[u][c] = *reinterpret_cast<const f4u *>(a.Ap2 + c * nn + n0);
+        }
+      }
+    }
+  };
+  load(q0);
+  if (ss.done) return;
+  if (ss.pending) {
+    if (blk == 0 && threadIdx.x == 0) st->done = ss.pending;
+    return;
+  }
+  const double pAp = block_total<BS>(P.PA, P.nA, 1, scr);
+  const int it = ss.iter_cur;
+  const double alpha_d = ((it & 1) ? ss.rz_hist[1] : ss.rz_hist[0]) / pAp;
+  if (blk == 0 && threadIdx.x == 0) st->alpha_hist[it & 1] = alpha_d;
+  const float alpha = (float)alpha_d;
+  double rr = 0.0, rzn = 0.0;
+  auto node = [&](float sv, float apw, float &rv, int c) {
+    const float ri = fma_t(-alpha, apw, rv);
+    rv = ri;
+    rr = fma_t((double)ri, (double)ri, rr);
+    rzn = fma_t((double)ri, (double)(jacobi_inv(sv, a.ikd[c]) * ri), rzn);
+  };
+  for (long long base = q0; base < nq; base += U * stride) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long q = base + u * stride;
+      if (q < nq) {
+        const long long n0 = 4 * q;
+        const float sv[4] = {S[u].x, S[u].y, S[u].z, S[u].w};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          f4u out = R[u][c];
+#pragma unroll
+          for (int w = 0; w < 4; ++w) {
+            float apw = A[u][c][w];
+            if ((sf[u] >> w) & 1) apw += A2[u][c][w];
+            float rv = out[w];
+            node(sv[w], apw, rv, c);
+            out[w] = rv;
+          }
+          *reinterpret_cast<f4u *>(a.r + c * nn + n0) = out;
+        }
+      }
+    }
+    if (base + U * stride < nq) load(base + U * stride);
+  }
+  if (blk == 0 && threadIdx.x == 0) {
+    for (long long n = 4 * nq; n < nn; ++n) {
+      const bool sfl = seams && seam_flags((uint32_t)n, 1, a.sg);
+      for (int c = 0; c < 3; ++c) {
+        float apw = a.Ap[c * nn + n];
+        if (sfl) apw += a.Ap2[c * nn + n];
+        float rv = a.r[c * nn + n];
+        node(a.invs[n], apw, rv, c);
+        a.r[c * nn + n] = rv;
+      }
+    }
+  }
+  rr = wave_sum(rr);
+  rzn = wave_sum(rzn);
+  if ((threadIdx.x & 63) == 0) {
+    scr2[threadIdx.x >> 6] = rr;
+    scr2[BS / 64 + (threadIdx.x >> 6)] = rzn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int w = 0; w < BS / 64; ++w) {
+      s0 += scr2[w];
+      s1 += scr2[BS / 64 + w];
+    }
+    double *slot = pb_slot(P);
+    slot[2 * blk] = s0;
+    slot[2 * blk + 1] = s1;
+    if (blk == 0) st->iter = it + 1;
+  }
+}
+
+template <typename T, int U>
+__device__ __forceinline__ void fusedB_any(const BArgs<T> &a, int blk) {
+  if constexpr (sizeof(T) == 4) {
+    if (a.invs) {
+      fusedB_node_block<2>(a, blk);
+      return;
+    }
+  }
+  fusedB_block<T, U>(a, blk);
+}
+
 template <typename T, int U>
 __global__ void __launch_bounds__(BS) k_pcg_fusedB(BArgs<T> a) {
-  fusedB_block<T, U>(a, blockIdx.x);
+  fusedB_any<T, U>(a, blockIdx.x);
 }
 template <typename T, int U>
 __global__ void __launch_bounds__(BS)
     k_pcg_fusedB_batch(const BArgs<T> *__restrict__ args, const int *__restrict__ off, int njobs) {
   int j, blk;
   batch_block(off, njobs, j, blk);
-  fusedB_block<T, U>(args[j], blk);
+  fusedB_any<T, U>(args[j], blk);
 }
 
 template <typename T>
 void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T *invd, PcgPartials P,
-                       const T *Ap2, const SeamGeom &sg, hipStream_t s, const LaunchEvents *ev) {
-  const BArgs<T> a{n, st, r, Ap, invd, P, Ap2, sg};
+                       const T *Ap2, const SeamGeom &sg, hipStream_t s, const LaunchEvents *ev,
+                       const T *invs, const T *ikd) {
+  BArgs<T> a{n, st, r, Ap, invd, P, Ap2, sg, invs, {T(0), T(0), T(0)}, n / 3};
+  if (ikd)
+    for (int c = 0; c < 3; ++c) a.ikd[c] = ikd[c];
   launch_ev((k_pcg_fusedB<T, 4>), P.nB, BS, s, ev, a);
   ICX_CHECK_LAUNCH();
 }
@@ -301,7 +432,8 @@ template void launch_pcg_fusedB_batch<float>(const BArgs<float> *, const int *, 
   template void launch_pcg_setup<T>(long long, SolverState *, const T *, T *, const T *, PcgPartials,    \
                                     double, int, double, hipStream_t);                                   \
   template void launch_pcg_fusedB<T>(long long, SolverState *, T *, const T *, const T *, PcgPartials, \
-                                     const T *, const SeamGeom &, hipStream_t, const LaunchEvents *);
+                                     const T *, const SeamGeom &, hipStream_t, const LaunchEvents *,  \
+                                     const T *, const T *);
 ICX_INST_PCG(double)
 ICX_INST_PCG(float)
 

@@ -140,3 +140,29 @@ def test_single_reduction_batch_equals_alone(ctx):
         ctx.set_option(icx.OPT_SINGLE_REDUCTION, 0)
     for j in jobs:
         j.close()
+
+
+def test_large_hexa8_job_per_node_kernel_b(ctx):
+    """A hexa8 job of more than 2^20 nodes takes kernel B with the per-node Jacobi factor
+    (node chunks, dword-aligned 16-byte accesses of the component planes, the nn mod 4
+    tail nodes in one thread).  128 x 64 x 126 elements: 1 064 895 nodes (tail 3).  The
+    mixed solve meets the true residual through the FP64 operator, agrees with the FP64
+    plain form, and equals itself batched with a small job bit for bit."""
+    big = icx.cantilever_3d(ctx, 128, 64, 126)
+    assert big.numnp >= 1 << 20 and big.numnp % 4 == 3
+    big.set_design(icx.random_density(big.nelx, 5))
+    f = big.fext()
+    u, info = big.solve(icx.SOLVE_MIXED, tol=1e-9, maxit=50000)
+    assert info["converged"]
+    assert np.linalg.norm(f - big.matvec(u)) / np.linalg.norm(f) < 1e-9
+    up, _ = big.solve(icx.SOLVE_PLAIN, tol=1e-11, maxit=50000)
+    assert np.linalg.norm(u - up) / np.linalg.norm(up) < 1e-6
+    small = icx.cantilever_3d(ctx, 12, 6, 6)
+    small.set_design(icx.random_density(small.nelx, 6))
+    ref = _alone([big, small])
+    infos = ctx.solve_batch([big, small], tol=1e-8)
+    for j, (u0, i0), i1 in zip([big, small], ref, infos):
+        assert i1["iterations"] == i0["iterations"]
+        assert np.array_equal(j.displacement(), u0)
+    big.close()
+    small.close()
