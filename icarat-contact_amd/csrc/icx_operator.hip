@@ -33,7 +33,7 @@ __device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverSt
 // hexa8: a block of (TI+1)(TJ+1) threads owns a TI x TJ x TK brick of nodes and
 // marches along z.  Node layers rotate through three LDS buffers.  Every global load
 // is issued one layer step before its data is used -- the nodes of layer ez+2, the
-// moduli of element layer ez+1 and the owned nodes' x / p_old / constraint mask of
+// moduli of element layer ez+1 and the owned nodes' x / constraint mask of
 // layer ez+1 go out before the element phase of layer ez -- and the loop is unrolled
 // by two with alternating register sets, so no wait on a load in flight is ever
 // forced before its consumer.  Per layer: every thread applies one element (the
@@ -46,8 +46,9 @@ __device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverSt
 //
 // MODE 0: y = K x.   MODE 1 (fused kernel A of PCG iteration i): p_i = invd r +
 // beta_i p_{i-1} formed in the LDS loader, y = K p_i, x += alpha_{i-1} p_{i-1} on the
-// owned nodes, partial p_i . y.  The loads of the first node layer are issued before
-// the iteration scalars are reduced.
+// owned nodes (p_{i-1} kept in LDS beside p_i), partial p_i . y.  The loads of the
+// first node layer and of the first owned layer's x are issued before the iteration
+// scalars are reduced.
 //
 // Tiling along z.  MODE 0 and the FP64 MODE 1: a block owns node layers [k0, k1) and
 // applies the element layers k0-1 .. k1-1, one of them also applied by the block
@@ -60,7 +61,7 @@ __device__ __forceinline__ bool fused_scalars(const OpArgs<T> &a, const SolverSt
 // seam once.
 // ---------------------------------------------------------------------------------
 template <typename T> struct OwnedNode {
-  T x[3], po[3];
+  T x[3];
   uint32_t m;
 };
 
@@ -71,6 +72,9 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   constexpr int NPOS = (LN + NT - 1) / NT;
   constexpr uint32_t W = sizeof(T);
   __shared__ T s_in[3][3][LN];
+  // MODE 1: p_{i-1} of the committed node layers, kept beside p_i for the owned nodes'
+  // x update (no second global load of p_{i-1}, and 6 VGPRs fewer)
+  __shared__ T s_po[MODE == 1 ? 3 : 1][3][MODE == 1 ? LN : 1];
   static_assert(TI + 1 == 16, "element rows are 16-lane DPP rows");
   __shared__ T s_h[6][NT];
   __shared__ double s_red[NT / 64];
@@ -146,9 +150,11 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
       const int idx = tid + p * NT;
       if (idx < LN) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c) {
           s_in[buf][c][idx] =
               MODE == 0 ? pr[p][c] : fma_t(beta, po[p][c], jacobi_inv(pv[p], a.op.ikd[c]) * pr[p][c]);
+          if (MODE == 1) s_po[buf][c][idx] = po[p][c];
+        }
       }
     }
   };
@@ -162,10 +168,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     o.m = a.mask ? (uint32_t)ld_off(a.mask, n / W) : 0u;
     if (MODE == 1) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        o.x[c] = ld_off(a.x, n + c * cstr);
-        o.po[c] = ld_off(pold, n + c * cstr);
-      }
+      for (int c = 0; c < 3; ++c) o.x[c] = ld_off(a.x, n + c * cstr);
     }
   };
 
@@ -187,7 +190,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
         if (MODE == 1) {
           const T pv_ = s_in[buf][c][ctr];
           st_off(pnew, n + c * cstr, pv_);
-          st_off(a.x, n + c * cstr, fma_t(alpha_x, o.po[c], o.x[c]));
+          st_off(a.x, n + c * cstr, fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]));
           dot = fma_t((double)pv_, (double)yv[c], dot);
         }
       }
@@ -269,6 +272,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   }
   T Ea = load_E(kfirst), Eb = T(0);
   OwnedNode<T> na{}, nb{};
+  if (SEAM) load_node(k0, na);  // the first step's node phase completes node layer k0
   double rr = 0.0, rz = 0.0;
   // solve already finished (uniform: no kernel writes `done` it reads): leave before the
   // partials reduction, whose results only a running iteration uses
@@ -286,7 +290,6 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     }
     if (!fused_scalars<T>(a, ss, it, blk, rr, rz, beta, alpha_x)) return;
   }
-  if (SEAM) load_node(k0, na);  // the first step's node phase completes node layer k0 (p_old known now)
   wait_vmem();
   commit(0);
   prefetch(kfirst + 1);
@@ -304,7 +307,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
       if (top_tile) {
         st_off(a.y, n + c * cstr, yt);
         st_off(pnew, n + c * cstr, pv_);
-        st_off(a.x, n + c * cstr, fma_t(alpha_x, o.po[c], o.x[c]));
+        st_off(a.x, n + c * cstr, fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]));
       } else {
         st_off(a.y2, n + c * cstr, yt);
       }
