@@ -3427,7 +3427,7 @@ icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
     add(*d);
   // mesh jobs and the output step
   add(j->d_conn); add(j->d_adj_ptr); add(j->d_slot); add(j->d_xyz); add(j->d_ye); add(j->d_vol);
-  add(j->ow.ufull); add(j->ow.nodes); add(j->ow.mises); add(j->d_oc);
+  add(j->ow.ufull); add(j->ow.nodes); add(j->ow.mises); add(j->d_oc); add(j->d_mdual);
   *bytes = b;
   return ICX_OK;
 }
