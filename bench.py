@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--serve-elems", type=float, nargs=2, default=[5e3, 3e5])
     ap.add_argument("--serve-budget", type=float, default=240e9)
     ap.add_argument("--serve-limit", type=int, default=None, help="serve the first N jobs of the stream")
+    ap.add_argument("--serve-offset", type=int, default=0,
+                    help="start at job N of the stream (with --serve-limit: one segment of it)")
+    ap.add_argument("--serve-dump", default=None, help="write the latencies of the run to this .npz file")
     ap.add_argument("--serve-max-dofs", type=float, default=None,
                     help="admit jobs while the resident DOFs stay within this many")
     ap.add_argument("--pmc-iters", type=int, default=200)
@@ -631,8 +634,14 @@ def serving(args):
     out = serve.run(ctx, n_jobs=args.serve_jobs, rate=args.serve_rate, iters=args.serve_iters,
                     elems=tuple(args.serve_elems), budget=args.serve_budget, tol=args.tol,
                     log=lambda m: print(m, file=sys.stderr, flush=True), limit=args.serve_limit,
-                    max_resident_dofs=int(args.serve_max_dofs) if args.serve_max_dofs else None)
+                    max_resident_dofs=int(args.serve_max_dofs) if args.serve_max_dofs else None,
+                    offset=args.serve_offset)
     out.pop("designs", None)
+    lat = out.pop("latencies_ms")
+    waits = out.pop("admission_wait_ms")
+    if args.serve_dump:  # every design iteration's latency and every admission wait, for
+        import numpy as np  # combining the segments of a stream served in parts
+        np.savez(args.serve_dump, latencies_ms=np.array(lat), admission_wait_ms=np.array(waits))
     line = {"metric": "serving: design-iteration latency p50/p99 and DOF-matvec/s",
             "value": out["dof_matvec_per_s"], "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
             "dtype": "f32 operator / f64 residual", "data": "synthetic (Poisson arrivals, seed 20261205)",

@@ -95,7 +95,7 @@ def make_job(ctx, ndim, div):
 
 def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, seed=SEED,
         tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None, limit=None,
-        cancel=None, max_resident_dofs=None):
+        cancel=None, max_resident_dofs=None, offset=0):
     """Serve n_jobs jobs arriving at `rate` per second (exponential gaps), each running
     `iters` OC design iterations.
     specs: optional list of requests instead of the drawn sizes -- (ndim, div)
@@ -104,6 +104,9 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
     keep_designs: return each job's final design under "designs" and its objective
       history under "objectives".
     limit: serve only the first `limit` jobs of the drawn stream.
+    offset: start at job `offset` of the stream (its arrival at t = 0, the later ones at
+      their offsets from it): with limit, one segment of a stream too long to serve in
+      one sitting, starting empty.
     cancel: {job: design iteration} -- the job is withdrawn while that iteration's solve
       runs (icx_cancel), its budget freed.
     max_resident_dofs: admit a job only while the resident jobs' DOFs, its own included,
@@ -118,6 +121,9 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
     arrival = np.cumsum(gaps) - gaps[0]
     drawn = [job_spec(rng, *elems) for _ in range(n_jobs)]
     specs = list(specs) if specs is not None else drawn
+    if offset:
+        specs = specs[offset:]
+        arrival = arrival[offset:] - arrival[offset]
     if limit is not None:
         specs = specs[:limit]
     n_jobs = len(specs)
@@ -247,6 +253,9 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         "cancelled": cancelled,
         "rejected": rejected,
         "wall_split_s": split,
+        "offset": offset,
+        "latencies_ms": lat_ms.tolist(),
+        "admission_wait_ms": (np.array(admit_wait) * 1e3).tolist(),
         "designs": designs if keep_designs else None,
         "objectives": histories if keep_designs else None,
         "max_used_within_budget": peak_used <= budget,
