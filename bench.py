@@ -637,11 +637,24 @@ def serving(args):
                     max_resident_dofs=int(args.serve_max_dofs) if args.serve_max_dofs else None,
                     offset=args.serve_offset)
     out.pop("designs", None)
+    import numpy as np
     lat = out.pop("latencies_ms")
     waits = out.pop("admission_wait_ms")
+    recs = np.array(out.pop("iteration_records"), dtype=float).reshape(-1, 10)
+    # latency by job size (free DOFs): where the tail comes from
+    by_size = {}
+    for lo, hi in ((0, 1e5), (1e5, 1e6), (1e6, 1e12)):
+        sel = recs[(recs[:, 2] >= lo) & (recs[:, 2] < hi)]
+        if len(sel):
+            by_size["%g-%g" % (lo, hi)] = {
+                "design_iterations": int(len(sel)), "ms_p50": float(np.percentile(sel[:, 6], 50)),
+                "ms_p99": float(np.percentile(sel[:, 6], 99)),
+                "pcg_iterations_p50": float(np.percentile(sel[:, 3], 50)),
+                "pcg_iterations_max": int(sel[:, 3].max())}
+    out["latency_by_dofs"] = by_size
     if args.serve_dump:  # every design iteration's latency and every admission wait, for
-        import numpy as np  # combining the segments of a stream served in parts
-        np.savez(args.serve_dump, latencies_ms=np.array(lat), admission_wait_ms=np.array(waits))
+        # combining the segments of a stream served in parts
+        np.savez(args.serve_dump, latencies_ms=np.array(lat), admission_wait_ms=np.array(waits), records=recs)
     line = {"metric": "serving: design-iteration latency p50/p99 and DOF-matvec/s",
             "value": out["dof_matvec_per_s"], "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
             "dtype": "f32 operator / f64 residual", "data": "synthetic (Poisson arrivals, seed 20261205)",

@@ -590,9 +590,10 @@ struct icx_job {
   DBuf<int> d_conn, d_adj_ptr, d_slot;
   DBuf<double> d_xyz, d_ye, d_vol;
   MeshOp mop{};
-  // state of a submitted (batched) mixed solve
+  // state of a submitted (batched) solve: mixed form, or FP64 fused form
   struct Submitted {
     bool active = false;     // in the context's batch
+    int form = ICX_SOLVE_MIXED;
     int was = 0;             // stop code that triggered the pending replacement
     bool replacing = false;
     double tol = 0.0;
@@ -1627,21 +1628,52 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
 // (SolverState::iter, advanced by kernel B), so every chunk is the same launch sequence
 // and the host only launches chunks and polls.
 // FP64 fused form:
-static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info &info) {
+// FP64 fused form, in pieces shared by icx_solve and the batched path.
+static PcgPartials fused_parts(icx_job *j) {
+  return j->parts(elastic_fusedA_blocks<double>(j->g), pcg_stream_blocks(j->nvec()));
+}
+
+// Set-up of an FP64 fused solve: the Jacobi factor, x = 0, r = b, the partials of
+// iteration 0; st receives the state after the set-up (done = 1 at once for a zero
+// right-hand side).
+static icx_status fused_begin(icx_job *j, double tol, int maxit, SolverState &st) {
   icx_ctx *c = j->ctx;
   const long long N = j->nvec();
   FusedWork &w = j->fw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N));
   HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.invs.alloc(j->g.nn));
-  const PcgPartials P = j->parts(elastic_fusedA_blocks<double>(j->g), pcg_stream_blocks(N));
+  const PcgPartials P = fused_parts(j);
   launch_elastic_invdiag_node<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(double), S(j)));
   launch_pcg_setup<double>(N, j->d_state.p, j->d_b.p, w.r.p, w.invd.p, P, tol, maxit, 0.0, S(j));
+  return read_state(j, st);
+}
+
+// x += alpha p still owed at a stop of the fused form (none when the set-up converged)
+static void fused_flush(icx_job *j, const SolverState &st) {
+  if (st.done != 1) launch_pcg_flush(j->nvec(), j->d_state.p, j->d_u.p, j->fw.pa.p, j->fw.pb.p, S(j));
+}
+
+// info of a stopped fused solve (its state read after the flush); true if converged
+static bool fused_info(const SolverState &st, icx_solve_info &info) {
+  info.matvecs = st.done == 1 ? 0 : st.stop_iter;
+  info.fp64_matvecs = info.matvecs;
+  info.iterations = st.done == 3 ? st.stop_iter - 1 : (st.done == 1 ? 0 : st.stop_iter);
+  info.refinements = 0;
+  info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.rr / st.rhsNorm2) : 0.0;
+  return st.done == 1 || st.done == 3;
+}
+
+static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info &info) {
+  icx_ctx *c = j->ctx;
+  const long long N = j->nvec();
+  FusedWork &w = j->fw;
   SolverState st;
-  icx_status rc = read_state(j, st);
+  icx_status rc = fused_begin(j, tol, maxit, st);
   if (rc) return rc;
+  const PcgPartials P = fused_parts(j);
   const int every = poll_every(N);
   auto chunk = [&]() {
     for (int k = 0; k < every; ++k) {
@@ -1658,15 +1690,10 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
     }
   };
   if ((rc = run_to_stop(j, j->gr_fused, chunk, st))) return rc;
-  if (st.done != 1) launch_pcg_flush(N, j->d_state.p, j->d_u.p, w.pa.p, w.pb.p, S(j));
+  fused_flush(j, st);
   rc = read_state(j, st);
   if (rc) return rc;
-  const bool conv = st.done == 1 || st.done == 3;
-  info.matvecs = st.done == 1 ? 0 : st.stop_iter;
-  info.fp64_matvecs = info.matvecs;
-  info.iterations = st.done == 3 ? st.stop_iter - 1 : (st.done == 1 ? 0 : st.stop_iter);
-  info.relres = st.rhsNorm2 > 0 ? std::sqrt(st.rr / st.rhsNorm2) : 0.0;
-  return conv ? ICX_OK : ICX_ERR_NOCONV;
+  return fused_info(st, info) ? ICX_OK : ICX_ERR_NOCONV;
 }
 
 // FP32 inner PCG on the FP32 operator, FP64 x accumulation and FP64 residual
@@ -1858,20 +1885,25 @@ static icx_status solve_mixed(icx_job *j, double tol, int maxit, icx_solve_info 
 }
 
 // -------------------------------------------------------------------------------------
-// Batched mixed solves.  Submitted solves of any jobs advance together: every chunk is
-// `every` iterations of [kernel A for the hexa8 jobs, kernel A for the quad4 jobs,
-// kernel B for all jobs], one launch each (Batch tables in device memory), captured
-// once per job set as a HIP graph.  Each job's blocks compute exactly what its own
-// launches would (same tiles, same partial sums), so a job's result is bit-identical
-// alone and in any mix; a job whose solve has stopped exits at once in every launch.
-// After each chunk the states of all jobs are gathered with one copy; replacements
-// run per job (FP64 residual), finished jobs leave the set.
+// Batched solves.  Submitted solves of any jobs advance together: every chunk is `every`
+// iterations of [kernel A for the hexa8 jobs, kernel A for the quad4 jobs, kernel B for
+// all jobs], one launch each per precision (FP32 for mixed-form jobs, FP64 for
+// fused-form jobs; Batch tables in device memory), captured once per job set as a HIP
+// graph.  Each job's blocks compute exactly what its own launches would (same tiles,
+// same partial sums), so a job's result is bit-identical alone and in any mix; a job
+// whose solve has stopped exits at once in every launch.  After each chunk the states
+// of all jobs are gathered with one copy; mixed-form jobs run their replacements (FP64
+// residual), fused-form jobs their owed x step at the stop; finished jobs leave the set.
 // -------------------------------------------------------------------------------------
 struct BatchRun {
   std::vector<icx_job *> active;
   bool dirty = true;
   DBuf<OpArgs<float>> a3, a2;
   DBuf<BArgs<float>> ab;
+  DBuf<OpArgs<double>> d3, d2;  // fused-form (FP64) jobs
+  DBuf<BArgs<double>> db;
+  DBuf<int> offd3, offd2, offdb;
+  int nd3 = 0, nd2 = 0, ndb = 0, nbd3 = 0, nbd2 = 0, nbdb = 0;
   DBuf<SrArgs> as0, as1;  // single-reduction jobs, launch parity 0 / 1
   DBuf<int> off3, off2, offb, offs;
   DBuf<SolverState *> sptr;
@@ -1892,14 +1924,29 @@ static int pad8(int n) { return (n + 7) & ~7; }
 static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
   std::vector<OpArgs<float>> a3, a2;
   std::vector<BArgs<float>> ab;
+  std::vector<OpArgs<double>> d3, d2;
+  std::vector<BArgs<double>> db;
   std::vector<SrArgs> as0, as1;
-  std::vector<int> o3{0}, o2{0}, ob{0}, os{0};
+  std::vector<int> o3{0}, o2{0}, ob{0}, os{0}, od3{0}, od2{0}, odb{0};
   std::vector<SolverState *> sp;
   long long maxN = 0;
   for (icx_job *j : B.active) {
     MixedWork &w = j->mw;
     sp.push_back(j->d_state.p);
     maxN = std::max(maxN, j->nvec());
+    if (j->sub.form == ICX_SOLVE_FUSED) {
+      FusedWork &f = j->fw;
+      const PcgPartials P = fused_parts(j);
+      OpArgs<double> a;
+      const int nb = fusedA_args<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, f.r.p, f.invd.p, f.invs.p, f.pa.p,
+                                         f.pb.p, f.Ap.p, nullptr, j->d_u.p, j->d_state.p, P, a);
+      if (j->g.ndim == 3) { d3.push_back(a); od3.push_back(od3.back() + pad8(nb)); }
+      else { d2.push_back(a); od2.push_back(od2.back() + pad8(nb)); }
+      db.push_back(BArgs<double>{j->nvec(), j->d_state.p, f.r.p, f.Ap.p, f.invd.p, P, nullptr, SeamGeom{}, nullptr,
+                                 {0.0, 0.0, 0.0}, j->g.nn});
+      odb.push_back(odb.back() + pad8(P.nB));
+      continue;
+    }
     if (w.sr) {
       const SrVectors v = sr_vectors(j);
       as0.push_back(sr_args(j->g, j->op32, v, j->d_state.p, 0));
@@ -1926,6 +1973,8 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
   const int n = (int)B.active.size();
   B.n3 = (int)a3.size(); B.n2 = (int)a2.size(); B.nab = (int)ab.size(); B.ns = (int)as0.size();
   B.nb3 = o3.back(); B.nb2 = o2.back(); B.nbb = ob.back(); B.nbs = os.back();
+  B.nd3 = (int)d3.size(); B.nd2 = (int)d2.size(); B.ndb = (int)db.size();
+  B.nbd3 = od3.back(); B.nbd2 = od2.back(); B.nbdb = odb.back();
   B.every = poll_every(maxN);
   auto up = [&](auto &dbuf, const auto &vec) -> icx_status {
     if (vec.empty()) return ICX_OK;
@@ -1936,7 +1985,8 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
   icx_status rc;
   if ((rc = up(B.a3, a3)) || (rc = up(B.a2, a2)) || (rc = up(B.ab, ab)) || (rc = up(B.off3, o3)) ||
       (rc = up(B.off2, o2)) || (rc = up(B.offb, ob)) || (rc = up(B.sptr, sp)) || (rc = up(B.as0, as0)) ||
-      (rc = up(B.as1, as1)) || (rc = up(B.offs, os)))
+      (rc = up(B.as1, as1)) || (rc = up(B.offs, os)) || (rc = up(B.d3, d3)) || (rc = up(B.d2, d2)) ||
+      (rc = up(B.db, db)) || (rc = up(B.offd3, od3)) || (rc = up(B.offd2, od2)) || (rc = up(B.offdb, odb)))
     return rc;
   HIPCHK(c, B.sdev.alloc(std::max(n, 1)));
   if (B.h_cap < n) {
@@ -1959,6 +2009,9 @@ static void batch_chunk(icx_ctx *c, BatchRun &B) {
     launch_fusedA_batch<float>(3, B.a3.p, B.off3.p, B.n3, B.nb3, c->stream);
     launch_fusedA_batch<float>(2, B.a2.p, B.off2.p, B.n2, B.nb2, c->stream);
     launch_pcg_fusedB_batch<float>(B.ab.p, B.offb.p, B.nab, B.nbb, c->stream);
+    launch_fusedA_batch<double>(3, B.d3.p, B.offd3.p, B.nd3, B.nbd3, c->stream);
+    launch_fusedA_batch<double>(2, B.d2.p, B.offd2.p, B.nd2, B.nbd2, c->stream);
+    launch_pcg_fusedB_batch<double>(B.db.p, B.offdb.p, B.ndb, B.nbdb, c->stream);
   }
 }
 
@@ -1974,7 +2027,8 @@ static icx_status batch_states(icx_ctx *c, BatchRun &B) {
 static void batch_finish(icx_job *j, const SolverState &st, icx_status rc) {
   auto &b = j->sub;
   b.info = icx_solve_info{};
-  mixed_info(st, b.refinements, b.info);
+  if (b.form == ICX_SOLVE_FUSED) fused_info(st, b.info);
+  else mixed_info(st, b.refinements, b.info);
   b.info.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - b.t0).count();
   b.rc = rc;
   b.has_info = true;
@@ -3240,10 +3294,14 @@ icx_status icx_opt_get_design(const icx_job *jc, double *x) {
   return ICX_OK;
 }
 
-icx_status icx_submit(icx_job *j, double tol, int maxit) {
+icx_status icx_submit(icx_job *j, double tol, int maxit) { return icx_submit_form(j, ICX_SOLVE_MIXED, tol, maxit); }
+
+icx_status icx_submit_form(icx_job *j, int form, double tol, int maxit) {
   if (j && j->mesh) return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: %s is for box jobs only", "submitted solves");
   if (!j) return ICX_ERR_ARG;
   icx_ctx *c = j->ctx;
+  if (form != ICX_SOLVE_MIXED && form != ICX_SOLVE_FUSED)
+    return fail(c, ICX_ERR_ARG, "submitted solves: form ICX_SOLVE_MIXED or ICX_SOLVE_FUSED");
   if (j->numeq < 0 || !j->design_set)
     return fail(c, ICX_ERR_STATE, "icx_submit needs icx_make_dof and icx_set_design");
   if (!(tol > 0.0) || maxit < 1) return fail(c, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
@@ -3258,10 +3316,11 @@ icx_status icx_submit(icx_job *j, double tol, int maxit) {
   b.refinements = 0;
   b.replacing = false;
   b.has_info = false;
+  b.form = form;
   icx_status rc = build_rhs(j);
   if (rc) return rc;
   SolverState st;
-  rc = mixed_begin(j, tol, maxit, st, use_sr(j));
+  rc = form == ICX_SOLVE_FUSED ? fused_begin(j, tol, maxit, st) : mixed_begin(j, tol, maxit, st, use_sr(j));
   if (rc) return rc;
   b.active = true;
   c->batch->active.push_back(j);
@@ -3316,11 +3375,21 @@ icx_status icx_poll(icx_ctx *c, int chunks, icx_job **finished, int cap, int *nf
       }
       if ((rc = batch_states(c, B))) return rc;
     }
-    // stopped solves: converged at set-up, or a residual replacement is due
-    bool replaced = false;
+    // stopped solves: converged at set-up, a fused-form stop (the owed x step, then
+    // done), or a residual replacement is due
+    bool replaced = false, flushed = false;
     for (size_t q = 0; q < B.active.size(); ++q) {
       icx_job *j = B.active[q];
       const SolverState &st = B.h_states[q];
+      if (j->sub.form == ICX_SOLVE_FUSED) {
+        if (st.done) {
+          fused_flush(j, st);
+          flushed = true;
+          batch_finish(j, st, (st.done == 1 || st.done == 3) ? ICX_OK : ICX_ERR_NOCONV);
+          done.push_back(j);
+        }
+        continue;
+      }
       if (st.done == 1) {
         batch_finish(j, st, ICX_OK);
         done.push_back(j);
@@ -3348,6 +3417,7 @@ icx_status icx_poll(icx_ctx *c, int chunks, icx_job **finished, int cap, int *nf
         }
       }
     }
+    if (flushed) HIPCHK(c, hipStreamSynchronize(c->stream));  // x complete before the caller reads it
     for (icx_job *j : done) {
       auto it = std::find(B.active.begin(), B.active.end(), j);
       if (it != B.active.end()) B.active.erase(it);
@@ -3372,7 +3442,12 @@ icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, do
                            icx_solve_info *infos) {
   if (!ctx || (!jobs && njobs)) return ICX_ERR_ARG;
   icx_status worst = ICX_OK;
-  if (form != ICX_SOLVE_MIXED) {
+  // mixed and (box elastic) fused-form solves run batched; the plain form, and fused
+  // solves of mesh or heat jobs (which refuse it alone too), one after the other
+  bool batched = form == ICX_SOLVE_MIXED || form == ICX_SOLVE_FUSED;
+  for (int i = 0; i < njobs && form == ICX_SOLVE_FUSED; ++i)
+    if (jobs[i] && (jobs[i]->mesh || jobs[i]->heat())) batched = false;
+  if (!batched) {
     for (int i = 0; i < njobs; ++i) {
       icx_status rc = icx_solve(jobs[i], form, tol, maxit, nullptr, infos ? &infos[i] : nullptr);
       if (rc && rc != ICX_ERR_NOCONV) return rc;
@@ -3382,7 +3457,7 @@ icx_status icx_solve_batch(icx_ctx *ctx, icx_job **jobs, int njobs, int form, do
   }
   for (int i = 0; i < njobs; ++i) {
     if (!jobs[i] || jobs[i]->ctx != ctx) return fail(ctx, ICX_ERR_ARG, "job %d is not a job of this context", i);
-    icx_status rc = icx_submit(jobs[i], tol, maxit);
+    icx_status rc = icx_submit_form(jobs[i], form, tol, maxit);
     if (rc) return rc;
   }
   int left = njobs;

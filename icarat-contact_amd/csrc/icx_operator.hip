@@ -336,9 +336,9 @@ template <typename T, int TI, int TJ, int MODE>
 __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
   hex8_block<T, TI, TJ, MODE>(a, blockIdx.x);
 }
-// (FP32: four waves per SIMD, as the single-job kernel reaches)
+// (four waves per SIMD in FP32, three in FP64: what the single-job kernels reach)
 template <typename T, int TI, int TJ, int MODE>
-__global__ void __launch_bounds__((TI + 1) * (TJ + 1)) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__((TI + 1) * (TJ + 1)) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
     k_hex8_batch(const OpArgs<T> *__restrict__ args, const int *__restrict__ off, int njobs) {
   int j, blk;
   batch_block(off, njobs, j, blk);
@@ -600,6 +600,10 @@ template int fusedA_args<float>(const Grid &, const ElasticOp<float> &, const fl
 template SeamGeom fusedA_seams<float>(const OpArgs<float> &);
 template SeamGeom fusedA_seams<double>(const OpArgs<double> &);
 template void launch_fusedA_batch<float>(int, const OpArgs<float> *, const int *, int, int, hipStream_t);
+template void launch_fusedA_batch<double>(int, const OpArgs<double> *, const int *, int, int, hipStream_t);
+template int fusedA_args<double>(const Grid &, const ElasticOp<double> &, const double *, const uint8_t *,
+                                 const double *, const double *, const double *, double *, double *, double *,
+                                 double *, double *, SolverState *, PcgPartials, OpArgs<double> &);
 
 template int elastic_fusedA_blocks<double>(const Grid &);
 template int elastic_fusedA_blocks<float>(const Grid &);

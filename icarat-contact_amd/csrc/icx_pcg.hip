@@ -427,6 +427,7 @@ void launch_pcg_fusedB_batch(const BArgs<T> *args, const int *off, int njobs, in
   ICX_CHECK_LAUNCH();
 }
 template void launch_pcg_fusedB_batch<float>(const BArgs<float> *, const int *, int, int, hipStream_t);
+template void launch_pcg_fusedB_batch<double>(const BArgs<double> *, const int *, int, int, hipStream_t);
 
 #define ICX_INST_PCG(T)                                                                                  \
   template void launch_pcg_setup<T>(long long, SolverState *, const T *, T *, const T *, PcgPartials,    \

@@ -32,7 +32,7 @@ EXPORTS = [
     "icx_gcmma_reset", "icx_gcmma_outer_update", "icx_gcmma_inner_update", "icx_gcmma_concheck",
     "icx_mma_set_asymptotes", "icx_stress_constraint",
     "icx_opt_init", "icx_opt_step", "icx_opt_step_begin", "icx_opt_step_end", "icx_opt_get_design",
-    "icx_submit", "icx_poll", "icx_job_last_info", "icx_cancel", "icx_solve_batch", "icx_job_footprint",
+    "icx_submit", "icx_submit_form", "icx_poll", "icx_job_last_info", "icx_cancel", "icx_solve_batch", "icx_job_footprint",
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
 ]
 OPT_GRAPHS, OPT_TILE_DEPTH, OPT_PIPELINE, OPT_SINGLE_REDUCTION, OPT_HEAT_DENSE, OPT_MMA_HOST_DUAL = 1, 2, 3, 4, 5, 6
@@ -527,9 +527,13 @@ class Job:
         self.ctx.check(lib().icx_opt_step_end(self.h, C.byref(si) if info else None, C.byref(r)))
         return r.as_dict()
 
-    def submit(self, tol=1e-8, maxit=50000):
-        """Start a batched mixed-form solve (icx_submit); it advances in Context.poll."""
-        self.ctx.check(lib().icx_submit(self.h, C.c_double(tol), int(maxit)))
+    def submit(self, tol=1e-8, maxit=50000, form=None):
+        """Start a batched solve (icx_submit: mixed form; icx_submit_form with form
+        SOLVE_MIXED or SOLVE_FUSED); it advances in Context.poll."""
+        if form is None:
+            self.ctx.check(lib().icx_submit(self.h, C.c_double(tol), int(maxit)))
+        else:
+            self.ctx.check(lib().icx_submit_form(self.h, int(form), C.c_double(tol), int(maxit)))
 
     def cancel(self):
         """icx_cancel: stop this job's submitted solve"""

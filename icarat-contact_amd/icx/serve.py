@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-from . import HEXA8, OPT_SINGLE_REDUCTION, PLANE_STRESS, QUAD4, SOLID3D, SOLVE_MIXED, IcxError, Job
+from . import HEXA8, OPT_SINGLE_REDUCTION, PLANE_STRESS, QUAD4, SOLID3D, SOLVE_FUSED, SOLVE_MIXED, IcxError, Job
 
 SEED = 20261205
 
@@ -35,17 +35,36 @@ def job_spec(rng, elems_lo, elems_hi):
     return (3, (d0, d1, d1))
 
 
-def footprint_estimate(ndim, div, single_reduction=False):
-    """Device bytes of one served job (mixed solve, cone filter, OC design loop):
-    84 B per DOF, 92 B per element, 5 B per node (constraint mask and the FP32 inverse
-    modulus sums), 432 KB of partial sums and state; a hexa8 job on the
-    single-reduction iteration (ICX_OPT_SINGLE_REDUCTION) holds one more FP32 vector
-    (88 B per DOF) and 0.375 MB more partial sums.  icx_job_footprint reports the same
-    buffers once they exist."""
+def solve_form(ndim):
+    """Solve form of a served job: the mixed form for hexa8 jobs, the FP64 fused form for
+    quad4 jobs.  A quad4 cantilever of n elements along its length has a Jacobi-scaled
+    condition number of order n^2 (6.4e5 at 120 x 40), so FP32 inner iterations lose
+    orthogonality: they need about twice the FP64 iteration count at 120 x 40 and
+    stall short of 1e-8 from about 840 x 280 up, while the FP64 form converges in
+    thousands of iterations; 2D jobs are also too small for the mixed form's halved
+    bytes to pay.  Hexa8 jobs of the same element counts are 10 times shorter, and the
+    mixed form needs about the FP64 iteration count there."""
+    return SOLVE_FUSED if ndim == 2 else SOLVE_MIXED
+
+
+def footprint_estimate(ndim, div, single_reduction=False, form=None):
+    """Device bytes of one served job (cone filter, OC design loop), by its solve form
+    (default: solve_form(ndim)).  Mixed form: 84 B per DOF, 92 B per element, 5 B per
+    node (constraint mask and the FP32 inverse modulus sums), 432 KB of partial sums
+    and state; a hexa8 job on the single-reduction iteration (ICX_OPT_SINGLE_REDUCTION)
+    holds one more FP32 vector (88 B per DOF) and 0.375 MB more partial sums.  FP64
+    fused form: the same 84 B per DOF (five FP64 vectors in place of the mixed form's
+    eight FP32 and one FP64), 88 B per element (no FP32 copy of the moduli), 9 B per
+    node (FP64 inverse modulus sums).  icx_job_footprint reports the same buffers once
+    they exist."""
+    if form is None:
+        form = solve_form(ndim)
     d0, d1, d2 = div
     nn = (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
     ne = d0 * d1 * (d2 if ndim == 3 else 1)
     n = ndim * nn
+    if form == SOLVE_FUSED:
+        return 84 * n + 88 * ne + 9 * nn + 432 * 1024
     if ndim == 3 and single_reduction:
         return 88 * n + 92 * ne + 5 * nn + 816 * 1024
     return 84 * n + 92 * ne + 5 * nn + 432 * 1024
@@ -95,9 +114,11 @@ def make_job(ctx, ndim, div):
 
 def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, seed=SEED,
         tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None, limit=None,
-        cancel=None, max_resident_dofs=None, offset=0):
+        cancel=None, max_resident_dofs=None, offset=0, forms="auto"):
     """Serve n_jobs jobs arriving at `rate` per second (exponential gaps), each running
     `iters` OC design iterations.
+    forms: "auto" (solve_form: FP64 fused form for quad4 jobs, mixed form for hexa8
+      jobs), or one form (SOLVE_MIXED / SOLVE_FUSED) for every job.
     specs: optional list of requests instead of the drawn sizes -- (ndim, div)
       cantilevers or job-file texts (TOML, the reference's keys; x0 = its designs_0); a
       request the library refuses is recorded under "rejected" and the loop goes on.
@@ -128,11 +149,13 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         specs = specs[:limit]
     n_jobs = len(specs)
     sr = bool(getattr(ctx, "options", {}).get(OPT_SINGLE_REDUCTION, 0))
+
+    def form_of(ndim):
+        return solve_form(ndim) if forms == "auto" else forms
+
     def estimate(spec):
-        if isinstance(spec, str):
-            m = toml_mesh(spec)
-            return footprint_estimate(*m, single_reduction=sr) if m else 0.0
-        return footprint_estimate(*spec, single_reduction=sr)
+        m = toml_mesh(spec) if isinstance(spec, str) else spec
+        return footprint_estimate(*m, single_reduction=sr, form=form_of(m[0])) if m else 0.0
 
     est = [estimate(s) for s in specs]
 
@@ -151,6 +174,9 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
     pending = collections.deque(range(n_jobs))
     active = {}
     lat, admit_wait, footprints = [], [], []
+    # one record per design iteration: (job, step, DOFs, PCG iterations, FP64
+    # replacements, converged, latency ms, resident jobs, resident DOFs, solve form)
+    records = []
     designs, histories = {}, {}
     dof_matvecs = 0
     used = 0.0
@@ -168,7 +194,7 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         tb = time.perf_counter()
         rec["t_begin"] = tb
         rec["job"].opt_step_begin()
-        rec["job"].submit(tol=tol, maxit=maxit)
+        rec["job"].submit(tol=tol, maxit=maxit, form=rec["form"])
         split["step_begin_submit"] += time.perf_counter() - tb
 
     while pending or active:
@@ -183,10 +209,11 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
                 rejected.append((i, str(e)))
                 finished_jobs += 1
                 continue
+            form = form_of(job.ndim)
             job.opt_init(np.full(job.nelx, x0 if x_init is None else x_init), driver=0, radius=radius,
-                         projection=True, form=SOLVE_MIXED, tol=tol, maxit=maxit)
+                         projection=True, form=form, tol=tol, maxit=maxit)
             split["admit"] += time.perf_counter() - ta
-            rec = {"job": job, "spec": i, "step": 0, "numeq": job.numeq}
+            rec = {"job": job, "spec": i, "step": 0, "numeq": job.numeq, "form": form}
             active[id(job)] = rec
             used += est[i]
             resident += ndofs[i]
@@ -218,6 +245,8 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
                 if keep_designs:
                     histories.setdefault(rec["spec"], []).append(step_rec["objective"])
                 lat.append(time.perf_counter() - rec["t_begin"])
+                records.append((rec["spec"], rec["step"], rec["numeq"], info["iterations"], info["refinements"],
+                                bool(info["converged"]), lat[-1] * 1e3, len(active), resident, rec["form"]))
                 dof_matvecs += rec["numeq"] * info["matvecs"]
                 rec["step"] += 1
                 if rec["step"] < iters:
@@ -255,6 +284,8 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         "wall_split_s": split,
         "offset": offset,
         "latencies_ms": lat_ms.tolist(),
+        "iteration_records": records,
+        "unconverged_solves": sum(1 for r in records if not r[5]),
         "admission_wait_ms": (np.array(admit_wait) * 1e3).tolist(),
         "designs": designs if keep_designs else None,
         "objectives": histories if keep_designs else None,

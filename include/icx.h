@@ -343,20 +343,28 @@ icx_status icx_opt_get_design(const icx_job *job, double *x);
 
 /* ---- batched / serving -------------------------------------------------------------
  * No reference counterpart: the reference runs one process per job.  Submitted
- * mixed-form solves of any jobs of a context advance together: each launch of a PCG
- * kernel phase covers every job whose solve is running, so jobs of uneven sizes share
- * the device; each job's result is bit-identical to its own icx_solve(MIXED).
+ * solves of any jobs of a context advance together: each launch of a PCG kernel phase
+ * covers every job whose solve is running in that precision, so jobs of uneven sizes
+ * share the device; each job's result is bit-identical to its own icx_solve of the
+ * same form.  The solve is the reference's Jacobi-PCG (solvers.cpp:64-92) in the mixed
+ * or the fused FP64 form.
  *
  * icx_submit: start a mixed solve of the job's current design and load (set-up runs
  *   at once; the iterations run in icx_poll).  One submitted solve per job at a time.
+ * icx_submit_form: the same with the form given: ICX_SOLVE_MIXED, or ICX_SOLVE_FUSED
+ *   (FP64 throughout; the form for ill-conditioned jobs such as long quad4 grids, on
+ *   which FP32 inner iterations need several times the FP64 iteration count or
+ *   stall).  ICX_ERR_ARG for another form.
  * icx_poll: run up to `chunks` chunks of iterations of all submitted solves (with
  *   their FP64 residual replacements), returning as soon as at least one solve has
  *   finished; the finished jobs are written to finished[0 .. *nfinished).
  * icx_job_last_info: the info and status (ICX_OK / ICX_ERR_NOCONV) of the job's last
  *   finished submitted solve; its displacement is then the job's current one.
- * icx_solve_batch: submit all, poll until all have finished (form ICX_SOLVE_MIXED;
- *   other forms solve the jobs one after the other). */
+ * icx_solve_batch: submit all, poll until all have finished (forms ICX_SOLVE_MIXED and
+ *   ICX_SOLVE_FUSED; the plain form, and fused solves of mesh or heat jobs, solve the
+ *   jobs one after the other). */
 icx_status icx_submit(icx_job *job, double tol, int maxit);
+icx_status icx_submit_form(icx_job *job, int form, double tol, int maxit);
 icx_status icx_poll(icx_ctx *ctx, int chunks, icx_job **finished, int cap, int *nfinished);
 icx_status icx_job_last_info(const icx_job *job, icx_solve_info *info);
 /* icx_cancel: stop the job's submitted solve and take it out of the batch (the other

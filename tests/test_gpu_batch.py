@@ -83,6 +83,61 @@ def test_batch_zero_load_and_iteration_limit(ctx):
         j.close()
 
 
+def test_fused_batch_equals_alone(ctx):
+    """FP64 fused-form solves (icx_submit_form / icx_solve_batch with SOLVE_FUSED) of
+    quad4 and hexa8 jobs: each equals its own icx_solve(FUSED) bit for bit."""
+    jobs = _jobs(ctx, 41, 6)
+    ref = []
+    for j in jobs:
+        u, info = j.solve(icx.SOLVE_FUSED, tol=1e-9, maxit=50000)
+        assert info["converged"]
+        ref.append((u, info))
+    infos = ctx.solve_batch(jobs, form=icx.SOLVE_FUSED, tol=1e-9)
+    for j, (u0, i0), i1 in zip(jobs, ref, infos):
+        for k in ("iterations", "matvecs", "relres"):
+            assert i1[k] == i0[k], k
+        assert np.array_equal(j.displacement(), u0)
+    for j in jobs:
+        j.close()
+
+
+def test_mixed_and_fused_solves_in_one_batch(ctx):
+    """Both forms in one batch, submitted at different times, with an iteration-limit
+    stop and a zero load among the fused ones: every job equals its solo solve."""
+    rng = np.random.default_rng(43)
+    jobs = [icx.random_job(ctx, rng, size2d=(20, 90), size3d=(6, 20)) for _ in range(8)]
+    forms = [icx.SOLVE_FUSED if q % 2 == 0 else icx.SOLVE_MIXED for q in range(len(jobs))]
+    jobs[2].set_fext(np.zeros(jobs[2].numeq))
+    limits = [50000] * len(jobs)
+    limits[4] = 7
+    ref = [j.solve(f, tol=1e-8, maxit=m) for j, f, m in zip(jobs, forms, limits)]
+    pending = list(range(len(jobs)))
+    running, finished = set(), {}
+    polls = 0
+    while pending or running:
+        if pending and polls % 2 == 0:
+            q = pending.pop(0)
+            jobs[q].submit(tol=1e-8, maxit=limits[q], form=forms[q])
+            running.add(q)
+        for j in ctx.poll(chunks=1):
+            q = jobs.index(j)
+            finished[q] = (j.displacement(), j.last_info())
+            running.discard(q)
+        polls += 1
+        assert polls < 100000
+    for q, (u0, i0) in enumerate(ref):
+        u1, i1 = finished[q]
+        for k in ("iterations", "converged", "relres", "refinements"):
+            assert i1[k] == i0[k], (q, k)
+        assert np.array_equal(u1, u0), q
+    assert not finished[4][1]["converged"] and finished[4][1]["iterations"] == 7
+    assert finished[2][1]["iterations"] == 0 and not np.any(finished[2][0])
+    with pytest.raises(icx.IcxError):
+        jobs[0].submit(tol=1e-8, form=icx.SOLVE_PLAIN)
+    for j in jobs:
+        j.close()
+
+
 def test_design_step_halves_equal_whole_step(ctx):
     """icx_opt_step == icx_opt_step_begin + submitted solve + icx_opt_step_end."""
     x0 = np.full(30 * 10, 0.5)

@@ -137,7 +137,7 @@ def test_cancel_a_served_job(ctx):
     for i in (0, 2):
         ndim, div = specs[i]
         j = serve.make_job(ctx, ndim, div)
-        j.opt_init(np.full(j.nelx, 0.4), driver=0, radius=1.5, projection=True, form=icx.SOLVE_MIXED,
+        j.opt_init(np.full(j.nelx, 0.4), driver=0, radius=1.5, projection=True, form=serve.solve_form(ndim),
                    tol=1e-8, maxit=50000)
         for _ in range(3):
             j.opt_step()

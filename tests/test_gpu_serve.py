@@ -11,10 +11,10 @@ pytestmark = pytest.mark.gpu
 SPECS = [(2, (30, 10, 1)), (3, (10, 5, 5)), (2, (24, 8, 1)), (3, (8, 4, 4))]
 
 
-def _alone(ctx, ndim, div, iters, tol):
+def _alone(ctx, ndim, div, iters, tol, form=None):
     j = serve.make_job(ctx, ndim, div)
     j.opt_init(np.full(j.nelx, 0.4), driver=0, radius=1.5, projection=True,
-               form=icx.SOLVE_MIXED, tol=tol, maxit=50000)
+               form=serve.solve_form(ndim) if form is None else form, tol=tol, maxit=50000)
     for _ in range(iters):
         j.opt_step()
     x = j.opt_design()
@@ -23,12 +23,27 @@ def _alone(ctx, ndim, div, iters, tol):
 
 
 def test_served_designs_equal_alone(ctx):
+    """quad4 jobs are served in the FP64 fused form and hexa8 jobs in the mixed form,
+    in one batch; every design equals its solo run bit for bit."""
     iters, tol = 3, 1e-8
     out = serve.run(ctx, rate=200.0, iters=iters, specs=SPECS, tol=tol, keep_designs=True)
     assert out["design_iterations"] == iters * len(SPECS)
     assert out["iteration_ms_p99"] >= out["iteration_ms_p50"] > 0
+    assert out["unconverged_solves"] == 0
+    forms = {r[0]: r[9] for r in out["iteration_records"]}
+    assert forms == {i: serve.solve_form(nd) for i, (nd, _) in enumerate(SPECS)}
+    assert set(forms.values()) == {icx.SOLVE_FUSED, icx.SOLVE_MIXED}
     for i, (ndim, div) in enumerate(SPECS):
         assert np.array_equal(out["designs"][i], _alone(ctx, ndim, div, iters, tol)), i
+
+
+def test_served_with_one_form_for_all(ctx):
+    """forms=SOLVE_MIXED serves every job in the mixed form (quad4 included)."""
+    iters, tol = 2, 1e-8
+    out = serve.run(ctx, rate=200.0, iters=iters, specs=SPECS, tol=tol, keep_designs=True, forms=icx.SOLVE_MIXED)
+    assert {r[9] for r in out["iteration_records"]} == {icx.SOLVE_MIXED}
+    for i, (ndim, div) in enumerate(SPECS):
+        assert np.array_equal(out["designs"][i], _alone(ctx, ndim, div, iters, tol, icx.SOLVE_MIXED)), i
 
 
 def test_admission_by_footprint(ctx):
@@ -70,7 +85,7 @@ young = [0.001, 1.0]
 def test_served_toml_job_equals_the_cxx_driver(ctx, tmp_path):
     """A job given as the text of a reference job file is served (icx_job_create_toml,
     the front end the C++ drivers use) and follows the C++ driver A on the same file step
-    by step (same mixed-form solves at the reference's 1e-10, cone r = 3, projection);
+    by step (same FP64 fused-form solves at the reference's 1e-10, cone r = 3, projection);
     a malformed request is refused and the stream goes on."""
     import os
     import subprocess
@@ -84,7 +99,7 @@ def test_served_toml_job_equals_the_cxx_driver(ctx, tmp_path):
     f.write_text(TOML_Q4)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "icarat-contact_amd", "_build", "opt_macro_oc")
-    res = subprocess.run([exe, str(f), "HIP_PCG_MIXED", str(steps), "3.0"], cwd=tmp_path, capture_output=True,
+    res = subprocess.run([exe, str(f), "HIP_PCG_FUSED", str(steps), "3.0"], cwd=tmp_path, capture_output=True,
                          text=True, timeout=300)
     assert res.returncode == 0, res.stderr[-2000:]
     rows = np.loadtxt(tmp_path / "function.csv", delimiter=",", ndmin=2)
