@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--serve-offset", type=int, default=0,
                     help="start at job N of the stream (with --serve-limit: one segment of it)")
     ap.add_argument("--serve-dump", default=None, help="write the latencies of the run to this .npz file")
+    ap.add_argument("--serve-combine", nargs="+", default=None, metavar="STEM",
+                    help="combine segments of one stream: STEM.json (the segment's serve line) and STEM.npz "
+                         "(its --serve-dump) for each segment; no GPU")
     ap.add_argument("--serve-max-dofs", type=float, default=None,
                     help="admit jobs while the resident DOFs stay within this many")
     ap.add_argument("--pmc-iters", type=int, default=200)
@@ -641,7 +644,22 @@ def serving(args):
     lat = out.pop("latencies_ms")
     waits = out.pop("admission_wait_ms")
     recs = np.array(out.pop("iteration_records"), dtype=float).reshape(-1, 10)
-    # latency by job size (free DOFs): where the tail comes from
+    out["latency_by_dofs"] = latency_by_dofs(recs)  # where the tail comes from
+    if args.serve_dump:  # every design iteration's latency and every admission wait, for
+        # combining the segments of a stream served in parts
+        np.savez(args.serve_dump, latencies_ms=np.array(lat), admission_wait_ms=np.array(waits), records=recs)
+    line = {"metric": "serving: design-iteration latency p50/p99 and DOF-matvec/s",
+            "value": out["dof_matvec_per_s"], "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "hexa8 jobs: f32 operator / f64 residual; quad4 jobs: f64",
+            "data": "synthetic (Poisson arrivals, seed 20261205)",
+            "config": out}
+    print(json.dumps(line))
+
+
+def latency_by_dofs(recs):
+    """Design-iteration latency and PCG iterations by job size (free DOFs), from serving
+    records (job, step, DOFs, PCG iterations, replacements, converged, latency ms, ...)."""
+    import numpy as np
     by_size = {}
     for lo, hi in ((0, 1e5), (1e5, 1e6), (1e6, 1e12)):
         sel = recs[(recs[:, 2] >= lo) & (recs[:, 2] < hi)]
@@ -651,14 +669,40 @@ def serving(args):
                 "ms_p99": float(np.percentile(sel[:, 6], 99)),
                 "pcg_iterations_p50": float(np.percentile(sel[:, 3], 50)),
                 "pcg_iterations_max": int(sel[:, 3].max())}
-    out["latency_by_dofs"] = by_size
-    if args.serve_dump:  # every design iteration's latency and every admission wait, for
-        # combining the segments of a stream served in parts
-        np.savez(args.serve_dump, latencies_ms=np.array(lat), admission_wait_ms=np.array(waits), records=recs)
+    return by_size
+
+
+def serve_combine(stems):
+    """One serve line for a stream served in segments (bench.py --serve --serve-offset K
+    --serve-limit L --serve-dump STEM.npz > STEM.json, one segment per sitting, each
+    starting empty): wall times and DOF-matvecs add up, the percentiles are over every
+    design iteration and admission of all segments."""
+    import numpy as np
+    lines = [json.loads(open(s + ".json").read().strip().splitlines()[-1]) for s in stems]
+    dumps = [np.load(s + ".npz") for s in stems]
+    wall = sum(l["config"]["wall_s"] for l in lines)
+    dmv = sum(l["value"] * l["config"]["wall_s"] for l in lines)
+    lat = np.concatenate([d["latencies_ms"] for d in dumps])
+    waits = np.concatenate([d["admission_wait_ms"] for d in dumps])
+    recs = np.concatenate([d["records"] for d in dumps])
+    segs = [{"offset": l["config"]["offset"], "jobs": l["config"]["jobs"], "wall_s": l["config"]["wall_s"],
+             "iteration_ms_p50": l["config"]["iteration_ms_p50"], "iteration_ms_p99": l["config"]["iteration_ms_p99"],
+             "unconverged_solves": l["config"].get("unconverged_solves")} for l in lines]
+    c0 = lines[0]["config"]
     line = {"metric": "serving: design-iteration latency p50/p99 and DOF-matvec/s",
-            "value": out["dof_matvec_per_s"], "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
-            "dtype": "f32 operator / f64 residual", "data": "synthetic (Poisson arrivals, seed 20261205)",
-            "config": out}
+            "value": dmv / wall, "unit": "DOF-matvec/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": lines[0]["dtype"], "data": lines[0]["data"],
+            "config": {"workload": "served in %d segments, each starting empty" % len(lines),
+                       "jobs": int(sum(s["jobs"] for s in segs)), "design_iterations": int(len(lat)),
+                       "wall_s": wall, "iteration_ms_p50": float(np.percentile(lat, 50)),
+                       "iteration_ms_p99": float(np.percentile(lat, 99)),
+                       "admission_wait_ms_p50": float(np.percentile(waits, 50)),
+                       "admission_wait_ms_p99": float(np.percentile(waits, 99)),
+                       "unconverged_solves": int((recs[:, 5] == 0).sum()),
+                       "latency_by_dofs": latency_by_dofs(recs),
+                       "iterations_per_job": c0["iterations_per_job"], "elements": c0["elements"],
+                       "arrival_rate_per_s": c0["arrival_rate_per_s"], "budget_bytes": c0["budget_bytes"],
+                       "max_resident_dofs": c0["max_resident_dofs"], "tol": c0["tol"], "segments": segs}}
     print(json.dumps(line))
 
 
@@ -766,6 +810,9 @@ def main():
         return
     if args.batched:
         batched(args)
+        return
+    if args.serve_combine:
+        serve_combine(args.serve_combine)
         return
     if args.serve:
         serving(args)

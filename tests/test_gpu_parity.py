@@ -88,13 +88,20 @@ def _solve_case(ctx, nx=12, ny=6, nz=6, seed=5):
     return job, pr, rho, ud
 
 
-def test_plain_form_follows_eigen_loop(ctx):
-    job, pr, rho, ud = _solve_case(ctx)
-    u, info = job.solve(icx.SOLVE_PLAIN, tol=1e-10)
-    uo, ito, erro = pr.cg(pr.assemble(rho), tol=1e-10)
+@pytest.mark.parametrize("tol", [1e-10, 1e-13])
+@pytest.mark.parametrize("dims", [(12, 6, 6, 5), (20, 10, 10, 7), (17, 19, 9, 3)])
+def test_plain_form_follows_eigen_loop(ctx, dims, tol):
+    """The plain form is Eigen's loop in FP64; only the summation order of the dot
+    products and of the operator differs from the oracle's, so the iteration counts agree
+    within one and the iterates within rounding (measured: 0-1 iterations apart,
+    |du|/|u| <= 8e-14)."""
+    nx, ny, nz, seed = dims
+    job, pr, rho, ud = _solve_case(ctx, nx, ny, nz, seed)
+    u, info = job.solve(icx.SOLVE_PLAIN, tol=tol, maxit=100000)
+    uo, ito, erro = pr.cg(pr.assemble(rho), tol=tol, maxit=100000)
     assert info["converged"]
-    assert abs(info["iterations"] - ito) <= max(3, int(0.01 * ito))
-    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-8
+    assert abs(info["iterations"] - ito) <= 1
+    assert np.linalg.norm(u - uo) / np.linalg.norm(uo) < 1e-12
 
 
 @pytest.mark.parametrize("form", [icx.SOLVE_PLAIN, icx.SOLVE_FUSED])
