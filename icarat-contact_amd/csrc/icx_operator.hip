@@ -345,68 +345,146 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) __attribute__((amdgpu_wav
   hex8_block<T, TI, TJ, MODE>(args[j], blk);
 }
 
-// quad4: a block owns TI nodes along x and TK node rows, marching along y (same
-// scheme as hexa8 in one dimension less).
+// quad4: a block of TI + 1 threads owns TI nodes along x and the node rows [j0, j1),
+// marching along y with the pipeline of the hexa8 kernel in one dimension less: node
+// rows rotate through three LDS buffers; every global load is issued one row step
+// before its use (node row ey+2, the moduli of element row ey+1, the owned node's x and
+// constraint mask of row ey+1) and the loop is unrolled by two with alternating
+// register sets; the outputs of a node row are stored one step late, beside the next
+// step's loads, and one explicit wait per step covers both.  Per row step every thread
+// applies one element (ex = i0 - 1 + tid); node tid adds its element's right corners
+// and the left corners of element tid + 1 (through LDS) to the carry of the row below,
+// in a fixed order.  The block applies the element rows j0-1 .. j1-1, one of them also
+// applied by the block below.
+// MODE 0: y = K x.  MODE 1 (fused kernel A of PCG iteration i): p_i = M^-1 r + beta_i
+// p_{i-1} formed in the LDS loader (M^-1 from the per-node inverse modulus sum),
+// y = K p_i, x += alpha_{i-1} p_{i-1} on the owned nodes (p_{i-1} kept in LDS beside
+// p_i), partial p_i . y; the first node row's loads go out before the iteration scalars
+// are reduced, from both direction buffers.
 template <typename T, int TI, int MODE>
 __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   constexpr int NT = TI + 1;
   constexpr int LW = TI + 2;
+  constexpr int NPOS = (LW + NT - 1) / NT;
+  constexpr uint32_t W = sizeof(T);
+  static_assert(NT % 64 == 0, "whole waves");
   __shared__ T s_in[3][2][LW];
-  __shared__ T s_out[8][NT];
-  __shared__ double s_red[(NT + 63) / 64];
-  __shared__ double s_sc[2 * ((NT + 63) / 64)];
+  __shared__ T s_po[MODE == 1 ? 3 : 1][2][MODE == 1 ? LW : 1];
+  __shared__ T s_h[4][NT];  // left corners (qa = 0) of each element, [2 qb + c]
+  __shared__ double s_red[NT / 64];
+  __shared__ double s_sc[2 * (NT / 64)];
   const Grid &g = a.g;
   SolverState ss{};
   if (a.st) ss = load_state(a.st);
-  if (ss.done) return;
   const int nblocks = a.tiles_x * a.tiles_y;
   if (blk >= nblocks) return;  // padding block of a batched launch
   int it = 0;
   T beta = T(0), alpha_x = T(0);
   const T *pold = nullptr;
   T *pnew = nullptr;
-  if (MODE == 1) {
-    it = ss.iter;
-    pold = (it & 1) ? a.P0 : a.P1;
-    pnew = (it & 1) ? a.P1 : a.P0;
-    double rr, rz;
-    block_total2<((NT + 63) / 64) * 64>(a.P.PB, a.P.nB, s_sc, rr, rz);
-    if (!fused_scalars<T>(a, ss, it, blk, rr, rz, beta, alpha_x)) return;
-  }
   const int t = xcd_tile(blk, nblocks);
   const int ti = t % a.tiles_x, tj = t / a.tiles_x;
   const int i0 = ti * TI, j0 = tj * a.TK;
   const int j1 = min(j0 + a.TK, g.sy);
+  const int kfirst = j0 - 1;  // first node row and first element row
   const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ex = i0 - 1 + tid;
   const bool ex_ok = ex >= 0 && ex < g.nx;
+  const uint32_t e_off = ex_ok ? (uint32_t)ex * W : 0u;
+  const uint32_t e_row = (uint32_t)g.nx * W;
   const bool node_thread = tid < TI;
   const int ni = i0 + tid;
   const bool node_ok = node_thread && ni < g.sx;
-  const long long nn = g.nn;
-  auto load_row = [&](int j, int buf) {
-    for (int idx = tid; idx < LW; idx += NT) {
-      const int gi = i0 - 1 + idx;
-      const bool ok = gi >= 0 && gi < g.sx && j >= 0 && j < g.sy;
-      const long long n = ok ? gi + (long long)g.sx * j : 0;
+  const uint32_t cstr = (uint32_t)g.nn * W;  // component stride in bytes
+  const uint32_t n_row = (uint32_t)g.sx * W;
+  const uint32_t n_off = node_ok ? (uint32_t)ni * W : 0u;
+  // loader positions; nodes outside the grid are loaded from offset 0 (corners of
+  // elements outside the grid only, whose modulus is zero)
+  uint32_t pos_off[NPOS];
+#pragma unroll
+  for (int p = 0; p < NPOS; ++p) {
+    const int idx = tid + p * NT;
+    const int gi = i0 - 1 + idx;
+    pos_off[p] = (idx < LW && gi >= 0 && gi < g.sx) ? (uint32_t)gi * W : 0u;
+  }
+  auto pos_live = [&](int p) { return p * NT + wave * 64 < LW; };
+  T pr[NPOS][2], pv[NPOS], po[NPOS][2];
+  auto prefetch = [&](int k) {
+    const uint32_t lo = (k >= 0 && k < g.sy) ? (uint32_t)k * n_row : 0u;
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p) {
+      if (!pos_live(p)) continue;
+      const uint32_t o = pos_off[p] + lo;
+      if (MODE == 1) pv[p] = ld_off(a.invs, o);
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
-        T v = T(0);
-        if (ok) {
-          if (MODE == 0) v = a.in[c * nn + n];
-          else v = fma_t(beta, pold[c * nn + n], a.invd[c * nn + n] * a.in[c * nn + n]);
-        }
-        s_in[buf][c][idx] = v;
+        pr[p][c] = ld_off(a.in, o + c * cstr);
+        if (MODE == 1) po[p][c] = ld_off(pold, o + c * cstr);
       }
+    }
+  };
+  auto commit = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p) {
+      if (!pos_live(p)) continue;
+      const int idx = tid + p * NT;
+      if (idx < LW) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          s_in[buf][c][idx] =
+              MODE == 0 ? pr[p][c] : fma_t(beta, po[p][c], jacobi_inv(pv[p], a.op.ikd[c]) * pr[p][c]);
+          if (MODE == 1) s_po[buf][c][idx] = po[p][c];
+        }
+      }
+    }
+  };
+  auto load_E = [&](int ey) -> T {
+    const uint32_t lo = (ey >= 0 && ey < g.ny) ? (uint32_t)ey * e_row : 0u;
+    return ld_off(a.E, e_off + lo);
+  };
+  auto load_node = [&](int k, OwnedNode<T> &o) {
+    const bool in = k >= j0 && k < j1;
+    const uint32_t n = n_off + (in ? (uint32_t)k * n_row : 0u);
+    o.m = a.mask ? (uint32_t)ld_off(a.mask, n / W) : 0u;
+    if (MODE == 1) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) o.x[c] = ld_off(a.x, n + c * cstr);
     }
   };
   double dot = 0.0;
   T acc_top[2] = {T(0), T(0)};
-  load_row(j0 - 1, 0);
-  for (int ey = j0 - 1, tt = 0; ey < j1; ++ey, ++tt) {
+  T yv[2] = {T(0), T(0)};
+  bool have_out = false;
+  auto store_out = [&](int ey, int buf, const OwnedNode<T> &o) {
+    if (have_out) {
+      const uint32_t n = n_off + (uint32_t)ey * n_row;
+      const int ctr = tid + 1;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        st_off(a.y, n + c * cstr, yv[c]);
+        if (MODE == 1) {
+          const T pv_ = s_in[buf][c][ctr];
+          st_off(pnew, n + c * cstr, pv_);
+          st_off(a.x, n + c * cstr, fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]));
+          dot = fma_t((double)pv_, (double)yv[c], dot);
+        }
+      }
+    }
+  };
+  // one row step: element row ey between node rows ey and ey+1 (LDS buffers tt % 3 and
+  // (tt+1) % 3); Ec / oc are this step's operands, En / on receive the next step's (on
+  // still holds the previous step's operands until its outputs are stored)
+  auto row = [&](int ey, int tt, const T &Ec, T &En, const OwnedNode<T> &oc, OwnedNode<T> &on) {
     const int b0 = tt % 3, b1 = (tt + 1) % 3;
-    load_row(ey + 1, b1);
+    wait_vmem();                          // the previous step's loads and stores
+    commit(b1);                           // node row ey+1
+    store_out(ey - 1, (tt + 2) % 3, on);  // node row ey-1
+    if (ey + 2 <= j1) prefetch(ey + 2);   // committed at the next step
+    En = load_E(ey + 1);
+    load_node(ey + 1, on);
     __syncthreads();
+    T hown[2][2];  // this element's right corners (qa = 1), [qb][c]
     {
       T v[2][4];
       const bool ok = ex_ok && ey >= 0 && ey < g.ny;
@@ -414,47 +492,86 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[c][q] = s_in[(q >> 1) ? b1 : b0][c][tid + (q & 1)];
-      const T E = ok ? a.E[ex + (long long)g.nx * ey] : T(0);
-      elastic_quad4<T>(v, a.op, E);
+      elastic_quad4<T>(v, a.op, ok ? Ec : T(0));
 #pragma unroll
-      for (int c = 0; c < 2; ++c)
+      for (int qb = 0; qb < 2; ++qb)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) s_out[c * 4 + q][tid] = v[c][q];
+        for (int c = 0; c < 2; ++c) {
+          s_h[2 * qb + c][tid] = v[c][2 * qb];
+          hown[qb][c] = v[c][2 * qb + 1];
+        }
     }
     __syncthreads();
+    have_out = false;
     if (node_thread) {
       T yb[2];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) { yb[c] = acc_top[c]; acc_top[c] = T(0); }
-#pragma unroll
-      for (int da = 0; da < 2; ++da) {
-        const int t2 = tid + da;
-        const int q = 1 - da;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          yb[c] += s_out[c * 4 + q][t2];
-          acc_top[c] += s_out[c * 4 + q + 2][t2];
-        }
+      for (int c = 0; c < 2; ++c) {
+        yb[c] = (acc_top[c] + hown[0][c]) + s_h[c][tid + 1];
+        acc_top[c] = hown[1][c] + s_h[2 + c][tid + 1];
       }
       if (ey >= j0 && node_ok) {
-        const long long n = ni + (long long)g.sx * ey;
-        const uint8_t m = a.mask ? a.mask[n] : 0;
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const T yv = ((m >> c) & 1) ? T(0) : yb[c];
-          a.y[c * nn + n] = yv;
-          if (MODE == 1) {
-            const T pv_ = s_in[b0][c][tid + 1];
-            pnew[c * nn + n] = pv_;
-            a.x[c * nn + n] = fma_t(alpha_x, pold[c * nn + n], a.x[c * nn + n]);
-            dot = fma_t((double)pv_, (double)yv, dot);
-          }
+        for (int c = 0; c < 2; ++c) yv[c] = ((oc.m >> c) & 1) ? T(0) : yb[c];
+        have_out = true;
+      }
+    }
+  };
+
+  // the first node row and its moduli go out before the iteration index is known:
+  // p_old is read from both direction buffers and chosen afterwards
+  T po_b[NPOS][2];
+  {
+    const uint32_t lo = (kfirst >= 0 && kfirst < g.sy) ? (uint32_t)kfirst * n_row : 0u;
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p) {
+      if (!pos_live(p)) continue;
+      const uint32_t o = pos_off[p] + lo;
+      if (MODE == 1) pv[p] = ld_off(a.invs, o);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        pr[p][c] = ld_off(a.in, o + c * cstr);
+        if (MODE == 1) {
+          po[p][c] = ld_off((const T *)a.P0, o + c * cstr);
+          po_b[p][c] = ld_off((const T *)a.P1, o + c * cstr);
         }
       }
     }
   }
+  T Ea = load_E(kfirst), Eb = T(0);
+  OwnedNode<T> na{}, nb{};
+  // solve already finished (uniform): leave before the partials reduction
+  if (ss.done) return;
   if (MODE == 1) {
-    const double tot = block_sum<((NT + 63) / 64) * 64>(dot, s_red);
+    double rr, rz;
+    block_total2<NT>(a.P.PB, a.P.nB, s_sc, rr, rz);
+    it = ss.iter;
+    pold = (it & 1) ? a.P0 : a.P1;
+    pnew = (it & 1) ? a.P1 : a.P0;
+    if (!(it & 1)) {
+#pragma unroll
+      for (int p = 0; p < NPOS; ++p)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) po[p][c] = po_b[p][c];
+    }
+    if (!fused_scalars<T>(a, ss, it, blk, rr, rz, beta, alpha_x)) return;
+  }
+  wait_vmem();
+  commit(0);
+  prefetch(kfirst + 1);
+  int ey = kfirst, tt = 0;
+  for (; ey + 1 < j1; ey += 2, tt += 2) {
+    row(ey, tt, Ea, Eb, na, nb);
+    row(ey + 1, tt + 1, Eb, Ea, nb, na);
+  }
+  if (ey < j1) {
+    row(ey, tt, Ea, Eb, na, nb);
+    store_out(ey, tt % 3, na);
+  } else {
+    store_out(ey - 1, (tt + 2) % 3, nb);
+  }
+  if (MODE == 1) {
+    const double tot = block_sum<NT>(dot, s_red);
     if (threadIdx.x == 0) a.P.PA[blk] = tot;
   }
 }
@@ -513,7 +630,9 @@ static void setup_tiles(const Grid &g, OpArgs<T> &a, int &nblocks, int &nthreads
   } else {
     a.tiles_x = (g.sx + QUAD_TI - 1) / QUAD_TI;
     int tk = 16;
-    while (tk > 2 && (long long)a.tiles_x * ((g.sy + tk - 1) / tk) < 512) tk >>= 1;
+    if (g.tile_depth > 0) tk = g.tile_depth;
+    else
+      while (tk > 2 && (long long)a.tiles_x * ((g.sy + tk - 1) / tk) < 512) tk >>= 1;
     while ((long long)a.tiles_x * ((g.sy + tk - 1) / tk) > RED_MAXBLOCKS) tk <<= 1;
     a.TK = tk;
     a.tiles_y = (g.sy + tk - 1) / tk;
