@@ -73,7 +73,9 @@ def parse():
                     help="combine segments of one stream: STEM.json (the segment's serve line) and STEM.npz "
                          "(its --serve-dump) for each segment; no GPU")
     ap.add_argument("--serve-max-dofs", type=float, default=None,
-                    help="admit jobs while the resident DOFs stay within this many")
+                    help="admit jobs while the resident DOFs (of a size class) stay within this many")
+    ap.add_argument("--serve-classes", type=float, nargs="*", default=None,
+                    help="DOF bounds of the size classes, each served on a stream and batch of its own")
     ap.add_argument("--pmc-iters", type=int, default=200)
     ap.add_argument("--output", action="store_true", help="time the output step (mises, reactions, VTK)")
     ap.add_argument("--heat", action="store_true", help="heat-conduction solve (FP64 plain form)")
@@ -84,6 +86,8 @@ def parse():
     ap.add_argument("--stress", action="store_true",
                     help="stress-constrained driver (C++ executable) on the reference's 200 x 100 job")
     ap.add_argument("--opt-steps", type=int, default=30)
+    ap.add_argument("--q4", type=int, nargs=2, default=None, metavar=("NX", "NY"),
+                    help="single solves of a quad4 cantilever NX x NY (seeded densities): FP64 fused and mixed")
     return ap.parse_args()
 
 
@@ -656,6 +660,47 @@ def serving(args):
     print(json.dumps(line))
 
 
+def quad4_solve(args):
+    """Single solves of a quad4 cantilever (plane stress, seeded U[0.01, 1] densities) to
+    the tolerance in the FP64 fused and the mixed form: wall time of graph-replayed solves
+    and kernel A / B per launch from HIP events."""
+    import numpy as np
+    import icx
+    nx, ny = args.q4
+    ctx = icx.Context(0)
+    job = icx.cantilever_2d(ctx, nx, ny)
+    job.set_design(icx.random_density(job.nelx, SEED))
+    N, Ne, nodes = job.numeq, job.nelx, (nx + 1) * (ny + 1)
+    forms = {}
+    for name, form, w in (("fp64_fused", icx.SOLVE_FUSED, 8), ("mixed", icx.SOLVE_MIXED, 4)):
+        for _ in range(args.warmup):
+            job.solve(form, tol=args.tol, maxit=200000, want_u=False)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, info = job.solve(form, tol=args.tol, maxit=200000, want_u=False)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        k = {}
+        for kn, kid in (("A", icx.PROF_FUSED_A), ("B", icx.PROF_FUSED_B)):
+            ctx.profile_begin(kid)
+            _, pi = job.solve(form, tol=args.tol, maxit=200000, want_u=False)
+            secs, _ = ctx.profile_end()
+            k[kn] = secs / max(pi["iterations"], 1) * 1e6
+        bytes_A = (6 * N + nodes + Ne) * w  # reads r, p_old, x, invs, E; writes p, Ap, x
+        forms[name] = {"ms_per_solve": ms, "iterations": info["iterations"], "relres": info["relres"],
+                       "converged": info["converged"], "matvecs": info["matvecs"],
+                       "dof_matvec_per_s": N * info["matvecs"] / (ms * 1e-3),
+                       "kernel_A_us": k["A"], "kernel_B_us": k["B"], "kernel_A_bytes": bytes_A,
+                       "kernel_A_GBps": bytes_A / (k["A"] * 1e-6) / 1e9}
+    line = {"metric": "quad4 single solve: time to rel-res %g, FP64 fused and mixed" % args.tol,
+            "value": forms["fp64_fused"]["ms_per_solve"], "unit": "ms", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "f64 (fused) / f32 operator + f64 residual (mixed)", "data": "synthetic (seeded densities)",
+            "config": {"workload": "quad4 plane-stress cantilever %dx%d" % (nx, ny), "free_dofs": N,
+                       "elements": Ne, "forms": forms}}
+    print(json.dumps(line))
+
+
 def latency_by_dofs(recs):
     """Design-iteration latency and PCG iterations by job size (free DOFs), from serving
     records (job, step, DOFs, PCG iterations, replacements, converged, latency ms, ...)."""
@@ -810,6 +855,9 @@ def main():
         return
     if args.batched:
         batched(args)
+        return
+    if args.q4:
+        quad4_solve(args)
         return
     if args.serve_combine:
         serve_combine(args.serve_combine)

@@ -130,6 +130,7 @@ def _f64(a):
 class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self.device = device
         self.jobs = weakref.WeakSet()
         self.options = {}  # values set through set_option
         rc = lib().icx_ctx_create(device, C.byref(self.h))
