@@ -204,11 +204,9 @@ __host__ __device__ constexpr bool blk_nz2(int chi, int c, int cc) {
   return (chi == 0 || chi == 3) ? true : (c != ((chi & 1) ? 0 : 1) && cc != ((chi & 1) ? 0 : 1));
 }
 
-// v <- E * K0 v for one hexa8 element (v[c][q], corner q = a + 2b + 4c)
+// o <- the character blocks of K0 applied to the Haar coefficients v of a hexa8 element
 template <typename T>
-__device__ __forceinline__ void elastic_hex8(T (&v)[3][8], const ElasticOp<T> &op, T E) {
-  haar_fwd8(v[0]); haar_fwd8(v[1]); haar_fwd8(v[2]);
-  T o[3][8];
+__device__ __forceinline__ void haar_blocks8(const T (&v)[3][8], const ElasticOp<T> &op, T (&o)[3][8]) {
 #pragma unroll
   for (int chi = 0; chi < 8; ++chi) {
     const T m[3] = {v[0][chi ^ 1], v[1][chi ^ 2], v[2][chi ^ 4]};
@@ -226,6 +224,14 @@ __device__ __forceinline__ void elastic_hex8(T (&v)[3][8], const ElasticOp<T> &o
       o[c][chi ^ (1 << c)] = acc;
     }
   }
+}
+
+// v <- E * K0 v for one hexa8 element (v[c][q], corner q = a + 2b + 4c)
+template <typename T>
+__device__ __forceinline__ void elastic_hex8(T (&v)[3][8], const ElasticOp<T> &op, T E) {
+  haar_fwd8(v[0]); haar_fwd8(v[1]); haar_fwd8(v[2]);
+  T o[3][8];
+  haar_blocks8(v, op, o);
   haar_inv8_scaled(o[0], E); haar_inv8_scaled(o[1], E); haar_inv8_scaled(o[2], E);
 #pragma unroll
   for (int c = 0; c < 3; ++c)

@@ -76,7 +76,10 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   // x update (no second global load of p_{i-1}, and 6 VGPRs fewer)
   __shared__ T s_po[MODE == 1 ? 3 : 1][3][MODE == 1 ? LN : 1];
   static_assert(TI + 1 == 16, "element rows are 16-lane DPP rows");
-  __shared__ T s_h[6][NT];
+  // FP32: xy face transforms shared by the element layers on both sides of a node layer
+  // (FP64 keeps the per-element transform: the carried faces would cost it a wave per SIMD)
+  constexpr bool SHARE = sizeof(T) == 4;
+  __shared__ T s_h[SHARE ? 3 : 6][NT];
   __shared__ double s_red[NT / 64];
   __shared__ double s_sc[2 * (NT / 64)];
   const Grid &g = a.g;
@@ -173,7 +176,39 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   };
 
   double dot = 0.0;
-  T acc_top[3] = {T(0), T(0), T(0)};
+  T acc_top[3] = {T(0), T(0), T(0)};  // FP64: carried node-space contributions of the layer above
+  // FP32, per element column (tx, ty): F, the xy Haar transform of the lower node layer of the
+  // next element layer (computed once per node layer, used by the element layers below
+  // and above it), and H, the upper-face outputs of the last element layer in the same
+  // xy Haar domain, added to the next element layer's lower face before the one y / x
+  // inverse per node layer.
+  T F[3][4], H[3][4];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) H[c][q] = T(0);
+  auto face = [&](int buf, T (&f)[3][4]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) f[c][q] = s_in[buf][c][(tx + (q & 1)) + LW * (ty + (q >> 1))];
+      haar_fwd4(f[c]);
+    }
+  };
+  // node-space contributions of the face sums S (corner q = qa + 2 qb of the column) to
+  // node row ty (qb = 0, through LDS to the thread one row up) and ty + 1 (returned):
+  // the y then x stages of the inverse, and the pair sums along x across DPP lanes
+  auto face_out = [&](T (&S)[3][4], T (&hn)[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) { const T a_ = S[c][q], b_ = S[c][q + 2]; S[c][q] = a_ - b_; S[c][q + 2] = a_ + b_; }
+#pragma unroll
+      for (int q = 0; q < 4; q += 2) { const T a_ = S[c][q], b_ = S[c][q + 1]; S[c][q] = a_ - b_; S[c][q + 1] = a_ + b_; }
+      s_h[c][tid] = S[c][1] + row_next(S[c][0]);
+      hn[c] = S[c][3] + row_next(S[c][2]);
+    }
+  };
   // Outputs of an owned node are stored one step late, next to that step's loads:
   // loads and stores share one counter, and a store still in flight when a load is
   // waited for makes the wait total, so stores issued at the end of a step would put
@@ -209,8 +244,8 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     En = load_E(ez + 1);
     load_node(ez + 1, on);
     __syncthreads();
-    T hown[2][3];
-    {
+    T hown[3];
+    if constexpr (!SHARE) {
       T v[3][8];
       const bool ok = ex_ok && ez >= 0 && ez < g.nz;
 #pragma unroll
@@ -224,24 +259,70 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
       // corner q = qa + 2 qb + 4 qc of element (tx, ty) is node (tx + qa, ty + qb) of
       // the halo patch; owned node (lx, ly) is patch node (lx + 1, ly + 1), the corner
       // qa = qb = 1 of element (lx, ly).  Pair sums along x: this element's qa = 1
-      // corner plus the qa = 0 corner of element tx + 1.
+      // corner plus the qa = 0 corner of element tx + 1.  Layer ez's sums complete node
+      // layer ez with the carry; layer ez + 1's are carried.
+      T up[3];
 #pragma unroll
-      for (int qc = 0; qc < 2; ++qc)
+      for (int c = 0; c < 3; ++c) {
+        s_h[c][tid] = v[c][1] + row_next(v[c][0]);      // qb = 0: node row ty - 1
+        hown[c] = v[c][3] + row_next(v[c][2]);          // qb = 1: node row ty
+        s_h[3 + c][tid] = v[c][5] + row_next(v[c][4]);
+        up[c] = v[c][7] + row_next(v[c][6]);
+      }
+      __syncthreads();
+      if (node_thread) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          s_h[3 * qc + c][tid] = v[c][1 + 4 * qc] + row_next(v[c][4 * qc]);  // qb = 0: node row ty - 1
-          hown[qc][c] = v[c][3 + 4 * qc] + row_next(v[c][2 + 4 * qc]);     // qb = 1: node row ty
+          hown[c] = (acc_top[c] + hown[c]) + s_h[c][tid + TI + 1];
+          acc_top[c] = up[c] + s_h[3 + c][tid + TI + 1];
         }
+      }
+    } else {
+      const bool ok = ex_ok && ez >= 0 && ez < g.nz;
+      if (ez == kfirst) face(b0, F);
+      T Fn[3][4];
+      face(b1, Fn);
+      // the z stage of the element's Haar transform from the two face transforms
+      T v[3][8];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          v[c][q] = F[c][q] + Fn[c][q];
+          v[c][q + 4] = Fn[c][q] - F[c][q];
+          F[c][q] = Fn[c][q];
+        }
+      T o[3][8];
+      haar_blocks8(v, a.op, o);
+      // z stage of the inverse with the modulus folded in: the lower face goes to node
+      // layer ez with the carried upper face of element layer ez - 1, the upper face is
+      // carried to node layer ez + 1
+      const T E = ok ? Ec : T(0);
+      T S[3][4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const T t = E * o[c][q + 4];
+          S[c][q] = fma_t(E, o[c][q], -t) + H[c][q];
+          H[c][q] = fma_t(E, o[c][q], t);
+        }
+      // corner q = qa + 2 qb of column (tx, ty) is node (tx + qa, ty + qb) of the halo
+      // patch; owned node (lx, ly) is patch node (lx + 1, ly + 1), the corner qa = qb = 1
+      // of column (lx, ly).  Pair sums along x: this column's qa = 1 corner plus the
+      // qa = 0 corner of column tx + 1.
+      face_out(S, hown);
+      __syncthreads();
+      if (node_thread) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hown[c] += s_h[c][tid + TI + 1];
+      }
     }
-    __syncthreads();
     have_out = false;
     if (node_thread) {
       T yb[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        yb[c] = (acc_top[c] + hown[0][c]) + s_h[c][tid + TI + 1];
-        acc_top[c] = hown[1][c] + s_h[3 + c][tid + TI + 1];
-      }
+      for (int c = 0; c < 3; ++c) yb[c] = hown[c];
       if (ez >= k0 && node_ok) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) yv[c] = ((oc.m >> c) & 1) ? T(0) : yb[c];
@@ -293,15 +374,22 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   wait_vmem();
   commit(0);
   prefetch(kfirst + 1);
-  // top node layer k1 of the seam split: the carried contributions of element layer
-  // k1-1 (node o, LDS buffer buf)
+  // top node layer k1 of the seam split: the carried upper faces of element layer k1-1
+  // (node o, LDS buffer buf); every thread takes part in the node sums
   auto store_top = [&](int buf, const OwnedNode<T> &o) {
-    if (!SEAM || !node_ok) return;
+    if (!SEAM) return;
+    T top[3];
+    __syncthreads();  // the last step's node sums have been read
+    face_out(H, top);
+    __syncthreads();
+    if (!node_ok) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) top[c] += s_h[c][tid + TI + 1];
     const uint32_t n = n_off + (uint32_t)k1 * n_layer;
     const int ctr = (lx + 1) + LW * (ly + 1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T yt = ((o.m >> c) & 1) ? T(0) : acc_top[c];
+      const T yt = ((o.m >> c) & 1) ? T(0) : top[c];
       const T pv_ = s_in[buf][c][ctr];
       dot = fma_t((double)pv_, (double)yt, dot);
       if (top_tile) {
