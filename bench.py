@@ -642,7 +642,7 @@ def serving(args):
                     elems=tuple(args.serve_elems), budget=args.serve_budget, tol=args.tol,
                     log=lambda m: print(m, file=sys.stderr, flush=True), limit=args.serve_limit,
                     max_resident_dofs=int(args.serve_max_dofs) if args.serve_max_dofs else None,
-                    offset=args.serve_offset)
+                    offset=args.serve_offset, classes=args.serve_classes)
     out.pop("designs", None)
     import numpy as np
     lat = out.pop("latencies_ms")

@@ -114,7 +114,7 @@ def make_job(ctx, ndim, div):
 
 def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, seed=SEED,
         tol=1e-8, maxit=50000, x0=0.4, radius=1.5, specs=None, keep_designs=False, log=None, limit=None,
-        cancel=None, max_resident_dofs=None, offset=0, forms="auto"):
+        cancel=None, max_resident_dofs=None, offset=0, forms="auto", classes=None):
     """Serve n_jobs jobs arriving at `rate` per second (exponential gaps), each running
     `iters` OC design iterations.
     forms: "auto" (solve_form: FP64 fused form for quad4 jobs, mixed form for hexa8
@@ -130,12 +130,22 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
       one sitting, starting empty.
     cancel: {job: design iteration} -- the job is withdrawn while that iteration's solve
       runs (icx_cancel), its budget freed.
-    max_resident_dofs: admit a job only while the resident jobs' DOFs, its own included,
-      stay within this many (a job larger than the cap is admitted alone).  The PCG
-      iterations of all resident jobs share each launch, so this bounds the work of one
-      batched iteration and with it a design iteration's latency; the wait moves to
-      admission (first come, first served: a job never overtakes an earlier one).
+    max_resident_dofs: admit a job only while the resident jobs' DOFs of its class, its
+      own included, stay within this many (a job larger than the cap is admitted alone).
+      The PCG iterations of all resident jobs of a class share each launch, so this
+      bounds the work of one batched iteration and with it a design iteration's latency;
+      the wait moves to admission (first come, first served within a class).
+    classes: DOF bounds splitting the jobs into size classes, e.g. [2e5]: jobs below
+      2e5 DOFs and the rest.  Each class has a context of its own (its own stream and
+      batch) on ctx's device, served by a host thread of its own, so small jobs' PCG
+      iterations no longer share launches with, and wait for, those of large jobs, and
+      the classes' kernels overlap on the device.  The HBM budget is shared.  None: one
+      class on ctx.
     Returns a dict of the measurements."""
+    import threading
+
+    from . import Context
+
     cancel = dict(cancel or {})
     rng = np.random.default_rng(np.random.PCG64(seed))
     gaps = rng.exponential(1.0 / rate, n_jobs)
@@ -167,128 +177,177 @@ def run(ctx, n_jobs=500, rate=20.0, iters=30, elems=(5e3, 1.5e6), budget=240e9, 
         return ndim * (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
 
     ndofs = [dofs(s) for s in specs]
-    resident = peak_resident = 0
     if max(est) > budget:
         raise ValueError("a job needs %.3g bytes, more than the budget %.3g" % (max(est), budget))
+    bounds = sorted(float(b) for b in (classes or []))
+    n_cls = len(bounds) + 1
+    cls_of = [sum(1 for b in bounds if nd >= b) for nd in ndofs]
+    queues = [collections.deque(i for i in range(n_jobs) if cls_of[i] == k) for k in range(n_cls)]
+    ctxs = [ctx]
+    for _ in range(n_cls - 1):
+        c = Context(getattr(ctx, "device", 0))
+        for opt, val in getattr(ctx, "options", {}).items():
+            c.set_option(opt, val)
+        ctxs.append(c)
 
-    pending = collections.deque(range(n_jobs))
-    active = {}
+    lock = threading.Lock()
+    sh = {"used": 0.0, "peak_used": 0.0, "peak_active": 0, "active": 0, "peak_resident": 0, "finished": 0,
+          "dof_matvecs": 0}
     lat, admit_wait, footprints = [], [], []
     # one record per design iteration: (job, step, DOFs, PCG iterations, FP64
     # replacements, converged, latency ms, resident jobs, resident DOFs, solve form)
     records = []
     designs, histories = {}, {}
-    dof_matvecs = 0
-    used = 0.0
-    peak_used = 0.0
-    peak_active = 0
-    finished_jobs = 0
     cancelled, rejected = [], []
-    # wall time by phase: batched PCG chunks (poll), per-job work after a solve (step_end:
-    # sensitivity, adjoint filter, OC) and before one (step_begin + submit: filter,
-    # projection, set-up), admission (create + opt_init)
-    split = {"poll": 0.0, "step_end": 0.0, "step_begin_submit": 0.0, "admit": 0.0}
+    errors = []
+    # wall time by phase, summed over the classes' threads: batched PCG chunks (poll),
+    # per-job work after a solve (step_end: sensitivity, adjoint filter, OC) and before
+    # one (step_begin + submit: filter, projection, set-up), admission (create + opt_init)
+    splits = [{"poll": 0.0, "step_end": 0.0, "step_begin_submit": 0.0, "admit": 0.0} for _ in range(n_cls)]
     t0 = time.perf_counter()
 
-    def begin(rec):
-        tb = time.perf_counter()
-        rec["t_begin"] = tb
-        rec["job"].opt_step_begin()
-        rec["job"].submit(tol=tol, maxit=maxit, form=rec["form"])
-        split["step_begin_submit"] += time.perf_counter() - tb
+    def serve_class(k):
+        cctx, pending, split = ctxs[k], queues[k], splits[k]
+        active = {}
+        resident = 0
 
-    while pending or active:
-        now = time.perf_counter() - t0
-        while (pending and arrival[pending[0]] <= now and used + est[pending[0]] <= budget and
-               (max_resident_dofs is None or not active or resident + ndofs[pending[0]] <= max_resident_dofs)):
-            i = pending.popleft()
-            ta = time.perf_counter()
-            try:
-                job, x_init = create(ctx, specs[i])
-            except IcxError as e:
-                rejected.append((i, str(e)))
-                finished_jobs += 1
-                continue
-            form = form_of(job.ndim)
-            job.opt_init(np.full(job.nelx, x0 if x_init is None else x_init), driver=0, radius=radius,
-                         projection=True, form=form, tol=tol, maxit=maxit)
-            split["admit"] += time.perf_counter() - ta
-            rec = {"job": job, "spec": i, "step": 0, "numeq": job.numeq, "form": form}
-            active[id(job)] = rec
-            used += est[i]
-            resident += ndofs[i]
-            peak_resident = max(peak_resident, resident)
-            peak_used = max(peak_used, used)
-            peak_active = max(peak_active, len(active))
-            admit_wait.append(time.perf_counter() - t0 - arrival[i])
-            begin(rec)
-        if active:
-            tp = time.perf_counter()
-            finished = ctx.poll(chunks=1)
-            split["poll"] += time.perf_counter() - tp
-            for rec in [r for r in active.values() if cancel.get(r["spec"]) == r["step"]]:
-                if rec["job"] in finished:
+        def begin(rec):
+            tb = time.perf_counter()
+            rec["t_begin"] = tb
+            rec["job"].opt_step_begin()
+            rec["job"].submit(tol=tol, maxit=maxit, form=rec["form"])
+            split["step_begin_submit"] += time.perf_counter() - tb
+
+        def leave(rec):
+            nonlocal resident
+            with lock:
+                sh["used"] -= est[rec["spec"]]
+                sh["active"] -= 1
+                sh["finished"] += 1
+            resident -= ndofs[rec["spec"]]
+
+        while pending or active:
+            now = time.perf_counter() - t0
+            while pending and arrival[pending[0]] <= now:
+                i = pending[0]
+                if max_resident_dofs is not None and active and resident + ndofs[i] > max_resident_dofs:
+                    break
+                with lock:  # the budget is reserved before the job is built
+                    if sh["used"] + est[i] > budget:
+                        break
+                    sh["used"] += est[i]
+                pending.popleft()
+                ta = time.perf_counter()
+                try:
+                    job, x_init = create(cctx, specs[i])
+                except IcxError as e:
+                    rejected.append((i, str(e)))
+                    with lock:
+                        sh["used"] -= est[i]
+                        sh["finished"] += 1
                     continue
-                rec["job"].cancel()
-                rec["job"].close()
-                del active[id(rec["job"])]
-                used -= est[rec["spec"]]
-                resident -= ndofs[rec["spec"]]
-                cancelled.append(rec["spec"])
-                finished_jobs += 1
-            for job in finished:
-                rec = active[id(job)]
-                info = job.last_info()
-                te = time.perf_counter()
-                step_rec = job.opt_step_end(info)
-                split["step_end"] += time.perf_counter() - te
-                if keep_designs:
-                    histories.setdefault(rec["spec"], []).append(step_rec["objective"])
-                lat.append(time.perf_counter() - rec["t_begin"])
-                records.append((rec["spec"], rec["step"], rec["numeq"], info["iterations"], info["refinements"],
-                                bool(info["converged"]), lat[-1] * 1e3, len(active), resident, rec["form"]))
-                dof_matvecs += rec["numeq"] * info["matvecs"]
-                rec["step"] += 1
-                if rec["step"] < iters:
-                    begin(rec)
-                else:
-                    if len(footprints) < 8:
-                        footprints.append((job.footprint(), est[rec["spec"]]))
+                with lock:
+                    sh["active"] += 1
+                    sh["peak_used"] = max(sh["peak_used"], sh["used"])
+                    sh["peak_active"] = max(sh["peak_active"], sh["active"])
+                form = form_of(job.ndim)
+                job.opt_init(np.full(job.nelx, x0 if x_init is None else x_init), driver=0, radius=radius,
+                             projection=True, form=form, tol=tol, maxit=maxit)
+                split["admit"] += time.perf_counter() - ta
+                rec = {"job": job, "spec": i, "step": 0, "numeq": job.numeq, "form": form}
+                active[id(job)] = rec
+                resident += ndofs[i]
+                with lock:
+                    sh["peak_resident"] = max(sh["peak_resident"], resident)
+                admit_wait.append(time.perf_counter() - t0 - arrival[i])
+                begin(rec)
+            if active:
+                tp = time.perf_counter()
+                finished = cctx.poll(chunks=1)
+                split["poll"] += time.perf_counter() - tp
+                for rec in [r for r in active.values() if cancel.get(r["spec"]) == r["step"]]:
+                    if rec["job"] in finished:
+                        continue
+                    rec["job"].cancel()
+                    rec["job"].close()
+                    del active[id(rec["job"])]
+                    leave(rec)
+                    cancelled.append(rec["spec"])
+                for job in finished:
+                    rec = active[id(job)]
+                    info = job.last_info()
+                    te = time.perf_counter()
+                    step_rec = job.opt_step_end(info)
+                    split["step_end"] += time.perf_counter() - te
                     if keep_designs:
-                        designs[rec["spec"]] = job.opt_design()
-                    job.close()
-                    del active[id(job)]
-                    used -= est[rec["spec"]]
-                    resident -= ndofs[rec["spec"]]
-                    finished_jobs += 1
-                    if log:
-                        log("job %d done (%d left)" % (rec["spec"], n_jobs - finished_jobs))
-        elif pending:
-            time.sleep(max(0.0, min(0.01, arrival[pending[0]] - (time.perf_counter() - t0))))
+                        histories.setdefault(rec["spec"], []).append(step_rec["objective"])
+                    lat.append(time.perf_counter() - rec["t_begin"])
+                    records.append((rec["spec"], rec["step"], rec["numeq"], info["iterations"], info["refinements"],
+                                    bool(info["converged"]), lat[-1] * 1e3, len(active), resident, rec["form"]))
+                    with lock:
+                        sh["dof_matvecs"] += rec["numeq"] * info["matvecs"]
+                    rec["step"] += 1
+                    if rec["step"] < iters:
+                        begin(rec)
+                    else:
+                        if len(footprints) < 8:
+                            footprints.append((job.footprint(), est[rec["spec"]]))
+                        if keep_designs:
+                            designs[rec["spec"]] = job.opt_design()
+                        job.close()
+                        del active[id(job)]
+                        leave(rec)
+                        if log:
+                            log("job %d done (%d left)" % (rec["spec"], n_jobs - sh["finished"]))
+            elif pending:  # next arrival, or budget held by the other classes
+                time.sleep(max(0.001, min(0.01, arrival[pending[0]] - (time.perf_counter() - t0))))
+
+    def guarded(k):
+        try:
+            serve_class(k)
+        except BaseException as e:  # re-raised by the caller once every class has stopped
+            errors.append(e)
+            for q in queues:  # the other classes admit nothing more
+                q.clear()
+
+    if n_cls == 1:
+        serve_class(0)
+    else:
+        threads = [threading.Thread(target=guarded, args=(k,), daemon=True) for k in range(n_cls)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for c in ctxs[1:]:
+            c.synchronize()
+            c.close()
+        if errors:
+            raise errors[0]
     wall = time.perf_counter() - t0
     lat_ms = np.array(lat) * 1e3
     return {
         "jobs": n_jobs, "design_iterations": len(lat), "wall_s": wall,
-        "jobs_per_s": n_jobs / wall, "dof_matvec_per_s": dof_matvecs / wall,
+        "jobs_per_s": n_jobs / wall, "dof_matvec_per_s": sh["dof_matvecs"] / wall,
         "iteration_ms_p50": float(np.percentile(lat_ms, 50)),
         "iteration_ms_p99": float(np.percentile(lat_ms, 99)),
         "admission_wait_ms_p50": float(np.percentile(admit_wait, 50) * 1e3),
         "admission_wait_ms_p99": float(np.percentile(admit_wait, 99) * 1e3),
-        "peak_admitted_jobs": peak_active, "peak_budget_bytes": peak_used,
+        "peak_admitted_jobs": sh["peak_active"], "peak_budget_bytes": sh["peak_used"],
         "arrival_rate_per_s": rate, "iterations_per_job": iters, "elements": list(elems),
         "stream_jobs_drawn": len(drawn),
         "budget_bytes": budget, "tol": tol,
         "footprint_checks": footprints,
         "cancelled": cancelled,
         "rejected": rejected,
-        "wall_split_s": split,
+        "wall_split_s": {key: sum(sp[key] for sp in splits) for key in splits[0]},
         "offset": offset,
+        "size_classes_dofs": bounds,
         "latencies_ms": lat_ms.tolist(),
         "iteration_records": records,
         "unconverged_solves": sum(1 for r in records if not r[5]),
         "admission_wait_ms": (np.array(admit_wait) * 1e3).tolist(),
         "designs": designs if keep_designs else None,
         "objectives": histories if keep_designs else None,
-        "max_used_within_budget": peak_used <= budget,
-        "max_resident_dofs": max_resident_dofs, "peak_resident_dofs": peak_resident,
+        "max_used_within_budget": sh["peak_used"] <= budget,
+        "max_resident_dofs": max_resident_dofs, "peak_resident_dofs": sh["peak_resident"],
     }

@@ -117,3 +117,25 @@ def test_resident_dof_cap(ctx):
     assert out["peak_resident_dofs"] <= cap and out["peak_admitted_jobs"] < len(SPECS)
     for i, (ndim, div) in enumerate(SPECS):
         assert np.array_equal(out["designs"][i], _alone(ctx, ndim, div, iters, tol)), i
+
+
+def test_size_classes_equal_alone(ctx):
+    """Two size classes (a context, stream and batch each, served by two host threads,
+    kernels overlapping on the device): every design still equals its solo run, and the
+    classes hold the jobs of their DOF ranges."""
+    iters, tol = 3, 1e-8
+    ndofs = [nd * (d0 + 1) * (d1 + 1) * ((d2 + 1) if nd == 3 else 1) for nd, (d0, d1, d2) in SPECS]
+    out = serve.run(ctx, rate=500.0, iters=iters, specs=SPECS, tol=tol, keep_designs=True, classes=[700])
+    assert out["size_classes_dofs"] == [700.0]
+    assert sorted(ndofs) == [450, 675, 682, 1188]  # three jobs below the bound, one above
+    assert out["design_iterations"] == iters * len(SPECS) and out["unconverged_solves"] == 0
+    for i, (ndim, div) in enumerate(SPECS):
+        assert np.array_equal(out["designs"][i], _alone(ctx, ndim, div, iters, tol)), i
+
+
+def test_size_classes_share_the_budget(ctx):
+    """A budget that holds one job: with two classes still one job at a time."""
+    est = [serve.footprint_estimate(*s) for s in SPECS]
+    out = serve.run(ctx, rate=1000.0, iters=1, specs=SPECS, budget=max(est) * 1.01, classes=[700])
+    assert out["peak_admitted_jobs"] == 1 and out["max_used_within_budget"]
+    assert out["design_iterations"] == len(SPECS)
