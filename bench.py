@@ -88,6 +88,7 @@ def parse():
     ap.add_argument("--opt-steps", type=int, default=30)
     ap.add_argument("--q4", type=int, nargs=2, default=None, metavar=("NX", "NY"),
                     help="single solves of a quad4 cantilever NX x NY (seeded densities): FP64 fused and mixed")
+    ap.add_argument("--q4-maxit", type=int, default=200000, help="iteration limit of the --q4 solves")
     return ap.parse_args()
 
 
@@ -668,23 +669,25 @@ def quad4_solve(args):
     import icx
     nx, ny = args.q4
     ctx = icx.Context(0)
+    if args.no_graphs:
+        ctx.set_option(icx.OPT_GRAPHS, 0)
     job = icx.cantilever_2d(ctx, nx, ny)
     job.set_design(icx.random_density(job.nelx, SEED))
     N, Ne, nodes = job.numeq, job.nelx, (nx + 1) * (ny + 1)
     forms = {}
     for name, form, w in (("fp64_fused", icx.SOLVE_FUSED, 8), ("mixed", icx.SOLVE_MIXED, 4)):
         for _ in range(args.warmup):
-            job.solve(form, tol=args.tol, maxit=200000, want_u=False)
+            job.solve(form, tol=args.tol, maxit=args.q4_maxit, want_u=False)
         ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            _, info = job.solve(form, tol=args.tol, maxit=200000, want_u=False)
+            _, info = job.solve(form, tol=args.tol, maxit=args.q4_maxit, want_u=False)
         ctx.synchronize()
         ms = (time.perf_counter() - t0) / args.steps * 1e3
         k = {}
         for kn, kid in (("A", icx.PROF_FUSED_A), ("B", icx.PROF_FUSED_B)):
             ctx.profile_begin(kid)
-            _, pi = job.solve(form, tol=args.tol, maxit=200000, want_u=False)
+            _, pi = job.solve(form, tol=args.tol, maxit=args.q4_maxit, want_u=False)
             secs, _ = ctx.profile_end()
             k[kn] = secs / max(pi["iterations"], 1) * 1e6
         bytes_A = (6 * N + nodes + Ne) * w  # reads r, p_old, x, invs, E; writes p, Ap, x
