@@ -79,6 +79,7 @@ def parse():
     ap.add_argument("--pmc-iters", type=int, default=200)
     ap.add_argument("--output", action="store_true", help="time the output step (mises, reactions, VTK)")
     ap.add_argument("--heat", action="store_true", help="heat-conduction solve (FP64 plain form)")
+    ap.add_argument("--mfma", action="store_true", help="FP32 hexa8 matvec: Haar VALU kernel vs dense-K0 MFMA kernel")
     ap.add_argument("--mdpa", action="store_true", help="unstructured hexa8 mesh from an MDPA file (FP64 plain form)")
     ap.add_argument("--heat-div", type=int, default=100)
     ap.add_argument("--gcmma", action="store_true", help="GCMMA design loop (driver 2) on the C1 cantilever")
@@ -250,6 +251,76 @@ def output_step(args):
                        "mises_ms_incl_copy": mises_ms, "reactions_ms_incl_copy": react_ms,
                        "vtk_device_and_copies_ms": begin_ms, "vtk_file_write_ms": write_ms,
                        "vtk_file_MB": size / 1e6, "vtk_write_MBps": size / 1e6 / (write_ms / 1e3)}}
+    print(json.dumps(line))
+
+
+def _hex8_matvec_blocks(nx, ny, nz, depth=0):
+    """Blocks and element layers per block of the MODE 0 hexa8 operator tiling
+    (15 x 15 node tiles, node layers along z, one element layer below each brick)."""
+    sx, sy, sz = nx + 1, ny + 1, nz + 1
+    cols = -(-sx // 15) * -(-sy // 15)
+
+    def blocks(t):
+        return cols * -(-sz // t)
+    tk = 16
+    if depth:
+        tk = depth
+    else:
+        while tk > 2 and blocks(tk) < 2048:
+            tk = tk >> 1 if tk > 4 else tk - 1
+    while blocks(tk) > 4096:
+        tk <<= 1
+    tz = -(-sz // tk)
+    layers = sum(min(k0 + tk, sz) - (k0 - 1) for k0 in range(0, sz, tk))
+    return cols * tz, cols * layers
+
+
+def mfma_matvec(args):
+    """The FP32 hexa8 matvec y = K(E) x two ways on the same job: the Haar-block kernel on
+    the vector units (the operator of the solve forms) and the dense-K0 contraction on the
+    matrix cores (v_mfma_f32_32x32x2f32, ICX_OPT_MFMA_MATVEC), each timed per launch with
+    launch-bound events, at the headline and the large cantilever."""
+    import numpy as np
+
+    import icx
+    ctx = icx.Context(0)
+    out = {}
+    reps = max(args.steps, 5)
+    for name, (nx, ny, nz) in (("headline", args.size), ("large", args.large)):
+        job = icx.cantilever_3d(ctx, nx, ny, nz)
+        job.set_design(icx.random_density(job.nelx, SEED))
+        xv = np.random.default_rng(SEED).standard_normal(job.numeq)
+        res = {}
+        ys = {}
+        for form, opt in (("haar_valu", 0), ("dense_mfma", 1)):
+            ctx.set_option(icx.OPT_MFMA_MATVEC, opt)
+            ys[form] = job.matvec(xv, prec=1)
+            ctx.profile_begin(icx.PROF_MATVEC32)
+            for _ in range(reps):
+                job.matvec(xv, prec=1)
+            sec, nl = ctx.profile_end()
+            res[form] = {"us": sec / nl * 1e6, "launches": nl}
+        ctx.set_option(icx.OPT_MFMA_MATVEC, 0)
+        nn, ne = job.numnp, job.nelx
+        nb, elayers = _hex8_matvec_blocks(nx, ny, nz)
+        applied = elayers * 256  # element slots of the launch (patch halo and repeated layer included)
+        byts = (6 * nn + ne) * 4 + nn
+        for form, r in res.items():
+            r["GBps_algorithmic"] = byts / (r["us"] * 1e-6) / 1e9
+            r["hbm_frac"] = r["GBps_algorithmic"] / 8000.0
+        m = res["dense_mfma"]
+        m["mfma_flops_issued"] = applied * 32 * 24 * 2
+        m["mfma_TFLOPs_issued"] = m["mfma_flops_issued"] / (m["us"] * 1e-6) / 1e12
+        m["mfma_frac_of_fp32_peak"] = m["mfma_TFLOPs_issued"] / 157.3
+        m["dense_flops_per_element"] = 24 * 24 * 2
+        res["haar_valu"]["flops_per_element"] = 240
+        d = np.abs(ys["dense_mfma"] - ys["haar_valu"]).max() / np.abs(ys["haar_valu"]).max()
+        out[name] = {"workload": "hexa8 cantilever %dx%dx%d, FP32 y = K(E) x" % (nx, ny, nz), "elements": ne,
+                     "nodes": nn, "blocks": nb, "bytes_per_launch": byts, "max_rel_diff_mfma_vs_valu": d, **res}
+        job.close()
+    ctx.close()
+    line = {"metric": "FP32 hexa8 matvec per launch: Haar VALU vs dense-K0 MFMA", "n_gpus": 1, "dtype": "f32",
+            "data": "synthetic (seeded densities)", "config": out}
     print(json.dumps(line))
 
 
@@ -879,6 +950,9 @@ def main():
         return stress_driver(args)
     if args.heat:
         heat_solve(args)
+        return
+    if args.mfma:
+        mfma_matvec(args)
         return
     if args.mdpa:
         mesh_solve(args)

@@ -53,6 +53,7 @@ struct icx_ctx {
   int tile_depth = 0;              // hexa8 operator z depth per block, 0 = automatic
   bool heat_dense = false;         // heat jobs: dense element rows even where Haar-diagonal
   bool mma_host_dual = false;      // MMA / GCMMA dual: Newton loop on the host
+  bool mfma_matvec = false;        // FP32 hexa8 icx_matvec: dense K0 on the matrix cores
   // profile window (icx_profile.h)
   std::vector<icx_job *> jobs;  // live jobs, destroyed with the context
   int prof_kernel = 0;
@@ -555,6 +556,7 @@ struct icx_job {
   double volume = 0.0; // element volume
   ElasticOp<double> op64{};
   ElasticOp<float> op32{};
+  std::vector<float> K0f;  // hexa8 elastic: dense unit-modulus K0 (24 x 24) for the MFMA matvec
   // host-side BC state (reference Node::dirich)
   std::vector<int> has, node_flag;
   std::vector<double> node_val;
@@ -833,7 +835,11 @@ icx_status matvec_staged(icx_job *j, int prec, const double *x, double *y, const
   } else {
     launch_convert_f64_f32(N, j->d_vec.p, sg.xf, S(j));
     launch_convert_f64_f32(j->g.ne, j->d_E.p, sg.Ef, S(j));
-    launch_elastic_matvec<float>(j->g, j->op32, sg.Ef, j->d_mask.p, sg.xf, sg.yf, nullptr, S(j));
+    ProfScope ps(c, ICX_PROF_MATVEC32);
+    if (c->mfma_matvec && j->g.ndim == 3)
+      launch_elastic_matvec_mfma(j->g, j->K0f.data(), sg.Ef, j->d_mask.p, sg.xf, sg.yf, S(j), ps.ev());
+    else
+      launch_elastic_matvec<float>(j->g, j->op32, sg.Ef, j->d_mask.p, sg.xf, sg.yf, nullptr, S(j), ps.ev());
     launch_convert_f32_f64(N, sg.yf, j->d_vec2.p, S(j));
   }
   launch_grid_to_dof(j->g, j->nc, j->d_dof.p, j->numeq, j->d_vec2.p, sg.dof, S(j));
@@ -873,7 +879,7 @@ icx_status icx_ctx_create(int device, icx_ctx **out) {
 
 icx_status icx_profile_begin(icx_ctx *ctx, int kernel) {
   if (!ctx) return ICX_ERR_ARG;
-  if (kernel < ICX_PROF_MATVEC64 || kernel > ICX_PROF_FUSED_B) return fail(ctx, ICX_ERR_ARG, "unknown kernel id");
+  if (kernel < ICX_PROF_MATVEC64 || kernel > ICX_PROF_MATVEC32) return fail(ctx, ICX_ERR_ARG, "unknown kernel id");
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   prof_collect(ctx);
   ctx->prof_kernel = kernel;
@@ -900,6 +906,10 @@ icx_status icx_set_option(icx_ctx *ctx, int option, long long value) {
   }
   if (option == ICX_OPT_MMA_HOST_DUAL) {
     ctx->mma_host_dual = value != 0;
+    return ICX_OK;
+  }
+  if (option == ICX_OPT_MFMA_MATVEC) {
+    ctx->mfma_matvec = value != 0;
     return ICX_OK;
   }
   if (option == ICX_OPT_HEAT_DENSE) {
@@ -1023,6 +1033,7 @@ icx_status icx_job_create(icx_ctx *ctx, const icx_problem *p, icx_job **out) {
     std::string why;
     if (!build_elastic_op(j->el, K, j->op64, why)) { delete j; return fail(ctx, ICX_ERR_NUMERIC, "%s", why.c_str()); }
     j->op32 = cast_op<float>(j->op64);
+    if (ndim == 3) j->K0f.assign(K.begin(), K.end());
   }
   return job_finish(ctx, j, out);
 }

@@ -184,6 +184,11 @@ void launch_elastic_matvec(const Grid &g, const ElasticOp<T> &op, const T *E, co
                            const T *x, T *y, const SolverState *st, hipStream_t s,
                            const LaunchEvents *ev = nullptr);
 
+// y = K x on the node grid with the dense unit-modulus element stiffness K0 (24 x 24
+// floats, row-major, DOF 3 q + c) contracted on the matrix cores; hexa8 only, FP32.
+void launch_elastic_matvec_mfma(const Grid &g, const float *K0, const float *E, const uint8_t *mask, const float *x,
+                                float *y, hipStream_t s, const LaunchEvents *ev = nullptr);
+
 // Fused kernel A of iteration i = st->iter (published as st->iter_cur): scalars from the partials,
 // p_i = invd r + beta_i p_{i-1} (P[i&1] <- ..., P[(i-1)&1] read), Ap_i = K p_i,
 // x += alpha_{i-1} p_{i-1}, partial p_i.Ap_i -> PA[i&1].  FP32 hexa8 (seam split): the

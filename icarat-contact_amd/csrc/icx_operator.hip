@@ -832,6 +832,182 @@ template void launch_elastic_fusedA<float>(const Grid &, const ElasticOp<float> 
                                            const LaunchEvents *);
 
 // ---------------------------------------------------------------------------------
+// Dense-K0 FP32 matvec on the matrix cores (hexa8 box jobs, y = K(E) x; selected by
+// ICX_OPT_MFMA_MATVEC).  Same bricks, z-march and halo patch as the MODE 0 kernel
+// above (15 x 15 owned nodes, a 16 x 16 patch of element columns, node layers
+// [k0, k1), element layers k0-1 .. k1-1), but each element layer is contracted densely:
+// Y (24 x 32 elements) = K0 (24 x 24, rows padded to 32) * U (24 x 32) with twelve
+// v_mfma_f32_32x32x2f32 per 32 elements (two chains per wave), E_e applied to the
+// element's output column, outputs staged in LDS and summed per node in a fixed order.
+// 768 multiply-add slots (1536 flops) per element against the Haar form's ~240 flops;
+// kept to measure that trade on this part, not used by the solve forms.
+//
+// Operand layouts of v_mfma_f32_32x32x2f32 (wave64): A (32 x 2): lane l holds
+// A[l % 32][l / 32]; B (2 x 32): lane l holds B[l / 32][l % 32]; D (32 x 32): lane l,
+// register i holds D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
+// ---------------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+struct MfmaOpArgs {
+  Grid g;
+  int TK, tiles_x, tiles_y, tiles_z;
+  const float *E;
+  const uint8_t *mask;
+  const float *x;
+  float *y;
+  float K0[24][24];  // unit-modulus element stiffness, DOF 3 q + c, corner q = qa + 2 qb + 4 qc
+};
+
+__global__ void __launch_bounds__(256) k_hex8_mfma(MfmaOpArgs a) {
+  constexpr int TI = 15, TJ = 15, NT = 256, LW = TI + 2, LN = (TI + 2) * (TJ + 2), NPOS = (LN + NT - 1) / NT;
+  constexpr int L3 = 3 * LN;
+  // node layers in two slots by parity (slot 2 mirrors slot 0, so that the upper face of
+  // an element layer whose lower face sits in slot 1 is read at a fixed offset)
+  __shared__ float s_in[3 * L3];
+  __shared__ float s_y[24][NT];
+  __shared__ float s_E[NT];
+  const Grid &g = a.g;
+  const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
+  const int t = xcd_tile(blockIdx.x, nblocks);
+  const int ti = t % a.tiles_x, tj = (t / a.tiles_x) % a.tiles_y, tk = t / (a.tiles_x * a.tiles_y);
+  const int i0 = ti * TI, j0 = tj * TJ, k0 = tk * a.TK;
+  const int k1 = min(k0 + a.TK, g.sz);
+  const int kfirst = k0 - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tx = tid % (TI + 1), ty = tid / (TI + 1);
+  const int ex = i0 - 1 + tx, ey = j0 - 1 + ty;
+  const bool ex_ok = ex >= 0 && ex < g.nx && ey >= 0 && ey < g.ny;
+  const uint32_t e_off = ex_ok ? (uint32_t)(ex + g.nx * ey) * 4u : 0u;
+  const uint32_t e_layer = (uint32_t)(g.nx * g.ny) * 4u;
+  const bool node_thread = tx < TI && ty < TJ;
+  const int ni = i0 + tx, nj = j0 + ty;
+  const bool node_ok = node_thread && ni < g.sx && nj < g.sy;
+  const uint32_t cstr = (uint32_t)g.nn * 4u;
+  const uint32_t n_layer = (uint32_t)(g.sx * g.sy) * 4u;
+  const uint32_t n_off = node_ok ? (uint32_t)(ni + g.sx * nj) * 4u : 0u;
+  uint32_t pos_off[NPOS];
+#pragma unroll
+  for (int p = 0; p < NPOS; ++p) {
+    const int idx = tid + p * NT;
+    const int gi = i0 - 1 + idx % LW, gj = j0 - 1 + idx / LW;
+    const bool ok = idx < LN && gi >= 0 && gi < g.sx && gj >= 0 && gj < g.sy;
+    pos_off[p] = ok ? (uint32_t)(gi + g.sx * gj) * 4u : 0u;
+  }
+  // A operand: row m = lane % 32 of K0 (zero below row 24), columns 2 kk + lane / 32
+  const int m = lane & 31, kh = lane >> 5;
+  float kA[12];
+#pragma unroll
+  for (int kk = 0; kk < 12; ++kk) kA[kk] = m < 24 ? a.K0[m][2 * kk + kh] : 0.0f;
+  // B operand: DOF d = 2 kk + lane / 32 of element 64 wave + 32 h + lane % 32, as an
+  // offset into s_in for a lower face in slot 0
+  int boff[2][12];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = 64 * wave + 32 * h + m;
+    const int etx = e % (TI + 1), ety = e / (TI + 1);
+#pragma unroll
+    for (int kk = 0; kk < 12; ++kk) {
+      const int d = 2 * kk + kh, q = d / 3, c = d % 3;
+      boff[h][kk] = ((q >> 2) ? L3 : 0) + c * LN + (etx + (q & 1)) + LW * (ety + ((q >> 1) & 1));
+    }
+  }
+  float pr[NPOS][3];
+  auto prefetch = [&](int k) {
+    const uint32_t lo = (k >= 0 && k < g.sz) ? (uint32_t)k * n_layer : 0u;
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pr[p][c] = ld_off(a.x, pos_off[p] + lo + c * cstr);
+  };
+  auto commit = [&](int slot) {
+#pragma unroll
+    for (int p = 0; p < NPOS; ++p) {
+      const int idx = tid + p * NT;
+      if (idx < LN) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          s_in[slot * L3 + c * LN + idx] = pr[p][c];
+          if (slot == 0) s_in[2 * L3 + c * LN + idx] = pr[p][c];
+        }
+      }
+    }
+  };
+  auto load_E = [&](int ez) -> float {
+    const bool ok = ex_ok && ez >= 0 && ez < g.nz;
+    return ok ? ld_off(a.E, e_off + (uint32_t)ez * e_layer) : 0.0f;
+  };
+
+  prefetch(kfirst);
+  commit(0);
+  prefetch(kfirst + 1);
+  float Ec = load_E(kfirst);
+  float carry[3] = {0.0f, 0.0f, 0.0f};
+  for (int ez = kfirst; ez < k1; ++ez) {
+    const int bl = (ez - kfirst) & 1;
+    commit(bl ^ 1);  // node layer ez + 1
+    s_E[tid] = Ec;
+    if (ez + 2 <= k1) prefetch(ez + 2);
+    Ec = load_E(ez + 1);
+    const bool own = node_ok && ez >= k0;
+    uint32_t msk = 0u;
+    if (own && a.mask) msk = a.mask[(n_off >> 2) + (uint32_t)ez * (uint32_t)(g.sx * g.sy)];
+    __syncthreads();
+    const float *sb = s_in + bl * L3;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      f32x16 acc = {};
+#pragma unroll
+      for (int kk = 0; kk < 12; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(kA[kk], sb[boff[h][kk]], acc, 0, 0, 0);
+      const int e = 64 * wave + 32 * h + m;
+      const float Ee = s_E[e];
+#pragma unroll
+      for (int i = 0; i < 12; ++i) s_y[8 * (i >> 2) + 4 * kh + (i & 3)][e] = Ee * acc[i];
+    }
+    __syncthreads();
+    if (node_thread) {
+      // elements (tx + da, ty + db) around patch node (tx + 1, ty + 1): corner
+      // qa = 1 - da, qb = 1 - db; lower faces complete node layer ez, upper faces carry
+      float lo_[3] = {0.0f, 0.0f, 0.0f}, up_[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int db = 0; db < 2; ++db)
+#pragma unroll
+        for (int da = 0; da < 2; ++da) {
+          const int e = (tx + da) + (TI + 1) * (ty + db);
+          const int q = (1 - da) + 2 * (1 - db);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            lo_[c] += s_y[3 * q + c][e];
+            up_[c] += s_y[3 * (q + 4) + c][e];
+          }
+        }
+      if (own) {
+        const uint32_t n = n_off + (uint32_t)ez * n_layer;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) st_off(a.y, n + c * cstr, ((msk >> c) & 1) ? 0.0f : carry[c] + lo_[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) carry[c] = up_[c];
+    }
+  }
+}
+
+void launch_elastic_matvec_mfma(const Grid &g, const float *K0, const float *E, const uint8_t *mask, const float *x,
+                                float *y, hipStream_t s, const LaunchEvents *ev) {
+  OpArgs<float> t{};
+  t.g = g;
+  int nb, nt;
+  setup_tiles(g, t, nb, nt);
+  MfmaOpArgs a{};
+  a.g = g; a.TK = t.TK; a.tiles_x = t.tiles_x; a.tiles_y = t.tiles_y; a.tiles_z = t.tiles_z;
+  a.E = E; a.mask = mask; a.x = x; a.y = y;
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) a.K0[i][j] = K0[i * 24 + j];
+  launch_ev(k_hex8_mfma, nb, 256, s, ev, a);
+  ICX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------------
 // Helmholtz scalar operator: node-parallel gather over the (up to) 8 elements
 // around each node; the element matrix is diagonal in the Haar basis.
 // ---------------------------------------------------------------------------------

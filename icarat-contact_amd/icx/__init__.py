@@ -36,7 +36,8 @@ EXPORTS = [
     "icx_profile_begin", "icx_profile_end", "icx_set_option",
 ]
 OPT_GRAPHS, OPT_TILE_DEPTH, OPT_PIPELINE, OPT_SINGLE_REDUCTION, OPT_HEAT_DENSE, OPT_MMA_HOST_DUAL = 1, 2, 3, 4, 5, 6
-PROF_MATVEC64, PROF_FUSED_A, PROF_FUSED_B = 1, 2, 3
+OPT_MFMA_MATVEC = 7
+PROF_MATVEC64, PROF_FUSED_A, PROF_FUSED_B, PROF_MATVEC32 = 1, 2, 3, 4
 
 
 class Problem(C.Structure):

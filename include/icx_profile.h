@@ -17,7 +17,8 @@ extern "C" {
 enum {
   ICX_PROF_MATVEC64 = 1,  /* FP64 operator kernel of the plain form */
   ICX_PROF_FUSED_A = 2,   /* fused direction-update + operator kernel (fused/mixed forms) */
-  ICX_PROF_FUSED_B = 3    /* fused residual-update kernel (fused/mixed forms) */
+  ICX_PROF_FUSED_B = 3,   /* fused residual-update kernel (fused/mixed forms) */
+  ICX_PROF_MATVEC32 = 4   /* FP32 operator kernel of icx_matvec (Haar VALU or dense MFMA form) */
 };
 
 icx_status icx_profile_begin(icx_ctx *ctx, int kernel);
@@ -45,9 +46,14 @@ icx_status icx_profile_end(icx_ctx *ctx, double *seconds, long long *launches);
  * ICX_OPT_MMA_HOST_DUAL (default 0): 1 runs the Newton loop of the MMA / GCMMA dual on
  *   the host, reading the two sums of every step, instead of on the device with one read
  *   of its state per 16 steps; the same arithmetic, for checking one against the other.
+ * ICX_OPT_MFMA_MATVEC (default 0): 1 makes icx_matvec / icx_matvec_noalloc at FP32 on
+ *   hexa8 box jobs contract the dense element stiffness on the matrix cores
+ *   (v_mfma_f32_32x32x2f32) instead of applying its Haar blocks on the vector units;
+ *   FP32 results differ in rounding only.  A measurement of the two forms; the solve
+ *   forms always use the Haar kernels.
  */
 enum { ICX_OPT_GRAPHS = 1, ICX_OPT_TILE_DEPTH = 2, ICX_OPT_PIPELINE = 3, ICX_OPT_SINGLE_REDUCTION = 4,
-       ICX_OPT_HEAT_DENSE = 5, ICX_OPT_MMA_HOST_DUAL = 6 };
+       ICX_OPT_HEAT_DENSE = 5, ICX_OPT_MMA_HOST_DUAL = 6, ICX_OPT_MFMA_MATVEC = 7 };
 icx_status icx_set_option(icx_ctx *ctx, int option, long long value);
 
 #ifdef __cplusplus

@@ -79,6 +79,38 @@ def test_matvec_fp64_and_fp32(ctx, case):
     assert np.abs(d - K.diagonal()).max() <= 1e-14 * np.abs(K.diagonal()).max()
 
 
+@pytest.mark.parametrize("div,depth", [((5, 3, 4), 0), ((33, 17, 9), 0), ((33, 17, 9), 3), ((16, 31, 20), 2)])
+def test_matvec_fp32_on_matrix_cores(ctx, div, depth):
+    """The dense-K0 MFMA matvec (ICX_OPT_MFMA_MATVEC) against the oracle's CSR product
+    and the Haar VALU kernel: one and several 15x15 node tiles, several z bricks,
+    constrained nodes on two faces."""
+    L = tuple(float(d) for d in div)
+    job, pr = _pair(ctx, 8, div, L, O.SOLID3D,
+                    [([1, 1, 1, 0, 0, 0], [0] * 6, [0, 0, 0, div[1], 0, div[2]]),
+                     ([0, 0, 1, 0, 0, 0], [0] * 6, [0, div[0], 0, div[1], 0, 0])])
+    rho = O.random_density(pr.nelx, 7 + div[0])
+    job.set_design(rho)
+    pr.assemble(rho)
+    xv = np.random.default_rng(div[0] + depth).standard_normal(pr.numeq)
+    y_ref = pr.spmv(xv)
+    scale = np.abs(y_ref).max()
+    ctx.set_option(icx.OPT_TILE_DEPTH, depth)
+    try:
+        y_valu = job.matvec(xv, prec=1)
+        ctx.set_option(icx.OPT_MFMA_MATVEC, 1)
+        y_mfma = job.matvec(xv, prec=1)
+        y_mfma_na = job.matvec_noalloc(xv, prec=1)
+    finally:
+        ctx.set_option(icx.OPT_MFMA_MATVEC, 0)
+        ctx.set_option(icx.OPT_TILE_DEPTH, 0)
+    assert np.abs(y_mfma - y_ref).max() <= 2e-5 * scale
+    assert np.abs(y_mfma - y_valu).max() <= 2e-5 * scale
+    assert np.array_equal(y_mfma, y_mfma_na)
+    # constrained DOFs are not in the numeq vector; a free DOF next to the clamped face
+    # and one on the loaded far face are nonzero
+    assert np.count_nonzero(y_mfma) > 0.9 * pr.numeq
+
+
 def _solve_case(ctx, nx=12, ny=6, nz=6, seed=5):
     job = icx.cantilever_3d(ctx, nx, ny, nz)
     pr = O.cantilever_3d(nx, ny, nz)
