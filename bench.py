@@ -623,6 +623,14 @@ def mesh_solve(args):
     job.solve(icx.SOLVE_MIXED, tol=args.tol, maxit=200000, want_u=False)
     s32, nl32 = ctx.profile_end()
     el32_us = s32 / max(nl32, 1) * 1e6
+    # the FP64 fused form on the same job: the plain form's element pass and gather, with
+    # the vector updates fused into a direction kernel and kernel B
+    job.solve(icx.SOLVE_FUSED, tol=args.tol, maxit=200000, want_u=False)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    _, fi = job.solve(icx.SOLVE_FUSED, tol=args.tol, maxit=200000, want_u=False)
+    ctx.synchronize()
+    fused_wall = time.perf_counter() - t0
     # FilterHelmholtz on the same mesh (R = 2 element widths, the reference's 1e-10): the
     # scalar operator integrated per element in the same way
     job.helmholtz_setup(2.0)
@@ -654,6 +662,9 @@ def mesh_solve(args):
                        "element_pass_GBps_algorithmic": bytes_el / (el_us * 1e-6) / 1e9 if el_us else None,
                        "element_pass_flops_per_launch": flops_el,
                        "helmholtz_filter": {"R": 2.0, "tol": 1e-10, "iterations": h_it, "ms": h_ms},
+                       "fused_fp64": {"ms_per_solve": fused_wall * 1e3, "iterations": fi["iterations"],
+                                      "relres": fi["relres"], "dof_matvec_per_s": N * fi["matvecs"] / fused_wall,
+                                      "speedup_vs_plain": wall / fused_wall},
                        "mixed": {"ms_per_solve": mixed_wall * 1e3, "iterations": mi["iterations"],
                                  "refinements": mi["refinements"], "relres": mi["relres"],
                                  "dof_matvec_per_s": N * mi["matvecs"] / mixed_wall,

@@ -1641,6 +1641,7 @@ static icx_status solve_plain(icx_job *j, double tol, int maxit, icx_solve_info 
 // FP64 fused form:
 // FP64 fused form, in pieces shared by icx_solve and the batched path.
 static PcgPartials fused_parts(icx_job *j) {
+  if (j->mesh) return j->parts(mesh_gather_blocks(j->mop), pcg_stream_blocks(j->nvec()));
   return j->parts(elastic_fusedA_blocks<double>(j->g), pcg_stream_blocks(j->nvec()));
 }
 
@@ -1654,7 +1655,9 @@ static icx_status fused_begin(icx_job *j, double tol, int maxit, SolverState &st
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N));
   HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.invs.alloc(j->g.nn));
   const PcgPartials P = fused_parts(j);
-  launch_elastic_invdiag_node<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
+  // mesh jobs: the plain form's Jacobi inverse (1 on constrained components)
+  if (j->mesh) launch_mesh_diag(j->mop, j->d_E.p, j->d_mask.p, w.invd.p, true, S(j));
+  else launch_elastic_invdiag_node<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(double), S(j)));
@@ -1664,7 +1667,9 @@ static icx_status fused_begin(icx_job *j, double tol, int maxit, SolverState &st
 
 // x += alpha p still owed at a stop of the fused form (none when the set-up converged)
 static void fused_flush(icx_job *j, const SolverState &st) {
-  if (st.done != 1) launch_pcg_flush(j->nvec(), j->d_state.p, j->d_u.p, j->fw.pa.p, j->fw.pb.p, S(j));
+  // mesh jobs update p in place in pa: both parities read it
+  if (st.done != 1)
+    launch_pcg_flush(j->nvec(), j->d_state.p, j->d_u.p, j->fw.pa.p, j->mesh ? j->fw.pa.p : j->fw.pb.p, S(j));
 }
 
 // info of a stopped fused solve (its state read after the flush); true if converged
@@ -1688,7 +1693,11 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
   const int every = poll_every(N);
   auto chunk = [&]() {
     for (int k = 0; k < every; ++k) {
-      {
+      if (j->mesh) {
+        ProfScope ps(c, ICX_PROF_FUSED_A);
+        launch_mesh_pcg_iteration<double>(j->mop, j->d_state.p, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p,
+                                          w.Ap.p, j->d_u.p, P, S(j), ps.ev());
+      } else {
         ProfScope ps(c, ICX_PROF_FUSED_A);
         launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p,
                                       w.pb.p, w.Ap.p, nullptr, j->d_u.p, j->d_state.p, P, S(j), ps.ev());
@@ -1812,8 +1821,8 @@ static void mixed_iteration(icx_job *j) {
   if (j->mesh) {  // one buffer holds p (updated in place)
     {
       ProfScope ps(c, ICX_PROF_FUSED_A);
-      launch_mesh_mixed_iteration(j->mop, j->d_state.p, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.Ap.p,
-                                  w.xlo.p, P, S(j), ps.ev());
+      launch_mesh_pcg_iteration<float>(j->mop, j->d_state.p, w.E.p, j->d_mask.p, w.r.p, w.invd.p, w.pa.p, w.Ap.p,
+                                       w.xlo.p, P, S(j), ps.ev());
     }
     ProfScope ps(c, ICX_PROF_FUSED_B);
     launch_pcg_fusedB<float>(j->nvec(), j->d_state.p, w.r.p, w.Ap.p, w.invd.p, P, nullptr, SeamGeom{}, S(j),
@@ -2066,8 +2075,6 @@ icx_status icx_solve(icx_job *j, int form, double tol, int maxit, double *u, icx
     return fail(j->ctx, ICX_ERR_STATE, "icx_solve needs icx_make_dof and icx_set_design");
   if (!(tol > 0.0) || maxit < 1) return fail(j->ctx, ICX_ERR_ARG, "tol must be > 0 and maxit >= 1");
   if (j->heat() && form != ICX_SOLVE_PLAIN) return fail(j->ctx, ICX_ERR_ARG, "heat jobs: ICX_SOLVE_PLAIN only");
-  if (j->mesh && form == ICX_SOLVE_FUSED)
-    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: ICX_SOLVE_PLAIN or ICX_SOLVE_MIXED");
   icx_ctx *c = j->ctx;
   (void)hipSetDevice(c->device);
   icx_solve_info inf{};
@@ -3072,8 +3079,6 @@ icx_status icx_opt_init(icx_job *j, const icx_opt_params *par, const double *x0)
   if (j->numeq < 0) return fail(j->ctx, ICX_ERR_STATE, "icx_make_dof first");
   if (j->heat()) return fail(j->ctx, ICX_ERR_ARG, "the device design loop drives elastic jobs");
   if (par->driver < 0 || par->driver > 2) return fail(j->ctx, ICX_ERR_ARG, "driver must be 0 (OC), 1 (MMA) or 2 (GCMMA)");
-  if (j->mesh && par->form == ICX_SOLVE_FUSED)
-    return fail(j->ctx, ICX_ERR_ARG, "mesh jobs: the design loop takes ICX_SOLVE_PLAIN or ICX_SOLVE_MIXED");
   icx_ctx *c = j->ctx;
   OptWork &o = j->opt;
   o.par = *par;

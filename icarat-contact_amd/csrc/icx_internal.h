@@ -245,13 +245,15 @@ void launch_mesh_sensitivity(const MeshOp &m, double y1, double y2, double p, co
 void launch_mesh_energy(const MeshOp &m, const double *E, const double *u, double *energy, hipStream_t s);
 // element von Mises stress of the Gauss-point mean strain (u: node values, SoA)
 void launch_mesh_mises(const MeshOp &m, const double *E, const double *u, double *out, hipStream_t s);
-// Mixed form on a mesh job: one inner iteration (direction and deferred x step, FP32
-// element pass, gather with the p.Ap partials into P.PA) up to kernel B, which the
-// caller launches; P.nA must be mesh_gather_blocks(m).  ev times the element pass.
+// Fused (T = double) and mixed (T = float, the inner iteration) forms on a mesh job: one
+// PCG iteration (direction and deferred x step in place, element pass, gather with the
+// p.Ap partials into P.PA) up to kernel B, which the caller launches; P.nA must be
+// mesh_gather_blocks(m).  ev times the element pass.
 int mesh_gather_blocks(const MeshOp &m);
-void launch_mesh_mixed_iteration(const MeshOp &m, SolverState *st, const float *E, const uint8_t *mask,
-                                 const float *r, const float *invd, float *p, float *Ap, float *x,
-                                 const PcgPartials &P, hipStream_t s, const LaunchEvents *ev = nullptr);
+template <typename T>
+void launch_mesh_pcg_iteration(const MeshOp &m, SolverState *st, const T *E, const uint8_t *mask, const T *r,
+                               const T *invd, T *p, T *Ap, T *x, const PcgPartials &P, hipStream_t s,
+                               const LaunchEvents *ev = nullptr);
 // Helmholtz filter on a mesh job, one scalar per node (rh2 = (R / (2 sqrt 3))^2):
 // y = K x (both passes exit at once when st->done is set; st may be null), the Jacobi
 // inverse of diag K (1 on a zero diagonal), the right-hand side sum_e rho_e int N / div,

@@ -270,7 +270,8 @@ __global__ void __launch_bounds__(256) k_mesh_gather(MeshOp m, const double *ye,
   }
 }
 
-// ---- mixed form on mesh jobs: FP32 inner Jacobi-PCG, FP64 residual replacement ------
+// ---- fused and mixed forms on mesh jobs (FP64 Jacobi-PCG; FP32 inner Jacobi-PCG with
+// FP64 residual replacement) ------
 // Per iteration: k_mesh_dir (the iteration scalars from kernel B's partials and the stop
 // protocol of the structured kernel A, then p = invd r + beta p and x += alpha_{i-1}
 // p_{i-1} in place: one thread per vector entry reads and writes it), the FP32 element
@@ -593,25 +594,32 @@ void launch_mesh_mises(const MeshOp &m, const double *E, const double *u, double
 
 int mesh_gather_blocks(const MeshOp &m) { return mesh_blocks(m.nn); }
 
-void launch_mesh_mixed_iteration(const MeshOp &m, SolverState *st, const float *E, const uint8_t *mask,
-                                 const float *r, const float *invd, float *p, float *Ap, float *x,
-                                 const PcgPartials &P, hipStream_t s, const LaunchEvents *ev) {
+template <typename T>
+void launch_mesh_pcg_iteration(const MeshOp &m, SolverState *st, const T *E, const uint8_t *mask, const T *r,
+                               const T *invd, T *p, T *Ap, T *x, const PcgPartials &P, hipStream_t s,
+                               const LaunchEvents *ev) {
   const long long n = (long long)m.nc * m.nn;
-  hipLaunchKernelGGL(k_mesh_dir<float>, dim3(ew_blocks(n, 4, 4096)), dim3(256), 0, s, n, st, r, invd, p, x, P);
-  float *ye = reinterpret_cast<float *>(m.ye);
+  hipLaunchKernelGGL(k_mesh_dir<T>, dim3(ew_blocks(n, 4, 4096)), dim3(256), 0, s, n, st, r, invd, p, x, P);
+  T *ye = reinterpret_cast<T *>(m.ye);
   if (m.ne == 8) {
-    launch_ev(k_mesh_element<8, 3, float>, mesh_blocks(m.nelx), 256, s, ev, m, E, (const float *)p, ye,
+    launch_ev(k_mesh_element<8, 3, T>, mesh_blocks(m.nelx), 256, s, ev, m, E, (const T *)p, ye,
               (const SolverState *)st);
-    hipLaunchKernelGGL((k_mesh_gather_dot<3, float>), dim3(mesh_gather_blocks(m)), dim3(256), 0, s, m,
-                       (const float *)ye, mask, (const float *)p, Ap, (const SolverState *)st, P.PA);
+    hipLaunchKernelGGL((k_mesh_gather_dot<3, T>), dim3(mesh_gather_blocks(m)), dim3(256), 0, s, m, (const T *)ye,
+                       mask, (const T *)p, Ap, (const SolverState *)st, P.PA);
   } else {
-    launch_ev(k_mesh_element<4, 2, float>, mesh_blocks(m.nelx), 256, s, ev, m, E, (const float *)p, ye,
+    launch_ev(k_mesh_element<4, 2, T>, mesh_blocks(m.nelx), 256, s, ev, m, E, (const T *)p, ye,
               (const SolverState *)st);
-    hipLaunchKernelGGL((k_mesh_gather_dot<2, float>), dim3(mesh_gather_blocks(m)), dim3(256), 0, s, m,
-                       (const float *)ye, mask, (const float *)p, Ap, (const SolverState *)st, P.PA);
+    hipLaunchKernelGGL((k_mesh_gather_dot<2, T>), dim3(mesh_gather_blocks(m)), dim3(256), 0, s, m, (const T *)ye,
+                       mask, (const T *)p, Ap, (const SolverState *)st, P.PA);
   }
   ICX_CHECK_LAUNCH();
 }
+template void launch_mesh_pcg_iteration<float>(const MeshOp &, SolverState *, const float *, const uint8_t *,
+                                               const float *, const float *, float *, float *, float *,
+                                               const PcgPartials &, hipStream_t, const LaunchEvents *);
+template void launch_mesh_pcg_iteration<double>(const MeshOp &, SolverState *, const double *, const uint8_t *,
+                                                const double *, const double *, double *, double *, double *,
+                                                const PcgPartials &, hipStream_t, const LaunchEvents *);
 
 void launch_mesh_helm_matvec(const MeshOp &m, double rh2, const double *x, double *y, const SolverState *st,
                              hipStream_t s) {

@@ -106,10 +106,9 @@ icx_status icx_job_create_toml(icx_ctx *ctx, const char *toml_text, icx_job **ou
  * and 0-based connectivity in the reference's local node order (quad4: counter-clockwise;
  * hexa8: the z-low face counter-clockwise, then the z-high face).  Element matrices are
  * integrated per element (2 x 2 (x 2) Gauss points); an element with det J <= 0 at a
- * Gauss point is rejected (ICX_ERR_NUMERIC).  Mesh jobs take the FP64 plain and the mixed
- * solve forms, the cone and Helmholtz filters, von Mises stress and the design loop; the
- * box-only entry points (icx_matvec at FP32, the fused form, submitted solves) return
- * ICX_ERR_ARG. */
+ * Gauss point is rejected (ICX_ERR_NUMERIC).  Mesh jobs take every solve form (plain,
+ * fused, mixed), the cone and Helmholtz filters, von Mises stress and the design loop;
+ * the box-only entry points (icx_matvec at FP32, submitted solves) return ICX_ERR_ARG. */
 typedef struct {
   int eltype;          /* ICX_QUAD4 or ICX_HEXA8 */
   int numnp, nelx;

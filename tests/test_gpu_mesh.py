@@ -94,6 +94,56 @@ def test_mesh_mixed_form_meets_true_residual(ctx, ne, state, div, L):
     assert np.linalg.norm(u2 - ud) / np.linalg.norm(ud) < 1e-10
 
 
+@pytest.mark.parametrize("ne,state,div,L", CASES)
+def test_mesh_fused_form_matches_plain_form_and_oracle(ctx, ne, state, div, L):
+    """The FP64 fused form on a distorted mesh (direction and deferred x step in place,
+    FP64 element pass, gather with the p.Ap partials, FP64 kernel B): the same PCG as the
+    plain form up to rounding, so the iteration counts agree within one and both follow
+    the direct solution; the residual through the oracle's CSR meets the tolerance."""
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 5, L)
+    job, pr, rho, b = _pair(ctx, ne, state, xd, conn, L)
+    ud = pr.direct(b)
+    K = pr.csr()
+    for tol in (1e-10, 1e-13):
+        up, pi = job.solve(icx.SOLVE_PLAIN, tol=tol, maxit=50000)
+        uf, fi = job.solve(icx.SOLVE_FUSED, tol=tol, maxit=50000)
+        assert fi["converged"] and pi["converged"], (fi, pi)
+        assert abs(fi["iterations"] - pi["iterations"]) <= 1, (fi, pi)
+        # the true residual meets the tolerance down to FP64 CG's attainable accuracy
+        # (about 1e-12 here, the same for both forms)
+        tr_f = np.linalg.norm(b - K @ uf) / np.linalg.norm(b)
+        tr_p = np.linalg.norm(b - K @ up) / np.linalg.norm(b)
+        assert tr_f < max(1.01 * tol, 10 * tr_p), (tol, tr_f, tr_p)
+        assert np.linalg.norm(uf - up) / np.linalg.norm(up) < 1e-8
+    assert np.linalg.norm(uf - ud) / np.linalg.norm(ud) < 1e-10
+    c_d = float(b @ ud)
+    assert abs(job.compliance() - c_d) <= 1e-10 * abs(c_d)
+
+
+def test_mesh_driver_a_fused_form(ctx):
+    """Driver A on a distorted quad4 mesh with the FP64 fused form at 1e-13: objectives
+    and design follow the oracle's loop with direct solves."""
+    ne, div, L, steps = 4, (24, 8), (24.0, 8.0), 8
+    x, conn = box_mesh(ne, div, L)
+    xd = distort(x, conn, ne, 0.2, 21, L)
+    job = icx.Job.from_mesh(ctx, ne, xd, conn, icx.PLANE_STRESS)
+    pr = O.Problem.from_mesh(ne, xd, conn, O.PLANE_STRESS)
+    for o in (job, pr):
+        o.add_dirichlet([1, 1, 0, 0, 0, 0], [0] * 6, [0, 0, 0, L[1], 0, 0])
+    assert job.make_dof() == pr.make_dof()
+    for o in (job, pr):
+        o.add_neumann([0.0, -0.05, 0.0], [L[0], L[0], 0.0, 2.0, 0, 0])
+    x0 = np.full(pr.nelx, 0.5)
+    ref = pr.run_driver(0, x0, steps, 1.5, projection=True, direct=True)
+    job.opt_init(x0, driver=0, filter=icx.FILTER_CONE, radius=1.5, form=icx.SOLVE_FUSED,
+                 tol=1e-13, maxit=50000)
+    obj = np.array([job.opt_step()["objective"] for _ in range(steps)])
+    rel = np.abs(obj - ref["obj"]) / ref["obj"]
+    assert rel.max() < 1e-8, rel
+    assert np.abs(job.opt_design() - ref["design"]).max() < 1e-7
+
+
 def test_mesh_driver_a_mixed_form(ctx):
     """Driver A on a distorted hexa8 mesh with the mixed form at 1e-11: objectives and
     design follow the oracle's loop with direct solves."""
@@ -171,9 +221,7 @@ def test_mesh_job_rejects_box_only_calls(ctx):
     job.make_dof()
     job.set_design(np.full(job.nelx, 0.5))
     v = np.ones(job.numeq)
-    for call in (lambda: job.matvec(v, prec=1), lambda: job.solve(icx.SOLVE_FUSED),
-                 lambda: job.submit(tol=1e-8),
-                 lambda: job.opt_init(np.full(job.nelx, 0.5), form=icx.SOLVE_FUSED)):
+    for call in (lambda: job.matvec(v, prec=1), lambda: job.submit(tol=1e-8)):
         with pytest.raises(icx.IcxError) as e:
             call()
         assert e.value.code == 1
