@@ -147,6 +147,17 @@ __host__ __device__ inline uint32_t seam_flags(uint32_t i0, int nw, const SeamGe
   return f;
 }
 
+// Seam flags of the nw consecutive entries i0 .. i0+nw-1 (nw <= 4) of a node-interleaved
+// vector (entry 3 n + c of node n): bit w set when the node of entry i0 + w lies on a seam
+// layer.  The run spans at most two nodes.
+__host__ __device__ inline uint32_t seam_flags_aos(uint32_t i0, int nw, const SeamGeom &sg) {
+  const uint32_t n0 = i0 / 3, r = i0 - 3 * n0;
+  const uint32_t f = seam_flags(n0, 2, sg);  // nodes n0 and n0 + 1 (node indices, plane 0)
+  uint32_t out = 0;
+  for (int w = 0; w < nw; ++w) out |= ((f >> ((r + (uint32_t)w) / 3)) & 1u) << w;
+  return out;
+}
+
 // ---- structured operators (icx_operator.hip) -------------------------------------
 // Arguments of one launch of an elastic operator kernel (one job).
 template <typename T> struct OpArgs {
@@ -364,13 +375,16 @@ void launch_pcg_fusedB(long long n, SolverState *st, T *r, const T *Ap, const T 
 // fused FP64 form at the stop: x += alpha_owed p_{stop-1}
 void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0, const double *P1,
                       hipStream_t s);
-// mixed form replacement: xhi += xlo + alpha_owed p_{stop-1}; xlo = 0
+// Mixed form replacement.  The FP64 vectors (xhi, b, Kx) are component planes; the FP32
+// ones (xlo, P0, P1, r32, invd32) are too when nn_aos = 0, else node-interleaved over
+// nn_aos nodes of 3 components (entry 3 n + c is FP64 entry c nn_aos + n).
+// xhi += xlo + alpha_owed p_{stop-1}; xlo = 0
 void launch_refine_x(long long n, SolverState *st, double *xhi, float *xlo, const float *P0,
-                     const float *P1, hipStream_t s);
-// mixed form replacement: r32 = b - Kx (FP64), (|r|^2, r.z) -> PB[(stop-1)&1], then a
-// one-block kernel decides converged / continue and resets the recurrence
+                     const float *P1, long long nn_aos, hipStream_t s);
+// r32 = b - Kx (FP64), (|r|^2, r.z) -> PB[(stop-1)&1], then a one-block kernel decides
+// converged / continue and resets the recurrence
 void launch_refine_r(long long n, SolverState *st, const double *b, const double *Kx, float *r32,
-                     const float *invd32, PcgPartials P, hipStream_t s);
+                     const float *invd32, PcgPartials P, long long nn_aos, hipStream_t s);
 
 // the Eigen ConjugateGradient loop after tmp = A p, one kernel per vector expression
 // (p.tmp, x += a p, r -= a tmp, |r|^2, z = M^-1 r, r.z, p = z + beta p)
@@ -395,12 +409,12 @@ void launch_elastic_invdiag(const Grid &g, const double *kd, const double *E, co
                             double *invd64, float *invd32, hipStream_t s);
 // Jacobi preconditioner of the fused and mixed forms on a box job, in T: per node
 // invs[n] = 1 / S_n (S_n = sum of the moduli of the elements around node n; -1 where
-// S_n = 0) and per DOF invd[c][n] = jacobi_inv(invs[n], op.ikd[c]) (0 at constrained
-// DOFs).  Kernel A of a hexa8 job forms the same factor from invs (one word per node
-// instead of one per DOF); kernel B and the set-up read invd.
+// S_n = 0) and per DOF invd[c][n] (aos: invd[3 n + c]) = jacobi_inv(invs[n], op.ikd[c])
+// (0 at constrained DOFs).  Kernel A of a hexa8 job forms the same factor from invs (one
+// word per node instead of one per DOF); kernel B and the set-up read invd.
 template <typename T>
 void launch_elastic_invdiag_node(const Grid &g, const ElasticOp<T> &op, const double *E, const uint8_t *mask,
-                                 T *invs, T *invd, hipStream_t s);
+                                 T *invs, T *invd, hipStream_t s, bool aos = false);
 void launch_helm_invdiag(const Grid &g, double kdiag, double *invd, hipStream_t s);
 void launch_dof_to_grid(const Grid &g, int nc, const int *dof, int numeq, const double *v,
                         const double *dirich /* nullable: per node*6 */, double *grid,
@@ -410,6 +424,9 @@ void launch_grid_to_dof(const Grid &g, int nc, const int *dof, int numeq, const 
 void launch_node_values(const Grid &g, int nc, const int *dof, int numeq, const double *grid,
                         const double *dirich, double *nodeval, hipStream_t s);
 void launch_convert_f64_f32(long long n, const double *a, float *b, hipStream_t s);
+// b[3 n + c] = a[c nn + n] for nn nodes of 3 components (component planes in FP64 to a
+// node-interleaved FP32 vector)
+void launch_convert_f64_f32_aos(long long nn, const double *a, float *b, hipStream_t s);
 void launch_convert_f32_f64(long long n, const float *a, double *b, hipStream_t s);
 void launch_axpby(long long n, double a, const double *x, double b, double *y, hipStream_t s);
 void launch_mul(long long n, const double *a, double *b, hipStream_t s);  // b *= a

@@ -66,7 +66,7 @@ void launch_elastic_invdiag(const Grid &g, const double *kd, const double *E, co
 
 template <typename T>
 __global__ void k_elastic_invdiag_node(Grid g, T ik0, T ik1, T ik2, const double *E, const uint8_t *mask, T *invs,
-                                       T *invd) {
+                                       T *invd, bool aos) {
   const int nc = g.ndim;
   const T ik[3] = {ik0, ik1, ik2};
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
@@ -83,20 +83,21 @@ __global__ void k_elastic_invdiag_node(Grid g, T ik0, T ik1, T ik2, const double
     const T s = S != 0.0 ? (T)(1.0 / S) : T(-1);
     invs[n] = s;
     const uint8_t m = mask ? mask[n] : 0;
-    for (int c = 0; c < nc; ++c) invd[c * g.nn + n] = ((m >> c) & 1) ? T(0) : jacobi_inv(s, ik[c]);
+    for (int c = 0; c < nc; ++c)
+      invd[aos ? nc * n + c : c * g.nn + n] = ((m >> c) & 1) ? T(0) : jacobi_inv(s, ik[c]);
   }
 }
 template <typename T>
 void launch_elastic_invdiag_node(const Grid &g, const ElasticOp<T> &op, const double *E, const uint8_t *mask,
-                                 T *invs, T *invd, hipStream_t s) {
+                                 T *invs, T *invd, hipStream_t s, bool aos) {
   hipLaunchKernelGGL(k_elastic_invdiag_node<T>, dim3(ew_blocks(g.nn)), dim3(256), 0, s, g, op.ikd[0], op.ikd[1],
-                     op.ikd[2], E, mask, invs, invd);
+                     op.ikd[2], E, mask, invs, invd, aos);
   ICX_CHECK_LAUNCH();
 }
 template void launch_elastic_invdiag_node<float>(const Grid &, const ElasticOp<float> &, const double *,
-                                                 const uint8_t *, float *, float *, hipStream_t);
+                                                 const uint8_t *, float *, float *, hipStream_t, bool);
 template void launch_elastic_invdiag_node<double>(const Grid &, const ElasticOp<double> &, const double *,
-                                                  const uint8_t *, double *, double *, hipStream_t);
+                                                  const uint8_t *, double *, double *, hipStream_t, bool);
 
 __global__ void k_helm_invdiag(Grid g, double kdiag, double *invd) {
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < g.nn;
@@ -177,6 +178,17 @@ __global__ void k_f64_f32(long long n, const double *a, float *b) {
 }
 void launch_convert_f64_f32(long long n, const double *a, float *b, hipStream_t s) {
   hipLaunchKernelGGL(k_f64_f32, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, a, b);
+  ICX_CHECK_LAUNCH();
+}
+
+__global__ void k_f64_f32_aos(long long nn, const double *a, float *b) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < nn;
+       n += (long long)gridDim.x * blockDim.x)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) b[3 * n + c] = (float)a[c * nn + n];
+}
+void launch_convert_f64_f32_aos(long long nn, const double *a, float *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_f64_f32_aos, dim3(ew_blocks(nn, 4)), dim3(256), 0, s, nn, a, b);
   ICX_CHECK_LAUNCH();
 }
 

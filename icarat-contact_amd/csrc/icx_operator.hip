@@ -65,7 +65,7 @@ template <typename T> struct OwnedNode {
   uint32_t m;
 };
 
-template <typename T, int TI, int TJ, int MODE>
+template <typename T, int TI, int TJ, int MODE, bool AOS = false>
 __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   constexpr int NT = (TI + 1) * (TJ + 1);
   constexpr int LW = TI + 2, LN = (TI + 2) * (TJ + 2);
@@ -113,6 +113,27 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   const int ni = i0 + lx, nj = j0 + ly;
   const bool node_ok = node_thread && ni < g.sx && nj < g.sy;
   const uint32_t cstr = (uint32_t)g.nn * W;  // component stride in bytes
+  // Per-node vectors (r, p, x, y): component planes (c * nn + node), or with AOS the
+  // node's three components together (3 node + c), read and written as one access.
+  auto st3 = [&](T *base, uint32_t n, const T (&v)[3]) {
+    if constexpr (AOS) {
+      T *q = reinterpret_cast<T *>(reinterpret_cast<char *>(base) + 3u * n);
+      q[0] = v[0]; q[1] = v[1]; q[2] = v[2];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) st_off(base, n + (uint32_t)c * cstr, v[c]);
+    }
+  };
+  // the three components of the node at byte offset n (AOS: one access of three words)
+  auto ld3 = [&](const T *base, uint32_t n, T (&v)[3]) {
+    if constexpr (AOS) {
+      const T *q = reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + 3u * n);
+      v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = ld_off(base, n + (uint32_t)c * cstr);
+    }
+  };
   const uint32_t n_layer = (uint32_t)(g.sx * g.sy) * W;
   const uint32_t n_off = node_ok ? (uint32_t)(ni + g.sx * nj) * W : 0u;
 
@@ -138,11 +159,8 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
       if (!pos_live(p)) continue;
       const uint32_t o = pos_off[p] + lo;
       if (MODE == 1) pv[p] = ld_off(a.invs, o);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        pr[p][c] = ld_off(a.in, o + c * cstr);
-        if (MODE == 1) po[p][c] = ld_off(pold, o + c * cstr);
-      }
+      ld3(a.in, o, pr[p]);
+      if (MODE == 1) ld3(pold, o, po[p]);
     }
   };
   T beta = T(0), alpha_x = T(0);
@@ -169,10 +187,7 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     const bool in = k >= k0 && (SEAM ? k <= k1 : k < k1);
     const uint32_t n = n_off + (in ? (uint32_t)k * n_layer : 0u);
     o.m = a.mask ? (uint32_t)ld_off(a.mask, n / W) : 0u;
-    if (MODE == 1) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) o.x[c] = ld_off(a.x, n + c * cstr);
-    }
+    if (MODE == 1) ld3(a.x, n, o.x);
   };
 
   double dot = 0.0;
@@ -219,15 +234,17 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     if (have_out) {
       const uint32_t n = n_off + (uint32_t)ez * n_layer;
       const int ctr = (lx + 1) + LW * (ly + 1);
+      st3(a.y, n, yv);
+      if (MODE == 1) {
+        T pv_[3], xv[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        st_off(a.y, n + c * cstr, yv[c]);
-        if (MODE == 1) {
-          const T pv_ = s_in[buf][c][ctr];
-          st_off(pnew, n + c * cstr, pv_);
-          st_off(a.x, n + c * cstr, fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]));
-          dot = fma_t((double)pv_, (double)yv[c], dot);
+        for (int c = 0; c < 3; ++c) {
+          pv_[c] = s_in[buf][c][ctr];
+          xv[c] = fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]);
+          dot = fma_t((double)pv_[c], (double)yv[c], dot);
         }
+        st3(pnew, n, pv_);
+        st3(a.x, n, xv);
       }
     }
   };
@@ -341,13 +358,10 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
       if (!pos_live(p)) continue;
       const uint32_t o = pos_off[p] + lo;
       if (MODE == 1) pv[p] = ld_off(a.invs, o);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        pr[p][c] = ld_off(a.in, o + c * cstr);
-        if (MODE == 1) {
-          po[p][c] = ld_off((const T *)a.P0, o + c * cstr);
-          po_b[p][c] = ld_off((const T *)a.P1, o + c * cstr);
-        }
+      ld3(a.in, o, pr[p]);
+      if (MODE == 1) {
+        ld3((const T *)a.P0, o, po[p]);
+        ld3((const T *)a.P1, o, po_b[p]);
       }
     }
   }
@@ -387,18 +401,20 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
     for (int c = 0; c < 3; ++c) top[c] += s_h[c][tid + TI + 1];
     const uint32_t n = n_off + (uint32_t)k1 * n_layer;
     const int ctr = (lx + 1) + LW * (ly + 1);
+    T yt[3], pv_[3], xv[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T yt = ((o.m >> c) & 1) ? T(0) : top[c];
-      const T pv_ = s_in[buf][c][ctr];
-      dot = fma_t((double)pv_, (double)yt, dot);
-      if (top_tile) {
-        st_off(a.y, n + c * cstr, yt);
-        st_off(pnew, n + c * cstr, pv_);
-        st_off(a.x, n + c * cstr, fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]));
-      } else {
-        st_off(a.y2, n + c * cstr, yt);
-      }
+      yt[c] = ((o.m >> c) & 1) ? T(0) : top[c];
+      pv_[c] = s_in[buf][c][ctr];
+      dot = fma_t((double)pv_[c], (double)yt[c], dot);
+      xv[c] = fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]);
+    }
+    if (top_tile) {
+      st3(a.y, n, yt);
+      st3(pnew, n, pv_);
+      st3(a.x, n, xv);
+    } else {
+      st3(a.y2, n, yt);
     }
   };
   int ez = kfirst, tt = 0;
@@ -420,9 +436,13 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   }
 }
 
+// The FP32 kernel A (the mixed form's inner iteration) works on node-interleaved
+// vectors; the matvec and the FP64 kernels on component planes.
+template <typename T, int MODE> constexpr bool HexAos = MODE == 1 && sizeof(T) == 4;
+
 template <typename T, int TI, int TJ, int MODE>
 __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
-  hex8_block<T, TI, TJ, MODE>(a, blockIdx.x);
+  hex8_block<T, TI, TJ, MODE, HexAos<T, MODE>>(a, blockIdx.x);
 }
 // (four waves per SIMD in FP32, three in FP64: what the single-job kernels reach)
 template <typename T, int TI, int TJ, int MODE>
@@ -430,7 +450,7 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) __attribute__((amdgpu_wav
     k_hex8_batch(const OpArgs<T> *__restrict__ args, const int *__restrict__ off, int njobs) {
   int j, blk;
   batch_block(off, njobs, j, blk);
-  hex8_block<T, TI, TJ, MODE>(args[j], blk);
+  hex8_block<T, TI, TJ, MODE, HexAos<T, MODE>>(args[j], blk);
 }
 
 // quad4: a block of TI + 1 threads owns TI nodes along x and the node rows [j0, j1),

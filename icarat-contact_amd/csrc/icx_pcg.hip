@@ -122,7 +122,7 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
         A[u] = av[v];
         D[u] = dv[v];
         if (seams) {
-          const uint32_t f = seam_flags((uint32_t)(v * W), W, a.sg);
+          const uint32_t f = seam_flags_aos((uint32_t)(v * W), W, a.sg);
           sf |= f << (W * u);
           if (f) A2[u] = a2v[v];
         }
@@ -167,7 +167,7 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   const long long tail = nv * W + v0;
   if (tail < n) {
     T apt = Ap[tail];
-    if (seams && seam_flags((uint32_t)tail, 1, a.sg)) apt += a.Ap2[tail];
+    if (seams && seam_flags_aos((uint32_t)tail, 1, a.sg)) apt += a.Ap2[tail];
     const T ri = fma_t(-alpha, apt, r[tail]);
     r[tail] = ri;
     rr = fma_t((double)ri, (double)ri, rr);
@@ -194,13 +194,11 @@ __device__ __forceinline__ void fusedB_block(const BArgs<T> &a, int blk) {
   }
 }
 
-// Kernel B on a box elastic job with the per-node Jacobi factor: each thread takes
-// chunks of 4 consecutive nodes with their 3 components (3 vectors of r and Ap, one of
-// invs read once for the three), so the preconditioner costs one word per node instead
-// of one per entry.  Component planes start at c nn, not 16-byte aligned in general:
-// the r and Ap vectors are dword-aligned 16-byte accesses.  Nodes past the last full
-// chunk (nn mod 4) are taken by thread 0 of block 0.
-typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+// Kernel B on a hexa8 box job with the per-node Jacobi factor (node-interleaved r and Ap,
+// entry 3 n + c): each thread takes chunks of 4 consecutive nodes, 12 consecutive
+// entries (three 16-byte vectors each of r and Ap, one of invs read once for the three
+// components), so the preconditioner costs one word per node instead of one per entry.
+// Nodes past the last full chunk (nn mod 4) are taken by thread 0 of block 0.
 template <int U>
 __device__ __forceinline__ void fusedB_node_block(const BArgs<float> &a, int blk) {
   __shared__ double scr[BS / 64];
@@ -214,7 +212,7 @@ __device__ __forceinline__ void fusedB_node_block(const BArgs<float> &a, int blk
   const long long stride = (long long)nblk * BS;
   const long long q0 = blk * (long long)BS + threadIdx.x;
   const bool seams = a.sg.kmax > 0;
-  f4u R[U][3], A[U][3], A2[U][3];
+  float4 R[U][3], A[U][3], A2[U][3];
   float4 S[U];
   uint32_t sf[U];
   auto load = [&](long long base) {
@@ -223,14 +221,13 @@ __device__ __forceinline__ void fusedB_node_block(const BArgs<float> &a, int blk
       const long long q = base + u * stride;
       sf[u] = 0;
       if (q < nq) {
-        const long long n0 = 4 * q;
-        S[u] = *reinterpret_cast<const float4 *>(a.invs + n0);
-        if (seams) sf[u] = seam_flags((uint32_t)n0, 4, a.sg);
+        S[u] = *reinterpret_cast<const float4 *>(a.invs + 4 * q);
+        if (seams) sf[u] = seam_flags((uint32_t)(4 * q), 4, a.sg);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          R[u][c] = *reinterpret_cast<const f4u *>(a.r + c * nn + n0);
-          A[u][c] = *reinterpret_cast<const f4u *>(a.Ap + c * nn + n0);
-          if (sf[u]) A2[u][c] = *reinterpret_cast<const f4u *>(a.Ap2 + c * nn + n0);
+        for (int k = 0; k < 3; ++k) {
+          R[u][k] = *reinterpret_cast<const float4 *>(a.r + 12 * q + 4 * k);
+          A[u][k] = *reinterpret_cast<const float4 *>(a.Ap + 12 * q + 4 * k);
+          if (sf[u]) A2[u][k] = *reinterpret_cast<const float4 *>(a.Ap2 + 12 * q + 4 * k);
         }
       }
     }
@@ -258,21 +255,27 @@ __device__ __forceinline__ void fusedB_node_block(const BArgs<float> &a, int blk
     for (int u = 0; u < U; ++u) {
       const long long q = base + u * stride;
       if (q < nq) {
-        const long long n0 = 4 * q;
         const float sv[4] = {S[u].x, S[u].y, S[u].z, S[u].w};
+        float rv[12], av[12], a2[12];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          f4u out = R[u][c];
-#pragma unroll
-          for (int w = 0; w < 4; ++w) {
-            float apw = A[u][c][w];
-            if ((sf[u] >> w) & 1) apw += A2[u][c][w];
-            float rv = out[w];
-            node(sv[w], apw, rv, c);
-            out[w] = rv;
-          }
-          *reinterpret_cast<f4u *>(a.r + c * nn + n0) = out;
+        for (int k = 0; k < 3; ++k) {
+          rv[4 * k] = R[u][k].x; rv[4 * k + 1] = R[u][k].y; rv[4 * k + 2] = R[u][k].z; rv[4 * k + 3] = R[u][k].w;
+          av[4 * k] = A[u][k].x; av[4 * k + 1] = A[u][k].y; av[4 * k + 2] = A[u][k].z; av[4 * k + 3] = A[u][k].w;
+          a2[4 * k] = A2[u][k].x; a2[4 * k + 1] = A2[u][k].y; a2[4 * k + 2] = A2[u][k].z;
+          a2[4 * k + 3] = A2[u][k].w;
         }
+        // entry e = 3 w + c: component c of node w of the chunk
+#pragma unroll
+        for (int e = 0; e < 12; ++e) {
+          const int w = e / 3, c = e % 3;
+          float apw = av[e];
+          if ((sf[u] >> w) & 1) apw += a2[e];
+          node(sv[w], apw, rv[e], c);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          *reinterpret_cast<float4 *>(a.r + 12 * q + 4 * k) =
+              make_float4(rv[4 * k], rv[4 * k + 1], rv[4 * k + 2], rv[4 * k + 3]);
       }
     }
     if (base + U * stride < nq) load(base + U * stride);
@@ -281,11 +284,11 @@ __device__ __forceinline__ void fusedB_node_block(const BArgs<float> &a, int blk
     for (long long n = 4 * nq; n < nn; ++n) {
       const bool sfl = seams && seam_flags((uint32_t)n, 1, a.sg);
       for (int c = 0; c < 3; ++c) {
-        float apw = a.Ap[c * nn + n];
-        if (sfl) apw += a.Ap2[c * nn + n];
-        float rv = a.r[c * nn + n];
+        float apw = a.Ap[3 * n + c];
+        if (sfl) apw += a.Ap2[3 * n + c];
+        float rv = a.r[3 * n + c];
         node(a.invs[n], apw, rv, c);
-        a.r[c * nn + n] = rv;
+        a.r[3 * n + c] = rv;
       }
     }
   }
@@ -357,30 +360,40 @@ void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0,
   ICX_CHECK_LAUNCH();
 }
 
+// FP64 entry of FP32 entry i: the same index, or with a node-interleaved FP32 vector over
+// nn_aos nodes (entry 3 n + c) the component-plane entry c nn_aos + n
+__device__ __forceinline__ long long plane_index(long long i, long long nn_aos) {
+  if (!nn_aos) return i;
+  const long long n = i / 3;
+  return (i - 3 * n) * nn_aos + n;
+}
+
 __global__ void k_refine_x(long long n, const SolverState *st, double *xhi, float *xlo, const float *P0,
-                           const float *P1) {
+                           const float *P1, long long nn_aos) {
   const int it = st->stop_iter;
   const float a = (it > 0 && st->flush_iter != it) ? (float)st->alpha_owed : 0.0f;
   const float *p = ((it - 1) & 1) ? P1 : P0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const float xl = xlo[i] + a * p[i];
-    xhi[i] += (double)xl;
+    xhi[plane_index(i, nn_aos)] += (double)xl;
     xlo[i] = 0.0f;
   }
 }
 void launch_refine_x(long long n, SolverState *st, double *xhi, float *xlo, const float *P0, const float *P1,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_refine_x, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, st, xhi, xlo, P0, P1);
+                     long long nn_aos, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_x, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, st, xhi, xlo, P0, P1, nn_aos);
   ICX_CHECK_LAUNCH();
 }
 
 __global__ void __launch_bounds__(BS) k_refine_r(long long n, const SolverState *st, const double *b,
-                                                 const double *Kx, float *r32, const float *invd32, PcgPartials P) {
+                                                 const double *Kx, float *r32, const float *invd32, PcgPartials P,
+                                                 long long nn_aos) {
   __shared__ double scr[BS / 64];
   double rr = 0.0, rz = 0.0;
   for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
-    const double ri = b[i] - Kx[i];
+    const long long i64 = plane_index(i, nn_aos);
+    const double ri = b[i64] - Kx[i64];
     const float rf = (float)ri;
     r32[i] = rf;
     rr += ri * ri;
@@ -414,8 +427,8 @@ __global__ void __launch_bounds__(BS) k_refine_finalize(SolverState *st, PcgPart
   }
 }
 void launch_refine_r(long long n, SolverState *st, const double *b, const double *Kx, float *r32,
-                     const float *invd32, PcgPartials P, hipStream_t s) {
-  hipLaunchKernelGGL(k_refine_r, dim3(P.nB), dim3(BS), 0, s, n, st, b, Kx, r32, invd32, P);
+                     const float *invd32, PcgPartials P, long long nn_aos, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_r, dim3(P.nB), dim3(BS), 0, s, n, st, b, Kx, r32, invd32, P, nn_aos);
   hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(BS), 0, s, st, P);
   ICX_CHECK_LAUNCH();
 }

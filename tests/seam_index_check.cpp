@@ -1,7 +1,8 @@
 // Host-side check of the index arithmetic kernel B uses on the seam layers of the FP32
 // hexa8 kernel A: udiv (multiply-shift division for dividends < 2^29) against integer
 // division, and seam_flags against a brute-force layer test, over random box meshes
-// and tile depths.  Exit status 0: no mismatch.
+// and tile depths, for component-plane and node-interleaved vectors.  Exit status 0: no
+// mismatch.
 #include <hip/hip_runtime.h>
 #include "../icarat-contact_amd/csrc/icx_internal.h"
 #include <algorithm>
@@ -44,6 +45,14 @@ int main() {
         const bool ref = k > 0 && k % TK == 0 && k / TK < tiles;
         bad2 += ref != (bool)((f >> w) & 1);
         seams += ref;
+        ++dofs;
+      }
+      // the same vector node-interleaved (entry 3 n + c of node n)
+      const uint32_t fa = seam_flags_aos((uint32_t)i0, nw, sg);
+      for (int w = 0; w < nw; ++w) {
+        const long long k = ((i0 + w) / 3) / layer;
+        const bool ref = k > 0 && k % TK == 0 && k / TK < tiles;
+        bad2 += ref != (bool)((fa >> w) & 1);
         ++dofs;
       }
     }
