@@ -107,8 +107,21 @@ __device__ __forceinline__ double block_sum(double v, double *scratch) {
 template <int BS>
 __device__ __forceinline__ double block_total(const double *p, int n, int s, double *scratch) {
   double a = 0.0;
+  if (n <= 16 * BS) {
+    // every load issued before the first add (one memory latency instead of one per
+    // round of a partially unrolled loop); the same order of additions
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int q = threadIdx.x + k * BS;
+      v[k] = q < n ? p[(long long)q * s] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a += v[k];
+  } else {
 #pragma unroll 4
-  for (int q = threadIdx.x; q < n; q += BS) a += p[(long long)q * s];
+    for (int q = threadIdx.x; q < n; q += BS) a += p[(long long)q * s];
+  }
   return block_sum<BS>(a, scratch);
 }
 
