@@ -1015,7 +1015,7 @@ def main():
                      "traffic": measured_traffic(*args.size, sr),
                      "kernel": ("k_hex8_sr<0> (single-reduction PCG iteration: direction, operator, "
                                 "residual and solution updates)" if sr else
-                                "k_hex8<float,15,15,1> (fused direction update + operator)"),
+                                "k_hex8<float,15,7,1> (fused direction update + operator)"),
                      "avg_launch_us": r["avg_A"] * 1e6, "launches": r["launches_A"],
                      "executed_launches": r["executed_A"],
                      "bytes_per_launch": r["bytes_A"]},
