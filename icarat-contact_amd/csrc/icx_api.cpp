@@ -460,6 +460,11 @@ struct MixedWork {
 };
 struct FusedWork {
   DBuf<double> r, pa, pb, Ap, invd, invs;
+  // hexa8 box jobs: the iteration's vectors node-interleaved (entry 3 n + c), with x and
+  // b in that layout (xa, ba) during the solve; x goes back to the job's component-plane
+  // displacement at the stop
+  DBuf<double> xa, ba;
+  bool aos = false;
 };
 struct HelmWork {
   bool ready = false;
@@ -1657,22 +1662,33 @@ static icx_status fused_begin(icx_job *j, double tol, int maxit, SolverState &st
   FusedWork &w = j->fw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N));
   HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.invs.alloc(j->g.nn));
+  w.aos = !j->mesh && j->g.ndim == 3;
+  if (w.aos) { HIPCHK(c, w.xa.alloc(N)); HIPCHK(c, w.ba.alloc(N)); }
   const PcgPartials P = fused_parts(j);
   // mesh jobs: the plain form's Jacobi inverse (1 on constrained components)
   if (j->mesh) launch_mesh_diag(j->mop, j->d_E.p, j->d_mask.p, w.invd.p, true, S(j));
-  else launch_elastic_invdiag_node<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j));
+  else launch_elastic_invdiag_node<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.invs.p, w.invd.p, S(j), w.aos);
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
+  if (w.aos) {
+    HIPCHK(c, hipMemsetAsync(w.xa.p, 0, N * sizeof(double), S(j)));
+    launch_soa_to_aos_f64(j->g.nn, 3, j->d_b.p, w.ba.p, S(j));
+  }
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(double), S(j)));
-  launch_pcg_setup<double>(N, j->d_state.p, j->d_b.p, w.r.p, w.invd.p, P, tol, maxit, 0.0, S(j));
+  launch_pcg_setup<double>(N, j->d_state.p, w.aos ? w.ba.p : j->d_b.p, w.r.p, w.invd.p, P, tol, maxit, 0.0, S(j));
   return read_state(j, st);
 }
 
 // x += alpha p still owed at a stop of the fused form (none when the set-up converged)
+// the solution vector of the fused iteration: the job's displacement, or its
+// node-interleaved copy
+static double *fused_x(icx_job *j) { return j->fw.aos ? j->fw.xa.p : j->d_u.p; }
+
 static void fused_flush(icx_job *j, const SolverState &st) {
   // mesh jobs update p in place in pa: both parities read it
   if (st.done != 1)
-    launch_pcg_flush(j->nvec(), j->d_state.p, j->d_u.p, j->fw.pa.p, j->mesh ? j->fw.pa.p : j->fw.pb.p, S(j));
+    launch_pcg_flush(j->nvec(), j->d_state.p, fused_x(j), j->fw.pa.p, j->mesh ? j->fw.pa.p : j->fw.pb.p, S(j));
+  if (j->fw.aos) launch_aos_to_soa_f64(j->g.nn, 3, j->fw.xa.p, j->d_u.p, S(j));
 }
 
 // info of a stopped fused solve (its state read after the flush); true if converged
@@ -1703,7 +1719,7 @@ static icx_status solve_fused(icx_job *j, double tol, int maxit, icx_solve_info 
       } else {
         ProfScope ps(c, ICX_PROF_FUSED_A);
         launch_elastic_fusedA<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, w.r.p, w.invd.p, w.invs.p, w.pa.p,
-                                      w.pb.p, w.Ap.p, nullptr, j->d_u.p, j->d_state.p, P, S(j), ps.ev());
+                                      w.pb.p, w.Ap.p, nullptr, fused_x(j), j->d_state.p, P, S(j), ps.ev());
       }
       {
         ProfScope ps(c, ICX_PROF_FUSED_B);
@@ -1965,7 +1981,7 @@ static icx_status batch_rebuild(icx_ctx *c, BatchRun &B) {
       const PcgPartials P = fused_parts(j);
       OpArgs<double> a;
       const int nb = fusedA_args<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, f.r.p, f.invd.p, f.invs.p, f.pa.p,
-                                         f.pb.p, f.Ap.p, nullptr, j->d_u.p, j->d_state.p, P, a);
+                                         f.pb.p, f.Ap.p, nullptr, fused_x(j), j->d_state.p, P, a);
       if (j->g.ndim == 3) { d3.push_back(a); od3.push_back(od3.back() + pad8(nb)); }
       else { d2.push_back(a); od2.push_back(od2.back() + pad8(nb)); }
       db.push_back(BArgs<double>{j->nvec(), j->d_state.p, f.r.p, f.Ap.p, f.invd.p, P, nullptr, SeamGeom{}, nullptr,
@@ -3509,7 +3525,8 @@ icx_status icx_job_footprint(const icx_job *j, size_t *bytes) {
   add(j->d_design); add(j->d_E); add(j->d_vec); add(j->d_vec2); add(j->d_partial); add(j->d_pcgpart);
   add(j->d_state); add(j->d_scalars);
   for (auto *d : {&j->pw.r, &j->pw.p, &j->pw.t, &j->pw.z, &j->pw.invd}) add(*d);
-  for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd, &j->fw.invs}) add(*d);
+  for (auto *d : {&j->fw.r, &j->fw.pa, &j->fw.pb, &j->fw.Ap, &j->fw.invd, &j->fw.invs, &j->fw.xa, &j->fw.ba})
+    add(*d);
   for (auto *d : {&j->mw.r, &j->mw.pa, &j->mw.pb, &j->mw.Ap, &j->mw.Ap2, &j->mw.xlo, &j->mw.invd, &j->mw.invs, &j->mw.E,
                   &j->mw.b32, &j->mw.r1, &j->mw.ap1})
     add(*d);

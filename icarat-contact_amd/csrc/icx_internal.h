@@ -427,6 +427,9 @@ void launch_convert_f64_f32(long long n, const double *a, float *b, hipStream_t 
 // b[3 n + c] = a[c nn + n] for nn nodes of 3 components (component planes in FP64 to a
 // node-interleaved FP32 vector)
 void launch_convert_f64_f32_aos(long long nn, const double *a, float *b, hipStream_t s);
+// b[nc n + c] = a[c nn + n] (to node-interleaved) and back, FP64, nc components per node
+void launch_soa_to_aos_f64(long long nn, int nc, const double *a, double *b, hipStream_t s);
+void launch_aos_to_soa_f64(long long nn, int nc, const double *a, double *b, hipStream_t s);
 void launch_convert_f32_f64(long long n, const float *a, double *b, hipStream_t s);
 void launch_axpby(long long n, double a, const double *x, double b, double *y, hipStream_t s);
 void launch_mul(long long n, const double *a, double *b, hipStream_t s);  // b *= a

@@ -181,6 +181,24 @@ void launch_convert_f64_f32(long long n, const double *a, float *b, hipStream_t 
   ICX_CHECK_LAUNCH();
 }
 
+// component planes <-> node-interleaved FP64 vectors (nn nodes of nc components)
+__global__ void k_soa_aos_f64(long long nn, int nc, const double *a, double *b, bool to_aos) {
+  for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < nn;
+       n += (long long)gridDim.x * blockDim.x)
+    for (int c = 0; c < nc; ++c) {
+      if (to_aos) b[nc * n + c] = a[c * nn + n];
+      else b[c * nn + n] = a[nc * n + c];
+    }
+}
+void launch_soa_to_aos_f64(long long nn, int nc, const double *a, double *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_soa_aos_f64, dim3(ew_blocks(nn, 4)), dim3(256), 0, s, nn, nc, a, b, true);
+  ICX_CHECK_LAUNCH();
+}
+void launch_aos_to_soa_f64(long long nn, int nc, const double *a, double *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_soa_aos_f64, dim3(ew_blocks(nn, 4)), dim3(256), 0, s, nn, nc, a, b, false);
+  ICX_CHECK_LAUNCH();
+}
+
 __global__ void k_f64_f32_aos(long long nn, const double *a, float *b) {
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < nn;
        n += (long long)gridDim.x * blockDim.x)

@@ -436,9 +436,9 @@ __device__ __forceinline__ void hex8_block(const OpArgs<T> &a, int blk) {
   }
 }
 
-// The FP32 kernel A (the mixed form's inner iteration) works on node-interleaved
-// vectors; the matvec and the FP64 kernels on component planes.
-template <typename T, int MODE> constexpr bool HexAos = MODE == 1 && sizeof(T) == 4;
+// Kernel A (the inner iteration of the mixed and the FP64 fused forms) works on
+// node-interleaved vectors; the matvec on component planes.
+template <typename T, int MODE> constexpr bool HexAos = MODE == 1;
 
 template <typename T, int TI, int TJ, int MODE>
 __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) k_hex8(OpArgs<T> a) {
