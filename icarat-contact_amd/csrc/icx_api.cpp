@@ -460,8 +460,8 @@ struct MixedWork {
 };
 struct FusedWork {
   DBuf<double> r, pa, pb, Ap, invd, invs;
-  // hexa8 box jobs: the iteration's vectors node-interleaved (entry 3 n + c), with x and
-  // b in that layout (xa, ba) during the solve; x goes back to the job's component-plane
+  // box jobs: the iteration's vectors node-interleaved (entry nc n + c), with x and b in
+  // that layout (xa, ba) during the solve; x goes back to the job's component-plane
   // displacement at the stop
   DBuf<double> xa, ba;
   bool aos = false;
@@ -1662,7 +1662,7 @@ static icx_status fused_begin(icx_job *j, double tol, int maxit, SolverState &st
   FusedWork &w = j->fw;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N));
   HIPCHK(c, w.Ap.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.invs.alloc(j->g.nn));
-  w.aos = !j->mesh && j->g.ndim == 3;
+  w.aos = !j->mesh;
   if (w.aos) { HIPCHK(c, w.xa.alloc(N)); HIPCHK(c, w.ba.alloc(N)); }
   const PcgPartials P = fused_parts(j);
   // mesh jobs: the plain form's Jacobi inverse (1 on constrained components)
@@ -1671,7 +1671,7 @@ static icx_status fused_begin(icx_job *j, double tol, int maxit, SolverState &st
   HIPCHK(c, hipMemsetAsync(j->d_u.p, 0, N * sizeof(double), S(j)));
   if (w.aos) {
     HIPCHK(c, hipMemsetAsync(w.xa.p, 0, N * sizeof(double), S(j)));
-    launch_soa_to_aos_f64(j->g.nn, 3, j->d_b.p, w.ba.p, S(j));
+    launch_soa_to_aos_f64(j->g.nn, j->nc, j->d_b.p, w.ba.p, S(j));
   }
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(double), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pb.p, 0, N * sizeof(double), S(j)));
@@ -1688,7 +1688,7 @@ static void fused_flush(icx_job *j, const SolverState &st) {
   // mesh jobs update p in place in pa: both parities read it
   if (st.done != 1)
     launch_pcg_flush(j->nvec(), j->d_state.p, fused_x(j), j->fw.pa.p, j->mesh ? j->fw.pa.p : j->fw.pb.p, S(j));
-  if (j->fw.aos) launch_aos_to_soa_f64(j->g.nn, 3, j->fw.xa.p, j->d_u.p, S(j));
+  if (j->fw.aos) launch_aos_to_soa_f64(j->g.nn, j->nc, j->fw.xa.p, j->d_u.p, S(j));
 }
 
 // info of a stopped fused solve (its state read after the flush); true if converged

@@ -469,7 +469,7 @@ __global__ void __launch_bounds__((TI + 1) * (TJ + 1)) __attribute__((amdgpu_wav
 // y = K p_i, x += alpha_{i-1} p_{i-1} on the owned nodes (p_{i-1} kept in LDS beside
 // p_i), partial p_i . y; the first node row's loads go out before the iteration scalars
 // are reduced, from both direction buffers.
-template <typename T, int TI, int MODE>
+template <typename T, int TI, int MODE, bool AOS = false>
 __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   constexpr int NT = TI + 1;
   constexpr int LW = TI + 2;
@@ -505,6 +505,24 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   const int ni = i0 + tid;
   const bool node_ok = node_thread && ni < g.sx;
   const uint32_t cstr = (uint32_t)g.nn * W;  // component stride in bytes
+  // Per-node vectors: component planes, or with AOS the node's two components together
+  // (2 node + c), read and written as one access.
+  auto ld2 = [&](const T *base, uint32_t n, T *v) {
+    if constexpr (AOS) {
+      const T *q = reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + 2u * n);
+      v[0] = q[0]; v[1] = q[1];
+    } else {
+      v[0] = ld_off(base, n); v[1] = ld_off(base, n + cstr);
+    }
+  };
+  auto st2 = [&](T *base, uint32_t n, const T *v) {
+    if constexpr (AOS) {
+      T *q = reinterpret_cast<T *>(reinterpret_cast<char *>(base) + 2u * n);
+      q[0] = v[0]; q[1] = v[1];
+    } else {
+      st_off(base, n, v[0]); st_off(base, n + cstr, v[1]);
+    }
+  };
   const uint32_t n_row = (uint32_t)g.sx * W;
   const uint32_t n_off = node_ok ? (uint32_t)ni * W : 0u;
   // loader positions; nodes outside the grid are loaded from offset 0 (corners of
@@ -525,11 +543,8 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
       if (!pos_live(p)) continue;
       const uint32_t o = pos_off[p] + lo;
       if (MODE == 1) pv[p] = ld_off(a.invs, o);
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        pr[p][c] = ld_off(a.in, o + c * cstr);
-        if (MODE == 1) po[p][c] = ld_off(pold, o + c * cstr);
-      }
+      ld2(a.in, o, pr[p]);
+      if (MODE == 1) ld2(pold, o, po[p]);
     }
   };
   auto commit = [&](int buf) {
@@ -555,10 +570,7 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
     const bool in = k >= j0 && k < j1;
     const uint32_t n = n_off + (in ? (uint32_t)k * n_row : 0u);
     o.m = a.mask ? (uint32_t)ld_off(a.mask, n / W) : 0u;
-    if (MODE == 1) {
-#pragma unroll
-      for (int c = 0; c < 2; ++c) o.x[c] = ld_off(a.x, n + c * cstr);
-    }
+    if (MODE == 1) ld2(a.x, n, o.x);
   };
   double dot = 0.0;
   T acc_top[2] = {T(0), T(0)};
@@ -568,15 +580,17 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
     if (have_out) {
       const uint32_t n = n_off + (uint32_t)ey * n_row;
       const int ctr = tid + 1;
+      st2(a.y, n, yv);
+      if (MODE == 1) {
+        T pv_[2], xv[2];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        st_off(a.y, n + c * cstr, yv[c]);
-        if (MODE == 1) {
-          const T pv_ = s_in[buf][c][ctr];
-          st_off(pnew, n + c * cstr, pv_);
-          st_off(a.x, n + c * cstr, fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]));
-          dot = fma_t((double)pv_, (double)yv[c], dot);
+        for (int c = 0; c < 2; ++c) {
+          pv_[c] = s_in[buf][c][ctr];
+          xv[c] = fma_t(alpha_x, s_po[buf][c][ctr], o.x[c]);
+          dot = fma_t((double)pv_[c], (double)yv[c], dot);
         }
+        st2(pnew, n, pv_);
+        st2(a.x, n, xv);
       }
     }
   };
@@ -636,13 +650,10 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
       if (!pos_live(p)) continue;
       const uint32_t o = pos_off[p] + lo;
       if (MODE == 1) pv[p] = ld_off(a.invs, o);
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        pr[p][c] = ld_off(a.in, o + c * cstr);
-        if (MODE == 1) {
-          po[p][c] = ld_off((const T *)a.P0, o + c * cstr);
-          po_b[p][c] = ld_off((const T *)a.P1, o + c * cstr);
-        }
+      ld2(a.in, o, pr[p]);
+      if (MODE == 1) {
+        ld2((const T *)a.P0, o, po[p]);
+        ld2((const T *)a.P1, o, po_b[p]);
       }
     }
   }
@@ -684,16 +695,20 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   }
 }
 
+// The FP64 kernel A (fused form) works on node-interleaved vectors; the FP32 kernel A
+// (mixed form) and the matvec on component planes.
+template <typename T, int MODE> constexpr bool Quad4Aos = MODE == 1 && sizeof(T) == 8;
+
 template <typename T, int TI, int MODE>
 __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {
-  quad4_block<T, TI, MODE>(a, blockIdx.x);
+  quad4_block<T, TI, MODE, Quad4Aos<T, MODE>>(a, blockIdx.x);
 }
 template <typename T, int TI, int MODE>
 __global__ void __launch_bounds__(TI + 1)
     k_quad4_batch(const OpArgs<T> *__restrict__ args, const int *__restrict__ off, int njobs) {
   int j, blk;
   batch_block(off, njobs, j, blk);
-  quad4_block<T, TI, MODE>(args[j], blk);
+  quad4_block<T, TI, MODE, Quad4Aos<T, MODE>>(args[j], blk);
 }
 
 // tile shapes: FP32 15 x 15 (256 threads), FP64 15 x 7 (128 threads, LDS-bound)

@@ -54,7 +54,7 @@ def footprint_estimate(ndim, div, single_reduction=False, form=None):
     and state; a hexa8 job on the single-reduction iteration (ICX_OPT_SINGLE_REDUCTION)
     holds one more FP32 vector (88 B per DOF) and 0.375 MB more partial sums.  FP64
     fused form: the same 84 B per DOF (five FP64 vectors in place of the mixed form's
-    eight FP32 and one FP64; hexa8 jobs 100 B: node-interleaved copies of x and b),
+    eight FP32 and one FP64) and 16 B more (node-interleaved copies of x and b),
     88 B per element (no FP32 copy of the moduli), 9 B per node (FP64 inverse modulus
     sums).  icx_job_footprint reports the same buffers once they exist."""
     if form is None:
@@ -64,7 +64,7 @@ def footprint_estimate(ndim, div, single_reduction=False, form=None):
     ne = d0 * d1 * (d2 if ndim == 3 else 1)
     n = ndim * nn
     if form == SOLVE_FUSED:
-        return (100 if ndim == 3 else 84) * n + 88 * ne + 9 * nn + 432 * 1024
+        return 100 * n + 88 * ne + 9 * nn + 432 * 1024
     if ndim == 3 and single_reduction:
         return 88 * n + 92 * ne + 5 * nn + 816 * 1024
     return 84 * n + 92 * ne + 5 * nn + 432 * 1024

@@ -19,9 +19,9 @@ def test_solve_form_by_dimension():
 
 
 def test_footprint_estimate_by_form():
-    """The FP64 fused form holds the same 84 B per DOF as the mixed form (hexa8 jobs 16 B
-    per DOF more: node-interleaved copies of x and b), no FP32 copy of the moduli (4 B per
-    element less) and FP64 inverse modulus sums (4 B per node more)."""
+    """The FP64 fused form holds the mixed form's 84 B per DOF and 16 B more (node-
+    interleaved copies of x and b), no FP32 copy of the moduli (4 B per element less) and
+    FP64 inverse modulus sums (4 B per node more)."""
     for ndim, div in ((2, (300, 100, 1)), (3, (40, 20, 20))):
         d0, d1, d2 = div
         nn = (d0 + 1) * (d1 + 1) * ((d2 + 1) if ndim == 3 else 1)
@@ -29,7 +29,7 @@ def test_footprint_estimate_by_form():
         mixed = serve.footprint_estimate(ndim, div, form=SOLVE_MIXED)
         fused = serve.footprint_estimate(ndim, div, form=SOLVE_FUSED)
         n = ndim * nn
-        assert fused - mixed == 4 * nn - 4 * ne + (16 * n if ndim == 3 else 0)
+        assert fused - mixed == 4 * nn - 4 * ne + 16 * n
         assert serve.footprint_estimate(ndim, div) == (fused if ndim == 2 else mixed)
 
 
