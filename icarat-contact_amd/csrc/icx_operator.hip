@@ -1035,6 +1035,7 @@ __global__ void __launch_bounds__(256) k_hex8_mfma(MfmaOpArgs a) {
 
 void launch_elastic_matvec_mfma(const Grid &g, const float *K0, const float *E, const uint8_t *mask, const float *x,
                                 float *y, hipStream_t s, const LaunchEvents *ev) {
+  static_assert(HexTile<float>::TI == 15 && HexTile<float>::TJ == 15, "k_hex8_mfma assumes 15 x 15 node tiles");
   OpArgs<float> t{};
   t.g = g;
   int nb, nt;
