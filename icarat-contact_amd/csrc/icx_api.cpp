@@ -449,8 +449,8 @@ struct PcgWork64 {
 struct MixedWork {
   DBuf<float> r, pa, pb, Ap, Ap2, xlo, invd, invs, E, b32;  // Ap2: seam halves of kernel A
   DBuf<double> Kx;
-  // hexa8 box jobs in the two-kernel iteration: the FP32 vectors are node-interleaved
-  // (entry 3 n + c), the layout of kernels A and B; otherwise component planes
+  // box jobs in the two-kernel iteration: the FP32 vectors are node-interleaved (entry
+  // nc n + c), the layout of kernel A; otherwise component planes
   bool aos = false;
   // single-reduction form (hexa8 solves): r = {r, r1}, p = {pa, pb}, Ap = {Ap, ap1};
   // per-block partials by launch parity
@@ -1789,7 +1789,7 @@ static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st
   const long long N = j->nvec();
   MixedWork &w = j->mw;
   w.sr = sr;
-  w.aos = !sr && !j->mesh && j->g.ndim == 3;
+  w.aos = !sr && !j->mesh;
   HIPCHK(c, w.r.alloc(N)); HIPCHK(c, w.pa.alloc(N)); HIPCHK(c, w.pb.alloc(N)); HIPCHK(c, w.Ap.alloc(N));
   HIPCHK(c, w.xlo.alloc(N)); HIPCHK(c, w.invd.alloc(N)); HIPCHK(c, w.E.alloc(j->g.ne)); HIPCHK(c, w.Kx.alloc(N));
   HIPCHK(c, w.invs.alloc(j->g.nn));
@@ -1812,7 +1812,7 @@ static icx_status mixed_begin(icx_job *j, double tol, int maxit, SolverState &st
   HIPCHK(c, hipMemsetAsync(w.xlo.p, 0, N * sizeof(float), S(j)));
   HIPCHK(c, hipMemsetAsync(w.pa.p, 0, N * sizeof(float), S(j)));
   // r32 = b; the tolerance is tested on the FP64 |b| (the replacement residual is FP64)
-  if (w.aos) launch_convert_f64_f32_aos(j->g.nn, j->d_b.p, w.b32.p, S(j));
+  if (w.aos) launch_convert_f64_f32_aos(j->g.nn, j->nc, j->d_b.p, w.b32.p, S(j));
   else launch_convert_f64_f32(N, j->d_b.p, w.b32.p, S(j));
   launch_pcg_setup<float>(N, j->d_state.p, w.b32.p, w.r.p, w.invd.p, P, tol, maxit, 0.01, S(j));
   launch_dot(N, j->d_b.p, j->d_b.p, j->d_scalars.p, j->red(), S(j));
@@ -1873,15 +1873,15 @@ static void mixed_replace(icx_job *j) {
   const PcgPartials P = mixed_parts(j);
   // single-reduction form: x is complete at the stop (no step owed)
   if (j->mesh) {  // p in place: both parities read the one buffer
-    launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pa.p, 0, S(j));
+    launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pa.p, 0, j->nc, S(j));
     launch_mesh_matvec(j->mop, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
-    launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, 0, S(j));
+    launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, 0, j->nc, S(j));
     return;
   }
   const long long nn_aos = w.aos ? j->g.nn : 0;
-  launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pb.p, nn_aos, S(j));
+  launch_refine_x(N, j->d_state.p, j->d_u.p, w.xlo.p, w.pa.p, w.pb.p, nn_aos, j->nc, S(j));
   launch_elastic_matvec<double>(j->g, j->op64, j->d_E.p, j->d_mask.p, j->d_u.p, w.Kx.p, nullptr, S(j));
-  launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, nn_aos, S(j));
+  launch_refine_r(N, j->d_state.p, j->d_b.p, w.Kx.p, w.r.p, w.invd.p, P, nn_aos, j->nc, S(j));
   if (w.sr) launch_sr_restart(j->g, j->op32, sr_vectors(j), j->d_state.p, S(j));
 }
 

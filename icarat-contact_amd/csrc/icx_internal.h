@@ -377,14 +377,14 @@ void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0,
                       hipStream_t s);
 // Mixed form replacement.  The FP64 vectors (xhi, b, Kx) are component planes; the FP32
 // ones (xlo, P0, P1, r32, invd32) are too when nn_aos = 0, else node-interleaved over
-// nn_aos nodes of 3 components (entry 3 n + c is FP64 entry c nn_aos + n).
+// nn_aos nodes of nc components (entry nc n + c is FP64 entry c nn_aos + n).
 // xhi += xlo + alpha_owed p_{stop-1}; xlo = 0
 void launch_refine_x(long long n, SolverState *st, double *xhi, float *xlo, const float *P0,
-                     const float *P1, long long nn_aos, hipStream_t s);
+                     const float *P1, long long nn_aos, int nc, hipStream_t s);
 // r32 = b - Kx (FP64), (|r|^2, r.z) -> PB[(stop-1)&1], then a one-block kernel decides
 // converged / continue and resets the recurrence
 void launch_refine_r(long long n, SolverState *st, const double *b, const double *Kx, float *r32,
-                     const float *invd32, PcgPartials P, long long nn_aos, hipStream_t s);
+                     const float *invd32, PcgPartials P, long long nn_aos, int nc, hipStream_t s);
 
 // the Eigen ConjugateGradient loop after tmp = A p, one kernel per vector expression
 // (p.tmp, x += a p, r -= a tmp, |r|^2, z = M^-1 r, r.z, p = z + beta p)
@@ -424,9 +424,9 @@ void launch_grid_to_dof(const Grid &g, int nc, const int *dof, int numeq, const 
 void launch_node_values(const Grid &g, int nc, const int *dof, int numeq, const double *grid,
                         const double *dirich, double *nodeval, hipStream_t s);
 void launch_convert_f64_f32(long long n, const double *a, float *b, hipStream_t s);
-// b[3 n + c] = a[c nn + n] for nn nodes of 3 components (component planes in FP64 to a
+// b[nc n + c] = a[c nn + n] for nn nodes of nc components (component planes in FP64 to a
 // node-interleaved FP32 vector)
-void launch_convert_f64_f32_aos(long long nn, const double *a, float *b, hipStream_t s);
+void launch_convert_f64_f32_aos(long long nn, int nc, const double *a, float *b, hipStream_t s);
 // b[nc n + c] = a[c nn + n] (to node-interleaved) and back, FP64, nc components per node
 void launch_soa_to_aos_f64(long long nn, int nc, const double *a, double *b, hipStream_t s);
 void launch_aos_to_soa_f64(long long nn, int nc, const double *a, double *b, hipStream_t s);

@@ -199,14 +199,13 @@ void launch_aos_to_soa_f64(long long nn, int nc, const double *a, double *b, hip
   ICX_CHECK_LAUNCH();
 }
 
-__global__ void k_f64_f32_aos(long long nn, const double *a, float *b) {
+__global__ void k_f64_f32_aos(long long nn, int nc, const double *a, float *b) {
   for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < nn;
        n += (long long)gridDim.x * blockDim.x)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) b[3 * n + c] = (float)a[c * nn + n];
+    for (int c = 0; c < nc; ++c) b[nc * n + c] = (float)a[c * nn + n];
 }
-void launch_convert_f64_f32_aos(long long nn, const double *a, float *b, hipStream_t s) {
-  hipLaunchKernelGGL(k_f64_f32_aos, dim3(ew_blocks(nn, 4)), dim3(256), 0, s, nn, a, b);
+void launch_convert_f64_f32_aos(long long nn, int nc, const double *a, float *b, hipStream_t s) {
+  hipLaunchKernelGGL(k_f64_f32_aos, dim3(ew_blocks(nn, 4)), dim3(256), 0, s, nn, nc, a, b);
   ICX_CHECK_LAUNCH();
 }
 

@@ -695,9 +695,9 @@ __device__ __forceinline__ void quad4_block(const OpArgs<T> &a, int blk) {
   }
 }
 
-// The FP64 kernel A (fused form) works on node-interleaved vectors; the FP32 kernel A
-// (mixed form) and the matvec on component planes.
-template <typename T, int MODE> constexpr bool Quad4Aos = MODE == 1 && sizeof(T) == 8;
+// Kernel A (the inner iteration of the mixed and the FP64 fused forms) works on
+// node-interleaved vectors; the matvec on component planes.
+template <typename T, int MODE> constexpr bool Quad4Aos = MODE == 1;
 
 template <typename T, int TI, int MODE>
 __global__ void __launch_bounds__(TI + 1) k_quad4(OpArgs<T> a) {

@@ -361,38 +361,38 @@ void launch_pcg_flush(long long n, SolverState *st, double *x, const double *P0,
 }
 
 // FP64 entry of FP32 entry i: the same index, or with a node-interleaved FP32 vector over
-// nn_aos nodes (entry 3 n + c) the component-plane entry c nn_aos + n
-__device__ __forceinline__ long long plane_index(long long i, long long nn_aos) {
+// nn_aos nodes of nc components (entry nc n + c) the component-plane entry c nn_aos + n
+__device__ __forceinline__ long long plane_index(long long i, long long nn_aos, int nc) {
   if (!nn_aos) return i;
-  const long long n = i / 3;
-  return (i - 3 * n) * nn_aos + n;
+  const long long n = i / nc;
+  return (i - nc * n) * nn_aos + n;
 }
 
 __global__ void k_refine_x(long long n, const SolverState *st, double *xhi, float *xlo, const float *P0,
-                           const float *P1, long long nn_aos) {
+                           const float *P1, long long nn_aos, int nc) {
   const int it = st->stop_iter;
   const float a = (it > 0 && st->flush_iter != it) ? (float)st->alpha_owed : 0.0f;
   const float *p = ((it - 1) & 1) ? P1 : P0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const float xl = xlo[i] + a * p[i];
-    xhi[plane_index(i, nn_aos)] += (double)xl;
+    xhi[plane_index(i, nn_aos, nc)] += (double)xl;
     xlo[i] = 0.0f;
   }
 }
 void launch_refine_x(long long n, SolverState *st, double *xhi, float *xlo, const float *P0, const float *P1,
-                     long long nn_aos, hipStream_t s) {
-  hipLaunchKernelGGL(k_refine_x, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, st, xhi, xlo, P0, P1, nn_aos);
+                     long long nn_aos, int nc, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_x, dim3(ew_blocks(n, 4)), dim3(256), 0, s, n, st, xhi, xlo, P0, P1, nn_aos, nc);
   ICX_CHECK_LAUNCH();
 }
 
 __global__ void __launch_bounds__(BS) k_refine_r(long long n, const SolverState *st, const double *b,
                                                  const double *Kx, float *r32, const float *invd32, PcgPartials P,
-                                                 long long nn_aos) {
+                                                 long long nn_aos, int nc) {
   __shared__ double scr[BS / 64];
   double rr = 0.0, rz = 0.0;
   for (long long i = blockIdx.x * (long long)BS + threadIdx.x; i < n; i += (long long)gridDim.x * BS) {
-    const long long i64 = plane_index(i, nn_aos);
+    const long long i64 = plane_index(i, nn_aos, nc);
     const double ri = b[i64] - Kx[i64];
     const float rf = (float)ri;
     r32[i] = rf;
@@ -427,8 +427,8 @@ __global__ void __launch_bounds__(BS) k_refine_finalize(SolverState *st, PcgPart
   }
 }
 void launch_refine_r(long long n, SolverState *st, const double *b, const double *Kx, float *r32,
-                     const float *invd32, PcgPartials P, long long nn_aos, hipStream_t s) {
-  hipLaunchKernelGGL(k_refine_r, dim3(P.nB), dim3(BS), 0, s, n, st, b, Kx, r32, invd32, P, nn_aos);
+                     const float *invd32, PcgPartials P, long long nn_aos, int nc, hipStream_t s) {
+  hipLaunchKernelGGL(k_refine_r, dim3(P.nB), dim3(BS), 0, s, n, st, b, Kx, r32, invd32, P, nn_aos, nc);
   hipLaunchKernelGGL(k_refine_finalize, dim3(1), dim3(BS), 0, s, st, P);
   ICX_CHECK_LAUNCH();
 }
