@@ -199,8 +199,8 @@ def test_single_reduction_batch_equals_alone(ctx):
 
 def test_large_hexa8_job_per_node_kernel_b(ctx):
     """A hexa8 job of more than 2^20 nodes takes kernel B with the per-node Jacobi factor
-    (node chunks, dword-aligned 16-byte accesses of the component planes, the nn mod 4
-    tail nodes in one thread).  128 x 64 x 126 elements: 1 064 895 nodes (tail 3).  The
+    (chunks of 4 nodes, 12 consecutive entries of the node-interleaved vectors, the nn mod
+    4 tail nodes in one thread).  128 x 64 x 126 elements: 1 064 895 nodes (tail 3).  The
     mixed solve meets the true residual through the FP64 operator, agrees with the FP64
     plain form, and equals itself batched with a small job bit for bit."""
     big = icx.cantilever_3d(ctx, 128, 64, 126)
