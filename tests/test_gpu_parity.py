@@ -146,6 +146,29 @@ def test_fp64_forms_match_direct_solution(ctx, form):
     assert abs(job.compliance() - c_ref) / c_ref < 1e-10
 
 
+@pytest.mark.parametrize("dims,depth", [((33, 17, 9, 11), 0), ((33, 17, 9, 11), 3), ((16, 31, 20, 12), 0)])
+def test_fused_form_on_several_node_tiles(ctx, dims, depth):
+    """The FP64 fused form (node-interleaved vectors, 15 x 7 node tiles) on grids of
+    several tiles in x and y and several z bricks: the same PCG as the plain form up to
+    rounding (iteration counts within one), the direct solution to 1e-10 and the
+    compliance to 1e-10."""
+    nx, ny, nz, seed = dims
+    job, pr, rho, ud = _solve_case(ctx, nx, ny, nz, seed)
+    ctx.set_option(icx.OPT_TILE_DEPTH, depth)
+    try:
+        u, info = job.solve(icx.SOLVE_FUSED, tol=1e-13, maxit=100000)
+        up, pinfo = job.solve(icx.SOLVE_PLAIN, tol=1e-13, maxit=100000)
+    finally:
+        ctx.set_option(icx.OPT_TILE_DEPTH, 0)
+    assert info["converged"] and pinfo["converged"]
+    assert abs(info["iterations"] - pinfo["iterations"]) <= 1
+    assert np.linalg.norm(u - ud) / np.linalg.norm(ud) < 1e-10
+    assert np.linalg.norm(u - up) / np.linalg.norm(up) < 1e-11
+    job.solve(icx.SOLVE_FUSED, tol=1e-13, maxit=100000)
+    c_ref = ud @ pr.fext
+    assert abs(job.compliance() - c_ref) / c_ref < 1e-10
+
+
 def test_mixed_form_meets_true_residual(ctx):
     job, pr, rho, ud = _solve_case(ctx)
     u, info = job.solve(icx.SOLVE_MIXED, tol=1e-10, maxit=20000)
